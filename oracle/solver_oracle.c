@@ -1,0 +1,339 @@
+/*
+ * CPU ORACLE — test infrastructure only (see dec_oracle.c).
+ *
+ * Restates the 3D magnetostatics operator of the reference's ONELAB
+ * interface, its right-hand side and nodal permeability rules, the 2D
+ * Dirichlet Poisson operators the solvers run on, and textbook conjugate
+ * gradients.  Used only by tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ */
+/* magnetostatics 3D operator                                          */
+/* ------------------------------------------------------------------ */
+
+/* local_spacing (onelab_interface/magnetostatics_onelab.hpp:856-870) */
+static double local_spacing(const double* c, long n, long i)
+{
+    if (n < 2) return 1.0;
+    if (i == 0) return c[1] - c[0];
+    if (i + 1 == n) return c[n - 1] - c[n - 2];
+    return 0.5 * (c[i + 1] - c[i - 1]);
+}
+
+typedef struct {
+    long nx, ny, nz;
+    const double *x, *y, *z;
+} ms_grid;
+
+static long dof(const ms_grid* g, long i, long j, long k, int c) { return 3 * (i + g->nx * (j + g->ny * k)) + c; }
+
+static int is_boundary(const ms_grid* g, long i, long j, long k)
+{
+    return i == 0 || j == 0 || k == 0 || i + 1 == g->nx || j + 1 == g->ny || k + 1 == g->nz;
+}
+
+/* The moment (magnetic induction) stencils of forward_value
+ * (src/similie/physics/magnetostatics/magnetostatics_quantities.hpp:173-196):
+ * B_x = (dA)_yz, B_y = -(dA)_xz, B_z = (dA)_xy, each a forward difference of
+ * the vector potential around the node, with the sign layout of
+ * Coboundary::run.  Entries: (di,dj,dk, potential component, coefficient). */
+typedef struct {
+    int di, dj, dk, comp;
+    double coeff;
+} ms_tap;
+
+static const ms_tap MOMENT[3][4] = {
+    /* B_x = A_z(s+e_y) - A_z(s) - A_y(s+e_z) + A_y(s) */
+    {{0, 1, 0, 2, 1.0}, {0, 0, 0, 2, -1.0}, {0, 0, 1, 1, -1.0}, {0, 0, 0, 1, 1.0}},
+    /* B_y = -(A_z(s+e_x) - A_z(s) - A_x(s+e_z) + A_x(s)) */
+    {{1, 0, 0, 2, -1.0}, {0, 0, 0, 2, 1.0}, {0, 0, 1, 0, 1.0}, {0, 0, 0, 0, -1.0}},
+    /* B_z = A_y(s+e_x) - A_y(s) - A_x(s+e_y) + A_x(s) */
+    {{1, 0, 0, 1, 1.0}, {0, 0, 0, 1, -1.0}, {0, 1, 0, 0, -1.0}, {0, 0, 0, 0, 1.0}},
+};
+
+/* apply_with_precomputed_stencils (magnetostatics_onelab.hpp:2241-2396) in
+ * scatter form: for every interior sampled node s, B(s) is gathered from the
+ * interior potential DOFs, weighted by the induction Hodge factor
+ * (:873-893) and ∂H/∂B = B/μ of the linear Hamiltonian
+ * (physics/magnetostatics/linear_magnetostatics.py:19), and scattered back
+ * through the transposed stencil; interior rows add the vector-potential
+ * gauge penalty (:2359-2367), boundary rows are the identity. */
+void orc_ms_apply(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
+                  const double* mu, double penalty, double y_sign, const double* in, double* out)
+{
+    ms_grid g = {nx, ny, nz, xc, yc, zc};
+    long ndof = 3 * nx * ny * nz;
+    for (long r = 0; r < ndof; ++r) out[r] = 0.0;
+    for (long k = 1; k + 1 < nz; ++k)
+        for (long j = 1; j + 1 < ny; ++j)
+            for (long i = 1; i + 1 < nx; ++i) {
+                double dx = local_spacing(xc, nx, i), dy = local_spacing(yc, ny, j), dz = local_spacing(zc, nz, k);
+                double wb[3] = {dx / (dy * dz), dy / (dx * dz), dz / (dx * dy)};
+                double nu = 1.0 / mu[i + nx * (j + ny * k)];
+                double B[3];
+                for (int m = 0; m < 3; ++m) {
+                    double b = 0.0;
+                    for (int t = 0; t < 4; ++t) {
+                        const ms_tap* tp = &MOMENT[m][t];
+                        long pi = i + tp->di, pj = j + tp->dj, pk = k + tp->dk;
+                        if (is_boundary(&g, pi, pj, pk)) continue;
+                        b += tp->coeff * in[dof(&g, pi, pj, pk, tp->comp)];
+                    }
+                    B[m] = b;
+                }
+                for (int m = 0; m < 3; ++m) {
+                    double h = wb[m] * nu * B[m] * (m == 1 ? y_sign : 1.0);
+                    for (int t = 0; t < 4; ++t) {
+                        const ms_tap* tp = &MOMENT[m][t];
+                        long pi = i + tp->di, pj = j + tp->dj, pk = k + tp->dk;
+                        if (is_boundary(&g, pi, pj, pk)) continue;
+                        out[dof(&g, pi, pj, pk, tp->comp)] += tp->coeff * h;
+                    }
+                }
+            }
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i) {
+                if (is_boundary(&g, i, j, k)) {
+                    for (int c = 0; c < 3; ++c) out[dof(&g, i, j, k, c)] = in[dof(&g, i, j, k, c)];
+                    continue;
+                }
+                double dx = local_spacing(xc, nx, i), dy = local_spacing(yc, ny, j), dz = local_spacing(zc, nz, k);
+                double wa[3] = {dy * dz / dx, dx * dz / dy, dx * dy / dz};
+                for (int c = 0; c < 3; ++c) out[dof(&g, i, j, k, c)] += penalty * wa[c] * in[dof(&g, i, j, k, c)];
+            }
+}
+
+/* right-hand side: ⅛·|dual face|·J over the ≤8 cells around an interior
+ * node (magnetostatics_onelab.hpp:4919-4975).  cell_j is [cell][3] with
+ * cells indexed ci + (nx-1)(cj + (ny-1) ck). */
+void orc_ms_rhs(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
+                const double* cell_j, double* rhs)
+{
+    ms_grid g = {nx, ny, nz, xc, yc, zc};
+    long cx = nx - 1, cy = ny - 1, cz = nz - 1;
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i)
+                for (int c = 0; c < 3; ++c) {
+                    long r = dof(&g, i, j, k, c);
+                    rhs[r] = 0.0;
+                    if (is_boundary(&g, i, j, k)) continue;
+                    for (int dk = -1; dk <= 0; ++dk)
+                        for (int dj = -1; dj <= 0; ++dj)
+                            for (int di = -1; di <= 0; ++di) {
+                                long ci = i + di, cj = j + dj, ck = k + dk;
+                                if (ci < 0 || cj < 0 || ck < 0 || ci >= cx || cj >= cy || ck >= cz) continue;
+                                double wx = xc[ci + 1] - xc[ci], wy = yc[cj + 1] - yc[cj], wz = zc[ck + 1] - zc[ck];
+                                double face = wx * wy;
+                                if (c == 0) face = wy * wz;
+                                else if (c == 1) face = wx * wz;
+                                rhs[r] += 0.125 * face * cell_j[3 * (ci + cx * (cj + cy * ck)) + c];
+                            }
+                }
+}
+
+/* nodal permeability = mean over adjacent cells (magnetostatics_onelab.hpp
+ * :5029-5066) */
+void orc_ms_nodal_mu(long nx, long ny, long nz, const double* cell_mu, double mu0, double* mu)
+{
+    long cx = nx - 1, cy = ny - 1, cz = nz - 1;
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i) {
+                double acc = 0.0;
+                long cnt = 0;
+                for (int dk = -1; dk <= 0; ++dk)
+                    for (int dj = -1; dj <= 0; ++dj)
+                        for (int di = -1; di <= 0; ++di) {
+                            long ci = i + di, cj = j + dj, ck = k + dk;
+                            if (ci < 0 || cj < 0 || ck < 0 || ci >= cx || cj >= cy || ck >= cz) continue;
+                            acc += cell_mu[ci + cx * (cj + cy * ck)];
+                            ++cnt;
+                        }
+                mu[i + nx * (j + ny * k)] = cnt == 0 ? mu0 : acc / (double)cnt;
+            }
+}
+
+/* Jacobi diagonal of the operator above */
+void orc_ms_diagonal(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
+                     const double* mu, double penalty, double* diag)
+{
+    long ndof = 3 * nx * ny * nz;
+    double* e = (double*)calloc(ndof, sizeof(double));
+    double* o = (double*)malloc(sizeof(double) * ndof);
+    /* columns are probed in 27 colour classes so that probes never overlap */
+    for (int ci = 0; ci < 3; ++ci)
+        for (int cj = 0; cj < 3; ++cj)
+            for (int ck = 0; ck < 3; ++ck)
+                for (int c = 0; c < 3; ++c) {
+                    for (long r = 0; r < ndof; ++r) e[r] = 0.0;
+                    for (long k = ck; k < nz; k += 3)
+                        for (long j = cj; j < ny; j += 3)
+                            for (long i = ci; i < nx; i += 3) e[3 * (i + nx * (j + ny * k)) + c] = 1.0;
+                    orc_ms_apply(nx, ny, nz, xc, yc, zc, mu, penalty, 1.0, e, o);
+                    for (long k = ck; k < nz; k += 3)
+                        for (long j = cj; j < ny; j += 3)
+                            for (long i = ci; i < nx; i += 3) {
+                                long r = 3 * (i + nx * (j + ny * k)) + c;
+                                diag[r] = o[r];
+                            }
+                }
+    free(e);
+    free(o);
+}
+
+/* ------------------------------------------------------------------ */
+/* 2D Dirichlet Poisson operators                                      */
+/* ------------------------------------------------------------------ */
+
+/* ⋆0·Δ of the DEC 0-form Laplacian on a rectilinear 2D grid, restricted to
+ * interior nodes with zero Dirichlet values (= d0ᵀ ⋆1 d0): edge weights
+ * w_x(m) = h_y(m)/h_x(m), w_y(m) = h_x(m)/h_y(m) from the per-node ⋆ rule.
+ * u and out are full nx×ny arrays (x slowest); boundary entries of out are 0. */
+void orc_poisson2d_apply(long nx, long ny, const double* hx, const double* hy, const double* kappa,
+                         const double* u, double* out)
+{
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < nx; ++i)
+        for (long j = 0; j < ny; ++j) {
+            long n = i * ny + j;
+            if (i == 0 || j == 0 || i == nx - 1 || j == ny - 1) {
+                out[n] = 0.0;
+                continue;
+            }
+            double un = u[n];
+            double acc = 0.0;
+            /* edge weights: w_x at (i,j) couples (i,j)-(i+1,j) */
+            double kx_p = kappa ? 0.5 * (kappa[n] + kappa[n + ny]) : 1.0;
+            double kx_m = kappa ? 0.5 * (kappa[n] + kappa[n - ny]) : 1.0;
+            double ky_p = kappa ? 0.5 * (kappa[n] + kappa[n + 1]) : 1.0;
+            double ky_m = kappa ? 0.5 * (kappa[n] + kappa[n - 1]) : 1.0;
+            double wxp = kx_p * hy[j] / hx[i], wxm = kx_m * hy[j] / hx[i - 1];
+            double wyp = ky_p * hx[i] / hy[j], wym = ky_m * hx[i] / hy[j - 1];
+            double ue = (i + 1 < nx - 1) ? u[n + ny] : 0.0;
+            double uw = (i - 1 > 0) ? u[n - ny] : 0.0;
+            double un_ = (j + 1 < ny - 1) ? u[n + 1] : 0.0;
+            double us = (j - 1 > 0) ? u[n - 1] : 0.0;
+            acc = wxp * (un - ue) + wxm * (un - uw) + wyp * (un - un_) + wym * (un - us);
+            out[n] = acc;
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* textbook conjugate gradients (Hestenes–Stiefel)                     */
+/* ------------------------------------------------------------------ */
+
+typedef void (*orc_matvec_fn)(void* ctx, const double* in, double* out);
+
+static double dot(long n, const double* a, const double* b)
+{
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (long i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+/* x0 = 0; stop when ‖r_k‖2 ≤ tol·‖b‖2 on the recurrence residual or after
+ * max_it iterations (the stopping rule of the reference's Ginkgo CG call,
+ * src/similie/solvers/minimize_strong_formulation_residual.hpp:1063-1130).
+ * diag (optional) applies Jacobi preconditioning.  Returns iterations;
+ * res_out[0] = final recurrence ‖r‖2, res_out[1] = ‖b‖2. */
+int orc_cg(long n, orc_matvec_fn A, void* ctx, const double* b, const double* diag, double tol, int max_it,
+           double* x, double* res_out)
+{
+    double* r = (double*)malloc(sizeof(double) * n);
+    double* z = (double*)malloc(sizeof(double) * n);
+    double* p = (double*)malloc(sizeof(double) * n);
+    double* q = (double*)malloc(sizeof(double) * n);
+    for (long i = 0; i < n; ++i) {
+        x[i] = 0.0;
+        r[i] = b[i];
+        z[i] = diag ? b[i] / diag[i] : b[i];
+        p[i] = z[i];
+    }
+    double bnorm = sqrt(dot(n, b, b));
+    double rz = dot(n, r, z);
+    double rnorm = bnorm;
+    int it = 0;
+    if (bnorm == 0.0) {
+        res_out[0] = 0.0;
+        res_out[1] = 0.0;
+        free(r);
+        free(z);
+        free(p);
+        free(q);
+        return 0;
+    }
+    while (it < max_it && rnorm > tol * bnorm) {
+        A(ctx, p, q);
+        double pq = dot(n, p, q);
+        double alpha = rz / pq;
+        for (long i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+        }
+        for (long i = 0; i < n; ++i) z[i] = diag ? r[i] / diag[i] : r[i];
+        double rz_new = dot(n, r, z);
+        rnorm = sqrt(dot(n, r, r));
+        double beta = rz_new / rz;
+        rz = rz_new;
+        for (long i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        ++it;
+    }
+    res_out[0] = rnorm;
+    res_out[1] = bnorm;
+    free(r);
+    free(z);
+    free(p);
+    free(q);
+    return it;
+}
+
+/* CG on the 2D Dirichlet Poisson operator above (unknowns = all nodes; the
+ * boundary rows are excluded by keeping their entries of b, x at zero). */
+typedef struct {
+    long nx, ny;
+    const double *hx, *hy, *kappa;
+} p2_ctx;
+
+static void p2_mv(void* c, const double* in, double* out)
+{
+    p2_ctx* k = (p2_ctx*)c;
+    orc_poisson2d_apply(k->nx, k->ny, k->hx, k->hy, k->kappa, in, out);
+}
+
+int orc_poisson2d_cg(long nx, long ny, const double* hx, const double* hy, const double* kappa, const double* b,
+                     double tol, int max_it, double* x, double* res_out)
+{
+    p2_ctx c = {nx, ny, hx, hy, kappa};
+    return orc_cg(nx * ny, p2_mv, &c, b, NULL, tol, max_it, x, res_out);
+}
+
+typedef struct {
+    long nx, ny, nz;
+    const double *x, *y, *z, *mu;
+    double penalty;
+} ms_ctx;
+
+static void ms_mv(void* c, const double* in, double* out)
+{
+    ms_ctx* k = (ms_ctx*)c;
+    orc_ms_apply(k->nx, k->ny, k->nz, k->x, k->y, k->z, k->mu, k->penalty, 1.0, in, out);
+}
+
+int orc_ms_cg(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc, const double* mu,
+              double penalty, const double* b, const double* diag, double tol, int max_it, double* x, double* res_out)
+{
+    ms_ctx c = {nx, ny, nz, xc, yc, zc, mu, penalty};
+    return orc_cg(3 * nx * ny * nz, ms_mv, &c, b, diag, tol, max_it, x, res_out);
+}

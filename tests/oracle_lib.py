@@ -1,0 +1,272 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) — the checker.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
+this module.  Arrays are numpy float64 in the reference's index-last layout
+(axis 0 slowest, component fastest).
+"""
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+_LIB_PATH = ROOT / "oracle" / "liboracle.so"
+_lib = None
+
+POS_RECTILINEAR, POS_NODAL = 0, 1
+METRIC_IDENTITY, METRIC_DIAGONAL, METRIC_SYMMETRIC = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+        _lib = ctypes.CDLL(str(_LIB_PATH))
+        _lib.orc_binom.restype = ctypes.c_long
+        _lib.orc_hodge_coefficient.restype = ctypes.c_double
+        _lib.orc_hodge_coefficient_chain.restype = ctypes.c_double
+        _lib.orc_continuous_hodge_value.restype = ctypes.c_double
+        _lib.orc_cg.restype = ctypes.c_int
+        _lib.orc_poisson2d_cg.restype = ctypes.c_int
+        _lib.orc_ms_cg.restype = ctypes.c_int
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _l(seq):
+    arr = (ctypes.c_long * len(seq))(*[int(v) for v in seq])
+    return arr
+
+
+def binom(n, k):
+    return int(lib().orc_binom(n, k))
+
+
+def combo_mask(n, k, index):
+    return int(lib().orc_combo_mask(n, k, index))
+
+
+def combo_index(n, k, mask):
+    return int(lib().orc_combo_index(n, k, mask))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Geometry:
+    """Grid geometry for the oracle: rectilinear axis coordinates or nodal
+    positions, plus an identity / diagonal / symmetric metric per node."""
+
+    def __init__(self, ext, axes=None, nodal=None, metric_kind=METRIC_IDENTITY, metric=None):
+        self.ext = [int(e) for e in ext]
+        self.ndim = len(self.ext)
+        self.pos_kind = POS_NODAL if nodal is not None else POS_RECTILINEAR
+        self.axes = [_f64(a) for a in axes] if axes is not None else None
+        self.nodal = _f64(nodal) if nodal is not None else None
+        self.metric_kind = metric_kind
+        self.metric = _f64(metric) if metric is not None else None
+        self._keep = []
+
+    def args(self):
+        n = self.ndim
+        ext = _l(self.ext)
+        if self.axes is not None:
+            ax = (_dp * 4)(*([_d(a) for a in self.axes] + [None] * (4 - n)))
+        else:
+            ax = None
+        nodal = _d(self.nodal) if self.nodal is not None else None
+        metric = _d(self.metric) if self.metric is not None else None
+        self._keep = [ext, ax]
+        return [ctypes.c_int(n), ext, ctypes.c_int(self.pos_kind), ax, nodal, ctypes.c_int(self.metric_kind), metric]
+
+
+def coboundary(n, k, in_ext, arr, out_ext):
+    nout = binom(n, k + 1)
+    out = np.zeros(tuple(out_ext) + (nout,))
+    a = _f64(arr)
+    lib().orc_coboundary(n, k, _l(in_ext), _d(a), _l(out_ext), _d(out))
+    return out
+
+
+def transposed_coboundary(n, k, in_ext, arr, out_ext):
+    nout = binom(n, k + 1)
+    out = np.zeros(tuple(out_ext) + (nout,))
+    a = _f64(arr)
+    lib().orc_transposed_coboundary(n, k, _l(in_ext), _d(a), _l(out_ext), _d(out))
+    return out
+
+
+def hodge_star(geom, k, arr):
+    n = geom.ndim
+    out = np.zeros(tuple(geom.ext) + (binom(n, n - k),))
+    a = _f64(arr)
+    lib().orc_hodge_star(*geom.args(), ctypes.c_int(k), _d(a), _d(out))
+    return out
+
+
+def fill_hodge_star(geom, k, use_chain=False):
+    n = geom.ndim
+    out = np.zeros(tuple(geom.ext) + (n ** k, binom(n, n - k)))
+    lib().orc_fill_hodge_star(*geom.args(), ctypes.c_int(k), ctypes.c_int(1 if use_chain else 0), _d(out))
+    return out
+
+
+def codifferential(geom, k, arr, out_ext):
+    n = geom.ndim
+    out = np.zeros(tuple(out_ext) + (binom(n, k - 1),))
+    a = _f64(arr)
+    lib().orc_codifferential(*geom.args(), ctypes.c_int(k), _d(a), _l(out_ext), _d(out))
+    return out
+
+
+def laplacian(geom, k, arr, out_ext):
+    n = geom.ndim
+    out = np.zeros(tuple(out_ext) + (binom(n, k),))
+    a = _f64(arr)
+    lib().orc_laplacian(*geom.args(), ctypes.c_int(k), _d(a), _l(out_ext), _d(out))
+    return out
+
+
+def reduction(geom, k, arr):
+    n = geom.ndim
+    out = np.zeros(tuple(geom.ext) + (binom(n, k),))
+    a = _f64(arr)
+    args = geom.args()
+    lib().orc_reduction(args[0], args[1], args[2], args[3], args[4], ctypes.c_int(k), _d(a), _d(out))
+    return out
+
+
+def reconstruction(geom, k, arr):
+    n = geom.ndim
+    out = np.zeros(tuple(geom.ext) + (binom(n, k),))
+    a = _f64(arr)
+    args = geom.args()
+    lib().orc_reconstruction(args[0], args[1], args[2], args[3], args[4], ctypes.c_int(k), _d(a), _d(out))
+    return out
+
+
+def hodge_coefficient(geom, node, src_mask, tgt_mask, chain=False):
+    """Coefficient of ⋆ from the canonical source axes set to the target set
+    at one node, by the volume ratio or by the reference's full chain."""
+    import ctypes as C
+
+    class _G(C.Structure):
+        _fields_ = [
+            ("ndim", C.c_int),
+            ("ext", C.c_long * 4),
+            ("pos_kind", C.c_int),
+            ("axis", _dp * 4),
+            ("nodal", _dp),
+            ("metric_kind", C.c_int),
+            ("metric", _dp),
+        ]
+
+    g = _G()
+    g.ndim = geom.ndim
+    for a in range(geom.ndim):
+        g.ext[a] = geom.ext[a]
+        if geom.axes is not None:
+            g.axis[a] = _d(geom.axes[a])
+    g.pos_kind = geom.pos_kind
+    g.nodal = _d(geom.nodal) if geom.nodal is not None else None
+    g.metric_kind = geom.metric_kind
+    g.metric = _d(geom.metric) if geom.metric is not None else None
+    fn = lib().orc_hodge_coefficient_chain if chain else lib().orc_hodge_coefficient
+    return float(fn(C.byref(g), _l(node), C.c_uint(src_mask), C.c_uint(tgt_mask)))
+
+
+def continuous_hodge_value(ext, metric_kind, metric, node, src, tgt):
+    m = _f64(metric)
+    return float(lib().orc_continuous_hodge_value(len(ext), _l(ext), metric_kind, _d(m), _l(node), src, tgt))
+
+
+def boundary(n, vmask):
+    k1 = bin(vmask).count("1")
+    offs = (ctypes.c_long * (2 * k1 * n))()
+    vms = (ctypes.c_int * (2 * k1))()
+    negs = (ctypes.c_int * (2 * k1))()
+    nf = lib().orc_boundary(n, vmask, offs, vms, negs)
+    return [([offs[f * n + a] for a in range(n)], vms[f], negs[f]) for f in range(nf)]
+
+
+# ---------------------------------------------------------------- solvers ---
+
+def ms_apply(coords, mu, penalty, x, y_sign=1.0):
+    nx, ny, nz = (len(c) for c in coords)
+    xc, yc, zc = (_f64(c) for c in coords)
+    mu = _f64(mu)
+    xin = _f64(x)
+    out = np.zeros(3 * nx * ny * nz)
+    lib().orc_ms_apply(ctypes.c_long(nx), ctypes.c_long(ny), ctypes.c_long(nz), _d(xc), _d(yc), _d(zc), _d(mu),
+                       ctypes.c_double(penalty), ctypes.c_double(y_sign), _d(xin), _d(out))
+    return out
+
+
+def ms_rhs(coords, cell_current):
+    nx, ny, nz = (len(c) for c in coords)
+    xc, yc, zc = (_f64(c) for c in coords)
+    cj = _f64(cell_current)
+    out = np.zeros(3 * nx * ny * nz)
+    lib().orc_ms_rhs(ctypes.c_long(nx), ctypes.c_long(ny), ctypes.c_long(nz), _d(xc), _d(yc), _d(zc), _d(cj), _d(out))
+    return out
+
+
+def ms_nodal_mu(shape, cell_mu, mu0):
+    nx, ny, nz = shape
+    cm = _f64(cell_mu)
+    out = np.zeros(nx * ny * nz)
+    lib().orc_ms_nodal_mu(ctypes.c_long(nx), ctypes.c_long(ny), ctypes.c_long(nz), _d(cm), ctypes.c_double(mu0), _d(out))
+    return out
+
+
+def ms_diagonal(coords, mu, penalty):
+    nx, ny, nz = (len(c) for c in coords)
+    xc, yc, zc = (_f64(c) for c in coords)
+    mu = _f64(mu)
+    out = np.zeros(3 * nx * ny * nz)
+    lib().orc_ms_diagonal(ctypes.c_long(nx), ctypes.c_long(ny), ctypes.c_long(nz), _d(xc), _d(yc), _d(zc), _d(mu),
+                          ctypes.c_double(penalty), _d(out))
+    return out
+
+
+def ms_cg(coords, mu, penalty, b, tol, max_it, diag=None):
+    nx, ny, nz = (len(c) for c in coords)
+    xc, yc, zc = (_f64(c) for c in coords)
+    mu = _f64(mu)
+    bb = _f64(b)
+    x = np.zeros_like(bb)
+    res = np.zeros(2)
+    dg = _d(_f64(diag)) if diag is not None else None
+    it = lib().orc_ms_cg(ctypes.c_long(nx), ctypes.c_long(ny), ctypes.c_long(nz), _d(xc), _d(yc), _d(zc), _d(mu),
+                         ctypes.c_double(penalty), _d(bb), dg, ctypes.c_double(tol), ctypes.c_int(max_it), _d(x),
+                         _d(res))
+    return x, it, res[0], res[1]
+
+
+def poisson2d_apply(hx, hy, kappa, u):
+    nx, ny = u.shape
+    hx, hy, uu = _f64(hx), _f64(hy), _f64(u)
+    kp = _d(_f64(kappa)) if kappa is not None else None
+    out = np.zeros((nx, ny))
+    lib().orc_poisson2d_apply(ctypes.c_long(nx), ctypes.c_long(ny), _d(hx), _d(hy), kp, _d(uu), _d(out))
+    return out
+
+
+def poisson2d_cg(hx, hy, kappa, b, tol, max_it):
+    nx, ny = b.shape
+    hx, hy, bb = _f64(hx), _f64(hy), _f64(b)
+    kp = _d(_f64(kappa)) if kappa is not None else None
+    x = np.zeros((nx, ny))
+    res = np.zeros(2)
+    it = lib().orc_poisson2d_cg(ctypes.c_long(nx), ctypes.c_long(ny), _d(hx), _d(hy), kp, _d(bb),
+                                ctypes.c_double(tol), ctypes.c_int(max_it), _d(x), _d(res))
+    return x, it, res[0], res[1]
