@@ -1,0 +1,209 @@
+/*
+ * sil_hip.h — C ABI of the MI355X discrete-exterior-calculus library
+ * (libsil_hip.so).
+ *
+ * Drop-in boundary for SimiLie's sil::exterior operators and its
+ * strong-formulation CG solve.  Every entry point is `extern "C"`, takes
+ * plain pointers and sizes, never throws, returns SIL_OK (0) or a negative
+ * status with a message available from sil_last_error(), and runs
+ * asynchronously on the caller's HIP stream (passed as `void*`; NULL is the
+ * default stream) — the analogue of the Kokkos execution-space instance the
+ * reference launches on.  Array pointers are device pointers owned by the
+ * caller unless stated otherwise.
+ *
+ * Array layout follows the reference's Tensor over
+ * DiscreteDomain<DDim..., Index> (src/similie/tensor/tensor_impl.hpp
+ * :587-602): a sil_view gives per-axis strides and a component stride, so
+ * both index-last (layout_right, component fastest) and index-first
+ * (tests/exterior/derivative.cpp:19-58) storage are accepted, plus spectator
+ * (batch) dimensions for the coboundary (derivative.cpp:889-990).  Component
+ * order of a k-form is the lexicographic order of the k-subsets of the axes
+ * (antisymmetric_tensor.hpp:72-98).
+ */
+#ifndef SIL_HIP_H
+#define SIL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIL_MAX_DIM 4
+
+enum sil_status {
+    SIL_OK = 0,
+    SIL_ERR_INVALID = -1,     /* bad argument (shape, rank, null pointer) */
+    SIL_ERR_UNSUPPORTED = -2, /* valid request outside what the library implements */
+    SIL_ERR_HIP = -3,         /* a HIP runtime call failed */
+    SIL_ERR_NOT_CONVERGED = -4
+};
+
+enum sil_metric_kind {
+    SIL_METRIC_IDENTITY = 0,  /* TensorIdentityIndex: no storage */
+    SIL_METRIC_DIAGONAL = 1,  /* TensorDiagonalIndex: [node][ndim] */
+    SIL_METRIC_SYMMETRIC = 2  /* TensorSymmetricIndex: [node][ndim(ndim+1)/2], XX XY XZ YY YZ ZZ */
+};
+
+enum sil_precision {
+    SIL_PRECISION_FP64 = 0,  /* FP64 throughout */
+    SIL_PRECISION_MIXED = 1  /* FP32 operator applications, FP64 residual replacement, dots and scalars */
+};
+
+enum sil_preconditioner { SIL_PRECOND_IDENTITY = 0, SIL_PRECOND_JACOBI = 1 };
+
+/* Strided view of a k-form sampled on a node grid. */
+typedef struct sil_view {
+    int32_t ndim;   /* grid (form) axes, 1..4 */
+    int32_t nbatch; /* spectator dimensions, 0..4 */
+    int64_t extent[SIL_MAX_DIM];
+    int64_t stride[SIL_MAX_DIM]; /* element strides of the grid axes */
+    int64_t comp_stride;         /* element stride between components */
+    int64_t batch_extent[SIL_MAX_DIM];
+    int64_t batch_stride[SIL_MAX_DIM];
+} sil_view;
+
+const char* sil_last_error(void);
+int32_t sil_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* geometry                                                            */
+/* ------------------------------------------------------------------ */
+
+typedef struct sil_grid_s* sil_grid;
+
+/* Node grid geometry: the reference's position and metric tensors
+ * (fill_discrete_hodge_star's `metric`, `position` arguments,
+ * src/similie/exterior/hodge_star.hpp:461-465).
+ *  axis_coords: host arrays, axis_coords[a][i] = coordinate of node i on
+ *               axis a (rectilinear grids); or NULL when nodal_positions
+ *               is given.
+ *  nodal_positions: device [node][ndim] positions (general grids) or NULL.
+ *  metric: device pointer per metric_kind (NULL for the identity); the
+ *          grid keeps the pointer, the caller keeps the memory alive. */
+int sil_grid_create(sil_grid* out, int32_t ndim, const int64_t* extent, const double* const* axis_coords,
+                    const double* nodal_positions, int32_t metric_kind, const double* metric);
+int sil_grid_destroy(sil_grid grid);
+
+/* ------------------------------------------------------------------ */
+/* sil::exterior operators                                             */
+/* ------------------------------------------------------------------ */
+
+/* deriv / coboundary (src/similie/exterior/coboundary.hpp:614-654,
+ * 698-709): k-form `in` → (k+1)-form `out`, forward differences with samples
+ * clamped into the input domain (evaluators.hpp:30-46).  out extents may be
+ * any size (the reference's output tensor domain). */
+int sil_coboundary(int32_t k, const double* in, const sil_view* in_view, double* out, const sil_view* out_view,
+                   void* stream);
+
+/* transposed_coboundary (coboundary.hpp:656-696): k-form → (k+1)-form,
+ * backward differences with zero outside the input domain, negated. */
+int sil_transposed_coboundary(int32_t k, const double* in, const sil_view* in_view, double* out,
+                              const sil_view* out_view, void* stream);
+
+/* fill_discrete_hodge_star<I(k indices), J(N-k indices)>
+ * (hodge_star.hpp:453-494): per-node coefficient tensor in the reference
+ * layout [node][N^k source tuple][C(N, N-k) target], index-last. */
+int sil_fill_discrete_hodge_star(sil_grid grid, int32_t k, double* hodge, void* stream);
+
+/* Discrete ⋆ applied to a k-form: the tensor_prod of the filled star with
+ * the form (codifferential.hpp:584) — k-form → dual (N-k)-form, on the grid's
+ * node domain. */
+int sil_hodge_star(sil_grid grid, int32_t k, const double* in, const sil_view* in_view, double* out,
+                   const sil_view* out_view, void* stream);
+
+/* codifferential (codifferential.hpp:551-656): k-form on the grid's node
+ * domain → (k-1)-form on out_view's extents (≤ grid extents). */
+int sil_codifferential(sil_grid grid, int32_t k, const double* in, const sil_view* in_view, double* out,
+                       const sil_view* out_view, void* stream);
+
+/* laplacian (laplacian.hpp:814-951): Hodge Laplacian δd + dδ of a k-form on
+ * the grid's node domain, written on out_view's extents (the reference's
+ * callers use the node domain minus the last node of every axis).  Fused
+ * single-pass kernels serve 2D 0-forms and 3D 1-forms on rectilinear grids
+ * with identity or diagonal metrics; every other case runs the general
+ * per-node kernel. */
+int sil_laplacian(sil_grid grid, int32_t k, const double* in, const sil_view* in_view, double* out,
+                  const sil_view* out_view, void* stream);
+
+/* make_staged_laplacian(...).run(out, in) (laplacian.hpp:312-358, 599-647):
+ * validates the request once and fixes the kernel and tile configuration. */
+typedef struct sil_staged_laplacian_s* sil_staged_laplacian;
+int sil_make_staged_laplacian(sil_staged_laplacian* out, sil_grid grid, int32_t k, const sil_view* in_view,
+                              const sil_view* out_view);
+int sil_staged_laplacian_run(sil_staged_laplacian op, double* out, const double* in, void* stream);
+int sil_staged_laplacian_destroy(sil_staged_laplacian op);
+
+/* ------------------------------------------------------------------ */
+/* linear operators and the CG solve                                   */
+/* ------------------------------------------------------------------ */
+
+typedef struct sil_operator_s* sil_operator;
+
+/* MagnetostaticsOperator3D (onelab_interface/magnetostatics_onelab.hpp
+ * :1524-2789) with precomputed-stencil semantics (:2241-2396): DOF order
+ * 3·(i + nx·(j + ny·k)) + c, identity rows on boundary nodes.  Coordinates
+ * are host arrays (copied); mu_nodal is a device array of nx·ny·nz nodal
+ * permeabilities (copied). */
+int sil_magnetostatics3d_create(sil_operator* out, int64_t nx, int64_t ny, int64_t nz, const double* x_coords,
+                                const double* y_coords, const double* z_coords, const double* mu_nodal,
+                                double gauge_penalty);
+
+/* Nodal permeability = mean over adjacent cells (magnetostatics_onelab.hpp
+ * :5029-5066); cell_mu is [(nx-1)(ny-1)(nz-1)] with cells ordered
+ * ci + (nx-1)(cj + (ny-1)ck). */
+int sil_magnetostatics3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const double* cell_mu, double mu0,
+                                  double* mu_nodal, void* stream);
+
+/* Right-hand side ⅛·|dual face|·J summed over the cells around each interior
+ * node (magnetostatics_onelab.hpp:4919-4975); cell_current is [cell][3]. */
+int sil_magnetostatics3d_rhs(sil_operator op, const double* cell_current, double* rhs, void* stream);
+
+/* Dirichlet-zero 2D Poisson operator d0ᵀ ⋆1^κ d0 on the interior nodes of a
+ * rectilinear nx×ny grid (x slowest): the ⋆0-weighted DEC 0-form Laplacian of
+ * the reference, with optional nodal coefficient κ (edge value = mean of its
+ * two nodes).  Vectors span all nx·ny nodes; boundary entries stay zero. */
+int sil_poisson2d_create(sil_operator* out, int64_t nx, int64_t ny, const double* x_coords,
+                         const double* y_coords, const double* kappa);
+
+int64_t sil_operator_size(sil_operator op);
+int sil_operator_apply(sil_operator op, const double* in, double* out, void* stream);
+int sil_operator_apply_f32(sil_operator op, const float* in, float* out, void* stream);
+int sil_operator_diagonal(sil_operator op, double* diag, void* stream);
+int sil_operator_destroy(sil_operator op);
+
+/* StrongFormulationSolverSettings / Diagnostics
+ * (src/similie/solvers/minimize_strong_formulation_residual.hpp:119-144) */
+typedef struct sil_solver_settings {
+    int32_t max_iterations;
+    double relative_tolerance;
+    int32_t precision;      /* enum sil_precision */
+    int32_t preconditioner; /* enum sil_preconditioner */
+    int32_t check_interval; /* iterations between host convergence checks (0 = auto) */
+    int32_t replace_interval; /* mixed precision: FP64 residual replacement period (0 = auto) */
+} sil_solver_settings;
+
+typedef struct sil_solver_diagnostics {
+    int32_t iterations;
+    int32_t converged;
+    double initial_residual_l2;
+    double final_residual_l2;       /* recurrence (or replaced) residual */
+    double final_relative_residual;
+    double true_relative_residual;  /* FP64 ‖b − A x‖2 / ‖b‖2 recomputed at exit */
+    double duration;                /* seconds, host wall clock around the solve */
+} sil_solver_diagnostics;
+
+void sil_default_solver_settings(sil_solver_settings* settings);
+
+/* minimize_strong_formulation_residual for a linear operator
+ * (minimize_strong_formulation_residual.hpp:1308-1382): x0 = 0, CG until
+ * ‖r‖2 ≤ tol·‖b‖2 or max_iterations.  Synchronises `stream` on return. */
+int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs, double* solution,
+                                             const sil_solver_settings* settings,
+                                             sil_solver_diagnostics* diagnostics, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIL_HIP_H */

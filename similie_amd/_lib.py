@@ -1,0 +1,138 @@
+"""Loads libsil_hip.so (the HIP C-ABI library) and declares its signatures.
+
+The library is the only compute path: there is no CPU fallback.  When it
+is missing this module raises instead of degrading.
+"""
+import ctypes
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libsil_hip.so"
+
+SIL_MAX_DIM = 4
+SIL_OK = 0
+
+METRIC_IDENTITY, METRIC_DIAGONAL, METRIC_SYMMETRIC = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1
+
+
+class SilError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"libsil_hip status {status}: {message}")
+        self.status = status
+
+
+class View(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int32),
+        ("nbatch", ctypes.c_int32),
+        ("extent", ctypes.c_int64 * SIL_MAX_DIM),
+        ("stride", ctypes.c_int64 * SIL_MAX_DIM),
+        ("comp_stride", ctypes.c_int64),
+        ("batch_extent", ctypes.c_int64 * SIL_MAX_DIM),
+        ("batch_stride", ctypes.c_int64 * SIL_MAX_DIM),
+    ]
+
+
+class SolverSettings(ctypes.Structure):
+    _fields_ = [
+        ("max_iterations", ctypes.c_int32),
+        ("relative_tolerance", ctypes.c_double),
+        ("precision", ctypes.c_int32),
+        ("preconditioner", ctypes.c_int32),
+        ("check_interval", ctypes.c_int32),
+        ("replace_interval", ctypes.c_int32),
+    ]
+
+
+class SolverDiagnostics(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("initial_residual_l2", ctypes.c_double),
+        ("final_residual_l2", ctypes.c_double),
+        ("final_relative_residual", ctypes.c_double),
+        ("true_relative_residual", ctypes.c_double),
+        ("duration", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_VP = ctypes.POINTER(View)
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "sil_last_error": (ctypes.c_char_p, []),
+    "sil_abi_version": (_I32, []),
+    "sil_grid_create": (ctypes.c_int, [ctypes.POINTER(_P), _I32, ctypes.POINTER(_I64), ctypes.POINTER(_P), _P, _I32, _P]),
+    "sil_grid_destroy": (ctypes.c_int, [_P]),
+    "sil_coboundary": (ctypes.c_int, [_I32, _P, _VP, _P, _VP, _P]),
+    "sil_transposed_coboundary": (ctypes.c_int, [_I32, _P, _VP, _P, _VP, _P]),
+    "sil_fill_discrete_hodge_star": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "sil_hodge_star": (ctypes.c_int, [_P, _I32, _P, _VP, _P, _VP, _P]),
+    "sil_codifferential": (ctypes.c_int, [_P, _I32, _P, _VP, _P, _VP, _P]),
+    "sil_laplacian": (ctypes.c_int, [_P, _I32, _P, _VP, _P, _VP, _P]),
+    "sil_make_staged_laplacian": (ctypes.c_int, [ctypes.POINTER(_P), _P, _I32, _VP, _VP]),
+    "sil_staged_laplacian_run": (ctypes.c_int, [_P, _P, _P, _P]),
+    "sil_staged_laplacian_destroy": (ctypes.c_int, [_P]),
+    "sil_magnetostatics3d_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _I64, _P, _P, _P, _P, _D]),
+    "sil_magnetostatics3d_nodal_mu": (ctypes.c_int, [_I64, _I64, _I64, _P, _D, _P, _P]),
+    "sil_magnetostatics3d_rhs": (ctypes.c_int, [_P, _P, _P, _P]),
+    "sil_poisson2d_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _P, _P, _P]),
+    "sil_operator_size": (_I64, [_P]),
+    "sil_operator_apply": (ctypes.c_int, [_P, _P, _P, _P]),
+    "sil_operator_apply_f32": (ctypes.c_int, [_P, _P, _P, _P]),
+    "sil_operator_diagonal": (ctypes.c_int, [_P, _P, _P]),
+    "sil_operator_destroy": (ctypes.c_int, [_P]),
+    "sil_default_solver_settings": (None, [ctypes.POINTER(SolverSettings)]),
+    "sil_minimize_strong_formulation_residual": (
+        ctypes.c_int,
+        [_P, _P, _P, ctypes.POINTER(SolverSettings), ctypes.POINTER(SolverDiagnostics), _P],
+    ),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m similie_amd.build` "
+                "(there is no CPU fallback for the HIP operators)"
+            )
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(status):
+    if status != SIL_OK:
+        msg = lib().sil_last_error()
+        raise SilError(status, msg.decode() if msg else "")
+    return status
+
+
+def stream_handle(stream=None):
+    """HIP stream handle of a torch stream (torch's current stream by default)."""
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None

@@ -1,0 +1,237 @@
+// extern "C" entry points of the linear operators and the CG solve.
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "operators.hpp"
+
+namespace sil {
+struct Workspace;
+void destroy_workspace(Workspace* w);
+int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_solver_settings& st,
+             sil_solver_diagnostics& dg, hipStream_t s);
+int operator_apply64(sil_operator op, const double* in, double* out, hipStream_t s);
+int operator_apply32(sil_operator op, const float* in, float* out, hipStream_t s);
+int operator_diagonal(sil_operator op, double* d, hipStream_t s);
+}  // namespace sil
+
+using namespace sil;
+
+namespace {
+
+template <typename T>
+int upload(T** dst, const std::vector<T>& src)
+{
+    SIL_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(dst), sizeof(T) * src.size()));
+    SIL_CHECK_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    return SIL_OK;
+}
+
+// local_spacing (onelab_interface/magnetostatics_onelab.hpp:856-870)
+double local_spacing(const double* c, int64_t n, int64_t i)
+{
+    if (n < 2) return 1.0;
+    if (i == 0) return c[1] - c[0];
+    if (i + 1 == n) return c[n - 1] - c[n - 2];
+    return 0.5 * (c[i + 1] - c[i - 1]);
+}
+
+void free_operator(sil_operator op)
+{
+    if (!op) return;
+    destroy_workspace(op->ws);
+    auto f = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    for (int q = 0; q < 3; ++q) {
+        f(op->ms.ls[q]);
+        f(op->ms.inv_ls[q]);
+        f(op->ms.coord[q]);
+        f(op->ms.ls32[q]);
+        f(op->ms.inv_ls32[q]);
+    }
+    f(op->ms.nu);
+    f(op->ms.nu32);
+    f(op->p2.wx);
+    f(op->p2.wy);
+    f(op->p2.wx32);
+    f(op->p2.wy32);
+    delete op;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sil_magnetostatics3d_create(sil_operator* out, int64_t nx, int64_t ny, int64_t nz, const double* x_coords,
+                                const double* y_coords, const double* z_coords, const double* mu_nodal,
+                                double gauge_penalty)
+{
+    SIL_REQUIRE(out && x_coords && y_coords && z_coords && mu_nodal, SIL_ERR_INVALID,
+                "magnetostatics3d_create: null argument");
+    SIL_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, SIL_ERR_INVALID, "magnetostatics3d_create: need ≥ 2 nodes per axis");
+    auto* op = new (std::nothrow) sil_operator_s();
+    SIL_REQUIRE(op != nullptr, SIL_ERR_INVALID, "magnetostatics3d_create: out of host memory");
+    op->kind = OP_MAGNETOSTATICS3D;
+    op->n = 3 * nx * ny * nz;
+    Ms3dData& d = op->ms;
+    d.nx = nx;
+    d.ny = ny;
+    d.nz = nz;
+    d.penalty = gauge_penalty;
+    const double* cs[3] = {x_coords, y_coords, z_coords};
+    const int64_t ns[3] = {nx, ny, nz};
+    int rc = SIL_OK;
+    for (int q = 0; q < 3 && rc == SIL_OK; ++q) {
+        std::vector<double> ls(ns[q]), inv(ns[q]), co(cs[q], cs[q] + ns[q]);
+        std::vector<float> ls32(ns[q]), inv32(ns[q]);
+        for (int64_t i = 0; i < ns[q]; ++i) {
+            ls[i] = local_spacing(cs[q], ns[q], i);
+            inv[i] = 1.0 / ls[i];
+            ls32[i] = (float)ls[i];
+            inv32[i] = (float)inv[i];
+        }
+        if ((rc = upload(&d.ls[q], ls)) || (rc = upload(&d.inv_ls[q], inv)) || (rc = upload(&d.coord[q], co)) ||
+            (rc = upload(&d.ls32[q], ls32)) || (rc = upload(&d.inv_ls32[q], inv32)))
+            break;
+    }
+    const int64_t nn = nx * ny * nz;
+    if (rc == SIL_OK) {
+        hipError_t e1 = hipMalloc(&d.nu, sizeof(double) * nn);
+        hipError_t e2 = hipMalloc(&d.nu32, sizeof(float) * nn);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            set_error("magnetostatics3d_create: device allocation failed");
+            rc = SIL_ERR_HIP;
+        } else {
+            rc = ms3d_nu_from_mu(mu_nodal, d.nu, d.nu32, nn, 0);
+            hipError_t e3 = hipDeviceSynchronize();
+            if (rc == SIL_OK && e3 != hipSuccess) {
+                set_error("magnetostatics3d_create: %s", hipGetErrorString(e3));
+                rc = SIL_ERR_HIP;
+            }
+        }
+    }
+    if (rc != SIL_OK) {
+        free_operator(op);
+        return rc;
+    }
+    *out = op;
+    return SIL_OK;
+}
+
+int sil_magnetostatics3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const double* cell_mu, double mu0,
+                                  double* mu_nodal, void* stream)
+{
+    SIL_REQUIRE(cell_mu && mu_nodal, SIL_ERR_INVALID, "magnetostatics3d_nodal_mu: null argument");
+    SIL_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, SIL_ERR_INVALID, "magnetostatics3d_nodal_mu: need ≥ 2 nodes per axis");
+    return ms3d_nodal_mu(nx, ny, nz, cell_mu, mu0, mu_nodal, as_stream(stream));
+}
+
+int sil_magnetostatics3d_rhs(sil_operator op, const double* cell_current, double* rhs, void* stream)
+{
+    SIL_REQUIRE(op && cell_current && rhs, SIL_ERR_INVALID, "magnetostatics3d_rhs: null argument");
+    SIL_REQUIRE(op->kind == OP_MAGNETOSTATICS3D, SIL_ERR_INVALID, "magnetostatics3d_rhs: not a magnetostatics operator");
+    return ms3d_rhs(op->ms, cell_current, rhs, as_stream(stream));
+}
+
+int sil_poisson2d_create(sil_operator* out, int64_t nx, int64_t ny, const double* x_coords, const double* y_coords,
+                         const double* kappa)
+{
+    SIL_REQUIRE(out && x_coords && y_coords, SIL_ERR_INVALID, "poisson2d_create: null argument");
+    SIL_REQUIRE(nx >= 3 && ny >= 3, SIL_ERR_INVALID, "poisson2d_create: need ≥ 3 nodes per axis");
+    auto* op = new (std::nothrow) sil_operator_s();
+    SIL_REQUIRE(op != nullptr, SIL_ERR_INVALID, "poisson2d_create: out of host memory");
+    op->kind = OP_POISSON2D;
+    op->n = nx * ny;
+    Poisson2dData& d = op->p2;
+    d.nx = nx;
+    d.ny = ny;
+    std::vector<double> hx(nx), hy(ny);
+    for (int64_t i = 0; i < nx; ++i) hx[i] = i + 1 < nx ? x_coords[i + 1] - x_coords[i] : x_coords[i] - x_coords[i - 1];
+    for (int64_t j = 0; j < ny; ++j) hy[j] = j + 1 < ny ? y_coords[j + 1] - y_coords[j] : y_coords[j] - y_coords[j - 1];
+    double *hxd = nullptr, *hyd = nullptr;
+    int rc = upload(&hxd, hx);
+    if (rc == SIL_OK) rc = upload(&hyd, hy);
+    const int64_t nn = nx * ny;
+    if (rc == SIL_OK) {
+        hipError_t e = hipMalloc(&d.wx, sizeof(double) * nn);
+        if (e == hipSuccess) e = hipMalloc(&d.wy, sizeof(double) * nn);
+        if (e == hipSuccess) e = hipMalloc(&d.wx32, sizeof(float) * nn);
+        if (e == hipSuccess) e = hipMalloc(&d.wy32, sizeof(float) * nn);
+        if (e != hipSuccess) {
+            set_error("poisson2d_create: device allocation failed");
+            rc = SIL_ERR_HIP;
+        }
+    }
+    if (rc == SIL_OK) rc = p2d_weights(d, hxd, hyd, kappa, 0);
+    if (rc == SIL_OK) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            set_error("poisson2d_create: %s", hipGetErrorString(e));
+            rc = SIL_ERR_HIP;
+        }
+    }
+    if (hxd) (void)hipFree(hxd);
+    if (hyd) (void)hipFree(hyd);
+    if (rc != SIL_OK) {
+        free_operator(op);
+        return rc;
+    }
+    *out = op;
+    return SIL_OK;
+}
+
+int64_t sil_operator_size(sil_operator op) { return op ? op->n : -1; }
+
+int sil_operator_apply(sil_operator op, const double* in, double* out, void* stream)
+{
+    SIL_REQUIRE(op && in && out, SIL_ERR_INVALID, "operator_apply: null argument");
+    return operator_apply64(op, in, out, as_stream(stream));
+}
+
+int sil_operator_apply_f32(sil_operator op, const float* in, float* out, void* stream)
+{
+    SIL_REQUIRE(op && in && out, SIL_ERR_INVALID, "operator_apply_f32: null argument");
+    return operator_apply32(op, in, out, as_stream(stream));
+}
+
+int sil_operator_diagonal(sil_operator op, double* diag, void* stream)
+{
+    SIL_REQUIRE(op && diag, SIL_ERR_INVALID, "operator_diagonal: null argument");
+    return operator_diagonal(op, diag, as_stream(stream));
+}
+
+int sil_operator_destroy(sil_operator op)
+{
+    free_operator(op);
+    return SIL_OK;
+}
+
+void sil_default_solver_settings(sil_solver_settings* s)
+{
+    if (!s) return;
+    // StrongFormulationSolverSettings defaults (minimize_strong_formulation_residual.hpp:119-134)
+    s->max_iterations = 2000;
+    s->relative_tolerance = 1.0e-12;
+    s->precision = SIL_PRECISION_FP64;
+    s->preconditioner = SIL_PRECOND_JACOBI;
+    s->check_interval = 0;
+    s->replace_interval = 0;
+}
+
+int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs, double* solution,
+                                             const sil_solver_settings* settings,
+                                             sil_solver_diagnostics* diagnostics, void* stream)
+{
+    SIL_REQUIRE(op && rhs && solution && settings && diagnostics, SIL_ERR_INVALID,
+                "minimize_strong_formulation_residual: null argument");
+    SIL_REQUIRE(settings->relative_tolerance >= 0.0, SIL_ERR_INVALID, "negative relative tolerance");
+    SIL_REQUIRE(settings->precision == SIL_PRECISION_FP64 || settings->precision == SIL_PRECISION_MIXED,
+                SIL_ERR_INVALID, "unknown precision mode %d", settings->precision);
+    SIL_REQUIRE(settings->preconditioner == SIL_PRECOND_IDENTITY || settings->preconditioner == SIL_PRECOND_JACOBI,
+                SIL_ERR_INVALID, "unknown preconditioner %d", settings->preconditioner);
+    *diagnostics = sil_solver_diagnostics{};
+    return cg_solve(op, rhs, solution, *settings, *diagnostics, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,484 @@
+// Fused single-pass kernels for the hot shapes of the DEC operators on
+// rectilinear grids with identity or diagonal metric (index-last arrays):
+//
+//  * 3D Hodge Laplacian of a 1-form, Δω = δdω + dδω
+//    (src/similie/exterior/laplacian.hpp:599-647 runs it as six kernels on two
+//    execution-space partitions with five HBM buffers and 90 precomputed ⋆
+//    doubles per node).  Here one pass reads ω and the diagonal metric once
+//    and writes Δω once: 72 algorithmic bytes per node.
+//  * 2D Laplacian of a 0-form (laplacian.hpp:27-155).
+//  * 3D coboundary of a 0-form (coboundary.hpp:614-654).
+//
+// Per node m, with s_a = sqrt(g_aa)·h_a (h_a the forward edge, backward at the
+// last node: volume.hpp:55-82) the discrete stars are volume ratios
+// (reduction_and_reconstruction.hpp:403-432):
+//   ⋆(YZ→X) = sx/(sy sz), ⋆(XZ→Y) = −sy/(sx sz), ⋆(XY→Z) = sz/(sx sy)
+//   ⋆(X→YZ) = sy sz/sx,   ⋆(Y→XZ) = −sx sz/sy,  ⋆(Z→XY) = sx sy/sz
+//   ⋆(XYZ→∅) = 1/(sx sy sz)
+// and all seven come from one division 1/(sx sy sz).  Nodes the reference
+// treats as degenerate (a reconstruction Jacobian below 1e-14, a zero metric
+// entry) get zero coefficients, as in the reference.
+//
+// The 3D kernel marches along x (the slowest axis): a workgroup owns a
+// 64 (z, one wavefront) × TY (y) tile including a one-node halo, keeps the
+// planes x−1, x, x+1 in registers, prefetches plane x+2 while it computes,
+// and exchanges the y/z neighbours of ω, ⋆ω, ⋆dω and δω through LDS.  Each
+// node's coefficients are computed once per tile.  Workgroups are mapped to
+// tiles XCD-contiguously so the halos shared by neighbouring tiles are served
+// from the same L2.
+#include "dec_kernels.hpp"
+
+namespace sil {
+
+namespace {
+
+struct Coefs {
+    double a1, a2, a3; // ⋆ 2-set → 1-set magnitudes (YZ→X, XZ→Y, XY→Z)
+    double b1, b2, b3; // ⋆ 1-set → 2-set magnitudes (X→YZ, Y→XZ, Z→XY)
+    double c;          // ⋆ XYZ → ∅
+};
+
+__device__ __forceinline__ Coefs node_coefs(double gx, double gy, double gz, double hx, double hy, double hz)
+{
+    Coefs k;
+    const bool metric_ok = (gx != 0.0) && (gy != 0.0) && (gz != 0.0);
+    const double ax = fabs(hx), ay = fabs(hy), az = fabs(hz);
+    const bool ok1 = metric_ok && ax >= 1e-14 && ay >= 1e-14 && az >= 1e-14;
+    const bool ok2 = metric_ok && fabs(hx * hy) >= 1e-14 && fabs(hx * hz) >= 1e-14 && fabs(hy * hz) >= 1e-14;
+    const bool ok3 = metric_ok && fabs(hx * hy * hz) >= 1e-14;
+    const double sx = sqrt(fabs(gx)) * ax, sy = sqrt(fabs(gy)) * ay, sz = sqrt(fabs(gz)) * az;
+    const double vol = sx * sy * sz;
+    const double inv = (vol != 0.0) ? 1.0 / vol : 0.0;
+    const double syz = sy * sz, sxz = sx * sz, sxy = sx * sy;
+    k.a1 = ok2 ? sx * sx * inv : 0.0;
+    k.a2 = ok2 ? sy * sy * inv : 0.0;
+    k.a3 = ok2 ? sz * sz * inv : 0.0;
+    k.b1 = ok1 ? syz * syz * inv : 0.0;
+    k.b2 = ok1 ? sxz * sxz * inv : 0.0;
+    k.b3 = ok1 ? sxy * sxy * inv : 0.0;
+    k.c = ok3 ? inv : 0.0;
+    return k;
+}
+
+// ------------------------------------------------------ 3D 1-form, direct ---
+
+// One thread per output node, every neighbour read through the caches.
+template <bool DIAG>
+__global__ __launch_bounds__(256) void k_lap3d_direct(const double* __restrict__ w, const double* __restrict__ g,
+                                                      const double* __restrict__ hx_, const double* __restrict__ hy_,
+                                                      const double* __restrict__ hz_, double* __restrict__ out,
+                                                      int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
+                                                      int64_t Oz)
+{
+    const int64_t total = Ox * Oy * Oz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t z = t % Oz, y = (t / Oz) % Oy, x = t / (Oz * Oy);
+        auto inside = [&](int64_t i, int64_t j, int64_t k) {
+            return i >= 0 && j >= 0 && k >= 0 && i < Nx && j < Ny && k < Nz;
+        };
+        auto W = [&](int64_t i, int64_t j, int64_t k, int c) {
+            i = min(i, Nx - 1);
+            j = min(j, Ny - 1);
+            k = min(k, Nz - 1);
+            return w[((i * Ny + j) * Nz + k) * 3 + c];
+        };
+        auto K = [&](int64_t i, int64_t j, int64_t k) {
+            const int64_t lin = (i * Ny + j) * Nz + k;
+            double gx = 1.0, gy = 1.0, gz = 1.0;
+            if (DIAG) {
+                gx = g[lin * 3 + 0];
+                gy = g[lin * 3 + 1];
+                gz = g[lin * 3 + 2];
+            }
+            return node_coefs(gx, gy, gz, hx_[i], hy_[j], hz_[k]);
+        };
+        // η = ⋆dω at m (zero outside)
+        auto eta = [&](int64_t i, int64_t j, int64_t k, double e[3]) {
+            if (!inside(i, j, k)) {
+                e[0] = e[1] = e[2] = 0.0;
+                return;
+            }
+            const Coefs c = K(i, j, k);
+            const double w0 = W(i, j, k, 0), w1 = W(i, j, k, 1), w2 = W(i, j, k, 2);
+            const double Fxy = (W(i + 1, j, k, 1) - w1) - (W(i, j + 1, k, 0) - w0);
+            const double Fxz = (W(i + 1, j, k, 2) - w2) - (W(i, j, k + 1, 0) - w0);
+            const double Fyz = (W(i, j + 1, k, 2) - w2) - (W(i, j, k + 1, 1) - w1);
+            e[0] = c.a1 * Fyz;
+            e[1] = -c.a2 * Fxz;
+            e[2] = c.a3 * Fxy;
+        };
+        // ⋆ω components (YZ, XZ, XY) at m (zero outside)
+        auto etap = [&](int64_t i, int64_t j, int64_t k, double e[3]) {
+            if (!inside(i, j, k)) {
+                e[0] = e[1] = e[2] = 0.0;
+                return;
+            }
+            const Coefs c = K(i, j, k);
+            e[0] = c.b1 * W(i, j, k, 0);
+            e[1] = -c.b2 * W(i, j, k, 1);
+            e[2] = c.b3 * W(i, j, k, 2);
+        };
+        // δω at m, clamped into the domain
+        auto dw = [&](int64_t i, int64_t j, int64_t k) {
+            i = min(i, Nx - 1);
+            j = min(j, Ny - 1);
+            k = min(k, Nz - 1);
+            double h0[3], hx[3], hy[3], hz[3];
+            etap(i, j, k, h0);
+            etap(i - 1, j, k, hx);
+            etap(i, j - 1, k, hy);
+            etap(i, j, k - 1, hz);
+            const double H = (h0[0] - hx[0]) - (h0[1] - hy[1]) + (h0[2] - hz[2]);
+            return -K(i, j, k).c * H;
+        };
+        double e0[3], ex[3], ey[3], ez[3];
+        eta(x, y, z, e0);
+        eta(x - 1, y, z, ex);
+        eta(x, y - 1, z, ey);
+        eta(x, y, z - 1, ez);
+        const double Gxy = (e0[1] - ex[1]) - (e0[0] - ey[0]);
+        const double Gxz = (e0[2] - ex[2]) - (e0[0] - ez[0]);
+        const double Gyz = (e0[2] - ey[2]) - (e0[1] - ez[1]);
+        const Coefs cn = K(min(x, Nx - 1), min(y, Ny - 1), min(z, Nz - 1));
+        const double d0 = dw(x, y, z);
+        double* o = out + ((x * Oy + y) * Oz + z) * 3;
+        o[0] = cn.a1 * Gyz + (dw(x + 1, y, z) - d0);
+        o[1] = -cn.a2 * Gxz + (dw(x, y + 1, z) - d0);
+        o[2] = cn.a3 * Gxy + (dw(x, y, z + 1) - d0);
+    }
+}
+
+// ------------------------------------------------------ 3D 1-form, march ---
+
+template <int TY>
+struct MarchLds {
+    double w[3][TY][64];   // ω(p) for the y+1 / z+1 neighbours of dω
+    double ep[2][TY][64];  // ⋆ω(p+1): XZ and XY components for δω's y−1 / z−1 terms
+    double eta[3][TY][64]; // ⋆dω(p) for dᵀ's y−1 / z−1 terms
+    double dw[2][TY][64];  // δω ping-pong: planes p and p+1
+};
+
+template <int TY, bool DIAG>
+__global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restrict__ w, const double* __restrict__ g,
+                                                         const double* __restrict__ hxs, const double* __restrict__ hys,
+                                                         const double* __restrict__ hzs, double* __restrict__ out,
+                                                         int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
+                                                         int64_t Oz, int ntz, int nty, int nxc, int xchunk)
+{
+    __shared__ MarchLds<TY> s;
+    const int lane = threadIdx.x & 63;
+    const int row = threadIdx.x >> 6;
+
+    // XCD-contiguous tile order: hardware places workgroup b on XCD b % 8.
+    const int nb = gridDim.x;
+    const int b = blockIdx.x;
+    const int per = (nb + 7) / 8;
+    const int rem = nb % 8;
+    int logical;
+    {
+        const int xcd = b % 8, slot = b / 8;
+        // XCDs [0, rem) own `per` logical blocks, the rest `per - 1` (rem == 0: all `per`)
+        const int full = rem == 0 ? 8 : rem;
+        logical = (xcd < full) ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
+    }
+    const int tz = logical % ntz;
+    const int ty = (logical / ntz) % nty;
+    const int tx = logical / (ntz * nty);
+    if (tx >= nxc) return; // uniform per workgroup
+
+    const int64_t z = (int64_t)tz * 62 - 1 + lane;
+    const int64_t y = (int64_t)ty * (TY - 2) - 1 + row;
+    const int64_t xa = (int64_t)tx * xchunk;
+    const int64_t xb = min(xa + xchunk, Ox);
+    const bool pos_in = (z >= 0 && z < Nz && y >= 0 && y < Ny);
+    const bool writer = lane >= 1 && lane <= 62 && row >= 1 && row <= TY - 2 && z < Oz && y < Oy;
+    const bool has_y1 = (y + 1 < Ny), has_z1 = (z + 1 < Nz);
+    const double hy = pos_in ? hys[y] : 0.0;
+    const double hz = pos_in ? hzs[z] : 0.0;
+    const int64_t plane = Ny * Nz;
+    const int64_t lin_yz = y * Nz + z;
+
+    auto load = [&](int64_t q, double W[3], double G[3], double& hxq) {
+        if (pos_in && q >= 0 && q < Nx) {
+            const int64_t lin = q * plane + lin_yz;
+            W[0] = w[lin * 3 + 0];
+            W[1] = w[lin * 3 + 1];
+            W[2] = w[lin * 3 + 2];
+            if (DIAG) {
+                G[0] = g[lin * 3 + 0];
+                G[1] = g[lin * 3 + 1];
+                G[2] = g[lin * 3 + 2];
+            } else {
+                G[0] = G[1] = G[2] = 1.0;
+            }
+            hxq = hxs[q];
+        } else {
+            W[0] = W[1] = W[2] = 0.0;
+            G[0] = G[1] = G[2] = 0.0; // outside: zero coefficients (zero-extension)
+            hxq = 0.0;
+        }
+    };
+    auto coefs_of = [&](const double G[3], double hxq, bool valid) {
+        if (!valid) return Coefs{0, 0, 0, 0, 0, 0, 0};
+        return node_coefs(G[0], G[1], G[2], hxq, hy, hz);
+    };
+
+    // state: c = plane p, n = plane p+1, f = prefetched plane p+2
+    double Wc[3], Wn[3], Wf[3], Gc[3], Gn[3], Gf[3], hc, hn, hf;
+    int64_t p = xa - 1;
+    load(p, Wc, Gc, hc);
+    load(p + 1, Wn, Gn, hn);
+    Coefs Kc = coefs_of(Gc, hc, pos_in && p >= 0 && p < Nx);
+    Coefs Kn = coefs_of(Gn, hn, pos_in && p + 1 < Nx);
+    double etaPrev[3] = {0.0, 0.0, 0.0};
+    // ⋆ω(p) own: YZ, XZ, XY with signs
+    double epc0 = Kc.b1 * Wc[0], epc1 = -Kc.b2 * Wc[1], epc2 = Kc.b3 * Wc[2];
+    (void)epc1;
+    (void)epc2;
+    double dwc = 0.0;
+    int buf = 0;
+
+    for (; p < xb; ++p) {
+        load(p + 2, Wf, Gf, hf); // prefetch, consumed at the end of the step
+
+        // ⋆ω(p+1) own
+        const double epn0 = Kn.b1 * Wn[0], epn1 = -Kn.b2 * Wn[1], epn2 = Kn.b3 * Wn[2];
+        s.w[0][row][lane] = Wc[0];
+        s.w[1][row][lane] = Wc[1];
+        s.w[2][row][lane] = Wc[2];
+        s.ep[0][row][lane] = epn1;
+        s.ep[1][row][lane] = epn2;
+        __syncthreads();
+
+        // dω(p) own, clamped forward differences
+        const double* Wx = (p + 1 < Nx) ? Wn : Wc;
+        const double wy0 = has_y1 && row + 1 < TY ? s.w[0][row + 1][lane] : Wc[0];
+        const double wy2 = has_y1 && row + 1 < TY ? s.w[2][row + 1][lane] : Wc[2];
+        const double wz0 = has_z1 && lane + 1 < 64 ? s.w[0][row][lane + 1] : Wc[0];
+        const double wz1 = has_z1 && lane + 1 < 64 ? s.w[1][row][lane + 1] : Wc[1];
+        const double Fxy = (Wx[1] - Wc[1]) - (wy0 - Wc[0]);
+        const double Fxz = (Wx[2] - Wc[2]) - (wz0 - Wc[0]);
+        const double Fyz = (wy2 - Wc[2]) - (wz1 - Wc[1]);
+        const double eta0 = Kc.a1 * Fyz, eta1 = -Kc.a2 * Fxz, eta2 = Kc.a3 * Fxy;
+
+        // δω(p+1) own: −⋆(dᵀ ⋆ω) with zero outside
+        const double ep_y1 = row > 0 ? s.ep[0][row - 1][lane] : 0.0;
+        const double ep_z2 = lane > 0 ? s.ep[1][row][lane - 1] : 0.0;
+        const double H = (epn0 - epc0) - (epn1 - ep_y1) + (epn2 - ep_z2);
+        const double dwn = -Kn.c * H;
+
+        s.eta[0][row][lane] = eta0;
+        s.eta[1][row][lane] = eta1;
+        s.eta[2][row][lane] = eta2;
+        s.dw[buf ^ 1][row][lane] = dwn;
+        __syncthreads();
+
+        if (p >= xa && writer) {
+            const double ey0 = s.eta[0][row - 1][lane], ey2 = s.eta[2][row - 1][lane];
+            const double ez0 = s.eta[0][row][lane - 1], ez1 = s.eta[1][row][lane - 1];
+            const double Gxy = (eta1 - etaPrev[1]) - (eta0 - ey0);
+            const double Gxz = (eta2 - etaPrev[2]) - (eta0 - ez0);
+            const double Gyz = (eta2 - ey2) - (eta1 - ez1);
+            const double ddx = (p + 1 < Nx) ? dwn - dwc : 0.0;
+            const double ddy = has_y1 ? s.dw[buf][row + 1][lane] - dwc : 0.0;
+            const double ddz = has_z1 ? s.dw[buf][row][lane + 1] - dwc : 0.0;
+            double* o = out + ((p * Oy + y) * Oz + z) * 3;
+            o[0] = Kc.a1 * Gyz + ddx;
+            o[1] = -Kc.a2 * Gxz + ddy;
+            o[2] = Kc.a3 * Gxy + ddz;
+        }
+
+        // rotate
+        etaPrev[0] = eta0;
+        etaPrev[1] = eta1;
+        etaPrev[2] = eta2;
+        epc0 = epn0;
+        dwc = dwn;
+        buf ^= 1;
+        Wc[0] = Wn[0];
+        Wc[1] = Wn[1];
+        Wc[2] = Wn[2];
+        Kc = Kn;
+        Wn[0] = Wf[0];
+        Wn[1] = Wf[1];
+        Wn[2] = Wf[2];
+        Kn = coefs_of(Gf, hf, pos_in && p + 2 >= 0 && p + 2 < Nx);
+    }
+}
+
+// ------------------------------------------------------ 2D 0-form, direct ---
+
+// Δf(n) = −(1/(sx sy)) [ (sy/sx)(n)(f(n+x)−f(n)) − (sy/sx)(n−x)(f(n)−f(n−x))
+//                       + (sx/sy)(n)(f(n+y)−f(n)) − (sx/sy)(n−y)(f(n)−f(n−y)) ]
+template <bool DIAG>
+__global__ __launch_bounds__(256) void k_lap2d_0form(const double* __restrict__ f, const double* __restrict__ g,
+                                                     const double* __restrict__ hxs, const double* __restrict__ hys,
+                                                     double* __restrict__ out, int64_t Nx, int64_t Ny, int64_t Ox,
+                                                     int64_t Oy)
+{
+    const int64_t total = Ox * Oy;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t % Oy, i = t / Oy;
+        // per-node coefficients: r = sy/sx (⋆ X→Y), q = sx/sy (⋆ Y→X magnitude), c = 1/(sx sy)
+        auto coef = [&](int64_t a, int64_t b, double& r, double& q, double& c) {
+            const int64_t lin = a * Ny + b;
+            const double gx = DIAG ? g[lin * 2 + 0] : 1.0, gy = DIAG ? g[lin * 2 + 1] : 1.0;
+            const double hx = hxs[a], hy = hys[b];
+            const bool mok = gx != 0.0 && gy != 0.0;
+            const bool ok1 = mok && fabs(hx) >= 1e-14 && fabs(hy) >= 1e-14;
+            const bool ok2 = mok && fabs(hx * hy) >= 1e-14;
+            const double sx = sqrt(fabs(gx)) * fabs(hx), sy = sqrt(fabs(gy)) * fabs(hy);
+            const double v = sx * sy;
+            const double inv = v != 0.0 ? 1.0 / v : 0.0;
+            r = ok1 ? sy * sy * inv : 0.0;
+            q = ok1 ? sx * sx * inv : 0.0;
+            c = ok2 ? inv : 0.0;
+        };
+        const int64_t ic = min(i, Nx - 1), jc = min(j, Ny - 1);
+        const double fc = f[ic * Ny + jc];
+        const double fe = f[min(ic + 1, Nx - 1) * Ny + jc];
+        const double fn = f[ic * Ny + min(jc + 1, Ny - 1)];
+        double r0, q0, c0;
+        coef(ic, jc, r0, q0, c0);
+        double G = r0 * (fe - fc) + q0 * (fn - fc);
+        if (ic > 0) {
+            double r1, q1, c1;
+            coef(ic - 1, jc, r1, q1, c1);
+            G -= r1 * (fc - f[(ic - 1) * Ny + jc]);
+        }
+        if (jc > 0) {
+            double r1, q1, c1;
+            coef(ic, jc - 1, r1, q1, c1);
+            G -= q1 * (fc - f[ic * Ny + jc - 1]);
+        }
+        out[i * Oy + j] = -c0 * G;
+    }
+}
+
+// ------------------------------------------------ 3D coboundary of 0-form ---
+
+__global__ __launch_bounds__(256) void k_cob3d_0form(const double* __restrict__ f, double* __restrict__ out,
+                                                     int64_t Nx, int64_t Ny, int64_t Nz)
+{
+    const int64_t total = Nx * Ny * Nz;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t z = t % Nz, y = (t / Nz) % Ny, x = t / (Nz * Ny);
+        const double c = f[t];
+        const double fx = x + 1 < Nx ? f[t + Ny * Nz] : c;
+        const double fy = y + 1 < Ny ? f[t + Nz] : c;
+        const double fz = z + 1 < Nz ? f[t + 1] : c;
+        out[t * 3 + 0] = fx - c;
+        out[t * 3 + 1] = fy - c;
+        out[t * 3 + 2] = fz - c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ in, double2* __restrict__ out,
+                                                     int64_t n2)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n2; t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = in[t];
+}
+
+inline unsigned blocks_for(int64_t total, int block, int64_t cap = 1 << 20)
+{
+    int64_t b = (total + block - 1) / block;
+    if (b > cap) b = cap;
+    if (b < 1) b = 1;
+    return static_cast<unsigned>(b);
+}
+
+template <int TY>
+int launch_march(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk,
+                 hipStream_t stream)
+{
+    const int ntz = static_cast<int>((o[2] + 61) / 62);
+    const int nty = static_cast<int>((o[1] + TY - 3) / (TY - 2));
+    const int nxc = static_cast<int>((o[0] + xchunk - 1) / xchunk);
+    const int64_t nb = (int64_t)ntz * nty * nxc;
+    SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
+    if (g.metric)
+        hipLaunchKernelGGL((k_lap3d_march<TY, true>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
+                           o[2], ntz, nty, nxc, xchunk);
+    else
+        hipLaunchKernelGGL((k_lap3d_march<TY, false>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
+                           o[2], ntz, nty, nxc, xchunk);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+}  // namespace
+
+int launch_laplacian_3d_1form(const FusedGeom& g, const double* in, double* out, const int64_t* o,
+                              const FusedTile& tile, hipStream_t stream)
+{
+    if (o[0] * o[1] * o[2] == 0) return SIL_OK;
+    if (tile.variant == FUSED_DIRECT) {
+        const int block = 256;
+        const unsigned nb = blocks_for(o[0] * o[1] * o[2], block);
+        if (g.metric)
+            hipLaunchKernelGGL((k_lap3d_direct<true>), dim3(nb), dim3(block), 0, stream, in, g.metric, g.spacing[0],
+                               g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1], o[2]);
+        else
+            hipLaunchKernelGGL((k_lap3d_direct<false>), dim3(nb), dim3(block), 0, stream, in, g.metric, g.spacing[0],
+                               g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1], o[2]);
+        SIL_LAUNCH_CHECK();
+        return SIL_OK;
+    }
+    const int xchunk = tile.xchunk > 0 ? tile.xchunk : 32;
+    switch (tile.ty) {
+    case 4:
+        return launch_march<4>(g, in, out, o, xchunk, stream);
+    case 8:
+        return launch_march<8>(g, in, out, o, xchunk, stream);
+    case 16:
+        return launch_march<16>(g, in, out, o, xchunk, stream);
+    default:
+        set_error("laplacian: unsupported tile height %d (4, 8 or 16)", tile.ty);
+        return SIL_ERR_INVALID;
+    }
+}
+
+int launch_laplacian_2d_0form(const FusedGeom& g, const double* in, double* out, int64_t ox, int64_t oy,
+                              hipStream_t stream)
+{
+    if (ox * oy == 0) return SIL_OK;
+    const int block = 256;
+    const unsigned nb = blocks_for(ox * oy, block);
+    if (g.metric)
+        hipLaunchKernelGGL((k_lap2d_0form<true>), dim3(nb), dim3(block), 0, stream, in, g.metric, g.spacing[0],
+                           g.spacing[1], out, g.ext[0], g.ext[1], ox, oy);
+    else
+        hipLaunchKernelGGL((k_lap2d_0form<false>), dim3(nb), dim3(block), 0, stream, in, g.metric, g.spacing[0],
+                           g.spacing[1], out, g.ext[0], g.ext[1], ox, oy);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int launch_coboundary_3d_0form(const double* in, double* out, const int64_t* ext, hipStream_t stream)
+{
+    const int64_t total = ext[0] * ext[1] * ext[2];
+    if (total == 0) return SIL_OK;
+    const int block = 256;
+    hipLaunchKernelGGL(k_cob3d_0form, dim3(blocks_for(total, block)), dim3(block), 0, stream, in, out, ext[0], ext[1],
+                       ext[2]);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int launch_stream_copy(const double* in, double* out, int64_t n, hipStream_t stream)
+{
+    SIL_REQUIRE(n % 2 == 0, SIL_ERR_INVALID, "stream copy: even element count required");
+    const int block = 256;
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks_for(n / 2, block, 4096 * 8)), dim3(block), 0, stream,
+                       reinterpret_cast<const double2*>(in), reinterpret_cast<double2*>(out), n / 2);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+}  // namespace sil

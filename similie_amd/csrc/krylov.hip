@@ -1,0 +1,590 @@
+// Conjugate gradients on the GPU, with the CG scalars kept on the device.
+//
+// The reference hands its matrix-free operator to Ginkgo's CG
+// (src/similie/solvers/minimize_strong_formulation_residual.hpp:1028-1304),
+// which synchronises with the host at every iteration for its ResidualNorm
+// stopping criterion.  Here one iteration is four kernels —
+//   K1 p = z + β p_old fused into q = A p with per-block ⟨p, q⟩ partials,
+//   K2 α = ⟨r, z⟩ / ⟨p, q⟩,
+//   K3 x += α p, r −= α q with per-block ⟨r, z⟩ and ⟨r, r⟩ partials,
+//   K4 β, the stopping test ‖r‖2 ≤ tol·‖b‖2 and the iteration counter —
+// every scalar stays in device memory, every kernel returns at once when the
+// device `done` flag is set, and a chunk of iterations is captured once into a
+// hipGraph and replayed; the host reads the flag once per chunk.  Partial sums
+// are reduced in a fixed order, so a solve is bit-reproducible.
+//
+// Mixed precision: the recurrence (r, p, q) and every operator application
+// inside the loop run in FP32 while x, all dot products and the scalars are
+// FP64; the residual is periodically replaced by the FP64 residual b − A x
+// (van der Vorst & Ye reliable updating), and convergence is declared only
+// on that FP64 residual.
+#include <chrono>
+#include <cmath>
+#include <new>
+
+#include "operators.hpp"
+
+namespace sil {
+
+struct CgScalars {
+    double rz, rz_prev, pq, alpha, beta, rr, bb, tol2bb;
+    double rr_true;
+    int done, iter, max_iter, converged;
+};
+
+struct Workspace {
+    int64_t n = 0;
+    // FP64 recurrence
+    double *x = nullptr, *b = nullptr, *r = nullptr, *q = nullptr, *p[2] = {nullptr, nullptr};
+    double* inv_diag = nullptr;
+    double* tmp = nullptr;
+    // FP32 recurrence
+    float *r32 = nullptr, *q32 = nullptr, *p32[2] = {nullptr, nullptr};
+    double* partials = nullptr; // [3][max blocks]
+    CgScalars* sc = nullptr;
+    CgScalars* host_sc = nullptr; // pinned
+    int apply_blocks = 0;
+    bool jacobi_ready = false;
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}; // [precision][precond]
+    int graph_chunk[2][2] = {{0, 0}, {0, 0}};
+};
+
+namespace {
+
+constexpr int kUpdBlock = 256;
+constexpr int kUpdBlocks = 2048;
+constexpr int kRedThreads = 1024;
+
+__device__ double block_sum(double v)
+{
+    __shared__ double sh[kRedThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) s += sh[i];
+    __syncthreads();
+    return s; // valid in thread 0
+}
+
+// fixed-order reduction of `count` partials by one workgroup
+__device__ double reduce_partials(const double* p, int count)
+{
+    double v = 0.0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) v += p[i];
+    return block_sum(v);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kUpdBlock) void k_dot2(const T* __restrict__ a, const T* __restrict__ b, int64_t n,
+                                                    double* __restrict__ partials)
+{
+    double v = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v += (double)a[i] * (double)b[i];
+    __shared__ double sh[kUpdBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < kUpdBlock / 64; ++i) s += sh[i];
+        partials[blockIdx.x] = s;
+    }
+}
+
+// r = b, x = 0, p_old = 0; partials for <b,b>, <r,z>
+template <typename T>
+__global__ __launch_bounds__(kUpdBlock) void k_init(const double* __restrict__ b, double* __restrict__ x,
+                                                    T* __restrict__ r, T* __restrict__ pold,
+                                                    const double* __restrict__ inv_diag, int64_t n,
+                                                    double* __restrict__ part_bb, double* __restrict__ part_rz)
+{
+    double vb = 0.0, vz = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double bi = b[i];
+        x[i] = 0.0;
+        r[i] = (T)bi;
+        pold[i] = T(0);
+        const double ri = (double)(T)bi;
+        vb += bi * bi;
+        vz += ri * (inv_diag ? ri * inv_diag[i] : ri);
+    }
+    __shared__ double sh[2][kUpdBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        vb += __shfl_down(vb, off, 64);
+        vz += __shfl_down(vz, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = vb;
+        sh[1][threadIdx.x >> 6] = vz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int i = 0; i < kUpdBlock / 64; ++i) {
+            s0 += sh[0][i];
+            s1 += sh[1][i];
+        }
+        part_bb[blockIdx.x] = s0;
+        part_rz[blockIdx.x] = s1;
+    }
+}
+
+__global__ void k_scalars_init(CgScalars* sc, const double* part_bb, const double* part_rz, int count, double tol,
+                               int max_iter)
+{
+    const double bb = reduce_partials(part_bb, count);
+    const double rz = reduce_partials(part_rz, count);
+    if (threadIdx.x == 0) {
+        sc->bb = bb;
+        sc->rr = bb;
+        sc->rz = rz;
+        sc->rz_prev = rz;
+        sc->beta = 0.0;
+        sc->alpha = 0.0;
+        sc->tol2bb = tol * tol * bb;
+        sc->iter = 0;
+        sc->max_iter = max_iter;
+        sc->converged = 0;
+        sc->rr_true = bb;
+        sc->done = (bb == 0.0 || max_iter <= 0) ? 1 : 0;
+    }
+}
+
+__global__ void k_alpha(CgScalars* sc, const double* part, int count)
+{
+    if (sc->done) return;
+    const double pq = reduce_partials(part, count);
+    if (threadIdx.x == 0) {
+        sc->pq = pq;
+        sc->alpha = sc->rz / pq;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kUpdBlock) void k_update(const CgScalars* __restrict__ sc, double* __restrict__ x,
+                                                      const T* __restrict__ p, T* __restrict__ r,
+                                                      const T* __restrict__ q, const double* __restrict__ inv_diag,
+                                                      int64_t n, double* __restrict__ part_rz,
+                                                      double* __restrict__ part_rr)
+{
+    if (sc->done) return;
+    const double alpha = sc->alpha;
+    double vz = 0.0, vr = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += alpha * (double)p[i];
+        const T ri = (T)((double)r[i] - alpha * (double)q[i]);
+        r[i] = ri;
+        const double rd = (double)ri;
+        vr += rd * rd;
+        vz += rd * (inv_diag ? rd * inv_diag[i] : rd);
+    }
+    __shared__ double sh[2][kUpdBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        vz += __shfl_down(vz, off, 64);
+        vr += __shfl_down(vr, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = vz;
+        sh[1][threadIdx.x >> 6] = vr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int i = 0; i < kUpdBlock / 64; ++i) {
+            s0 += sh[0][i];
+            s1 += sh[1][i];
+        }
+        part_rz[blockIdx.x] = s0;
+        part_rr[blockIdx.x] = s1;
+    }
+}
+
+__global__ void k_beta(CgScalars* sc, const double* part_rz, const double* part_rr, int count)
+{
+    if (sc->done) return;
+    const double rz = reduce_partials(part_rz, count);
+    const double rr = reduce_partials(part_rr, count);
+    if (threadIdx.x == 0) {
+        sc->beta = rz / sc->rz;
+        sc->rz_prev = sc->rz;
+        sc->rz = rz;
+        sc->rr = rr;
+        sc->iter += 1;
+        if (rr <= sc->tol2bb) {
+            sc->done = 1;
+            sc->converged = 1;
+        } else if (sc->iter >= sc->max_iter) {
+            sc->done = 1;
+        }
+    }
+}
+
+// FP64 residual replacement: r64 = b − Ax (tmp = Ax) → r (T), partials of
+// <r,r> (FP64, exact residual) and <r,z> of the rounded residual
+template <typename T>
+__global__ __launch_bounds__(kUpdBlock) void k_replace(const double* __restrict__ b, const double* __restrict__ ax,
+                                                       T* __restrict__ r, const double* __restrict__ inv_diag,
+                                                       int64_t n, double* __restrict__ part_rr,
+                                                       double* __restrict__ part_rz)
+{
+    double vr = 0.0, vz = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double res = b[i] - ax[i];
+        const T rt = (T)res;
+        r[i] = rt;
+        vr += res * res;
+        const double rd = (double)rt;
+        vz += rd * (inv_diag ? rd * inv_diag[i] : rd);
+    }
+    __shared__ double sh[2][kUpdBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        vr += __shfl_down(vr, off, 64);
+        vz += __shfl_down(vz, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = vr;
+        sh[1][threadIdx.x >> 6] = vz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int i = 0; i < kUpdBlock / 64; ++i) {
+            s0 += sh[0][i];
+            s1 += sh[1][i];
+        }
+        part_rr[blockIdx.x] = s0;
+        part_rz[blockIdx.x] = s1;
+    }
+}
+
+// After a replacement: keep the direction p and continue from the new
+// residual; β of the next direction update is recomputed from the new <r,z>.
+__global__ void k_after_replace(CgScalars* sc, const double* part_rr, const double* part_rz, int count)
+{
+    const double rr = reduce_partials(part_rr, count);
+    const double rz = reduce_partials(part_rz, count);
+    if (threadIdx.x == 0) {
+        sc->rr_true = rr;
+        sc->rr = rr;
+        sc->beta = rz / sc->rz_prev;
+        sc->rz = rz;
+        sc->done = 0;
+        sc->converged = 0;
+        if (rr <= sc->tol2bb) {
+            sc->done = 1;
+            sc->converged = 1;
+        } else if (sc->iter >= sc->max_iter) {
+            sc->done = 1;
+        }
+    }
+}
+
+__global__ void k_inv(const double* __restrict__ d, double* __restrict__ inv, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        inv[i] = d[i] != 0.0 ? 1.0 / d[i] : 0.0;
+}
+
+__global__ void k_residual_f64(const double* __restrict__ b, const double* __restrict__ ax, int64_t n,
+                               double* __restrict__ part)
+{
+    double v = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double d = b[i] - ax[i];
+        v += d * d;
+    }
+    __shared__ double sh[kUpdBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < kUpdBlock / 64; ++i) s += sh[i];
+        part[blockIdx.x] = s;
+    }
+}
+
+__global__ void k_store_true(CgScalars* sc, const double* part, int count)
+{
+    const double rr = reduce_partials(part, count);
+    if (threadIdx.x == 0) sc->rr_true = rr;
+}
+
+// ------------------------------------------------------------ dispatch ---
+
+int apply64(sil_operator op, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu, int* nb,
+            hipStream_t s)
+{
+    if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_apply64(op->ms, in, out, dot, pu, nb, s);
+    return p2d_apply64(op->p2, in, out, dot, pu, nb, s);
+}
+
+int apply32(sil_operator op, const float* in, float* out, DotEpilogue dot, const PUpdate<float>* pu, int* nb,
+            hipStream_t s)
+{
+    if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_apply32(op->ms, in, out, dot, pu, nb, s);
+    return p2d_apply32(op->p2, in, out, dot, pu, nb, s);
+}
+
+int diagonal(sil_operator op, double* d, hipStream_t s)
+{
+    if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_diagonal(op->ms, d, s);
+    return p2d_diagonal(op->p2, d, s);
+}
+
+template <typename P>
+int alloc(P** ptr, int64_t n)
+{
+    SIL_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(ptr), sizeof(P) * n));
+    return SIL_OK;
+}
+
+int ensure_workspace(sil_operator op, bool mixed, hipStream_t s)
+{
+    if (!op->ws) {
+        op->ws = new (std::nothrow) Workspace();
+        SIL_REQUIRE(op->ws != nullptr, SIL_ERR_INVALID, "solver: out of host memory");
+    }
+    Workspace& w = *op->ws;
+    const int64_t n = op->n;
+    int rc;
+    if (!w.x) {
+        w.n = n;
+        if ((rc = alloc(&w.x, n)) || (rc = alloc(&w.b, n)) || (rc = alloc(&w.tmp, n)) ||
+            (rc = alloc(&w.partials, 3 * max_dot_blocks())) || (rc = alloc(&w.sc, 1)))
+            return rc;
+        SIL_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&w.host_sc), sizeof(CgScalars)));
+        // number of blocks of one operator application
+        int nb = 0;
+        if (op->kind == OP_MAGNETOSTATICS3D)
+            nb = static_cast<int>((op->ms.nx * op->ms.ny * op->ms.nz + 255) / 256);
+        else
+            nb = static_cast<int>((op->p2.nx * op->p2.ny + 255) / 256);
+        w.apply_blocks = nb;
+        (void)s;
+    }
+    if (!mixed && !w.r) {
+        if ((rc = alloc(&w.r, n)) || (rc = alloc(&w.q, n)) || (rc = alloc(&w.p[0], n)) || (rc = alloc(&w.p[1], n)))
+            return rc;
+    }
+    if (mixed && !w.r32) {
+        if ((rc = alloc(&w.r32, n)) || (rc = alloc(&w.q32, n)) || (rc = alloc(&w.p32[0], n)) ||
+            (rc = alloc(&w.p32[1], n)))
+            return rc;
+    }
+    return SIL_OK;
+}
+
+// enqueue one CG iteration; `parity` selects the p ping-pong buffers
+template <typename T>
+int enqueue_iteration(sil_operator op, T* r, T* q, T* pold, T* pnew, bool jacobi, hipStream_t s)
+{
+    Workspace& w = *op->ws;
+    double* part0 = w.partials;
+    double* part1 = w.partials + max_dot_blocks();
+    double* part2 = w.partials + 2 * max_dot_blocks();
+    PUpdate<T> pu{r, pold, pnew, &w.sc->beta, jacobi ? w.inv_diag : nullptr, &w.sc->done};
+    int nb = 0;
+    int rc;
+    if constexpr (sizeof(T) == 8)
+        rc = apply64(op, nullptr, q, DotEpilogue{part0}, &pu, &nb, s);
+    else
+        rc = apply32(op, nullptr, q, DotEpilogue{part0}, &pu, &nb, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, nb);
+    hipLaunchKernelGGL((k_update<T>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.sc, w.x, pnew, r, q,
+                       jacobi ? w.inv_diag : nullptr, op->n, part1, part2);
+    hipLaunchKernelGGL(k_beta, dim3(1), dim3(kRedThreads), 0, s, w.sc, part1, part2, kUpdBlocks);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+template <typename T>
+int get_graph(sil_operator op, int prec, bool jacobi, int chunk, T* r, T* q, T* const* p, hipStream_t s,
+              hipGraphExec_t* out)
+{
+    Workspace& w = *op->ws;
+    hipGraphExec_t& g = w.graph[prec][jacobi ? 1 : 0];
+    if (g && w.graph_chunk[prec][jacobi ? 1 : 0] == chunk) {
+        *out = g;
+        return SIL_OK;
+    }
+    if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    hipStream_t cs;
+    SIL_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph;
+    SIL_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int rc = SIL_OK;
+    for (int it = 0; it < chunk && rc == SIL_OK; ++it)
+        rc = enqueue_iteration<T>(op, r, q, p[it & 1], p[(it + 1) & 1], jacobi, cs);
+    hipError_t e = hipStreamEndCapture(cs, &graph);
+    if (rc != SIL_OK) {
+        (void)hipStreamDestroy(cs);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        set_error("graph capture failed: %s", hipGetErrorString(e));
+        return SIL_ERR_HIP;
+    }
+    e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+        set_error("graph instantiation failed: %s", hipGetErrorString(e));
+        return SIL_ERR_HIP;
+    }
+    w.graph_chunk[prec][jacobi ? 1 : 0] = chunk;
+    (void)s;
+    *out = g;
+    return SIL_OK;
+}
+
+int read_scalars(Workspace& w, hipStream_t s)
+{
+    SIL_CHECK_HIP(hipMemcpyAsync(w.host_sc, w.sc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+    SIL_CHECK_HIP(hipStreamSynchronize(s));
+    return SIL_OK;
+}
+
+int true_residual(sil_operator op, hipStream_t s)
+{
+    Workspace& w = *op->ws;
+    int rc = apply64(op, w.x, w.tmp, DotEpilogue{nullptr}, nullptr, nullptr, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_residual_f64, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.tmp, op->n, w.partials);
+    hipLaunchKernelGGL(k_store_true, dim3(1), dim3(kRedThreads), 0, s, w.sc, w.partials, kUpdBlocks);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+}  // namespace
+
+void destroy_workspace(Workspace* w)
+{
+    if (!w) return;
+    for (double* p : {w->x, w->b, w->r, w->q, w->p[0], w->p[1], w->inv_diag, w->tmp, w->partials})
+        if (p) (void)hipFree(p);
+    for (float* p : {w->r32, w->q32, w->p32[0], w->p32[1]})
+        if (p) (void)hipFree(p);
+    if (w->sc) (void)hipFree(w->sc);
+    if (w->host_sc) (void)hipHostFree(w->host_sc);
+    for (auto& row : w->graph)
+        for (auto& g : row)
+            if (g) (void)hipGraphExecDestroy(g);
+    delete w;
+}
+
+int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_solver_settings& st,
+             sil_solver_diagnostics& dg, hipStream_t s)
+{
+    const bool mixed = st.precision == SIL_PRECISION_MIXED;
+    const bool jacobi = st.preconditioner == SIL_PRECOND_JACOBI;
+    int rc = ensure_workspace(op, mixed, s);
+    if (rc) return rc;
+    Workspace& w = *op->ws;
+    const int64_t n = op->n;
+    const auto t0 = std::chrono::steady_clock::now();
+
+    if (jacobi && !w.inv_diag) {
+        if ((rc = alloc(&w.inv_diag, n))) return rc;
+        w.jacobi_ready = false;
+    }
+    if (jacobi && !w.jacobi_ready) {
+        if ((rc = diagonal(op, w.tmp, s))) return rc;
+        hipLaunchKernelGGL(k_inv, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.tmp, w.inv_diag, n);
+        SIL_LAUNCH_CHECK();
+        w.jacobi_ready = true;
+    }
+    SIL_CHECK_HIP(hipMemcpyAsync(w.b, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    double* part0 = w.partials;
+    double* part1 = w.partials + max_dot_blocks();
+    const double* invd = jacobi ? w.inv_diag : nullptr;
+    if (mixed)
+        hipLaunchKernelGGL((k_init<float>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r32, w.p32[0], invd,
+                           n, part0, part1);
+    else
+        hipLaunchKernelGGL((k_init<double>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r, w.p[0], invd, n,
+                           part0, part1);
+    hipLaunchKernelGGL(k_scalars_init, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, part1, kUpdBlocks,
+                       st.relative_tolerance, st.max_iterations);
+    SIL_LAUNCH_CHECK();
+
+    int chunk = st.check_interval > 0 ? st.check_interval : (n > (1 << 20) ? 16 : 32);
+    chunk += chunk & 1; // even: the p ping-pong returns to buffer 0
+    const int replace = st.replace_interval > 0 ? st.replace_interval : 200;
+    hipGraphExec_t g = nullptr;
+    if (mixed)
+        rc = get_graph<float>(op, 1, jacobi, chunk, w.r32, w.q32, w.p32, s, &g);
+    else
+        rc = get_graph<double>(op, 0, jacobi, chunk, w.r, w.q, w.p, s, &g);
+    if (rc) return rc;
+
+    if ((rc = read_scalars(w, s))) return rc;
+    dg.initial_residual_l2 = std::sqrt(w.host_sc->bb);
+    int since_replace = 0;
+    double rr_ref = w.host_sc->bb;
+    while (!w.host_sc->done) {
+        const int iter_before = w.host_sc->iter;
+        SIL_CHECK_HIP(hipGraphLaunch(g, s));
+        since_replace += chunk;
+        if ((rc = read_scalars(w, s))) return rc;
+        if (!mixed) continue;
+        const CgScalars& h = *w.host_sc;
+        // iteration `it` of a chunk writes its direction into p32[(it + 1) & 1]:
+        // after an odd number of executed iterations the latest one sits in p32[1]
+        const int executed = h.iter - iter_before;
+        if (h.done && (executed & 1))
+            SIL_CHECK_HIP(hipMemcpyAsync(w.p32[0], w.p32[1], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        const bool hit_max = h.iter >= h.max_iter;
+        const bool want = h.converged || since_replace >= replace || h.rr < 1e-6 * rr_ref;
+        if (want && !hit_max) {
+            // FP64 residual replacement; keeps p (the direction buffer in use is p32[0])
+            if ((rc = apply64(op, w.x, w.tmp, DotEpilogue{nullptr}, nullptr, nullptr, s))) return rc;
+            hipLaunchKernelGGL((k_replace<float>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.tmp, w.r32, invd,
+                               n, part0, part1);
+            hipLaunchKernelGGL(k_after_replace, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, part1, kUpdBlocks);
+            SIL_LAUNCH_CHECK();
+            if ((rc = read_scalars(w, s))) return rc;
+            since_replace = 0;
+            rr_ref = w.host_sc->rr_true;
+        } else if (h.converged && hit_max) {
+            break;
+        }
+    }
+    if ((rc = true_residual(op, s))) return rc;
+    if ((rc = read_scalars(w, s))) return rc;
+    SIL_CHECK_HIP(hipMemcpyAsync(solution, w.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    SIL_CHECK_HIP(hipStreamSynchronize(s));
+    const CgScalars& h = *w.host_sc;
+    dg.iterations = h.iter;
+    const double bnorm = std::sqrt(h.bb);
+    dg.final_residual_l2 = std::sqrt(mixed ? h.rr_true : h.rr);
+    dg.final_relative_residual = bnorm == 0.0 ? 0.0 : dg.final_residual_l2 / bnorm;
+    dg.true_relative_residual = bnorm == 0.0 ? 0.0 : std::sqrt(h.rr_true) / bnorm;
+    dg.converged = (bnorm == 0.0) ? 1 : (dg.true_relative_residual <= st.relative_tolerance * (1.0 + 1e-6) ? 1 : 0);
+    if (!mixed) dg.converged = (bnorm == 0.0) ? 1 : h.converged;
+    dg.duration = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return SIL_OK;
+}
+
+int operator_apply64(sil_operator op, const double* in, double* out, hipStream_t s)
+{
+    return apply64(op, in, out, DotEpilogue{nullptr}, nullptr, nullptr, s);
+}
+
+int operator_apply32(sil_operator op, const float* in, float* out, hipStream_t s)
+{
+    return apply32(op, in, out, DotEpilogue{nullptr}, nullptr, nullptr, s);
+}
+
+int operator_diagonal(sil_operator op, double* d, hipStream_t s) { return diagonal(op, d, s); }
+
+}  // namespace sil

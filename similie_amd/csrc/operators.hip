@@ -1,0 +1,421 @@
+// Matrix-free operators for the CG solve.
+//
+// Magnetostatics 3D (onelab_interface/magnetostatics_onelab.hpp:2241-2396):
+// for an interior DOF r = (n, c)
+//   (A x)_r = Σ_s Σ_m C_m(s; r) · w^B_m(s) · ν(s) · B_m(s) + P · w^A_c(n) · x_r
+// with B = (dA)_yz, −(dA)_xz, (dA)_xy at interior nodes s (boundary DOFs
+// dropped), w^B_x = ℓx/(ℓy ℓz), w^A_x = ℓy ℓz/ℓx (cyclic), ℓ = local_spacing
+// (central, one-sided at the ends, :856-933); boundary rows are the identity.
+// The reference gathers through ≤24 precomputed transposed-moment slots per
+// row; here each thread evaluates the curl of its node's neighbourhood and the
+// transposed curl directly: out = curlᵀ(W ν curl x).
+//
+// Poisson 2D: d0ᵀ ⋆1^κ d0 on interior nodes with zero Dirichlet values.
+//
+// Both come in FP64 and FP32 and can (a) emit per-block partial sums of
+// ⟨in, out⟩ and (b) form the CG direction p = z + β p_old on the fly.
+#include <hipcub/hipcub.hpp>
+
+#include "operators.hpp"
+
+namespace sil {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int64_t kMaxDotBlocks = 1 << 18;
+
+template <typename T>
+__device__ __forceinline__ T p_value(const PUpdate<T>* pu, const T* x, int64_t idx)
+{
+    // value of the direction at row idx: either x[idx] or z + β p_old
+    if (pu == nullptr) return x[idx];
+    const double beta = *pu->beta;
+    const double z = pu->inv_diag ? (double)pu->r[idx] * pu->inv_diag[idx] : (double)pu->r[idx];
+    return (T)(z + beta * (double)pu->p_old[idx]);
+}
+
+template <typename T>
+__device__ __forceinline__ void block_dot_store(double v, double* partials)
+{
+    using BR = hipcub::BlockReduce<double, kBlock>;
+    __shared__ typename BR::TempStorage tmp;
+    double s = BR(tmp).Sum(v);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// --------------------------------------------------------- magnetostatics ---
+
+template <typename T>
+struct MsArgs {
+    int64_t nx, ny, nz;
+    T penalty;
+    const T* ls[3];
+    const T* inv_ls[3];
+    const T* nu;
+};
+
+template <typename T, bool PU>
+__global__ __launch_bounds__(kBlock) void k_ms3d_apply(MsArgs<T> a, const T* __restrict__ x, T* __restrict__ out,
+                                                       double* __restrict__ partials, PUpdate<T> pu)
+{
+    if (PU && *pu.done) return;
+    const int64_t nn = a.nx * a.ny * a.nz;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double dot = 0.0;
+    if (t < nn) {
+        const int64_t i = t % a.nx, j = (t / a.nx) % a.ny, k = t / (a.nx * a.ny);
+        const int64_t sx = 1, sy = a.nx, sz = a.nx * a.ny;
+        auto interior = [&](int64_t ii, int64_t jj, int64_t kk) {
+            return ii > 0 && jj > 0 && kk > 0 && ii + 1 < a.nx && jj + 1 < a.ny && kk + 1 < a.nz;
+        };
+        const PUpdate<T>* pp = PU ? &pu : nullptr;
+        // potential component c at node (ii,jj,kk); boundary DOFs are dropped
+        auto A = [&](int64_t ii, int64_t jj, int64_t kk, int c) -> T {
+            if (!interior(ii, jj, kk)) return T(0);
+            return p_value<T>(pp, x, 3 * (ii + a.nx * (jj + a.ny * kk)) + c);
+        };
+        // H_m(s) = w^B_m(s) ν(s) B_m(s) at interior s, else 0
+        auto Hx = [&](int64_t ii, int64_t jj, int64_t kk) -> T {
+            if (!interior(ii, jj, kk)) return T(0);
+            const T B = A(ii, jj + 1, kk, 2) - A(ii, jj, kk, 2) - A(ii, jj, kk + 1, 1) + A(ii, jj, kk, 1);
+            const T w = a.ls[0][ii] * a.inv_ls[1][jj] * a.inv_ls[2][kk];
+            return w * a.nu[ii + a.nx * (jj + a.ny * kk)] * B;
+        };
+        auto Hy = [&](int64_t ii, int64_t jj, int64_t kk) -> T {
+            if (!interior(ii, jj, kk)) return T(0);
+            const T B = -A(ii + 1, jj, kk, 2) + A(ii, jj, kk, 2) + A(ii, jj, kk + 1, 0) - A(ii, jj, kk, 0);
+            const T w = a.ls[1][jj] * a.inv_ls[0][ii] * a.inv_ls[2][kk];
+            return w * a.nu[ii + a.nx * (jj + a.ny * kk)] * B;
+        };
+        auto Hz = [&](int64_t ii, int64_t jj, int64_t kk) -> T {
+            if (!interior(ii, jj, kk)) return T(0);
+            const T B = A(ii + 1, jj, kk, 1) - A(ii, jj, kk, 1) - A(ii, jj + 1, kk, 0) + A(ii, jj, kk, 0);
+            const T w = a.ls[2][kk] * a.inv_ls[0][ii] * a.inv_ls[1][jj];
+            return w * a.nu[ii + a.nx * (jj + a.ny * kk)] * B;
+        };
+        T o[3];
+        T pv[3];
+        for (int c = 0; c < 3; ++c) pv[c] = p_value<T>(pp, x, 3 * t + c);
+        if (!interior(i, j, k)) {
+            o[0] = pv[0];
+            o[1] = pv[1];
+            o[2] = pv[2];
+        } else {
+            const T hx0 = Hx(i, j, k), hy0 = Hy(i, j, k), hz0 = Hz(i, j, k);
+            const T lx = a.ls[0][i], ly = a.ls[1][j], lz = a.ls[2][k];
+            const T ilx = a.inv_ls[0][i], ily = a.inv_ls[1][j], ilz = a.inv_ls[2][k];
+            o[0] = Hy(i, j, k - 1) - hy0 - Hz(i, j - 1, k) + hz0 + a.penalty * (ly * lz * ilx) * pv[0];
+            o[1] = -Hx(i, j, k - 1) + hx0 + Hz(i - 1, j, k) - hz0 + a.penalty * (lx * lz * ily) * pv[1];
+            o[2] = Hx(i, j - 1, k) - hx0 - Hy(i - 1, j, k) + hy0 + a.penalty * (lx * ly * ilz) * pv[2];
+        }
+        (void)sx;
+        (void)sy;
+        (void)sz;
+        for (int c = 0; c < 3; ++c) {
+            out[3 * t + c] = o[c];
+            if (PU) pu.p_new[3 * t + c] = pv[c];
+            dot += (double)pv[c] * (double)o[c];
+        }
+    }
+    if (partials) block_dot_store<T>(dot, partials);
+}
+
+__global__ void k_ms3d_diag(MsArgs<double> a, double* __restrict__ diag)
+{
+    const int64_t nn = a.nx * a.ny * a.nz;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nn) return;
+    const int64_t i = t % a.nx, j = (t / a.nx) % a.ny, k = t / (a.nx * a.ny);
+    auto interior = [&](int64_t ii, int64_t jj, int64_t kk) {
+        return ii > 0 && jj > 0 && kk > 0 && ii + 1 < a.nx && jj + 1 < a.ny && kk + 1 < a.nz;
+    };
+    if (!interior(i, j, k)) {
+        diag[3 * t] = diag[3 * t + 1] = diag[3 * t + 2] = 1.0;
+        return;
+    }
+    auto W = [&](int m, int64_t ii, int64_t jj, int64_t kk) -> double {
+        if (!interior(ii, jj, kk)) return 0.0;
+        const double nu = a.nu[ii + a.nx * (jj + a.ny * kk)];
+        if (m == 0) return a.ls[0][ii] * a.inv_ls[1][jj] * a.inv_ls[2][kk] * nu;
+        if (m == 1) return a.ls[1][jj] * a.inv_ls[0][ii] * a.inv_ls[2][kk] * nu;
+        return a.ls[2][kk] * a.inv_ls[0][ii] * a.inv_ls[1][jj] * nu;
+    };
+    const double lx = a.ls[0][i], ly = a.ls[1][j], lz = a.ls[2][k];
+    const double ilx = a.inv_ls[0][i], ily = a.inv_ls[1][j], ilz = a.inv_ls[2][k];
+    diag[3 * t + 0] = W(1, i, j, k - 1) + W(1, i, j, k) + W(2, i, j - 1, k) + W(2, i, j, k) + a.penalty * ly * lz * ilx;
+    diag[3 * t + 1] = W(0, i, j, k - 1) + W(0, i, j, k) + W(2, i - 1, j, k) + W(2, i, j, k) + a.penalty * lx * lz * ily;
+    diag[3 * t + 2] = W(0, i, j - 1, k) + W(0, i, j, k) + W(1, i - 1, j, k) + W(1, i, j, k) + a.penalty * lx * ly * ilz;
+}
+
+__global__ void k_ms3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const double* __restrict__ cell_mu, double mu0,
+                                double* __restrict__ mu)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nx * ny * nz) return;
+    const int64_t i = t % nx, j = (t / nx) % ny, k = t / (nx * ny);
+    const int64_t cx = nx - 1, cy = ny - 1, cz = nz - 1;
+    double acc = 0.0;
+    int cnt = 0;
+    for (int dk = -1; dk <= 0; ++dk)
+        for (int dj = -1; dj <= 0; ++dj)
+            for (int di = -1; di <= 0; ++di) {
+                const int64_t ci = i + di, cj = j + dj, ck = k + dk;
+                if (ci < 0 || cj < 0 || ck < 0 || ci >= cx || cj >= cy || ck >= cz) continue;
+                acc += cell_mu[ci + cx * (cj + cy * ck)];
+                ++cnt;
+            }
+    mu[t] = cnt == 0 ? mu0 : acc / (double)cnt;
+}
+
+__global__ void k_ms3d_rhs(int64_t nx, int64_t ny, int64_t nz, const double* __restrict__ xc,
+                           const double* __restrict__ yc, const double* __restrict__ zc,
+                           const double* __restrict__ cell_j, double* __restrict__ rhs)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nx * ny * nz) return;
+    const int64_t i = t % nx, j = (t / nx) % ny, k = t / (nx * ny);
+    const int64_t cx = nx - 1, cy = ny - 1, cz = nz - 1;
+    const bool boundary = i == 0 || j == 0 || k == 0 || i + 1 == nx || j + 1 == ny || k + 1 == nz;
+    for (int c = 0; c < 3; ++c) {
+        double acc = 0.0;
+        if (!boundary) {
+            for (int dk = -1; dk <= 0; ++dk)
+                for (int dj = -1; dj <= 0; ++dj)
+                    for (int di = -1; di <= 0; ++di) {
+                        const int64_t ci = i + di, cj = j + dj, ck = k + dk;
+                        if (ci < 0 || cj < 0 || ck < 0 || ci >= cx || cj >= cy || ck >= cz) continue;
+                        const double wx = xc[ci + 1] - xc[ci], wy = yc[cj + 1] - yc[cj], wz = zc[ck + 1] - zc[ck];
+                        const double face = c == 0 ? wy * wz : (c == 1 ? wx * wz : wx * wy);
+                        acc += 0.125 * face * cell_j[3 * (ci + cx * (cj + cy * ck)) + c];
+                    }
+        }
+        rhs[3 * t + c] = acc;
+    }
+}
+
+// ------------------------------------------------------------- Poisson 2D ---
+
+template <typename T, bool PU>
+__global__ __launch_bounds__(kBlock) void k_p2d_apply(int64_t nx, int64_t ny, const T* __restrict__ wx,
+                                                      const T* __restrict__ wy, const T* __restrict__ x,
+                                                      T* __restrict__ out, double* __restrict__ partials, PUpdate<T> pu)
+{
+    if (PU && *pu.done) return;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double dot = 0.0;
+    if (t < nx * ny) {
+        const int64_t j = t % ny, i = t / ny;
+        const PUpdate<T>* pp = PU ? &pu : nullptr;
+        const bool inner = i > 0 && j > 0 && i + 1 < nx && j + 1 < ny;
+        T pv = inner ? p_value<T>(pp, x, t) : T(0);
+        T o = T(0);
+        if (inner) {
+            const T ue = (i + 2 < nx) ? p_value<T>(pp, x, t + ny) : T(0);
+            const T uw = (i > 1) ? p_value<T>(pp, x, t - ny) : T(0);
+            const T un = (j + 2 < ny) ? p_value<T>(pp, x, t + 1) : T(0);
+            const T us = (j > 1) ? p_value<T>(pp, x, t - 1) : T(0);
+            o = wx[t] * (pv - ue) + wx[t - ny] * (pv - uw) + wy[t] * (pv - un) + wy[t - 1] * (pv - us);
+        }
+        out[t] = o;
+        if (PU) pu.p_new[t] = pv;
+        dot = (double)pv * (double)o;
+    }
+    if (partials) block_dot_store<T>(dot, partials);
+}
+
+__global__ void k_p2d_weights(int64_t nx, int64_t ny, const double* __restrict__ hx, const double* __restrict__ hy,
+                              const double* __restrict__ kappa, double* __restrict__ wx, double* __restrict__ wy,
+                              float* __restrict__ wx32, float* __restrict__ wy32)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nx * ny) return;
+    const int64_t j = t % ny, i = t / ny;
+    // ⋆1 of the edge at node (i,j): h_y/h_x along x, h_x/h_y along y (per-node rule)
+    double kx = 1.0, ky = 1.0;
+    if (kappa) {
+        kx = (i + 1 < nx) ? 0.5 * (kappa[t] + kappa[t + ny]) : kappa[t];
+        ky = (j + 1 < ny) ? 0.5 * (kappa[t] + kappa[t + 1]) : kappa[t];
+    }
+    const double a = kx * hy[j] / hx[i];
+    const double b = ky * hx[i] / hy[j];
+    wx[t] = a;
+    wy[t] = b;
+    wx32[t] = (float)a;
+    wy32[t] = (float)b;
+}
+
+__global__ void k_p2d_diag(int64_t nx, int64_t ny, const double* __restrict__ wx, const double* __restrict__ wy,
+                           double* __restrict__ diag)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nx * ny) return;
+    const int64_t j = t % ny, i = t / ny;
+    const bool inner = i > 0 && j > 0 && i + 1 < nx && j + 1 < ny;
+    diag[t] = inner ? wx[t] + wx[t - ny] + wy[t] + wy[t - 1] : 1.0;
+}
+
+__global__ void k_nu_from_mu(const double* __restrict__ mu, double* __restrict__ nu, float* __restrict__ nu32,
+                             int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = 1.0 / mu[i];
+        nu[i] = v;
+        nu32[i] = (float)v;
+    }
+}
+
+inline unsigned nblk(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+template <typename T>
+MsArgs<T> ms_args(const Ms3dData& d);
+
+template <>
+MsArgs<double> ms_args<double>(const Ms3dData& d)
+{
+    MsArgs<double> a{};
+    a.nx = d.nx;
+    a.ny = d.ny;
+    a.nz = d.nz;
+    a.penalty = d.penalty;
+    for (int q = 0; q < 3; ++q) {
+        a.ls[q] = d.ls[q];
+        a.inv_ls[q] = d.inv_ls[q];
+    }
+    a.nu = d.nu;
+    return a;
+}
+
+template <>
+MsArgs<float> ms_args<float>(const Ms3dData& d)
+{
+    MsArgs<float> a{};
+    a.nx = d.nx;
+    a.ny = d.ny;
+    a.nz = d.nz;
+    a.penalty = (float)d.penalty;
+    for (int q = 0; q < 3; ++q) {
+        a.ls[q] = d.ls32[q];
+        a.inv_ls[q] = d.inv_ls32[q];
+    }
+    a.nu = d.nu32;
+    return a;
+}
+
+template <typename T>
+int ms_apply(const Ms3dData& d, const T* in, T* out, DotEpilogue dot, const PUpdate<T>* pu, int* nb_out,
+             hipStream_t s)
+{
+    const int64_t nn = d.nx * d.ny * d.nz;
+    const unsigned nb = nblk(nn);
+    if (dot.partials && nb > kMaxDotBlocks) {
+        // the dot epilogue needs one slot per block
+        set_error("operator too large for the fused dot epilogue (%u blocks)", nb);
+        return SIL_ERR_UNSUPPORTED;
+    }
+    if (nb_out) *nb_out = (int)nb;
+    MsArgs<T> a = ms_args<T>(d);
+    if (pu)
+        hipLaunchKernelGGL((k_ms3d_apply<T, true>), dim3(nb), dim3(kBlock), 0, s, a, in, out, dot.partials, *pu);
+    else
+        hipLaunchKernelGGL((k_ms3d_apply<T, false>), dim3(nb), dim3(kBlock), 0, s, a, in, out, dot.partials,
+                           PUpdate<T>{});
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+template <typename T>
+int p2_apply(const Poisson2dData& d, const T* wx, const T* wy, const T* in, T* out, DotEpilogue dot,
+             const PUpdate<T>* pu, int* nb_out, hipStream_t s)
+{
+    const int64_t nn = d.nx * d.ny;
+    const unsigned nb = nblk(nn);
+    if (dot.partials && nb > kMaxDotBlocks) {
+        set_error("operator too large for the fused dot epilogue (%u blocks)", nb);
+        return SIL_ERR_UNSUPPORTED;
+    }
+    if (nb_out) *nb_out = (int)nb;
+    if (pu)
+        hipLaunchKernelGGL((k_p2d_apply<T, true>), dim3(nb), dim3(kBlock), 0, s, d.nx, d.ny, wx, wy, in, out,
+                           dot.partials, *pu);
+    else
+        hipLaunchKernelGGL((k_p2d_apply<T, false>), dim3(nb), dim3(kBlock), 0, s, d.nx, d.ny, wx, wy, in, out,
+                           dot.partials, PUpdate<T>{});
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+}  // namespace
+
+int64_t max_dot_blocks() { return kMaxDotBlocks; }
+
+int ms3d_apply64(const Ms3dData& d, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,
+                 int* nb, hipStream_t s)
+{
+    return ms_apply<double>(d, in, out, dot, pu, nb, s);
+}
+
+int ms3d_apply32(const Ms3dData& d, const float* in, float* out, DotEpilogue dot, const PUpdate<float>* pu, int* nb,
+                 hipStream_t s)
+{
+    return ms_apply<float>(d, in, out, dot, pu, nb, s);
+}
+
+int ms3d_diagonal(const Ms3dData& d, double* diag, hipStream_t s)
+{
+    const int64_t nn = d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL(k_ms3d_diag, dim3(nblk(nn)), dim3(kBlock), 0, s, ms_args<double>(d), diag);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int ms3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const double* cell_mu, double mu0, double* mu, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ms3d_nodal_mu, dim3(nblk(nx * ny * nz)), dim3(kBlock), 0, s, nx, ny, nz, cell_mu, mu0, mu);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int ms3d_rhs(const Ms3dData& d, const double* cell_current, double* rhs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ms3d_rhs, dim3(nblk(d.nx * d.ny * d.nz)), dim3(kBlock), 0, s, d.nx, d.ny, d.nz, d.coord[0],
+                       d.coord[1], d.coord[2], cell_current, rhs);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int ms3d_nu_from_mu(const double* mu, double* nu, float* nu32, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nu_from_mu, dim3(1024), dim3(kBlock), 0, s, mu, nu, nu32, n);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int p2d_apply64(const Poisson2dData& d, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,
+                int* nb, hipStream_t s)
+{
+    return p2_apply<double>(d, d.wx, d.wy, in, out, dot, pu, nb, s);
+}
+
+int p2d_apply32(const Poisson2dData& d, const float* in, float* out, DotEpilogue dot, const PUpdate<float>* pu,
+                int* nb, hipStream_t s)
+{
+    return p2_apply<float>(d, d.wx32, d.wy32, in, out, dot, pu, nb, s);
+}
+
+int p2d_diagonal(const Poisson2dData& d, double* diag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_p2d_diag, dim3(nblk(d.nx * d.ny)), dim3(kBlock), 0, s, d.nx, d.ny, d.wx, d.wy, diag);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int p2d_weights(Poisson2dData& d, const double* hx_dev, const double* hy_dev, const double* kappa, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_p2d_weights, dim3(nblk(d.nx * d.ny)), dim3(kBlock), 0, s, d.nx, d.ny, hx_dev, hy_dev, kappa,
+                       d.wx, d.wy, d.wx32, d.wy32);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+}  // namespace sil

@@ -1,0 +1,78 @@
+// Linear operators behind sil_operator and the solver workspace (host side).
+#pragma once
+
+#include "common.hpp"
+
+namespace sil {
+
+enum OperatorKind { OP_MAGNETOSTATICS3D = 1, OP_POISSON2D = 2 };
+
+struct Workspace; // CG vectors, scalars and captured graphs (krylov.hip)
+
+struct Ms3dData {
+    int64_t nx, ny, nz;
+    double penalty;
+    double* ls[3];      // local_spacing per axis (device)
+    double* inv_ls[3];  // 1 / local_spacing
+    double* coord[3];   // coordinates (device)
+    double* nu;         // 1/μ per node (device, FP64)
+    float* nu32;        // FP32 copy
+    float* ls32[3];
+    float* inv_ls32[3];
+};
+
+struct Poisson2dData {
+    int64_t nx, ny;
+    double* wx;   // edge weight (i,j)→(i+1,j), per node (device)
+    double* wy;   // edge weight (i,j)→(i,j+1)
+    float* wx32;
+    float* wy32;
+};
+
+// Optional epilogue of an operator application: per-block partial sums of
+// ⟨in, out⟩ written to `partials[blockIdx]` (deterministic reduction later).
+struct DotEpilogue {
+    double* partials; // null: no dot
+};
+
+// CG "p = z + β p_old" prologue fused into an application: when enabled the
+// kernel forms p from (r, p_old, β, diag) for every node it reads, writes p
+// for its own rows, and applies the operator to the new p.
+template <typename T>
+struct PUpdate {
+    const T* r;       // residual
+    const T* p_old;
+    T* p_new;
+    const double* beta;  // device scalar
+    const double* inv_diag; // Jacobi (FP64) or null
+    const int* done;     // device flag: skip when set
+};
+
+int ms3d_apply64(const Ms3dData& d, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,
+                 int* nblocks_out, hipStream_t s);
+int ms3d_apply32(const Ms3dData& d, const float* in, float* out, DotEpilogue dot, const PUpdate<float>* pu,
+                 int* nblocks_out, hipStream_t s);
+int ms3d_diagonal(const Ms3dData& d, double* diag, hipStream_t s);
+int ms3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const double* cell_mu, double mu0, double* mu, hipStream_t s);
+int ms3d_rhs(const Ms3dData& d, const double* cell_current, double* rhs, hipStream_t s);
+int ms3d_nu_from_mu(const double* mu, double* nu, float* nu32, int64_t n, hipStream_t s);
+
+int p2d_apply64(const Poisson2dData& d, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,
+                int* nblocks_out, hipStream_t s);
+int p2d_apply32(const Poisson2dData& d, const float* in, float* out, DotEpilogue dot, const PUpdate<float>* pu,
+                int* nblocks_out, hipStream_t s);
+int p2d_diagonal(const Poisson2dData& d, double* diag, hipStream_t s);
+int p2d_weights(Poisson2dData& d, const double* hx_host, const double* hy_host, const double* kappa, hipStream_t s);
+
+// number of partial-sum slots an application with a dot epilogue may write
+int64_t max_dot_blocks();
+
+}  // namespace sil
+
+struct sil_operator_s {
+    int kind;
+    int64_t n;
+    sil::Ms3dData ms;
+    sil::Poisson2dData p2;
+    sil::Workspace* ws;
+};

@@ -1,0 +1,147 @@
+"""Linear operators and the strong-formulation CG solve on MI355X.
+
+Python mirror of the reference's solver boundary
+(src/similie/solvers/minimize_strong_formulation_residual.hpp:119-144,
+1308-1382) and of the 3D magnetostatics operator
+(onelab_interface/magnetostatics_onelab.hpp:1524-2789).  Vectors are torch
+float64 CUDA tensors; everything runs through libsil_hip.so.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib as L
+
+
+def _f64(t, name):
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise TypeError(f"{name} must be a contiguous float64 CUDA tensor")
+    return t
+
+
+def _host_array(seq):
+    vals = seq.tolist() if hasattr(seq, "tolist") else list(seq)
+    return (ctypes.c_double * len(vals))(*[float(v) for v in vals])
+
+
+class LinearOperator:
+    """A sil_operator handle: apply (FP64), apply_f32, diagonal."""
+
+    handle = None
+
+    def size(self):
+        return int(L.lib().sil_operator_size(self.handle))
+
+    def apply(self, x, out=None, stream=None):
+        _f64(x, "x")
+        out = torch.empty_like(x) if out is None else _f64(out, "out")
+        L.check(L.lib().sil_operator_apply(self.handle, L.ptr(x), L.ptr(out), L.stream_handle(stream)))
+        return out
+
+    def apply_f32(self, x, out=None, stream=None):
+        if x.dtype != torch.float32 or not x.is_cuda:
+            raise TypeError("x must be a float32 CUDA tensor")
+        out = torch.empty_like(x) if out is None else out
+        L.check(L.lib().sil_operator_apply_f32(self.handle, L.ptr(x), L.ptr(out), L.stream_handle(stream)))
+        return out
+
+    def diagonal(self, stream=None):
+        d = torch.empty(self.size(), dtype=torch.float64, device="cuda")
+        L.check(L.lib().sil_operator_diagonal(self.handle, L.ptr(d), L.stream_handle(stream)))
+        return d
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and L._lib is not None:
+            L.lib().sil_operator_destroy(h)
+            self.handle = None
+
+
+class MagnetostaticsOperator3D(LinearOperator):
+    """MagnetostaticsOperator3D with precomputed-stencil semantics: DOF order
+    3·(i + nx·(j + ny·k)) + c, identity rows on boundary nodes, gauge
+    penalty P·w^A on interior rows."""
+
+    def __init__(self, x_coords, y_coords, z_coords, mu_nodal, gauge_penalty=20.0):
+        self.shape = (len(x_coords), len(y_coords), len(z_coords))
+        _f64(mu_nodal, "mu_nodal")
+        h = ctypes.c_void_p()
+        xs, ys, zs = _host_array(x_coords), _host_array(y_coords), _host_array(z_coords)
+        nx, ny, nz = self.shape
+        L.check(L.lib().sil_magnetostatics3d_create(ctypes.byref(h), nx, ny, nz, ctypes.cast(xs, ctypes.c_void_p),
+                                                    ctypes.cast(ys, ctypes.c_void_p), ctypes.cast(zs, ctypes.c_void_p),
+                                                    L.ptr(mu_nodal), float(gauge_penalty)))
+        self.handle = h
+
+    @staticmethod
+    def nodal_mu(shape, cell_mu, mu0, stream=None):
+        """mean of the adjacent cells' μ (magnetostatics_onelab.hpp:5029-5066)"""
+        nx, ny, nz = shape
+        _f64(cell_mu, "cell_mu")
+        mu = torch.empty(nx * ny * nz, dtype=torch.float64, device="cuda")
+        L.check(L.lib().sil_magnetostatics3d_nodal_mu(nx, ny, nz, L.ptr(cell_mu), float(mu0), L.ptr(mu),
+                                                      L.stream_handle(stream)))
+        return mu
+
+    def rhs(self, cell_current, stream=None):
+        """⅛·|dual face|·J over the cells around each interior node
+        (magnetostatics_onelab.hpp:4919-4975); cell_current: (cells, 3)."""
+        _f64(cell_current, "cell_current")
+        b = torch.empty(self.size(), dtype=torch.float64, device="cuda")
+        L.check(L.lib().sil_magnetostatics3d_rhs(self.handle, L.ptr(cell_current), L.ptr(b), L.stream_handle(stream)))
+        return b
+
+
+class Poisson2D(LinearOperator):
+    """Dirichlet-zero d0ᵀ ⋆1^κ d0 on the interior of a rectilinear nx×ny grid
+    (x slowest); vectors have nx·ny entries with zero boundary entries."""
+
+    def __init__(self, x_coords, y_coords, kappa=None):
+        self.shape = (len(x_coords), len(y_coords))
+        if kappa is not None:
+            _f64(kappa, "kappa")
+        self.kappa = kappa
+        h = ctypes.c_void_p()
+        xs, ys = _host_array(x_coords), _host_array(y_coords)
+        L.check(L.lib().sil_poisson2d_create(ctypes.byref(h), self.shape[0], self.shape[1],
+                                             ctypes.cast(xs, ctypes.c_void_p), ctypes.cast(ys, ctypes.c_void_p),
+                                             L.ptr(kappa)))
+        self.handle = h
+
+
+@dataclass
+class StrongFormulationSolverSettings:
+    """minimize_strong_formulation_residual.hpp:119-134 (plus the precision
+    mode and the host polling interval of the GPU solver)."""
+
+    max_iterations: int = 2000
+    relative_tolerance: float = 1.0e-12
+    preconditioner: str = "jacobi"  # "jacobi" | "identity"
+    precision: str = "fp64"  # "fp64" | "mixed"
+    check_interval: int = 0
+    replace_interval: int = 0
+
+    def to_c(self):
+        s = L.SolverSettings()
+        s.max_iterations = int(self.max_iterations)
+        s.relative_tolerance = float(self.relative_tolerance)
+        s.precision = {"fp64": L.PRECISION_FP64, "mixed": L.PRECISION_MIXED}[self.precision]
+        s.preconditioner = {"identity": L.PRECOND_IDENTITY, "jacobi": L.PRECOND_JACOBI}[self.preconditioner]
+        s.check_interval = int(self.check_interval)
+        s.replace_interval = int(self.replace_interval)
+        return s
+
+
+def minimize_strong_formulation_residual(operator, rhs, solution=None, settings=None, stream=None):
+    """x0 = 0, CG until ‖r‖2 ≤ tol·‖b‖2 (minimize_strong_formulation_residual
+    .hpp:1308-1382).  Returns (solution, diagnostics dict)."""
+    settings = settings or StrongFormulationSolverSettings()
+    _f64(rhs, "rhs")
+    solution = torch.empty_like(rhs) if solution is None else _f64(solution, "solution")
+    diag = L.SolverDiagnostics()
+    cs = settings.to_c()
+    L.check(L.lib().sil_minimize_strong_formulation_residual(operator.handle, L.ptr(rhs), L.ptr(solution),
+                                                             ctypes.byref(cs), ctypes.byref(diag),
+                                                             L.stream_handle(stream)))
+    return solution, diag.as_dict()

@@ -270,3 +270,36 @@ def poisson2d_cg(hx, hy, kappa, b, tol, max_it):
     it = lib().orc_poisson2d_cg(ctypes.c_long(nx), ctypes.c_long(ny), _d(hx), _d(hy), kp, _d(bb),
                                 ctypes.c_double(tol), ctypes.c_int(max_it), _d(x), _d(res))
     return x, it, res[0], res[1]
+
+
+# ------------------------------------------------------- CPU baseline ---
+
+class StagedLaplacian3D1:
+    """Reference-structured CPU staged Laplacian (90 ⋆ doubles per node, six
+    OpenMP passes) used as bench.py's CPU baseline."""
+
+    def __init__(self, ext, axes, metric_diag):
+        L = lib()
+        L.orc_staged_lap3d1_create.restype = ctypes.c_void_p
+        L.orc_staged_lap3d1_run.argtypes = [ctypes.c_void_p, _dp, _dp, _lp]
+        L.orc_staged_lap3d1_destroy.argtypes = [ctypes.c_void_p]
+        self.ext = [int(e) for e in ext]
+        self.axes = [_f64(a) for a in axes]
+        self.metric = _f64(metric_diag) if metric_diag is not None else None
+        ax = (_dp * 3)(*[_d(a) for a in self.axes])
+        self.h = L.orc_staged_lap3d1_create(_l(self.ext), ax, _d(self.metric) if self.metric is not None else None)
+
+    def run(self, w, out_ext):
+        out = np.zeros(tuple(out_ext) + (3,))
+        ww = _f64(w)
+        lib().orc_staged_lap3d1_run(self.h, _d(ww), _d(out), _l(out_ext))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_staged_lap3d1_destroy(self.h)
+            self.h = None
+
+
+def threads():
+    return int(lib().orc_threads())
