@@ -202,6 +202,42 @@ int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs,
                                              const sil_solver_settings* settings,
                                              sil_solver_diagnostics* diagnostics, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* batched persistent solves                                           */
+/* ------------------------------------------------------------------ */
+
+/* No reference counterpart (the reference runs one Ginkgo solve per
+ * operator): many independent Dirichlet-zero 2D Poisson problems of uneven
+ * size on unit-spaced square grids, solved in one persistent launch captured
+ * in a hipGraph.  Problem p has sides[p]² nodes (x slowest) stored at offset
+ * sil_batched_poisson2d_offset(h, p) of every concatenated array; kappa is
+ * the nodal coefficient (device, concatenated; edge value = mean of its two
+ * nodes).  The solve is CG with FP32 operator applications and recurrence
+ * and FP64 x, dots and residual replacement; a problem converges only on its
+ * FP64 residual ‖b − A x‖2 ≤ tol·‖b‖2. */
+typedef struct sil_batch_result {
+    int32_t iterations;
+    int32_t converged;
+    double true_relative_residual;
+} sil_batch_result;
+
+typedef struct sil_batched_poisson_s* sil_batched_poisson;
+int sil_batched_poisson2d_create(sil_batched_poisson* out, int32_t nproblems, const int32_t* sides,
+                                 const double* kappa);
+int64_t sil_batched_poisson2d_offset(sil_batched_poisson h, int32_t problem);
+/* workgroup size of the persistent kernel (256, 512 or 1024) */
+int sil_batched_poisson2d_set_block(sil_batched_poisson h, int32_t block);
+/* times every workgroup size on `probe_iterations` iterations of every
+ * problem and keeps the fastest */
+int sil_batched_poisson2d_autotune(sil_batched_poisson h, const double* rhs, int32_t probe_iterations,
+                                   int32_t* chosen_block, double* seconds_per_candidate);
+/* results: host array of nproblems entries (or NULL); seconds: device time
+ * of the graph launch (or NULL) */
+int sil_batched_poisson2d_solve(sil_batched_poisson h, const double* rhs, double* solution,
+                                const sil_solver_settings* settings, sil_batch_result* results, double* seconds,
+                                void* stream);
+int sil_batched_poisson2d_destroy(sil_batched_poisson h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,14 @@ class SolverDiagnostics(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchResult(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("true_relative_residual", ctypes.c_double),
+    ]
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -96,6 +104,18 @@ SIGNATURES = {
         ctypes.c_int,
         [_P, _P, _P, ctypes.POINTER(SolverSettings), ctypes.POINTER(SolverDiagnostics), _P],
     ),
+    "sil_batched_poisson2d_create": (ctypes.c_int, [ctypes.POINTER(_P), _I32, ctypes.POINTER(_I32), _P]),
+    "sil_batched_poisson2d_offset": (_I64, [_P, _I32]),
+    "sil_batched_poisson2d_set_block": (ctypes.c_int, [_P, _I32]),
+    "sil_batched_poisson2d_autotune": (
+        ctypes.c_int,
+        [_P, _P, _I32, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)],
+    ),
+    "sil_batched_poisson2d_solve": (
+        ctypes.c_int,
+        [_P, _P, _P, ctypes.POINTER(SolverSettings), ctypes.POINTER(BatchResult), ctypes.POINTER(ctypes.c_double), _P],
+    ),
+    "sil_batched_poisson2d_destroy": (ctypes.c_int, [_P]),
 }
 
 _lib = None

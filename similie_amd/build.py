@@ -38,6 +38,22 @@ def needs_build():
     return any(p.stat().st_mtime > t for p in deps)
 
 
+CPP_TESTS = ROOT / "tests" / "cpp"
+
+
+def build_cpp_tests(verbose=False):
+    """C++ parity tests of include/sil/exterior.hpp, linked against the library."""
+    src = CPP_TESTS / "exterior_tests.cpp"
+    out = CPP_TESTS / "exterior_tests"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++20", f"-I{ROOT / 'include'}", str(src),
+           f"-L{PKG}", "-lsil_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../similie_amd",
+           "-o", str(out)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=str(ROOT))
+    return out
+
+
 def build(force=True, verbose=False):
     if not force and not needs_build():
         return LIB
@@ -47,6 +63,7 @@ def build(force=True, verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=str(ROOT))
     os.replace(tmp, LIB)
+    build_cpp_tests(verbose)
     return LIB
 
 

@@ -1,0 +1,304 @@
+// Batched persistent solver: many independent 2D Dirichlet Poisson problems
+// of uneven size in one launch.
+//
+// No reference counterpart: SimiLie solves one operator per Ginkgo call.  A
+// persistent grid of workgroups pulls problems from a device work queue
+// (largest first); each workgroup runs a whole CG solve for its problem with
+// workgroup-level reductions and barriers only.  The recurrence (r, p, q, and
+// every operator application inside the loop) is FP32 while x, all dot
+// products and the scalars are FP64; the residual is replaced by the FP64
+// residual b − A x (FP64 edge weights) whenever the recurrence reports
+// convergence or every `replace` iterations, and a problem is declared solved
+// only on that FP64 residual.  The launch (queue reset + solve) is captured
+// into a hipGraph by the host.  Workgroup size is chosen by the autotuner.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "batched.hpp"
+
+namespace sil {
+
+namespace {
+
+template <int BLOCK>
+__device__ __forceinline__ double wg_sum(double v, double* sh)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads(); // previous readers of sh are done
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / 64; ++i) s += sh[i];
+    return s; // every thread holds the sum
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void wg_sum2(double& a, double& b, double* sh)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        sh[w] = a;
+        sh[BLOCK / 64 + w] = b;
+    }
+    __syncthreads();
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / 64; ++i) {
+        s0 += sh[i];
+        s1 += sh[BLOCK / 64 + i];
+    }
+    a = s0;
+    b = s1;
+}
+
+// q = A p on the interior of an n×n problem (x slowest); boundary rows 0
+template <typename T>
+__device__ __forceinline__ T apply_row(const T* __restrict__ wx, const T* __restrict__ wy, const T* __restrict__ p,
+                                       int n, int i, int j, int t)
+{
+    const T pv = p[t];
+    const T ue = (i + 2 < n) ? p[t + n] : T(0);
+    const T uw = (i > 1) ? p[t - n] : T(0);
+    const T un = (j + 2 < n) ? p[t + 1] : T(0);
+    const T us = (j > 1) ? p[t - 1] : T(0);
+    return wx[t] * (pv - ue) + wx[t - n] * (pv - uw) + wy[t] * (pv - un) + wy[t - 1] * (pv - us);
+}
+
+struct BatchArgs {
+    const BatchProblem* problems;
+    const int* order;
+    int nprob;
+    int* queue;
+    const float* wx32;
+    const float* wy32;
+    const double* wx64;
+    const double* wy64;
+    const double* b;
+    double* x;
+    float* r;
+    float* p;
+    float* q;
+    double tol;
+    int max_it;
+    int replace;
+    BatchResult* results;
+};
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
+{
+    __shared__ double sh[2 * (BLOCK / 64)];
+    __shared__ int slot;
+    for (;;) {
+        if (threadIdx.x == 0) slot = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int s = slot;
+        __syncthreads();
+        if (s >= a.nprob) return; // every wave reaches this once the queue is drained
+        const int id = a.order[s];
+        const BatchProblem P = a.problems[id];
+        const int n = P.side;
+        const int64_t off = P.offset;
+        const int nn = n * n;
+        const float* wx = a.wx32 + off;
+        const float* wy = a.wy32 + off;
+        const double* wx64 = a.wx64 + off;
+        const double* wy64 = a.wy64 + off;
+        const double* b = a.b + off;
+        double* x = a.x + off;
+        float* r = a.r + off;
+        float* p = a.p + off;
+        float* q = a.q + off;
+
+        // x = 0, r = p = b (FP32), <b,b>
+        double bb = 0.0, rz = 0.0;
+        for (int t = threadIdx.x; t < nn; t += BLOCK) {
+            const double bi = b[t];
+            x[t] = 0.0;
+            const float rf = (float)bi;
+            r[t] = rf;
+            p[t] = rf;
+            bb += bi * bi;
+            rz += (double)rf * (double)rf;
+        }
+        wg_sum2<BLOCK>(bb, rz, sh);
+        const double tol2bb = a.tol * a.tol * bb;
+        int it = 0, since = 0;
+        bool converged = bb == 0.0;
+        double rr_true = bb;
+        while (!converged && it < a.max_it) {
+            __syncthreads(); // p visible to the whole workgroup
+            double pq = 0.0;
+            for (int t = threadIdx.x; t < nn; t += BLOCK) {
+                const int i = t / n, j = t - i * n;
+                float qv = 0.f;
+                if (i > 0 && j > 0 && i + 1 < n && j + 1 < n) qv = apply_row<float>(wx, wy, p, n, i, j, t);
+                q[t] = qv;
+                pq += (double)p[t] * (double)qv;
+            }
+            pq = wg_sum<BLOCK>(pq, sh);
+            const double alpha = rz / pq;
+            double rz_new = 0.0, dummy = 0.0;
+            for (int t = threadIdx.x; t < nn; t += BLOCK) {
+                x[t] += alpha * (double)p[t];
+                const float rf = (float)((double)r[t] - alpha * (double)q[t]);
+                r[t] = rf;
+                rz_new += (double)rf * (double)rf;
+            }
+            wg_sum2<BLOCK>(rz_new, dummy, sh);
+            ++it;
+            ++since;
+            const double rz_old = rz;
+            double beta = rz_new / rz_old;
+            rz = rz_new;
+            if (rz_new <= tol2bb || since >= a.replace) {
+                // FP64 residual replacement b − A x
+                __syncthreads();
+                double rr = 0.0, rzr = 0.0;
+                for (int t = threadIdx.x; t < nn; t += BLOCK) {
+                    const int i = t / n, j = t - i * n;
+                    double res = 0.0;
+                    if (i > 0 && j > 0 && i + 1 < n && j + 1 < n) res = b[t] - apply_row<double>(wx64, wy64, x, n, i, j, t);
+                    const float rf = (float)res;
+                    r[t] = rf;
+                    rr += res * res;
+                    rzr += (double)rf * (double)rf;
+                }
+                wg_sum2<BLOCK>(rr, rzr, sh);
+                rr_true = rr;
+                since = 0;
+                if (rr <= tol2bb) {
+                    converged = true;
+                    break;
+                }
+                beta = rzr / rz_old; // replaced <r,r> over the previous iteration's <r,r>
+                rz = rzr;
+            }
+            for (int t = threadIdx.x; t < nn; t += BLOCK) p[t] = r[t] + (float)beta * p[t];
+        }
+        if (!converged) {
+            // report the FP64 residual of the final iterate
+            __syncthreads();
+            double rr = 0.0, dummy2 = 0.0;
+            for (int t = threadIdx.x; t < nn; t += BLOCK) {
+                const int i = t / n, j = t - i * n;
+                double res = 0.0;
+                if (i > 0 && j > 0 && i + 1 < n && j + 1 < n) res = b[t] - apply_row<double>(wx64, wy64, x, n, i, j, t);
+                rr += res * res;
+            }
+            wg_sum2<BLOCK>(rr, dummy2, sh);
+            rr_true = rr;
+        }
+        if (threadIdx.x == 0) {
+            a.results[id].iterations = it;
+            a.results[id].converged = converged ? 1 : 0;
+            a.results[id].true_relative_residual = bb == 0.0 ? 0.0 : sqrt(rr_true / bb);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_reset_queue(int* q) { *q = 0; }
+
+__global__ void k_batch_weights(const BatchProblem* __restrict__ problems, int nprob, const double* __restrict__ kappa,
+                                float* __restrict__ wx32, float* __restrict__ wy32, double* __restrict__ wx64,
+                                double* __restrict__ wy64)
+{
+    const int id = blockIdx.y;
+    if (id >= nprob) return;
+    const BatchProblem P = problems[id];
+    const int n = P.side;
+    const int64_t off = P.offset;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n * n; t += gridDim.x * blockDim.x) {
+        const int i = t / n, j = t - i * n;
+        const double* k = kappa + off;
+        // uniform unit spacing: ⋆1 ratios are 1, edge value = mean of its two nodes
+        const double ax = (i + 1 < n) ? 0.5 * (k[t] + k[t + n]) : k[t];
+        const double ay = (j + 1 < n) ? 0.5 * (k[t] + k[t + 1]) : k[t];
+        wx64[off + t] = ax;
+        wy64[off + t] = ay;
+        wx32[off + t] = (float)ax;
+        wy32[off + t] = (float)ay;
+    }
+}
+
+template <int BLOCK>
+int launch_block(const BatchArgs& a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batched_cg<BLOCK>, dim3(grid), dim3(BLOCK), 0, s, a);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+}  // namespace
+
+int batched_weights(const BatchProblem* problems_dev, int nprob, int max_side, const double* kappa, float* wx32,
+                    float* wy32, double* wx64, double* wy64, hipStream_t s)
+{
+    const int64_t per = (int64_t)max_side * max_side;
+    const unsigned gx = (unsigned)std::min<int64_t>((per + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_batch_weights, dim3(gx, nprob), dim3(256), 0, s, problems_dev, nprob, kappa, wx32, wy32,
+                       wx64, wy64);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int batched_enqueue(const BatchLaunch& L, hipStream_t s)
+{
+    BatchArgs a{L.problems, L.order, L.nprob, L.queue, L.wx32, L.wy32, L.wx64, L.wy64, L.b, L.x,
+                L.r, L.p, L.q, L.tol, L.max_it, L.replace, L.results};
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, L.queue);
+    SIL_LAUNCH_CHECK();
+    switch (L.block) {
+    case 256:
+        return launch_block<256>(a, L.grid, s);
+    case 512:
+        return launch_block<512>(a, L.grid, s);
+    case 1024:
+        return launch_block<1024>(a, L.grid, s);
+    default:
+        set_error("batched solve: unsupported workgroup size %d", L.block);
+        return SIL_ERR_INVALID;
+    }
+}
+
+}  // namespace sil
+
+namespace sil {
+
+int batched_occupancy(int block, int* blocks_per_cu)
+{
+    hipError_t e;
+    switch (block) {
+    case 256:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batched_cg<256>, 256, 0);
+        break;
+    case 512:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batched_cg<512>, 512, 0);
+        break;
+    case 1024:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batched_cg<1024>, 1024, 0);
+        break;
+    default:
+        set_error("batched solve: unsupported workgroup size %d", block);
+        return SIL_ERR_INVALID;
+    }
+    if (e != hipSuccess) {
+        set_error("occupancy query failed: %s", hipGetErrorString(e));
+        return SIL_ERR_HIP;
+    }
+    return SIL_OK;
+}
+
+}  // namespace sil

@@ -1,0 +1,43 @@
+// Batched persistent Poisson solver (host side).
+#pragma once
+
+#include "common.hpp"
+
+namespace sil {
+
+struct BatchProblem {
+    int64_t offset; // first node of the problem in the concatenated arrays
+    int32_t side;   // n: the problem has n×n nodes, (n−2)² unknowns
+    int32_t pad;
+};
+
+struct BatchResult {
+    int32_t iterations;
+    int32_t converged;
+    double true_relative_residual;
+};
+
+struct BatchLaunch {
+    const BatchProblem* problems;
+    const int* order;
+    int nprob;
+    int* queue;
+    const float *wx32, *wy32;
+    const double *wx64, *wy64;
+    const double* b;
+    double* x;
+    float *r, *p, *q;
+    double tol;
+    int max_it;
+    int replace;
+    BatchResult* results;
+    int block; // workgroup size
+    int grid;  // persistent workgroups
+};
+
+int batched_weights(const BatchProblem* problems_dev, int nprob, int max_side, const double* kappa, float* wx32,
+                    float* wy32, double* wx64, double* wy64, hipStream_t s);
+int batched_enqueue(const BatchLaunch& L, hipStream_t s);
+int batched_occupancy(int block, int* blocks_per_cu);
+
+}  // namespace sil
