@@ -51,6 +51,9 @@ def build_cpp_tests(verbose=False):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=str(ROOT))
+    probe = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++20", "-fPIC", "-shared", f"-I{CSRC}",
+             str(CPP_TESTS / "mfma_probe.hip"), "-o", str(CPP_TESTS / "libmfma_probe.so")]
+    subprocess.run(probe, check=True, cwd=str(ROOT))
     return out
 
 

@@ -134,6 +134,7 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
         }
         wg_sum2<BLOCK>(bb, rz, sh);
         const double tol2bb = a.tol * a.tol * bb;
+        double rz_ref = rz; // <r,r> at the last replacement
         int it = 0, since = 0;
         bool converged = bb == 0.0;
         double rr_true = bb;
@@ -162,7 +163,10 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
             const double rz_old = rz;
             double beta = rz_new / rz_old;
             rz = rz_new;
-            if (rz_new <= tol2bb || since >= a.replace) {
+            // replace when the recurrence claims convergence, when its residual fell
+            // 100-fold since the last replacement (FP32 drift grows with that
+            // ratio), or every `replace` iterations
+            if (rz_new <= tol2bb || rz_new < 1e-4 * rz_ref || since >= a.replace) {
                 // FP64 residual replacement b − A x
                 __syncthreads();
                 double rr = 0.0, rzr = 0.0;
@@ -184,6 +188,7 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
                 }
                 beta = rzr / rz_old; // replaced <r,r> over the previous iteration's <r,r>
                 rz = rzr;
+                rz_ref = rzr;
             }
             for (int t = threadIdx.x; t < nn; t += BLOCK) p[t] = r[t] + (float)beta * p[t];
         }

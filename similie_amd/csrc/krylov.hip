@@ -544,7 +544,7 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
         if (h.done && (executed & 1))
             SIL_CHECK_HIP(hipMemcpyAsync(w.p32[0], w.p32[1], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         const bool hit_max = h.iter >= h.max_iter;
-        const bool want = h.converged || since_replace >= replace || h.rr < 1e-6 * rr_ref;
+        const bool want = h.converged || since_replace >= replace || h.rr < 1e-4 * rr_ref;
         if (want && !hit_max) {
             // FP64 residual replacement; keeps p (the direction buffer in use is p32[0])
             if ((rc = apply64(op, w.x, w.tmp, DotEpilogue{nullptr}, nullptr, nullptr, s))) return rc;

@@ -1,0 +1,638 @@
+// Learned preconditioner: a 4-level convolutional U-Net (32 channels, 3×3
+// kernels, ReLU) with BF16 weights and activations and FP32 accumulation on
+// the gfx950 matrix cores, forward and backward, trained on the GPU.
+//
+// No reference counterpart.  Layers (NHWC, C = 32, H = W = 256 at level 0):
+//   a0 = relu(conv_in(x))          x = [r/rms(r), (log κ − mean)/std]  (2 ch, FP32)
+//   e0 = relu(conv(a0))                                    skip 0   (256²)
+//   a1, e1 = relu(conv(pool(e0))), relu(conv(a1))          skip 1   (128²)
+//   a2, e2 = relu(conv(pool(e1))), relu(conv(a2))          skip 2   (64²)
+//   a3, b3 = relu(conv(pool(e2))), relu(conv(a3))                   (32²)
+//   c2, f2 = relu(conv(up(b3) + e2)), relu(conv(c2))                (64²)
+//   c1, f1 = relu(conv(up(f2) + e1)), relu(conv(c1))                (128²)
+//   c0     = relu(conv(up(f1) + e0))                                (256²)
+//   out    = conv_out(c0)           (1 ch, FP32, linear)
+// The 32→32 convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16:
+// a workgroup computes an 8×32-pixel output tile, stages the (8+2)×(32+2)
+// input tile and all 9·32·32 weights in LDS (pixel stride padded to 40
+// elements so 16-byte fragment reads are bank-conflict free), and each
+// wavefront accumulates two 32-pixel rows × 32 output channels over
+// K = 9 taps × 32 input channels.  The input gradient is the same kernel on
+// the flipped, channel-transposed weights with the ReLU mask applied while
+// the tile is staged; the weight gradient is a second MFMA GEMM over pixels
+// with per-workgroup partial sums reduced in a fixed order.
+#include <cmath>
+#include <cstdint>
+#include <new>
+#include <vector>
+
+#include "mfma_layout.hpp"
+#include "unet.hpp"
+
+namespace sil {
+
+namespace {
+
+constexpr int C = 32;      // channels
+constexpr int CP = 40;     // padded channel stride in LDS
+constexpr int TH = 8;      // output tile rows
+constexpr int TW = 32;     // output tile columns
+constexpr int NT = 256;    // threads per workgroup
+
+using bf16 = __bf16;
+
+struct alignas(16) Chunk8 {
+    bf16 v[8];
+};
+
+__device__ __forceinline__ Chunk8 zero_chunk()
+{
+    Chunk8 c;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) c.v[i] = to_bf16(0.f);
+    return c;
+}
+
+// ------------------------------------------------------ 3×3 32→32 conv ---
+
+// Y = act(conv(X ⊙ [M > 0]) + bias): forward (M null, bias, relu) or input
+// gradient (M = forward output of the layer, flipped weights, no bias/relu).
+// Optional `add` (same shape as Y) is added before the activation.
+__global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ M,
+                                               const bf16* __restrict__ Wt, const float* __restrict__ bias,
+                                               bf16* __restrict__ Y, int H, int W, int relu)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sX[(TH + 2) * (TW + 2) * CP];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * C * CP];
+    const int b = blockIdx.z;
+    const int oy0 = blockIdx.y * TH, ox0 = blockIdx.x * TW;
+    const int tid = threadIdx.x;
+    const size_t img = (size_t)b * H * W * C;
+
+    for (int c = tid; c < 9 * C * 4; c += NT) {
+        const int row = c >> 2, part = c & 3; // row = tap*32 + co
+        *reinterpret_cast<Chunk8*>(&sW[row * CP + part * 8]) = *reinterpret_cast<const Chunk8*>(&Wt[row * C + part * 8]);
+    }
+    for (int c = tid; c < (TH + 2) * (TW + 2) * 4; c += NT) {
+        const int pix = c >> 2, part = c & 3;
+        const int ty = pix / (TW + 2), tx = pix % (TW + 2);
+        const int gy = oy0 - 1 + ty, gx = ox0 - 1 + tx;
+        Chunk8 v = zero_chunk();
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+            const size_t off = img + ((size_t)gy * W + gx) * C + part * 8;
+            v = *reinterpret_cast<const Chunk8*>(&X[off]);
+            if (M) {
+                const Chunk8 m = *reinterpret_cast<const Chunk8*>(&M[off]);
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (!(from_bf16(m.v[i]) > 0.f)) v.v[i] = to_bf16(0.f);
+            }
+        }
+        *reinterpret_cast<Chunk8*>(&sX[pix * CP + part * 8]) = v;
+    }
+    __syncthreads();
+
+    const int lane = tid & 63, wave = tid >> 6;
+    f32x16 acc[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[m][r] = 0.f;
+    const int col = Mfma32::a_row(lane); // output column (pixel) of the A fragment
+    const int k0 = Mfma32::a_k0(lane);
+    const int co_b = Mfma32::b_col(lane);
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int ky = tap / 3, kx = tap % 3;
+#pragma unroll
+        for (int kh = 0; kh < 2; ++kh) {
+            const bf16x8 bw = *reinterpret_cast<const bf16x8*>(&sW[(tap * C + co_b) * CP + kh * 16 + k0]);
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const int tr = 2 * wave + m + ky, tc = col + kx;
+                const bf16x8 ax = *reinterpret_cast<const bf16x8*>(&sX[(tr * (TW + 2) + tc) * CP + kh * 16 + k0]);
+                acc[m] = Mfma32::mma(ax, bw, acc[m]);
+            }
+        }
+    }
+    const int co = Mfma32::d_col(lane);
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int gy = oy0 + 2 * wave + m;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int gx = ox0 + Mfma32::d_row(lane, r);
+            float v = acc[m][r] + bv;
+            if (relu) v = fmaxf(v, 0.f);
+            Y[img + ((size_t)gy * W + gx) * C + co] = to_bf16(v);
+        }
+    }
+}
+
+// weight gradient: dW[tap][co][ci] += Σ_q dZ[q][co] · X[q + off(tap)][ci],
+// dZ = dY ⊙ [Yfwd > 0]; db[co] += Σ_q dZ[q][co].  One partial per workgroup
+// (grid-stride over tiles), reduced later in a fixed order.
+__global__ __launch_bounds__(NT) void k_conv33_wgrad(const bf16* __restrict__ X, const bf16* __restrict__ dY,
+                                                     const bf16* __restrict__ Yf, float* __restrict__ part_w,
+                                                     float* __restrict__ part_b, int B, int H, int W)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sX[(TH + 2) * (TW + 2) * CP];
+    __shared__ __attribute__((aligned(16))) bf16 sZ[C * (TH * TW + 8)]; // transposed dZ: [co][pixel]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = W / TW, tiles_y = H / TH;
+    const int ntiles = tiles_x * tiles_y * B;
+    // taps per wave: 0:{0,4,8} 1:{1,5} 2:{2,6} 3:{3,7}
+    f32x16 acc[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+    float bsum = 0.f; // channel tid & 31, pixels strided by 8 rows of threads
+    const int zs = TH * TW + 8;
+
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int bx = tile % tiles_x, by = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
+        const int oy0 = by * TH, ox0 = bx * TW;
+        const size_t img = (size_t)b * H * W * C;
+        __syncthreads();
+        for (int c = tid; c < (TH + 2) * (TW + 2) * 4; c += NT) {
+            const int pix = c >> 2, part = c & 3;
+            const int ty = pix / (TW + 2), tx = pix % (TW + 2);
+            const int gy = oy0 - 1 + ty, gx = ox0 - 1 + tx;
+            Chunk8 v = zero_chunk();
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                v = *reinterpret_cast<const Chunk8*>(&X[img + ((size_t)gy * W + gx) * C + part * 8]);
+            *reinterpret_cast<Chunk8*>(&sX[pix * CP + part * 8]) = v;
+        }
+        for (int c = tid; c < TH * TW * 4; c += NT) {
+            const int pix = c >> 2, part = c & 3;
+            const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
+            const size_t off = img + ((size_t)gy * W + gx) * C + part * 8;
+            const Chunk8 d = *reinterpret_cast<const Chunk8*>(&dY[off]);
+            const Chunk8 y = *reinterpret_cast<const Chunk8*>(&Yf[off]);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float z = from_bf16(y.v[i]) > 0.f ? from_bf16(d.v[i]) : 0.f;
+                sZ[(part * 8 + i) * zs + pix] = to_bf16(z);
+            }
+        }
+        __syncthreads();
+        {
+            const int ch = tid & 31;
+            for (int pix = tid >> 5; pix < TH * TW; pix += NT / 32) bsum += from_bf16(sZ[ch * zs + pix]);
+        }
+        const int co_a = Mfma32::a_row(lane), kk = Mfma32::a_k0(lane);
+        const int ci_b = Mfma32::b_col(lane);
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int tap = wave + 4 * t;
+            if (tap > 8) continue;
+            const int ky = tap / 3, kx = tap % 3;
+            for (int s = 0; s < TH * TW / 16; ++s) {
+                const int p0 = 16 * s + kk; // 8 consecutive pixels of one tile row
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sZ[co_a * zs + p0]);
+                bf16x8 bb;
+                const int row = p0 / TW, c0 = p0 % TW;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) bb[i] = sX[((row + ky) * (TW + 2) + c0 + i + kx) * CP + ci_b];
+                acc[t] = Mfma32::mma(a, bb, acc[t]);
+            }
+        }
+    }
+    float* pw = part_w + (size_t)blockIdx.x * 9 * C * C;
+    const int ci = Mfma32::d_col(lane);
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int tap = wave + 4 * t;
+        if (tap > 8) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) pw[(tap * C + Mfma32::d_row(lane, r)) * C + ci] = acc[t][r];
+    }
+    __shared__ float sb[NT];
+    sb[tid] = bsum;
+    __syncthreads();
+    if (tid < C) {
+        float s = 0.f;
+        for (int j = tid; j < NT; j += C) s += sb[j];
+        part_b[blockIdx.x * C + tid] = s;
+    }
+}
+
+// sum of `parts` partial vectors of length n: out[i] = Σ_p part[p*n + i]
+__global__ void k_reduce_parts(const float* __restrict__ part, int parts, int n, float* __restrict__ out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int p = 0; p < parts; ++p) s += part[(size_t)p * n + i];
+        out[i] = s;
+    }
+}
+
+// -------------------------------------------------- input / output convs ---
+
+// a0 = relu(conv_in(x)): x FP32 [B][H][W][2], weights [tap][co][ci=2]
+__global__ void k_conv_in(const float* __restrict__ x, const float* __restrict__ w, const float* __restrict__ bias,
+                          bf16* __restrict__ y, int B, int H, int W)
+{
+    __shared__ float sw[9 * C * 2 + C];
+    for (int i = threadIdx.x; i < 9 * C * 2; i += blockDim.x) sw[i] = w[i];
+    for (int i = threadIdx.x; i < C; i += blockDim.x) sw[9 * C * 2 + i] = bias[i];
+    __syncthreads();
+    const size_t n = (size_t)B * H * W;
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+        const int xx = p % W, yy = (p / W) % H;
+        const size_t b = p / ((size_t)W * H);
+        float in[9][2];
+        for (int t = 0; t < 9; ++t) {
+            const int gy = yy + t / 3 - 1, gx = xx + t % 3 - 1;
+            const bool ok = gy >= 0 && gy < H && gx >= 0 && gx < W;
+            const size_t q = (b * H + gy) * W + gx;
+            in[t][0] = ok ? x[q * 2] : 0.f;
+            in[t][1] = ok ? x[q * 2 + 1] : 0.f;
+        }
+        for (int co = 0; co < C; ++co) {
+            float s = sw[9 * C * 2 + co];
+            for (int t = 0; t < 9; ++t) s += sw[(t * C + co) * 2] * in[t][0] + sw[(t * C + co) * 2 + 1] * in[t][1];
+            y[p * C + co] = to_bf16(fmaxf(s, 0.f));
+        }
+    }
+}
+
+// weight gradient of conv_in: partials per block of dW [9][32][2] and db [32]
+__global__ void k_conv_in_wgrad(const float* __restrict__ x, const bf16* __restrict__ dY, const bf16* __restrict__ Yf,
+                                float* __restrict__ part, int B, int H, int W)
+{
+    // thread (co, tap-chunk); loop over a strided set of pixels
+    const int co = threadIdx.x & 31;
+    const int lanegrp = threadIdx.x >> 5; // 8 groups of pixels
+    float acc[9 * 2 + 1];
+    for (int i = 0; i < 19; ++i) acc[i] = 0.f;
+    const size_t n = (size_t)B * H * W;
+    for (size_t p = blockIdx.x * 8 + lanegrp; p < n; p += (size_t)gridDim.x * 8) {
+        const float y = from_bf16(Yf[p * C + co]);
+        const float dz = y > 0.f ? from_bf16(dY[p * C + co]) : 0.f;
+        if (dz == 0.f) continue;
+        const int xx = p % W, yy = (p / W) % H;
+        const size_t b = p / ((size_t)W * H);
+        for (int t = 0; t < 9; ++t) {
+            const int gy = yy + t / 3 - 1, gx = xx + t % 3 - 1;
+            if (gy < 0 || gy >= H || gx < 0 || gx >= W) continue;
+            const size_t q = (b * H + gy) * W + gx;
+            acc[t * 2] += dz * x[q * 2];
+            acc[t * 2 + 1] += dz * x[q * 2 + 1];
+        }
+        acc[18] += dz;
+    }
+    __shared__ float s[256][19];
+    for (int i = 0; i < 19; ++i) s[threadIdx.x][i] = acc[i];
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float* out = part + (size_t)blockIdx.x * (9 * C * 2 + C);
+        for (int i = 0; i < 19; ++i) {
+            float v = 0.f;
+            for (int g = 0; g < 8; ++g) v += s[g * 32 + threadIdx.x][i];
+            if (i < 18) out[((i / 2) * C + threadIdx.x) * 2 + (i & 1)] = v;
+            else out[9 * C * 2 + threadIdx.x] = v;
+        }
+    }
+}
+
+// out = conv_out(c0): 32 → 1, weights [tap][ci], bias scalar
+__global__ void k_conv_out(const bf16* __restrict__ x, const float* __restrict__ w, const float* __restrict__ bias,
+                           float* __restrict__ y, int B, int H, int W)
+{
+    __shared__ float sw[9 * C + 1];
+    for (int i = threadIdx.x; i < 9 * C; i += blockDim.x) sw[i] = w[i];
+    if (threadIdx.x == 0) sw[9 * C] = bias[0];
+    __syncthreads();
+    const size_t n = (size_t)B * H * W;
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+        const int xx = p % W, yy = (p / W) % H;
+        const size_t b = p / ((size_t)W * H);
+        float s = sw[9 * C];
+        for (int t = 0; t < 9; ++t) {
+            const int gy = yy + t / 3 - 1, gx = xx + t % 3 - 1;
+            if (gy < 0 || gy >= H || gx < 0 || gx >= W) continue;
+            const Chunk8* px = reinterpret_cast<const Chunk8*>(&x[((b * H + gy) * W + gx) * C]);
+            for (int part = 0; part < 4; ++part) {
+                const Chunk8 v = px[part];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) s += sw[t * C + part * 8 + i] * from_bf16(v.v[i]);
+            }
+        }
+        y[p] = s;
+    }
+}
+
+// backward of conv_out: dX[q][ci] = Σ_t dout[q − off(t)] · w[t][ci] (BF16),
+// and per-block partials of dW [9][32] and db
+__global__ void k_conv_out_bwd(const bf16* __restrict__ x, const float* __restrict__ dout, const float* __restrict__ w,
+                               bf16* __restrict__ dx, float* __restrict__ part, int B, int H, int W)
+{
+    __shared__ float sw[9 * C];
+    for (int i = threadIdx.x; i < 9 * C; i += blockDim.x) sw[i] = w[i];
+    __syncthreads();
+    const int ci = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    float gw[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float gb = 0.f;
+    const size_t n = (size_t)B * H * W;
+    for (size_t p = blockIdx.x * 8 + grp; p < n; p += (size_t)gridDim.x * 8) {
+        const int xx = p % W, yy = (p / W) % H;
+        const size_t b = p / ((size_t)W * H);
+        float s = 0.f;
+        const float d0 = dout[p];
+        if (ci == 0) gb += d0;
+        for (int t = 0; t < 9; ++t) {
+            const int dy = t / 3 - 1, dx_ = t % 3 - 1;
+            // input gradient: q = p, output pixel p − off
+            const int oy = yy - dy, ox = xx - dx_;
+            if (oy >= 0 && oy < H && ox >= 0 && ox < W) s += dout[(b * H + oy) * W + ox] * sw[t * C + ci];
+            // weight gradient: output pixel p, input p + off
+            const int iy = yy + dy, ix = xx + dx_;
+            if (iy >= 0 && iy < H && ix >= 0 && ix < W) gw[t] += d0 * from_bf16(x[((b * H + iy) * W + ix) * C + ci]);
+        }
+        dx[p * C + ci] = to_bf16(s);
+    }
+    __shared__ float sp[256][10];
+    for (int t = 0; t < 9; ++t) sp[threadIdx.x][t] = gw[t];
+    sp[threadIdx.x][9] = gb;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float* out = part + (size_t)blockIdx.x * (9 * C + 1);
+        for (int t = 0; t < 9; ++t) {
+            float v = 0.f;
+            for (int g = 0; g < 8; ++g) v += sp[g * 32 + threadIdx.x][t];
+            out[t * C + threadIdx.x] = v;
+        }
+        if (threadIdx.x == 0) {
+            float v = 0.f;
+            for (int g = 0; g < 8; ++g) v += sp[g * 32][9];
+            out[9 * C] = v;
+        }
+    }
+}
+
+// ----------------------------------------------------- pooling, up-add ---
+
+__global__ void k_pool(const bf16* __restrict__ x, bf16* __restrict__ y, int B, int H, int W)
+{
+    const int h = H / 2, w = W / 2;
+    const size_t n = (size_t)B * h * w * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        const size_t p = i / C;
+        const int xx = p % w, yy = (p / w) % h;
+        const size_t b = p / ((size_t)w * h);
+        const size_t base = ((b * H + 2 * yy) * W + 2 * xx) * C + c;
+        const float s = from_bf16(x[base]) + from_bf16(x[base + C]) + from_bf16(x[base + (size_t)W * C]) +
+                        from_bf16(x[base + (size_t)W * C + C]);
+        y[i] = to_bf16(0.25f * s);
+    }
+}
+
+// dX (fine) += dY (coarse)/4 on each of the 4 children (accumulate into dx)
+__global__ void k_pool_bwd(const bf16* __restrict__ dy, bf16* __restrict__ dx, int B, int H, int W)
+{
+    const size_t n = (size_t)B * H * W * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        const size_t p = i / C;
+        const int xx = p % W, yy = (p / W) % H;
+        const size_t b = p / ((size_t)W * H);
+        const size_t q = ((b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c;
+        dx[i] = to_bf16(from_bf16(dx[i]) + 0.25f * from_bf16(dy[q]));
+    }
+}
+
+// y (fine) = up(xc) + skip
+__global__ void k_upadd(const bf16* __restrict__ xc, const bf16* __restrict__ skip, bf16* __restrict__ y, int B, int H,
+                        int W)
+{
+    const size_t n = (size_t)B * H * W * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        const size_t p = i / C;
+        const int xx = p % W, yy = (p / W) % H;
+        const size_t b = p / ((size_t)W * H);
+        const size_t q = ((b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c;
+        y[i] = to_bf16(from_bf16(xc[q]) + from_bf16(skip[i]));
+    }
+}
+
+// d xc (coarse) = Σ of the 4 children of dY (fine)
+__global__ void k_up_bwd(const bf16* __restrict__ dy, bf16* __restrict__ dxc, int B, int h, int w)
+{
+    const int H = 2 * h, W = 2 * w;
+    const size_t n = (size_t)B * h * w * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        const size_t p = i / C;
+        const int xx = p % w, yy = (p / w) % h;
+        const size_t b = p / ((size_t)w * h);
+        const size_t base = ((b * H + 2 * yy) * W + 2 * xx) * C + c;
+        const float s = from_bf16(dy[base]) + from_bf16(dy[base + C]) + from_bf16(dy[base + (size_t)W * C]) +
+                        from_bf16(dy[base + (size_t)W * C + C]);
+        dxc[i] = to_bf16(s);
+    }
+}
+
+__global__ void k_copy_bf16(const bf16* __restrict__ x, bf16* __restrict__ y, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = x[i];
+}
+
+// ------------------------------------------------------------- loss ---
+
+// d = e − out; Ad = A d (Dirichlet-zero 5-point, edge weights wx, wy per node);
+// loss_b = dᵀAd / norm_b (norm_b = eᵀ A e); dout = −2 Ad / norm_b / B.
+// Partial per block of Σ dᵀAd / norm_b.
+__global__ void k_anorm_loss(const float* __restrict__ out, const float* __restrict__ e, const float* __restrict__ wx,
+                             const float* __restrict__ wy, const float* __restrict__ norm, float* __restrict__ dout,
+                             float* __restrict__ part, int B, int n)
+{
+    const size_t total = (size_t)B * n * n;
+    float acc = 0.f;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int j = t % n, i = (t / n) % n;
+        const size_t b = t / ((size_t)n * n);
+        float ad = 0.f;
+        const bool inner = i > 0 && j > 0 && i + 1 < n && j + 1 < n;
+        if (inner) {
+            auto d = [&](size_t q, bool in) { return in ? e[q] - out[q] : 0.f; };
+            const float dc = d(t, true);
+            const float de = d(t + n, i + 2 < n), dw = d(t - n, i > 1), dn = d(t + 1, j + 2 < n), ds = d(t - 1, j > 1);
+            ad = wx[t] * (dc - de) + wx[t - n] * (dc - dw) + wy[t] * (dc - dn) + wy[t - 1] * (dc - ds);
+            acc += dc * ad / norm[b];
+        }
+        dout[t] = -2.f * ad / norm[b] / (float)B;
+    }
+    __shared__ float sh[256];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// ------------------------------------------------------------ Adam ---
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// BF16 copies of one 32→32 layer's weights: forward [tap][co][ci] and the
+// input-gradient form W'[t][a][b] = W[8 − t][b][a]
+__global__ void k_pack33(const float* __restrict__ w, bf16* __restrict__ wf, bf16* __restrict__ wb)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 9 * C * C; i += gridDim.x * blockDim.x) {
+        const int t = i / (C * C), a = (i / C) % C, b = i % C;
+        wf[i] = to_bf16(w[i]);
+        wb[i] = to_bf16(w[((8 - t) * C + b) * C + a]);
+    }
+}
+
+inline unsigned grid1(size_t n, int block = 256, size_t cap = 8192)
+{
+    size_t g = (n + block - 1) / block;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+// -------------------------------------------------------------- host ---
+
+namespace unet {
+
+int conv33(const bf16* X, const bf16* M, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W, bool relu,
+           hipStream_t s)
+{
+    if (H % TH || W % TW) {
+        set_error("unet: %dx%d is not a multiple of the %dx%d tile", H, W, TH, TW);
+        return SIL_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(k_conv33, dim3(W / TW, H / TH, B), dim3(NT), 0, s, X, M, Wt, bias, Y, H, W, relu ? 1 : 0);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int conv33_wgrad(const bf16* X, const bf16* dY, const bf16* Yf, float* part_w, float* part_b, int parts, float* dW,
+                 float* db, int B, int H, int W, hipStream_t s)
+{
+    const int ntiles = (W / TW) * (H / TH) * B;
+    const int grid = ntiles < parts ? ntiles : parts;
+    hipLaunchKernelGGL(k_conv33_wgrad, dim3(grid), dim3(NT), 0, s, X, dY, Yf, part_w, part_b, B, H, W);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(36), dim3(256), 0, s, part_w, grid, 9 * C * C, dW);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(32), 0, s, part_b, grid, C, db);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_conv_in, dim3(grid1((size_t)B * H * W)), dim3(256), 0, s, x, w, bias, y, B, H, W);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, int parts, float* dWb, int B, int H,
+                  int W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_conv_in_wgrad, dim3(parts), dim3(256), 0, s, x, dY, Yf, part, B, H, W);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(4), dim3(256), 0, s, part, parts, 9 * C * 2 + C, dWb);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int conv_out(const bf16* x, const float* w, const float* bias, float* y, int B, int H, int W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_conv_out, dim3(grid1((size_t)B * H * W)), dim3(256), 0, s, x, w, bias, y, B, H, W);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int conv_out_bwd(const bf16* x, const float* dout, const float* w, bf16* dx, float* part, int parts, float* dWb, int B,
+                 int H, int W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_conv_out_bwd, dim3(parts), dim3(256), 0, s, x, dout, w, dx, part, B, H, W);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(2), dim3(256), 0, s, part, parts, 9 * C + 1, dWb);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int pool(const bf16* x, bf16* y, int B, int H, int W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pool, dim3(grid1((size_t)B * H * W * C / 4)), dim3(256), 0, s, x, y, B, H, W);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int pool_bwd_accumulate(const bf16* dy, bf16* dx, int B, int H, int W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid1((size_t)B * H * W * C)), dim3(256), 0, s, dy, dx, B, H, W);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int upadd(const bf16* xc, const bf16* skip, bf16* y, int B, int H, int W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_upadd, dim3(grid1((size_t)B * H * W * C)), dim3(256), 0, s, xc, skip, y, B, H, W);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int up_bwd(const bf16* dy, bf16* dxc, int B, int h, int w, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_up_bwd, dim3(grid1((size_t)B * h * w * C)), dim3(256), 0, s, dy, dxc, B, h, w);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int copy(const bf16* x, bf16* y, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_bf16, dim3(grid1(n)), dim3(256), 0, s, x, y, n);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int anorm_loss(const float* out, const float* e, const float* wx, const float* wy, const float* norm, float* dout,
+               float* part, int parts, float* loss, int B, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_anorm_loss, dim3(parts), dim3(256), 0, s, out, e, wx, wy, norm, dout, part, B, n);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1), 0, s, part, parts, 1, loss);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int adam(float* w, const float* g, float* m, float* v, int n, float lr, int step, hipStream_t s)
+{
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(grid1(n)), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2, 1e-8f, c1, c2);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int pack33(const float* w, bf16* wf, bf16* wb, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack33, dim3(36), dim3(256), 0, s, w, wf, wb);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+}  // namespace unet
+
+}  // namespace sil

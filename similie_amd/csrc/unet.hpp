@@ -1,0 +1,49 @@
+// Kernels of the learned U-Net preconditioner (BF16 activations and
+// weights, FP32 accumulation).  Tensors are NHWC with 32 channels; every
+// launcher returns a sil_status and records the message on failure.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "common.hpp"
+
+namespace sil::unet {
+
+using bf16 = __bf16;
+
+constexpr int kChannels = 32;
+constexpr int kTileH = 8;
+constexpr int kTileW = 32;
+
+// Y = act(conv3x3(X ⊙ [M > 0]) + bias), 32 → 32 channels, zero padding.
+int conv33(const bf16* X, const bf16* M, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W, bool relu,
+           hipStream_t s);
+// dW[tap][co][ci], db[co] of conv33 from its input X, output gradient dY and
+// forward output Yf (ReLU mask); part_w holds parts·9·32·32 floats, part_b parts·32
+int conv33_wgrad(const bf16* X, const bf16* dY, const bf16* Yf, float* part_w, float* part_b, int parts, float* dW,
+                 float* db, int B, int H, int W, hipStream_t s);
+// 2 → 32 input convolution (FP32 input) and its weight gradient (dWb = [dW(9·32·2) | db(32)])
+int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s);
+int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, int parts, float* dWb, int B, int H,
+                  int W, hipStream_t s);
+// 32 → 1 output convolution (FP32 output) and its backward (dWb = [dW(9·32) | db])
+int conv_out(const bf16* x, const float* w, const float* bias, float* y, int B, int H, int W, hipStream_t s);
+int conv_out_bwd(const bf16* x, const float* dout, const float* w, bf16* dx, float* part, int parts, float* dWb, int B,
+                 int H, int W, hipStream_t s);
+// 2×2 average pooling of an H×W tensor; its backward accumulates into dx
+int pool(const bf16* x, bf16* y, int B, int H, int W, hipStream_t s);
+int pool_bwd_accumulate(const bf16* dy, bf16* dx, int B, int H, int W, hipStream_t s);
+// y (H×W) = nearest-upsampled xc (H/2×W/2) + skip; backward to the coarse input
+int upadd(const bf16* xc, const bf16* skip, bf16* y, int B, int H, int W, hipStream_t s);
+int up_bwd(const bf16* dy, bf16* dxc, int B, int h, int w, hipStream_t s);
+int copy(const bf16* x, bf16* y, std::size_t n, hipStream_t s);
+// energy-norm loss Σ_b (e−out)ᵀA(e−out)/norm_b and its gradient dout
+int anorm_loss(const float* out, const float* e, const float* wx, const float* wy, const float* norm, float* dout,
+               float* part, int parts, float* loss, int B, int n, hipStream_t s);
+int adam(float* w, const float* g, float* m, float* v, int n, float lr, int step, hipStream_t s);
+// BF16 forward weights and the flipped, channel-transposed input-gradient weights of one 32→32 layer
+int pack33(const float* w, bf16* wf, bf16* wb, hipStream_t s);
+
+}  // namespace sil::unet
