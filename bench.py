@@ -132,7 +132,7 @@ def main():
     achieved = B / (kernel_ms * 1e-3) / 1e9
 
     # plain copy of the same byte count for reference (torch's copy kernel)
-    a = torch.empty(B // 16, dtype=torch.float64, device="cuda")
+    a = torch.ones(B // 16, dtype=torch.float64, device="cuda")
     b2 = torch.empty_like(a)
     for _ in range(3):
         b2.copy_(a)

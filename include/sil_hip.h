@@ -50,7 +50,11 @@ enum sil_precision {
     SIL_PRECISION_MIXED = 1  /* FP32 operator applications, FP64 residual replacement, dots and scalars */
 };
 
-enum sil_preconditioner { SIL_PRECOND_IDENTITY = 0, SIL_PRECOND_JACOBI = 1 };
+enum sil_preconditioner {
+    SIL_PRECOND_IDENTITY = 0,
+    SIL_PRECOND_JACOBI = 1,
+    SIL_PRECOND_LEARNED = 2 /* the U-Net of a sil_unet handle, inside flexible CG (equal-size batches only) */
+};
 
 /* Strided view of a k-form sampled on a node grid. */
 typedef struct sil_view {
@@ -237,6 +241,75 @@ int sil_batched_poisson2d_solve(sil_batched_poisson h, const double* rhs, double
                                 const sil_solver_settings* settings, sil_batch_result* results, double* seconds,
                                 void* stream);
 int sil_batched_poisson2d_destroy(sil_batched_poisson h);
+
+/* ------------------------------------------------------------------ */
+/* learned preconditioner                                              */
+/* ------------------------------------------------------------------ */
+
+/* No reference counterpart.  A 4-level convolutional U-Net (32 channels,
+ * 3x3 kernels, ReLU, 2x2 average pooling, nearest upsampling with additive
+ * skips) with BF16 weights and activations and FP32 accumulation on the
+ * matrix cores.  Input [B][H][W][2] FP32 (residual / rms, normalised log κ),
+ * output [B][H][W] FP32.  Parameters are one FP32 vector of
+ * sil_unet_param_count() entries: conv_in W[9][32][2] b[32]; twelve 32->32
+ * layers W[9][32][32] ([tap][out][in], tap = 3 dy + dx) b[32]; conv_out
+ * W[9][32] b[1].  H and W must be multiples of 8.  All pointers are device
+ * pointers; every call is asynchronous on `stream`. */
+typedef struct sil_unet_s* sil_unet;
+int64_t sil_unet_param_count(void);
+int sil_unet_create(sil_unet* out, int32_t max_batch, int32_t height, int32_t width, uint64_t seed);
+int sil_unet_destroy(sil_unet net);
+int sil_unet_get_params(sil_unet net, float* params, void* stream);
+int sil_unet_set_params(sil_unet net, const float* params, void* stream);
+int sil_unet_forward(sil_unet net, int32_t batch, const float* input, float* output, void* stream);
+/* parameter gradient of <dout, forward(input)>; uses the activations of the
+ * preceding sil_unet_forward on the same input */
+int sil_unet_backward(sil_unet net, int32_t batch, const float* input, const float* dout, float* grad,
+                      void* stream);
+/* direction loss of an output N against the error e of the residual
+ * r = input[..., 0] = A e: loss[b] = 1 - (r.N)^2 / ((e.r)(N.A N)), A the
+ * Dirichlet-zero 5-point operator with edge weights wx, wy [B][H][W];
+ * dout = d(mean loss)/dN.  Square images only. */
+int sil_unet_loss(sil_unet net, int32_t batch, const float* output, const float* input, const float* err,
+                  const float* wx, const float* wy, float* dout, float* loss, void* stream);
+/* forward, loss, backward and one Adam step (β 0.9 / 0.999, ε 1e-8) */
+int sil_unet_train_step(sil_unet net, int32_t batch, const float* input, const float* err, const float* wx,
+                        const float* wy, float learning_rate, float* loss, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* equal-size batches of 2D Poisson problems                           */
+/* ------------------------------------------------------------------ */
+
+/* No reference counterpart.  B Dirichlet-zero problems on n x n unit-spaced
+ * grids, arrays [B][n][n] (x slowest), nodal coefficient kappa (edge value
+ * = mean of its two nodes).  Methods, chosen by the settings:
+ *   JACOBI / IDENTITY: CG; precision FP64, or MIXED (FP32 directions and
+ *   operator applications, FP64 iterate, residual, dots and scalars);
+ *   LEARNED: flexible CG (truncation 1) preconditioned by `net`.
+ * A problem converges only on an FP64 residual b - A x recomputed every
+ * replace_interval iterations (even; 0 = 10 for CG, 4 for flexible CG).
+ * results (host, B entries) report iterations and the FP64 relative
+ * residual of every problem at exit; seconds is the host wall time of the
+ * solve.  An optional record stores FP32 copies of r and x of all problems
+ * before every graph of `every` iterations (the training data of the
+ * learned preconditioner). */
+typedef struct sil_solve_record {
+    int32_t every;       /* even, >= 2; overrides replace_interval */
+    int32_t max_records;
+    float* r;            /* device [max_records][B][n][n] */
+    float* x;            /* device [max_records][B][n][n] */
+    int32_t* count;      /* host: records written */
+} sil_solve_record;
+
+typedef struct sil_poisson2d_batch_s* sil_poisson2d_batch;
+int sil_poisson2d_batch_create(sil_poisson2d_batch* out, int32_t batch, int32_t n, const double* kappa);
+int sil_poisson2d_batch_destroy(sil_poisson2d_batch h);
+int sil_poisson2d_batch_apply(sil_poisson2d_batch h, const double* x, double* y, void* stream);
+/* FP32 edge weights and normalised log κ (any pointer may be NULL) */
+int sil_poisson2d_batch_weights(sil_poisson2d_batch h, float* wx, float* wy, float* logk, void* stream);
+int sil_poisson2d_batch_solve(sil_poisson2d_batch h, const double* rhs, double* solution,
+                              const sil_solver_settings* settings, sil_unet net, sil_batch_result* results,
+                              double* seconds, const sil_solve_record* record, void* stream);
 
 #ifdef __cplusplus
 }

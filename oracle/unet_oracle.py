@@ -1,0 +1,187 @@
+"""CPU oracle of the learned preconditioner's network (test infrastructure).
+
+FP64 numpy restatement of the 4-level U-Net that similie_amd/csrc/unet.hip
+and unet_net.cpp run in BF16/FP32 on the GPU: forward pass, parameter
+gradient of a linear functional of the output, and the direction loss with
+its gradient.  Used only by the tests, which check the GPU kernels against
+it; its own gradients are pinned by central finite differences in
+tests/test_unet_oracle.py.
+
+The network has no counterpart in the reference (it is new work), so there
+is no reference file to cite; the layout follows the parameter vector
+documented in include/sil_hip.h (sil_unet_*).
+"""
+import numpy as np
+
+C = 32
+N_IN = 9 * C * 2 + C
+N_LAYER = 9 * C * C + C
+N_OUT = 9 * C + 1
+N_LAYERS = 12
+N_PARAMS = N_IN + N_LAYERS * N_LAYER + N_OUT
+
+
+def layer_slices(l):
+    """(W, b) slices of 32->32 layer l = 1..12 in the parameter vector."""
+    base = N_IN + (l - 1) * N_LAYER
+    return slice(base, base + 9 * C * C), slice(base + 9 * C * C, base + N_LAYER)
+
+
+OUT_BASE = N_IN + N_LAYERS * N_LAYER
+
+
+def unpack(params):
+    p = np.asarray(params, dtype=np.float64)
+    assert p.shape == (N_PARAMS,)
+    win = p[: 9 * C * 2].reshape(9, C, 2)
+    bin_ = p[9 * C * 2: N_IN]
+    layers = []
+    for l in range(1, N_LAYERS + 1):
+        ws, bs = layer_slices(l)
+        layers.append((p[ws].reshape(9, C, C), p[bs]))
+    wout = p[OUT_BASE: OUT_BASE + 9 * C].reshape(9, C)
+    bout = p[OUT_BASE + 9 * C]
+    return win, bin_, layers, wout, bout
+
+
+def conv(x, w, b):
+    """3x3 zero-padded convolution, x [B,H,W,Ci], w [9][Co][Ci] (tap = 3 dy + dx)."""
+    B, H, W, _ = x.shape
+    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+    y = np.zeros((B, H, W, w.shape[1]))
+    for t in range(9):
+        dy, dx = divmod(t, 3)
+        y += xp[:, dy:dy + H, dx:dx + W, :] @ w[t].T
+    return y + b
+
+
+def conv_bwd(x, w, dz):
+    """(dx, dw, db) of conv for the pre-activation gradient dz."""
+    B, H, W, _ = x.shape
+    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+    dxp = np.zeros_like(xp)
+    dw = np.zeros_like(w)
+    for t in range(9):
+        dy, dx = divmod(t, 3)
+        dxp[:, dy:dy + H, dx:dx + W, :] += dz @ w[t]
+        dw[t] = np.einsum("bhwo,bhwc->oc", dz, xp[:, dy:dy + H, dx:dx + W, :])
+    return dxp[:, 1:-1, 1:-1, :], dw, dz.sum(axis=(0, 1, 2))
+
+
+def pool(x):
+    return 0.25 * (x[:, 0::2, 0::2] + x[:, 1::2, 0::2] + x[:, 0::2, 1::2] + x[:, 1::2, 1::2])
+
+
+def pool_bwd(dy):
+    return 0.25 * np.repeat(np.repeat(dy, 2, axis=1), 2, axis=2)
+
+
+def up(x):
+    return np.repeat(np.repeat(x, 2, axis=1), 2, axis=2)
+
+
+def up_bwd(dy):
+    return dy[:, 0::2, 0::2] + dy[:, 1::2, 0::2] + dy[:, 0::2, 1::2] + dy[:, 1::2, 1::2]
+
+
+def relu(v):
+    return np.maximum(v, 0.0)
+
+
+# (layer, input name, output name) in forward order; "pool:" / "up:" entries are the
+# parameter-free steps
+GRAPH = [
+    ("in", "x", "a0"),
+    (1, "a0", "e0"),
+    ("pool", "e0", "p1"),
+    (2, "p1", "a1"), (3, "a1", "e1"),
+    ("pool", "e1", "p2"),
+    (4, "p2", "a2"), (5, "a2", "e2"),
+    ("pool", "e2", "p3"),
+    (6, "p3", "a3"), (7, "a3", "b3"),
+    ("up", ("b3", "e2"), "u2"),
+    (8, "u2", "c2"), (9, "c2", "f2"),
+    ("up", ("f2", "e1"), "u1"),
+    (10, "u1", "c1"), (11, "c1", "f1"),
+    ("up", ("f1", "e0"), "u0"),
+    (12, "u0", "c0"),
+    ("out", "c0", "out"),
+]
+
+
+def forward(params, x):
+    """Network output [B,H,W] and the activations, x [B,H,W,2]."""
+    win, bin_, layers, wout, bout = unpack(params)
+    act = {"x": np.asarray(x, dtype=np.float64)}
+    for op, src, dst in GRAPH:
+        if op == "in":
+            act[dst] = relu(conv(act[src], win, bin_))
+        elif op == "pool":
+            act[dst] = pool(act[src])
+        elif op == "up":
+            act[dst] = up(act[src[0]]) + act[src[1]]
+        elif op == "out":
+            act[dst] = conv(act[src], wout[:, None, :], np.array([bout]))[..., 0]
+        else:
+            w, b = layers[op - 1]
+            act[dst] = relu(conv(act[src], w, b))
+    return act["out"], act
+
+
+def backward(params, act, dout):
+    """Gradient of sum(dout * out) with respect to the parameter vector."""
+    win, bin_, layers, wout, bout = unpack(params)
+    g = np.zeros(N_PARAMS)
+    grad = {k: np.zeros_like(v) for k, v in act.items() if k != "out"}
+    for op, src, dst in reversed(GRAPH):
+        if op == "out":
+            dx, dw, db = conv_bwd(act[src], wout[:, None, :], np.asarray(dout, dtype=np.float64)[..., None])
+            g[OUT_BASE: OUT_BASE + 9 * C] = dw[:, 0, :].ravel()
+            g[OUT_BASE + 9 * C] = db[0]
+            grad[src] += dx
+        elif op == "up":
+            grad[src[0]] += up_bwd(grad[dst])
+            grad[src[1]] += grad[dst]
+        elif op == "pool":
+            grad[src] += pool_bwd(grad[dst])
+        else:
+            dz = grad[dst] * (act[dst] > 0)
+            if op == "in":
+                dx, dw, db = conv_bwd(act[src], win, dz)
+                g[: 9 * C * 2] = dw.ravel()
+                g[9 * C * 2: N_IN] = db
+            else:
+                w, _ = layers[op - 1]
+                dx, dw, db = conv_bwd(act[src], w, dz)
+                ws, bs = layer_slices(op)
+                g[ws] = dw.ravel()
+                g[bs] = db
+            grad[src] += dx
+    return g
+
+
+def apply5(u, wx, wy):
+    """Dirichlet-zero 5-point operator with edge weights (interior rows only)."""
+    u = np.asarray(u, dtype=np.float64)
+    B, n, _ = u.shape
+    out = np.zeros_like(u)
+    v = u.copy()
+    v[:, 0, :] = v[:, -1, :] = 0.0
+    v[:, :, 0] = v[:, :, -1] = 0.0
+    c = v[:, 1:-1, 1:-1]
+    out[:, 1:-1, 1:-1] = (wx[:, 1:-1, 1:-1] * (c - v[:, 2:, 1:-1]) + wx[:, :-2, 1:-1] * (c - v[:, :-2, 1:-1])
+                          + wy[:, 1:-1, 1:-1] * (c - v[:, 1:-1, 2:]) + wy[:, 1:-1, :-2] * (c - v[:, 1:-1, :-2]))
+    return out
+
+
+def direction_loss(N, r, e, wx, wy):
+    """Per-image loss 1 - (r.N)^2/((e.r)(N.A N)) and the gradient of its batch mean."""
+    N = np.asarray(N, dtype=np.float64)
+    AN = apply5(N, wx, wy)
+    B = N.shape[0]
+    a = np.einsum("bij,bij->b", r, N)
+    b = np.einsum("bij,bij->b", N, AN)
+    E = np.einsum("bij,bij->b", e, r)
+    loss = 1.0 - a * a / (E * b)
+    dN = (2 * a / (E * b * B))[:, None, None] * (-r + (a / b)[:, None, None] * AN)
+    return loss, dN

@@ -14,7 +14,7 @@ SIL_OK = 0
 
 METRIC_IDENTITY, METRIC_DIAGONAL, METRIC_SYMMETRIC = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1
+PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_LEARNED = 0, 1, 2
 
 
 class SilError(RuntimeError):
@@ -69,6 +69,16 @@ class BatchResult(ctypes.Structure):
     ]
 
 
+class SolveRecord(ctypes.Structure):
+    _fields_ = [
+        ("every", ctypes.c_int32),
+        ("max_records", ctypes.c_int32),
+        ("r", ctypes.c_void_p),
+        ("x", ctypes.c_void_p),
+        ("count", ctypes.POINTER(ctypes.c_int32)),
+    ]
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -116,6 +126,24 @@ SIGNATURES = {
         [_P, _P, _P, ctypes.POINTER(SolverSettings), ctypes.POINTER(BatchResult), ctypes.POINTER(ctypes.c_double), _P],
     ),
     "sil_batched_poisson2d_destroy": (ctypes.c_int, [_P]),
+    "sil_unet_param_count": (_I64, []),
+    "sil_unet_create": (ctypes.c_int, [ctypes.POINTER(_P), _I32, _I32, _I32, ctypes.c_uint64]),
+    "sil_unet_destroy": (ctypes.c_int, [_P]),
+    "sil_unet_get_params": (ctypes.c_int, [_P, _P, _P]),
+    "sil_unet_set_params": (ctypes.c_int, [_P, _P, _P]),
+    "sil_unet_forward": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
+    "sil_unet_backward": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P]),
+    "sil_unet_loss": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "sil_unet_train_step": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P, ctypes.c_float, _P, _P]),
+    "sil_poisson2d_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), _I32, _I32, _P]),
+    "sil_poisson2d_batch_destroy": (ctypes.c_int, [_P]),
+    "sil_poisson2d_batch_apply": (ctypes.c_int, [_P, _P, _P, _P]),
+    "sil_poisson2d_batch_weights": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "sil_poisson2d_batch_solve": (
+        ctypes.c_int,
+        [_P, _P, _P, ctypes.POINTER(SolverSettings), _P, ctypes.POINTER(BatchResult),
+         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(SolveRecord), _P],
+    ),
 }
 
 _lib = None

@@ -29,6 +29,7 @@ class BatchedPoisson2D:
         h = ctypes.c_void_p()
         L.check(L.lib().sil_batched_poisson2d_create(ctypes.byref(h), len(self.sides), arr, L.ptr(kappa)))
         self.handle = h
+        self._destroy = L.lib().sil_batched_poisson2d_destroy
         self.block = 256
 
     def offset(self, p):
@@ -62,7 +63,10 @@ class BatchedPoisson2D:
         return solution, results, secs.value
 
     def __del__(self):
+        # the destroy function was bound at construction: module globals may
+        # already be gone when this runs at interpreter exit
         h = getattr(self, "handle", None)
-        if h is not None and h.value and L._lib is not None:
-            L.lib().sil_batched_poisson2d_destroy(h)
+        fn = getattr(self, "_destroy", None)
+        if h is not None and h.value and fn is not None:
+            fn(h)
             self.handle = None

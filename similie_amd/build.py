@@ -58,13 +58,32 @@ def build_cpp_tests(verbose=False):
 
 
 def build(force=True, verbose=False):
+    """Compiles every source to an object in parallel (in a temporary
+    directory), then links libsil_hip.so."""
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+
     if not force and not needs_build():
         return LIB
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, *FLAGS, *map(str, SOURCES), "-o", str(tmp)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True, cwd=str(ROOT))
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="sil_build_") as tmpdir:
+        objs = [Path(tmpdir) / (src.name + ".o") for src in SOURCES]
+
+        def compile_one(pair):
+            src, obj = pair
+            cmd = [HIPCC, *compile_flags, "-c", str(src), "-o", str(obj)]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True, cwd=str(ROOT))
+
+        jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            list(pool.map(compile_one, zip(SOURCES, objs)))
+        tmp = LIB.with_suffix(".so.tmp")
+        link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+        if verbose:
+            print(" ".join(link), flush=True)
+        subprocess.run(link, check=True, cwd=str(ROOT))
     os.replace(tmp, LIB)
     build_cpp_tests(verbose)
     return LIB

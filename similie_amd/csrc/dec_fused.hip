@@ -38,25 +38,46 @@ struct Coefs {
     double c;          // ⋆ XYZ → ∅
 };
 
+// 1/sqrt(v) for v > 0: the hardware estimate refined by two Newton–Raphson
+// steps y ← y + y (1/2 − (v y / 2) y), accurate to a few ulp.  Outside
+// [1e-290, 1e290] the estimate's range is not relied on and the quotient
+// 1/(sqrt(X) sqrt(Y) sqrt(Z)) of the caller's factors is used instead.
+__device__ __forceinline__ double inv_volume(double X, double Y, double Z)
+{
+    const double v = X * Y * Z;
+    if (v == 0.0) return 0.0;
+    if (v >= 1e-290 && v <= 1e290) {
+        double y = __builtin_amdgcn_rsq(v);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const double h = 0.5 * v * y;
+            y = fma(y, fma(-h, y, 0.5), y);
+        }
+        return y;
+    }
+    return 1.0 / (sqrt(X) * sqrt(Y) * sqrt(Z));
+}
+
 __device__ __forceinline__ Coefs node_coefs(double gx, double gy, double gz, double hx, double hy, double hz)
 {
     Coefs k;
     const bool metric_ok = (gx != 0.0) && (gy != 0.0) && (gz != 0.0);
     const double ax = fabs(hx), ay = fabs(hy), az = fabs(hz);
     const bool ok1 = metric_ok && ax >= 1e-14 && ay >= 1e-14 && az >= 1e-14;
-    const bool ok2 = metric_ok && fabs(hx * hy) >= 1e-14 && fabs(hx * hz) >= 1e-14 && fabs(hy * hz) >= 1e-14;
-    const bool ok3 = metric_ok && fabs(hx * hy * hz) >= 1e-14;
-    const double sx = sqrt(fabs(gx)) * ax, sy = sqrt(fabs(gy)) * ay, sz = sqrt(fabs(gz)) * az;
-    const double vol = sx * sy * sz;
-    const double inv = (vol != 0.0) ? 1.0 / vol : 0.0;
-    const double syz = sy * sz, sxz = sx * sz, sxy = sx * sy;
-    k.a1 = ok2 ? sx * sx * inv : 0.0;
-    k.a2 = ok2 ? sy * sy * inv : 0.0;
-    k.a3 = ok2 ? sz * sz * inv : 0.0;
-    k.b1 = ok1 ? syz * syz * inv : 0.0;
-    k.b2 = ok1 ? sxz * sxz * inv : 0.0;
-    k.b3 = ok1 ? sxy * sxy * inv : 0.0;
-    k.c = ok3 ? inv : 0.0;
+    const bool ok2 = metric_ok && ax * ay >= 1e-14 && ax * az >= 1e-14 && ay * az >= 1e-14;
+    const bool ok3 = metric_ok && ax * ay * az >= 1e-14;
+    // squared edge measures s_a² = |g_aa| h_a²; every coefficient is a
+    // product of them with c = 1/(sx sy sz)
+    const double X = fabs(gx) * ax * ax, Y = fabs(gy) * ay * ay, Z = fabs(gz) * az * az;
+    const double c = inv_volume(X, Y, Z);
+    const double c1 = ok1 ? c : 0.0, c2 = ok2 ? c : 0.0;
+    k.a1 = X * c2;
+    k.a2 = Y * c2;
+    k.a3 = Z * c2;
+    k.b1 = (Y * Z) * c1;
+    k.b2 = (X * Z) * c1;
+    k.b3 = (X * Y) * c1;
+    k.c = ok3 ? c : 0.0;
     return k;
 }
 
@@ -151,15 +172,25 @@ __global__ __launch_bounds__(256) void k_lap3d_direct(const double* __restrict__
 
 // ------------------------------------------------------ 3D 1-form, march ---
 
+constexpr int kMaxXChunk = 256;
+
+typedef __attribute__((address_space(1))) double GlobalDouble;
+
 template <int TY>
 struct MarchLds {
     double w[3][TY][64];   // ω(p) for the y+1 / z+1 neighbours of dω
     double ep[2][TY][64];  // ⋆ω(p+1): XZ and XY components for δω's y−1 / z−1 terms
     double eta[3][TY][64]; // ⋆dω(p) for dᵀ's y−1 / z−1 terms
     double dw[2][TY][64];  // δω ping-pong: planes p and p+1
+    double hx[kMaxXChunk + 4]; // x edge lengths of planes xa−1 … xb+2 (0 outside)
 };
 
-template <int TY, bool DIAG>
+// Loads are unconditional (indices clamped into the grid, values masked
+// afterwards) so no branch separates them from their consumers.  DEPTH is
+// the prefetch distance in planes: plane p+1+DEPTH is requested while plane
+// p is computed.  DEPTH 2 keeps two planes in flight (more registers, fewer
+// waves); DEPTH 1 keeps one (more waves per SIMD).
+template <int TY, bool DIAG, int DEPTH>
 __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restrict__ w, const double* __restrict__ g,
                                                          const double* __restrict__ hxs, const double* __restrict__ hys,
                                                          const double* __restrict__ hzs, double* __restrict__ out,
@@ -194,53 +225,66 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
     const bool pos_in = (z >= 0 && z < Nz && y >= 0 && y < Ny);
     const bool writer = lane >= 1 && lane <= 62 && row >= 1 && row <= TY - 2 && z < Oz && y < Oy;
     const bool has_y1 = (y + 1 < Ny), has_z1 = (z + 1 < Nz);
-    const double hy = pos_in ? hys[y] : 0.0;
-    const double hz = pos_in ? hzs[z] : 0.0;
+    const int64_t yc = min(max(y, (int64_t)0), Ny - 1), zc = min(max(z, (int64_t)0), Nz - 1);
+    const double hy = pos_in ? hys[yc] : 0.0;
+    const double hz = pos_in ? hzs[zc] : 0.0;
     const int64_t plane = Ny * Nz;
-    const int64_t lin_yz = y * Nz + z;
+    const int64_t lin_yz = yc * Nz + zc;
+    const int64_t x0 = xa - 1; // plane of s.hx[0]
 
-    auto load = [&](int64_t q, double W[3], double G[3], double& hxq) {
-        if (pos_in && q >= 0 && q < Nx) {
-            const int64_t lin = q * plane + lin_yz;
-            W[0] = w[lin * 3 + 0];
-            W[1] = w[lin * 3 + 1];
-            W[2] = w[lin * 3 + 2];
-            if (DIAG) {
-                G[0] = g[lin * 3 + 0];
-                G[1] = g[lin * 3 + 1];
-                G[2] = g[lin * 3 + 2];
-            } else {
-                G[0] = G[1] = G[2] = 1.0;
-            }
-            hxq = hxs[q];
+    for (int i = threadIdx.x; i < (int)(xb - xa) + 4; i += 64 * TY) {
+        const int64_t q = x0 + i;
+        s.hx[i] = (q >= 0 && q < Nx) ? hxs[q] : 0.0;
+    }
+
+    auto valid = [&](int64_t q) { return pos_in && q >= 0 && q < Nx; };
+    auto load = [&](int64_t q, double W[3], double G[3]) {
+        const int64_t qc = min(max(q, (int64_t)0), Nx - 1);
+        const int64_t lin = qc * plane + lin_yz;
+        W[0] = w[lin * 3 + 0];
+        W[1] = w[lin * 3 + 1];
+        W[2] = w[lin * 3 + 2];
+        if (DIAG) {
+            G[0] = g[lin * 3 + 0];
+            G[1] = g[lin * 3 + 1];
+            G[2] = g[lin * 3 + 2];
         } else {
-            W[0] = W[1] = W[2] = 0.0;
-            G[0] = G[1] = G[2] = 0.0; // outside: zero coefficients (zero-extension)
-            hxq = 0.0;
+            G[0] = G[1] = G[2] = 1.0;
         }
     };
-    auto coefs_of = [&](const double G[3], double hxq, bool valid) {
-        if (!valid) return Coefs{0, 0, 0, 0, 0, 0, 0};
-        return node_coefs(G[0], G[1], G[2], hxq, hy, hz);
+    // ω outside the grid is zero; the coefficients of outside nodes are zero
+    auto mask = [&](bool ok, double W[3]) {
+        W[0] = ok ? W[0] : 0.0;
+        W[1] = ok ? W[1] : 0.0;
+        W[2] = ok ? W[2] : 0.0;
+    };
+    auto coefs_of = [&](const double G[3], int64_t q) {
+        const Coefs k = node_coefs(G[0], G[1], G[2], s.hx[q - x0], hy, hz);
+        return valid(q) ? k : Coefs{0, 0, 0, 0, 0, 0, 0};
     };
 
-    // state: c = plane p, n = plane p+1, f = prefetched plane p+2
-    double Wc[3], Wn[3], Wf[3], Gc[3], Gn[3], Gf[3], hc, hn, hf;
-    int64_t p = xa - 1;
-    load(p, Wc, Gc, hc);
-    load(p + 1, Wn, Gn, hn);
-    Coefs Kc = coefs_of(Gc, hc, pos_in && p >= 0 && p < Nx);
-    Coefs Kn = coefs_of(Gn, hn, pos_in && p + 1 < Nx);
+    // planes: c = p, n = p+1 (ready); f1 = p+2, f2 = p+3 (in flight)
+    double Wc[3], Wn[3], W1[3], W2[3], Gc[3], Gn[3], G1[3], G2[3];
+    int64_t p = x0;
+    load(p, Wc, Gc);
+    load(p + 1, Wn, Gn);
+    if (DEPTH == 2) load(p + 2, W1, G1);
+    __syncthreads(); // s.hx
+    mask(valid(p), Wc);
+    mask(valid(p + 1), Wn);
+    Coefs Kc = coefs_of(Gc, p);
+    Coefs Kn = coefs_of(Gn, p + 1);
     double etaPrev[3] = {0.0, 0.0, 0.0};
-    // ⋆ω(p) own: YZ, XZ, XY with signs
-    double epc0 = Kc.b1 * Wc[0], epc1 = -Kc.b2 * Wc[1], epc2 = Kc.b3 * Wc[2];
-    (void)epc1;
-    (void)epc2;
+    double epc0 = Kc.b1 * Wc[0]; // ⋆ω(p) YZ component
     double dwc = 0.0;
-    int buf = 0;
+    double po[3] = {0.0, 0.0, 0.0}; // previous step's stored values
+    uint64_t pa = 0;                // and their address
 
-    for (; p < xb; ++p) {
-        load(p + 2, Wf, Gf, hf); // prefetch, consumed at the end of the step
+    // One step computes plane p.  The two register sets of the in-flight
+    // planes alternate between steps (the loop is unrolled by two), so a
+    // load's destination is never copied before it is consumed.
+    auto step = [&](double (&Wl)[3], double (&Gl)[3], const double (&Wu)[3], const double (&Gu)[3], const int buf) {
+        load(p + 1 + DEPTH, Wl, Gl);
 
         // ⋆ω(p+1) own
         const double epn0 = Kn.b1 * Wn[0], epn1 = -Kn.b2 * Wn[1], epn2 = Kn.b3 * Wn[2];
@@ -252,19 +296,28 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
         __syncthreads();
 
         // dω(p) own, clamped forward differences
-        const double* Wx = (p + 1 < Nx) ? Wn : Wc;
-        const double wy0 = has_y1 && row + 1 < TY ? s.w[0][row + 1][lane] : Wc[0];
-        const double wy2 = has_y1 && row + 1 < TY ? s.w[2][row + 1][lane] : Wc[2];
-        const double wz0 = has_z1 && lane + 1 < 64 ? s.w[0][row][lane + 1] : Wc[0];
-        const double wz1 = has_z1 && lane + 1 < 64 ? s.w[1][row][lane + 1] : Wc[1];
-        const double Fxy = (Wx[1] - Wc[1]) - (wy0 - Wc[0]);
-        const double Fxz = (Wx[2] - Wc[2]) - (wz0 - Wc[0]);
+        const bool has_x1 = p + 1 < Nx;
+        const double wx1 = has_x1 ? Wn[1] : Wc[1], wx2 = has_x1 ? Wn[2] : Wc[2];
+        // LDS neighbours are read unconditionally (indices kept in the tile)
+        // and selected afterwards, so no pointer ever mixes LDS and registers
+        const int ry = row + 1 < TY ? row + 1 : row, lz = lane + 1 < 64 ? lane + 1 : lane;
+        const int rm = row > 0 ? row - 1 : row, lm = lane > 0 ? lane - 1 : lane;
+        const double sy0 = s.w[0][ry][lane], sy2 = s.w[2][ry][lane];
+        const double sz0 = s.w[0][row][lz], sz1 = s.w[1][row][lz];
+        const bool ny = has_y1 && row + 1 < TY, nz = has_z1 && lane + 1 < 64;
+        const double wy0 = ny ? sy0 : Wc[0];
+        const double wy2 = ny ? sy2 : Wc[2];
+        const double wz0 = nz ? sz0 : Wc[0];
+        const double wz1 = nz ? sz1 : Wc[1];
+        const double Fxy = (wx1 - Wc[1]) - (wy0 - Wc[0]);
+        const double Fxz = (wx2 - Wc[2]) - (wz0 - Wc[0]);
         const double Fyz = (wy2 - Wc[2]) - (wz1 - Wc[1]);
         const double eta0 = Kc.a1 * Fyz, eta1 = -Kc.a2 * Fxz, eta2 = Kc.a3 * Fxy;
 
         // δω(p+1) own: −⋆(dᵀ ⋆ω) with zero outside
-        const double ep_y1 = row > 0 ? s.ep[0][row - 1][lane] : 0.0;
-        const double ep_z2 = lane > 0 ? s.ep[1][row][lane - 1] : 0.0;
+        const double sep0 = s.ep[0][rm][lane], sep1 = s.ep[1][row][lm];
+        const double ep_y1 = row > 0 ? sep0 : 0.0;
+        const double ep_z2 = lane > 0 ? sep1 : 0.0;
         const double H = (epn0 - epc0) - (epn1 - ep_y1) + (epn2 - ep_z2);
         const double dwn = -Kn.c * H;
 
@@ -274,20 +327,44 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
         s.dw[buf ^ 1][row][lane] = dwn;
         __syncthreads();
 
+        // plane p+2 (requested one step ago) becomes the next plane n; this
+        // wait precedes the stores below, so it never waits for them
+        const double(&Wr)[3] = DEPTH == 2 ? Wu : Wl;
+        const double(&Gr)[3] = DEPTH == 2 ? Gu : Gl;
+        double Wnn[3] = {Wr[0], Wr[1], Wr[2]};
+        mask(valid(p + 2), Wnn);
+        const Coefs Knn = coefs_of(Gr, p + 2);
+
+        // Δω(p); every lane computes it (halo lanes read clamped LDS
+        // indices), only the tile's interior stores it
+        const double ey0 = s.eta[0][rm][lane], ey2 = s.eta[2][rm][lane];
+        const double ez0 = s.eta[0][row][lm], ez1 = s.eta[1][row][lm];
+        const double Gxy = (eta1 - etaPrev[1]) - (eta0 - ey0);
+        const double Gxz = (eta2 - etaPrev[2]) - (eta0 - ez0);
+        const double Gyz = (eta2 - ey2) - (eta1 - ez1);
+        const double sdy = s.dw[buf][ry][lane], sdz = s.dw[buf][row][lz];
+        const double ddx = has_x1 ? dwn - dwc : 0.0;
+        const double ddy = has_y1 ? sdy - dwc : 0.0;
+        const double ddz = has_z1 ? sdz - dwc : 0.0;
+        const double o0 = Kc.a1 * Gyz + ddx, o1 = -Kc.a2 * Gxz + ddy, o2 = Kc.a3 * Gxy + ddz;
+        // the address is made opaque so the store uses this very register
+        uint64_t oa = reinterpret_cast<uint64_t>(out + ((p * Oy + y) * Oz + z) * 3);
+        asm volatile("" : "+v"(oa));
+        GlobalDouble* o = reinterpret_cast<GlobalDouble*>(oa);
         if (p >= xa && writer) {
-            const double ey0 = s.eta[0][row - 1][lane], ey2 = s.eta[2][row - 1][lane];
-            const double ez0 = s.eta[0][row][lane - 1], ez1 = s.eta[1][row][lane - 1];
-            const double Gxy = (eta1 - etaPrev[1]) - (eta0 - ey0);
-            const double Gxz = (eta2 - etaPrev[2]) - (eta0 - ez0);
-            const double Gyz = (eta2 - ey2) - (eta1 - ez1);
-            const double ddx = (p + 1 < Nx) ? dwn - dwc : 0.0;
-            const double ddy = has_y1 ? s.dw[buf][row + 1][lane] - dwc : 0.0;
-            const double ddz = has_z1 ? s.dw[buf][row][lane + 1] - dwc : 0.0;
-            double* o = out + ((p * Oy + y) * Oz + z) * 3;
-            o[0] = Kc.a1 * Gyz + ddx;
-            o[1] = -Kc.a2 * Gxz + ddy;
-            o[2] = Kc.a3 * Gxy + ddz;
+            o[0] = o0;
+            o[1] = o1;
+            o[2] = o2;
         }
+        // The registers a store reads (data and address) stay reserved until the next step has
+        // issued its own stores: reusing them earlier would make the wave
+        // wait for this step's stores, and with them for every load issued
+        // before them, including the prefetch of plane p+3.
+        asm volatile("" ::"v"(po[0]), "v"(po[1]), "v"(po[2]), "v"(pa));
+        pa = oa;
+        po[0] = o0;
+        po[1] = o1;
+        po[2] = o2;
 
         // rotate
         etaPrev[0] = eta0;
@@ -295,15 +372,25 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
         etaPrev[2] = eta2;
         epc0 = epn0;
         dwc = dwn;
-        buf ^= 1;
-        Wc[0] = Wn[0];
-        Wc[1] = Wn[1];
-        Wc[2] = Wn[2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Wc[c] = Wn[c];
+            Wn[c] = Wnn[c];
+        }
         Kc = Kn;
-        Wn[0] = Wf[0];
-        Wn[1] = Wf[1];
-        Wn[2] = Wf[2];
-        Kn = coefs_of(Gf, hf, pos_in && p + 2 >= 0 && p + 2 < Nx);
+        Kn = Knn;
+    };
+
+    // drain the prologue's loads once (vmcnt(0)) so the loop's wait counts
+    // depend only on the loop's own memory operations
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    for (;;) {
+        if (p >= xb) break;
+        step(W2, G2, W1, G1, 0);
+        ++p;
+        if (p >= xb) break;
+        step(W1, G1, W2, G2, 1);
+        ++p;
     }
 }
 
@@ -391,21 +478,23 @@ inline unsigned blocks_for(int64_t total, int block, int64_t cap = 1 << 20)
     return static_cast<unsigned>(b);
 }
 
-template <int TY>
+template <int TY, int DEPTH>
 int launch_march(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk,
                  hipStream_t stream)
 {
     const int ntz = static_cast<int>((o[2] + 61) / 62);
     const int nty = static_cast<int>((o[1] + TY - 3) / (TY - 2));
+    SIL_REQUIRE(xchunk >= 1 && xchunk <= kMaxXChunk, SIL_ERR_INVALID,
+                "laplacian: x chunk %d outside [1, %d]", xchunk, kMaxXChunk);
     const int nxc = static_cast<int>((o[0] + xchunk - 1) / xchunk);
     const int64_t nb = (int64_t)ntz * nty * nxc;
     SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
     if (g.metric)
-        hipLaunchKernelGGL((k_lap3d_march<TY, true>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+        hipLaunchKernelGGL((k_lap3d_march<TY, true, DEPTH>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
                            g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
                            o[2], ntz, nty, nxc, xchunk);
     else
-        hipLaunchKernelGGL((k_lap3d_march<TY, false>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+        hipLaunchKernelGGL((k_lap3d_march<TY, false, DEPTH>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
                            g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
                            o[2], ntz, nty, nxc, xchunk);
     SIL_LAUNCH_CHECK();
@@ -431,13 +520,15 @@ int launch_laplacian_3d_1form(const FusedGeom& g, const double* in, double* out,
         return SIL_OK;
     }
     const int xchunk = tile.xchunk > 0 ? tile.xchunk : 32;
+    const bool deep = tile.variant == FUSED_MARCH_DEEP;
     switch (tile.ty) {
     case 4:
-        return launch_march<4>(g, in, out, o, xchunk, stream);
+        return deep ? launch_march<4, 2>(g, in, out, o, xchunk, stream) : launch_march<4, 1>(g, in, out, o, xchunk, stream);
     case 8:
-        return launch_march<8>(g, in, out, o, xchunk, stream);
+        return deep ? launch_march<8, 2>(g, in, out, o, xchunk, stream) : launch_march<8, 1>(g, in, out, o, xchunk, stream);
     case 16:
-        return launch_march<16>(g, in, out, o, xchunk, stream);
+        return deep ? launch_march<16, 2>(g, in, out, o, xchunk, stream)
+                    : launch_march<16, 1>(g, in, out, o, xchunk, stream);
     default:
         set_error("laplacian: unsupported tile height %d (4, 8 or 16)", tile.ty);
         return SIL_ERR_INVALID;

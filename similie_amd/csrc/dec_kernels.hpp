@@ -24,7 +24,7 @@ struct FusedGeom {
     const double* metric; // diagonal [node][ndim] or null for the identity
 };
 
-enum FusedVariant { FUSED_DIRECT = 0, FUSED_MARCH = 1 };
+enum FusedVariant { FUSED_DIRECT = 0, FUSED_MARCH = 1, FUSED_MARCH_DEEP = 2 };
 
 struct FusedTile {
     int variant;  // FusedVariant

@@ -57,7 +57,6 @@ __device__ __forceinline__ Chunk8 zero_chunk()
 
 // Y = act(conv(X ⊙ [M > 0]) + bias): forward (M null, bias, relu) or input
 // gradient (M = forward output of the layer, flipped weights, no bias/relu).
-// Optional `add` (same shape as Y) is added before the activation.
 __global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ M,
                                                const bf16* __restrict__ Wt, const float* __restrict__ bias,
                                                bf16* __restrict__ Y, int H, int W, int relu)
@@ -125,7 +124,7 @@ __global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const
             const int gx = ox0 + Mfma32::d_row(lane, r);
             float v = acc[m][r] + bv;
             if (relu) v = fmaxf(v, 0.f);
-            Y[img + ((size_t)gy * W + gx) * C + co] = to_bf16(v);
+            if (gy < H && gx < W) Y[img + ((size_t)gy * W + gx) * C + co] = to_bf16(v);
         }
     }
 }
@@ -140,7 +139,7 @@ __global__ __launch_bounds__(NT) void k_conv33_wgrad(const bf16* __restrict__ X,
     __shared__ __attribute__((aligned(16))) bf16 sX[(TH + 2) * (TW + 2) * CP];
     __shared__ __attribute__((aligned(16))) bf16 sZ[C * (TH * TW + 8)]; // transposed dZ: [co][pixel]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tiles_x = W / TW, tiles_y = H / TH;
+    const int tiles_x = (W + TW - 1) / TW, tiles_y = (H + TH - 1) / TH;
     const int ntiles = tiles_x * tiles_y * B;
     // taps per wave: 0:{0,4,8} 1:{1,5} 2:{2,6} 3:{3,7}
     f32x16 acc[3];
@@ -168,9 +167,12 @@ __global__ __launch_bounds__(NT) void k_conv33_wgrad(const bf16* __restrict__ X,
         for (int c = tid; c < TH * TW * 4; c += NT) {
             const int pix = c >> 2, part = c & 3;
             const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
-            const size_t off = img + ((size_t)gy * W + gx) * C + part * 8;
-            const Chunk8 d = *reinterpret_cast<const Chunk8*>(&dY[off]);
-            const Chunk8 y = *reinterpret_cast<const Chunk8*>(&Yf[off]);
+            Chunk8 d = zero_chunk(), y = zero_chunk();
+            if (gy < H && gx < W) {
+                const size_t off = img + ((size_t)gy * W + gx) * C + part * 8;
+                d = *reinterpret_cast<const Chunk8*>(&dY[off]);
+                y = *reinterpret_cast<const Chunk8*>(&Yf[off]);
+            }
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const float z = from_bf16(y.v[i]) > 0.f ? from_bf16(d.v[i]) : 0.f;
@@ -231,7 +233,8 @@ __global__ void k_reduce_parts(const float* __restrict__ part, int parts, int n,
 
 // -------------------------------------------------- input / output convs ---
 
-// a0 = relu(conv_in(x)): x FP32 [B][H][W][2], weights [tap][co][ci=2]
+// a0 = relu(conv_in(x)): x FP32 [B][H][W][2], weights [tap][co][ci=2];
+// one thread per (pixel, group of 8 output channels)
 __global__ void k_conv_in(const float* __restrict__ x, const float* __restrict__ w, const float* __restrict__ bias,
                           bf16* __restrict__ y, int B, int H, int W)
 {
@@ -239,23 +242,31 @@ __global__ void k_conv_in(const float* __restrict__ x, const float* __restrict__
     for (int i = threadIdx.x; i < 9 * C * 2; i += blockDim.x) sw[i] = w[i];
     for (int i = threadIdx.x; i < C; i += blockDim.x) sw[9 * C * 2 + i] = bias[i];
     __syncthreads();
-    const size_t n = (size_t)B * H * W;
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-        const int xx = p % W, yy = (p / W) % H;
+    const size_t n = (size_t)B * H * W * 4;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+        const int grp = (int)(idx & 3);
+        const size_t p = idx >> 2;
+        const int xx = (int)(p % W), yy = (int)((p / W) % H);
         const size_t b = p / ((size_t)W * H);
-        float in[9][2];
+        float acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = sw[9 * C * 2 + grp * 8 + c];
+#pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int gy = yy + t / 3 - 1, gx = xx + t % 3 - 1;
-            const bool ok = gy >= 0 && gy < H && gx >= 0 && gx < W;
+            if (gy < 0 || gy >= H || gx < 0 || gx >= W) continue;
             const size_t q = (b * H + gy) * W + gx;
-            in[t][0] = ok ? x[q * 2] : 0.f;
-            in[t][1] = ok ? x[q * 2 + 1] : 0.f;
+            const float i0 = x[q * 2], i1 = x[q * 2 + 1];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int co = grp * 8 + c;
+                acc[c] += sw[(t * C + co) * 2] * i0 + sw[(t * C + co) * 2 + 1] * i1;
+            }
         }
-        for (int co = 0; co < C; ++co) {
-            float s = sw[9 * C * 2 + co];
-            for (int t = 0; t < 9; ++t) s += sw[(t * C + co) * 2] * in[t][0] + sw[(t * C + co) * 2 + 1] * in[t][1];
-            y[p * C + co] = to_bf16(fmaxf(s, 0.f));
-        }
+        Chunk8 o;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) o.v[c] = to_bf16(fmaxf(acc[c], 0.f));
+        *reinterpret_cast<Chunk8*>(&y[p * C + grp * 8]) = o;
     }
 }
 
@@ -444,37 +455,87 @@ __global__ void k_copy_bf16(const bf16* __restrict__ x, bf16* __restrict__ y, si
 
 // ------------------------------------------------------------- loss ---
 
-// d = e − out; Ad = A d (Dirichlet-zero 5-point, edge weights wx, wy per node);
-// loss_b = dᵀAd / norm_b (norm_b = eᵀ A e); dout = −2 Ad / norm_b / B.
-// Partial per block of Σ dᵀAd / norm_b.
-__global__ void k_anorm_loss(const float* __restrict__ out, const float* __restrict__ e, const float* __restrict__ wx,
-                             const float* __restrict__ wy, const float* __restrict__ norm, float* __restrict__ dout,
-                             float* __restrict__ part, int B, int n)
+// Direction loss of the preconditioner output N against the error e of the
+// residual r = A e (A the Dirichlet-zero 5-point operator with edge weights
+// wx, wy): L = 1 − cos²_A(e, N) = 1 − (rᵀN)² / ((eᵀr)(NᵀAN)), the relative
+// A-norm error left after the best step along N.  Stage 1 stores AN and the
+// per-block partial sums of rᵀN, NᵀAN and eᵀr.
+__device__ __forceinline__ float apply5(const float* __restrict__ u, const float* __restrict__ wx,
+                                        const float* __restrict__ wy, size_t t, int i, int j, int n)
 {
-    const size_t total = (size_t)B * n * n;
-    float acc = 0.f;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int j = t % n, i = (t / n) % n;
-        const size_t b = t / ((size_t)n * n);
-        float ad = 0.f;
-        const bool inner = i > 0 && j > 0 && i + 1 < n && j + 1 < n;
-        if (inner) {
-            auto d = [&](size_t q, bool in) { return in ? e[q] - out[q] : 0.f; };
-            const float dc = d(t, true);
-            const float de = d(t + n, i + 2 < n), dw = d(t - n, i > 1), dn = d(t + 1, j + 2 < n), ds = d(t - 1, j > 1);
-            ad = wx[t] * (dc - de) + wx[t - n] * (dc - dw) + wy[t] * (dc - dn) + wy[t - 1] * (dc - ds);
-            acc += dc * ad / norm[b];
-        }
-        dout[t] = -2.f * ad / norm[b] / (float)B;
-    }
-    __shared__ float sh[256];
-    sh[threadIdx.x] = acc;
+    if (!(i > 0 && j > 0 && i + 1 < n && j + 1 < n)) return 0.f;
+    const float uc = u[t];
+    const float ue = i + 2 < n ? u[t + n] : 0.f, uw = i > 1 ? u[t - n] : 0.f;
+    const float un = j + 2 < n ? u[t + 1] : 0.f, us = j > 1 ? u[t - 1] : 0.f;
+    return wx[t] * (uc - ue) + wx[t - n] * (uc - uw) + wy[t] * (uc - un) + wy[t - 1] * (uc - us);
+}
+
+template <int NT_>
+__device__ __forceinline__ void block_sum3(float v0, float v1, float v2, float* out)
+{
+    __shared__ float sh[3][NT_];
+    sh[0][threadIdx.x] = v0;
+    sh[1][threadIdx.x] = v1;
+    sh[2][threadIdx.x] = v2;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    for (int st = NT_ / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + st];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + st];
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + st];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) {
+        out[0] = sh[0][0];
+        out[1] = sh[1][0];
+        out[2] = sh[2][0];
+    }
+}
+
+// grid (nbx, B); r is read with stride rs (the network input holds it
+// interleaved with the coefficient channel)
+__global__ __launch_bounds__(256) void k_cos_loss_stats(const float* __restrict__ N, const float* __restrict__ r,
+                                                        int rs, const float* __restrict__ e,
+                                                        const float* __restrict__ wx, const float* __restrict__ wy,
+                                                        float* __restrict__ AN, float* __restrict__ part, int n)
+{
+    const int b = blockIdx.y;
+    const size_t nn = (size_t)n * n, base = (size_t)b * nn;
+    float sa = 0.f, sb = 0.f, se = 0.f;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nn; t += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)(t % n), i = (int)(t / n);
+        const float an = apply5(N + base, wx + base, wy + base, t, i, j, n);
+        AN[base + t] = an;
+        const float rv = r[(base + t) * rs];
+        sa += rv * N[base + t];
+        sb += N[base + t] * an;
+        se += e[base + t] * rv;
+    }
+    block_sum3<256>(sa, sb, se, part + ((size_t)b * gridDim.x + blockIdx.x) * 3);
+}
+
+// dN = (2a / (E b B)) (−r + (a/b) AN); loss[b] = 1 − a² / (E b)
+__global__ __launch_bounds__(256) void k_cos_loss_grad(const float* __restrict__ r, int rs,
+                                                       const float* __restrict__ AN, const float* __restrict__ part,
+                                                       int parts, float* __restrict__ dN, float* __restrict__ loss,
+                                                       int n, int B)
+{
+    const int b = blockIdx.y;
+    const size_t nn = (size_t)n * n, base = (size_t)b * nn;
+    double a = 0.0, bb = 0.0, E = 0.0;
+    for (int q = 0; q < parts; ++q) {
+        const float* pq = part + ((size_t)b * parts + q) * 3;
+        a += pq[0];
+        bb += pq[1];
+        E += pq[2];
+    }
+    const bool ok = bb > 0.0 && E > 0.0;
+    const float f = ok ? (float)(2.0 * a / (E * bb * B)) : 0.f;
+    const float g = ok ? (float)(a / bb) : 0.f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) loss[b] = ok ? (float)(1.0 - a * a / (E * bb)) : 1.f;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nn; t += (size_t)gridDim.x * blockDim.x)
+        dN[base + t] = f * (-r[(base + t) * rs] + g * AN[base + t]);
 }
 
 // ------------------------------------------------------------ Adam ---
@@ -520,11 +581,8 @@ namespace unet {
 int conv33(const bf16* X, const bf16* M, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W, bool relu,
            hipStream_t s)
 {
-    if (H % TH || W % TW) {
-        set_error("unet: %dx%d is not a multiple of the %dx%d tile", H, W, TH, TW);
-        return SIL_ERR_UNSUPPORTED;
-    }
-    hipLaunchKernelGGL(k_conv33, dim3(W / TW, H / TH, B), dim3(NT), 0, s, X, M, Wt, bias, Y, H, W, relu ? 1 : 0);
+    hipLaunchKernelGGL(k_conv33, dim3((W + TW - 1) / TW, (H + TH - 1) / TH, B), dim3(NT), 0, s, X, M, Wt, bias, Y, H,
+                       W, relu ? 1 : 0);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
@@ -532,7 +590,7 @@ int conv33(const bf16* X, const bf16* M, const bf16* Wt, const float* bias, bf16
 int conv33_wgrad(const bf16* X, const bf16* dY, const bf16* Yf, float* part_w, float* part_b, int parts, float* dW,
                  float* db, int B, int H, int W, hipStream_t s)
 {
-    const int ntiles = (W / TW) * (H / TH) * B;
+    const int ntiles = ((W + TW - 1) / TW) * ((H + TH - 1) / TH) * B;
     const int grid = ntiles < parts ? ntiles : parts;
     hipLaunchKernelGGL(k_conv33_wgrad, dim3(grid), dim3(NT), 0, s, X, dY, Yf, part_w, part_b, B, H, W);
     hipLaunchKernelGGL(k_reduce_parts, dim3(36), dim3(256), 0, s, part_w, grid, 9 * C * C, dW);
@@ -543,7 +601,7 @@ int conv33_wgrad(const bf16* X, const bf16* dY, const bf16* Yf, float* part_w, f
 
 int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_conv_in, dim3(grid1((size_t)B * H * W)), dim3(256), 0, s, x, w, bias, y, B, H, W);
+    hipLaunchKernelGGL(k_conv_in, dim3(grid1((size_t)B * H * W * 4)), dim3(256), 0, s, x, w, bias, y, B, H, W);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
@@ -608,11 +666,11 @@ int copy(const bf16* x, bf16* y, size_t n, hipStream_t s)
     return SIL_OK;
 }
 
-int anorm_loss(const float* out, const float* e, const float* wx, const float* wy, const float* norm, float* dout,
-               float* part, int parts, float* loss, int B, int n, hipStream_t s)
+int cos_loss(const float* N, const float* r, int rs, const float* e, const float* wx, const float* wy, float* AN,
+             float* part, int parts, float* dN, float* loss, int B, int n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_anorm_loss, dim3(parts), dim3(256), 0, s, out, e, wx, wy, norm, dout, part, B, n);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(1), 0, s, part, parts, 1, loss);
+    hipLaunchKernelGGL(k_cos_loss_stats, dim3(parts, B), dim3(256), 0, s, N, r, rs, e, wx, wy, AN, part, n);
+    hipLaunchKernelGGL(k_cos_loss_grad, dim3(parts, B), dim3(256), 0, s, r, rs, AN, part, parts, dN, loss, n, B);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }

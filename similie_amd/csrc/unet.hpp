@@ -9,6 +9,8 @@
 
 #include "common.hpp"
 
+struct sil_unet_s;
+
 namespace sil::unet {
 
 using bf16 = __bf16;
@@ -39,11 +41,15 @@ int pool_bwd_accumulate(const bf16* dy, bf16* dx, int B, int H, int W, hipStream
 int upadd(const bf16* xc, const bf16* skip, bf16* y, int B, int H, int W, hipStream_t s);
 int up_bwd(const bf16* dy, bf16* dxc, int B, int h, int w, hipStream_t s);
 int copy(const bf16* x, bf16* y, std::size_t n, hipStream_t s);
-// energy-norm loss Σ_b (e−out)ᵀA(e−out)/norm_b and its gradient dout
-int anorm_loss(const float* out, const float* e, const float* wx, const float* wy, const float* norm, float* dout,
-               float* part, int parts, float* loss, int B, int n, hipStream_t s);
+// direction loss 1 − cos²_A(e, N) per image and its gradient dN (batch mean);
+// r is read with element stride rs; part holds B·parts·3 floats
+int cos_loss(const float* N, const float* r, int rs, const float* e, const float* wx, const float* wy, float* AN,
+             float* part, int parts, float* dN, float* loss, int B, int n, hipStream_t s);
 int adam(float* w, const float* g, float* m, float* v, int n, float lr, int step, hipStream_t s);
 // BF16 forward weights and the flipped, channel-transposed input-gradient weights of one 32→32 layer
 int pack33(const float* w, bf16* wf, bf16* wb, hipStream_t s);
+
+// maximum batch and image size of a network handle (defined with the handle)
+void net_shape(const struct ::sil_unet_s* net, int* max_batch, int* height, int* width);
 
 }  // namespace sil::unet

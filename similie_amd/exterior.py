@@ -88,11 +88,15 @@ class Grid:
             L.check(L.lib().sil_grid_create(ctypes.byref(handle), self.ndim, ext, None, L.ptr(self.nodal), kind,
                                             L.ptr(metric)))
         self.handle = handle
+        self._destroy = L.lib().sil_grid_destroy
 
     def __del__(self):
+        # the destroy function was bound at construction: module globals may
+        # already be gone when this runs at interpreter exit
         h = getattr(self, "handle", None)
-        if h is not None and h.value and L._lib is not None:
-            L.lib().sil_grid_destroy(h)
+        fn = getattr(self, "_destroy", None)
+        if h is not None and h.value and fn is not None:
+            fn(h)
             self.handle = None
 
 
@@ -180,6 +184,7 @@ class StagedLaplacian:
         L.check(L.lib().sil_make_staged_laplacian(ctypes.byref(h), grid.handle, degree, ctypes.byref(vin),
                                                   ctypes.byref(vout)))
         self.handle = h
+        self._destroy = L.lib().sil_staged_laplacian_destroy
 
     def run(self, out, tensor, stream=None):
         if tuple(out.shape) != self.shape_out or tuple(tensor.shape) != self.shape_in:
@@ -188,9 +193,12 @@ class StagedLaplacian:
         return out
 
     def __del__(self):
+        # the destroy function was bound at construction: module globals may
+        # already be gone when this runs at interpreter exit
         h = getattr(self, "handle", None)
-        if h is not None and h.value and L._lib is not None:
-            L.lib().sil_staged_laplacian_destroy(h)
+        fn = getattr(self, "_destroy", None)
+        if h is not None and h.value and fn is not None:
+            fn(h)
             self.handle = None
 
 

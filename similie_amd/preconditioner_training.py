@@ -1,0 +1,178 @@
+"""Problem generator, training-data generation, training and evaluation of
+the learned preconditioner (no reference counterpart).
+
+Problems: Dirichlet-zero Poisson problems -div(κ grad u) = b on n x n
+unit-spaced grids, κ = exp(g) with g a Gaussian random field (squared-
+exponential covariance, standard deviation σ, correlation length ℓ cells),
+b ~ N(0, 1) on interior nodes.
+
+Training data are (residual, error) pairs that plain CG itself produces:
+Jacobi CG runs every training problem to 1e-10 while the batch solver
+records r_k and x_k; the error of record k is e_k = x* − x_k with x* the
+converged solution.  The network maps [r/rms(r), normalised log κ] to a
+search direction and is trained with the direction loss
+1 − cos²_A(e, N) (sil_unet_train_step).
+
+    python -m similie_amd.preconditioner_training --out similie_amd/weights/unet_poisson256.npy
+
+The random fields and right-hand sides are synthetic inputs built with torch
+(device memory plumbing); every solve and every network pass runs through
+libsil_hip.so.
+"""
+import argparse
+import json
+import math
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import learned as Lr
+
+DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "weights" / "unet_poisson256.npy"
+
+
+def gaussian_random_field(batch, n, sigma, corr, seed, device="cuda"):
+    """[batch][n][n] fields with covariance σ² exp(−r²/(2 ℓ²)) (spectral
+    synthesis on a periodic 2n grid, cropped), each rescaled to mean 0 and
+    standard deviation σ."""
+    gen = torch.Generator(device=device).manual_seed(seed)
+    m = 2 * n
+    white = torch.randn((batch, m, m), generator=gen, dtype=torch.float64, device=device)
+    k = torch.fft.fftfreq(m, d=1.0, device=device, dtype=torch.float64) * 2 * math.pi
+    k2 = k[:, None] ** 2 + k[None, :] ** 2
+    amp = torch.exp(-0.25 * k2 * corr * corr)  # sqrt of the squared-exponential spectrum
+    g = torch.fft.ifft2(torch.fft.fft2(white) * amp).real[:, :n, :n]
+    g = g - g.mean(dim=(1, 2), keepdim=True)
+    g = g / g.std(dim=(1, 2), keepdim=True)
+    return sigma * g
+
+
+def make_problems(batch, n, sigma, corr, seed, device="cuda"):
+    """(κ, b) of `batch` problems; seeds of different problem sets must differ."""
+    kappa = torch.exp(gaussian_random_field(batch, n, sigma, corr, seed, device))
+    gen = torch.Generator(device=device).manual_seed(seed + 7919)
+    b = torch.randn((batch, n, n), generator=gen, dtype=torch.float64, device=device)
+    b[:, 0, :] = 0
+    b[:, -1, :] = 0
+    b[:, :, 0] = 0
+    b[:, :, -1] = 0
+    return kappa, b
+
+
+def generate_pairs(n_problems, n, sigma, corr, seed, pairs_per_problem, batch=64, record_every=8, log=print):
+    """(inputs [P][n][n][2], errors [P][n][n], wx, wy, problem index) with
+    P = n_problems · pairs_per_problem, residual and error scaled by 1/rms(r)."""
+    inputs, errors, wxs, wys, owners = [], [], [], [], []
+    for start in range(0, n_problems, batch):
+        B = min(batch, n_problems - start)
+        kappa, b = make_problems(B, n, sigma, corr, seed + start)
+        pb = Lr.PoissonBatch(kappa)
+        t0 = time.time()
+        x, res, secs, (rr, xr) = pb.solve(b, method="cg", tol=1e-10, record_every=record_every, max_records=100000)
+        its = np.array([r["iterations"] for r in res])
+        log(f"pairs: problems {start}..{start + B - 1}: CG iterations {its.min()}..{its.max()}, "
+            f"{rr.shape[0]} records, {time.time() - t0:.1f} s")
+        wx, wy, lk = pb.weights()
+        xstar = x.float()
+        for p in range(B):
+            valid = int(min(rr.shape[0], max(1, its[p] // record_every)))
+            pick = np.unique(np.linspace(0, valid - 1, pairs_per_problem).round().astype(int))
+            for k in pick:
+                r = rr[k, p]
+                rho = torch.sqrt(torch.mean(r.double() ** 2)).float().clamp_min(1e-30)
+                e = (xstar[p] - xr[k, p]) / rho
+                inputs.append(torch.stack([r / rho, lk[p]], dim=-1))
+                errors.append(e)
+                owners.append(start + p)
+            wxs.append(wx[p])
+            wys.append(wy[p])
+        del rr, xr
+    return (torch.stack(inputs), torch.stack(errors), torch.stack(wxs), torch.stack(wys),
+            torch.tensor(owners, device="cuda"))
+
+
+def train(net, data, epochs, batch=32, lr_max=1e-3, lr_min=1e-5, seed=0, log=print):
+    inputs, errors, wx, wy, owners = data
+    P = inputs.shape[0]
+    steps = epochs * (P // batch)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    step = 0
+    history = []
+    t0 = time.time()
+    for ep in range(epochs):
+        perm = torch.randperm(P, generator=gen, device="cuda")
+        losses = []
+        for s in range(P // batch):
+            idx = perm[s * batch: (s + 1) * batch]
+            lr = lr_min + 0.5 * (lr_max - lr_min) * (1 + math.cos(math.pi * step / max(1, steps)))
+            own = owners[idx]
+            loss = net.train_step(inputs[idx].contiguous(), errors[idx].contiguous(), wx[own].contiguous(),
+                                  wy[own].contiguous(), lr)
+            losses.append(loss.mean())
+            step += 1
+        mean = torch.stack(losses).mean().item()
+        history.append(mean)
+        log(f"epoch {ep}: mean direction loss {mean:.4f} ({time.time() - t0:.0f} s)")
+    return history
+
+
+def evaluate(net, n_problems, n, sigma, corr, seed, methods=("cg", "cg_mixed", "fcg_learned", "fcg_learned_mixed"),
+             tol=1e-10, log=print):
+    """Time to an FP64-verified tol of each method on one batch of held-out problems."""
+    kappa, b = make_problems(n_problems, n, sigma, corr, seed)
+    pb = Lr.PoissonBatch(kappa)
+    out = {}
+    for m in methods:
+        # first call warms up (graph instantiation, allocation), second is timed
+        pb.solve(b, method=m, tol=tol, net=net if m.startswith("fcg") else None, max_iterations=20)
+        x, res, secs = pb.solve(b, method=m, tol=tol, net=net if m.startswith("fcg") else None)
+        its = [r["iterations"] for r in res]
+        worst = max(r["true_relative_residual"] for r in res)
+        out[m] = {"seconds": secs, "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its)),
+                  "converged": int(sum(r["converged"] for r in res)), "worst_true_relative_residual": worst}
+        log(f"{m}: {secs * 1e3:.1f} ms, iterations mean {np.mean(its):.1f} max {max(its)}, worst residual {worst:.2e}")
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--sigma", type=float, default=1.5)
+    ap.add_argument("--corr", type=float, default=16.0)
+    ap.add_argument("--problems", type=int, default=200)
+    ap.add_argument("--pairs-per-problem", type=int, default=100)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--eval-seed", type=int, default=900000)
+    ap.add_argument("--eval-problems", type=int, default=64)
+    ap.add_argument("--out", type=Path, default=DEFAULT_WEIGHTS)
+    ap.add_argument("--log", type=Path, default=None)
+    a = ap.parse_args(argv)
+    logf = open(a.log, "w") if a.log else None
+
+    def log(msg):
+        print(msg, flush=True)
+        if logf:
+            logf.write(msg + "\n")
+            logf.flush()
+
+    t0 = time.time()
+    data = generate_pairs(a.problems, a.n, a.sigma, a.corr, a.seed, a.pairs_per_problem, log=log)
+    log(f"{data[0].shape[0]} pairs in {time.time() - t0:.0f} s")
+    net = Lr.UNet(max(a.batch, a.eval_problems), a.n, a.n, seed=a.seed)
+    hist = train(net, data, a.epochs, batch=a.batch, lr_max=a.lr, log=log)
+    del data
+    params = net.get_params().cpu().numpy()
+    a.out.parent.mkdir(parents=True, exist_ok=True)
+    np.save(a.out, params)
+    log(f"saved {a.out} ({params.size} parameters)")
+    res = evaluate(net, a.eval_problems, a.n, a.sigma, a.corr, a.eval_seed, log=log)
+    log(json.dumps({"loss_history": hist, "evaluation": res}))
+
+
+if __name__ == "__main__":
+    main()

@@ -52,9 +52,12 @@ class LinearOperator:
         return d
 
     def __del__(self):
+        # the destroy function was bound at construction: module globals may
+        # already be gone when this runs at interpreter exit
         h = getattr(self, "handle", None)
-        if h is not None and h.value and L._lib is not None:
-            L.lib().sil_operator_destroy(h)
+        fn = getattr(self, "_destroy", None)
+        if h is not None and h.value and fn is not None:
+            fn(h)
             self.handle = None
 
 
@@ -73,6 +76,7 @@ class MagnetostaticsOperator3D(LinearOperator):
                                                     ctypes.cast(ys, ctypes.c_void_p), ctypes.cast(zs, ctypes.c_void_p),
                                                     L.ptr(mu_nodal), float(gauge_penalty)))
         self.handle = h
+        self._destroy = L.lib().sil_operator_destroy
 
     @staticmethod
     def nodal_mu(shape, cell_mu, mu0, stream=None):
@@ -108,6 +112,7 @@ class Poisson2D(LinearOperator):
                                              ctypes.cast(xs, ctypes.c_void_p), ctypes.cast(ys, ctypes.c_void_p),
                                              L.ptr(kappa)))
         self.handle = h
+        self._destroy = L.lib().sil_operator_destroy
 
 
 @dataclass
