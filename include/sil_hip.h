@@ -285,19 +285,25 @@ int sil_unet_train_step(sil_unet net, int32_t batch, const float* input, const f
  * = mean of its two nodes).  Methods, chosen by the settings:
  *   JACOBI / IDENTITY: CG; precision FP64, or MIXED (FP32 directions and
  *   operator applications, FP64 iterate, residual, dots and scalars);
- *   LEARNED: flexible CG (truncation 1) preconditioned by `net`.
+ *   LEARNED: flexible CG (truncation 1) whose preconditioner is a two-level
+ *   cycle: damped Jacobi pre-smoothing, the network's correction of the
+ *   remaining residual scaled to its A-optimal length, damped Jacobi
+ *   post-smoothing (`net` must be built for batch >= B of n x n); a problem
+ *   whose FP64 residual norm has not fallen by 0.8 over five consecutive
+ *   replacement intervals continues with the Jacobi preconditioner (plain
+ *   CG is the fallback).
  * A problem converges only on an FP64 residual b - A x recomputed every
  * replace_interval iterations (even; 0 = 10 for CG, 4 for flexible CG).
  * results (host, B entries) report iterations and the FP64 relative
  * residual of every problem at exit; seconds is the host wall time of the
- * solve.  An optional record stores FP32 copies of r and x of all problems
+ * solve.  An optional record stores copies of r (FP32) and x (FP64) of all problems
  * before every graph of `every` iterations (the training data of the
  * learned preconditioner). */
 typedef struct sil_solve_record {
     int32_t every;       /* even, >= 2; overrides replace_interval */
     int32_t max_records;
-    float* r;            /* device [max_records][B][n][n] */
-    float* x;            /* device [max_records][B][n][n] */
+    float* r;            /* device [max_records][B][n][n], FP32 residuals */
+    double* x;           /* device [max_records][B][n][n], FP64 iterates */
     int32_t* count;      /* host: records written */
 } sil_solve_record;
 

@@ -109,54 +109,91 @@ GRAPH = [
 ]
 
 
-def forward(params, x):
-    """Network output [B,H,W] and the activations, x [B,H,W,2]."""
+def bf16(a):
+    """Round to the nearest bfloat16 (ties to even), returned as float64."""
+    a32 = np.ascontiguousarray(a, dtype=np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def conv32(x, w, b):
+    """conv() accumulated in FP32 (as the matrix cores do)."""
+    return conv(np.float32(x), np.float32(w), np.float32(b)).astype(np.float32).astype(np.float64)
+
+
+def conv_bwd32(x, w, dz):
+    dx, dw, db = conv_bwd(np.float32(x), np.float32(w), np.float32(dz))
+    return tuple(a.astype(np.float32).astype(np.float64) for a in (dx, dw, db))
+
+
+def forward(params, x, emulate=False):
+    """Network output [B,H,W] and the activations, x [B,H,W,2].
+
+    emulate=True computes as the GPU kernels do: FP32 input and first/last
+    layer weights, BF16 weights of the 32->32 layers, BF16 activations and
+    FP32 accumulation.  BF16 storage makes a network of this depth sensitive
+    to the last bit of each accumulation (one in ten activations of the
+    coarse levels rounds to the other neighbouring BF16 value when only the
+    accumulation order changes), so only an oracle that accumulates in FP32
+    too can be compared tightly with the GPU."""
+    q = bf16 if emulate else (lambda a: a)
+    qw = f32 if emulate else (lambda a: a)
+    cv = conv32 if emulate else conv
     win, bin_, layers, wout, bout = unpack(params)
-    act = {"x": np.asarray(x, dtype=np.float64)}
+    act = {"x": qw(np.asarray(x, dtype=np.float64))}
     for op, src, dst in GRAPH:
         if op == "in":
-            act[dst] = relu(conv(act[src], win, bin_))
+            act[dst] = q(relu(cv(act[src], qw(win), qw(bin_))))
         elif op == "pool":
-            act[dst] = pool(act[src])
+            act[dst] = q(qw(pool(act[src])))
         elif op == "up":
-            act[dst] = up(act[src[0]]) + act[src[1]]
+            act[dst] = q(up(act[src[0]]) + act[src[1]])
         elif op == "out":
-            act[dst] = conv(act[src], wout[:, None, :], np.array([bout]))[..., 0]
+            act[dst] = cv(act[src], qw(wout)[:, None, :], qw(np.array([bout])))[..., 0]
         else:
             w, b = layers[op - 1]
-            act[dst] = relu(conv(act[src], w, b))
+            act[dst] = q(relu(cv(act[src], q(w), qw(b))))
     return act["out"], act
 
 
-def backward(params, act, dout):
-    """Gradient of sum(dout * out) with respect to the parameter vector."""
+def backward(params, act, dout, emulate=False):
+    """Gradient of sum(dout * out) with respect to the parameter vector
+    (emulate=True: BF16 storage of the activation gradients, as on the GPU)."""
+    q = bf16 if emulate else (lambda a: a)
+    qw = f32 if emulate else (lambda a: a)
+    cb = conv_bwd32 if emulate else conv_bwd
     win, bin_, layers, wout, bout = unpack(params)
     g = np.zeros(N_PARAMS)
     grad = {k: np.zeros_like(v) for k, v in act.items() if k != "out"}
     for op, src, dst in reversed(GRAPH):
         if op == "out":
-            dx, dw, db = conv_bwd(act[src], wout[:, None, :], np.asarray(dout, dtype=np.float64)[..., None])
+            dx, dw, db = cb(act[src], qw(wout)[:, None, :], qw(np.asarray(dout, dtype=np.float64))[..., None])
             g[OUT_BASE: OUT_BASE + 9 * C] = dw[:, 0, :].ravel()
             g[OUT_BASE + 9 * C] = db[0]
-            grad[src] += dx
+            grad[src] = q(grad[src] + dx)
         elif op == "up":
-            grad[src[0]] += up_bwd(grad[dst])
-            grad[src[1]] += grad[dst]
+            grad[src[0]] = q(qw(grad[src[0]] + up_bwd(grad[dst])))
+            grad[src[1]] = q(grad[src[1]] + grad[dst])
         elif op == "pool":
-            grad[src] += pool_bwd(grad[dst])
+            grad[src] = q(qw(grad[src] + pool_bwd(grad[dst])))
         else:
             dz = grad[dst] * (act[dst] > 0)
             if op == "in":
-                dx, dw, db = conv_bwd(act[src], win, dz)
+                dx, dw, db = cb(act[src], qw(win), dz)
                 g[: 9 * C * 2] = dw.ravel()
                 g[9 * C * 2: N_IN] = db
             else:
                 w, _ = layers[op - 1]
-                dx, dw, db = conv_bwd(act[src], w, dz)
+                dx, dw, db = cb(act[src], q(w), dz)
                 ws, bs = layer_slices(op)
                 g[ws] = dw.ravel()
                 g[bs] = db
-            grad[src] += dx
+            grad[src] = q(grad[src] + dx)
     return g
 
 

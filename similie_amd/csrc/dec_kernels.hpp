@@ -24,7 +24,9 @@ struct FusedGeom {
     const double* metric; // diagonal [node][ndim] or null for the identity
 };
 
-enum FusedVariant { FUSED_DIRECT = 0, FUSED_MARCH = 1, FUSED_MARCH_DEEP = 2 };
+// FUSED_MARCH: x-march, guarded loads, one plane prefetched;
+// FUSED_MARCH_DEEP / _CLAMPED: clamped loads, two / one planes prefetched
+enum FusedVariant { FUSED_DIRECT = 0, FUSED_MARCH = 1, FUSED_MARCH_DEEP = 2, FUSED_MARCH_CLAMPED = 3, FUSED_MARCH_V0 = 4 };
 
 struct FusedTile {
     int variant;  // FusedVariant

@@ -4,8 +4,8 @@
 //   * Jacobi-preconditioned CG in FP64,
 //   * the same CG with FP32 directions, products and operator applications
 //     and FP64 iterate, residual, dots and scalars (residual replacement),
-//   * flexible CG (Notay's FCG, truncation 1) whose preconditioner is the
-//     learned U-Net, in either precision.
+//   * flexible CG (Notay's FCG, truncation 1) whose preconditioner is a
+//     two-level cycle around the learned U-Net, in either precision.
 // No reference counterpart: the reference's CG (Ginkgo, src/similie/solvers/
 // minimize_strong_formulation_residual.hpp:1028-1304) solves one system at a
 // time.  Every solver declares convergence only on a residual recomputed in
@@ -38,9 +38,17 @@ constexpr int NT = 256;
 constexpr int kParts = 4; // partial sums per block
 
 struct Scal {
-    double rz, pq, pq_old, pr, zq, rr, bb, alpha, beta, rho;
-    int done, iters, pad0, pad1;
+    double rz, pq, pq_old, pr, zq, rr, bb, alpha, beta, best, rho1, cnet;
+    int done, iters, stall, fallback;
 };
+
+// damping of the Jacobi smoothing around the network's correction
+constexpr double kOmega = 2.0 / 3.0;
+
+// flexible CG: a problem whose FP64 residual has not fallen by 0.8 (in norm)
+// over kStallChecks consecutive replacement intervals switches from the
+// network to the Jacobi preconditioner, i.e. continues as plain CG
+constexpr int kStallChecks = 5;
 
 template <class T>
 __device__ __forceinline__ T apply_at(const T* __restrict__ u, const T* __restrict__ wx, const T* __restrict__ wy,
@@ -261,34 +269,122 @@ __global__ __launch_bounds__(NT) void k_cg_update(int n, const T* __restrict__ p
 
 // ---------------------------------------------------------------- FCG ---
 
-// network input: channel 0 = r / rms(r), channel 1 = normalised log κ
-__global__ __launch_bounds__(NT) void k_fcg_net_input(int n, const double* __restrict__ r,
-                                                      const float* __restrict__ logk, const Scal* __restrict__ sc,
-                                                      float* __restrict__ xin)
+// Preconditioner of flexible CG, a two-level cycle around the network:
+//   s = ω D⁻¹ r                         (pre-smoothing)
+//   r₁ = r − A s                        (its residual)
+//   n = N(r₁ / rms(r₁), log κ)          (the network's correction)
+//   c = n·r₁ / n·A n                    (its A-optimal length)
+//   z = s + c n + ω D⁻¹ (r₁ − c A n)    (post-smoothing)
+// A problem that has fallen back to plain CG uses z = D⁻¹ r.
+
+// r₁ = r − A(ω D⁻¹ r) (FP32 copy); partial r₁ᵀr₁
+template <class T>
+__global__ __launch_bounds__(NT) void k_tl_pre(int n, const T* __restrict__ wx, const T* __restrict__ wy,
+                                               const double* __restrict__ r, const double* __restrict__ dinv,
+                                               float* __restrict__ r1, const Scal* __restrict__ sc,
+                                               double* __restrict__ part)
 {
+    if (sc[blockIdx.y].done) return;
     const size_t nn = (size_t)n * n, base = (size_t)blockIdx.y * nn;
-    const double rr = sc[blockIdx.y].rr;
-    const double inv = rr > 0.0 ? 1.0 / sqrt(rr / (double)nn) : 0.0;
+    const double* rb = r + base;
+    const double* db = dinv + base;
+    auto S = [&](size_t u) { return kOmega * db[u] * rb[u]; };
+    double v[1] = {0.0};
     NODE_LOOP
     {
-        xin[(base + t) * 2] = (float)(r[base + t] * inv);
+        const int j = (int)(t % n), i = (int)(t / n);
+        double o = 0.0;
+        if (interior(i, j, n)) {
+            const double sv = S(t);
+            const double se = i + 2 < n ? S(t + n) : 0.0, sw = i > 1 ? S(t - n) : 0.0;
+            const double sn = j + 2 < n ? S(t + 1) : 0.0, ss = j > 1 ? S(t - 1) : 0.0;
+            const T* ax = wx + base;
+            const T* ay = wy + base;
+            o = rb[t] - ((double)ax[t] * (sv - se) + (double)ax[t - n] * (sv - sw) + (double)ay[t] * (sv - sn) +
+                         (double)ay[t - 1] * (sv - ss));
+        }
+        r1[base + t] = (float)o;
+        v[0] += o * o;
+    }
+    block_partials<1>(v, part);
+}
+
+// network input: channel 0 = r₁ / rms(r₁), channel 1 = normalised log κ
+__global__ __launch_bounds__(NT) void k_tl_input(int n, const float* __restrict__ r1, const float* __restrict__ logk,
+                                                 const Scal* __restrict__ sc, float* __restrict__ xin)
+{
+    const size_t nn = (size_t)n * n, base = (size_t)blockIdx.y * nn;
+    const double rho = sc[blockIdx.y].rho1;
+    const float inv = rho > 0.0 ? (float)(1.0 / rho) : 0.f;
+    NODE_LOOP
+    {
+        xin[(base + t) * 2] = r1[base + t] * inv;
         xin[(base + t) * 2 + 1] = logk[base + t];
     }
 }
 
-// partial zᵀq_old
+// A n (n zero on the boundary ring); partials nᵀr₁, nᵀA n
 template <class T>
-__global__ __launch_bounds__(NT) void k_fcg_zq(int n, const float* __restrict__ z, const T* __restrict__ q_old,
-                                               const Scal* __restrict__ sc, double* __restrict__ part)
+__global__ __launch_bounds__(NT) void k_tl_an(int n, const T* __restrict__ wx, const T* __restrict__ wy,
+                                              const float* __restrict__ zn, const float* __restrict__ r1,
+                                              float* __restrict__ an, const Scal* __restrict__ sc,
+                                              double* __restrict__ part)
 {
     if (sc[blockIdx.y].done) return;
     const size_t nn = (size_t)n * n, base = (size_t)blockIdx.y * nn;
+    const float* nb = zn + base;
+    double v[2] = {0.0, 0.0};
+    NODE_LOOP
+    {
+        const int j = (int)(t % n), i = (int)(t / n);
+        float o = 0.f, nv = 0.f;
+        if (interior(i, j, n)) {
+            nv = nb[t];
+            const float ne = i + 2 < n ? nb[t + n] : 0.f, nw = i > 1 ? nb[t - n] : 0.f;
+            const float nnb = j + 2 < n ? nb[t + 1] : 0.f, ns = j > 1 ? nb[t - 1] : 0.f;
+            const T* ax = wx + base;
+            const T* ay = wy + base;
+            o = (float)((double)ax[t] * (nv - ne) + (double)ax[t - n] * (nv - nw) + (double)ay[t] * (nv - nnb) +
+                        (double)ay[t - 1] * (nv - ns));
+        }
+        an[base + t] = o;
+        v[0] += (double)nv * (double)r1[base + t];
+        v[1] += (double)nv * (double)o;
+    }
+    block_partials<2>(v, part);
+}
+
+// z = ω D⁻¹ r + c n + ω D⁻¹ (r₁ − c A n), or D⁻¹ r after the fallback;
+// partial zᵀq_old
+template <class T>
+__global__ __launch_bounds__(NT) void k_tl_post(int n, const double* __restrict__ r, const double* __restrict__ dinv,
+                                                const float* __restrict__ r1, const float* __restrict__ zn,
+                                                const float* __restrict__ an, const T* __restrict__ q_old,
+                                                float* __restrict__ z, const Scal* __restrict__ sc,
+                                                double* __restrict__ part)
+{
+    const Scal& s = sc[blockIdx.y];
+    if (s.done) return;
+    const double c = s.cnet;
+    const bool fb = s.fallback != 0;
+    const size_t nn = (size_t)n * n, base = (size_t)blockIdx.y * nn;
     double v[1] = {0.0};
-    NODE_LOOP v[0] += (double)z[base + t] * (double)q_old[base + t];
+    NODE_LOOP
+    {
+        const int j = (int)(t % n), i = (int)(t / n);
+        const size_t g = base + t;
+        double zv = 0.0;
+        if (interior(i, j, n))
+            zv = fb ? dinv[g] * r[g]
+                    : kOmega * dinv[g] * r[g] + c * (double)zn[g] +
+                          kOmega * dinv[g] * ((double)r1[g] - c * (double)an[g]);
+        z[g] = (float)zv;
+        v[0] += (double)(float)zv * (double)q_old[g];
+    }
     block_partials<1>(v, part);
 }
 
-// p = z − β p_old (z zero on the boundary ring), q = A p; partials pᵀq, pᵀr
+// p = z − β p_old, q = A p; partials pᵀq, pᵀr
 template <class T>
 __global__ __launch_bounds__(NT) void k_fcg_apply(int n, const T* __restrict__ wx, const T* __restrict__ wy,
                                                   const float* __restrict__ z, const T* __restrict__ p_old,
@@ -353,13 +449,16 @@ enum ScalPhase {
     PH_VERIFY,      // rr, bb of a recomputed residual, for every problem
     PH_CG_ALPHA,    // pq → α = rz/pq
     PH_CG_BETA,     // rz', rr → β = rz'/rz, iteration count, convergence on a recomputed residual
+    PH_TL_RHO,      // r₁ᵀr₁ → rms(r₁)
+    PH_TL_C,        // nᵀr₁, nᵀA n → length of the network's correction
     PH_FCG_BETA,    // zq → β = zq / pq_old (0 on the first iteration)
     PH_FCG_ALPHA,   // pq, pr → α = pr/pq
     PH_FCG_UPDATE,  // rr → iteration count, convergence on a recomputed residual
 };
 
 __global__ void k_pb_scalars(int phase, int nbx, const double* __restrict__ part, Scal* __restrict__ sc,
-                             double tol2, int max_iter, int verified, int* __restrict__ done_flags)
+                             double tol2, int max_iter, int verified, int* __restrict__ done_flags,
+                             double nn_total)
 {
     const int b = blockIdx.x;
     if (threadIdx.x != 0) return;
@@ -376,6 +475,9 @@ __global__ void k_pb_scalars(int phase, int nbx, const double* __restrict__ part
         s.beta = 0.0;
         s.pq_old = 0.0;
         s.iters = 0;
+        s.best = s.rr;
+        s.stall = 0;
+        s.fallback = 0;
         s.done = (s.rr <= tol2 * s.bb) ? 1 : 0;
         break;
     case PH_CG_ALPHA:
@@ -388,6 +490,12 @@ __global__ void k_pb_scalars(int phase, int nbx, const double* __restrict__ part
         s.rr = v[1];
         s.iters += 1;
         if ((verified && s.rr <= tol2 * s.bb) || s.iters >= max_iter || s.pq <= 0.0) s.done = 1;
+        break;
+    case PH_TL_RHO:
+        s.rho1 = sqrt(v[0] / (double)nn_total);
+        break;
+    case PH_TL_C:
+        s.cnet = v[1] > 0.0 ? v[0] / v[1] : 0.0;
         break;
     case PH_FCG_BETA:
         s.zq = v[0];
@@ -406,20 +514,29 @@ __global__ void k_pb_scalars(int phase, int nbx, const double* __restrict__ part
         s.rr = v[1];
         s.pq_old = s.pq;
         s.iters += 1;
+        if (verified) {
+            if (s.rr < 0.64 * s.best) {
+                s.best = s.rr;
+                s.stall = 0;
+            } else if (++s.stall >= kStallChecks) {
+                s.fallback = 1;
+            }
+        }
         if ((verified && s.rr <= tol2 * s.bb) || s.iters >= max_iter || s.pq <= 0.0) s.done = 1;
         break;
     }
     if (done_flags) done_flags[b] = s.done;
 }
 
-// FP32 snapshots of r and x (training data of the learned preconditioner)
+// snapshots of r (FP32) and x (FP64: errors x* − x of late iterates are
+// far below FP32 resolution of x) for the learned preconditioner's data
 __global__ __launch_bounds__(NT) void k_pb_record(size_t total, const double* __restrict__ r,
                                                   const double* __restrict__ x, float* __restrict__ rr,
-                                                  float* __restrict__ xr)
+                                                  double* __restrict__ xr)
 {
     for (size_t t = blockIdx.x * (size_t)NT + threadIdx.x; t < total; t += (size_t)gridDim.x * NT) {
         rr[t] = (float)r[t];
-        xr[t] = (float)x[t];
+        xr[t] = x[t];
     }
 }
 
@@ -442,6 +559,9 @@ struct sil_poisson2d_batch_s {
     float *p32[2] = {nullptr, nullptr}, *q32[2] = {nullptr, nullptr}, *z32 = nullptr;
     float* xin = nullptr;  // network input [B][n][n][2]
     float* zn = nullptr;   // network output [B][n][n]
+    float* r1 = nullptr;   // residual after pre-smoothing
+    float* an = nullptr;   // A · network output
+    float* zf = nullptr;   // preconditioned residual of flexible CG
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     std::vector<void*> allocations;
@@ -485,7 +605,7 @@ void fill(T* a, size_t n, T v, hipStream_t s)
 void scalars(const Batch* h, int phase, double tol2, int max_iter, int verified, hipStream_t s)
 {
     hipLaunchKernelGGL(sil::k_pb_scalars, dim3(h->B), dim3(64), 0, s, phase, h->nbx, h->part, h->sc, tol2, max_iter,
-                       verified, h->done);
+                       verified, h->done, (double)h->nn);
 }
 
 // one CG iteration; `parity` selects the direction buffer written
@@ -508,13 +628,19 @@ template <class T>
 int fcg_iteration(Batch* h, sil_unet net, T* const* p, T* const* q, const T* wx, const T* wy, const double* b,
                   double* x, int parity, bool replace, double tol2, int max_iter, hipStream_t s)
 {
-    hipLaunchKernelGGL(sil::k_fcg_net_input, grid_of(h), dim3(NT), 0, s, h->n, h->r, h->logk, h->sc, h->xin);
+    hipLaunchKernelGGL(sil::k_tl_pre<T>, grid_of(h), dim3(NT), 0, s, h->n, wx, wy, h->r, h->dinv, h->r1, h->sc,
+                       h->part);
+    scalars(h, sil::PH_TL_RHO, tol2, max_iter, 0, s);
+    hipLaunchKernelGGL(sil::k_tl_input, grid_of(h), dim3(NT), 0, s, h->n, h->r1, h->logk, h->sc, h->xin);
     const int rc = sil_unet_forward(net, h->B, h->xin, h->zn, s);
     if (rc != SIL_OK) return rc;
-    hipLaunchKernelGGL(sil::k_fcg_zq<T>, grid_of(h), dim3(NT), 0, s, h->n, h->zn, q[parity ^ 1], h->sc, h->part);
+    hipLaunchKernelGGL(sil::k_tl_an<T>, grid_of(h), dim3(NT), 0, s, h->n, wx, wy, h->zn, h->r1, h->an, h->sc, h->part);
+    scalars(h, sil::PH_TL_C, tol2, max_iter, 0, s);
+    hipLaunchKernelGGL(sil::k_tl_post<T>, grid_of(h), dim3(NT), 0, s, h->n, h->r, h->dinv, h->r1, h->zn, h->an,
+                       q[parity ^ 1], h->zf, h->sc, h->part);
     scalars(h, sil::PH_FCG_BETA, tol2, max_iter, 0, s);
-    hipLaunchKernelGGL(sil::k_fcg_apply<T>, grid_of(h), dim3(NT), 0, s, h->n, wx, wy, h->zn, p[parity ^ 1],
-                       p[parity], q[parity], h->r, h->sc, h->part);
+    hipLaunchKernelGGL(sil::k_fcg_apply<T>, grid_of(h), dim3(NT), 0, s, h->n, wx, wy, h->zf, p[parity ^ 1], p[parity],
+                       q[parity], h->r, h->sc, h->part);
     scalars(h, sil::PH_FCG_ALPHA, tol2, max_iter, 0, s);
     hipLaunchKernelGGL(sil::k_fcg_update<T>, grid_of(h), dim3(NT), 0, s, h->n, p[parity], q[parity], x, h->r,
                        h->sc, h->part);
@@ -654,7 +780,7 @@ int sil_poisson2d_batch_solve(sil_poisson2d_batch h, const double* b, double* x,
         A(h->p64[0], h->total), A(h->p64[1], h->total), A(h->q64[0], h->total), A(h->q64[1], h->total);
         if (!learned) A(h->z64, h->total);
     }
-    if (learned) A(h->xin, h->total * 2), A(h->zn, h->total);
+    if (learned) A(h->xin, h->total * 2), A(h->zn, h->total), A(h->r1, h->total), A(h->an, h->total), A(h->zf, h->total);
     if (rc != SIL_OK) return rc;
 
     SIL_CHECK_HIP(hipEventRecord(h->ev, us));
@@ -663,7 +789,7 @@ int sil_poisson2d_batch_solve(sil_poisson2d_batch h, const double* b, double* x,
     const auto t0 = std::chrono::steady_clock::now();
 
     hipLaunchKernelGGL(sil::k_pb_dinv, grid_of(h), dim3(NT), 0, s, h->n, h->wx, h->wy,
-                       st->preconditioner == SIL_PRECOND_JACOBI ? 1 : 0, h->dinv, h->dinv32);
+                       st->preconditioner == SIL_PRECOND_IDENTITY ? 0 : 1, h->dinv, h->dinv32);
     if (mixed) {
         fill(h->p32[0], h->total, 0.f, s), fill(h->p32[1], h->total, 0.f, s);
         fill(h->q32[0], h->total, 0.f, s), fill(h->q32[1], h->total, 0.f, s);
@@ -724,7 +850,7 @@ int sil_poisson2d_batch_solve(sil_poisson2d_batch h, const double* b, double* x,
         for (int i = 0; i < h->B; ++i) ndone += h->done_host[i] ? 1 : 0;
         if (record && records < record->max_records && (launched == 0 || ndone < h->B)) {
             float* rr = record->r + (size_t)records * h->total;
-            float* xr = record->x + (size_t)records * h->total;
+            double* xr = record->x + (size_t)records * h->total;
             hipLaunchKernelGGL(sil::k_pb_record, dim3(4096), dim3(NT), 0, s, h->total, h->r, x, rr, xr);
             ++records;
         }

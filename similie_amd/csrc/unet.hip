@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdint>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "mfma_layout.hpp"
@@ -41,6 +42,8 @@ constexpr int NT = 256;    // threads per workgroup
 
 using bf16 = __bf16;
 
+enum StageMode { ST_PLAIN = 0, ST_UPADD = 1, ST_POOL = 2, ST_MASK = 3 };
+
 struct alignas(16) Chunk8 {
     bf16 v[8];
 };
@@ -55,9 +58,67 @@ __device__ __forceinline__ Chunk8 zero_chunk()
 
 // ------------------------------------------------------ 3×3 32→32 conv ---
 
-// Y = act(conv(X ⊙ [M > 0]) + bias): forward (M null, bias, relu) or input
-// gradient (M = forward output of the layer, flipped weights, no bias/relu).
-__global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ M,
+// The input tile of a convolution is staged from one of four sources, so the
+// parameter-free layers of the network are never materialised:
+//   ST_PLAIN  X                              [B][H][W][32]
+//   ST_UPADD  X + up(X2)                     X the skip [B][H][W], X2 [B][H/2][W/2]
+//   ST_POOL   2x2 average of X               X [B][2H][2W]
+//   ST_MASK   X ⊙ [X2 > 0]                   input gradient: X = dY, X2 = forward output
+// Values are rounded to BF16 exactly as a stored tensor would be.
+template <int MODE>
+__device__ __forceinline__ Chunk8 stage_chunk(const bf16* __restrict__ X, const bf16* __restrict__ X2, int b, int gy,
+                                              int gx, int part, int H, int W)
+{
+    Chunk8 v;
+    if (MODE == ST_POOL) {
+        const int W2 = 2 * W;
+        const size_t base = (((size_t)b * 2 * H + 2 * gy) * W2 + 2 * gx) * C + part * 8;
+        const Chunk8 p00 = *reinterpret_cast<const Chunk8*>(&X[base]);
+        const Chunk8 p01 = *reinterpret_cast<const Chunk8*>(&X[base + C]);
+        const Chunk8 p10 = *reinterpret_cast<const Chunk8*>(&X[base + (size_t)W2 * C]);
+        const Chunk8 p11 = *reinterpret_cast<const Chunk8*>(&X[base + (size_t)W2 * C + C]);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float s = from_bf16(p00.v[i]) + from_bf16(p01.v[i]) + from_bf16(p10.v[i]) + from_bf16(p11.v[i]);
+            v.v[i] = to_bf16(0.25f * s);
+        }
+        return v;
+    }
+    const size_t off = (((size_t)b * H + gy) * W + gx) * C + part * 8;
+    v = *reinterpret_cast<const Chunk8*>(&X[off]);
+    if (MODE == ST_UPADD) {
+        const size_t q = (((size_t)b * (H / 2) + gy / 2) * (W / 2) + gx / 2) * C + part * 8;
+        const Chunk8 c = *reinterpret_cast<const Chunk8*>(&X2[q]);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v.v[i] = to_bf16(from_bf16(c.v[i]) + from_bf16(v.v[i]));
+    } else if (MODE == ST_MASK) {
+        const Chunk8 m = *reinterpret_cast<const Chunk8*>(&X2[off]);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (!(from_bf16(m.v[i]) > 0.f)) v.v[i] = to_bf16(0.f);
+    }
+    return v;
+}
+
+// the (TH+2)×(TW+2) input tile with origin (oy0−1, ox0−1), zero outside
+template <int MODE>
+__device__ __forceinline__ void stage_tile(bf16* __restrict__ sX, const bf16* __restrict__ X,
+                                           const bf16* __restrict__ X2, int b, int oy0, int ox0, int H, int W)
+{
+    for (int c = threadIdx.x; c < (TH + 2) * (TW + 2) * 4; c += NT) {
+        const int pix = c >> 2, part = c & 3;
+        const int ty = pix / (TW + 2), tx = pix % (TW + 2);
+        const int gy = oy0 - 1 + ty, gx = ox0 - 1 + tx;
+        Chunk8 v = zero_chunk();
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W) v = stage_chunk<MODE>(X, X2, b, gy, gx, part, H, W);
+        *reinterpret_cast<Chunk8*>(&sX[pix * CP + part * 8]) = v;
+    }
+}
+
+// Y = act(conv(staged input) + bias) on H×W images: the forward pass, or
+// (ST_MASK, flipped weights, no bias, no ReLU) the input gradient.
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ X2,
                                                const bf16* __restrict__ Wt, const float* __restrict__ bias,
                                                bf16* __restrict__ Y, int H, int W, int relu)
 {
@@ -72,23 +133,7 @@ __global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const
         const int row = c >> 2, part = c & 3; // row = tap*32 + co
         *reinterpret_cast<Chunk8*>(&sW[row * CP + part * 8]) = *reinterpret_cast<const Chunk8*>(&Wt[row * C + part * 8]);
     }
-    for (int c = tid; c < (TH + 2) * (TW + 2) * 4; c += NT) {
-        const int pix = c >> 2, part = c & 3;
-        const int ty = pix / (TW + 2), tx = pix % (TW + 2);
-        const int gy = oy0 - 1 + ty, gx = ox0 - 1 + tx;
-        Chunk8 v = zero_chunk();
-        if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
-            const size_t off = img + ((size_t)gy * W + gx) * C + part * 8;
-            v = *reinterpret_cast<const Chunk8*>(&X[off]);
-            if (M) {
-                const Chunk8 m = *reinterpret_cast<const Chunk8*>(&M[off]);
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    if (!(from_bf16(m.v[i]) > 0.f)) v.v[i] = to_bf16(0.f);
-            }
-        }
-        *reinterpret_cast<Chunk8*>(&sX[pix * CP + part * 8]) = v;
-    }
+    stage_tile<MODE>(sX, X, X2, b, oy0, ox0, H, W);
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6;
@@ -132,9 +177,11 @@ __global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const
 // weight gradient: dW[tap][co][ci] += Σ_q dZ[q][co] · X[q + off(tap)][ci],
 // dZ = dY ⊙ [Yfwd > 0]; db[co] += Σ_q dZ[q][co].  One partial per workgroup
 // (grid-stride over tiles), reduced later in a fixed order.
-__global__ __launch_bounds__(NT) void k_conv33_wgrad(const bf16* __restrict__ X, const bf16* __restrict__ dY,
-                                                     const bf16* __restrict__ Yf, float* __restrict__ part_w,
-                                                     float* __restrict__ part_b, int B, int H, int W)
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_conv33_wgrad(const bf16* __restrict__ X, const bf16* __restrict__ X2,
+                                                     const bf16* __restrict__ dY, const bf16* __restrict__ Yf,
+                                                     float* __restrict__ part_w, float* __restrict__ part_b, int B,
+                                                     int H, int W)
 {
     __shared__ __attribute__((aligned(16))) bf16 sX[(TH + 2) * (TW + 2) * CP];
     __shared__ __attribute__((aligned(16))) bf16 sZ[C * (TH * TW + 8)]; // transposed dZ: [co][pixel]
@@ -155,15 +202,7 @@ __global__ __launch_bounds__(NT) void k_conv33_wgrad(const bf16* __restrict__ X,
         const int oy0 = by * TH, ox0 = bx * TW;
         const size_t img = (size_t)b * H * W * C;
         __syncthreads();
-        for (int c = tid; c < (TH + 2) * (TW + 2) * 4; c += NT) {
-            const int pix = c >> 2, part = c & 3;
-            const int ty = pix / (TW + 2), tx = pix % (TW + 2);
-            const int gy = oy0 - 1 + ty, gx = ox0 - 1 + tx;
-            Chunk8 v = zero_chunk();
-            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-                v = *reinterpret_cast<const Chunk8*>(&X[img + ((size_t)gy * W + gx) * C + part * 8]);
-            *reinterpret_cast<Chunk8*>(&sX[pix * CP + part * 8]) = v;
-        }
+        stage_tile<MODE>(sX, X, X2, b, oy0, ox0, H, W);
         for (int c = tid; c < TH * TW * 4; c += NT) {
             const int pix = c >> 2, part = c & 3;
             const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
@@ -233,40 +272,48 @@ __global__ void k_reduce_parts(const float* __restrict__ part, int parts, int n,
 
 // -------------------------------------------------- input / output convs ---
 
-// a0 = relu(conv_in(x)): x FP32 [B][H][W][2], weights [tap][co][ci=2];
-// one thread per (pixel, group of 8 output channels)
-__global__ void k_conv_in(const float* __restrict__ x, const float* __restrict__ w, const float* __restrict__ bias,
-                          bf16* __restrict__ y, int B, int H, int W)
+// a0 = relu(conv_in(x)): x FP32 [B][H][W][2], weights [tap][co][ci=2].
+// A workgroup stages a (TH+2)×(TW+2) tile of x and writes the 32 channels of
+// one pixel per thread (four 16-byte stores).
+__global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, const float* __restrict__ w,
+                                                const float* __restrict__ bias, bf16* __restrict__ y, int H, int W)
 {
     __shared__ float sw[9 * C * 2 + C];
-    for (int i = threadIdx.x; i < 9 * C * 2; i += blockDim.x) sw[i] = w[i];
-    for (int i = threadIdx.x; i < C; i += blockDim.x) sw[9 * C * 2 + i] = bias[i];
+    __shared__ float sx[(TH + 2) * (TW + 2) * 2];
+    const int b = blockIdx.z, oy0 = blockIdx.y * TH, ox0 = blockIdx.x * TW;
+    for (int i = threadIdx.x; i < 9 * C * 2; i += NT) sw[i] = w[i];
+    for (int i = threadIdx.x; i < C; i += NT) sw[9 * C * 2 + i] = bias[i];
+    for (int i = threadIdx.x; i < (TH + 2) * (TW + 2); i += NT) {
+        const int gy = oy0 - 1 + i / (TW + 2), gx = ox0 - 1 + i % (TW + 2);
+        const bool ok = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        const size_t q = ((size_t)b * H + gy) * W + gx;
+        sx[i * 2] = ok ? x[q * 2] : 0.f;
+        sx[i * 2 + 1] = ok ? x[q * 2 + 1] : 0.f;
+    }
     __syncthreads();
-    const size_t n = (size_t)B * H * W * 4;
-    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
-        const int grp = (int)(idx & 3);
-        const size_t p = idx >> 2;
-        const int xx = (int)(p % W), yy = (int)((p / W) % H);
-        const size_t b = p / ((size_t)W * H);
-        float acc[8];
+    const int ty = threadIdx.x / TW, tx = threadIdx.x % TW;
+    const int gy = oy0 + ty, gx = ox0 + tx;
+    if (gy >= H || gx >= W) return;
+    float in[18];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc[c] = sw[9 * C * 2 + grp * 8 + c];
+    for (int t = 0; t < 9; ++t) {
+        const int i = (ty + t / 3) * (TW + 2) + tx + t % 3;
+        in[2 * t] = sx[i * 2];
+        in[2 * t + 1] = sx[i * 2 + 1];
+    }
+    bf16* out = y + (((size_t)b * H + gy) * W + gx) * C;
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int gy = yy + t / 3 - 1, gx = xx + t % 3 - 1;
-            if (gy < 0 || gy >= H || gx < 0 || gx >= W) continue;
-            const size_t q = (b * H + gy) * W + gx;
-            const float i0 = x[q * 2], i1 = x[q * 2 + 1];
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int co = grp * 8 + c;
-                acc[c] += sw[(t * C + co) * 2] * i0 + sw[(t * C + co) * 2 + 1] * i1;
-            }
-        }
+    for (int part = 0; part < 4; ++part) {
         Chunk8 o;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) o.v[c] = to_bf16(fmaxf(acc[c], 0.f));
-        *reinterpret_cast<Chunk8*>(&y[p * C + grp * 8]) = o;
+        for (int c = 0; c < 8; ++c) {
+            const int co = part * 8 + c;
+            float a = sw[9 * C * 2 + co];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) a += sw[(t * C + co) * 2] * in[2 * t] + sw[(t * C + co) * 2 + 1] * in[2 * t + 1];
+            o.v[c] = to_bf16(fmaxf(a, 0.f));
+        }
+        *reinterpret_cast<Chunk8*>(&out[part * 8]) = o;
     }
 }
 
@@ -309,31 +356,34 @@ __global__ void k_conv_in_wgrad(const float* __restrict__ x, const bf16* __restr
     }
 }
 
-// out = conv_out(c0): 32 → 1, weights [tap][ci], bias scalar
-__global__ void k_conv_out(const bf16* __restrict__ x, const float* __restrict__ w, const float* __restrict__ bias,
-                           float* __restrict__ y, int B, int H, int W)
+// out = conv_out(c0): 32 → 1, weights [tap][ci], bias scalar.  A workgroup
+// stages the c0 tile in LDS (as the 32→32 convolutions do) and each thread
+// forms one output pixel from 36 sixteen-byte LDS reads.
+__global__ __launch_bounds__(NT) void k_conv_out(const bf16* __restrict__ x, const float* __restrict__ w,
+                                                 const float* __restrict__ bias, float* __restrict__ y, int H, int W)
 {
+    __shared__ __attribute__((aligned(16))) bf16 sX[(TH + 2) * (TW + 2) * CP];
     __shared__ float sw[9 * C + 1];
-    for (int i = threadIdx.x; i < 9 * C; i += blockDim.x) sw[i] = w[i];
+    const int b = blockIdx.z, oy0 = blockIdx.y * TH, ox0 = blockIdx.x * TW;
+    for (int i = threadIdx.x; i < 9 * C; i += NT) sw[i] = w[i];
     if (threadIdx.x == 0) sw[9 * C] = bias[0];
+    stage_tile<ST_PLAIN>(sX, x, nullptr, b, oy0, ox0, H, W);
     __syncthreads();
-    const size_t n = (size_t)B * H * W;
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-        const int xx = p % W, yy = (p / W) % H;
-        const size_t b = p / ((size_t)W * H);
-        float s = sw[9 * C];
-        for (int t = 0; t < 9; ++t) {
-            const int gy = yy + t / 3 - 1, gx = xx + t % 3 - 1;
-            if (gy < 0 || gy >= H || gx < 0 || gx >= W) continue;
-            const Chunk8* px = reinterpret_cast<const Chunk8*>(&x[((b * H + gy) * W + gx) * C]);
-            for (int part = 0; part < 4; ++part) {
-                const Chunk8 v = px[part];
+    const int ty = threadIdx.x / TW, tx = threadIdx.x % TW;
+    const int gy = oy0 + ty, gx = ox0 + tx;
+    if (gy >= H || gx >= W) return;
+    float a = sw[9 * C];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) s += sw[t * C + part * 8 + i] * from_bf16(v.v[i]);
-            }
+    for (int t = 0; t < 9; ++t) {
+        const bf16* px = &sX[((ty + t / 3) * (TW + 2) + tx + t % 3) * CP];
+#pragma unroll
+        for (int part = 0; part < 4; ++part) {
+            const Chunk8 v = *reinterpret_cast<const Chunk8*>(&px[part * 8]);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) a += sw[t * C + part * 8 + i] * from_bf16(v.v[i]);
         }
-        y[p] = s;
     }
+    y[((size_t)b * H + gy) * W + gx] = a;
 }
 
 // backward of conv_out: dX[q][ci] = Σ_t dout[q − off(t)] · w[t][ci] (BF16),
@@ -386,71 +436,45 @@ __global__ void k_conv_out_bwd(const bf16* __restrict__ x, const float* __restri
 
 // ----------------------------------------------------- pooling, up-add ---
 
-__global__ void k_pool(const bf16* __restrict__ x, bf16* __restrict__ y, int B, int H, int W)
-{
-    const int h = H / 2, w = W / 2;
-    const size_t n = (size_t)B * h * w * C;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = i % C;
-        const size_t p = i / C;
-        const int xx = p % w, yy = (p / w) % h;
-        const size_t b = p / ((size_t)w * h);
-        const size_t base = ((b * H + 2 * yy) * W + 2 * xx) * C + c;
-        const float s = from_bf16(x[base]) + from_bf16(x[base + C]) + from_bf16(x[base + (size_t)W * C]) +
-                        from_bf16(x[base + (size_t)W * C + C]);
-        y[i] = to_bf16(0.25f * s);
-    }
-}
-
-// dX (fine) += dY (coarse)/4 on each of the 4 children (accumulate into dx)
+// dX (fine) += dY (coarse)/4 on each of its 4 children; 8 channels per thread
 __global__ void k_pool_bwd(const bf16* __restrict__ dy, bf16* __restrict__ dx, int B, int H, int W)
 {
-    const size_t n = (size_t)B * H * W * C;
+    const size_t n = (size_t)B * H * W * 4;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = i % C;
-        const size_t p = i / C;
-        const int xx = p % W, yy = (p / W) % H;
+        const int part = (int)(i & 3);
+        const size_t p = i >> 2;
+        const int xx = (int)(p % W), yy = (int)((p / W) % H);
         const size_t b = p / ((size_t)W * H);
-        const size_t q = ((b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c;
-        dx[i] = to_bf16(from_bf16(dx[i]) + 0.25f * from_bf16(dy[q]));
+        const size_t q = ((b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + part * 8;
+        Chunk8 d = *reinterpret_cast<Chunk8*>(&dx[p * C + part * 8]);
+        const Chunk8 c = *reinterpret_cast<const Chunk8*>(&dy[q]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) d.v[k] = to_bf16(from_bf16(d.v[k]) + 0.25f * from_bf16(c.v[k]));
+        *reinterpret_cast<Chunk8*>(&dx[p * C + part * 8]) = d;
     }
 }
 
-// y (fine) = up(xc) + skip
-__global__ void k_upadd(const bf16* __restrict__ xc, const bf16* __restrict__ skip, bf16* __restrict__ y, int B, int H,
-                        int W)
-{
-    const size_t n = (size_t)B * H * W * C;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = i % C;
-        const size_t p = i / C;
-        const int xx = p % W, yy = (p / W) % H;
-        const size_t b = p / ((size_t)W * H);
-        const size_t q = ((b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c;
-        y[i] = to_bf16(from_bf16(xc[q]) + from_bf16(skip[i]));
-    }
-}
-
-// d xc (coarse) = Σ of the 4 children of dY (fine)
+// d xc (coarse) = Σ of the 4 children of dY (fine); 8 channels per thread
 __global__ void k_up_bwd(const bf16* __restrict__ dy, bf16* __restrict__ dxc, int B, int h, int w)
 {
     const int H = 2 * h, W = 2 * w;
-    const size_t n = (size_t)B * h * w * C;
+    const size_t n = (size_t)B * h * w * 4;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = i % C;
-        const size_t p = i / C;
-        const int xx = p % w, yy = (p / w) % h;
+        const int part = (int)(i & 3);
+        const size_t p = i >> 2;
+        const int xx = (int)(p % w), yy = (int)((p / w) % h);
         const size_t b = p / ((size_t)w * h);
-        const size_t base = ((b * H + 2 * yy) * W + 2 * xx) * C + c;
-        const float s = from_bf16(dy[base]) + from_bf16(dy[base + C]) + from_bf16(dy[base + (size_t)W * C]) +
-                        from_bf16(dy[base + (size_t)W * C + C]);
-        dxc[i] = to_bf16(s);
+        const size_t base = ((b * H + 2 * yy) * W + 2 * xx) * C + part * 8;
+        const Chunk8 a0 = *reinterpret_cast<const Chunk8*>(&dy[base]);
+        const Chunk8 a1 = *reinterpret_cast<const Chunk8*>(&dy[base + C]);
+        const Chunk8 a2 = *reinterpret_cast<const Chunk8*>(&dy[base + (size_t)W * C]);
+        const Chunk8 a3 = *reinterpret_cast<const Chunk8*>(&dy[base + (size_t)W * C + C]);
+        Chunk8 o;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            o.v[k] = to_bf16(from_bf16(a0.v[k]) + from_bf16(a1.v[k]) + from_bf16(a2.v[k]) + from_bf16(a3.v[k]));
+        *reinterpret_cast<Chunk8*>(&dxc[p * C + part * 8]) = o;
     }
-}
-
-__global__ void k_copy_bf16(const bf16* __restrict__ x, bf16* __restrict__ y, size_t n)
-{
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = x[i];
 }
 
 // ------------------------------------------------------------- loss ---
@@ -578,21 +602,43 @@ inline unsigned grid1(size_t n, int block = 256, size_t cap = 8192)
 
 namespace unet {
 
-int conv33(const bf16* X, const bf16* M, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W, bool relu,
-           hipStream_t s)
+namespace {
+template <class F>
+int by_mode(int mode, F&& f)
 {
-    hipLaunchKernelGGL(k_conv33, dim3((W + TW - 1) / TW, (H + TH - 1) / TH, B), dim3(NT), 0, s, X, M, Wt, bias, Y, H,
-                       W, relu ? 1 : 0);
-    SIL_LAUNCH_CHECK();
-    return SIL_OK;
+    switch (mode) {
+    case ST_PLAIN: return f(std::integral_constant<int, ST_PLAIN>());
+    case ST_UPADD: return f(std::integral_constant<int, ST_UPADD>());
+    case ST_POOL: return f(std::integral_constant<int, ST_POOL>());
+    case ST_MASK: return f(std::integral_constant<int, ST_MASK>());
+    default: set_error("unet: unknown staging mode %d", mode); return SIL_ERR_INVALID;
+    }
+}
+}  // namespace
+
+int conv33(int mode, const bf16* X, const bf16* X2, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W,
+           bool relu, hipStream_t s)
+{
+    return by_mode(mode, [&](auto m) -> int {
+        hipLaunchKernelGGL(k_conv33<decltype(m)::value>, dim3((W + TW - 1) / TW, (H + TH - 1) / TH, B), dim3(NT), 0,
+                           s, X, X2, Wt, bias, Y, H, W, relu ? 1 : 0);
+        SIL_LAUNCH_CHECK();
+        return (int)SIL_OK;
+    });
 }
 
-int conv33_wgrad(const bf16* X, const bf16* dY, const bf16* Yf, float* part_w, float* part_b, int parts, float* dW,
-                 float* db, int B, int H, int W, hipStream_t s)
+int conv33_wgrad(int mode, const bf16* X, const bf16* X2, const bf16* dY, const bf16* Yf, float* part_w,
+                 float* part_b, int parts, float* dW, float* db, int B, int H, int W, hipStream_t s)
 {
     const int ntiles = ((W + TW - 1) / TW) * ((H + TH - 1) / TH) * B;
     const int grid = ntiles < parts ? ntiles : parts;
-    hipLaunchKernelGGL(k_conv33_wgrad, dim3(grid), dim3(NT), 0, s, X, dY, Yf, part_w, part_b, B, H, W);
+    const int rc = by_mode(mode, [&](auto m) -> int {
+        hipLaunchKernelGGL(k_conv33_wgrad<decltype(m)::value>, dim3(grid), dim3(NT), 0, s, X, X2, dY, Yf, part_w,
+                           part_b, B, H, W);
+        SIL_LAUNCH_CHECK();
+        return (int)SIL_OK;
+    });
+    if (rc != SIL_OK) return rc;
     hipLaunchKernelGGL(k_reduce_parts, dim3(36), dim3(256), 0, s, part_w, grid, 9 * C * C, dW);
     hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(32), 0, s, part_b, grid, C, db);
     SIL_LAUNCH_CHECK();
@@ -601,7 +647,7 @@ int conv33_wgrad(const bf16* X, const bf16* dY, const bf16* Yf, float* part_w, f
 
 int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_conv_in, dim3(grid1((size_t)B * H * W * 4)), dim3(256), 0, s, x, w, bias, y, B, H, W);
+    hipLaunchKernelGGL(k_conv_in, dim3((W + TW - 1) / TW, (H + TH - 1) / TH, B), dim3(NT), 0, s, x, w, bias, y, H, W);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
@@ -617,7 +663,7 @@ int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, i
 
 int conv_out(const bf16* x, const float* w, const float* bias, float* y, int B, int H, int W, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_conv_out, dim3(grid1((size_t)B * H * W)), dim3(256), 0, s, x, w, bias, y, B, H, W);
+    hipLaunchKernelGGL(k_conv_out, dim3((W + TW - 1) / TW, (H + TH - 1) / TH, B), dim3(NT), 0, s, x, w, bias, y, H, W);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
@@ -631,37 +677,16 @@ int conv_out_bwd(const bf16* x, const float* dout, const float* w, bf16* dx, flo
     return SIL_OK;
 }
 
-int pool(const bf16* x, bf16* y, int B, int H, int W, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_pool, dim3(grid1((size_t)B * H * W * C / 4)), dim3(256), 0, s, x, y, B, H, W);
-    SIL_LAUNCH_CHECK();
-    return SIL_OK;
-}
-
 int pool_bwd_accumulate(const bf16* dy, bf16* dx, int B, int H, int W, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pool_bwd, dim3(grid1((size_t)B * H * W * C)), dim3(256), 0, s, dy, dx, B, H, W);
-    SIL_LAUNCH_CHECK();
-    return SIL_OK;
-}
-
-int upadd(const bf16* xc, const bf16* skip, bf16* y, int B, int H, int W, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_upadd, dim3(grid1((size_t)B * H * W * C)), dim3(256), 0, s, xc, skip, y, B, H, W);
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid1((size_t)B * H * W * 4)), dim3(256), 0, s, dy, dx, B, H, W);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
 
 int up_bwd(const bf16* dy, bf16* dxc, int B, int h, int w, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_up_bwd, dim3(grid1((size_t)B * h * w * C)), dim3(256), 0, s, dy, dxc, B, h, w);
-    SIL_LAUNCH_CHECK();
-    return SIL_OK;
-}
-
-int copy(const bf16* x, bf16* y, size_t n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_copy_bf16, dim3(grid1(n)), dim3(256), 0, s, x, y, n);
+    hipLaunchKernelGGL(k_up_bwd, dim3(grid1((size_t)B * h * w * 4)), dim3(256), 0, s, dy, dxc, B, h, w);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }

@@ -19,13 +19,17 @@ constexpr int kChannels = 32;
 constexpr int kTileH = 8;
 constexpr int kTileW = 32;
 
-// Y = act(conv3x3(X ⊙ [M > 0]) + bias), 32 → 32 channels, zero padding.
-int conv33(const bf16* X, const bf16* M, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W, bool relu,
-           hipStream_t s);
-// dW[tap][co][ci], db[co] of conv33 from its input X, output gradient dY and
-// forward output Yf (ReLU mask); part_w holds parts·9·32·32 floats, part_b parts·32
-int conv33_wgrad(const bf16* X, const bf16* dY, const bf16* Yf, float* part_w, float* part_b, int parts, float* dW,
-                 float* db, int B, int H, int W, hipStream_t s);
+// Input staging of a 3x3 convolution: X itself, X + nearest-upsampled X2
+// (skip + coarse), the 2x2 average of X (X twice the size), or X ⊙ [X2 > 0]
+enum StageModeHost { STAGE_PLAIN = 0, STAGE_UPADD = 1, STAGE_POOL = 2, STAGE_MASK = 3 };
+
+// Y = act(conv3x3(staged input) + bias), 32 → 32 channels on H×W images, zero padding
+int conv33(int mode, const bf16* X, const bf16* X2, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W,
+           bool relu, hipStream_t s);
+// dW[tap][co][ci], db[co] of conv33 from its staged input, output gradient dY
+// and forward output Yf (ReLU mask); part_w holds parts·9·32·32 floats, part_b parts·32
+int conv33_wgrad(int mode, const bf16* X, const bf16* X2, const bf16* dY, const bf16* Yf, float* part_w,
+                 float* part_b, int parts, float* dW, float* db, int B, int H, int W, hipStream_t s);
 // 2 → 32 input convolution (FP32 input) and its weight gradient (dWb = [dW(9·32·2) | db(32)])
 int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s);
 int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, int parts, float* dWb, int B, int H,
@@ -34,13 +38,10 @@ int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, i
 int conv_out(const bf16* x, const float* w, const float* bias, float* y, int B, int H, int W, hipStream_t s);
 int conv_out_bwd(const bf16* x, const float* dout, const float* w, bf16* dx, float* part, int parts, float* dWb, int B,
                  int H, int W, hipStream_t s);
-// 2×2 average pooling of an H×W tensor; its backward accumulates into dx
-int pool(const bf16* x, bf16* y, int B, int H, int W, hipStream_t s);
+// backward of the 2x2 average pooling of an H×W tensor (accumulates into dx)
 int pool_bwd_accumulate(const bf16* dy, bf16* dx, int B, int H, int W, hipStream_t s);
-// y (H×W) = nearest-upsampled xc (H/2×W/2) + skip; backward to the coarse input
-int upadd(const bf16* xc, const bf16* skip, bf16* y, int B, int H, int W, hipStream_t s);
+// backward of nearest upsampling to the coarse h×w tensor
 int up_bwd(const bf16* dy, bf16* dxc, int B, int h, int w, hipStream_t s);
-int copy(const bf16* x, bf16* y, std::size_t n, hipStream_t s);
 // direction loss 1 − cos²_A(e, N) per image and its gradient dN (batch mean);
 // r is read with element stride rs; part holds B·parts·3 floats
 int cos_loss(const float* N, const float* r, int rs, const float* e, const float* wx, const float* wy, float* AN,

@@ -43,10 +43,11 @@ struct sil_unet_s {
     bf16* wf = nullptr; // kLayers × 9·32·32 forward weights
     bf16* wb = nullptr; // kLayers × 9·32·32 input-gradient weights
     // activations
-    bf16 *a0 = nullptr, *e0 = nullptr, *u0 = nullptr, *c0 = nullptr;
-    bf16 *p1 = nullptr, *a1 = nullptr, *e1 = nullptr, *u1 = nullptr, *c1 = nullptr, *f1 = nullptr;
-    bf16 *p2 = nullptr, *a2 = nullptr, *e2 = nullptr, *u2 = nullptr, *c2 = nullptr, *f2 = nullptr;
-    bf16 *p3 = nullptr, *a3 = nullptr, *b3 = nullptr;
+    // (pooled and upsampled-plus-skip inputs are formed while staging)
+    bf16 *a0 = nullptr, *e0 = nullptr, *c0 = nullptr;
+    bf16 *a1 = nullptr, *e1 = nullptr, *c1 = nullptr, *f1 = nullptr;
+    bf16 *a2 = nullptr, *e2 = nullptr, *c2 = nullptr, *f2 = nullptr;
+    bf16 *a3 = nullptr, *b3 = nullptr;
     // gradient buffers
     bf16 *g0a = nullptr, *g0b = nullptr, *g1a = nullptr, *g1b = nullptr, *g2a = nullptr, *g2b = nullptr,
          *g3a = nullptr, *g3b = nullptr;
@@ -101,67 +102,64 @@ const float* BIAS(const sil_unet_s* u, int l) { return u->params + layer_off(l) 
 int forward(sil_unet_s* u, int B, const float* x, float* out, hipStream_t s)
 {
     namespace un = sil::unet;
+    constexpr int P = un::STAGE_PLAIN, UP = un::STAGE_UPADD, PO = un::STAGE_POOL;
     const int *H = u->Hl, *W = u->Wl;
     UNET_TRY(un::conv_in(x, u->params, u->params + 9 * C * 2, u->a0, B, H[0], W[0], s));
-    UNET_TRY(un::conv33(u->a0, nullptr, WF(u, 1), BIAS(u, 1), u->e0, B, H[0], W[0], true, s));
-    UNET_TRY(un::pool(u->e0, u->p1, B, H[0], W[0], s));
-    UNET_TRY(un::conv33(u->p1, nullptr, WF(u, 2), BIAS(u, 2), u->a1, B, H[1], W[1], true, s));
-    UNET_TRY(un::conv33(u->a1, nullptr, WF(u, 3), BIAS(u, 3), u->e1, B, H[1], W[1], true, s));
-    UNET_TRY(un::pool(u->e1, u->p2, B, H[1], W[1], s));
-    UNET_TRY(un::conv33(u->p2, nullptr, WF(u, 4), BIAS(u, 4), u->a2, B, H[2], W[2], true, s));
-    UNET_TRY(un::conv33(u->a2, nullptr, WF(u, 5), BIAS(u, 5), u->e2, B, H[2], W[2], true, s));
-    UNET_TRY(un::pool(u->e2, u->p3, B, H[2], W[2], s));
-    UNET_TRY(un::conv33(u->p3, nullptr, WF(u, 6), BIAS(u, 6), u->a3, B, H[3], W[3], true, s));
-    UNET_TRY(un::conv33(u->a3, nullptr, WF(u, 7), BIAS(u, 7), u->b3, B, H[3], W[3], true, s));
-    UNET_TRY(un::upadd(u->b3, u->e2, u->u2, B, H[2], W[2], s));
-    UNET_TRY(un::conv33(u->u2, nullptr, WF(u, 8), BIAS(u, 8), u->c2, B, H[2], W[2], true, s));
-    UNET_TRY(un::conv33(u->c2, nullptr, WF(u, 9), BIAS(u, 9), u->f2, B, H[2], W[2], true, s));
-    UNET_TRY(un::upadd(u->f2, u->e1, u->u1, B, H[1], W[1], s));
-    UNET_TRY(un::conv33(u->u1, nullptr, WF(u, 10), BIAS(u, 10), u->c1, B, H[1], W[1], true, s));
-    UNET_TRY(un::conv33(u->c1, nullptr, WF(u, 11), BIAS(u, 11), u->f1, B, H[1], W[1], true, s));
-    UNET_TRY(un::upadd(u->f1, u->e0, u->u0, B, H[0], W[0], s));
-    UNET_TRY(un::conv33(u->u0, nullptr, WF(u, 12), BIAS(u, 12), u->c0, B, H[0], W[0], true, s));
+    UNET_TRY(un::conv33(P, u->a0, nullptr, WF(u, 1), BIAS(u, 1), u->e0, B, H[0], W[0], true, s));
+    UNET_TRY(un::conv33(PO, u->e0, nullptr, WF(u, 2), BIAS(u, 2), u->a1, B, H[1], W[1], true, s));
+    UNET_TRY(un::conv33(P, u->a1, nullptr, WF(u, 3), BIAS(u, 3), u->e1, B, H[1], W[1], true, s));
+    UNET_TRY(un::conv33(PO, u->e1, nullptr, WF(u, 4), BIAS(u, 4), u->a2, B, H[2], W[2], true, s));
+    UNET_TRY(un::conv33(P, u->a2, nullptr, WF(u, 5), BIAS(u, 5), u->e2, B, H[2], W[2], true, s));
+    UNET_TRY(un::conv33(PO, u->e2, nullptr, WF(u, 6), BIAS(u, 6), u->a3, B, H[3], W[3], true, s));
+    UNET_TRY(un::conv33(P, u->a3, nullptr, WF(u, 7), BIAS(u, 7), u->b3, B, H[3], W[3], true, s));
+    UNET_TRY(un::conv33(UP, u->e2, u->b3, WF(u, 8), BIAS(u, 8), u->c2, B, H[2], W[2], true, s));
+    UNET_TRY(un::conv33(P, u->c2, nullptr, WF(u, 9), BIAS(u, 9), u->f2, B, H[2], W[2], true, s));
+    UNET_TRY(un::conv33(UP, u->e1, u->f2, WF(u, 10), BIAS(u, 10), u->c1, B, H[1], W[1], true, s));
+    UNET_TRY(un::conv33(P, u->c1, nullptr, WF(u, 11), BIAS(u, 11), u->f1, B, H[1], W[1], true, s));
+    UNET_TRY(un::conv33(UP, u->e0, u->f1, WF(u, 12), BIAS(u, 12), u->c0, B, H[0], W[0], true, s));
     UNET_TRY(un::conv_out(u->c0, u->params + kOutOff, u->params + kOutOff + 9 * C, out, B, H[0], W[0], s));
     return SIL_OK;
 }
 
-// weight gradient + input gradient of layer l: X input, Y forward output,
-// dY gradient wrt Y (pre-mask), dX receives the input gradient (may be null)
-int layer_bwd(sil_unet_s* u, int l, const bf16* X, const bf16* Y, const bf16* dY, bf16* dX, int B, int level,
-              float* grad, hipStream_t s)
+// weight gradient + input gradient of layer l: (mode, X, X2) its staged
+// input, Y its forward output, dY the gradient wrt Y (pre-mask); dX receives
+// the input gradient (may be null)
+int layer_bwd(sil_unet_s* u, int l, int mode, const bf16* X, const bf16* X2, const bf16* Y, const bf16* dY, bf16* dX,
+              int B, int level, float* grad, hipStream_t s)
 {
     namespace un = sil::unet;
     const int H = u->Hl[level], W = u->Wl[level];
-    UNET_TRY(un::conv33_wgrad(X, dY, Y, u->part_w, u->part_b, kParts, grad + layer_off(l),
+    UNET_TRY(un::conv33_wgrad(mode, X, X2, dY, Y, u->part_w, u->part_b, kParts, grad + layer_off(l),
                               grad + layer_off(l) + 9 * C * C, B, H, W, s));
-    if (dX) UNET_TRY(un::conv33(dY, Y, WB(u, l), nullptr, dX, B, H, W, false, s));
+    if (dX) UNET_TRY(un::conv33(un::STAGE_MASK, dY, Y, WB(u, l), nullptr, dX, B, H, W, false, s));
     return SIL_OK;
 }
 
 int backward(sil_unet_s* u, int B, const float* x, const float* dout, float* grad, hipStream_t s)
 {
     namespace un = sil::unet;
+    constexpr int P = un::STAGE_PLAIN, UP = un::STAGE_UPADD, PO = un::STAGE_POOL;
     const int *H = u->Hl, *W = u->Wl;
     UNET_TRY(un::conv_out_bwd(u->c0, dout, u->params + kOutOff, u->g0a, u->part_s, kParts, grad + kOutOff, B, H[0],
                               W[0], s));
-    UNET_TRY(layer_bwd(u, 12, u->u0, u->c0, u->g0a, u->g0b, B, 0, grad, s));  // g0b = du0 = de0 (skip)
-    UNET_TRY(un::up_bwd(u->g0b, u->g1a, B, H[1], W[1], s));                    // g1a = df1
-    UNET_TRY(layer_bwd(u, 11, u->c1, u->f1, u->g1a, u->g1b, B, 1, grad, s));  // g1b = dc1
-    UNET_TRY(layer_bwd(u, 10, u->u1, u->c1, u->g1b, u->g1a, B, 1, grad, s));  // g1a = du1 = de1
-    UNET_TRY(un::up_bwd(u->g1a, u->g2a, B, H[2], W[2], s));                    // g2a = df2
-    UNET_TRY(layer_bwd(u, 9, u->c2, u->f2, u->g2a, u->g2b, B, 2, grad, s));   // g2b = dc2
-    UNET_TRY(layer_bwd(u, 8, u->u2, u->c2, u->g2b, u->g2a, B, 2, grad, s));   // g2a = du2 = de2
-    UNET_TRY(un::up_bwd(u->g2a, u->g3a, B, H[3], W[3], s));                    // g3a = db3
-    UNET_TRY(layer_bwd(u, 7, u->a3, u->b3, u->g3a, u->g3b, B, 3, grad, s));   // g3b = da3
-    UNET_TRY(layer_bwd(u, 6, u->p3, u->a3, u->g3b, u->g3a, B, 3, grad, s));   // g3a = dp3
-    UNET_TRY(un::pool_bwd_accumulate(u->g3a, u->g2a, B, H[2], W[2], s));       // de2 += pool'(dp3)
-    UNET_TRY(layer_bwd(u, 5, u->a2, u->e2, u->g2a, u->g2b, B, 2, grad, s));   // g2b = da2
-    UNET_TRY(layer_bwd(u, 4, u->p2, u->a2, u->g2b, u->g2a, B, 2, grad, s));   // g2a = dp2
-    UNET_TRY(un::pool_bwd_accumulate(u->g2a, u->g1a, B, H[1], W[1], s));       // de1 += pool'(dp2)
-    UNET_TRY(layer_bwd(u, 3, u->a1, u->e1, u->g1a, u->g1b, B, 1, grad, s));   // g1b = da1
-    UNET_TRY(layer_bwd(u, 2, u->p1, u->a1, u->g1b, u->g1a, B, 1, grad, s));   // g1a = dp1
-    UNET_TRY(un::pool_bwd_accumulate(u->g1a, u->g0b, B, H[0], W[0], s));       // de0 += pool'(dp1)
-    UNET_TRY(layer_bwd(u, 1, u->a0, u->e0, u->g0b, u->g0a, B, 0, grad, s));   // g0a = da0
+    UNET_TRY(layer_bwd(u, 12, UP, u->e0, u->f1, u->c0, u->g0a, u->g0b, B, 0, grad, s)); // g0b = du0 = de0 (skip)
+    UNET_TRY(un::up_bwd(u->g0b, u->g1a, B, H[1], W[1], s));                              // g1a = df1
+    UNET_TRY(layer_bwd(u, 11, P, u->c1, nullptr, u->f1, u->g1a, u->g1b, B, 1, grad, s)); // g1b = dc1
+    UNET_TRY(layer_bwd(u, 10, UP, u->e1, u->f2, u->c1, u->g1b, u->g1a, B, 1, grad, s));  // g1a = du1 = de1
+    UNET_TRY(un::up_bwd(u->g1a, u->g2a, B, H[2], W[2], s));                              // g2a = df2
+    UNET_TRY(layer_bwd(u, 9, P, u->c2, nullptr, u->f2, u->g2a, u->g2b, B, 2, grad, s));  // g2b = dc2
+    UNET_TRY(layer_bwd(u, 8, UP, u->e2, u->b3, u->c2, u->g2b, u->g2a, B, 2, grad, s));   // g2a = du2 = de2
+    UNET_TRY(un::up_bwd(u->g2a, u->g3a, B, H[3], W[3], s));                              // g3a = db3
+    UNET_TRY(layer_bwd(u, 7, P, u->a3, nullptr, u->b3, u->g3a, u->g3b, B, 3, grad, s));  // g3b = da3
+    UNET_TRY(layer_bwd(u, 6, PO, u->e2, nullptr, u->a3, u->g3b, u->g3a, B, 3, grad, s)); // g3a = dp3
+    UNET_TRY(un::pool_bwd_accumulate(u->g3a, u->g2a, B, H[2], W[2], s));                 // de2 += pool'(dp3)
+    UNET_TRY(layer_bwd(u, 5, P, u->a2, nullptr, u->e2, u->g2a, u->g2b, B, 2, grad, s));  // g2b = da2
+    UNET_TRY(layer_bwd(u, 4, PO, u->e1, nullptr, u->a2, u->g2b, u->g2a, B, 2, grad, s)); // g2a = dp2
+    UNET_TRY(un::pool_bwd_accumulate(u->g2a, u->g1a, B, H[1], W[1], s));                 // de1 += pool'(dp2)
+    UNET_TRY(layer_bwd(u, 3, P, u->a1, nullptr, u->e1, u->g1a, u->g1b, B, 1, grad, s));  // g1b = da1
+    UNET_TRY(layer_bwd(u, 2, PO, u->e0, nullptr, u->a1, u->g1b, u->g1a, B, 1, grad, s)); // g1a = dp1
+    UNET_TRY(un::pool_bwd_accumulate(u->g1a, u->g0b, B, H[0], W[0], s));                 // de0 += pool'(dp1)
+    UNET_TRY(layer_bwd(u, 1, P, u->a0, nullptr, u->e0, u->g0b, u->g0a, B, 0, grad, s));  // g0a = da0
     UNET_TRY(un::conv_in_wgrad(x, u->g0a, u->a0, u->part_s, kParts, grad, B, H[0], W[0], s));
     return SIL_OK;
 }
@@ -215,10 +213,10 @@ int sil_unet_create(sil_unet* out, int32_t max_batch, int32_t height, int32_t wi
     A(u->adam_v, kParams);
     A(u->wf, (size_t)kLayers * 9 * C * C);
     A(u->wb, (size_t)kLayers * 9 * C * C);
-    for (bf16** p : {&u->a0, &u->e0, &u->u0, &u->c0, &u->g0a, &u->g0b}) A(*p, s0);
-    for (bf16** p : {&u->p1, &u->a1, &u->e1, &u->u1, &u->c1, &u->f1, &u->g1a, &u->g1b}) A(*p, s1);
-    for (bf16** p : {&u->p2, &u->a2, &u->e2, &u->u2, &u->c2, &u->f2, &u->g2a, &u->g2b}) A(*p, s2);
-    for (bf16** p : {&u->p3, &u->a3, &u->b3, &u->g3a, &u->g3b}) A(*p, s3);
+    for (bf16** p : {&u->a0, &u->e0, &u->c0, &u->g0a, &u->g0b}) A(*p, s0);
+    for (bf16** p : {&u->a1, &u->e1, &u->c1, &u->f1, &u->g1a, &u->g1b}) A(*p, s1);
+    for (bf16** p : {&u->a2, &u->e2, &u->c2, &u->f2, &u->g2a, &u->g2b}) A(*p, s2);
+    for (bf16** p : {&u->a3, &u->b3, &u->g3a, &u->g3b}) A(*p, s3);
     A(u->out, npx);
     A(u->dout, npx);
     A(u->an, npx);

@@ -167,7 +167,7 @@ class PoissonBatch:
 
         method: "cg" (Jacobi CG, FP64), "cg_mixed", "fcg_learned" or
         "fcg_learned_mixed".  With record_every > 0 the FP32 residuals and
-        iterates before every record_every iterations are returned as
+        iterates (FP64) before every record_every iterations are returned as
         (r_records, x_records) of shape [count][B][n][n]."""
         _f64(rhs, "rhs", self.shape)
         if x is None:
@@ -188,7 +188,7 @@ class PoissonBatch:
         count = ctypes.c_int32(0)
         if record_every:
             rr = torch.empty((max_records,) + self.shape, dtype=torch.float32, device="cuda")
-            xr = torch.empty_like(rr)
+            xr = torch.empty((max_records,) + self.shape, dtype=torch.float64, device="cuda")
             rec = L.SolveRecord(record_every, max_records, L.ptr(rr), L.ptr(xr), ctypes.pointer(count))
         L.check(L.lib().sil_poisson2d_batch_solve(self.handle, L.ptr(rhs), L.ptr(x), ctypes.byref(st),
                                                   net.handle if net is not None else None, res,

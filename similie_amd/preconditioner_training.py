@@ -8,9 +8,11 @@ b ~ N(0, 1) on interior nodes.
 
 Training data are (residual, error) pairs that plain CG itself produces:
 Jacobi CG runs every training problem to 1e-10 while the batch solver
-records r_k and x_k; the error of record k is e_k = x* − x_k with x* the
-converged solution.  The network maps [r/rms(r), normalised log κ] to a
-search direction and is trained with the direction loss
+records its iterates x_k; the error of record k is e_k = x* − x_k with x*
+the converged solution.  The solver applies the network inside a two-level
+cycle, after a damped-Jacobi pre-smoothing step, so each pair is taken after
+that step (see generate_pairs).  The network maps [r/rms(r), normalised
+log κ] to a correction direction and is trained with the direction loss
 1 − cos²_A(e, N) (sil_unet_train_step).
 
     python -m similie_amd.preconditioner_training --out similie_amd/weights/unet_poisson256.npy
@@ -61,36 +63,71 @@ def make_problems(batch, n, sigma, corr, seed, device="cuda"):
     return kappa, b
 
 
-def generate_pairs(n_problems, n, sigma, corr, seed, pairs_per_problem, batch=64, record_every=8, log=print):
-    """(inputs [P][n][n][2], errors [P][n][n], wx, wy, problem index) with
-    P = n_problems · pairs_per_problem, residual and error scaled by 1/rms(r)."""
+OMEGA = 2.0 / 3.0  # damping of the Jacobi smoothing in the solver's two-level preconditioner
+
+
+def jacobi_inverse(kappa):
+    """D⁻¹ of the batch operator (interior nodes; 0 on the boundary ring), FP64."""
+    wx = torch.empty_like(kappa)
+    wy = torch.empty_like(kappa)
+    wx[:, :-1, :] = 0.5 * (kappa[:, :-1, :] + kappa[:, 1:, :])
+    wx[:, -1, :] = kappa[:, -1, :]
+    wy[:, :, :-1] = 0.5 * (kappa[:, :, :-1] + kappa[:, :, 1:])
+    wy[:, :, -1] = kappa[:, :, -1]
+    d = torch.zeros_like(kappa)
+    d[:, 1:-1, 1:-1] = 1.0 / (wx[:, 1:-1, 1:-1] + wx[:, :-2, 1:-1] + wy[:, 1:-1, 1:-1] + wy[:, 1:-1, :-2])
+    return d
+
+
+def generate_pairs(n_problems, n, sigma, corr, seed, pairs_per_problem, batch=64, record_every=16, max_records=400,
+                   log=print):
+    """Training pairs of the network as the solver uses it.
+
+    Jacobi CG solves every problem to 1e-10 while recording its iterates
+    x_k.  For each picked record the pair is formed after the solver's
+    pre-smoothing step: r = b − A x_k, e = x* − x_k, s = ω D⁻¹ r,
+    r₁ = r − A s, e₁ = e − s (so A e₁ = r₁), both scaled by 1/rms(r₁).
+    Returns (inputs [P][n][n][2], errors [P][n][n], wx, wy, problem index)."""
     inputs, errors, wxs, wys, owners = [], [], [], [], []
     for start in range(0, n_problems, batch):
         B = min(batch, n_problems - start)
         kappa, b = make_problems(B, n, sigma, corr, seed + start)
+        if not bool(torch.isfinite(kappa).all()):
+            raise RuntimeError(f"non-finite coefficients for seed {seed + start}")
         pb = Lr.PoissonBatch(kappa)
         t0 = time.time()
-        x, res, secs, (rr, xr) = pb.solve(b, method="cg", tol=1e-10, record_every=record_every, max_records=100000)
+        x, res, secs, (rr, xr) = pb.solve(b, method="cg", tol=1e-10, record_every=record_every,
+                                          max_records=max_records)
         its = np.array([r["iterations"] for r in res])
+        ok = np.array([r["converged"] for r in res])
         log(f"pairs: problems {start}..{start + B - 1}: CG iterations {its.min()}..{its.max()}, "
-            f"{rr.shape[0]} records, {time.time() - t0:.1f} s")
+            f"{int(ok.sum())}/{B} converged, {xr.shape[0]} records, {time.time() - t0:.1f} s")
         wx, wy, lk = pb.weights()
-        xstar = x.float()
+        dinv = jacobi_inverse(kappa)
+        nrec = xr.shape[0]
+        pick = np.zeros((nrec, B), dtype=bool)
         for p in range(B):
-            valid = int(min(rr.shape[0], max(1, its[p] // record_every)))
-            pick = np.unique(np.linspace(0, valid - 1, pairs_per_problem).round().astype(int))
-            for k in pick:
-                r = rr[k, p]
-                rho = torch.sqrt(torch.mean(r.double() ** 2)).float().clamp_min(1e-30)
-                e = (xstar[p] - xr[k, p]) / rho
-                inputs.append(torch.stack([r / rho, lk[p]], dim=-1))
-                errors.append(e)
-                owners.append(start + p)
-            wxs.append(wx[p])
-            wys.append(wy[p])
+            if not ok[p]:
+                continue
+            valid = int(min(nrec, max(1, its[p] // record_every)))
+            pick[np.unique(np.linspace(0, valid - 1, pairs_per_problem).round().astype(int)), p] = True
+        for k in range(nrec):
+            sel = torch.from_numpy(np.nonzero(pick[k])[0]).cuda()
+            if sel.numel() == 0:
+                continue
+            xk = xr[k]
+            r = b - pb.apply(xk)
+            s = OMEGA * dinv * r
+            r1 = r - pb.apply(s.contiguous())
+            e1 = (x - xk) - s
+            rho = torch.sqrt(torch.mean(r1 * r1, dim=(1, 2), keepdim=True)).clamp_min(1e-300)
+            inputs.append(torch.stack([(r1 / rho).float()[sel], lk[sel]], dim=-1))
+            errors.append((e1 / rho).float()[sel])
+            owners.append(sel + start)
+        wxs.append(wx)
+        wys.append(wy)
         del rr, xr
-    return (torch.stack(inputs), torch.stack(errors), torch.stack(wxs), torch.stack(wys),
-            torch.tensor(owners, device="cuda"))
+    return (torch.cat(inputs), torch.cat(errors), torch.cat(wxs), torch.cat(wys), torch.cat(owners))
 
 
 def train(net, data, epochs, batch=32, lr_max=1e-3, lr_min=1e-5, seed=0, log=print):
@@ -119,20 +156,30 @@ def train(net, data, epochs, batch=32, lr_max=1e-3, lr_min=1e-5, seed=0, log=pri
 
 
 def evaluate(net, n_problems, n, sigma, corr, seed, methods=("cg", "cg_mixed", "fcg_learned", "fcg_learned_mixed"),
-             tol=1e-10, log=print):
-    """Time to an FP64-verified tol of each method on one batch of held-out problems."""
+             tol=1e-10, ablation=True, log=print):
+    """Time to an FP64-verified tol of each method on one batch of held-out
+    problems.  With ablation, flexible CG also runs with a network whose
+    parameters are all zero: the same two-level cycle without the learned
+    correction."""
     kappa, b = make_problems(n_problems, n, sigma, corr, seed)
     pb = Lr.PoissonBatch(kappa)
+    runs = [(m, net if m.startswith("fcg") else None) for m in methods]
+    if ablation:
+        zero = Lr.UNet(net.max_batch, n, n)
+        zero.set_params(torch.zeros(Lr.param_count()))
+        runs.append(("fcg_without_network", zero))
     out = {}
-    for m in methods:
+    for name, nn in runs:
+        m = "fcg_learned" if name == "fcg_without_network" else name
         # first call warms up (graph instantiation, allocation), second is timed
-        pb.solve(b, method=m, tol=tol, net=net if m.startswith("fcg") else None, max_iterations=20)
-        x, res, secs = pb.solve(b, method=m, tol=tol, net=net if m.startswith("fcg") else None)
+        pb.solve(b, method=m, tol=tol, net=nn, max_iterations=20)
+        x, res, secs = pb.solve(b, method=m, tol=tol, net=nn)
         its = [r["iterations"] for r in res]
         worst = max(r["true_relative_residual"] for r in res)
-        out[m] = {"seconds": secs, "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its)),
-                  "converged": int(sum(r["converged"] for r in res)), "worst_true_relative_residual": worst}
-        log(f"{m}: {secs * 1e3:.1f} ms, iterations mean {np.mean(its):.1f} max {max(its)}, worst residual {worst:.2e}")
+        out[name] = {"seconds": secs, "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its)),
+                     "converged": int(sum(r["converged"] for r in res)), "worst_true_relative_residual": worst}
+        log(f"{name}: {secs * 1e3:.1f} ms, iterations mean {np.mean(its):.1f} max {max(its)}, "
+            f"worst residual {worst:.2e}")
     return out
 
 
@@ -143,7 +190,7 @@ def main(argv=None):
     ap.add_argument("--corr", type=float, default=16.0)
     ap.add_argument("--problems", type=int, default=200)
     ap.add_argument("--pairs-per-problem", type=int, default=100)
-    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1000)

@@ -51,10 +51,14 @@ def weights_of(kappa):
     return wx, wy
 
 
-# BF16 weights and activations with FP32 accumulation against FP64: the
-# tolerance is a few BF16 roundings (2^-8) accumulated over 14 layers
-FWD_TOL = 3e-2
-GRAD_TOL = 6e-2
+# The kernels store weights and activations in BF16 and accumulate in FP32.
+# They are compared with the oracle computing the same way
+# (oracle.unet_oracle emulate=True), which they match up to accumulation
+# order, and with exact FP64, from which BF16 storage alone moves them by
+# about 1e-2 forward and 0.05-0.15 in the weight gradients of white-noise
+# inputs (the emulating oracle shows the same distance).
+FWD_TOL_EMULATED, FWD_TOL_FP64 = 3e-3, 3e-2
+GRAD_TOL_EMULATED, GRAD_TOL_FP64 = 1e-2, 0.25
 
 
 @pytest.mark.parametrize("shape", [(2, 32, 32), (1, 16, 40), (3, 64, 64)])
@@ -66,8 +70,9 @@ def test_unet_forward_matches_oracle(learned, shape):
     net = learned.UNet(B, H, W, seed=1)
     net.set_params(torch.from_numpy(params))
     out = net.forward(torch.from_numpy(x).cuda()).cpu().numpy()
-    ref, _ = U.forward(params.astype(np.float64), x)
-    assert rel(out, ref) <= FWD_TOL
+    p64 = params.astype(np.float64)
+    assert rel(out, U.forward(p64, x, emulate=True)[0]) <= FWD_TOL_EMULATED
+    assert rel(out, U.forward(p64, x)[0]) <= FWD_TOL_FP64
 
 
 def test_unet_backward_matches_oracle(learned):
@@ -81,13 +86,18 @@ def test_unet_backward_matches_oracle(learned):
     xt = torch.from_numpy(x).cuda()
     net.forward(xt)
     g = net.backward(xt, torch.from_numpy(dout).cuda()).cpu().numpy()
-    _, act = U.forward(params.astype(np.float64), x)
-    gref = U.backward(params.astype(np.float64), act, dout)
+    p64 = params.astype(np.float64)
+    _, act_em = U.forward(p64, x, emulate=True)
+    g_em = U.backward(p64, act_em, dout, emulate=True)
+    _, act = U.forward(p64, x)
+    g64 = U.backward(p64, act, dout)
     # every parameter block separately: conv_in, the twelve layers, conv_out
     blocks = [slice(0, U.N_IN)] + [slice(U.layer_slices(l)[0].start, U.layer_slices(l)[1].stop)
                                     for l in range(1, U.N_LAYERS + 1)] + [slice(U.OUT_BASE, U.N_PARAMS)]
-    errs = [rel(g[s], gref[s]) for s in blocks]
-    assert max(errs) <= GRAD_TOL, errs
+    e_em = [rel(g[s], g_em[s]) for s in blocks]
+    e64 = [rel(g[s], g64[s]) for s in blocks]
+    assert max(e_em) <= GRAD_TOL_EMULATED, e_em
+    assert max(e64) <= GRAD_TOL_FP64, e64
 
 
 def test_unet_direction_loss_matches_oracle(learned):
@@ -151,21 +161,76 @@ def test_batch_apply_matches_oracle(learned):
     assert rel(y, U.apply5(x, wx, wy)) <= 1e-14
 
 
-@pytest.mark.parametrize("method", ["cg", "cg_mixed", "fcg_learned", "fcg_learned_mixed"])
-def test_batch_solvers_reach_fp64_verified_tolerance(learned, method):
+def check_solution(kappa, b, x, res, tol=1e-10):
+    wx, wy = weights_of(kappa)
+    xr = x.cpu().numpy()
+    for p in range(b.shape[0]):
+        r = b[p] - U.apply5(xr[p: p + 1], wx[p: p + 1], wy[p: p + 1])[0]
+        true_rel = np.linalg.norm(r) / np.linalg.norm(b[p])
+        assert res[p]["converged"], (p, res[p])
+        assert true_rel <= tol * 1.01, (p, true_rel)
+        assert abs(true_rel - res[p]["true_relative_residual"]) <= 1e-3 * tol + 1e-3 * true_rel
+
+
+@pytest.mark.parametrize("method", ["cg", "cg_mixed"])
+def test_batch_cg_reaches_fp64_verified_tolerance(learned, method):
     B, n = 5, 32
     kappa, b = problems(B, n, 7)
     pb = learned.PoissonBatch(torch.from_numpy(kappa))
-    net = learned.UNet(B, n, n, seed=2) if method.startswith("fcg") else None
-    x, res, secs = pb.solve(torch.from_numpy(b).cuda(), method=method, tol=1e-10, max_iterations=4000, net=net)
-    wx, wy = weights_of(kappa)
-    xr = x.cpu().numpy()
-    for p in range(B):
-        r = b[p] - U.apply5(xr[p: p + 1], wx[p: p + 1], wy[p: p + 1])[0]
-        true_rel = np.linalg.norm(r) / np.linalg.norm(b[p])
-        assert res[p]["converged"], (method, p, res[p])
-        assert true_rel <= 1e-10 * 1.01, (method, p, true_rel)
-        assert abs(true_rel - res[p]["true_relative_residual"]) <= 1e-3 * 1e-10 + 1e-3 * true_rel
+    x, res, secs = pb.solve(torch.from_numpy(b).cuda(), method=method, tol=1e-10, max_iterations=4000)
+    check_solution(kappa, b, x, res)
+
+
+def identity_params():
+    """Parameters with which the network returns its residual channel:
+    conv_in splits r into relu(r) and relu(-r), layers 1 and 12 pass both
+    channels through their centre tap, every other layer is zero (the
+    coarse path adds nothing), conv_out recombines relu(r) - relu(-r)."""
+    p = np.zeros(U.N_PARAMS, dtype=np.float32)
+    win = np.zeros((9, U.C, 2), dtype=np.float32)
+    win[4, 0, 0], win[4, 1, 0] = 1.0, -1.0
+    p[: 9 * U.C * 2] = win.ravel()
+    for l in (1, 12):
+        w = np.zeros((9, U.C, U.C), dtype=np.float32)
+        w[4, 0, 0] = w[4, 1, 1] = 1.0
+        p[U.layer_slices(l)[0]] = w.ravel()
+    wout = np.zeros((9, U.C), dtype=np.float32)
+    wout[4, 0], wout[4, 1] = 1.0, -1.0
+    p[U.OUT_BASE: U.OUT_BASE + 9 * U.C] = wout.ravel()
+    return p
+
+
+@pytest.mark.parametrize("method", ["fcg_learned", "fcg_learned_mixed"])
+def test_fcg_with_an_identity_network_behaves_as_cg(learned, method):
+    """With N(r) = r (in BF16) flexible CG is unpreconditioned CG up to the
+    BF16 perturbation of the preconditioner: it must converge, to the
+    FP64-verified tolerance, and no problem may need the fallback."""
+    B, n = 4, 32
+    kappa, b = problems(B, n, 9)
+    rng = np.random.default_rng(3)
+    x_in = rng.standard_normal((2, n, n, 2)).astype(np.float32)
+    net = learned.UNet(B, n, n)
+    net.set_params(torch.from_numpy(identity_params()))
+    out = net.forward(torch.from_numpy(x_in).cuda()).cpu().numpy()
+    assert rel(out, x_in[..., 0]) <= 1e-2  # the network is the identity on channel 0
+    pb = learned.PoissonBatch(torch.from_numpy(kappa))
+    x, res, secs = pb.solve(torch.from_numpy(b).cuda(), method=method, tol=1e-10, max_iterations=3000, net=net)
+    check_solution(kappa, b, x, res)
+    # unpreconditioned CG on 30x30 interior unknowns with this contrast needs
+    # O(100) iterations; stagnation (and the switch to Jacobi) would show as
+    # many more
+    assert max(r["iterations"] for r in res) <= 600
+
+
+def test_fcg_falls_back_to_cg_when_the_network_is_useless(learned):
+    """An untrained network is no preconditioner; flexible CG must still
+    reach the FP64-verified tolerance through its plain-CG fallback."""
+    B, n = 3, 32
+    kappa, b = problems(B, n, 11)
+    net = learned.UNet(B, n, n, seed=5)
+    pb = learned.PoissonBatch(torch.from_numpy(kappa))
+    x, res, secs = pb.solve(torch.from_numpy(b).cuda(), method="fcg_learned", tol=1e-10, max_iterations=4000, net=net)
+    check_solution(kappa, b, x, res)
 
 
 def test_batch_record_returns_residual_history(learned):
@@ -174,10 +239,10 @@ def test_batch_record_returns_residual_history(learned):
     pb = learned.PoissonBatch(torch.from_numpy(kappa))
     x, res, secs, (rr, xr) = pb.solve(torch.from_numpy(b).cuda(), method="cg", tol=1e-10, record_every=4,
                                       max_records=400)
-    assert rr.shape[0] >= 2
+    assert rr.shape[0] >= 2 and xr.dtype == torch.float64
     # record 0 is the initial residual b (x0 = 0)
     assert rel(rr[0].cpu().numpy(), b) <= 1e-6
     wx, wy = weights_of(kappa)
     k = rr.shape[0] // 2
-    r_k = b - U.apply5(xr[k].cpu().numpy().astype(np.float64), wx, wy)
-    assert rel(rr[k].cpu().numpy(), r_k) <= 1e-4
+    r_k = b - U.apply5(xr[k].cpu().numpy(), wx, wy)
+    assert rel(rr[k].cpu().numpy(), r_k) <= 1e-5
