@@ -17,6 +17,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -93,6 +95,120 @@ def cpu_baseline(seconds):
     }
 
 
+def _guard(name, fn):
+    """Runs one solver measurement with a progress line on stderr; a failure
+    is reported in the line instead of ending the benchmark."""
+    t0 = time.perf_counter()
+    print(f"bench: {name} ...", file=sys.stderr, flush=True)
+    try:
+        out = fn()
+    except Exception as exc:  # noqa: BLE001 - reported, not swallowed
+        out = {"error": f"{type(exc).__name__}: {exc}"}
+    print(f"bench: {name} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    return out
+
+
+def laplacian_2d_0form(torch):
+    """Δ of a 0-form on the 256² Cartesian grid (spacing 1/256, Euclidean), N(0,1) values."""
+    from similie_amd import exterior as E
+
+    n = 256
+    x = np.arange(n) / 256.0
+    grid = E.Grid([n, n], axis_coords=[x, x])
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    f = torch.randn((n, n, 1), generator=gen, dtype=torch.float64, device="cuda")
+    out = torch.empty((n - 1, n - 1, 1), dtype=torch.float64, device="cuda")
+    st = E.make_staged_laplacian(grid, 0, out, f)
+    for _ in range(10):
+        st.run(out, f)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(200):
+        st.run(out, f)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / 200 * 1e3
+    nbytes = 8 * (n * n + (n - 1) ** 2)
+    return {"workload": "0-form Laplacian on 256^2, h = 1/256, Euclidean metric", "us_per_application": round(us, 2),
+            "GBps": round(nbytes / (us * 1e-6) / 1e9, 1),
+            "note": "1 MB in + out: the application is launch-latency bound, not HBM bound"}
+
+
+def solve_poisson_bumps(torch):
+    """256² Dirichlet Poisson, three Gaussian bumps, plain FP64 Jacobi CG to 1e-10."""
+    from similie_amd import solvers as S
+    from similie_amd.workloads import poisson_bumps
+
+    x, y, b = poisson_bumps()
+    op = S.Poisson2D(x, y)
+    rhs = torch.from_numpy(b.reshape(-1)).cuda()
+    st = S.StrongFormulationSolverSettings(max_iterations=20000, relative_tolerance=1e-10)
+    S.minimize_strong_formulation_residual(op, rhs, settings=st)  # warm-up (graph capture)
+    _, d = S.minimize_strong_formulation_residual(op, rhs, settings=st)
+    return {"workload": "Dirichlet Poisson 256^2, h = 1/256, three Gaussian bumps", "method": "FP64 Jacobi CG",
+            "seconds": round(d["duration"], 5), "iterations": d["iterations"],
+            "true_relative_residual": d["true_relative_residual"]}
+
+
+def solve_magnetostatics(torch, max_iterations=20000):
+    """192³-cell inductor (μ_r 1000 core, coil annulus): FP64 vs mixed CG to 1e-10."""
+    from similie_amd import solvers as S
+    from similie_amd.workloads import MU0, magnetostatics_inductor
+
+    coords, cell_mu, cell_j = magnetostatics_inductor()
+    shape = tuple(len(c) for c in coords)
+    mu = S.MagnetostaticsOperator3D.nodal_mu(shape, torch.from_numpy(cell_mu).cuda(), MU0)
+    op = S.MagnetostaticsOperator3D(*coords, mu, gauge_penalty=1e7)
+    rhs = op.rhs(torch.from_numpy(np.ascontiguousarray(cell_j)).cuda())
+    out = {"workload": "magnetostatics curl-curl, 192^3 uniform cells, mu_r 1000 core 48x48x96, coil annulus "
+                       "half-widths 32-40 cells, 64 high, J = 1 A/m^2", "dofs": int(rhs.numel())}
+    for prec in ("fp64", "mixed"):
+        st = S.StrongFormulationSolverSettings(max_iterations=max_iterations, relative_tolerance=1e-10,
+                                               precision=prec)
+        print(f"bench:   magnetostatics {prec} ...", file=sys.stderr, flush=True)
+        _, d = S.minimize_strong_formulation_residual(op, rhs, settings=st)
+        out[prec] = {"seconds": round(d["duration"], 4), "iterations": d["iterations"],
+                     "converged": bool(d["converged"]), "true_relative_residual": d["true_relative_residual"]}
+    if out["mixed"]["seconds"] > 0:
+        out["mixed_speedup"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    return out
+
+
+def solve_uneven_batch(torch):
+    """512 problems, sides log-uniform in [32, 512], one persistent launch."""
+    from similie_amd.batched import BatchedPoisson2D
+    from similie_amd.workloads import uneven_poisson_batch
+
+    sides, kappa, rhs = uneven_poisson_batch()
+    bp = BatchedPoisson2D(sides, kappa)
+    block, cands = bp.autotune(rhs, probe_iterations=20)
+    bp.solve(rhs)  # warm-up
+    _, res, secs = bp.solve(rhs)
+    return {"workload": "512 Dirichlet Poisson problems, sides log-uniform in [32, 512], kappa = exp(GRF sigma 1, "
+                        "corr 8), N(0,1) rhs", "method": "mixed CG (FP32 matvec, FP64 refinement), one hipGraph launch",
+            "autotuned_block": block, "autotune_seconds": {str(k): v for k, v in cands.items()},
+            "seconds": round(secs, 4), "converged": int(sum(bool(r["converged"]) for r in res)),
+            "worst_true_relative_residual": max(r["true_relative_residual"] for r in res),
+            "unknowns": int(rhs.numel())}
+
+
+def solve_learned(torch):
+    """64 held-out 256² problems (κ = exp(GRF σ 1.5, corr 16)): plain FP64 CG,
+    mixed CG and flexible CG with the trained network, to an FP64-verified 1e-10."""
+    from similie_amd import learned as Lr
+    from similie_amd.preconditioner_training import DEFAULT_WEIGHTS, evaluate
+
+    if not DEFAULT_WEIGHTS.exists():
+        return {"error": f"no trained weights at {DEFAULT_WEIGHTS.relative_to(ROOT)}"}
+    net = Lr.UNet(64, 256, 256)
+    net.set_params(torch.from_numpy(np.load(DEFAULT_WEIGHTS)))
+    res = evaluate(net, 64, 256, 1.5, 16.0, 900000, log=lambda m: None)
+    base = res["cg"]["seconds"]
+    best = min(res["fcg_learned"]["seconds"], res["fcg_learned_mixed"]["seconds"])
+    return {"workload": "64 held-out Poisson problems 256^2, kappa = exp(GRF sigma 1.5, corr 16), N(0,1) rhs",
+            "methods": res, "learned_speedup_vs_plain_fp64_cg": round(base / best, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +216,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solves", action="store_true", help="skip the solver workloads")
+    ap.add_argument("--ms-iterations", type=int, default=20000,
+                    help="iteration cap of each magnetostatics solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this build has one MI355X, and the Laplacian "
@@ -177,6 +296,17 @@ def main():
         },
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds),
     }
+    if not args.no_solves:
+        del grid, g, w, out, staged
+        torch.cuda.empty_cache()
+        solves = {
+            "laplacian_2d_0form": lambda: laplacian_2d_0form(torch),
+            "poisson2d_bumps": lambda: solve_poisson_bumps(torch),
+            "uneven_batch_512": lambda: solve_uneven_batch(torch),
+            "learned_preconditioner": lambda: solve_learned(torch),
+            "magnetostatics_inductor": lambda: solve_magnetostatics(torch, args.ms_iterations),
+        }
+        line["solves"] = {k: _guard(k, f) for k, f in solves.items()}
     print(json.dumps(line), flush=True)
 
 

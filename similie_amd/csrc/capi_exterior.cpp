@@ -269,7 +269,7 @@ namespace sil {
 // the environment (SIL_LAP3D_TILE="variant,ty,xchunk").
 FusedTile default_lap3d_tile()
 {
-    FusedTile t{FUSED_MARCH, 64, 8, 32};
+    FusedTile t{FUSED_MARCH, 64, 16, 32};
     if (const char* e = std::getenv("SIL_LAP3D_TILE")) {
         int v = 1, ty = 8, xc = 32;
         if (std::sscanf(e, "%d,%d,%d", &v, &ty, &xc) == 3) {

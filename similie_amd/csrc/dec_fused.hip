@@ -204,8 +204,8 @@ struct MarchLds0 {
     double dw[2][TY][64];  // δω ping-pong: planes p and p+1
 };
 
-template <int TY, bool DIAG>
-__global__ __launch_bounds__(64 * TY) void k_lap3d_march0(const double* __restrict__ w, const double* __restrict__ g,
+template <int TY, bool DIAG, int MINW>
+__global__ __launch_bounds__(64 * TY, MINW) void k_lap3d_march0(const double* __restrict__ w, const double* __restrict__ g,
                                                          const double* __restrict__ hxs, const double* __restrict__ hys,
                                                          const double* __restrict__ hzs, double* __restrict__ out,
                                                          int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
@@ -351,6 +351,170 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march0(const double* __restri
         Kn = coefs_of(Gf, hf, pos_in && p + 2 >= 0 && p + 2 < Nx);
     }
 }
+
+
+// v0 with two prefetched planes in alternating register sets
+template <int TY, bool DIAG>
+__global__ __launch_bounds__(64 * TY) void k_lap3d_march5(const double* __restrict__ w, const double* __restrict__ g,
+                                                         const double* __restrict__ hxs, const double* __restrict__ hys,
+                                                         const double* __restrict__ hzs, double* __restrict__ out,
+                                                         int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
+                                                         int64_t Oz, int ntz, int nty, int nxc, int xchunk)
+{
+    __shared__ MarchLds0<TY> s;
+    const int lane = threadIdx.x & 63;
+    const int row = threadIdx.x >> 6;
+
+    // XCD-contiguous tile order: hardware places workgroup b on XCD b % 8.
+    const int nb = gridDim.x;
+    const int b = blockIdx.x;
+    const int per = (nb + 7) / 8;
+    const int rem = nb % 8;
+    int logical;
+    {
+        const int xcd = b % 8, slot = b / 8;
+        // XCDs [0, rem) own `per` logical blocks, the rest `per - 1` (rem == 0: all `per`)
+        const int full = rem == 0 ? 8 : rem;
+        logical = (xcd < full) ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
+    }
+    const int tz = logical % ntz;
+    const int ty = (logical / ntz) % nty;
+    const int tx = logical / (ntz * nty);
+    if (tx >= nxc) return; // uniform per workgroup
+
+    const int64_t z = (int64_t)tz * 62 - 1 + lane;
+    const int64_t y = (int64_t)ty * (TY - 2) - 1 + row;
+    const int64_t xa = (int64_t)tx * xchunk;
+    const int64_t xb = min(xa + xchunk, Ox);
+    const bool pos_in = (z >= 0 && z < Nz && y >= 0 && y < Ny);
+    const bool writer = lane >= 1 && lane <= 62 && row >= 1 && row <= TY - 2 && z < Oz && y < Oy;
+    const bool has_y1 = (y + 1 < Ny), has_z1 = (z + 1 < Nz);
+    const double hy = pos_in ? hys[y] : 0.0;
+    const double hz = pos_in ? hzs[z] : 0.0;
+    const int64_t plane = Ny * Nz;
+    const int64_t lin_yz = y * Nz + z;
+
+    auto load = [&](int64_t q, double W[3], double G[3], double& hxq) {
+        if (pos_in && q >= 0 && q < Nx) {
+            const int64_t lin = q * plane + lin_yz;
+            W[0] = w[lin * 3 + 0];
+            W[1] = w[lin * 3 + 1];
+            W[2] = w[lin * 3 + 2];
+            if (DIAG) {
+                G[0] = g[lin * 3 + 0];
+                G[1] = g[lin * 3 + 1];
+                G[2] = g[lin * 3 + 2];
+            } else {
+                G[0] = G[1] = G[2] = 1.0;
+            }
+            hxq = hxs[q];
+        } else {
+            W[0] = W[1] = W[2] = 0.0;
+            G[0] = G[1] = G[2] = 0.0; // outside: zero coefficients (zero-extension)
+            hxq = 0.0;
+        }
+    };
+    auto coefs_of = [&](const double G[3], double hxq, bool valid) {
+        if (!valid) return Coefs{0, 0, 0, 0, 0, 0, 0};
+        return node_coefs0(G[0], G[1], G[2], hxq, hy, hz);
+    };
+
+    // state: c = plane p, n = plane p+1; two prefetched planes (p+2, p+3)
+    // in register sets that alternate between the two halves of the
+    // unrolled loop, so a load's destination is consumed in place
+    double Wc[3], Wn[3], Wf[3], Gc[3], Gn[3], Gf[3], hc, hn, hf;
+    double Wf2[3], Gf2[3], hf2;
+    int64_t p = xa - 1;
+    load(p, Wc, Gc, hc);
+    load(p + 1, Wn, Gn, hn);
+    Coefs Kc = coefs_of(Gc, hc, pos_in && p >= 0 && p < Nx);
+    Coefs Kn = coefs_of(Gn, hn, pos_in && p + 1 < Nx);
+    double etaPrev[3] = {0.0, 0.0, 0.0};
+    // ⋆ω(p) own: YZ, XZ, XY with signs
+    double epc0 = Kc.b1 * Wc[0], epc1 = -Kc.b2 * Wc[1], epc2 = Kc.b3 * Wc[2];
+    (void)epc1;
+    (void)epc2;
+    double dwc = 0.0;
+    int buf = 0;
+
+    load(p + 2, Wf, Gf, hf);
+    auto step = [&](double (&Wl)[3], double (&Gl)[3], double& hl, const double (&Wu)[3], const double (&Gu)[3],
+                    const double hu) {
+        load(p + 3, Wl, Gl, hl); // prefetch, consumed at the end of the next step
+
+        // ⋆ω(p+1) own
+        const double epn0 = Kn.b1 * Wn[0], epn1 = -Kn.b2 * Wn[1], epn2 = Kn.b3 * Wn[2];
+        s.w[0][row][lane] = Wc[0];
+        s.w[1][row][lane] = Wc[1];
+        s.w[2][row][lane] = Wc[2];
+        s.ep[0][row][lane] = epn1;
+        s.ep[1][row][lane] = epn2;
+        __syncthreads();
+
+        // dω(p) own, clamped forward differences
+        const double* Wx = (p + 1 < Nx) ? Wn : Wc;
+        const double wy0 = has_y1 && row + 1 < TY ? s.w[0][row + 1][lane] : Wc[0];
+        const double wy2 = has_y1 && row + 1 < TY ? s.w[2][row + 1][lane] : Wc[2];
+        const double wz0 = has_z1 && lane + 1 < 64 ? s.w[0][row][lane + 1] : Wc[0];
+        const double wz1 = has_z1 && lane + 1 < 64 ? s.w[1][row][lane + 1] : Wc[1];
+        const double Fxy = (Wx[1] - Wc[1]) - (wy0 - Wc[0]);
+        const double Fxz = (Wx[2] - Wc[2]) - (wz0 - Wc[0]);
+        const double Fyz = (wy2 - Wc[2]) - (wz1 - Wc[1]);
+        const double eta0 = Kc.a1 * Fyz, eta1 = -Kc.a2 * Fxz, eta2 = Kc.a3 * Fxy;
+
+        // δω(p+1) own: −⋆(dᵀ ⋆ω) with zero outside
+        const double ep_y1 = row > 0 ? s.ep[0][row - 1][lane] : 0.0;
+        const double ep_z2 = lane > 0 ? s.ep[1][row][lane - 1] : 0.0;
+        const double H = (epn0 - epc0) - (epn1 - ep_y1) + (epn2 - ep_z2);
+        const double dwn = -Kn.c * H;
+
+        s.eta[0][row][lane] = eta0;
+        s.eta[1][row][lane] = eta1;
+        s.eta[2][row][lane] = eta2;
+        s.dw[buf ^ 1][row][lane] = dwn;
+        __syncthreads();
+
+        if (p >= xa && writer) {
+            const double ey0 = s.eta[0][row - 1][lane], ey2 = s.eta[2][row - 1][lane];
+            const double ez0 = s.eta[0][row][lane - 1], ez1 = s.eta[1][row][lane - 1];
+            const double Gxy = (eta1 - etaPrev[1]) - (eta0 - ey0);
+            const double Gxz = (eta2 - etaPrev[2]) - (eta0 - ez0);
+            const double Gyz = (eta2 - ey2) - (eta1 - ez1);
+            const double ddx = (p + 1 < Nx) ? dwn - dwc : 0.0;
+            const double ddy = has_y1 ? s.dw[buf][row + 1][lane] - dwc : 0.0;
+            const double ddz = has_z1 ? s.dw[buf][row][lane + 1] - dwc : 0.0;
+            double* o = out + ((p * Oy + y) * Oz + z) * 3;
+            o[0] = Kc.a1 * Gyz + ddx;
+            o[1] = -Kc.a2 * Gxz + ddy;
+            o[2] = Kc.a3 * Gxy + ddz;
+        }
+
+        // rotate
+        etaPrev[0] = eta0;
+        etaPrev[1] = eta1;
+        etaPrev[2] = eta2;
+        epc0 = epn0;
+        dwc = dwn;
+        buf ^= 1;
+        Wc[0] = Wn[0];
+        Wc[1] = Wn[1];
+        Wc[2] = Wn[2];
+        Kc = Kn;
+        Wn[0] = Wu[0];
+        Wn[1] = Wu[1];
+        Wn[2] = Wu[2];
+        Kn = coefs_of(Gu, hu, pos_in && p + 2 >= 0 && p + 2 < Nx);
+    };
+    for (;;) {
+        if (p >= xb) break;
+        step(Wf2, Gf2, hf2, Wf, Gf, hf);
+        ++p;
+        if (p >= xb) break;
+        step(Wf, Gf, hf, Wf2, Gf2, hf2);
+        ++p;
+    }
+}
+
 
 
 // ------------------------------------------------- 3D 1-form, march (v1) ---
@@ -828,7 +992,7 @@ inline unsigned blocks_for(int64_t total, int block, int64_t cap = 1 << 20)
     return static_cast<unsigned>(b);
 }
 
-template <int TY>
+template <int TY, int MINW>
 int launch_march0(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk,
                   hipStream_t stream)
 {
@@ -838,11 +1002,32 @@ int launch_march0(const FusedGeom& g, const double* in, double* out, const int64
     const int64_t nb = (int64_t)ntz * nty * nxc;
     SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
     if (g.metric)
-        hipLaunchKernelGGL((k_lap3d_march0<TY, true>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+        hipLaunchKernelGGL((k_lap3d_march0<TY, true, MINW>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
                            g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
                            o[2], ntz, nty, nxc, xchunk);
     else
-        hipLaunchKernelGGL((k_lap3d_march0<TY, false>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+        hipLaunchKernelGGL((k_lap3d_march0<TY, false, MINW>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
+                           o[2], ntz, nty, nxc, xchunk);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+template <int TY>
+int launch_march5(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk,
+                  hipStream_t stream)
+{
+    const int ntz = static_cast<int>((o[2] + 61) / 62);
+    const int nty = static_cast<int>((o[1] + TY - 3) / (TY - 2));
+    const int nxc = static_cast<int>((o[0] + xchunk - 1) / xchunk);
+    const int64_t nb = (int64_t)ntz * nty * nxc;
+    SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
+    if (g.metric)
+        hipLaunchKernelGGL((k_lap3d_march5<TY, true>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
+                           o[2], ntz, nty, nxc, xchunk);
+    else
+        hipLaunchKernelGGL((k_lap3d_march5<TY, false>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
                            g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
                            o[2], ntz, nty, nxc, xchunk);
     SIL_LAUNCH_CHECK();
@@ -912,12 +1097,26 @@ int launch_laplacian_3d_1form(const FusedGeom& g, const double* in, double* out,
         return SIL_OK;
     }
     const int xchunk = tile.xchunk > 0 ? tile.xchunk : 32;
-    if (tile.variant == FUSED_MARCH_V0) {
+    if (tile.variant == FUSED_MARCH_V5) {
         switch (tile.ty) {
         case 8:
-            return launch_march0<8>(g, in, out, o, xchunk, stream);
+            return launch_march5<8>(g, in, out, o, xchunk, stream);
         case 16:
-            return launch_march0<16>(g, in, out, o, xchunk, stream);
+            return launch_march5<16>(g, in, out, o, xchunk, stream);
+        default:
+            set_error("laplacian: unsupported tile height %d (8 or 16)", tile.ty);
+            return SIL_ERR_INVALID;
+        }
+    }
+    if (tile.variant == FUSED_MARCH_V0 || tile.variant == FUSED_MARCH_V0_OCC8) {
+        const bool occ8 = tile.variant == FUSED_MARCH_V0_OCC8;
+        switch (tile.ty) {
+        case 8:
+            return occ8 ? launch_march0<8, 8>(g, in, out, o, xchunk, stream)
+                        : launch_march0<8, 1>(g, in, out, o, xchunk, stream);
+        case 16:
+            return occ8 ? launch_march0<16, 8>(g, in, out, o, xchunk, stream)
+                        : launch_march0<16, 1>(g, in, out, o, xchunk, stream);
         default:
             set_error("laplacian: unsupported tile height %d (8 or 16)", tile.ty);
             return SIL_ERR_INVALID;

@@ -39,6 +39,7 @@ constexpr int CP = 40;     // padded channel stride in LDS
 constexpr int TH = 8;      // output tile rows
 constexpr int TW = 32;     // output tile columns
 constexpr int NT = 256;    // threads per workgroup
+constexpr long kConvGrid = 256 * 2; // persistent convolution workgroups: two per compute unit
 
 using bf16 = __bf16;
 
@@ -117,60 +118,121 @@ __device__ __forceinline__ void stage_tile(bf16* __restrict__ sX, const bf16* __
 
 // Y = act(conv(staged input) + bias) on H×W images: the forward pass, or
 // (ST_MASK, flipped weights, no bias, no ReLU) the input gradient.
+//
+// Persistent workgroups walk the 8×32-pixel output tiles with stride
+// gridDim.x.  Each lane keeps its B fragments (the weights of its output
+// channel, 9 taps × 32 input channels) in registers for the whole walk; the
+// input tile is double-buffered in LDS, the next tile's global loads being
+// issued before the current tile's 36 MFMAs per wavefront; the results go
+// back through the consumed buffer so that every global store is a 16-byte
+// vector store of 8 channels.
+constexpr int kStageChunks = (TH + 2) * (TW + 2) * 4; // 16-byte chunks of one input tile
+constexpr int kPerThread = (kStageChunks + NT - 1) / NT;
+
 template <int MODE>
 __global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ X2,
                                                const bf16* __restrict__ Wt, const float* __restrict__ bias,
-                                               bf16* __restrict__ Y, int H, int W, int relu)
+                                               bf16* __restrict__ Y, int B, int H, int W, int relu)
 {
-    __shared__ __attribute__((aligned(16))) bf16 sX[(TH + 2) * (TW + 2) * CP];
-    __shared__ __attribute__((aligned(16))) bf16 sW[9 * C * CP];
-    const int b = blockIdx.z;
-    const int oy0 = blockIdx.y * TH, ox0 = blockIdx.x * TW;
-    const int tid = threadIdx.x;
-    const size_t img = (size_t)b * H * W * C;
+    __shared__ __attribute__((aligned(16))) bf16 sX[2][(TH + 2) * (TW + 2) * CP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = (W + TW - 1) / TW, tiles_y = (H + TH - 1) / TH;
+    const int ntiles = tiles_x * tiles_y * B;
+    int tile = blockIdx.x;
+    if (tile >= ntiles) return; // uniform per workgroup
 
-    for (int c = tid; c < 9 * C * 4; c += NT) {
-        const int row = c >> 2, part = c & 3; // row = tap*32 + co
-        *reinterpret_cast<Chunk8*>(&sW[row * CP + part * 8]) = *reinterpret_cast<const Chunk8*>(&Wt[row * C + part * 8]);
-    }
-    stage_tile<MODE>(sX, X, X2, b, oy0, ox0, H, W);
-    __syncthreads();
-
-    const int lane = tid & 63, wave = tid >> 6;
-    f32x16 acc[2];
-#pragma unroll
-    for (int m = 0; m < 2; ++m)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[m][r] = 0.f;
     const int col = Mfma32::a_row(lane); // output column (pixel) of the A fragment
     const int k0 = Mfma32::a_k0(lane);
     const int co_b = Mfma32::b_col(lane);
+    bf16x8 bw[9][2];
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-        const int ky = tap / 3, kx = tap % 3;
+    for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
-        for (int kh = 0; kh < 2; ++kh) {
-            const bf16x8 bw = *reinterpret_cast<const bf16x8*>(&sW[(tap * C + co_b) * CP + kh * 16 + k0]);
-#pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                const int tr = 2 * wave + m + ky, tc = col + kx;
-                const bf16x8 ax = *reinterpret_cast<const bf16x8*>(&sX[(tr * (TW + 2) + tc) * CP + kh * 16 + k0]);
-                acc[m] = Mfma32::mma(ax, bw, acc[m]);
-            }
-        }
-    }
+        for (int kh = 0; kh < 2; ++kh)
+            bw[tap][kh] = *reinterpret_cast<const bf16x8*>(&Wt[(tap * C + co_b) * C + kh * 16 + k0]);
     const int co = Mfma32::d_col(lane);
     const float bv = bias ? bias[co] : 0.f;
+
+    auto coords = [&](int t, int& b, int& oy0, int& ox0) {
+        b = t / (tiles_x * tiles_y);
+        oy0 = ((t / tiles_x) % tiles_y) * TH;
+        ox0 = (t % tiles_x) * TW;
+    };
+    auto fetch = [&](int t, Chunk8 (&pre)[kPerThread]) {
+        int b, oy0, ox0;
+        coords(t, b, oy0, ox0);
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
-        const int gy = oy0 + 2 * wave + m;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int gx = ox0 + Mfma32::d_row(lane, r);
-            float v = acc[m][r] + bv;
-            if (relu) v = fmaxf(v, 0.f);
-            if (gy < H && gx < W) Y[img + ((size_t)gy * W + gx) * C + co] = to_bf16(v);
+        for (int k = 0; k < kPerThread; ++k) {
+            const int c = tid + k * NT;
+            pre[k] = zero_chunk();
+            if (c < kStageChunks) {
+                const int pix = c >> 2, part = c & 3;
+                const int gy = oy0 - 1 + pix / (TW + 2), gx = ox0 - 1 + pix % (TW + 2);
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W) pre[k] = stage_chunk<MODE>(X, X2, b, gy, gx, part, H, W);
+            }
         }
+    };
+    auto put = [&](bf16* dst, const Chunk8 (&pre)[kPerThread]) {
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) {
+            const int c = tid + k * NT;
+            if (c < kStageChunks) *reinterpret_cast<Chunk8*>(&dst[(c >> 2) * CP + (c & 3) * 8]) = pre[k];
+        }
+    };
+
+    Chunk8 pre[kPerThread];
+    fetch(tile, pre);
+    put(sX[0], pre);
+    __syncthreads();
+    int buf = 0;
+    for (; tile < ntiles; tile += gridDim.x) {
+        const int next = tile + gridDim.x;
+        if (next < ntiles) fetch(next, pre);
+
+        f32x16 acc[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[m][r] = 0.f;
+        const bf16* cur = sX[buf];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+#pragma unroll
+            for (int kh = 0; kh < 2; ++kh) {
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const int tr = 2 * wave + m + ky, tc = col + kx;
+                    const bf16x8 ax = *reinterpret_cast<const bf16x8*>(&cur[(tr * (TW + 2) + tc) * CP + kh * 16 + k0]);
+                    acc[m] = Mfma32::mma(ax, bw[tap][kh], acc[m]);
+                }
+            }
+        }
+        __syncthreads(); // every wavefront is done with the input tile in sX[buf]
+
+        // results → sX[buf] as [8 rows][32 pixels][32 channels] (stride CP)
+        bf16* out_s = sX[buf];
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                float v = acc[m][r] + bv;
+                if (relu) v = fmaxf(v, 0.f);
+                out_s[((2 * wave + m) * TW + Mfma32::d_row(lane, r)) * CP + co] = to_bf16(v);
+            }
+        if (next < ntiles) put(sX[buf ^ 1], pre);
+        __syncthreads();
+
+        int b, oy0, ox0;
+        coords(tile, b, oy0, ox0);
+        for (int c = tid; c < TH * TW * 4; c += NT) {
+            const int pix = c >> 2, part = c & 3;
+            const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
+            if (gy < H && gx < W)
+                *reinterpret_cast<Chunk8*>(&Y[(((size_t)b * H + gy) * W + gx) * C + part * 8]) =
+                    *reinterpret_cast<const Chunk8*>(&out_s[pix * CP + part * 8]);
+        }
+        buf ^= 1;
     }
 }
 
@@ -619,9 +681,13 @@ int by_mode(int mode, F&& f)
 int conv33(int mode, const bf16* X, const bf16* X2, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W,
            bool relu, hipStream_t s)
 {
+    // persistent grid: two workgroups per compute unit (the LDS of a
+    // workgroup is 54 KB), capped by the number of tiles
+    const long ntiles = (long)((W + TW - 1) / TW) * ((H + TH - 1) / TH) * B;
+    const unsigned grid = (unsigned)(ntiles < kConvGrid ? ntiles : kConvGrid);
     return by_mode(mode, [&](auto m) -> int {
-        hipLaunchKernelGGL(k_conv33<decltype(m)::value>, dim3((W + TW - 1) / TW, (H + TH - 1) / TH, B), dim3(NT), 0,
-                           s, X, X2, Wt, bias, Y, H, W, relu ? 1 : 0);
+        hipLaunchKernelGGL(k_conv33<decltype(m)::value>, dim3(grid), dim3(NT), 0, s, X, X2, Wt, bias, Y, B, H, W,
+                           relu ? 1 : 0);
         SIL_LAUNCH_CHECK();
         return (int)SIL_OK;
     });

@@ -79,15 +79,17 @@ def jacobi_inverse(kappa):
     return d
 
 
-def generate_pairs(n_problems, n, sigma, corr, seed, pairs_per_problem, batch=64, record_every=16, max_records=400,
-                   log=print):
+def generate_pairs(n_problems, n, sigma, corr, seed, pairs_per_problem, batch=64, record_every=8, max_records=400,
+                   net=None, max_iterations=20000, log=print):
     """Training pairs of the network as the solver uses it.
 
     Jacobi CG solves every problem to 1e-10 while recording its iterates
-    x_k.  For each picked record the pair is formed after the solver's
-    pre-smoothing step: r = b − A x_k, e = x* − x_k, s = ω D⁻¹ r,
-    r₁ = r − A s, e₁ = e − s (so A e₁ = r₁), both scaled by 1/rms(r₁).
-    Returns (inputs [P][n][n][2], errors [P][n][n], wx, wy, problem index)."""
+    x_k (with `net`, flexible CG with that network records instead, and a
+    separate CG solve provides x*).  For each picked record the pair is
+    formed after the solver's pre-smoothing step: r = b − A x_k,
+    e = x* − x_k, s = ω D⁻¹ r, r₁ = r − A s, e₁ = e − s (so A e₁ = r₁), both
+    scaled by 1/rms(r₁).  Returns (inputs [P][n][n][2], errors [P][n][n],
+    wx, wy, problem index)."""
     inputs, errors, wxs, wys, owners = [], [], [], [], []
     for start in range(0, n_problems, batch):
         B = min(batch, n_problems - start)
@@ -96,10 +98,17 @@ def generate_pairs(n_problems, n, sigma, corr, seed, pairs_per_problem, batch=64
             raise RuntimeError(f"non-finite coefficients for seed {seed + start}")
         pb = Lr.PoissonBatch(kappa)
         t0 = time.time()
-        x, res, secs, (rr, xr) = pb.solve(b, method="cg", tol=1e-10, record_every=record_every,
-                                          max_records=max_records)
+        if net is None:
+            x, res, secs, (rr, xr) = pb.solve(b, method="cg", tol=1e-10, record_every=record_every,
+                                              max_records=max_records)
+            ok = np.array([r["converged"] for r in res])
+        else:
+            x, res0, _ = pb.solve(b, method="cg", tol=1e-10)
+            ok = np.array([r["converged"] for r in res0])
+            _, res, secs, (rr, xr) = pb.solve(b, method="fcg_learned", tol=1e-10, net=net,
+                                              record_every=record_every, max_records=max_records,
+                                              max_iterations=max_iterations)
         its = np.array([r["iterations"] for r in res])
-        ok = np.array([r["converged"] for r in res])
         log(f"pairs: problems {start}..{start + B - 1}: CG iterations {its.min()}..{its.max()}, "
             f"{int(ok.sum())}/{B} converged, {xr.shape[0]} records, {time.time() - t0:.1f} s")
         wx, wy, lk = pb.weights()
@@ -196,6 +205,9 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--eval-seed", type=int, default=900000)
     ap.add_argument("--eval-problems", type=int, default=64)
+    ap.add_argument("--solver-rounds", type=int, default=0,
+                    help="extra rounds on pairs recorded along flexible CG with the current network")
+    ap.add_argument("--solver-epochs", type=int, default=4)
     ap.add_argument("--out", type=Path, default=DEFAULT_WEIGHTS)
     ap.add_argument("--log", type=Path, default=None)
     a = ap.parse_args(argv)
@@ -210,8 +222,18 @@ def main(argv=None):
     t0 = time.time()
     data = generate_pairs(a.problems, a.n, a.sigma, a.corr, a.seed, a.pairs_per_problem, log=log)
     log(f"{data[0].shape[0]} pairs in {time.time() - t0:.0f} s")
-    net = Lr.UNet(max(a.batch, a.eval_problems), a.n, a.n, seed=a.seed)
+    net = Lr.UNet(max(a.batch, a.eval_problems, 64), a.n, a.n, seed=a.seed)
     hist = train(net, data, a.epochs, batch=a.batch, lr_max=a.lr, log=log)
+    # optional further rounds on the residuals flexible CG itself meets
+    for rnd in range(a.solver_rounds):
+        t0 = time.time()
+        more = generate_pairs(a.problems, a.n, a.sigma, a.corr, a.seed, a.pairs_per_problem // 2, net=net,
+                              record_every=4, max_records=300, max_iterations=1200, log=log)
+        data = tuple(torch.cat([u, v]) if k != 2 and k != 3 else u for k, (u, v) in enumerate(zip(data, more)))
+        log(f"round {rnd + 1}: {more[0].shape[0]} pairs from flexible CG ({time.time() - t0:.0f} s), "
+            f"{data[0].shape[0]} in total")
+        del more
+        hist += train(net, data, a.solver_epochs, batch=a.batch, lr_max=a.lr / 3, log=log)
     del data
     params = net.get_params().cpu().numpy()
     a.out.parent.mkdir(parents=True, exist_ok=True)
