@@ -216,9 +216,10 @@ int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs,
  * in a hipGraph.  Problem p has sides[p]² nodes (x slowest) stored at offset
  * sil_batched_poisson2d_offset(h, p) of every concatenated array; kappa is
  * the nodal coefficient (device, concatenated; edge value = mean of its two
- * nodes).  The solve is CG with FP32 operator applications and recurrence
- * and FP64 x, dots and residual replacement; a problem converges only on its
- * FP64 residual ‖b − A x‖2 ≤ tol·‖b‖2. */
+ * nodes).  The solve is CG (settings.preconditioner: Jacobi or identity)
+ * with FP32 operator applications and recurrence and FP64 x, dots and
+ * residual replacement; a problem converges only on its FP64 residual
+ * ‖b − A x‖2 ≤ tol·‖b‖2. */
 typedef struct sil_batch_result {
     int32_t iterations;
     int32_t converged;

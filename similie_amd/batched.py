@@ -51,7 +51,7 @@ class BatchedPoisson2D:
 
     def solve(self, rhs, solution=None, settings=None, stream=None):
         settings = settings or StrongFormulationSolverSettings(relative_tolerance=1e-10, precision="mixed",
-                                                               preconditioner="identity", max_iterations=20000)
+                                                               preconditioner="jacobi", max_iterations=20000)
         solution = torch.empty_like(rhs) if solution is None else solution
         res = (L.BatchResult * len(self.sides))()
         secs = ctypes.c_double()

@@ -89,6 +89,8 @@ struct BatchArgs {
     float* r;
     float* p;
     float* q;
+    const float* dinv32; // Jacobi: inverse diagonal (0 on the boundary ring)
+    int jacobi;
     double tol;
     int max_it;
     int replace;
@@ -121,20 +123,26 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
         float* p = a.p + off;
         float* q = a.q + off;
 
-        // x = 0, r = p = b (FP32), <b,b>
+        const float* dinv = a.dinv32 + off;
+        const bool jac = a.jacobi != 0;
+        // z = D⁻¹ r (Jacobi) or r
+        auto zval = [&](int t, float rf) { return jac ? rf * dinv[t] : rf; };
+
+        // x = 0, r = b (FP32), p = z; <b,b>, <r,z>
         double bb = 0.0, rz = 0.0;
         for (int t = threadIdx.x; t < nn; t += BLOCK) {
             const double bi = b[t];
             x[t] = 0.0;
             const float rf = (float)bi;
             r[t] = rf;
-            p[t] = rf;
+            const float zf = zval(t, rf);
+            p[t] = zf;
             bb += bi * bi;
-            rz += (double)rf * (double)rf;
+            rz += (double)rf * (double)zf;
         }
         wg_sum2<BLOCK>(bb, rz, sh);
         const double tol2bb = a.tol * a.tol * bb;
-        double rz_ref = rz; // <r,r> at the last replacement
+        double rr_ref = bb; // <r,r> at the last replacement
         int it = 0, since = 0;
         bool converged = bb == 0.0;
         double rr_true = bb;
@@ -150,14 +158,15 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
             }
             pq = wg_sum<BLOCK>(pq, sh);
             const double alpha = rz / pq;
-            double rz_new = 0.0, dummy = 0.0;
+            double rz_new = 0.0, rr_new = 0.0;
             for (int t = threadIdx.x; t < nn; t += BLOCK) {
                 x[t] += alpha * (double)p[t];
                 const float rf = (float)((double)r[t] - alpha * (double)q[t]);
                 r[t] = rf;
-                rz_new += (double)rf * (double)rf;
+                rz_new += (double)rf * (double)zval(t, rf);
+                rr_new += (double)rf * (double)rf;
             }
-            wg_sum2<BLOCK>(rz_new, dummy, sh);
+            wg_sum2<BLOCK>(rz_new, rr_new, sh);
             ++it;
             ++since;
             const double rz_old = rz;
@@ -166,7 +175,7 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
             // replace when the recurrence claims convergence, when its residual fell
             // 100-fold since the last replacement (FP32 drift grows with that
             // ratio), or every `replace` iterations
-            if (rz_new <= tol2bb || rz_new < 1e-4 * rz_ref || since >= a.replace) {
+            if (rr_new <= tol2bb || rr_new < 1e-4 * rr_ref || since >= a.replace) {
                 // FP64 residual replacement b − A x
                 __syncthreads();
                 double rr = 0.0, rzr = 0.0;
@@ -177,7 +186,7 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
                     const float rf = (float)res;
                     r[t] = rf;
                     rr += res * res;
-                    rzr += (double)rf * (double)rf;
+                    rzr += (double)rf * (double)zval(t, rf);
                 }
                 wg_sum2<BLOCK>(rr, rzr, sh);
                 rr_true = rr;
@@ -186,11 +195,11 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
                     converged = true;
                     break;
                 }
-                beta = rzr / rz_old; // replaced <r,r> over the previous iteration's <r,r>
+                beta = rzr / rz_old; // replaced <r,z> over the previous iteration's <r,z>
                 rz = rzr;
-                rz_ref = rzr;
+                rr_ref = rr;
             }
-            for (int t = threadIdx.x; t < nn; t += BLOCK) p[t] = r[t] + (float)beta * p[t];
+            for (int t = threadIdx.x; t < nn; t += BLOCK) p[t] = zval(t, r[t]) + (float)beta * p[t];
         }
         if (!converged) {
             // report the FP64 residual of the final iterate
@@ -218,7 +227,7 @@ __global__ void k_reset_queue(int* q) { *q = 0; }
 
 __global__ void k_batch_weights(const BatchProblem* __restrict__ problems, int nprob, const double* __restrict__ kappa,
                                 float* __restrict__ wx32, float* __restrict__ wy32, double* __restrict__ wx64,
-                                double* __restrict__ wy64)
+                                double* __restrict__ wy64, float* __restrict__ dinv32)
 {
     const int id = blockIdx.y;
     if (id >= nprob) return;
@@ -235,6 +244,11 @@ __global__ void k_batch_weights(const BatchProblem* __restrict__ problems, int n
         wy64[off + t] = ay;
         wx32[off + t] = (float)ax;
         wy32[off + t] = (float)ay;
+        // Jacobi: the diagonal Σ of the four edge weights of an interior node
+        double d = 0.0;
+        if (i > 0 && j > 0 && i + 1 < n && j + 1 < n)
+            d = 1.0 / (ax + 0.5 * (k[t] + k[t - n]) + ay + 0.5 * (k[t] + k[t - 1]));
+        dinv32[off + t] = (float)d;
     }
 }
 
@@ -249,12 +263,12 @@ int launch_block(const BatchArgs& a, int grid, hipStream_t s)
 }  // namespace
 
 int batched_weights(const BatchProblem* problems_dev, int nprob, int max_side, const double* kappa, float* wx32,
-                    float* wy32, double* wx64, double* wy64, hipStream_t s)
+                    float* wy32, double* wx64, double* wy64, float* dinv32, hipStream_t s)
 {
     const int64_t per = (int64_t)max_side * max_side;
     const unsigned gx = (unsigned)std::min<int64_t>((per + 255) / 256, 1024);
     hipLaunchKernelGGL(k_batch_weights, dim3(gx, nprob), dim3(256), 0, s, problems_dev, nprob, kappa, wx32, wy32,
-                       wx64, wy64);
+                       wx64, wy64, dinv32);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
@@ -262,7 +276,7 @@ int batched_weights(const BatchProblem* problems_dev, int nprob, int max_side, c
 int batched_enqueue(const BatchLaunch& L, hipStream_t s)
 {
     BatchArgs a{L.problems, L.order, L.nprob, L.queue, L.wx32, L.wy32, L.wx64, L.wy64, L.b, L.x,
-                L.r, L.p, L.q, L.tol, L.max_it, L.replace, L.results};
+                L.r, L.p, L.q, L.dinv32, L.jacobi, L.tol, L.max_it, L.replace, L.results};
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, L.queue);
     SIL_LAUNCH_CHECK();
     switch (L.block) {

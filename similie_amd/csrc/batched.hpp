@@ -27,6 +27,8 @@ struct BatchLaunch {
     const double* b;
     double* x;
     float *r, *p, *q;
+    const float* dinv32;
+    int jacobi;
     double tol;
     int max_it;
     int replace;
@@ -36,7 +38,7 @@ struct BatchLaunch {
 };
 
 int batched_weights(const BatchProblem* problems_dev, int nprob, int max_side, const double* kappa, float* wx32,
-                    float* wy32, double* wx64, double* wy64, hipStream_t s);
+                    float* wy32, double* wx64, double* wy64, float* dinv32, hipStream_t s);
 int batched_enqueue(const BatchLaunch& L, hipStream_t s);
 int batched_occupancy(int block, int* blocks_per_cu);
 

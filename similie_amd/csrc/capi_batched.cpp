@@ -18,14 +18,15 @@ struct sil_batched_poisson_s {
     BatchProblem* problems_dev = nullptr;
     int* order_dev = nullptr;
     int* queue = nullptr;
-    float *wx32 = nullptr, *wy32 = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+    float *wx32 = nullptr, *wy32 = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *dinv32 = nullptr;
     double *wx64 = nullptr, *wy64 = nullptr, *b = nullptr, *x = nullptr;
     BatchResult* results_dev = nullptr;
     int block = 256;
     int cus = 0;
     hipGraphExec_t graph = nullptr;
     int graph_block = 0;
-    int graph_max_it = -1, graph_replace = -1;
+    int graph_max_it = -1, graph_replace = -1, graph_jacobi = -1;
+    int jacobi = 1; // preconditioner of the captured solve
     double graph_tol = -1.0;
 };
 
@@ -35,6 +36,7 @@ void release(sil_batched_poisson h)
 {
     if (!h) return;
     for (void* ptr : {(void*)h->problems_dev, (void*)h->order_dev, (void*)h->queue, (void*)h->wx32, (void*)h->wy32,
+                      (void*)h->dinv32,
                       (void*)h->r, (void*)h->p, (void*)h->q, (void*)h->wx64, (void*)h->wy64, (void*)h->b,
                       (void*)h->x, (void*)h->results_dev})
         if (ptr) (void)hipFree(ptr);
@@ -74,6 +76,8 @@ int enqueue(sil_batched_poisson h, int block, double tol, int max_it, int replac
     L.r = h->r;
     L.p = h->p;
     L.q = h->q;
+    L.dinv32 = h->dinv32;
+    L.jacobi = h->jacobi;
     L.tol = tol;
     L.max_it = max_it;
     L.replace = replace;
@@ -87,7 +91,7 @@ int enqueue(sil_batched_poisson h, int block, double tol, int max_it, int replac
 int capture(sil_batched_poisson h, int block, double tol, int max_it, int replace)
 {
     if (h->graph && h->graph_block == block && h->graph_tol == tol && h->graph_max_it == max_it &&
-        h->graph_replace == replace)
+        h->graph_replace == replace && h->graph_jacobi == h->jacobi)
         return SIL_OK;
     if (h->graph) {
         (void)hipGraphExecDestroy(h->graph);
@@ -114,6 +118,7 @@ int capture(sil_batched_poisson h, int block, double tol, int max_it, int replac
     }
     h->graph_block = block;
     h->graph_tol = tol;
+    h->graph_jacobi = h->jacobi;
     h->graph_max_it = max_it;
     h->graph_replace = replace;
     return SIL_OK;
@@ -151,7 +156,7 @@ int sil_batched_poisson2d_create(sil_batched_poisson* out, int32_t nproblems, co
         (rc = dalloc(&h->queue, 1)) || (rc = dalloc(&h->wx32, off)) || (rc = dalloc(&h->wy32, off)) ||
         (rc = dalloc(&h->r, off)) || (rc = dalloc(&h->p, off)) || (rc = dalloc(&h->q, off)) ||
         (rc = dalloc(&h->wx64, off)) || (rc = dalloc(&h->wy64, off)) || (rc = dalloc(&h->b, off)) ||
-        (rc = dalloc(&h->x, off)) || (rc = dalloc(&h->results_dev, nproblems))) {
+        (rc = dalloc(&h->x, off)) || (rc = dalloc(&h->dinv32, off)) || (rc = dalloc(&h->results_dev, nproblems))) {
         release(h);
         return rc;
     }
@@ -166,7 +171,8 @@ int sil_batched_poisson2d_create(sil_batched_poisson* out, int32_t nproblems, co
         set_error("batched_poisson2d_create: %s", hipGetErrorString(e));
         return SIL_ERR_HIP;
     }
-    rc = batched_weights(h->problems_dev, nproblems, h->max_side, kappa, h->wx32, h->wy32, h->wx64, h->wy64, 0);
+    rc = batched_weights(h->problems_dev, nproblems, h->max_side, kappa, h->wx32, h->wy32, h->wx64, h->wy64,
+                         h->dinv32, 0);
     if (rc == SIL_OK && hipDeviceSynchronize() != hipSuccess) rc = SIL_ERR_HIP;
     if (rc) {
         release(h);
@@ -197,8 +203,11 @@ int sil_batched_poisson2d_solve(sil_batched_poisson h, const double* rhs, double
     SIL_REQUIRE(h && rhs && solution && st, SIL_ERR_INVALID, "batched_poisson2d_solve: null argument");
     SIL_REQUIRE(st->precision == SIL_PRECISION_MIXED || st->precision == SIL_PRECISION_FP64, SIL_ERR_INVALID,
                 "batched_poisson2d_solve: unknown precision");
+    SIL_REQUIRE(st->preconditioner == SIL_PRECOND_IDENTITY || st->preconditioner == SIL_PRECOND_JACOBI,
+                SIL_ERR_INVALID, "batched_poisson2d_solve: preconditioner must be identity or Jacobi");
     hipStream_t s = as_stream(stream);
     const int replace = st->replace_interval > 0 ? st->replace_interval : 400;
+    h->jacobi = st->preconditioner == SIL_PRECOND_JACOBI ? 1 : 0;
     int rc = capture(h, h->block, st->relative_tolerance, st->max_iterations, replace);
     if (rc) return rc;
     SIL_CHECK_HIP(hipMemcpyAsync(h->b, rhs, sizeof(double) * h->total, hipMemcpyDeviceToDevice, s));

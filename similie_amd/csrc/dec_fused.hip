@@ -530,8 +530,8 @@ struct MarchLds1 {
     double dw[2][TY][64];  // δω ping-pong: planes p and p+1
 };
 
-template <int TY, bool DIAG>
-__global__ __launch_bounds__(64 * TY) void k_lap3d_march1(const double* __restrict__ w, const double* __restrict__ g,
+template <int TY, bool DIAG, int MINW>
+__global__ __launch_bounds__(64 * TY, MINW) void k_lap3d_march1(const double* __restrict__ w, const double* __restrict__ g,
                                                          const double* __restrict__ hxs, const double* __restrict__ hys,
                                                          const double* __restrict__ hzs, double* __restrict__ out,
                                                          int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
@@ -1034,7 +1034,7 @@ int launch_march5(const FusedGeom& g, const double* in, double* out, const int64
     return SIL_OK;
 }
 
-template <int TY>
+template <int TY, int MINW>
 int launch_march1(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk,
                   hipStream_t stream)
 {
@@ -1044,11 +1044,11 @@ int launch_march1(const FusedGeom& g, const double* in, double* out, const int64
     const int64_t nb = (int64_t)ntz * nty * nxc;
     SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
     if (g.metric)
-        hipLaunchKernelGGL((k_lap3d_march1<TY, true>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+        hipLaunchKernelGGL((k_lap3d_march1<TY, true, MINW>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
                            g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
                            o[2], ntz, nty, nxc, xchunk);
     else
-        hipLaunchKernelGGL((k_lap3d_march1<TY, false>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+        hipLaunchKernelGGL((k_lap3d_march1<TY, false, MINW>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
                            g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
                            o[2], ntz, nty, nxc, xchunk);
     SIL_LAUNCH_CHECK();
@@ -1122,18 +1122,23 @@ int launch_laplacian_3d_1form(const FusedGeom& g, const double* in, double* out,
             return SIL_ERR_INVALID;
         }
     }
-    if (tile.variant == FUSED_MARCH) {
+    if (tile.variant == FUSED_MARCH || tile.variant == FUSED_MARCH_OCC6) {
+        // FUSED_MARCH_OCC6 caps the registers at 80 so that six wavefronts
+        // share each SIMD (a few spills to scratch)
+        const bool o6 = tile.variant == FUSED_MARCH_OCC6;
+#define SIL_MARCH1(T) return o6 ? launch_march1<T, 6>(g, in, out, o, xchunk, stream) : launch_march1<T, 1>(g, in, out, o, xchunk, stream)
         switch (tile.ty) {
-        case 4:
-            return launch_march1<4>(g, in, out, o, xchunk, stream);
-        case 8:
-            return launch_march1<8>(g, in, out, o, xchunk, stream);
-        case 16:
-            return launch_march1<16>(g, in, out, o, xchunk, stream);
+        case 4: SIL_MARCH1(4);
+        case 6: SIL_MARCH1(6);
+        case 8: SIL_MARCH1(8);
+        case 10: SIL_MARCH1(10);
+        case 12: SIL_MARCH1(12);
+        case 16: SIL_MARCH1(16);
         default:
-            set_error("laplacian: unsupported tile height %d (4, 8 or 16)", tile.ty);
+            set_error("laplacian: unsupported tile height %d (4, 6, 8, 10, 12 or 16)", tile.ty);
             return SIL_ERR_INVALID;
         }
+#undef SIL_MARCH1
     }
     SIL_REQUIRE(tile.variant == FUSED_MARCH_DEEP || tile.variant == FUSED_MARCH_CLAMPED, SIL_ERR_INVALID,
                 "laplacian: unknown kernel variant %d", tile.variant);

@@ -208,6 +208,7 @@ def main(argv=None):
     ap.add_argument("--solver-rounds", type=int, default=0,
                     help="extra rounds on pairs recorded along flexible CG with the current network")
     ap.add_argument("--solver-epochs", type=int, default=4)
+    ap.add_argument("--init", type=Path, default=None, help="start from these parameters (.npy)")
     ap.add_argument("--out", type=Path, default=DEFAULT_WEIGHTS)
     ap.add_argument("--log", type=Path, default=None)
     a = ap.parse_args(argv)
@@ -223,7 +224,10 @@ def main(argv=None):
     data = generate_pairs(a.problems, a.n, a.sigma, a.corr, a.seed, a.pairs_per_problem, log=log)
     log(f"{data[0].shape[0]} pairs in {time.time() - t0:.0f} s")
     net = Lr.UNet(max(a.batch, a.eval_problems, 64), a.n, a.n, seed=a.seed)
-    hist = train(net, data, a.epochs, batch=a.batch, lr_max=a.lr, log=log)
+    if a.init is not None:
+        net.set_params(torch.from_numpy(np.load(a.init)))
+        log(f"initial parameters from {a.init}")
+    hist = train(net, data, a.epochs, batch=a.batch, lr_max=a.lr, log=log) if a.epochs > 0 else []
     # optional further rounds on the residuals flexible CG itself meets
     for rnd in range(a.solver_rounds):
         t0 = time.time()

@@ -9,8 +9,8 @@
   in the square coil annulus between half-widths 32 and 40 cells, 64 cells
   high, centred.
 * uneven_poisson_batch: 512 problems with sides log-uniform in [32, 512],
-  κ = exp(g), g a Gaussian random field (σ = 1, correlation 8 cells),
-  N(0, 1) right-hand sides.
+  κ = exp(g), g a Gaussian random field (σ = 1, correlation 8 cells; disjoint
+  windows of large periodic fields), N(0, 1) right-hand sides.
 The learned-preconditioner problems are in preconditioner_training.
 """
 import math
@@ -74,17 +74,48 @@ def gaussian_random_field(shape, sigma, corr, generator, device="cuda"):
     return sigma * g
 
 
-def uneven_poisson_batch(count=512, lo=32, hi=512, sigma=1.0, corr=8.0, seed=4242, device="cuda"):
+def periodic_random_fields(count, size, sigma, corr, generator, device="cuda"):
+    """`count` periodic fields of size × size (one FFT size for all: the FFT
+    library compiles a kernel per transform size), covariance
+    σ² exp(−r²/(2ℓ²)), each rescaled to mean 0 / standard deviation σ."""
+    white = torch.randn((count, size, size), generator=generator, dtype=torch.float64, device=device)
+    k = torch.fft.fftfreq(size, device=device, dtype=torch.float64) * 2 * math.pi
+    amp = torch.exp(-0.25 * (k[:, None] ** 2 + k[None, :] ** 2) * corr * corr)
+    g = torch.fft.ifft2(torch.fft.fft2(white) * amp).real
+    g = (g - g.mean(dim=(1, 2), keepdim=True)) / g.std(dim=(1, 2), keepdim=True)
+    return sigma * g
+
+
+def uneven_poisson_batch(count=512, lo=32, hi=512, sigma=1.0, corr=8.0, seed=4242, device="cuda", field=2048):
     """(sides, κ concatenated, rhs concatenated) in the layout of
-    sil_batched_poisson2d (each problem n_p × n_p, x slowest)."""
+    sil_batched_poisson2d (each problem n_p × n_p, x slowest).
+
+    The coefficient fields are disjoint windows of a few large periodic
+    Gaussian random fields (shelf-packed, largest first): windows far apart
+    compared with the correlation length are independent samples of the same
+    field, and one transform size serves every problem."""
     rng = np.random.default_rng(seed)
     sides = np.exp(rng.uniform(math.log(lo), math.log(hi), count)).round().astype(np.int32)
     sides = np.clip(sides, lo, hi)
+    # shelf packing of the windows into field × field sheets
+    place = [None] * count
+    sheet, x, y, shelf = 0, 0, 0, 0
+    for p in sorted(range(count), key=lambda q: -int(sides[q])):
+        n = int(sides[p])
+        if x + n > field:
+            x, y, shelf = 0, y + shelf, 0
+        if y + n > field:
+            sheet, x, y, shelf = sheet + 1, 0, 0, 0
+        place[p] = (sheet, x, y)
+        x += n
+        shelf = max(shelf, n)
     gen = torch.Generator(device=device).manual_seed(seed)
+    fields = periodic_random_fields(sheet + 1, field, sigma, corr, gen, device)
     kappas, rhss = [], []
-    for n in sides:
-        n = int(n)
-        kappas.append(torch.exp(gaussian_random_field((n, n), sigma, corr, gen, device)).reshape(-1))
+    for p in range(count):
+        n = int(sides[p])
+        s_, x0, y0 = place[p]
+        kappas.append(torch.exp(fields[s_, y0:y0 + n, x0:x0 + n]).reshape(-1))
         b = torch.randn((n, n), generator=gen, dtype=torch.float64, device=device)
         b[0, :] = b[-1, :] = 0
         b[:, 0] = b[:, -1] = 0

@@ -206,7 +206,7 @@ def test_laplacian_matches_oracle(ext, n, k, kind):
         assert rel_err(out, orc.laplacian(og, k, w, oext)) <= TOL
 
 
-TILES = ["0,8,32", "1,4,3", "1,8,32", "1,8,5", "1,16,7", "1,8,1", "2,4,3", "2,8,32", "2,8,2", "2,16,7", "2,16,1", "3,8,5", "3,16,32"]
+TILES = ["0,8,32", "1,4,3", "1,8,32", "1,8,5", "1,16,7", "1,8,1", "1,12,32", "1,12,5", "1,6,3", "1,10,32", "2,4,3", "2,8,32", "2,16,7", "3,8,5"]
 
 
 @pytest.mark.parametrize("tile", TILES)

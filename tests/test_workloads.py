@@ -37,3 +37,11 @@ def test_gaussian_random_field_statistics():
     assert float(g.std()) == pytest.approx(1.5, rel=1e-12)
     # correlated: neighbouring values are close compared with the spread
     assert float((g[1:] - g[:-1]).std()) < 0.5 * 1.5
+
+
+def test_uneven_batch_windows_are_disjoint_and_sized():
+    sides, kappa, rhs = W.uneven_poisson_batch(count=40, lo=16, hi=96, device="cpu", field=256)
+    assert len(sides) == 40 and sides.min() >= 16 and sides.max() <= 96
+    assert kappa.numel() == rhs.numel() == int((sides.astype(np.int64) ** 2).sum())
+    assert bool((kappa > 0).all())
+    assert float(torch.log(kappa).std()) == pytest.approx(1.0, rel=0.25)
