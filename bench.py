@@ -263,6 +263,20 @@ def main():
     torch.cuda.synchronize()
     copy_gbps = 2 * a.numel() * 8 / (e0.elapsed_time(e1) / 20 * 1e-3) / 1e9
     del a, b2
+    # the Laplacian's byte mix as plain streaming: read two arrays, write one
+    # (torch's elementwise add), B bytes in total
+    a1 = torch.ones(B // 24, dtype=torch.float64, device="cuda")
+    a2 = torch.ones_like(a1)
+    c3 = torch.empty_like(a1)
+    for _ in range(3):
+        torch.add(a1, a2, out=c3)
+    e0.record()
+    for _ in range(20):
+        torch.add(a1, a2, out=c3)
+    e1.record()
+    torch.cuda.synchronize()
+    mix_gbps = 3 * a1.numel() * 8 / (e0.elapsed_time(e1) / 20 * 1e-3) / 1e9
+    del a1, a2, c3
 
     line = {
         "metric": METRIC,
@@ -285,6 +299,7 @@ def main():
             "algorithmic_bytes_per_step": B,
             "hbm_peak_fraction": round(value / HBM_PEAK_GBPS, 4),
             "torch_copy_GBps": round(copy_gbps, 1),
+            "torch_read2_write1_GBps": round(mix_gbps, 1),
         },
         "roofline": {
             "bound": "hbm",

@@ -266,16 +266,18 @@ struct sil_staged_laplacian_s {
 
 namespace sil {
 // Tile configuration of the fused 3D Laplacian; set by the autotuner or by
-// the environment (SIL_LAP3D_TILE="variant,ty,xchunk").
+// the environment (SIL_LAP3D_TILE="variant,ty,xchunk[,zw]").
 FusedTile default_lap3d_tile()
 {
-    FusedTile t{FUSED_MARCH, 64, 16, 32};
+    FusedTile t{FUSED_PIPE, 64, 12, 32};
     if (const char* e = std::getenv("SIL_LAP3D_TILE")) {
-        int v = 1, ty = 8, xc = 32;
-        if (std::sscanf(e, "%d,%d,%d", &v, &ty, &xc) == 3) {
+        int v = 1, ty = 8, xc = 32, zw = 64;
+        const int n = std::sscanf(e, "%d,%d,%d,%d", &v, &ty, &xc, &zw);
+        if (n >= 3) {
             t.variant = v;
             t.ty = ty;
             t.xchunk = xc;
+            t.tz = n == 4 ? zw : 64;
         }
     }
     return t;

@@ -19,14 +19,19 @@
 // treats as degenerate (a reconstruction Jacobian below 1e-14, a zero metric
 // entry) get zero coefficients, as in the reference.
 //
-// The 3D kernel marches along x (the slowest axis): a workgroup owns a
-// 64 (z, one wavefront) × TY (y) tile including a one-node halo, keeps the
-// planes x−1, x, x+1 in registers, prefetches plane x+2 while it computes,
+// The 3D kernels march along x (the slowest axis): a workgroup owns a tile of
+// rows (y) × lanes (z) including a one-node halo, keeps the planes x−1, x,
+// x+1 in registers, has the next planes' loads in flight while it computes,
 // and exchanges the y/z neighbours of ω, ⋆ω, ⋆dω and δω through LDS.  Each
 // node's coefficients are computed once per tile.  Workgroups are mapped to
 // tiles XCD-contiguously so the halos shared by neighbouring tiles are served
-// from the same L2.
+// from the same L2.  The pipelined variant addresses memory through buffer
+// resources so that no branch separates a load from its consumer: the
+// compiler's wait counts then let a step's loads stay in flight across the
+// whole step instead of being drained at its top.
 #include "dec_kernels.hpp"
+
+#include <utility>
 
 namespace sil {
 
@@ -170,374 +175,27 @@ __global__ __launch_bounds__(256) void k_lap3d_direct(const double* __restrict__
     }
 }
 
-// --------------------------------------- 3D 1-form, march (v0, comparison) ---
-
-__device__ __forceinline__ Coefs node_coefs0(double gx, double gy, double gz, double hx, double hy, double hz)
-{
-    Coefs k;
-    const bool metric_ok = (gx != 0.0) && (gy != 0.0) && (gz != 0.0);
-    const double ax = fabs(hx), ay = fabs(hy), az = fabs(hz);
-    const bool ok1 = metric_ok && ax >= 1e-14 && ay >= 1e-14 && az >= 1e-14;
-    const bool ok2 = metric_ok && fabs(hx * hy) >= 1e-14 && fabs(hx * hz) >= 1e-14 && fabs(hy * hz) >= 1e-14;
-    const bool ok3 = metric_ok && fabs(hx * hy * hz) >= 1e-14;
-    const double sx = sqrt(fabs(gx)) * ax, sy = sqrt(fabs(gy)) * ay, sz = sqrt(fabs(gz)) * az;
-    const double vol = sx * sy * sz;
-    const double inv = (vol != 0.0) ? 1.0 / vol : 0.0;
-    const double syz = sy * sz, sxz = sx * sz, sxy = sx * sy;
-    k.a1 = ok2 ? sx * sx * inv : 0.0;
-    k.a2 = ok2 ? sy * sy * inv : 0.0;
-    k.a3 = ok2 ? sz * sz * inv : 0.0;
-    k.b1 = ok1 ? syz * syz * inv : 0.0;
-    k.b2 = ok1 ? sxz * sxz * inv : 0.0;
-    k.b3 = ok1 ? sxy * sxy * inv : 0.0;
-    k.c = ok3 ? inv : 0.0;
-    return k;
-}
-
-
-
-template <int TY>
-struct MarchLds0 {
-    double w[3][TY][64];   // ω(p) for the y+1 / z+1 neighbours of dω
-    double ep[2][TY][64];  // ⋆ω(p+1): XZ and XY components for δω's y−1 / z−1 terms
-    double eta[3][TY][64]; // ⋆dω(p) for dᵀ's y−1 / z−1 terms
-    double dw[2][TY][64];  // δω ping-pong: planes p and p+1
-};
-
-template <int TY, bool DIAG, int MINW>
-__global__ __launch_bounds__(64 * TY, MINW) void k_lap3d_march0(const double* __restrict__ w, const double* __restrict__ g,
-                                                         const double* __restrict__ hxs, const double* __restrict__ hys,
-                                                         const double* __restrict__ hzs, double* __restrict__ out,
-                                                         int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
-                                                         int64_t Oz, int ntz, int nty, int nxc, int xchunk)
-{
-    __shared__ MarchLds0<TY> s;
-    const int lane = threadIdx.x & 63;
-    const int row = threadIdx.x >> 6;
-
-    // XCD-contiguous tile order: hardware places workgroup b on XCD b % 8.
-    const int nb = gridDim.x;
-    const int b = blockIdx.x;
-    const int per = (nb + 7) / 8;
-    const int rem = nb % 8;
-    int logical;
-    {
-        const int xcd = b % 8, slot = b / 8;
-        // XCDs [0, rem) own `per` logical blocks, the rest `per - 1` (rem == 0: all `per`)
-        const int full = rem == 0 ? 8 : rem;
-        logical = (xcd < full) ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
-    }
-    const int tz = logical % ntz;
-    const int ty = (logical / ntz) % nty;
-    const int tx = logical / (ntz * nty);
-    if (tx >= nxc) return; // uniform per workgroup
-
-    const int64_t z = (int64_t)tz * 62 - 1 + lane;
-    const int64_t y = (int64_t)ty * (TY - 2) - 1 + row;
-    const int64_t xa = (int64_t)tx * xchunk;
-    const int64_t xb = min(xa + xchunk, Ox);
-    const bool pos_in = (z >= 0 && z < Nz && y >= 0 && y < Ny);
-    const bool writer = lane >= 1 && lane <= 62 && row >= 1 && row <= TY - 2 && z < Oz && y < Oy;
-    const bool has_y1 = (y + 1 < Ny), has_z1 = (z + 1 < Nz);
-    const double hy = pos_in ? hys[y] : 0.0;
-    const double hz = pos_in ? hzs[z] : 0.0;
-    const int64_t plane = Ny * Nz;
-    const int64_t lin_yz = y * Nz + z;
-
-    auto load = [&](int64_t q, double W[3], double G[3], double& hxq) {
-        if (pos_in && q >= 0 && q < Nx) {
-            const int64_t lin = q * plane + lin_yz;
-            W[0] = w[lin * 3 + 0];
-            W[1] = w[lin * 3 + 1];
-            W[2] = w[lin * 3 + 2];
-            if (DIAG) {
-                G[0] = g[lin * 3 + 0];
-                G[1] = g[lin * 3 + 1];
-                G[2] = g[lin * 3 + 2];
-            } else {
-                G[0] = G[1] = G[2] = 1.0;
-            }
-            hxq = hxs[q];
-        } else {
-            W[0] = W[1] = W[2] = 0.0;
-            G[0] = G[1] = G[2] = 0.0; // outside: zero coefficients (zero-extension)
-            hxq = 0.0;
-        }
-    };
-    auto coefs_of = [&](const double G[3], double hxq, bool valid) {
-        if (!valid) return Coefs{0, 0, 0, 0, 0, 0, 0};
-        return node_coefs0(G[0], G[1], G[2], hxq, hy, hz);
-    };
-
-    // state: c = plane p, n = plane p+1, f = prefetched plane p+2
-    double Wc[3], Wn[3], Wf[3], Gc[3], Gn[3], Gf[3], hc, hn, hf;
-    int64_t p = xa - 1;
-    load(p, Wc, Gc, hc);
-    load(p + 1, Wn, Gn, hn);
-    Coefs Kc = coefs_of(Gc, hc, pos_in && p >= 0 && p < Nx);
-    Coefs Kn = coefs_of(Gn, hn, pos_in && p + 1 < Nx);
-    double etaPrev[3] = {0.0, 0.0, 0.0};
-    // ⋆ω(p) own: YZ, XZ, XY with signs
-    double epc0 = Kc.b1 * Wc[0], epc1 = -Kc.b2 * Wc[1], epc2 = Kc.b3 * Wc[2];
-    (void)epc1;
-    (void)epc2;
-    double dwc = 0.0;
-    int buf = 0;
-
-    for (; p < xb; ++p) {
-        load(p + 2, Wf, Gf, hf); // prefetch, consumed at the end of the step
-
-        // ⋆ω(p+1) own
-        const double epn0 = Kn.b1 * Wn[0], epn1 = -Kn.b2 * Wn[1], epn2 = Kn.b3 * Wn[2];
-        s.w[0][row][lane] = Wc[0];
-        s.w[1][row][lane] = Wc[1];
-        s.w[2][row][lane] = Wc[2];
-        s.ep[0][row][lane] = epn1;
-        s.ep[1][row][lane] = epn2;
-        __syncthreads();
-
-        // dω(p) own, clamped forward differences
-        const double* Wx = (p + 1 < Nx) ? Wn : Wc;
-        const double wy0 = has_y1 && row + 1 < TY ? s.w[0][row + 1][lane] : Wc[0];
-        const double wy2 = has_y1 && row + 1 < TY ? s.w[2][row + 1][lane] : Wc[2];
-        const double wz0 = has_z1 && lane + 1 < 64 ? s.w[0][row][lane + 1] : Wc[0];
-        const double wz1 = has_z1 && lane + 1 < 64 ? s.w[1][row][lane + 1] : Wc[1];
-        const double Fxy = (Wx[1] - Wc[1]) - (wy0 - Wc[0]);
-        const double Fxz = (Wx[2] - Wc[2]) - (wz0 - Wc[0]);
-        const double Fyz = (wy2 - Wc[2]) - (wz1 - Wc[1]);
-        const double eta0 = Kc.a1 * Fyz, eta1 = -Kc.a2 * Fxz, eta2 = Kc.a3 * Fxy;
-
-        // δω(p+1) own: −⋆(dᵀ ⋆ω) with zero outside
-        const double ep_y1 = row > 0 ? s.ep[0][row - 1][lane] : 0.0;
-        const double ep_z2 = lane > 0 ? s.ep[1][row][lane - 1] : 0.0;
-        const double H = (epn0 - epc0) - (epn1 - ep_y1) + (epn2 - ep_z2);
-        const double dwn = -Kn.c * H;
-
-        s.eta[0][row][lane] = eta0;
-        s.eta[1][row][lane] = eta1;
-        s.eta[2][row][lane] = eta2;
-        s.dw[buf ^ 1][row][lane] = dwn;
-        __syncthreads();
-
-        if (p >= xa && writer) {
-            const double ey0 = s.eta[0][row - 1][lane], ey2 = s.eta[2][row - 1][lane];
-            const double ez0 = s.eta[0][row][lane - 1], ez1 = s.eta[1][row][lane - 1];
-            const double Gxy = (eta1 - etaPrev[1]) - (eta0 - ey0);
-            const double Gxz = (eta2 - etaPrev[2]) - (eta0 - ez0);
-            const double Gyz = (eta2 - ey2) - (eta1 - ez1);
-            const double ddx = (p + 1 < Nx) ? dwn - dwc : 0.0;
-            const double ddy = has_y1 ? s.dw[buf][row + 1][lane] - dwc : 0.0;
-            const double ddz = has_z1 ? s.dw[buf][row][lane + 1] - dwc : 0.0;
-            double* o = out + ((p * Oy + y) * Oz + z) * 3;
-            o[0] = Kc.a1 * Gyz + ddx;
-            o[1] = -Kc.a2 * Gxz + ddy;
-            o[2] = Kc.a3 * Gxy + ddz;
-        }
-
-        // rotate
-        etaPrev[0] = eta0;
-        etaPrev[1] = eta1;
-        etaPrev[2] = eta2;
-        epc0 = epn0;
-        dwc = dwn;
-        buf ^= 1;
-        Wc[0] = Wn[0];
-        Wc[1] = Wn[1];
-        Wc[2] = Wn[2];
-        Kc = Kn;
-        Wn[0] = Wf[0];
-        Wn[1] = Wf[1];
-        Wn[2] = Wf[2];
-        Kn = coefs_of(Gf, hf, pos_in && p + 2 >= 0 && p + 2 < Nx);
-    }
-}
-
-
-// v0 with two prefetched planes in alternating register sets
-template <int TY, bool DIAG>
-__global__ __launch_bounds__(64 * TY) void k_lap3d_march5(const double* __restrict__ w, const double* __restrict__ g,
-                                                         const double* __restrict__ hxs, const double* __restrict__ hys,
-                                                         const double* __restrict__ hzs, double* __restrict__ out,
-                                                         int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
-                                                         int64_t Oz, int ntz, int nty, int nxc, int xchunk)
-{
-    __shared__ MarchLds0<TY> s;
-    const int lane = threadIdx.x & 63;
-    const int row = threadIdx.x >> 6;
-
-    // XCD-contiguous tile order: hardware places workgroup b on XCD b % 8.
-    const int nb = gridDim.x;
-    const int b = blockIdx.x;
-    const int per = (nb + 7) / 8;
-    const int rem = nb % 8;
-    int logical;
-    {
-        const int xcd = b % 8, slot = b / 8;
-        // XCDs [0, rem) own `per` logical blocks, the rest `per - 1` (rem == 0: all `per`)
-        const int full = rem == 0 ? 8 : rem;
-        logical = (xcd < full) ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
-    }
-    const int tz = logical % ntz;
-    const int ty = (logical / ntz) % nty;
-    const int tx = logical / (ntz * nty);
-    if (tx >= nxc) return; // uniform per workgroup
-
-    const int64_t z = (int64_t)tz * 62 - 1 + lane;
-    const int64_t y = (int64_t)ty * (TY - 2) - 1 + row;
-    const int64_t xa = (int64_t)tx * xchunk;
-    const int64_t xb = min(xa + xchunk, Ox);
-    const bool pos_in = (z >= 0 && z < Nz && y >= 0 && y < Ny);
-    const bool writer = lane >= 1 && lane <= 62 && row >= 1 && row <= TY - 2 && z < Oz && y < Oy;
-    const bool has_y1 = (y + 1 < Ny), has_z1 = (z + 1 < Nz);
-    const double hy = pos_in ? hys[y] : 0.0;
-    const double hz = pos_in ? hzs[z] : 0.0;
-    const int64_t plane = Ny * Nz;
-    const int64_t lin_yz = y * Nz + z;
-
-    auto load = [&](int64_t q, double W[3], double G[3], double& hxq) {
-        if (pos_in && q >= 0 && q < Nx) {
-            const int64_t lin = q * plane + lin_yz;
-            W[0] = w[lin * 3 + 0];
-            W[1] = w[lin * 3 + 1];
-            W[2] = w[lin * 3 + 2];
-            if (DIAG) {
-                G[0] = g[lin * 3 + 0];
-                G[1] = g[lin * 3 + 1];
-                G[2] = g[lin * 3 + 2];
-            } else {
-                G[0] = G[1] = G[2] = 1.0;
-            }
-            hxq = hxs[q];
-        } else {
-            W[0] = W[1] = W[2] = 0.0;
-            G[0] = G[1] = G[2] = 0.0; // outside: zero coefficients (zero-extension)
-            hxq = 0.0;
-        }
-    };
-    auto coefs_of = [&](const double G[3], double hxq, bool valid) {
-        if (!valid) return Coefs{0, 0, 0, 0, 0, 0, 0};
-        return node_coefs0(G[0], G[1], G[2], hxq, hy, hz);
-    };
-
-    // state: c = plane p, n = plane p+1; two prefetched planes (p+2, p+3)
-    // in register sets that alternate between the two halves of the
-    // unrolled loop, so a load's destination is consumed in place
-    double Wc[3], Wn[3], Wf[3], Gc[3], Gn[3], Gf[3], hc, hn, hf;
-    double Wf2[3], Gf2[3], hf2;
-    int64_t p = xa - 1;
-    load(p, Wc, Gc, hc);
-    load(p + 1, Wn, Gn, hn);
-    Coefs Kc = coefs_of(Gc, hc, pos_in && p >= 0 && p < Nx);
-    Coefs Kn = coefs_of(Gn, hn, pos_in && p + 1 < Nx);
-    double etaPrev[3] = {0.0, 0.0, 0.0};
-    // ⋆ω(p) own: YZ, XZ, XY with signs
-    double epc0 = Kc.b1 * Wc[0], epc1 = -Kc.b2 * Wc[1], epc2 = Kc.b3 * Wc[2];
-    (void)epc1;
-    (void)epc2;
-    double dwc = 0.0;
-    int buf = 0;
-
-    load(p + 2, Wf, Gf, hf);
-    auto step = [&](double (&Wl)[3], double (&Gl)[3], double& hl, const double (&Wu)[3], const double (&Gu)[3],
-                    const double hu) {
-        load(p + 3, Wl, Gl, hl); // prefetch, consumed at the end of the next step
-
-        // ⋆ω(p+1) own
-        const double epn0 = Kn.b1 * Wn[0], epn1 = -Kn.b2 * Wn[1], epn2 = Kn.b3 * Wn[2];
-        s.w[0][row][lane] = Wc[0];
-        s.w[1][row][lane] = Wc[1];
-        s.w[2][row][lane] = Wc[2];
-        s.ep[0][row][lane] = epn1;
-        s.ep[1][row][lane] = epn2;
-        __syncthreads();
-
-        // dω(p) own, clamped forward differences
-        const double* Wx = (p + 1 < Nx) ? Wn : Wc;
-        const double wy0 = has_y1 && row + 1 < TY ? s.w[0][row + 1][lane] : Wc[0];
-        const double wy2 = has_y1 && row + 1 < TY ? s.w[2][row + 1][lane] : Wc[2];
-        const double wz0 = has_z1 && lane + 1 < 64 ? s.w[0][row][lane + 1] : Wc[0];
-        const double wz1 = has_z1 && lane + 1 < 64 ? s.w[1][row][lane + 1] : Wc[1];
-        const double Fxy = (Wx[1] - Wc[1]) - (wy0 - Wc[0]);
-        const double Fxz = (Wx[2] - Wc[2]) - (wz0 - Wc[0]);
-        const double Fyz = (wy2 - Wc[2]) - (wz1 - Wc[1]);
-        const double eta0 = Kc.a1 * Fyz, eta1 = -Kc.a2 * Fxz, eta2 = Kc.a3 * Fxy;
-
-        // δω(p+1) own: −⋆(dᵀ ⋆ω) with zero outside
-        const double ep_y1 = row > 0 ? s.ep[0][row - 1][lane] : 0.0;
-        const double ep_z2 = lane > 0 ? s.ep[1][row][lane - 1] : 0.0;
-        const double H = (epn0 - epc0) - (epn1 - ep_y1) + (epn2 - ep_z2);
-        const double dwn = -Kn.c * H;
-
-        s.eta[0][row][lane] = eta0;
-        s.eta[1][row][lane] = eta1;
-        s.eta[2][row][lane] = eta2;
-        s.dw[buf ^ 1][row][lane] = dwn;
-        __syncthreads();
-
-        if (p >= xa && writer) {
-            const double ey0 = s.eta[0][row - 1][lane], ey2 = s.eta[2][row - 1][lane];
-            const double ez0 = s.eta[0][row][lane - 1], ez1 = s.eta[1][row][lane - 1];
-            const double Gxy = (eta1 - etaPrev[1]) - (eta0 - ey0);
-            const double Gxz = (eta2 - etaPrev[2]) - (eta0 - ez0);
-            const double Gyz = (eta2 - ey2) - (eta1 - ez1);
-            const double ddx = (p + 1 < Nx) ? dwn - dwc : 0.0;
-            const double ddy = has_y1 ? s.dw[buf][row + 1][lane] - dwc : 0.0;
-            const double ddz = has_z1 ? s.dw[buf][row][lane + 1] - dwc : 0.0;
-            double* o = out + ((p * Oy + y) * Oz + z) * 3;
-            o[0] = Kc.a1 * Gyz + ddx;
-            o[1] = -Kc.a2 * Gxz + ddy;
-            o[2] = Kc.a3 * Gxy + ddz;
-        }
-
-        // rotate
-        etaPrev[0] = eta0;
-        etaPrev[1] = eta1;
-        etaPrev[2] = eta2;
-        epc0 = epn0;
-        dwc = dwn;
-        buf ^= 1;
-        Wc[0] = Wn[0];
-        Wc[1] = Wn[1];
-        Wc[2] = Wn[2];
-        Kc = Kn;
-        Wn[0] = Wu[0];
-        Wn[1] = Wu[1];
-        Wn[2] = Wu[2];
-        Kn = coefs_of(Gu, hu, pos_in && p + 2 >= 0 && p + 2 < Nx);
-    };
-    for (;;) {
-        if (p >= xb) break;
-        step(Wf2, Gf2, hf2, Wf, Gf, hf);
-        ++p;
-        if (p >= xb) break;
-        step(Wf, Gf, hf, Wf2, Gf2, hf2);
-        ++p;
-    }
-}
-
-
-
-// ------------------------------------------------- 3D 1-form, march (v1) ---
+// ------------------------------------------------------ 3D 1-form, march ---
 
 // Registers hold planes p, p+1 and the prefetched p+2; loads are guarded by
 // the grid bounds.
 
 template <int TY>
-struct MarchLds1 {
+struct MarchLds {
     double w[3][TY][64];   // ω(p) for the y+1 / z+1 neighbours of dω
     double ep[2][TY][64];  // ⋆ω(p+1): XZ and XY components for δω's y−1 / z−1 terms
     double eta[3][TY][64]; // ⋆dω(p) for dᵀ's y−1 / z−1 terms
     double dw[2][TY][64];  // δω ping-pong: planes p and p+1
 };
 
-template <int TY, bool DIAG, int MINW>
-__global__ __launch_bounds__(64 * TY, MINW) void k_lap3d_march1(const double* __restrict__ w, const double* __restrict__ g,
+template <int TY, bool DIAG>
+__global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restrict__ w, const double* __restrict__ g,
                                                          const double* __restrict__ hxs, const double* __restrict__ hys,
                                                          const double* __restrict__ hzs, double* __restrict__ out,
                                                          int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
                                                          int64_t Oz, int ntz, int nty, int nxc, int xchunk)
 {
-    __shared__ MarchLds1<TY> s;
+    __shared__ MarchLds<TY> s;
     const int lane = threadIdx.x & 63;
     const int row = threadIdx.x >> 6;
 
@@ -684,36 +342,110 @@ __global__ __launch_bounds__(64 * TY, MINW) void k_lap3d_march1(const double* __
 }
 
 
-// ------------------------------------------------------ 3D 1-form, march ---
+// ------------------------------------------ 3D 1-form, pipelined march ---
 
 constexpr int kMaxXChunk = 256;
 
-typedef __attribute__((address_space(1))) double GlobalDouble;
+typedef unsigned int U32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int U32x4 __attribute__((ext_vector_type(4)));
 
-template <int TY>
-struct MarchLds {
-    double w[3][TY][64];   // ω(p) for the y+1 / z+1 neighbours of dω
-    double ep[2][TY][64];  // ⋆ω(p+1): XZ and XY components for δω's y−1 / z−1 terms
-    double eta[3][TY][64]; // ⋆dω(p) for dᵀ's y−1 / z−1 terms
-    double dw[2][TY][64];  // δω ping-pong: planes p and p+1
+// Raw buffer resources (stride 0, gfx9 data format word): a load at an offset
+// at or beyond the resource's byte range returns zero and a store there is
+// dropped, so lanes and planes outside the grid are masked without a branch.
+constexpr uint32_t kOutside = 0x80000000u;
+constexpr int64_t kMaxBufferBytes = 0x7FFFFFFFLL;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* base, int64_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
+}
+
+__device__ __forceinline__ double as_double(unsigned lo, unsigned hi)
+{
+    return __builtin_bit_cast(double, static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32));
+}
+
+// three consecutive doubles at byte offset off (16 + 8 byte requests)
+__device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, uint32_t off, double v[3])
+{
+    const U32x4 a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    const U32x2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off + 16, 0, 0);
+    v[0] = as_double(a[0], a[1]);
+    v[1] = as_double(a[2], a[3]);
+    v[2] = as_double(b[0], b[1]);
+}
+
+__device__ __forceinline__ void store3(__amdgpu_buffer_rsrc_t r, uint32_t off, double a, double b, double c)
+{
+    const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b),
+                   uc = __builtin_bit_cast(uint64_t, c);
+    const U32x4 lo = {static_cast<unsigned>(ua), static_cast<unsigned>(ua >> 32), static_cast<unsigned>(ub),
+                      static_cast<unsigned>(ub >> 32)};
+    const U32x2 hi = {static_cast<unsigned>(uc), static_cast<unsigned>(uc >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b128(lo, r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(hi, r, off + 16, 0, 0);
+}
+
+// A copy into a register distinct from the source.  A loaded ω value lives
+// three steps (in flight, plane p+1, plane p); moving it out of its load
+// registers when it is consumed lets the next load reuse them, so the loop
+// unrolled by two needs no register copies of data still in flight (a copy
+// would make the wavefront wait for it).
+__device__ __forceinline__ double moved(double x)
+{
+    double r;
+    asm volatile("v_mov_b64 %0, %1" : "=&v"(r) : "v"(x));
+    return r;
+}
+
+// f(integral_constant<int, 0>) … f(integral_constant<int, N−1>), unrolled
+template <int N, class F>
+__device__ __forceinline__ void unrolled(F&& f)
+{
+    [&]<int... K>(std::integer_sequence<int, K...>) {
+        (f(std::integral_constant<int, K>{}), ...);
+    }(std::make_integer_sequence<int, N>{});
+}
+
+template <int TY, int ZW>
+struct PipeLds {
+    double w[3][TY][ZW];   // ω(p) for the y+1 / z+1 neighbours of dω
+    double ep[2][TY][ZW];  // ⋆ω(p+1): XZ and XY components for δω's y−1 / z−1 terms
+    double eta[3][TY][ZW]; // ⋆dω(p) for dᵀ's y−1 / z−1 terms
+    double dw[2][TY][ZW];  // δω ping-pong: planes p and p+1
     double hx[kMaxXChunk + 4]; // x edge lengths of planes xa−1 … xb+2 (0 outside)
 };
 
-// Loads are unconditional (indices clamped into the grid, values masked
-// afterwards) so no branch separates them from their consumers.  DEPTH is
-// the prefetch distance in planes: plane p+1+DEPTH is requested while plane
-// p is computed.  DEPTH 2 keeps two planes in flight (more registers, fewer
-// waves); DEPTH 1 keeps one (more waves per SIMD).
-template <int TY, bool DIAG, int DEPTH>
-__global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restrict__ w, const double* __restrict__ g,
-                                                         const double* __restrict__ hxs, const double* __restrict__ hys,
-                                                         const double* __restrict__ hzs, double* __restrict__ out,
-                                                         int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
-                                                         int64_t Oz, int ntz, int nty, int nxc, int xchunk)
+// Number of tiles of `width` nodes along an axis with n nodes and o outputs.
+// A tile's first and last nodes are halos (their outputs belong to the
+// neighbouring tiles) except at the grid's ends, where no neighbour exists:
+// tile t starts at node t·(width−2).
+__host__ __device__ inline int64_t halo_tiles(int64_t n, int64_t o, int width)
 {
-    __shared__ MarchLds<TY> s;
-    const int lane = threadIdx.x & 63;
-    const int row = threadIdx.x >> 6;
+    const int64_t covered = o - 1 - (o == n ? 1 : 0);
+    return std::max<int64_t>(1, (covered + width - 3) / (width - 2));
+}
+
+// The march of k_lap3d_march with every memory access made branch-free
+// through buffer resources, so the compiler's wait counts follow the data:
+// plane p+1+D is requested at the top of step p and consumed D−1 steps
+// later, D planes stay in flight per wavefront, and a step never waits for
+// its predecessors' stores.  A workgroup's resources cover only the planes it
+// reads (xa−1 … xb) and writes (xa … xb−1), which the host keeps below 2 GiB.
+//
+// A workgroup is TY rows (y) of ZW lanes (z, ZW/64 wavefronts).  Lines of
+// ZW ≥ Nz nodes read whole contiguous z-lines, and consecutive rows are
+// adjacent in memory, so each plane of a tile is one contiguous block.
+template <int TY, int ZW, bool DIAG, int D>
+__global__ __launch_bounds__(ZW * TY) void k_lap3d_pipe(const double* __restrict__ w, const double* __restrict__ g,
+                                                        const double* __restrict__ hxs, const double* __restrict__ hys,
+                                                        const double* __restrict__ hzs, double* __restrict__ out,
+                                                        int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
+                                                        int64_t Oz, int ntz, int nty, int nxc, int xchunk)
+{
+    __shared__ PipeLds<TY, ZW> s;
+    const int lane = threadIdx.x % ZW;
+    const int row = threadIdx.x / ZW;
 
     // XCD-contiguous tile order: hardware places workgroup b on XCD b % 8.
     const int nb = gridDim.x;
@@ -723,7 +455,6 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
     int logical;
     {
         const int xcd = b % 8, slot = b / 8;
-        // XCDs [0, rem) own `per` logical blocks, the rest `per - 1` (rem == 0: all `per`)
         const int full = rem == 0 ? 8 : rem;
         logical = (xcd < full) ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
     }
@@ -732,73 +463,80 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
     const int tx = logical / (ntz * nty);
     if (tx >= nxc) return; // uniform per workgroup
 
-    const int64_t z = (int64_t)tz * 62 - 1 + lane;
-    const int64_t y = (int64_t)ty * (TY - 2) - 1 + row;
+    const int64_t z = (int64_t)tz * (ZW - 2) + lane;
+    const int64_t y = (int64_t)ty * (TY - 2) + row;
     const int64_t xa = (int64_t)tx * xchunk;
     const int64_t xb = min(xa + xchunk, Ox);
-    const bool pos_in = (z >= 0 && z < Nz && y >= 0 && y < Ny);
-    const bool writer = lane >= 1 && lane <= 62 && row >= 1 && row <= TY - 2 && z < Oz && y < Oy;
+    const bool pos_in = (z < Nz && y < Ny);
+    // a lane writes unless it is a halo: the first / last of its row or
+    // column, except at the grid's first node and last node
+    const bool wz = (lane >= 1 || z == 0) && (lane <= ZW - 2 || z == Nz - 1) && z < Oz;
+    const bool wy = (row >= 1 || y == 0) && (row <= TY - 2 || y == Ny - 1) && y < Oy;
+    const bool writer = wz && wy;
     const bool has_y1 = (y + 1 < Ny), has_z1 = (z + 1 < Nz);
-    const int64_t yc = min(max(y, (int64_t)0), Ny - 1), zc = min(max(z, (int64_t)0), Nz - 1);
-    const double hy = pos_in ? hys[yc] : 0.0;
-    const double hz = pos_in ? hzs[zc] : 0.0;
-    const int64_t plane = Ny * Nz;
-    const int64_t lin_yz = yc * Nz + zc;
+    const double hy = pos_in ? hys[y] : 0.0;
+    const double hz = pos_in ? hzs[z] : 0.0;
     const int64_t x0 = xa - 1; // plane of s.hx[0]
 
-    for (int i = threadIdx.x; i < (int)(xb - xa) + 4; i += 64 * TY) {
+    // planes read: qa … qe−1; planes written: xa … xb−1
+    const int64_t qa = max(x0, (int64_t)0), qe = min(xb + 1, Nx);
+    const int64_t plane_bytes = Ny * Nz * 24;
+    const __amdgpu_buffer_rsrc_t rw = buffer_of(w + qa * Ny * Nz * 3, (qe - qa) * plane_bytes);
+    const __amdgpu_buffer_rsrc_t rg = buffer_of(DIAG ? g + qa * Ny * Nz * 3 : w, DIAG ? (qe - qa) * plane_bytes : 0);
+    const __amdgpu_buffer_rsrc_t ro = buffer_of(out + xa * Oy * Oz * 3, (xb - xa) * Oy * Oz * 24);
+    const uint32_t yz_bytes = pos_in ? static_cast<uint32_t>((y * Nz + z) * 24) : 0u;
+    const uint32_t oyz_bytes = writer ? static_cast<uint32_t>((y * Oz + z) * 24) : 0u;
+    const uint32_t oplane_bytes = static_cast<uint32_t>(Oy * Oz * 24);
+
+    for (int i = threadIdx.x; i < (int)(xb - xa) + 4; i += ZW * TY) {
         const int64_t q = x0 + i;
-        s.hx[i] = (q >= 0 && q < Nx) ? hxs[q] : 0.0;
+        s.hx[i] = (q >= 0 && q < qe) ? hxs[q] : 0.0;
     }
 
-    auto valid = [&](int64_t q) { return pos_in && q >= 0 && q < Nx; };
+    // byte offset of this lane's node in plane q (outside: beyond the range)
+    auto offset = [&](int64_t q) {
+        const bool ok = pos_in && q >= qa && q < qe;
+        return ok ? static_cast<uint32_t>((q - qa) * plane_bytes) + yz_bytes : kOutside;
+    };
     auto load = [&](int64_t q, double W[3], double G[3]) {
-        const int64_t qc = min(max(q, (int64_t)0), Nx - 1);
-        const int64_t lin = qc * plane + lin_yz;
-        W[0] = w[lin * 3 + 0];
-        W[1] = w[lin * 3 + 1];
-        W[2] = w[lin * 3 + 2];
+        const uint32_t off = offset(q);
+        load3(rw, off, W);
         if (DIAG) {
-            G[0] = g[lin * 3 + 0];
-            G[1] = g[lin * 3 + 1];
-            G[2] = g[lin * 3 + 2];
+            load3(rg, off, G);
         } else {
             G[0] = G[1] = G[2] = 1.0;
         }
     };
-    // ω outside the grid is zero; the coefficients of outside nodes are zero
-    auto mask = [&](bool ok, double W[3]) {
-        W[0] = ok ? W[0] : 0.0;
-        W[1] = ok ? W[1] : 0.0;
-        W[2] = ok ? W[2] : 0.0;
-    };
-    auto coefs_of = [&](const double G[3], int64_t q) {
-        const Coefs k = node_coefs(G[0], G[1], G[2], s.hx[q - x0], hy, hz);
-        return valid(q) ? k : Coefs{0, 0, 0, 0, 0, 0, 0};
-    };
+    // nodes outside the grid or the workgroup's planes have zero edge
+    // lengths in the tables, hence zero coefficients; their ω reads as zero
+    auto coefs_of = [&](const double G[3], int64_t q) { return node_coefs(G[0], G[1], G[2], s.hx[q - x0], hy, hz); };
 
-    // planes: c = p, n = p+1 (ready); f1 = p+2, f2 = p+3 (in flight)
-    double Wc[3], Wn[3], W1[3], W2[3], Gc[3], Gn[3], G1[3], G2[3];
+    // planes: c = p, n = p+1 (ready); in-flight register set j holds plane
+    // p+1+j for j = 1 … D−1 at the top of a step, set 0 is requested in it
+    double Wc[3], Wn[3], Gc[3], Gn[3], WF[D][3], GF[D][3];
     int64_t p = x0;
     load(p, Wc, Gc);
     load(p + 1, Wn, Gn);
-    if (DEPTH == 2) load(p + 2, W1, G1);
+    unrolled<D - 1>([&](auto J) { load(p + 2 + J(), WF[J() + 1], GF[J() + 1]); });
     __syncthreads(); // s.hx
-    mask(valid(p), Wc);
-    mask(valid(p + 1), Wn);
     Coefs Kc = coefs_of(Gc, p);
     Coefs Kn = coefs_of(Gn, p + 1);
     double etaPrev[3] = {0.0, 0.0, 0.0};
     double epc0 = Kc.b1 * Wc[0]; // ⋆ω(p) YZ component
     double dwc = 0.0;
-    double po[3] = {0.0, 0.0, 0.0}; // previous step's stored values
-    uint64_t pa = 0;                // and their address
 
-    // One step computes plane p.  The two register sets of the in-flight
-    // planes alternate between steps (the loop is unrolled by two), so a
-    // load's destination is never copied before it is consumed.
-    auto step = [&](double (&Wl)[3], double (&Gl)[3], const double (&Wu)[3], const double (&Gu)[3], const int buf) {
-        load(p + 1 + DEPTH, Wl, Gl);
+    // neighbour indices inside the tile; reads at a clamped index are
+    // replaced by the boundary value afterwards
+    const int ry = row + 1 < TY ? row + 1 : row, lz = lane + 1 < ZW ? lane + 1 : lane;
+    const int rm = row > 0 ? row - 1 : row, lm = lane > 0 ? lane - 1 : lane;
+    const bool ny = has_y1 && row + 1 < TY, nz = has_z1 && lane + 1 < ZW;
+
+    // step p in the k-th position of an unrolled group of D: requests plane
+    // p+1+D into set k and consumes plane p+2 from set (k+1) mod D
+    auto step = [&](auto K) {
+        constexpr int k = K(), u = (K() + 1) % D;
+        const int buf = static_cast<int>(p - x0) & 1;
+        load(p + 1 + D, WF[k], GF[k]);
 
         // ⋆ω(p+1) own
         const double epn0 = Kn.b1 * Wn[0], epn1 = -Kn.b2 * Wn[1], epn2 = Kn.b3 * Wn[2];
@@ -812,13 +550,8 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
         // dω(p) own, clamped forward differences
         const bool has_x1 = p + 1 < Nx;
         const double wx1 = has_x1 ? Wn[1] : Wc[1], wx2 = has_x1 ? Wn[2] : Wc[2];
-        // LDS neighbours are read unconditionally (indices kept in the tile)
-        // and selected afterwards, so no pointer ever mixes LDS and registers
-        const int ry = row + 1 < TY ? row + 1 : row, lz = lane + 1 < 64 ? lane + 1 : lane;
-        const int rm = row > 0 ? row - 1 : row, lm = lane > 0 ? lane - 1 : lane;
         const double sy0 = s.w[0][ry][lane], sy2 = s.w[2][ry][lane];
         const double sz0 = s.w[0][row][lz], sz1 = s.w[1][row][lz];
-        const bool ny = has_y1 && row + 1 < TY, nz = has_z1 && lane + 1 < 64;
         const double wy0 = ny ? sy0 : Wc[0];
         const double wy2 = ny ? sy2 : Wc[2];
         const double wz0 = nz ? sz0 : Wc[0];
@@ -828,7 +561,8 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
         const double Fyz = (wy2 - Wc[2]) - (wz1 - Wc[1]);
         const double eta0 = Kc.a1 * Fyz, eta1 = -Kc.a2 * Fxz, eta2 = Kc.a3 * Fxy;
 
-        // δω(p+1) own: −⋆(dᵀ ⋆ω) with zero outside
+        // δω(p+1) own: −⋆(dᵀ ⋆ω); neighbours below the tile read as zero
+        // (outside the grid, or a halo whose output is not stored)
         const double sep0 = s.ep[0][rm][lane], sep1 = s.ep[1][row][lm];
         const double ep_y1 = row > 0 ? sep0 : 0.0;
         const double ep_z2 = lane > 0 ? sep1 : 0.0;
@@ -841,18 +575,14 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
         s.dw[buf ^ 1][row][lane] = dwn;
         __syncthreads();
 
-        // plane p+2 (requested one step ago) becomes the next plane n; this
-        // wait precedes the stores below, so it never waits for them
-        const double(&Wr)[3] = DEPTH == 2 ? Wu : Wl;
-        const double(&Gr)[3] = DEPTH == 2 ? Gu : Gl;
-        double Wnn[3] = {Wr[0], Wr[1], Wr[2]};
-        mask(valid(p + 2), Wnn);
-        const Coefs Knn = coefs_of(Gr, p + 2);
+        // plane p+2, requested D−1 steps ago
+        const Coefs Knn = coefs_of(GF[u], p + 2);
 
-        // Δω(p); every lane computes it (halo lanes read clamped LDS
-        // indices), only the tile's interior stores it
-        const double ey0 = s.eta[0][rm][lane], ey2 = s.eta[2][rm][lane];
-        const double ez0 = s.eta[0][row][lm], ez1 = s.eta[1][row][lm];
+        // Δω(p); every lane computes it, writers store it
+        const double sey0 = s.eta[0][rm][lane], sey2 = s.eta[2][rm][lane];
+        const double sez0 = s.eta[0][row][lm], sez1 = s.eta[1][row][lm];
+        const double ey0 = row > 0 ? sey0 : 0.0, ey2 = row > 0 ? sey2 : 0.0;
+        const double ez0 = lane > 0 ? sez0 : 0.0, ez1 = lane > 0 ? sez1 : 0.0;
         const double Gxy = (eta1 - etaPrev[1]) - (eta0 - ey0);
         const double Gxz = (eta2 - etaPrev[2]) - (eta0 - ez0);
         const double Gyz = (eta2 - ey2) - (eta1 - ez1);
@@ -860,25 +590,9 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
         const double ddx = has_x1 ? dwn - dwc : 0.0;
         const double ddy = has_y1 ? sdy - dwc : 0.0;
         const double ddz = has_z1 ? sdz - dwc : 0.0;
-        const double o0 = Kc.a1 * Gyz + ddx, o1 = -Kc.a2 * Gxz + ddy, o2 = Kc.a3 * Gxy + ddz;
-        // the address is made opaque so the store uses this very register
-        uint64_t oa = reinterpret_cast<uint64_t>(out + ((p * Oy + y) * Oz + z) * 3);
-        asm volatile("" : "+v"(oa));
-        GlobalDouble* o = reinterpret_cast<GlobalDouble*>(oa);
-        if (p >= xa && writer) {
-            o[0] = o0;
-            o[1] = o1;
-            o[2] = o2;
-        }
-        // The registers a store reads (data and address) stay reserved until the next step has
-        // issued its own stores: reusing them earlier would make the wave
-        // wait for this step's stores, and with them for every load issued
-        // before them, including the prefetch of plane p+3.
-        asm volatile("" ::"v"(po[0]), "v"(po[1]), "v"(po[2]), "v"(pa));
-        pa = oa;
-        po[0] = o0;
-        po[1] = o1;
-        po[2] = o2;
+        const bool store = writer && p >= xa;
+        const uint32_t ooff = store ? static_cast<uint32_t>(p - xa) * oplane_bytes + oyz_bytes : kOutside;
+        store3(ro, ooff, Kc.a1 * Gyz + ddx, -Kc.a2 * Gxz + ddy, Kc.a3 * Gxy + ddz);
 
         // rotate
         etaPrev[0] = eta0;
@@ -889,23 +603,24 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             Wc[c] = Wn[c];
-            Wn[c] = Wnn[c];
+            Wn[c] = moved(WF[u][c]);
         }
         Kc = Kn;
         Kn = Knn;
+        ++p;
     };
 
     // drain the prologue's loads once (vmcnt(0)) so the loop's wait counts
     // depend only on the loop's own memory operations
     __builtin_amdgcn_s_waitcnt(0x0F70);
-    for (;;) {
-        if (p >= xb) break;
-        step(W2, G2, W1, G1, 0);
-        ++p;
-        if (p >= xb) break;
-        step(W1, G1, W2, G2, 1);
-        ++p;
-    }
+    // steps p = xa−1 … xb−1 in groups of D, then the remainder
+    const int steps = static_cast<int>(xb - x0);
+    int done = 0;
+    for (; done + D <= steps; done += D) unrolled<D>(step);
+    const int rest = steps - done;
+    unrolled<D - 1>([&](auto K) {
+        if (K() < rest) step(K);
+    });
 }
 
 // ------------------------------------------------------ 2D 0-form, direct ---
@@ -992,88 +707,50 @@ inline unsigned blocks_for(int64_t total, int block, int64_t cap = 1 << 20)
     return static_cast<unsigned>(b);
 }
 
-template <int TY, int MINW>
-int launch_march0(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk,
-                  hipStream_t stream)
-{
-    const int ntz = static_cast<int>((o[2] + 61) / 62);
-    const int nty = static_cast<int>((o[1] + TY - 3) / (TY - 2));
-    const int nxc = static_cast<int>((o[0] + xchunk - 1) / xchunk);
-    const int64_t nb = (int64_t)ntz * nty * nxc;
-    SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
-    if (g.metric)
-        hipLaunchKernelGGL((k_lap3d_march0<TY, true, MINW>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
-                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
-                           o[2], ntz, nty, nxc, xchunk);
-    else
-        hipLaunchKernelGGL((k_lap3d_march0<TY, false, MINW>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
-                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
-                           o[2], ntz, nty, nxc, xchunk);
-    SIL_LAUNCH_CHECK();
-    return SIL_OK;
-}
-
 template <int TY>
-int launch_march5(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk,
-                  hipStream_t stream)
-{
-    const int ntz = static_cast<int>((o[2] + 61) / 62);
-    const int nty = static_cast<int>((o[1] + TY - 3) / (TY - 2));
-    const int nxc = static_cast<int>((o[0] + xchunk - 1) / xchunk);
-    const int64_t nb = (int64_t)ntz * nty * nxc;
-    SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
-    if (g.metric)
-        hipLaunchKernelGGL((k_lap3d_march5<TY, true>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
-                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
-                           o[2], ntz, nty, nxc, xchunk);
-    else
-        hipLaunchKernelGGL((k_lap3d_march5<TY, false>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
-                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
-                           o[2], ntz, nty, nxc, xchunk);
-    SIL_LAUNCH_CHECK();
-    return SIL_OK;
-}
-
-template <int TY, int MINW>
-int launch_march1(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk,
-                  hipStream_t stream)
-{
-    const int ntz = static_cast<int>((o[2] + 61) / 62);
-    const int nty = static_cast<int>((o[1] + TY - 3) / (TY - 2));
-    const int nxc = static_cast<int>((o[0] + xchunk - 1) / xchunk);
-    const int64_t nb = (int64_t)ntz * nty * nxc;
-    SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
-    if (g.metric)
-        hipLaunchKernelGGL((k_lap3d_march1<TY, true, MINW>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
-                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
-                           o[2], ntz, nty, nxc, xchunk);
-    else
-        hipLaunchKernelGGL((k_lap3d_march1<TY, false, MINW>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
-                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
-                           o[2], ntz, nty, nxc, xchunk);
-    SIL_LAUNCH_CHECK();
-    return SIL_OK;
-}
-
-template <int TY, int DEPTH>
 int launch_march(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk,
-                 hipStream_t stream)
+                  hipStream_t stream)
 {
     const int ntz = static_cast<int>((o[2] + 61) / 62);
     const int nty = static_cast<int>((o[1] + TY - 3) / (TY - 2));
-    SIL_REQUIRE(xchunk >= 1 && xchunk <= kMaxXChunk, SIL_ERR_INVALID,
-                "laplacian: x chunk %d outside [1, %d]", xchunk, kMaxXChunk);
     const int nxc = static_cast<int>((o[0] + xchunk - 1) / xchunk);
     const int64_t nb = (int64_t)ntz * nty * nxc;
     SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
     if (g.metric)
-        hipLaunchKernelGGL((k_lap3d_march<TY, true, DEPTH>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+        hipLaunchKernelGGL((k_lap3d_march<TY, true>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
                            g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
                            o[2], ntz, nty, nxc, xchunk);
     else
-        hipLaunchKernelGGL((k_lap3d_march<TY, false, DEPTH>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
+        hipLaunchKernelGGL((k_lap3d_march<TY, false>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in, g.metric,
                            g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
                            o[2], ntz, nty, nxc, xchunk);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+template <int TY, int ZW, int D>
+int launch_pipe(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk, hipStream_t stream)
+{
+    const int64_t ntz = halo_tiles(g.ext[2], o[2], ZW), nty = halo_tiles(g.ext[1], o[1], TY);
+    // a workgroup's buffer ranges (xchunk + 1 input planes, xchunk output
+    // planes) stay below 2 GiB: shorter chunks for large cross-sections
+    const int64_t in_plane = g.ext[1] * g.ext[2] * 24, out_plane = o[1] * o[2] * 24;
+    const int64_t fit = std::min(kMaxBufferBytes / in_plane - 1, kMaxBufferBytes / std::max<int64_t>(out_plane, 1));
+    SIL_REQUIRE(fit >= 1, SIL_ERR_UNSUPPORTED, "laplacian: a %lld x %lld cross-section exceeds the pipelined kernel",
+                (long long)g.ext[1], (long long)g.ext[2]);
+    xchunk = static_cast<int>(std::min<int64_t>({(int64_t)xchunk, fit, (int64_t)kMaxXChunk}));
+    SIL_REQUIRE(xchunk >= 1, SIL_ERR_INVALID, "laplacian: x chunk %d outside [1, %d]", xchunk, kMaxXChunk);
+    const int64_t nxc = (o[0] + xchunk - 1) / xchunk;
+    const int64_t nb = ntz * nty * nxc;
+    SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
+    if (g.metric)
+        hipLaunchKernelGGL((k_lap3d_pipe<TY, ZW, true, D>), dim3((unsigned)nb), dim3(ZW * TY), 0, stream, in, g.metric,
+                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
+                           o[2], (int)ntz, (int)nty, (int)nxc, xchunk);
+    else
+        hipLaunchKernelGGL((k_lap3d_pipe<TY, ZW, false, D>), dim3((unsigned)nb), dim3(ZW * TY), 0, stream, in, g.metric,
+                           g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0], o[1],
+                           o[2], (int)ntz, (int)nty, (int)nxc, xchunk);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
@@ -1097,62 +774,28 @@ int launch_laplacian_3d_1form(const FusedGeom& g, const double* in, double* out,
         return SIL_OK;
     }
     const int xchunk = tile.xchunk > 0 ? tile.xchunk : 32;
-    if (tile.variant == FUSED_MARCH_V5) {
-        switch (tile.ty) {
-        case 8:
-            return launch_march5<8>(g, in, out, o, xchunk, stream);
-        case 16:
-            return launch_march5<16>(g, in, out, o, xchunk, stream);
-        default:
-            set_error("laplacian: unsupported tile height %d (8 or 16)", tile.ty);
-            return SIL_ERR_INVALID;
-        }
+    if (tile.variant == FUSED_PIPE) {
+        // tile.tz: lanes per row (64, 128 or 256); tile.ty: rows
+        const int zw = tile.tz;
+#define SIL_PIPE(ZW, TY) if (zw == ZW && tile.ty == TY) return launch_pipe<TY, ZW, 2>(g, in, out, o, xchunk, stream)
+        SIL_PIPE(64, 8);
+        SIL_PIPE(64, 12);
+        SIL_PIPE(64, 16);
+        SIL_PIPE(128, 4);
+        SIL_PIPE(128, 8);
+        SIL_PIPE(256, 4);
+#undef SIL_PIPE
+        set_error("laplacian: unsupported pipelined tile %d x %d", zw, tile.ty);
+        return SIL_ERR_INVALID;
     }
-    if (tile.variant == FUSED_MARCH_V0 || tile.variant == FUSED_MARCH_V0_OCC8) {
-        const bool occ8 = tile.variant == FUSED_MARCH_V0_OCC8;
-        switch (tile.ty) {
-        case 8:
-            return occ8 ? launch_march0<8, 8>(g, in, out, o, xchunk, stream)
-                        : launch_march0<8, 1>(g, in, out, o, xchunk, stream);
-        case 16:
-            return occ8 ? launch_march0<16, 8>(g, in, out, o, xchunk, stream)
-                        : launch_march0<16, 1>(g, in, out, o, xchunk, stream);
-        default:
-            set_error("laplacian: unsupported tile height %d (8 or 16)", tile.ty);
-            return SIL_ERR_INVALID;
-        }
-    }
-    if (tile.variant == FUSED_MARCH || tile.variant == FUSED_MARCH_OCC6) {
-        // FUSED_MARCH_OCC6 caps the registers at 80 so that six wavefronts
-        // share each SIMD (a few spills to scratch)
-        const bool o6 = tile.variant == FUSED_MARCH_OCC6;
-#define SIL_MARCH1(T) return o6 ? launch_march1<T, 6>(g, in, out, o, xchunk, stream) : launch_march1<T, 1>(g, in, out, o, xchunk, stream)
-        switch (tile.ty) {
-        case 4: SIL_MARCH1(4);
-        case 6: SIL_MARCH1(6);
-        case 8: SIL_MARCH1(8);
-        case 10: SIL_MARCH1(10);
-        case 12: SIL_MARCH1(12);
-        case 16: SIL_MARCH1(16);
-        default:
-            set_error("laplacian: unsupported tile height %d (4, 6, 8, 10, 12 or 16)", tile.ty);
-            return SIL_ERR_INVALID;
-        }
-#undef SIL_MARCH1
-    }
-    SIL_REQUIRE(tile.variant == FUSED_MARCH_DEEP || tile.variant == FUSED_MARCH_CLAMPED, SIL_ERR_INVALID,
-                "laplacian: unknown kernel variant %d", tile.variant);
-    const bool deep = tile.variant == FUSED_MARCH_DEEP;
+    SIL_REQUIRE(tile.variant == FUSED_MARCH, SIL_ERR_INVALID, "laplacian: unknown kernel variant %d", tile.variant);
     switch (tile.ty) {
-    case 4:
-        return deep ? launch_march<4, 2>(g, in, out, o, xchunk, stream) : launch_march<4, 1>(g, in, out, o, xchunk, stream);
-    case 8:
-        return deep ? launch_march<8, 2>(g, in, out, o, xchunk, stream) : launch_march<8, 1>(g, in, out, o, xchunk, stream);
-    case 16:
-        return deep ? launch_march<16, 2>(g, in, out, o, xchunk, stream)
-                    : launch_march<16, 1>(g, in, out, o, xchunk, stream);
+    case 4: return launch_march<4>(g, in, out, o, xchunk, stream);
+    case 8: return launch_march<8>(g, in, out, o, xchunk, stream);
+    case 12: return launch_march<12>(g, in, out, o, xchunk, stream);
+    case 16: return launch_march<16>(g, in, out, o, xchunk, stream);
     default:
-        set_error("laplacian: unsupported tile height %d (4, 8 or 16)", tile.ty);
+        set_error("laplacian: unsupported tile height %d (4, 8, 12 or 16)", tile.ty);
         return SIL_ERR_INVALID;
     }
 }

@@ -24,10 +24,10 @@ struct FusedGeom {
     const double* metric; // diagonal [node][ndim] or null for the identity
 };
 
-// FUSED_MARCH: x-march, guarded loads, one plane prefetched;
-// FUSED_MARCH_DEEP / _CLAMPED: clamped loads, two / one planes prefetched
-enum FusedVariant { FUSED_DIRECT = 0, FUSED_MARCH = 1, FUSED_MARCH_DEEP = 2, FUSED_MARCH_CLAMPED = 3, FUSED_MARCH_V0 = 4, FUSED_MARCH_V5 = 5,
-                   FUSED_MARCH_V0_OCC8 = 6, FUSED_MARCH_OCC6 = 7 };
+// FUSED_DIRECT: one thread per output node, neighbours through the caches;
+// FUSED_MARCH: x-march with guarded global loads, one plane prefetched;
+// FUSED_PIPE: x-march with branch-free buffer accesses, two planes in flight
+enum FusedVariant { FUSED_DIRECT = 0, FUSED_MARCH = 1, FUSED_PIPE = 2 };
 
 struct FusedTile {
     int variant;  // FusedVariant

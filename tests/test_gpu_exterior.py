@@ -206,7 +206,8 @@ def test_laplacian_matches_oracle(ext, n, k, kind):
         assert rel_err(out, orc.laplacian(og, k, w, oext)) <= TOL
 
 
-TILES = ["0,8,32", "1,4,3", "1,8,32", "1,8,5", "1,16,7", "1,8,1", "1,12,32", "1,12,5", "1,6,3", "1,10,32", "2,4,3", "2,8,32", "2,16,7", "3,8,5"]
+TILES = ["0,8,32", "1,4,3", "1,8,32", "1,8,5", "1,16,7", "1,8,1", "1,12,32", "1,12,5",
+         "2,8,5", "2,16,7", "2,12,32", "2,8,1", "2,16,64", "2,4,3,256", "2,4,7,128", "2,8,32,128", "2,4,2,256"]
 
 
 @pytest.mark.parametrize("tile", TILES)
