@@ -126,6 +126,24 @@ def conv32(x, w, b):
     return conv(np.float32(x), np.float32(w), np.float32(b)).astype(np.float32).astype(np.float64)
 
 
+def conv_in_fma32(x, w, b):
+    """The first layer as the GPU computes it: per output channel an FP32
+    fused multiply-add chain a = b; a = fma(w[t][c][ci], x_t[ci], a) over
+    taps t = 0..8 and input channels ci = 0, 1.  Each step is emulated in
+    FP64 (the product of two FP32 values is exact there) with one rounding
+    to FP32."""
+    B, H, W, _ = x.shape
+    x = f32(x)
+    w = f32(w)
+    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+    a = np.broadcast_to(f32(b), (B, H, W, w.shape[1])).copy()
+    for t in range(9):
+        dy, dx = divmod(t, 3)
+        for ci in range(2):
+            a = f32(xp[:, dy:dy + H, dx:dx + W, ci][..., None] * w[t][:, ci] + a)
+    return a
+
+
 def conv_bwd32(x, w, dz):
     dx, dw, db = conv_bwd(np.float32(x), np.float32(w), np.float32(dz))
     return tuple(a.astype(np.float32).astype(np.float64) for a in (dx, dw, db))
@@ -136,7 +154,8 @@ def forward(params, x, emulate=False):
 
     emulate=True computes as the GPU kernels do: FP32 input and first/last
     layer weights, BF16 weights of the 32->32 layers, BF16 activations and
-    FP32 accumulation.  BF16 storage makes a network of this depth sensitive
+    FP32 accumulation (the first layer as the GPU's chain of fused
+    multiply-adds, bit for bit).  BF16 storage makes a network of this depth sensitive
     to the last bit of each accumulation (one in ten activations of the
     coarse levels rounds to the other neighbouring BF16 value when only the
     accumulation order changes), so only an oracle that accumulates in FP32
@@ -148,7 +167,7 @@ def forward(params, x, emulate=False):
     act = {"x": qw(np.asarray(x, dtype=np.float64))}
     for op, src, dst in GRAPH:
         if op == "in":
-            act[dst] = q(relu(cv(act[src], qw(win), qw(bin_))))
+            act[dst] = q(relu(conv_in_fma32(act[src], win, bin_) if emulate else conv(act[src], win, bin_)))
         elif op == "pool":
             act[dst] = q(qw(pool(act[src])))
         elif op == "up":

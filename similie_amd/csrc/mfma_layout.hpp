@@ -1,4 +1,4 @@
-// Lane layouts of the gfx950 BF16 matrix-core instructions used by the
+// Lane layouts of the gfx950 matrix-core instructions used by the
 // learned preconditioner's convolutions (FP32 accumulate).
 //
 // v_mfma_f32_32x32x16_bf16  D(32×32) += A(32×16) · B(16×32)
@@ -10,6 +10,10 @@
 //   A: lane l holds row m = l % 16, columns k = 8·(l / 16) + i
 //   B: lane l holds column n = l % 16, rows k = 8·(l / 16) + i
 //   D: lane l holds column n = l % 16, rows m = 4·(l / 16) + j in acc[j]
+// v_mfma_f32_32x32x2_f32   D(32×32) += A(32×2) · B(2×32), FP32 operands
+//   A: lane l holds row m = l % 32, column k = l / 32
+//   B: lane l holds column n = l % 32, row k = l / 32
+//   D: as for v_mfma_f32_32x32x16_bf16
 // These mappings are checked against a host matrix product by
 // tests/test_gpu_mfma_layout.py before any kernel relies on them.
 #pragma once
@@ -32,6 +36,19 @@ struct Mfma32 {
     __device__ static f32x16 mma(bf16x8 a, bf16x8 b, f32x16 c)
     {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+struct Mfma32F32 {
+    __device__ static int a_row(int lane) { return lane & 31; }
+    __device__ static int a_k(int lane) { return lane >> 5; }
+    __device__ static int b_col(int lane) { return lane & 31; }
+    __device__ static int b_k(int lane) { return lane >> 5; }
+    __device__ static int d_col(int lane) { return lane & 31; }
+    __device__ static int d_row(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+    __device__ static f32x16 mma(float a, float b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
     }
 };
 

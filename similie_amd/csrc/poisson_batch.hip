@@ -309,30 +309,55 @@ __global__ __launch_bounds__(NT) void k_tl_pre(int n, const T* __restrict__ wx, 
     block_partials<1>(v, part);
 }
 
-// network input: channel 0 = r₁ / rms(r₁), channel 1 = normalised log κ
-__global__ __launch_bounds__(NT) void k_tl_input(int n, const float* __restrict__ r1, const float* __restrict__ logk,
-                                                 const Scal* __restrict__ sc, float* __restrict__ xin)
+// The problems that still use the network (not converged, not fallen back
+// to Jacobi) get consecutive slots of the network's batch, in problem order:
+// slot[b] = rank among them or −1, *count = their number.  One wavefront.
+__global__ __launch_bounds__(64) void k_tl_compact(int B, const Scal* __restrict__ sc, int* __restrict__ slot,
+                                                   int* __restrict__ count)
 {
-    const size_t nn = (size_t)n * n, base = (size_t)blockIdx.y * nn;
+    const int lane = threadIdx.x;
+    int next = 0;
+    for (int b0 = 0; b0 < B; b0 += 64) {
+        const int b = b0 + lane;
+        const bool active = b < B && !sc[b].done && !sc[b].fallback;
+        const unsigned long long mask = __ballot(active);
+        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        if (b < B) slot[b] = active ? next + rank : -1;
+        next += __popcll(mask);
+    }
+    if (lane == 0) *count = next;
+}
+
+// network input in the problem's slot: channel 0 = r₁ / rms(r₁), channel 1 =
+// normalised log κ
+__global__ __launch_bounds__(NT) void k_tl_input(int n, const float* __restrict__ r1, const float* __restrict__ logk,
+                                                 const Scal* __restrict__ sc, const int* __restrict__ slot,
+                                                 float* __restrict__ xin)
+{
+    const int sl = slot[blockIdx.y];
+    if (sl < 0) return;
+    const size_t nn = (size_t)n * n, base = (size_t)blockIdx.y * nn, sbase = (size_t)sl * nn;
     const double rho = sc[blockIdx.y].rho1;
     const float inv = rho > 0.0 ? (float)(1.0 / rho) : 0.f;
     NODE_LOOP
     {
-        xin[(base + t) * 2] = r1[base + t] * inv;
-        xin[(base + t) * 2 + 1] = logk[base + t];
+        xin[(sbase + t) * 2] = r1[base + t] * inv;
+        xin[(sbase + t) * 2 + 1] = logk[base + t];
     }
 }
 
-// A n (n zero on the boundary ring); partials nᵀr₁, nᵀA n
+// A n (n, the network output in the problem's slot, zero on the boundary
+// ring); partials nᵀr₁, nᵀA n
 template <class T>
 __global__ __launch_bounds__(NT) void k_tl_an(int n, const T* __restrict__ wx, const T* __restrict__ wy,
-                                              const float* __restrict__ zn, const float* __restrict__ r1,
-                                              float* __restrict__ an, const Scal* __restrict__ sc,
-                                              double* __restrict__ part)
+                                              const float* __restrict__ zn, const int* __restrict__ slot,
+                                              const float* __restrict__ r1, float* __restrict__ an,
+                                              const Scal* __restrict__ sc, double* __restrict__ part)
 {
-    if (sc[blockIdx.y].done) return;
+    const int sl = slot[blockIdx.y];
+    if (sl < 0) return; // converged, or on the Jacobi fallback: no network output
     const size_t nn = (size_t)n * n, base = (size_t)blockIdx.y * nn;
-    const float* nb = zn + base;
+    const float* nb = zn + (size_t)sl * nn;
     double v[2] = {0.0, 0.0};
     NODE_LOOP
     {
@@ -359,15 +384,17 @@ __global__ __launch_bounds__(NT) void k_tl_an(int n, const T* __restrict__ wx, c
 template <class T>
 __global__ __launch_bounds__(NT) void k_tl_post(int n, const double* __restrict__ r, const double* __restrict__ dinv,
                                                 const float* __restrict__ r1, const float* __restrict__ zn,
-                                                const float* __restrict__ an, const T* __restrict__ q_old,
-                                                float* __restrict__ z, const Scal* __restrict__ sc,
-                                                double* __restrict__ part)
+                                                const int* __restrict__ slot, const float* __restrict__ an,
+                                                const T* __restrict__ q_old, float* __restrict__ z,
+                                                const Scal* __restrict__ sc, double* __restrict__ part)
 {
     const Scal& s = sc[blockIdx.y];
     if (s.done) return;
     const double c = s.cnet;
-    const bool fb = s.fallback != 0;
+    const int sl = slot[blockIdx.y];
+    const bool fb = sl < 0; // on the Jacobi fallback since the iteration began
     const size_t nn = (size_t)n * n, base = (size_t)blockIdx.y * nn;
+    const float* nb = zn + (size_t)(fb ? 0 : sl) * nn;
     double v[1] = {0.0};
     NODE_LOOP
     {
@@ -376,7 +403,7 @@ __global__ __launch_bounds__(NT) void k_tl_post(int n, const double* __restrict_
         double zv = 0.0;
         if (interior(i, j, n))
             zv = fb ? dinv[g] * r[g]
-                    : kOmega * dinv[g] * r[g] + c * (double)zn[g] +
+                    : kOmega * dinv[g] * r[g] + c * (double)nb[t] +
                           kOmega * dinv[g] * ((double)r1[g] - c * (double)an[g]);
         z[g] = (float)zv;
         v[0] += (double)(float)zv * (double)q_old[g];
@@ -558,7 +585,9 @@ struct sil_poisson2d_batch_s {
     double *p64[2] = {nullptr, nullptr}, *q64[2] = {nullptr, nullptr}, *z64 = nullptr;
     float *p32[2] = {nullptr, nullptr}, *q32[2] = {nullptr, nullptr}, *z32 = nullptr;
     float* xin = nullptr;  // network input [B][n][n][2]
-    float* zn = nullptr;   // network output [B][n][n]
+    float* zn = nullptr;   // network output [B][n][n], problems in their slots
+    int* slot = nullptr;   // network slot of each problem, −1 when it needs none
+    int* nact = nullptr;   // number of occupied slots
     float* r1 = nullptr;   // residual after pre-smoothing
     float* an = nullptr;   // A · network output
     float* zf = nullptr;   // preconditioned residual of flexible CG
@@ -631,13 +660,17 @@ int fcg_iteration(Batch* h, sil_unet net, T* const* p, T* const* q, const T* wx,
     hipLaunchKernelGGL(sil::k_tl_pre<T>, grid_of(h), dim3(NT), 0, s, h->n, wx, wy, h->r, h->dinv, h->r1, h->sc,
                        h->part);
     scalars(h, sil::PH_TL_RHO, tol2, max_iter, 0, s);
-    hipLaunchKernelGGL(sil::k_tl_input, grid_of(h), dim3(NT), 0, s, h->n, h->r1, h->logk, h->sc, h->xin);
-    const int rc = sil_unet_forward(net, h->B, h->xin, h->zn, s);
+    // the network runs only on the problems that still use it, packed into
+    // the first slots of its batch
+    hipLaunchKernelGGL(sil::k_tl_compact, dim3(1), dim3(64), 0, s, h->B, h->sc, h->slot, h->nact);
+    hipLaunchKernelGGL(sil::k_tl_input, grid_of(h), dim3(NT), 0, s, h->n, h->r1, h->logk, h->sc, h->slot, h->xin);
+    const int rc = sil::unet::forward_active(net, h->B, h->nact, h->xin, h->zn, s);
     if (rc != SIL_OK) return rc;
-    hipLaunchKernelGGL(sil::k_tl_an<T>, grid_of(h), dim3(NT), 0, s, h->n, wx, wy, h->zn, h->r1, h->an, h->sc, h->part);
+    hipLaunchKernelGGL(sil::k_tl_an<T>, grid_of(h), dim3(NT), 0, s, h->n, wx, wy, h->zn, h->slot, h->r1, h->an, h->sc,
+                       h->part);
     scalars(h, sil::PH_TL_C, tol2, max_iter, 0, s);
-    hipLaunchKernelGGL(sil::k_tl_post<T>, grid_of(h), dim3(NT), 0, s, h->n, h->r, h->dinv, h->r1, h->zn, h->an,
-                       q[parity ^ 1], h->zf, h->sc, h->part);
+    hipLaunchKernelGGL(sil::k_tl_post<T>, grid_of(h), dim3(NT), 0, s, h->n, h->r, h->dinv, h->r1, h->zn, h->slot,
+                       h->an, q[parity ^ 1], h->zf, h->sc, h->part);
     scalars(h, sil::PH_FCG_BETA, tol2, max_iter, 0, s);
     hipLaunchKernelGGL(sil::k_fcg_apply<T>, grid_of(h), dim3(NT), 0, s, h->n, wx, wy, h->zf, p[parity ^ 1], p[parity],
                        q[parity], h->r, h->sc, h->part);
@@ -780,7 +813,10 @@ int sil_poisson2d_batch_solve(sil_poisson2d_batch h, const double* b, double* x,
         A(h->p64[0], h->total), A(h->p64[1], h->total), A(h->q64[0], h->total), A(h->q64[1], h->total);
         if (!learned) A(h->z64, h->total);
     }
-    if (learned) A(h->xin, h->total * 2), A(h->zn, h->total), A(h->r1, h->total), A(h->an, h->total), A(h->zf, h->total);
+    if (learned) {
+        A(h->xin, h->total * 2), A(h->zn, h->total), A(h->r1, h->total), A(h->an, h->total), A(h->zf, h->total);
+        A(h->slot, h->B), A(h->nact, 1);
+    }
     if (rc != SIL_OK) return rc;
 
     SIL_CHECK_HIP(hipEventRecord(h->ev, us));

@@ -39,11 +39,16 @@ constexpr int CP = 40;     // padded channel stride in LDS
 constexpr int TH = 8;      // output tile rows
 constexpr int TW = 32;     // output tile columns
 constexpr int NT = 256;    // threads per workgroup
-constexpr long kConvGrid = 256 * 2; // persistent convolution workgroups: two per compute unit
 
 using bf16 = __bf16;
 
 enum StageMode { ST_PLAIN = 0, ST_UPADD = 1, ST_POOL = 2, ST_MASK = 3 };
+
+// persistent convolution workgroups per compute unit: two, one for the
+// pooling mode (its prefetch of four raw chunks per staged chunk needs more
+// than half of a SIMD's registers)
+constexpr int conv_wgs_per_cu(int mode) { return mode == ST_POOL ? 1 : 2; }
+constexpr long kComputeUnits = 256;
 
 struct alignas(16) Chunk8 {
     bf16 v[8];
@@ -66,39 +71,74 @@ __device__ __forceinline__ Chunk8 zero_chunk()
 //   ST_POOL   2x2 average of X               X [B][2H][2W]
 //   ST_MASK   X ⊙ [X2 > 0]                   input gradient: X = dY, X2 = forward output
 // Values are rounded to BF16 exactly as a stored tensor would be.
+// The raw chunks one staged chunk is made of (the loads of a prefetch), and
+// their combination (done when the chunk is written to LDS, so that the
+// loads of the next tile stay in flight while the current one is computed).
+template <int MODE>
+struct RawChunk {
+    static constexpr int n = MODE == ST_POOL ? 4 : (MODE == ST_PLAIN ? 1 : 2);
+    Chunk8 c[n];
+};
+
+template <int MODE>
+__device__ __forceinline__ void load_raw(RawChunk<MODE>& r, const bf16* __restrict__ X, const bf16* __restrict__ X2,
+                                         int b, int gy, int gx, int part, int H, int W)
+{
+    if (MODE == ST_POOL) {
+        const int W2 = 2 * W;
+        const size_t base = (((size_t)b * 2 * H + 2 * gy) * W2 + 2 * gx) * C + part * 8;
+        r.c[0] = *reinterpret_cast<const Chunk8*>(&X[base]);
+        r.c[1] = *reinterpret_cast<const Chunk8*>(&X[base + C]);
+        r.c[2] = *reinterpret_cast<const Chunk8*>(&X[base + (size_t)W2 * C]);
+        r.c[3] = *reinterpret_cast<const Chunk8*>(&X[base + (size_t)W2 * C + C]);
+        return;
+    }
+    const size_t off = (((size_t)b * H + gy) * W + gx) * C + part * 8;
+    r.c[0] = *reinterpret_cast<const Chunk8*>(&X[off]);
+    if (MODE == ST_UPADD) {
+        const size_t q = (((size_t)b * (H / 2) + gy / 2) * (W / 2) + gx / 2) * C + part * 8;
+        r.c[1] = *reinterpret_cast<const Chunk8*>(&X2[q]);
+    } else if (MODE == ST_MASK) {
+        r.c[1] = *reinterpret_cast<const Chunk8*>(&X2[off]);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void zero_raw(RawChunk<MODE>& r)
+{
+#pragma unroll
+    for (int i = 0; i < RawChunk<MODE>::n; ++i) r.c[i] = zero_chunk();
+}
+
+template <int MODE>
+__device__ __forceinline__ Chunk8 combine(const RawChunk<MODE>& r)
+{
+    Chunk8 v = r.c[0];
+    if (MODE == ST_POOL) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float s = from_bf16(r.c[0].v[i]) + from_bf16(r.c[1].v[i]) + from_bf16(r.c[2].v[i]) +
+                            from_bf16(r.c[3].v[i]);
+            v.v[i] = to_bf16(0.25f * s);
+        }
+    } else if (MODE == ST_UPADD) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v.v[i] = to_bf16(from_bf16(r.c[1].v[i]) + from_bf16(r.c[0].v[i]));
+    } else if (MODE == ST_MASK) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (!(from_bf16(r.c[1].v[i]) > 0.f)) v.v[i] = to_bf16(0.f);
+    }
+    return v;
+}
+
 template <int MODE>
 __device__ __forceinline__ Chunk8 stage_chunk(const bf16* __restrict__ X, const bf16* __restrict__ X2, int b, int gy,
                                               int gx, int part, int H, int W)
 {
-    Chunk8 v;
-    if (MODE == ST_POOL) {
-        const int W2 = 2 * W;
-        const size_t base = (((size_t)b * 2 * H + 2 * gy) * W2 + 2 * gx) * C + part * 8;
-        const Chunk8 p00 = *reinterpret_cast<const Chunk8*>(&X[base]);
-        const Chunk8 p01 = *reinterpret_cast<const Chunk8*>(&X[base + C]);
-        const Chunk8 p10 = *reinterpret_cast<const Chunk8*>(&X[base + (size_t)W2 * C]);
-        const Chunk8 p11 = *reinterpret_cast<const Chunk8*>(&X[base + (size_t)W2 * C + C]);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const float s = from_bf16(p00.v[i]) + from_bf16(p01.v[i]) + from_bf16(p10.v[i]) + from_bf16(p11.v[i]);
-            v.v[i] = to_bf16(0.25f * s);
-        }
-        return v;
-    }
-    const size_t off = (((size_t)b * H + gy) * W + gx) * C + part * 8;
-    v = *reinterpret_cast<const Chunk8*>(&X[off]);
-    if (MODE == ST_UPADD) {
-        const size_t q = (((size_t)b * (H / 2) + gy / 2) * (W / 2) + gx / 2) * C + part * 8;
-        const Chunk8 c = *reinterpret_cast<const Chunk8*>(&X2[q]);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) v.v[i] = to_bf16(from_bf16(c.v[i]) + from_bf16(v.v[i]));
-    } else if (MODE == ST_MASK) {
-        const Chunk8 m = *reinterpret_cast<const Chunk8*>(&X2[off]);
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (!(from_bf16(m.v[i]) > 0.f)) v.v[i] = to_bf16(0.f);
-    }
-    return v;
+    RawChunk<MODE> r;
+    load_raw<MODE>(r, X, X2, b, gy, gx, part, H, W);
+    return combine<MODE>(r);
 }
 
 // the (TH+2)×(TW+2) input tile with origin (oy0−1, ox0−1), zero outside
@@ -129,11 +169,14 @@ __device__ __forceinline__ void stage_tile(bf16* __restrict__ sX, const bf16* __
 constexpr int kStageChunks = (TH + 2) * (TW + 2) * 4; // 16-byte chunks of one input tile
 constexpr int kPerThread = (kStageChunks + NT - 1) / NT;
 
-template <int MODE>
-__global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ X2,
+template <int MODE, bool HEAD = false>
+__global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ X2,
                                                const bf16* __restrict__ Wt, const float* __restrict__ bias,
-                                               bf16* __restrict__ Y, int B, int H, int W, int relu)
+                                               bf16* __restrict__ Y, float* __restrict__ Yh, int Bmax,
+                                               const int* __restrict__ dB, int H, int W, int relu)
 {
+    // images: the first *dB of the batch when a device-side count is given
+    const int B = dB ? min(*dB, Bmax) : Bmax;
     __shared__ __attribute__((aligned(16))) bf16 sX[2][(TH + 2) * (TW + 2) * CP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_x = (W + TW - 1) / TW, tiles_y = (H + TH - 1) / TH;
@@ -151,36 +194,37 @@ __global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const
         for (int kh = 0; kh < 2; ++kh)
             bw[tap][kh] = *reinterpret_cast<const bf16x8*>(&Wt[(tap * C + co_b) * C + kh * 16 + k0]);
     const int co = Mfma32::d_col(lane);
-    const float bv = bias ? bias[co] : 0.f;
+    const float bv = (bias && !HEAD) ? bias[co] : 0.f;
 
     auto coords = [&](int t, int& b, int& oy0, int& ox0) {
         b = t / (tiles_x * tiles_y);
         oy0 = ((t / tiles_x) % tiles_y) * TH;
         ox0 = (t % tiles_x) * TW;
     };
-    auto fetch = [&](int t, Chunk8 (&pre)[kPerThread]) {
+    // fetch issues the loads of a tile; put combines them into LDS
+    auto fetch = [&](int t, RawChunk<MODE> (&pre)[kPerThread]) {
         int b, oy0, ox0;
         coords(t, b, oy0, ox0);
 #pragma unroll
         for (int k = 0; k < kPerThread; ++k) {
             const int c = tid + k * NT;
-            pre[k] = zero_chunk();
+            zero_raw<MODE>(pre[k]);
             if (c < kStageChunks) {
                 const int pix = c >> 2, part = c & 3;
                 const int gy = oy0 - 1 + pix / (TW + 2), gx = ox0 - 1 + pix % (TW + 2);
-                if (gy >= 0 && gy < H && gx >= 0 && gx < W) pre[k] = stage_chunk<MODE>(X, X2, b, gy, gx, part, H, W);
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W) load_raw<MODE>(pre[k], X, X2, b, gy, gx, part, H, W);
             }
         }
     };
-    auto put = [&](bf16* dst, const Chunk8 (&pre)[kPerThread]) {
+    auto put = [&](bf16* dst, const RawChunk<MODE> (&pre)[kPerThread]) {
 #pragma unroll
         for (int k = 0; k < kPerThread; ++k) {
             const int c = tid + k * NT;
-            if (c < kStageChunks) *reinterpret_cast<Chunk8*>(&dst[(c >> 2) * CP + (c & 3) * 8]) = pre[k];
+            if (c < kStageChunks) *reinterpret_cast<Chunk8*>(&dst[(c >> 2) * CP + (c & 3) * 8]) = combine<MODE>(pre[k]);
         }
     };
 
-    Chunk8 pre[kPerThread];
+    RawChunk<MODE> pre[kPerThread];
     fetch(tile, pre);
     put(sX[0], pre);
     __syncthreads();
@@ -210,27 +254,47 @@ __global__ __launch_bounds__(NT) void k_conv33(const bf16* __restrict__ X, const
         }
         __syncthreads(); // every wavefront is done with the input tile in sX[buf]
 
-        // results → sX[buf] as [8 rows][32 pixels][32 channels] (stride CP)
-        bf16* out_s = sX[buf];
+        if constexpr (HEAD) {
+            // output head: columns 0 and 1 (hi / lo weights) → sX[buf] as
+            // [8 rows][32 pixels][2] FP32, then one FP32 output per pixel
+            float* ps = reinterpret_cast<float*>(sX[buf]);
+            if (co < 2) {
 #pragma unroll
-        for (int m = 0; m < 2; ++m)
+                for (int m = 0; m < 2; ++m)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                float v = acc[m][r] + bv;
-                if (relu) v = fmaxf(v, 0.f);
-                out_s[((2 * wave + m) * TW + Mfma32::d_row(lane, r)) * CP + co] = to_bf16(v);
+                    for (int r = 0; r < 16; ++r) ps[((2 * wave + m) * TW + Mfma32::d_row(lane, r)) * 2 + co] = acc[m][r];
             }
-        if (next < ntiles) put(sX[buf ^ 1], pre);
-        __syncthreads();
+            if (next < ntiles) put(sX[buf ^ 1], pre);
+            __syncthreads();
+            int b, oy0, ox0;
+            coords(tile, b, oy0, ox0);
+            for (int c = tid; c < TH * TW; c += NT) {
+                const int gy = oy0 + c / TW, gx = ox0 + c % TW;
+                if (gy < H && gx < W) Yh[((size_t)b * H + gy) * W + gx] = ps[c * 2] + ps[c * 2 + 1] + bias[0];
+            }
+        } else {
+            // results → sX[buf] as [8 rows][32 pixels][32 channels] (stride CP)
+            bf16* out_s = sX[buf];
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    float v = acc[m][r] + bv;
+                    if (relu) v = fmaxf(v, 0.f);
+                    out_s[((2 * wave + m) * TW + Mfma32::d_row(lane, r)) * CP + co] = to_bf16(v);
+                }
+            if (next < ntiles) put(sX[buf ^ 1], pre);
+            __syncthreads();
 
-        int b, oy0, ox0;
-        coords(tile, b, oy0, ox0);
-        for (int c = tid; c < TH * TW * 4; c += NT) {
-            const int pix = c >> 2, part = c & 3;
-            const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
-            if (gy < H && gx < W)
-                *reinterpret_cast<Chunk8*>(&Y[(((size_t)b * H + gy) * W + gx) * C + part * 8]) =
-                    *reinterpret_cast<const Chunk8*>(&out_s[pix * CP + part * 8]);
+            int b, oy0, ox0;
+            coords(tile, b, oy0, ox0);
+            for (int c = tid; c < TH * TW * 4; c += NT) {
+                const int pix = c >> 2, part = c & 3;
+                const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
+                if (gy < H && gx < W)
+                    *reinterpret_cast<Chunk8*>(&Y[(((size_t)b * H + gy) * W + gx) * C + part * 8]) =
+                        *reinterpret_cast<const Chunk8*>(&out_s[pix * CP + part * 8]);
+            }
         }
         buf ^= 1;
     }
@@ -337,15 +401,21 @@ __global__ void k_reduce_parts(const float* __restrict__ part, int parts, int n,
 // a0 = relu(conv_in(x)): x FP32 [B][H][W][2], weights [tap][co][ci=2].
 // A workgroup stages a (TH+2)×(TW+2) tile of x and writes the 32 channels of
 // one pixel per thread (four 16-byte stores).
+// 3×3 conv 2→32 + ReLU on FP32 input, FP32 fused multiply-adds.  The
+// weights are read at workgroup-uniform addresses (scalar loads, scalar
+// operands of the FMAs), each thread computes the 32 channels of one pixel,
+// and the results go through LDS so that every global store is 8 channels
+// (16 bytes).
 __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, const float* __restrict__ w,
-                                                const float* __restrict__ bias, bf16* __restrict__ y, int H, int W)
+                                                const float* __restrict__ bias, bf16* __restrict__ y, int H, int W,
+                                                const int* __restrict__ dB)
 {
-    __shared__ float sw[9 * C * 2 + C];
     __shared__ float sx[(TH + 2) * (TW + 2) * 2];
+    __shared__ __attribute__((aligned(16))) bf16 sY[TH * TW * CP];
     const int b = blockIdx.z, oy0 = blockIdx.y * TH, ox0 = blockIdx.x * TW;
-    for (int i = threadIdx.x; i < 9 * C * 2; i += NT) sw[i] = w[i];
-    for (int i = threadIdx.x; i < C; i += NT) sw[9 * C * 2 + i] = bias[i];
-    for (int i = threadIdx.x; i < (TH + 2) * (TW + 2); i += NT) {
+    if (dB && b >= *dB) return; // uniform per workgroup
+    const int tid = threadIdx.x;
+    for (int i = tid; i < (TH + 2) * (TW + 2); i += NT) {
         const int gy = oy0 - 1 + i / (TW + 2), gx = ox0 - 1 + i % (TW + 2);
         const bool ok = gy >= 0 && gy < H && gx >= 0 && gx < W;
         const size_t q = ((size_t)b * H + gy) * W + gx;
@@ -353,9 +423,7 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
         sx[i * 2 + 1] = ok ? x[q * 2 + 1] : 0.f;
     }
     __syncthreads();
-    const int ty = threadIdx.x / TW, tx = threadIdx.x % TW;
-    const int gy = oy0 + ty, gx = ox0 + tx;
-    if (gy >= H || gx >= W) return;
+    const int ty = tid / TW, tx = tid % TW;
     float in[18];
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
@@ -363,19 +431,29 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
         in[2 * t] = sx[i * 2];
         in[2 * t + 1] = sx[i * 2 + 1];
     }
-    bf16* out = y + (((size_t)b * H + gy) * W + gx) * C;
 #pragma unroll
     for (int part = 0; part < 4; ++part) {
         Chunk8 o;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const int co = part * 8 + c;
-            float a = sw[9 * C * 2 + co];
+            float a = bias[co];
 #pragma unroll
-            for (int t = 0; t < 9; ++t) a += sw[(t * C + co) * 2] * in[2 * t] + sw[(t * C + co) * 2 + 1] * in[2 * t + 1];
+            for (int t = 0; t < 9; ++t) {
+                a = fmaf(w[(t * C + co) * 2], in[2 * t], a);
+                a = fmaf(w[(t * C + co) * 2 + 1], in[2 * t + 1], a);
+            }
             o.v[c] = to_bf16(fmaxf(a, 0.f));
         }
-        *reinterpret_cast<Chunk8*>(&out[part * 8]) = o;
+        *reinterpret_cast<Chunk8*>(&sY[tid * CP + part * 8]) = o;
+    }
+    __syncthreads();
+    for (int c = tid; c < TH * TW * 4; c += NT) {
+        const int pix = c >> 2, part = c & 3;
+        const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
+        if (gy < H && gx < W)
+            *reinterpret_cast<Chunk8*>(&y[(((size_t)b * H + gy) * W + gx) * C + part * 8]) =
+                *reinterpret_cast<const Chunk8*>(&sY[pix * CP + part * 8]);
     }
 }
 
@@ -418,34 +496,16 @@ __global__ void k_conv_in_wgrad(const float* __restrict__ x, const bf16* __restr
     }
 }
 
-// out = conv_out(c0): 32 → 1, weights [tap][ci], bias scalar.  A workgroup
-// stages the c0 tile in LDS (as the 32→32 convolutions do) and each thread
-// forms one output pixel from 36 sixteen-byte LDS reads.
-__global__ __launch_bounds__(NT) void k_conv_out(const bf16* __restrict__ x, const float* __restrict__ w,
-                                                 const float* __restrict__ bias, float* __restrict__ y, int H, int W)
+// head weights w[tap][ci] (FP32) → B matrix [tap][co][ci] (BF16) with
+// co = 0: hi = bf16(w), co = 1: lo = bf16(w − hi), other columns zero
+__global__ void k_pack_head(const float* __restrict__ w, bf16* __restrict__ Wt)
 {
-    __shared__ __attribute__((aligned(16))) bf16 sX[(TH + 2) * (TW + 2) * CP];
-    __shared__ float sw[9 * C + 1];
-    const int b = blockIdx.z, oy0 = blockIdx.y * TH, ox0 = blockIdx.x * TW;
-    for (int i = threadIdx.x; i < 9 * C; i += NT) sw[i] = w[i];
-    if (threadIdx.x == 0) sw[9 * C] = bias[0];
-    stage_tile<ST_PLAIN>(sX, x, nullptr, b, oy0, ox0, H, W);
-    __syncthreads();
-    const int ty = threadIdx.x / TW, tx = threadIdx.x % TW;
-    const int gy = oy0 + ty, gx = ox0 + tx;
-    if (gy >= H || gx >= W) return;
-    float a = sw[9 * C];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const bf16* px = &sX[((ty + t / 3) * (TW + 2) + tx + t % 3) * CP];
-#pragma unroll
-        for (int part = 0; part < 4; ++part) {
-            const Chunk8 v = *reinterpret_cast<const Chunk8*>(&px[part * 8]);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) a += sw[t * C + part * 8 + i] * from_bf16(v.v[i]);
-        }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 9 * C * C; i += gridDim.x * blockDim.x) {
+        const int t = i / (C * C), co = (i / C) % C, ci = i % C;
+        const float wv = w[t * C + ci];
+        const bf16 hi = to_bf16(wv);
+        Wt[i] = co == 0 ? hi : (co == 1 ? to_bf16(wv - from_bf16(hi)) : to_bf16(0.f));
     }
-    y[((size_t)b * H + gy) * W + gx] = a;
 }
 
 // backward of conv_out: dX[q][ci] = Σ_t dout[q − off(t)] · w[t][ci] (BF16),
@@ -679,15 +739,17 @@ int by_mode(int mode, F&& f)
 }  // namespace
 
 int conv33(int mode, const bf16* X, const bf16* X2, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W,
-           bool relu, hipStream_t s)
+           bool relu, hipStream_t s, const int* dB)
 {
-    // persistent grid: two workgroups per compute unit (the LDS of a
-    // workgroup is 54 KB), capped by the number of tiles
+    // persistent grid (the LDS of a workgroup is 54 KB), capped by the
+    // number of tiles
     const long ntiles = (long)((W + TW - 1) / TW) * ((H + TH - 1) / TH) * B;
-    const unsigned grid = (unsigned)(ntiles < kConvGrid ? ntiles : kConvGrid);
     return by_mode(mode, [&](auto m) -> int {
-        hipLaunchKernelGGL(k_conv33<decltype(m)::value>, dim3(grid), dim3(NT), 0, s, X, X2, Wt, bias, Y, B, H, W,
-                           relu ? 1 : 0);
+        constexpr int M = decltype(m)::value;
+        const long cap = kComputeUnits * conv_wgs_per_cu(M);
+        const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
+        hipLaunchKernelGGL((k_conv33<M, false>), dim3(grid), dim3(NT), 0, s, X, X2, Wt, bias, Y, nullptr, B, dB, H,
+                           W, relu ? 1 : 0);
         SIL_LAUNCH_CHECK();
         return (int)SIL_OK;
     });
@@ -711,9 +773,11 @@ int conv33_wgrad(int mode, const bf16* X, const bf16* X2, const bf16* dY, const 
     return SIL_OK;
 }
 
-int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s)
+int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s,
+            const int* dB)
 {
-    hipLaunchKernelGGL(k_conv_in, dim3((W + TW - 1) / TW, (H + TH - 1) / TH, B), dim3(NT), 0, s, x, w, bias, y, H, W);
+    hipLaunchKernelGGL(k_conv_in, dim3((W + TW - 1) / TW, (H + TH - 1) / TH, B), dim3(NT), 0, s, x, w, bias, y, H, W,
+                       dB);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
@@ -727,9 +791,19 @@ int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, i
     return SIL_OK;
 }
 
-int conv_out(const bf16* x, const float* w, const float* bias, float* y, int B, int H, int W, hipStream_t s)
+// 3×3 conv 32→1 (the output head, no activation): the persistent
+// convolution with the B matrix of k_pack_head.  The FP32 weights are split
+// into w = hi + lo with hi, lo BF16 in columns 0 and 1, so the two output
+// columns sum to the product with the weights to about 16 bits, and the
+// activations (BF16 already) are multiplied exactly with FP32 accumulation.
+int conv_out(const bf16* x, const bf16* Wt, const float* bias, float* y, int B, int H, int W, hipStream_t s,
+             const int* dB)
 {
-    hipLaunchKernelGGL(k_conv_out, dim3((W + TW - 1) / TW, (H + TH - 1) / TH, B), dim3(NT), 0, s, x, w, bias, y, H, W);
+    const long ntiles = (long)((W + TW - 1) / TW) * ((H + TH - 1) / TH) * B;
+    const long cap = kComputeUnits * conv_wgs_per_cu(ST_PLAIN);
+    const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
+    hipLaunchKernelGGL((k_conv33<ST_PLAIN, true>), dim3(grid), dim3(NT), 0, s, x, nullptr, Wt, bias, nullptr, y, B, dB,
+                       H, W, 0);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
@@ -778,6 +852,13 @@ int adam(float* w, const float* g, float* m, float* v, int n, float lr, int step
 int pack33(const float* w, bf16* wf, bf16* wb, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pack33, dim3(36), dim3(256), 0, s, w, wf, wb);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int pack_head(const float* w, bf16* Wt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack_head, dim3(36), dim3(256), 0, s, w, Wt);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }

@@ -24,18 +24,22 @@ constexpr int kTileW = 32;
 enum StageModeHost { STAGE_PLAIN = 0, STAGE_UPADD = 1, STAGE_POOL = 2, STAGE_MASK = 3 };
 
 // Y = act(conv3x3(staged input) + bias), 32 → 32 channels on H×W images, zero padding
+// With dB, only the first *dB (device memory, read by the kernels) of the B
+// images are computed.
 int conv33(int mode, const bf16* X, const bf16* X2, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W,
-           bool relu, hipStream_t s);
+           bool relu, hipStream_t s, const int* dB = nullptr);
 // dW[tap][co][ci], db[co] of conv33 from its staged input, output gradient dY
 // and forward output Yf (ReLU mask); part_w holds parts·9·32·32 floats, part_b parts·32
 int conv33_wgrad(int mode, const bf16* X, const bf16* X2, const bf16* dY, const bf16* Yf, float* part_w,
                  float* part_b, int parts, float* dW, float* db, int B, int H, int W, hipStream_t s);
 // 2 → 32 input convolution (FP32 input) and its weight gradient (dWb = [dW(9·32·2) | db(32)])
-int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s);
+int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s,
+            const int* dB = nullptr);
 int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, int parts, float* dWb, int B, int H,
                   int W, hipStream_t s);
 // 32 → 1 output convolution (FP32 output) and its backward (dWb = [dW(9·32) | db])
-int conv_out(const bf16* x, const float* w, const float* bias, float* y, int B, int H, int W, hipStream_t s);
+int conv_out(const bf16* x, const bf16* Wt, const float* bias, float* y, int B, int H, int W, hipStream_t s,
+             const int* dB = nullptr);
 int conv_out_bwd(const bf16* x, const float* dout, const float* w, bf16* dx, float* part, int parts, float* dWb, int B,
                  int H, int W, hipStream_t s);
 // backward of the 2x2 average pooling of an H×W tensor (accumulates into dx)
@@ -49,8 +53,15 @@ int cos_loss(const float* N, const float* r, int rs, const float* e, const float
 int adam(float* w, const float* g, float* m, float* v, int n, float lr, int step, hipStream_t s);
 // BF16 forward weights and the flipped, channel-transposed input-gradient weights of one 32→32 layer
 int pack33(const float* w, bf16* wf, bf16* wb, hipStream_t s);
+// output-head weights [tap][ci] → BF16 B matrix with hi/lo columns
+int pack_head(const float* w, bf16* Wt, hipStream_t s);
 
 // maximum batch and image size of a network handle (defined with the handle)
+// Forward pass over the first *d_count (device memory, at most `batch`) of
+// the images in x; the rest of `out` is left as it is.  Used inside captured
+// solver iterations, where the number of images still needing the network is
+// only known on the device.
+int forward_active(sil_unet_s* net, int batch, const int* d_count, const float* x, float* out, hipStream_t s);
 void net_shape(const struct ::sil_unet_s* net, int* max_batch, int* height, int* width);
 
 }  // namespace sil::unet

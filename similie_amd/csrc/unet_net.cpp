@@ -42,6 +42,7 @@ struct sil_unet_s {
     float* adam_v = nullptr;
     bf16* wf = nullptr; // kLayers × 9·32·32 forward weights
     bf16* wb = nullptr; // kLayers × 9·32·32 input-gradient weights
+    bf16* wh = nullptr; // 9·32·32 output-head weights (hi / lo columns)
     // activations
     // (pooled and upsampled-plus-skip inputs are formed while staging)
     bf16 *a0 = nullptr, *e0 = nullptr, *c0 = nullptr;
@@ -92,6 +93,7 @@ int repack(sil_unet_s* u, hipStream_t s)
     for (int l = 1; l <= kLayers; ++l)
         UNET_TRY(sil::unet::pack33(u->params + layer_off(l), u->wf + (size_t)(l - 1) * 9 * C * C,
                                    u->wb + (size_t)(l - 1) * 9 * C * C, s));
+    UNET_TRY(sil::unet::pack_head(u->params + kOutOff, u->wh, s));
     return SIL_OK;
 }
 
@@ -99,25 +101,25 @@ const bf16* WF(const sil_unet_s* u, int l) { return u->wf + (size_t)(l - 1) * 9 
 const bf16* WB(const sil_unet_s* u, int l) { return u->wb + (size_t)(l - 1) * 9 * C * C; }
 const float* BIAS(const sil_unet_s* u, int l) { return u->params + layer_off(l) + 9 * C * C; }
 
-int forward(sil_unet_s* u, int B, const float* x, float* out, hipStream_t s)
+int forward(sil_unet_s* u, int B, const float* x, float* out, hipStream_t s, const int* dB = nullptr)
 {
     namespace un = sil::unet;
     constexpr int P = un::STAGE_PLAIN, UP = un::STAGE_UPADD, PO = un::STAGE_POOL;
     const int *H = u->Hl, *W = u->Wl;
-    UNET_TRY(un::conv_in(x, u->params, u->params + 9 * C * 2, u->a0, B, H[0], W[0], s));
-    UNET_TRY(un::conv33(P, u->a0, nullptr, WF(u, 1), BIAS(u, 1), u->e0, B, H[0], W[0], true, s));
-    UNET_TRY(un::conv33(PO, u->e0, nullptr, WF(u, 2), BIAS(u, 2), u->a1, B, H[1], W[1], true, s));
-    UNET_TRY(un::conv33(P, u->a1, nullptr, WF(u, 3), BIAS(u, 3), u->e1, B, H[1], W[1], true, s));
-    UNET_TRY(un::conv33(PO, u->e1, nullptr, WF(u, 4), BIAS(u, 4), u->a2, B, H[2], W[2], true, s));
-    UNET_TRY(un::conv33(P, u->a2, nullptr, WF(u, 5), BIAS(u, 5), u->e2, B, H[2], W[2], true, s));
-    UNET_TRY(un::conv33(PO, u->e2, nullptr, WF(u, 6), BIAS(u, 6), u->a3, B, H[3], W[3], true, s));
-    UNET_TRY(un::conv33(P, u->a3, nullptr, WF(u, 7), BIAS(u, 7), u->b3, B, H[3], W[3], true, s));
-    UNET_TRY(un::conv33(UP, u->e2, u->b3, WF(u, 8), BIAS(u, 8), u->c2, B, H[2], W[2], true, s));
-    UNET_TRY(un::conv33(P, u->c2, nullptr, WF(u, 9), BIAS(u, 9), u->f2, B, H[2], W[2], true, s));
-    UNET_TRY(un::conv33(UP, u->e1, u->f2, WF(u, 10), BIAS(u, 10), u->c1, B, H[1], W[1], true, s));
-    UNET_TRY(un::conv33(P, u->c1, nullptr, WF(u, 11), BIAS(u, 11), u->f1, B, H[1], W[1], true, s));
-    UNET_TRY(un::conv33(UP, u->e0, u->f1, WF(u, 12), BIAS(u, 12), u->c0, B, H[0], W[0], true, s));
-    UNET_TRY(un::conv_out(u->c0, u->params + kOutOff, u->params + kOutOff + 9 * C, out, B, H[0], W[0], s));
+    UNET_TRY(un::conv_in(x, u->params, u->params + 9 * C * 2, u->a0, B, H[0], W[0], s, dB));
+    UNET_TRY(un::conv33(P, u->a0, nullptr, WF(u, 1), BIAS(u, 1), u->e0, B, H[0], W[0], true, s, dB));
+    UNET_TRY(un::conv33(PO, u->e0, nullptr, WF(u, 2), BIAS(u, 2), u->a1, B, H[1], W[1], true, s, dB));
+    UNET_TRY(un::conv33(P, u->a1, nullptr, WF(u, 3), BIAS(u, 3), u->e1, B, H[1], W[1], true, s, dB));
+    UNET_TRY(un::conv33(PO, u->e1, nullptr, WF(u, 4), BIAS(u, 4), u->a2, B, H[2], W[2], true, s, dB));
+    UNET_TRY(un::conv33(P, u->a2, nullptr, WF(u, 5), BIAS(u, 5), u->e2, B, H[2], W[2], true, s, dB));
+    UNET_TRY(un::conv33(PO, u->e2, nullptr, WF(u, 6), BIAS(u, 6), u->a3, B, H[3], W[3], true, s, dB));
+    UNET_TRY(un::conv33(P, u->a3, nullptr, WF(u, 7), BIAS(u, 7), u->b3, B, H[3], W[3], true, s, dB));
+    UNET_TRY(un::conv33(UP, u->e2, u->b3, WF(u, 8), BIAS(u, 8), u->c2, B, H[2], W[2], true, s, dB));
+    UNET_TRY(un::conv33(P, u->c2, nullptr, WF(u, 9), BIAS(u, 9), u->f2, B, H[2], W[2], true, s, dB));
+    UNET_TRY(un::conv33(UP, u->e1, u->f2, WF(u, 10), BIAS(u, 10), u->c1, B, H[1], W[1], true, s, dB));
+    UNET_TRY(un::conv33(P, u->c1, nullptr, WF(u, 11), BIAS(u, 11), u->f1, B, H[1], W[1], true, s, dB));
+    UNET_TRY(un::conv33(UP, u->e0, u->f1, WF(u, 12), BIAS(u, 12), u->c0, B, H[0], W[0], true, s, dB));
+    UNET_TRY(un::conv_out(u->c0, u->wh, u->params + kOutOff + 9 * C, out, B, H[0], W[0], s, dB));
     return SIL_OK;
 }
 
@@ -175,6 +177,14 @@ int check_batch(const sil_unet_s* u, int B)
 
 }  // namespace
 
+int sil::unet::forward_active(sil_unet_s* net, int batch, const int* d_count, const float* x, float* out, hipStream_t s)
+{
+    SIL_REQUIRE(net != nullptr && d_count != nullptr && x != nullptr && out != nullptr, SIL_ERR_INVALID,
+                "unet: null argument");
+    UNET_TRY(check_batch(net, batch));
+    return forward(net, batch, x, out, s, d_count);
+}
+
 void sil::unet::net_shape(const sil_unet_s* net, int* max_batch, int* height, int* width)
 {
     *max_batch = net->max_batch;
@@ -213,6 +223,7 @@ int sil_unet_create(sil_unet* out, int32_t max_batch, int32_t height, int32_t wi
     A(u->adam_v, kParams);
     A(u->wf, (size_t)kLayers * 9 * C * C);
     A(u->wb, (size_t)kLayers * 9 * C * C);
+    A(u->wh, (size_t)9 * C * C);
     for (bf16** p : {&u->a0, &u->e0, &u->c0, &u->g0a, &u->g0b}) A(*p, s0);
     for (bf16** p : {&u->a1, &u->e1, &u->c1, &u->f1, &u->g1a, &u->g1b}) A(*p, s1);
     for (bf16** p : {&u->a2, &u->e2, &u->c2, &u->f2, &u->g2a, &u->g2b}) A(*p, s2);

@@ -114,6 +114,7 @@ class UNet:
 
 
 METHODS = {
+    "cg_plain": (L.PRECOND_IDENTITY, L.PRECISION_FP64),
     "cg": (L.PRECOND_JACOBI, L.PRECISION_FP64),
     "cg_mixed": (L.PRECOND_JACOBI, L.PRECISION_MIXED),
     "fcg_learned": (L.PRECOND_LEARNED, L.PRECISION_FP64),
@@ -165,8 +166,8 @@ class PoissonBatch:
               record_every=0, max_records=0, stream=None):
         """Solves every problem; returns (x, results list, seconds[, records]).
 
-        method: "cg" (Jacobi CG, FP64), "cg_mixed", "fcg_learned" or
-        "fcg_learned_mixed".  With record_every > 0 the FP32 residuals and
+        method: "cg_plain" (unpreconditioned CG, FP64), "cg" (Jacobi CG,
+        FP64), "cg_mixed", "fcg_learned" or "fcg_learned_mixed".  With record_every > 0 the FP32 residuals and
         iterates (FP64) before every record_every iterations are returned as
         (r_records, x_records) of shape [count][B][n][n]."""
         _f64(rhs, "rhs", self.shape)

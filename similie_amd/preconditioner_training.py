@@ -164,7 +164,8 @@ def train(net, data, epochs, batch=32, lr_max=1e-3, lr_min=1e-5, seed=0, log=pri
     return history
 
 
-def evaluate(net, n_problems, n, sigma, corr, seed, methods=("cg", "cg_mixed", "fcg_learned", "fcg_learned_mixed"),
+def evaluate(net, n_problems, n, sigma, corr, seed,
+             methods=("cg_plain", "cg", "cg_mixed", "fcg_learned", "fcg_learned_mixed"),
              tol=1e-10, ablation=True, log=print):
     """Time to an FP64-verified tol of each method on one batch of held-out
     problems.  With ablation, flexible CG also runs with a network whose

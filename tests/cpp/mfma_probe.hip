@@ -1,6 +1,7 @@
-// Lane-layout check of the BF16 matrix-core instructions (one wavefront,
+// Lane-layout check of the matrix-core instructions (one wavefront,
 // in-bounds loads from host-provided A (M×K) and B (K×N) row-major FP32
-// matrices, rounded to BF16; D (M×N) row-major FP32).  Built by
+// matrices, rounded to BF16 for the BF16 instructions; D (M×N) row-major
+// FP32).  Built by
 // `python -m similie_amd.build` into tests/cpp/libmfma_probe.so.
 #include <hip/hip_runtime.h>
 
@@ -21,6 +22,24 @@ __global__ void k_probe(const float* A, const float* B, float* D)
     for (int r = 0; r < R; ++r) acc[r] = 0.f;
     acc = L::mma(a, b, acc);
     for (int r = 0; r < R; ++r) D[L::d_row(lane, r) * N + L::d_col(lane)] = acc[r];
+}
+
+__global__ void k_probe_f32(const float* A, const float* B, float* D)
+{
+    using L = Mfma32F32;
+    const int lane = threadIdx.x;
+    const float a = A[L::a_row(lane) * 2 + L::a_k(lane)];
+    const float b = B[L::b_k(lane) * 32 + L::b_col(lane)];
+    f32x16 acc;
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    acc = L::mma(a, b, acc);
+    for (int r = 0; r < 16; ++r) D[L::d_row(lane, r) * 32 + L::d_col(lane)] = acc[r];
+}
+
+extern "C" int probe_mfma32_f32(const float* A, const float* B, float* D)
+{
+    hipLaunchKernelGGL(k_probe_f32, dim3(1), dim3(64), 0, 0, A, B, D);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : 1;
 }
 
 extern "C" int probe_mfma32(const float* A, const float* B, float* D)

@@ -56,9 +56,15 @@ def weights_of(kappa):
 # (oracle.unet_oracle emulate=True), which they match up to accumulation
 # order, and with exact FP64, from which BF16 storage alone moves them by
 # about 1e-2 forward and 0.05-0.15 in the weight gradients of white-noise
-# inputs (the emulating oracle shows the same distance).
+# inputs (the emulating oracle shows the same distance).  The order still
+# matters through BF16 storage: the first layer matches the oracle bit for
+# bit, yet the matrix cores' summation order flips 4 of the 90k encoder
+# activations of the backward test's input to the neighbouring BF16 value,
+# the decoder grows that to one c0 value in twelve, and the weight
+# gradients move by 1-3 %.  A wrong index or a dropped term moves them by
+# tens of percent.
 FWD_TOL_EMULATED, FWD_TOL_FP64 = 1e-2, 3e-2
-GRAD_TOL_EMULATED, GRAD_TOL_FP64 = 1e-2, 0.25
+GRAD_TOL_EMULATED, GRAD_TOL_FP64 = 5e-2, 0.25
 
 
 @pytest.mark.parametrize("shape", [(2, 32, 32), (1, 16, 40), (3, 64, 64)])
