@@ -97,6 +97,8 @@ struct BatchArgs {
     BatchResult* results;
 };
 
+constexpr int kU = 4; // nodes per thread per pass
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
 {
@@ -146,25 +148,78 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
         int it = 0, since = 0;
         bool converged = bb == 0.0;
         double rr_true = bb;
+        // The three passes of an iteration handle kU nodes per thread at a
+        // time (nodes BLOCK apart, so every load stays coalesced) and issue
+        // all of their loads before using any: a workgroup alone streams its
+        // problem, and only loads in flight hide the memory latency.
+        const bool wide = n >= 3; // node n+1 is interior: a safe address for inactive lanes
         while (!converged && it < a.max_it) {
             __syncthreads(); // p visible to the whole workgroup
             double pq = 0.0;
-            for (int t = threadIdx.x; t < nn; t += BLOCK) {
-                const int i = t / n, j = t - i * n;
-                float qv = 0.f;
-                if (i > 0 && j > 0 && i + 1 < n && j + 1 < n) qv = apply_row<float>(wx, wy, p, n, i, j, t);
-                q[t] = qv;
-                pq += (double)p[t] * (double)qv;
+            for (int t0 = threadIdx.x; t0 < nn; t0 += kU * BLOCK) {
+                float pc[kU], pe[kU], pw[kU], pn[kU], ps[kU], ax[kU], aw[kU], ay[kU], as[kU];
+                bool in[kU], ok[kU], he[kU], hw[kU], hn[kU], hs[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int t = t0 + u * BLOCK;
+                    const int i = t / n, j = t - i * n;
+                    ok[u] = t < nn;
+                    in[u] = wide && ok[u] && i > 0 && j > 0 && i + 1 < n && j + 1 < n;
+                    he[u] = i + 2 < n;
+                    hw[u] = i > 1;
+                    hn[u] = j + 2 < n;
+                    hs[u] = j > 1;
+                    const int tc = in[u] ? t : (wide ? n + 1 : 0);
+                    pc[u] = p[ok[u] ? t : 0];
+                    if (wide) {
+                        pe[u] = p[tc + n];
+                        pw[u] = p[tc - n];
+                        pn[u] = p[tc + 1];
+                        ps[u] = p[tc - 1];
+                        ax[u] = wx[tc];
+                        aw[u] = wx[tc - n];
+                        ay[u] = wy[tc];
+                        as[u] = wy[tc - 1];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    if (!ok[u]) continue;
+                    float qv = 0.f;
+                    if (in[u]) {
+                        const float v = pc[u];
+                        qv = ax[u] * (v - (he[u] ? pe[u] : 0.f)) + aw[u] * (v - (hw[u] ? pw[u] : 0.f)) +
+                             ay[u] * (v - (hn[u] ? pn[u] : 0.f)) + as[u] * (v - (hs[u] ? ps[u] : 0.f));
+                    }
+                    q[t0 + u * BLOCK] = qv;
+                    pq += (double)pc[u] * (double)qv;
+                }
             }
             pq = wg_sum<BLOCK>(pq, sh);
             const double alpha = rz / pq;
             double rz_new = 0.0, rr_new = 0.0;
-            for (int t = threadIdx.x; t < nn; t += BLOCK) {
-                x[t] += alpha * (double)p[t];
-                const float rf = (float)((double)r[t] - alpha * (double)q[t]);
-                r[t] = rf;
-                rz_new += (double)rf * (double)zval(t, rf);
-                rr_new += (double)rf * (double)rf;
+            for (int t0 = threadIdx.x; t0 < nn; t0 += kU * BLOCK) {
+                double xv[kU];
+                float pv[kU], rv[kU], qv[kU], dv[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int t = t0 + u * BLOCK, tc = t < nn ? t : 0;
+                    xv[u] = x[tc];
+                    pv[u] = p[tc];
+                    rv[u] = r[tc];
+                    qv[u] = q[tc];
+                    dv[u] = jac ? dinv[tc] : 1.f;
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int t = t0 + u * BLOCK;
+                    if (t >= nn) continue;
+                    x[t] = xv[u] + alpha * (double)pv[u];
+                    const float rf = (float)((double)rv[u] - alpha * (double)qv[u]);
+                    r[t] = rf;
+                    rz_new += (double)rf * (double)(jac ? rf * dv[u] : rf);
+                    rr_new += (double)rf * (double)rf;
+                }
             }
             wg_sum2<BLOCK>(rz_new, rr_new, sh);
             ++it;
@@ -199,7 +254,22 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
                 rz = rzr;
                 rr_ref = rr;
             }
-            for (int t = threadIdx.x; t < nn; t += BLOCK) p[t] = zval(t, r[t]) + (float)beta * p[t];
+            const float bf = (float)beta;
+            for (int t0 = threadIdx.x; t0 < nn; t0 += kU * BLOCK) {
+                float rv[kU], dv[kU], pv[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int t = t0 + u * BLOCK, tc = t < nn ? t : 0;
+                    rv[u] = r[tc];
+                    dv[u] = jac ? dinv[tc] : 1.f;
+                    pv[u] = p[tc];
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int t = t0 + u * BLOCK;
+                    if (t < nn) p[t] = (jac ? rv[u] * dv[u] : rv[u]) + bf * pv[u];
+                }
+            }
         }
         if (!converged) {
             // report the FP64 residual of the final iterate

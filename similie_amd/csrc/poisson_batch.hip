@@ -483,17 +483,23 @@ enum ScalPhase {
     PH_FCG_UPDATE,  // rr → iteration count, convergence on a recomputed residual
 };
 
-__global__ void k_pb_scalars(int phase, int nbx, const double* __restrict__ part, Scal* __restrict__ sc,
+__global__ __launch_bounds__(64) void k_pb_scalars(int phase, int nbx, const double* __restrict__ part, Scal* __restrict__ sc,
                              double tol2, int max_iter, int verified, int* __restrict__ done_flags,
                              double nn_total)
 {
     const int b = blockIdx.x;
-    if (threadIdx.x != 0) return;
     Scal& s = sc[b];
-    if (s.done && phase != PH_INIT && phase != PH_VERIFY) return;
+    if (s.done && phase != PH_INIT && phase != PH_VERIFY) return; // uniform per workgroup
+    // one wavefront sums the problem's block partials (lane-strided, then a
+    // shuffle tree: a fixed order)
     double v[kParts] = {0.0, 0.0, 0.0, 0.0};
-    for (int q = 0; q < nbx; ++q)
+    for (int q = threadIdx.x; q < nbx; q += 64)
+#pragma unroll
         for (int k = 0; k < kParts; ++k) v[k] += part[((size_t)b * nbx + q) * kParts + k];
+#pragma unroll
+    for (int k = 0; k < kParts; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    if (threadIdx.x != 0) return;
     switch (phase) {
     case PH_INIT:
         s.rz = v[0];
