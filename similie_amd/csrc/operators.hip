@@ -133,6 +133,137 @@ __global__ __launch_bounds__(kBlock) void k_ms3d_apply(MsArgs<T> a, const T* __r
     if (partials) block_dot_store<T>(dot, partials);
 }
 
+// The same operator by a march along z: a workgroup owns 64 (x, one
+// wavefront) × kMsTY (y) nodes including a one-node halo and walks a chunk of
+// kMsChunk z-planes.  Each node's input value is read once per tile (PU: the
+// direction p = z + β p_old is formed instead and written to p_new by the
+// node's owner); the x/y neighbours of A and of H = w^B ν B go through LDS,
+// the z neighbours stay in registers, so the ~40 neighbour reads of the
+// direct kernel become one staged read per node.  Boundary rows are the
+// identity; boundary DOFs are dropped from every H.
+constexpr int kMsTY = 8;
+constexpr int kMsChunk = 32;
+
+template <typename T, bool PU>
+__global__ __launch_bounds__(64 * kMsTY) void k_ms3d_march(MsArgs<T> a, const T* __restrict__ x, T* __restrict__ out,
+                                                           double* __restrict__ partials, PUpdate<T> pu, int ntx,
+                                                           int nty)
+{
+    __shared__ T sA[3][kMsTY][64];
+    __shared__ T sH[3][kMsTY][64];
+    __shared__ double sdot[kMsTY];
+    if (pu.done && *pu.done) return; // uniform
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int tx = blockIdx.x % ntx, ty = (blockIdx.x / ntx) % nty, tk = blockIdx.x / (ntx * nty);
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz, plane = nx * ny;
+    const int64_t i = (int64_t)tx * 62 - 1 + lane, j = (int64_t)ty * (kMsTY - 2) - 1 + row;
+    const int64_t ka = (int64_t)tk * kMsChunk, kb = min(ka + (int64_t)kMsChunk, nz);
+    const bool inx = i >= 0 && i < nx, iny = j >= 0 && j < ny;
+    const bool writer = lane >= 1 && lane <= 62 && row >= 1 && row <= kMsTY - 2 && inx && iny;
+    const bool in_ij = i > 0 && j > 0 && i + 1 < nx && j + 1 < ny;
+    const int64_t ic = min(max(i, (int64_t)0), nx - 1), jc = min(max(j, (int64_t)0), ny - 1);
+    const int64_t lin = ic + nx * jc;
+    const T lx = a.ls[0][ic], ly = a.ls[1][jc], ilx = a.inv_ls[0][ic], ily = a.inv_ls[1][jc];
+    const PUpdate<T>* pp = PU ? &pu : nullptr;
+
+    // raw direction value (3 components) of this lane's node in plane k
+    auto load = [&](int64_t k, T v[3]) {
+        const int64_t idx = 3 * (lin + plane * min(max(k, (int64_t)0), nz - 1));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = p_value<T>(pp, x, idx + c);
+    };
+    auto interior = [&](int64_t k) { return in_ij && k > 0 && k + 1 < nz; };
+
+    T P0[3], P1[3], P2[3], Hp[3] = {T(0), T(0), T(0)};
+    double dot = 0.0;
+    int64_t k = ka > 0 ? ka - 1 : ka;
+    load(k, P0);
+    load(k + 1, P1);
+
+    // H at plane k from A(k) (LDS for the +x / +y neighbours) and A(k+1);
+    // leaves A(k) in sA and H(k) in sH
+    auto h_plane = [&](T H[3]) {
+        const bool ik = interior(k), ik1 = interior(k + 1);
+        T A0[3], A1[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            A0[c] = ik ? P0[c] : T(0);
+            A1[c] = ik1 ? P1[c] : T(0);
+            sA[c][row][lane] = A0[c];
+        }
+        __syncthreads();
+        const int lp = lane + 1 < 64 ? lane + 1 : lane, rp = row + 1 < kMsTY ? row + 1 : row;
+        const T ax1 = sA[1][row][lp], ax2 = sA[2][row][lp]; // A(i+1, j, k)
+        const T ay0 = sA[0][rp][lane], ay2 = sA[2][rp][lane]; // A(i, j+1, k)
+        H[0] = H[1] = H[2] = T(0);
+        if (ik) {
+            const int64_t kk = k;
+            const T lz = a.ls[2][kk], ilz = a.inv_ls[2][kk];
+            const T nu = a.nu[lin + plane * kk];
+            const T bx = ay2 - A0[2] - A1[1] + A0[1];
+            const T by = -ax2 + A0[2] + A1[0] - A0[0];
+            const T bz = ax1 - A0[1] - ay0 + A0[0];
+            H[0] = (lx * ily * ilz) * nu * bx;
+            H[1] = (ly * ilx * ilz) * nu * by;
+            H[2] = (lz * ilx * ily) * nu * bz;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sH[c][row][lane] = H[c];
+        __syncthreads();
+    };
+
+    if (ka > 0) {
+        // plane ka−1: its H only
+        h_plane(Hp);
+        ++k;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) P0[c] = P1[c];
+        load(k + 1, P1);
+    }
+    for (; k < kb; ++k) {
+        load(k + 2, P2); // next step's A(k+1)
+        T H[3];
+        h_plane(H);
+        if (writer) {
+            const int lm = lane - 1, rm = row - 1; // in the tile for writers
+            T o[3];
+            if (interior(k)) {
+                const T lz = a.ls[2][k], ilz = a.inv_ls[2][k];
+                o[0] = Hp[1] - H[1] - sH[2][rm][lane] + H[2] + a.penalty * (ly * lz * ilx) * P0[0];
+                o[1] = -Hp[0] + H[0] + sH[2][row][lm] - H[2] + a.penalty * (lx * lz * ily) * P0[1];
+                o[2] = sH[0][rm][lane] - H[0] - sH[1][row][lm] + H[1] + a.penalty * (lx * ly * ilz) * P0[2];
+            } else {
+                o[0] = P0[0];
+                o[1] = P0[1];
+                o[2] = P0[2];
+            }
+            const int64_t idx = 3 * (lin + plane * k);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                out[idx + c] = o[c];
+                if (PU) pu.p_new[idx + c] = P0[c];
+                dot += (double)P0[c] * (double)o[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Hp[c] = H[c];
+            P0[c] = P1[c];
+            P1[c] = P2[c];
+        }
+    }
+    if (partials) {
+        for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
+        if (lane == 0) sdot[row] = dot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double sum = 0.0;
+            for (int r = 0; r < kMsTY; ++r) sum += sdot[r];
+            partials[blockIdx.x] = sum;
+        }
+    }
+}
+
 __global__ void k_ms3d_diag(MsArgs<double> a, double* __restrict__ diag)
 {
     const int64_t nn = a.nx * a.ny * a.nz;
@@ -327,10 +458,32 @@ int ms_apply(const Ms3dData& d, const T* in, T* out, DotEpilogue dot, const PUpd
     }
     if (nb_out) *nb_out = (int)nb;
     MsArgs<T> a = ms_args<T>(d);
-    if (pu) {
+    if (d.nx >= 3 && d.ny >= 3 && d.nz >= 3) {
+        // march: tiles of 62 × (kMsTY − 2) output nodes, chunks of kMsChunk planes
+        const int ntx = (int)((d.nx + 61) / 62), nty = (int)((d.ny + kMsTY - 3) / (kMsTY - 2));
+        const int nkc = (int)((d.nz + kMsChunk - 1) / kMsChunk);
+        const int64_t nbm = (int64_t)ntx * nty * nkc;
+        SIL_REQUIRE(!dot.partials || nbm <= kMaxDotBlocks, SIL_ERR_UNSUPPORTED,
+                    "operator too large for the fused dot epilogue (%lld workgroups)", (long long)nbm);
+        if (nb_out) *nb_out = (int)nbm;
+        const T* src = in;
+        PUpdate<T> skip{};
+        if (pu) {
+            // the CG direction in its own streaming pass (forming it while the
+            // march stages its tile reads three strided arrays per node and
+            // is slower than the pass)
+            const int64_t ndof = 3 * nn;
+            const unsigned g = (unsigned)std::min<int64_t>((ndof + kBlock - 1) / kBlock, 8192);
+            hipLaunchKernelGGL((k_direction<T>), dim3(g), dim3(kBlock), 0, s, *pu, ndof);
+            src = pu->p_new;
+            skip.done = pu->done;
+        }
+        hipLaunchKernelGGL((k_ms3d_march<T, false>), dim3((unsigned)nbm), dim3(64 * kMsTY), 0, s, a, src, out,
+                           dot.partials, skip, ntx, nty);
+    } else if (pu) {
         // the direction is formed in its own pass: formed inside the
-        // operator, every one of the ~40 neighbour reads of a node would
-        // read r, D⁻¹ and p_old instead of p (3.5x slower per application)
+        // direct kernel, every one of the ~40 neighbour reads of a node would
+        // read r, D⁻¹ and p_old instead of p
         const int64_t ndof = 3 * nn;
         const unsigned g = (unsigned)std::min<int64_t>((ndof + kBlock - 1) / kBlock, 8192);
         hipLaunchKernelGGL((k_direction<T>), dim3(g), dim3(kBlock), 0, s, *pu, ndof);

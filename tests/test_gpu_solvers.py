@@ -32,7 +32,7 @@ def ms_problem(shape, rng, stretch=True):
     return coords, cell_mu, cell_j
 
 
-@pytest.mark.parametrize("shape", [(6, 5, 7), (17, 9, 12)])
+@pytest.mark.parametrize("shape", [(6, 5, 7), (17, 9, 12), (70, 15, 41)])
 def test_magnetostatics_assembly_and_apply(S, shape):
     rng = np.random.default_rng(sum(shape))
     coords, cell_mu, cell_j = ms_problem(shape, rng)
