@@ -293,6 +293,249 @@ __global__ __launch_bounds__(BLOCK) void k_batched_cg(BatchArgs a)
     }
 }
 
+// ------------------------------------------------------------- wide path ---
+
+constexpr int kWideNT = 256;
+
+enum WidePhase { W_INIT = 0, W_ALPHA, W_BETA, W_REPLACED, W_FINAL };
+
+// block sums of two values into part[0], part[1]
+__device__ __forceinline__ void wide_partials(double v0, double v1, double* __restrict__ part)
+{
+    __shared__ double sh[2][kWideNT / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        v0 += __shfl_down(v0, off, 64);
+        v1 += __shfl_down(v1, off, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][w] = v0;
+        sh[1][w] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < kWideNT / 64; ++i) {
+            s0 += sh[0][i];
+            s1 += sh[1][i];
+        }
+        part[0] = s0;
+        part[1] = s1;
+    }
+}
+
+// the problem of wide slot blockIdx.y and its arrays
+struct WideView {
+    int n, nn;
+    int64_t off;
+    double* part;
+};
+
+__device__ __forceinline__ WideView wide_view(const WideLaunch& L)
+{
+    const BatchProblem P = L.problems[L.ids[blockIdx.y]];
+    WideView v;
+    v.n = P.side;
+    v.nn = P.side * P.side;
+    v.off = P.offset;
+    v.part = L.part + ((size_t)blockIdx.y * L.nbx + blockIdx.x) * 2;
+    return v;
+}
+
+#define WIDE_NODES(v) for (int t = blockIdx.x * kWideNT + threadIdx.x; t < (v).nn; t += L.nbx * kWideNT)
+
+__global__ __launch_bounds__(kWideNT) void k_wide_init(WideLaunch L)
+{
+    const WideView v = wide_view(L);
+    const double* b = L.b + v.off;
+    double* x = L.x + v.off;
+    float* r = L.r + v.off;
+    float* p = L.p + v.off;
+    const float* dinv = L.dinv32 + v.off;
+    double bb = 0.0, rz = 0.0;
+    WIDE_NODES(v)
+    {
+        const double bi = b[t];
+        x[t] = 0.0;
+        const float rf = (float)bi;
+        r[t] = rf;
+        const float zf = L.jacobi ? rf * dinv[t] : rf;
+        p[t] = zf;
+        bb += bi * bi;
+        rz += (double)rf * (double)zf;
+    }
+    wide_partials(bb, rz, v.part);
+}
+
+__global__ __launch_bounds__(kWideNT) void k_wide_apply(WideLaunch L)
+{
+    if (L.state[blockIdx.y].done) return; // uniform per workgroup
+    const WideView v = wide_view(L);
+    const int n = v.n;
+    const float* wx = L.wx32 + v.off;
+    const float* wy = L.wy32 + v.off;
+    const float* p = L.p + v.off;
+    float* q = L.q + v.off;
+    double pq = 0.0;
+    WIDE_NODES(v)
+    {
+        const int i = t / n, j = t - i * n;
+        float qv = 0.f;
+        if (i > 0 && j > 0 && i + 1 < n && j + 1 < n) qv = apply_row<float>(wx, wy, p, n, i, j, t);
+        q[t] = qv;
+        pq += (double)p[t] * (double)qv;
+    }
+    wide_partials(pq, 0.0, v.part);
+}
+
+__global__ __launch_bounds__(kWideNT) void k_wide_update(WideLaunch L)
+{
+    const WideState& S = L.state[blockIdx.y];
+    if (S.done) return;
+    const double alpha = S.alpha;
+    const WideView v = wide_view(L);
+    double* x = L.x + v.off;
+    const float* p = L.p + v.off;
+    float* r = L.r + v.off;
+    const float* q = L.q + v.off;
+    const float* dinv = L.dinv32 + v.off;
+    double rz = 0.0, rr = 0.0;
+    WIDE_NODES(v)
+    {
+        x[t] += alpha * (double)p[t];
+        const float rf = (float)((double)r[t] - alpha * (double)q[t]);
+        r[t] = rf;
+        rz += (double)rf * (double)(L.jacobi ? rf * dinv[t] : rf);
+        rr += (double)rf * (double)rf;
+    }
+    wide_partials(rz, rr, v.part);
+}
+
+// FP64 residual b − A x: replaces r (FP32) when the problem's scalar phase
+// asked for it; with `report`, only its square norm, for problems that did
+// not converge
+__global__ __launch_bounds__(kWideNT) void k_wide_residual(WideLaunch L, int report)
+{
+    const WideState& S = L.state[blockIdx.y];
+    if (report ? S.converged != 0 : (S.done || !S.replace_now)) return;
+    const WideView v = wide_view(L);
+    const int n = v.n;
+    const double* b = L.b + v.off;
+    const double* x = L.x + v.off;
+    const double* wx = L.wx64 + v.off;
+    const double* wy = L.wy64 + v.off;
+    float* r = L.r + v.off;
+    const float* dinv = L.dinv32 + v.off;
+    double rr = 0.0, rz = 0.0;
+    WIDE_NODES(v)
+    {
+        const int i = t / n, j = t - i * n;
+        double res = 0.0;
+        if (i > 0 && j > 0 && i + 1 < n && j + 1 < n) res = b[t] - apply_row<double>(wx, wy, x, n, i, j, t);
+        rr += res * res;
+        if (!report) {
+            const float rf = (float)res;
+            r[t] = rf;
+            rz += (double)rf * (double)(L.jacobi ? rf * dinv[t] : rf);
+        }
+    }
+    wide_partials(rr, rz, v.part);
+}
+
+__global__ __launch_bounds__(kWideNT) void k_wide_direction(WideLaunch L)
+{
+    const WideState& S = L.state[blockIdx.y];
+    if (S.done) return;
+    const float beta = (float)S.beta;
+    const WideView v = wide_view(L);
+    float* p = L.p + v.off;
+    const float* r = L.r + v.off;
+    const float* dinv = L.dinv32 + v.off;
+    WIDE_NODES(v) p[t] = (L.jacobi ? r[t] * dinv[t] : r[t]) + beta * p[t];
+}
+
+// one wavefront per problem: the block partials in a fixed order, then the
+// scalar recurrence of the persistent kernel
+__global__ __launch_bounds__(64) void k_wide_scalars(WideLaunch L, int phase)
+{
+    const int slot = blockIdx.x;
+    WideState& S = L.state[slot];
+    if (phase == W_FINAL ? S.converged != 0 : (phase != W_INIT && S.done)) return; // uniform
+    if (phase == W_REPLACED && !S.replace_now) return;
+    double v0 = 0.0, v1 = 0.0;
+    for (int q = threadIdx.x; q < L.nbx; q += 64) {
+        v0 += L.part[((size_t)slot * L.nbx + q) * 2];
+        v1 += L.part[((size_t)slot * L.nbx + q) * 2 + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        v0 += __shfl_down(v0, off, 64);
+        v1 += __shfl_down(v1, off, 64);
+    }
+    if (threadIdx.x != 0) return;
+    switch (phase) {
+    case W_INIT:
+        S.bb = v0;
+        S.rz = v1;
+        S.tol2bb = L.tol * L.tol * v0;
+        S.rr_ref = v0;
+        S.rr_true = v0;
+        S.it = 0;
+        S.since = 0;
+        S.replace_now = 0;
+        S.converged = v0 == 0.0 ? 1 : 0;
+        S.done = (S.converged || L.max_it <= 0) ? 1 : 0;
+        break;
+    case W_ALPHA:
+        S.pq = v0;
+        S.alpha = S.rz / v0;
+        break;
+    case W_BETA:
+        S.it += 1;
+        S.since += 1;
+        S.rz_old = S.rz;
+        S.rz_new = v0;
+        S.rr_new = v1;
+        S.beta = v0 / S.rz_old;
+        S.rz = v0;
+        // replace when the recurrence claims convergence, when its residual
+        // fell 100-fold since the last replacement, or every `replace`
+        // iterations
+        S.replace_now = (v1 <= S.tol2bb || v1 < 1e-4 * S.rr_ref || S.since >= L.replace) ? 1 : 0;
+        if (!S.replace_now && S.it >= L.max_it) S.done = 1;
+        break;
+    case W_REPLACED:
+        S.rr_true = v0;
+        S.since = 0;
+        S.replace_now = 0;
+        if (v0 <= S.tol2bb) {
+            S.converged = 1;
+            S.done = 1;
+        } else {
+            S.beta = v1 / S.rz_old; // replaced <r,z> over the previous iteration's <r,z>
+            S.rz = v1;
+            S.rr_ref = v0;
+            if (S.it >= L.max_it) S.done = 1;
+        }
+        break;
+    case W_FINAL:
+        S.rr_true = v0;
+        break;
+    }
+    L.done[slot] = S.done;
+}
+
+__global__ void k_wide_results(WideLaunch L)
+{
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= L.nwide) return;
+    const WideState& S = L.state[slot];
+    BatchResult& R = L.results[L.ids[slot]];
+    R.iterations = S.it;
+    R.converged = S.converged;
+    R.true_relative_residual = S.bb == 0.0 ? 0.0 : sqrt(S.rr_true / S.bb);
+}
+
 __global__ void k_reset_queue(int* q) { *q = 0; }
 
 __global__ void k_batch_weights(const BatchProblem* __restrict__ problems, int nprob, const double* __restrict__ kappa,
@@ -360,6 +603,37 @@ int batched_enqueue(const BatchLaunch& L, hipStream_t s)
         set_error("batched solve: unsupported workgroup size %d", L.block);
         return SIL_ERR_INVALID;
     }
+}
+
+int wide_init(const WideLaunch& L, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_wide_init, dim3(L.nbx, L.nwide), dim3(kWideNT), 0, s, L);
+    hipLaunchKernelGGL(k_wide_scalars, dim3(L.nwide), dim3(64), 0, s, L, (int)W_INIT);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int wide_iteration(const WideLaunch& L, hipStream_t s)
+{
+    const dim3 g(L.nbx, L.nwide);
+    hipLaunchKernelGGL(k_wide_apply, g, dim3(kWideNT), 0, s, L);
+    hipLaunchKernelGGL(k_wide_scalars, dim3(L.nwide), dim3(64), 0, s, L, (int)W_ALPHA);
+    hipLaunchKernelGGL(k_wide_update, g, dim3(kWideNT), 0, s, L);
+    hipLaunchKernelGGL(k_wide_scalars, dim3(L.nwide), dim3(64), 0, s, L, (int)W_BETA);
+    hipLaunchKernelGGL(k_wide_residual, g, dim3(kWideNT), 0, s, L, 0);
+    hipLaunchKernelGGL(k_wide_scalars, dim3(L.nwide), dim3(64), 0, s, L, (int)W_REPLACED);
+    hipLaunchKernelGGL(k_wide_direction, g, dim3(kWideNT), 0, s, L);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int wide_finish(const WideLaunch& L, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_wide_residual, dim3(L.nbx, L.nwide), dim3(kWideNT), 0, s, L, 1);
+    hipLaunchKernelGGL(k_wide_scalars, dim3(L.nwide), dim3(64), 0, s, L, (int)W_FINAL);
+    hipLaunchKernelGGL(k_wide_results, dim3((L.nwide + 255) / 256), dim3(256), 0, s, L);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
 }
 
 }  // namespace sil

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <new>
 #include <numeric>
 #include <vector>
@@ -16,7 +17,19 @@ struct sil_batched_poisson_s {
     int64_t total = 0;
     std::vector<BatchProblem> problems;
     BatchProblem* problems_dev = nullptr;
-    int* order_dev = nullptr;
+    int* order_dev = nullptr;   // queue of the persistent kernel: the small problems, largest first
+    int nsmall = 0;
+    // wide path: problems of side >= wide_side, each over nbx workgroups
+    int wide_side = 192;
+    int nwide = 0, nbx = 1;
+    int* wide_ids_dev = nullptr;
+    WideState* wide_state = nullptr;
+    double* wide_part = nullptr;
+    int* wide_done = nullptr;
+    int* wide_done_host = nullptr; // pinned
+    hipGraphExec_t wide_graph = nullptr;
+    int wide_max_it = -1, wide_replace = -1, wide_jacobi = -1;
+    double wide_tol = -1.0;
     int* queue = nullptr;
     float *wx32 = nullptr, *wy32 = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *dinv32 = nullptr;
     double *wx64 = nullptr, *wy64 = nullptr, *b = nullptr, *x = nullptr;
@@ -38,9 +51,12 @@ void release(sil_batched_poisson h)
     for (void* ptr : {(void*)h->problems_dev, (void*)h->order_dev, (void*)h->queue, (void*)h->wx32, (void*)h->wy32,
                       (void*)h->dinv32,
                       (void*)h->r, (void*)h->p, (void*)h->q, (void*)h->wx64, (void*)h->wy64, (void*)h->b,
-                      (void*)h->x, (void*)h->results_dev})
+                      (void*)h->x, (void*)h->results_dev, (void*)h->wide_ids_dev, (void*)h->wide_state,
+                      (void*)h->wide_part, (void*)h->wide_done})
         if (ptr) (void)hipFree(ptr);
+    if (h->wide_done_host) (void)hipHostFree(h->wide_done_host);
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    if (h->wide_graph) (void)hipGraphExecDestroy(h->wide_graph);
     delete h;
 }
 
@@ -56,7 +72,7 @@ int grid_for(sil_batched_poisson h, int block, int* grid)
     int per_cu = 0;
     int rc = batched_occupancy(block, &per_cu);
     if (rc) return rc;
-    *grid = std::max(1, std::min(h->nprob, h->cus * std::max(per_cu, 1)));
+    *grid = std::max(1, std::min(h->nsmall, h->cus * std::max(per_cu, 1)));
     return SIL_OK;
 }
 
@@ -65,7 +81,7 @@ int enqueue(sil_batched_poisson h, int block, double tol, int max_it, int replac
     BatchLaunch L{};
     L.problems = h->problems_dev;
     L.order = h->order_dev;
-    L.nprob = h->nprob;
+    L.nprob = h->nsmall;
     L.queue = h->queue;
     L.wx32 = h->wx32;
     L.wy32 = h->wy32;
@@ -124,6 +140,94 @@ int capture(sil_batched_poisson h, int block, double tol, int max_it, int replac
     return SIL_OK;
 }
 
+WideLaunch wide_launch(sil_batched_poisson h, double tol, int max_it, int replace)
+{
+    WideLaunch W{};
+    W.problems = h->problems_dev;
+    W.ids = h->wide_ids_dev;
+    W.nwide = h->nwide;
+    W.nbx = h->nbx;
+    W.wx32 = h->wx32;
+    W.wy32 = h->wy32;
+    W.wx64 = h->wx64;
+    W.wy64 = h->wy64;
+    W.b = h->b;
+    W.x = h->x;
+    W.r = h->r;
+    W.p = h->p;
+    W.q = h->q;
+    W.dinv32 = h->dinv32;
+    W.jacobi = h->jacobi;
+    W.tol = tol;
+    W.max_it = max_it;
+    W.replace = replace;
+    W.state = h->wide_state;
+    W.part = h->wide_part;
+    W.done = h->wide_done;
+    W.results = h->results_dev;
+    return W;
+}
+
+// iterations of the wide path per graph launch (the host reads the done
+// flags between launches)
+constexpr int kWideChunk = 8;
+
+int capture_wide(sil_batched_poisson h, double tol, int max_it, int replace)
+{
+    if (h->wide_graph && h->wide_tol == tol && h->wide_max_it == max_it && h->wide_replace == replace &&
+        h->wide_jacobi == h->jacobi)
+        return SIL_OK;
+    if (h->wide_graph) {
+        (void)hipGraphExecDestroy(h->wide_graph);
+        h->wide_graph = nullptr;
+    }
+    hipStream_t cs;
+    SIL_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        set_error("batched: capture failed: %s", hipGetErrorString(e));
+        return SIL_ERR_HIP;
+    }
+    const WideLaunch W = wide_launch(h, tol, max_it, replace);
+    int rc = SIL_OK;
+    for (int i = 0; i < kWideChunk && rc == SIL_OK; ++i) rc = wide_iteration(W, cs);
+    e = hipStreamEndCapture(cs, &g);
+    if (rc == SIL_OK && e == hipSuccess) e = hipGraphInstantiate(&h->wide_graph, g, nullptr, nullptr, 0);
+    if (e == hipSuccess || rc != SIL_OK) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cs);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+        set_error("batched: graph failed: %s", hipGetErrorString(e));
+        return SIL_ERR_HIP;
+    }
+    h->wide_tol = tol;
+    h->wide_max_it = max_it;
+    h->wide_replace = replace;
+    h->wide_jacobi = h->jacobi;
+    return SIL_OK;
+}
+
+// the wide path to completion on stream s
+int solve_wide(sil_batched_poisson h, double tol, int max_it, int replace, hipStream_t s)
+{
+    int rc = capture_wide(h, tol, max_it, replace);
+    if (rc) return rc;
+    const WideLaunch W = wide_launch(h, tol, max_it, replace);
+    if ((rc = wide_init(W, s))) return rc;
+    for (int64_t done_it = 0; done_it < max_it; done_it += kWideChunk) {
+        SIL_CHECK_HIP(hipMemcpyAsync(h->wide_done_host, h->wide_done, sizeof(int) * h->nwide,
+                                     hipMemcpyDeviceToHost, s));
+        SIL_CHECK_HIP(hipStreamSynchronize(s));
+        bool all = true;
+        for (int i = 0; i < h->nwide && all; ++i) all = h->wide_done_host[i] != 0;
+        if (all) break;
+        SIL_CHECK_HIP(hipGraphLaunch(h->wide_graph, s));
+    }
+    return wide_finish(W, s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -148,21 +252,50 @@ int sil_batched_poisson2d_create(sil_batched_poisson* out, int32_t nproblems, co
         h->max_side = std::max(h->max_side, (int)sides[i]);
     }
     h->total = off;
-    std::vector<int> order(nproblems);
-    std::iota(order.begin(), order.end(), 0);
+    if (const char* e = std::getenv("SIL_BATCH_WIDE_SIDE")) h->wide_side = std::max(3, std::atoi(e));
+    // small problems: one workgroup each in the persistent kernel, largest
+    // first; large ones: the wide path
+    std::vector<int> order, wide;
+    int64_t max_wide_nn = 0;
+    for (int i = 0; i < nproblems; ++i) {
+        if (sides[i] >= h->wide_side) {
+            wide.push_back(i);
+            max_wide_nn = std::max<int64_t>(max_wide_nn, (int64_t)sides[i] * sides[i]);
+        } else {
+            order.push_back(i);
+        }
+    }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sides[a] > sides[b]; });
+    h->nsmall = (int)order.size();
+    h->nwide = (int)wide.size();
+    // four nodes per thread of the largest wide problem, at most 256 workgroups per problem
+    h->nbx = (int)std::min<int64_t>(256, std::max<int64_t>(1, (max_wide_nn + 1023) / 1024));
     int rc;
-    if ((rc = dalloc(&h->problems_dev, nproblems)) || (rc = dalloc(&h->order_dev, nproblems)) ||
+    if ((rc = dalloc(&h->problems_dev, nproblems)) || (rc = dalloc(&h->order_dev, std::max(h->nsmall, 1))) ||
         (rc = dalloc(&h->queue, 1)) || (rc = dalloc(&h->wx32, off)) || (rc = dalloc(&h->wy32, off)) ||
         (rc = dalloc(&h->r, off)) || (rc = dalloc(&h->p, off)) || (rc = dalloc(&h->q, off)) ||
         (rc = dalloc(&h->wx64, off)) || (rc = dalloc(&h->wy64, off)) || (rc = dalloc(&h->b, off)) ||
-        (rc = dalloc(&h->x, off)) || (rc = dalloc(&h->dinv32, off)) || (rc = dalloc(&h->results_dev, nproblems))) {
+        (rc = dalloc(&h->x, off)) || (rc = dalloc(&h->dinv32, off)) || (rc = dalloc(&h->results_dev, nproblems)) ||
+        (rc = dalloc(&h->wide_ids_dev, std::max(h->nwide, 1))) ||
+        (rc = dalloc(&h->wide_state, std::max(h->nwide, 1))) ||
+        (rc = dalloc(&h->wide_part, (int64_t)std::max(h->nwide, 1) * h->nbx * 2)) ||
+        (rc = dalloc(&h->wide_done, std::max(h->nwide, 1)))) {
         release(h);
         return rc;
     }
+    if (hipHostMalloc(reinterpret_cast<void**>(&h->wide_done_host), sizeof(int) * std::max(h->nwide, 1)) !=
+        hipSuccess) {
+        h->wide_done_host = nullptr;
+        release(h);
+        set_error("batched_poisson2d_create: pinned allocation failed");
+        return SIL_ERR_HIP;
+    }
     hipError_t e = hipMemcpy(h->problems_dev, h->problems.data(), sizeof(BatchProblem) * nproblems,
                              hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(h->order_dev, order.data(), sizeof(int) * nproblems, hipMemcpyHostToDevice);
+    if (e == hipSuccess && h->nsmall)
+        e = hipMemcpy(h->order_dev, order.data(), sizeof(int) * h->nsmall, hipMemcpyHostToDevice);
+    if (e == hipSuccess && h->nwide)
+        e = hipMemcpy(h->wide_ids_dev, wide.data(), sizeof(int) * h->nwide, hipMemcpyHostToDevice);
     int dev = 0;
     if (e == hipSuccess) e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -208,12 +341,13 @@ int sil_batched_poisson2d_solve(sil_batched_poisson h, const double* rhs, double
     hipStream_t s = as_stream(stream);
     const int replace = st->replace_interval > 0 ? st->replace_interval : 400;
     h->jacobi = st->preconditioner == SIL_PRECOND_JACOBI ? 1 : 0;
-    int rc = capture(h, h->block, st->relative_tolerance, st->max_iterations, replace);
+    int rc = h->nsmall ? capture(h, h->block, st->relative_tolerance, st->max_iterations, replace) : SIL_OK;
     if (rc) return rc;
     SIL_CHECK_HIP(hipMemcpyAsync(h->b, rhs, sizeof(double) * h->total, hipMemcpyDeviceToDevice, s));
     SIL_CHECK_HIP(hipStreamSynchronize(s));
     const auto t0 = std::chrono::steady_clock::now();
-    SIL_CHECK_HIP(hipGraphLaunch(h->graph, s));
+    if (h->nwide && (rc = solve_wide(h, st->relative_tolerance, st->max_iterations, replace, s))) return rc;
+    if (h->nsmall) SIL_CHECK_HIP(hipGraphLaunch(h->graph, s));
     SIL_CHECK_HIP(hipStreamSynchronize(s));
     const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (seconds) *seconds = dt;
@@ -240,6 +374,13 @@ int sil_batched_poisson2d_autotune(sil_batched_poisson h, const double* rhs, int
     const int cands[3] = {256, 512, 1024};
     double best = 1e300;
     int best_block = 256;
+    if (h->nsmall == 0) {
+        // every problem takes the wide path: nothing to tune
+        *chosen_block = h->block;
+        if (seconds_per_candidate)
+            for (int c = 0; c < 3; ++c) seconds_per_candidate[c] = 0.0;
+        return SIL_OK;
+    }
     SIL_CHECK_HIP(hipMemcpy(h->b, rhs, sizeof(double) * h->total, hipMemcpyDeviceToDevice));
     for (int c = 0; c < 3; ++c) {
         // a fixed number of iterations on every problem (tolerance 0): equal work per candidate
