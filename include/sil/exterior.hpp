@@ -236,6 +236,17 @@ inline StrongFormulationSolverDiagnostics minimize_strong_formulation_residual(
     return d;
 }
 
+// build_matrix(gko_exec, operator_model) (minimize_strong_formulation_residual.hpp:623-633) over
+// assemble_matrix_data (onelab_interface/magnetostatics_onelab.hpp:3115-3536): the operator
+// assembled into a device CSR matrix, as a new operator the caller destroys with
+// sil_operator_destroy; solving on it is the use_matrix_free = false path
+inline sil_operator build_matrix(void* stream, sil_operator operator_model)
+{
+    sil_operator m = nullptr;
+    check(sil_magnetostatics3d_assemble(operator_model, &m, stream));
+    return m;
+}
+
 }  // namespace solvers
 
 }  // namespace sil

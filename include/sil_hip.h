@@ -53,7 +53,8 @@ enum sil_precision {
 enum sil_preconditioner {
     SIL_PRECOND_IDENTITY = 0,
     SIL_PRECOND_JACOBI = 1,
-    SIL_PRECOND_LEARNED = 2 /* the U-Net of a sil_unet handle, inside flexible CG (equal-size batches only) */
+    SIL_PRECOND_LEARNED = 2, /* the U-Net of a sil_unet handle, inside flexible CG (equal-size batches only) */
+    SIL_PRECOND_SPD_ISAI = 3 /* PreconditionerType::SpdIsai: M = Gᵀ G, G on the pattern of tril(A) (FP64 only) */
 };
 
 /* Strided view of a k-form sampled on a node grid. */
@@ -170,6 +171,40 @@ int sil_magnetostatics3d_rhs(sil_operator op, const double* cell_current, double
 int sil_poisson2d_create(sil_operator* out, int64_t nx, int64_t ny, const double* x_coords,
                          const double* y_coords, const double* kappa);
 
+/* assemble_matrix_data + csr_from_matrix_data
+ * (onelab_interface/magnetostatics_onelab.hpp:3115-3536,
+ * src/similie/solvers/minimize_strong_formulation_residual.hpp:565-646):
+ * the magnetostatics operator assembled on the device into a new operator
+ * holding a CSR matrix (gko::matrix::Csr<double, int32>: int32 row pointers
+ * and columns, columns ascending within a row, FP64 values) whose apply is
+ * the sparse product.  Solving on it is the reference's
+ * use_matrix_free = false path.  Synchronises `stream`. */
+int sil_magnetostatics3d_assemble(sil_operator op, sil_operator* csr_out, void* stream);
+
+/* gko::matrix::Csr<double, int32>::create(exec, size, values, columns,
+ * row_ptrs) (minimize_strong_formulation_residual.hpp:613-620): an operator
+ * over a caller's square CSR matrix (host or device arrays, copied).  Rows
+ * must run from 0 to nnz with strictly ascending columns in [0, n).
+ * Synchronises `stream`. */
+int sil_csr_create(sil_operator* out, int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* cols,
+                   const double* vals, void* stream);
+
+/* stored entries of an assembled operator; -1 for a matrix-free one */
+int64_t sil_csr_nnz(sil_operator op);
+
+/* copies the CSR arrays of an assembled operator into caller buffers (host
+ * or device) of n+1, nnz and nnz entries.  Synchronises `stream`. */
+int sil_csr_copy(sil_operator op, int32_t* row_ptr, int32_t* cols, double* vals, void* stream);
+
+/* gko::preconditioner::SpdIsai::generate (minimize_strong_formulation_residual
+ * .hpp:963-967): the factor G of M = Gᵀ G ≈ A⁻¹ as a new assembled operator
+ * (pattern of tril(A); row i solves A[J,J] y = e with J its stored columns ≤ i
+ * and is scaled by 1/√y_i).  `a` is an assembled operator or the
+ * magnetostatics operator (assembled for the purpose, as the reference
+ * does).  Rows may hold at most 32 entries in their lower triangle.
+ * Synchronises `stream`. */
+int sil_spd_isai_create(sil_operator a, sil_operator* g_out, void* stream);
+
 int64_t sil_operator_size(sil_operator op);
 int sil_operator_apply(sil_operator op, const double* in, double* out, void* stream);
 int sil_operator_apply_f32(sil_operator op, const float* in, float* out, void* stream);
@@ -201,7 +236,10 @@ void sil_default_solver_settings(sil_solver_settings* settings);
 
 /* minimize_strong_formulation_residual for a linear operator
  * (minimize_strong_formulation_residual.hpp:1308-1382): x0 = 0, CG until
- * ‖r‖2 ≤ tol·‖b‖2 or max_iterations.  Synchronises `stream` on return. */
+ * ‖r‖2 ≤ tol·‖b‖2 or max_iterations.  `op` is matrix-free
+ * (use_matrix_free = true) or assembled (false); the SpdIsai preconditioner
+ * is generated once per operator, from its assembled matrix.  Synchronises
+ * `stream` on return. */
 int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs, double* solution,
                                              const sil_solver_settings* settings,
                                              sil_solver_diagnostics* diagnostics, void* stream);

@@ -2,10 +2,12 @@
  * CPU ORACLE — test infrastructure only (see dec_oracle.c).
  *
  * Restates the 3D magnetostatics operator of the reference's ONELAB
- * interface, its right-hand side and nodal permeability rules, the 2D
- * Dirichlet Poisson operators the solvers run on, and textbook conjugate
- * gradients.  Used only by tests/, __graft_entry__.smoke() and bench.py's
- * CPU-baseline leg.
+ * interface, its right-hand side and nodal permeability rules, its assembled
+ * CSR form, the SPD sparse approximate inverse preconditioner of Ginkgo 1.8
+ * (a dependency the reference does not vendor: the published FSAI algorithm
+ * is restated), the 2D Dirichlet Poisson operators the solvers run on, and
+ * textbook conjugate gradients.  Used only by tests/, __graft_entry__.smoke()
+ * and bench.py's CPU-baseline leg.
  */
 #include <math.h>
 #include <stdint.h>
@@ -190,6 +192,192 @@ void orc_ms_diagonal(long nx, long ny, long nz, const double* xc, const double* 
                 }
     free(e);
     free(o);
+}
+
+/* ------------------------------------------------------------------ */
+/* assembled operator (CSR) and the SPD sparse approximate inverse      */
+/* ------------------------------------------------------------------ */
+
+#define ORC_ROW_MAX 128 /* MATRIX_ROW_MAX_SIZE, magnetostatics_onelab.hpp:3145 */
+
+typedef struct {
+    long moment_row;
+    double coeff;
+} orc_tslot;
+
+static void add_entry(int* count, int* lc, double* lv, long col, double value)
+{
+    if (value == 0.0) return;
+    for (int s = 0; s < *count; ++s)
+        if (lc[s] == (int)col) {
+            lv[s] += value;
+            return;
+        }
+    lc[*count] = (int)col;
+    lv[*count] = value;
+    ++*count;
+}
+
+/* assemble_matrix_data (onelab_interface/magnetostatics_onelab.hpp:3115-3536)
+ * for the linear equations (∂H/∂B = δ_ab/μ): the transposed moment slots of
+ * every potential DOF are collected in sampled-node order (the precompute of
+ * :1950-1981, restated serially); an interior row starts with the gauge term
+ * P·w^A_c on its diagonal and adds, for every slot (moment row 3s+m,
+ * coefficient c_r), c_r·(w^B_m(s)·ν(s))·c_t for every interior tap t of the
+ * moment stencil of m at s; zero contributions are skipped and repeated
+ * columns summed (add_entry, :3176-3190).  Boundary rows hold 1 on the
+ * diagonal.  Rows come out with ascending columns.  cols/vals must hold
+ * 17·3·nx·ny·nz entries; returns the number of stored entries, or -1 when a
+ * row exceeds ORC_ROW_MAX. */
+long orc_ms_assemble_csr(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
+                         const double* mu, double penalty, double y_sign, long* row_ptr, int* cols, double* vals)
+{
+    ms_grid g = {nx, ny, nz, xc, yc, zc};
+    long ndof = 3 * nx * ny * nz;
+    /* a component is a tap of two moment stencils, twice each: ≤ 4 slots per DOF */
+    int* tcount = (int*)calloc(ndof, sizeof(int));
+    orc_tslot* tslot = (orc_tslot*)malloc(sizeof(orc_tslot) * 4 * (ndof > 0 ? ndof : 1));
+    for (long k = 1; k + 1 < nz; ++k)
+        for (long j = 1; j + 1 < ny; ++j)
+            for (long i = 1; i + 1 < nx; ++i)
+                for (int m = 0; m < 3; ++m)
+                    for (int t = 0; t < 4; ++t) {
+                        const ms_tap* tp = &MOMENT[m][t];
+                        long pi = i + tp->di, pj = j + tp->dj, pk = k + tp->dk;
+                        if (is_boundary(&g, pi, pj, pk)) continue;
+                        long r = dof(&g, pi, pj, pk, tp->comp);
+                        orc_tslot* sl = &tslot[4 * r + tcount[r]++];
+                        sl->moment_row = 3 * (i + nx * (j + ny * k)) + m;
+                        sl->coeff = tp->coeff;
+                    }
+    long nnz = 0;
+    int lc[ORC_ROW_MAX];
+    double lv[ORC_ROW_MAX];
+    for (long r = 0; r < ndof; ++r) {
+        long node = r / 3;
+        int c = (int)(r % 3);
+        long i = node % nx, j = (node / nx) % ny, k = node / (nx * ny);
+        row_ptr[r] = nnz;
+        if (is_boundary(&g, i, j, k)) {
+            cols[nnz] = (int)r;
+            vals[nnz] = 1.0;
+            ++nnz;
+            continue;
+        }
+        int count = 0;
+        double dx = local_spacing(xc, nx, i), dy = local_spacing(yc, ny, j), dz = local_spacing(zc, nz, k);
+        double wa = c == 0 ? dy * dz / dx : (c == 1 ? dx * dz / dy : dx * dy / dz);
+        add_entry(&count, lc, lv, r, penalty * wa);
+        for (int sidx = 0; sidx < tcount[r]; ++sidx) {
+            long mrow = tslot[4 * r + sidx].moment_row;
+            double rc = tslot[4 * r + sidx].coeff;
+            long s = mrow / 3;
+            int m = (int)(mrow % 3);
+            long si = s % nx, sj = (s / nx) % ny, sk = s / (nx * ny);
+            double sx = local_spacing(xc, nx, si), sy = local_spacing(yc, ny, sj), sz = local_spacing(zc, nz, sk);
+            double hf = m == 0 ? sx / (sy * sz) : (m == 1 ? y_sign * (sy / (sx * sz)) : sz / (sx * sy));
+            double jc = hf * (1.0 / mu[s]);
+            if (jc == 0.0) continue;
+            for (int t = 0; t < 4; ++t) {
+                const ms_tap* tp = &MOMENT[m][t];
+                long pi = si + tp->di, pj = sj + tp->dj, pk = sk + tp->dk;
+                if (is_boundary(&g, pi, pj, pk)) continue;
+                if (count >= ORC_ROW_MAX) {
+                    free(tcount);
+                    free(tslot);
+                    return -1;
+                }
+                add_entry(&count, lc, lv, dof(&g, pi, pj, pk, tp->comp), rc * jc * tp->coeff);
+            }
+        }
+        /* ascending columns (insertion sort; rows are short) */
+        for (int a = 1; a < count; ++a) {
+            int kc = lc[a];
+            double kv = lv[a];
+            int b = a - 1;
+            while (b >= 0 && lc[b] > kc) {
+                lc[b + 1] = lc[b];
+                lv[b + 1] = lv[b];
+                --b;
+            }
+            lc[b + 1] = kc;
+            lv[b + 1] = kv;
+        }
+        for (int a = 0; a < count; ++a) {
+            cols[nnz] = lc[a];
+            vals[nnz] = lv[a];
+            ++nnz;
+        }
+    }
+    row_ptr[ndof] = nnz;
+    free(tcount);
+    free(tslot);
+    return nnz;
+}
+
+/* SpdIsai of Ginkgo 1.8 (gko::preconditioner::SpdIsai, sparsity power 1;
+ * selected at src/similie/solvers/minimize_strong_formulation_residual.hpp
+ * :963-967), i.e. the factorized sparse approximate inverse of Kolotilina &
+ * Yeremin (1993): for row i let J = the stored columns j ≤ i of row i
+ * (ascending, i last) and S = A[J, J] (entry (a, b), b ≤ a, read from row
+ * J[a]); solve S y = e_|J| by Cholesky and set G[i, J] = y / √y_|J|, so that
+ * M = Gᵀ G ≈ A⁻¹.  Columns of A must be ascending.  g_row_ptr has n+1
+ * entries, g_cols / g_vals room for nnz(A).  Returns 0, -(i+1) when row i
+ * lacks its diagonal, i+1 when its local system is not positive definite,
+ * or -(n+1) - i when J holds more than 64 columns. */
+long orc_spd_isai(long n, const long* row_ptr, const int* cols, const double* vals, long* g_row_ptr, int* g_cols,
+                  double* g_vals)
+{
+    enum { MAXJ = 64 };
+    double S[MAXJ][MAXJ], y[MAXJ];
+    int J[MAXJ];
+    long out = 0;
+    for (long i = 0; i < n; ++i) {
+        g_row_ptr[i] = out;
+        int m = 0;
+        for (long e = row_ptr[i]; e < row_ptr[i + 1] && cols[e] <= i; ++e) {
+            if (m == MAXJ) return -(n + 1) - i;
+            J[m++] = cols[e];
+        }
+        if (m == 0 || J[m - 1] != i) return -(i + 1);
+        for (int a = 0; a < m; ++a) {
+            for (int b = 0; b <= a; ++b) S[a][b] = 0.0;
+            for (long e = row_ptr[J[a]]; e < row_ptr[J[a] + 1]; ++e)
+                for (int b = 0; b <= a; ++b)
+                    if (cols[e] == J[b]) S[a][b] = vals[e];
+        }
+        /* Cholesky S = L Lᵀ, L stored in the lower triangle of S */
+        for (int a = 0; a < m; ++a) {
+            for (int b = 0; b < a; ++b) {
+                double v = S[a][b];
+                for (int c = 0; c < b; ++c) v -= S[a][c] * S[b][c];
+                S[a][b] = v / S[b][b];
+            }
+            double d = S[a][a];
+            for (int c = 0; c < a; ++c) d -= S[a][c] * S[a][c];
+            if (!(d > 0.0)) return i + 1;
+            S[a][a] = sqrt(d);
+        }
+        /* L w = e_m, then Lᵀ y = w */
+        for (int a = 0; a < m; ++a) {
+            double v = a == m - 1 ? 1.0 : 0.0;
+            for (int c = 0; c < a; ++c) v -= S[a][c] * y[c];
+            y[a] = v / S[a][a];
+        }
+        for (int a = m - 1; a >= 0; --a) {
+            double v = y[a];
+            for (int c = a + 1; c < m; ++c) v -= S[c][a] * y[c];
+            y[a] = v / S[a][a];
+        }
+        double scale = sqrt(y[m - 1]);
+        for (int a = 0; a < m; ++a) {
+            g_cols[out] = J[a];
+            g_vals[out] = y[a] / scale;
+            ++out;
+        }
+    }
+    g_row_ptr[n] = out;
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -14,7 +14,7 @@ SIL_OK = 0
 
 METRIC_IDENTITY, METRIC_DIAGONAL, METRIC_SYMMETRIC = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_LEARNED = 0, 1, 2
+PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_SPD_ISAI = 0, 1, 2, 3
 
 
 class SilError(RuntimeError):
@@ -104,6 +104,11 @@ SIGNATURES = {
     "sil_magnetostatics3d_nodal_mu": (ctypes.c_int, [_I64, _I64, _I64, _P, _D, _P, _P]),
     "sil_magnetostatics3d_rhs": (ctypes.c_int, [_P, _P, _P, _P]),
     "sil_poisson2d_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _P, _P, _P]),
+    "sil_magnetostatics3d_assemble": (ctypes.c_int, [_P, ctypes.POINTER(_P), _P]),
+    "sil_csr_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _P, _P, _P, _P]),
+    "sil_csr_nnz": (_I64, [_P]),
+    "sil_csr_copy": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "sil_spd_isai_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), _P]),
     "sil_operator_size": (_I64, [_P]),
     "sil_operator_apply": (ctypes.c_int, [_P, _P, _P, _P]),
     "sil_operator_apply_f32": (ctypes.c_int, [_P, _P, _P, _P]),

@@ -1,0 +1,734 @@
+// Assembled (CSR) operators.
+//
+// * The 3D magnetostatics operator assembled into a CSR matrix on the device
+//   (assemble_matrix_data, onelab_interface/magnetostatics_onelab.hpp
+//   :3115-3536, packed as csr_from_matrix_data does,
+//   src/similie/solvers/minimize_strong_formulation_residual.hpp:565-621).
+//   The reference gathers every row through precomputed transposed-moment
+//   lists (≤ 4 slots per DOF, filled by atomics) and merges repeated columns
+//   with a linear search over a 128-entry local array.  Here the row pattern
+//   is fixed by the stencil: for a row of component C at node n, the (moment,
+//   tap) pairs that reach it, the sampled nodes s = n − d_tap they come from
+//   and the ≤ 13 columns they produce are compile-time tables, so a thread
+//   accumulates its row in registers in one pass with no search, adds the
+//   contributions in the order the serial transposed lists hold them
+//   (sampled node ascending, then moment), and writes the row with ascending
+//   columns.  One thread handles the three rows of a node; a count pass, a
+//   scan and a fill pass produce row pointers, columns and values.
+// * y = A x with L lanes per row (L from the mean row length) and an
+//   optional per-block partial sum of ⟨w, y⟩.
+// * The Jacobi diagonal (gko::preconditioner::Jacobi, max block size 1, the
+//   reference's default preconditioner, :921-930).
+// * SpdIsai (:963-967): Ginkgo 1.8's factorized sparse approximate inverse on
+//   the pattern of tril(A) — for row i, J = stored columns ≤ i, solve
+//   A[J,J] y = e_|J| and scale by 1/√y_|J|; with A[J,J] = L Lᵀ this is
+//   G[i,J] = L⁻ᵀ e_|J|, one Cholesky factorisation and one back substitution
+//   per row, in registers.  Gᵀ is formed explicitly so that both products of
+//   z = Gᵀ (G r) are gathers.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+
+#include "operators.hpp"
+
+namespace sil {
+
+namespace {
+
+constexpr int kAsmBlock = 128;
+constexpr int kSpmvBlock = 256;
+constexpr int kSpmvMaxBlocks = 4096;
+constexpr int kFsaiMaxRow = 32;
+
+// ----------------------------------------------------------- stencil table ---
+
+struct Tap {
+    int di, dj, dk, comp, coeff;
+};
+
+// Moment stencils of the induction (magnetostatics_quantities.hpp:173-196):
+// B_x = (dA)_yz, B_y = −(dA)_xz, B_z = (dA)_xy as forward differences at s.
+constexpr Tap kMoment[3][4] = {
+    {{0, 1, 0, 2, 1}, {0, 0, 0, 2, -1}, {0, 0, 1, 1, -1}, {0, 0, 0, 1, 1}},
+    {{1, 0, 0, 2, -1}, {0, 0, 0, 2, 1}, {0, 0, 1, 0, 1}, {0, 0, 0, 0, -1}},
+    {{1, 0, 0, 1, 1}, {0, 0, 0, 1, -1}, {0, 1, 0, 0, -1}, {0, 0, 0, 0, 1}},
+};
+
+// column key of a relative offset (each in −1..1) and component: ascending
+// keys are ascending DOF indices whenever nx, ny ≥ 3
+constexpr int key_of(int di, int dj, int dk, int comp) { return (((dk + 1) * 3 + (dj + 1)) * 3 + (di + 1)) * 3 + comp; }
+
+// rank of the sampled node s = n − d in linear-index order: n−e_z, n−e_y, n−e_x, n
+constexpr int sampled_rank(const Tap& t) { return t.dk ? 0 : (t.dj ? 1 : (t.di ? 2 : 3)); }
+
+struct RowPlan {
+    int pair_m[4], pair_t[4]; // (moment, tap) pairs reaching the row, in slot order
+    int npairs;
+    int slot_of_key[81];      // -1: column not in the pattern
+    int key_of_slot[16];
+    int nslots;
+    int diag_slot;
+};
+
+constexpr RowPlan make_plan(int C)
+{
+    RowPlan p{};
+    int n = 0;
+    for (int rank = 0; rank < 4; ++rank)
+        for (int m = 0; m < 3; ++m)
+            for (int t = 0; t < 4; ++t)
+                if (kMoment[m][t].comp == C && sampled_rank(kMoment[m][t]) == rank) {
+                    p.pair_m[n] = m;
+                    p.pair_t[n] = t;
+                    ++n;
+                }
+    p.npairs = n;
+    bool used[81] = {};
+    used[key_of(0, 0, 0, C)] = true;
+    for (int q = 0; q < n; ++q) {
+        const Tap& t = kMoment[p.pair_m[q]][p.pair_t[q]];
+        for (int u = 0; u < 4; ++u) {
+            const Tap& v = kMoment[p.pair_m[q]][u];
+            used[key_of(v.di - t.di, v.dj - t.dj, v.dk - t.dk, v.comp)] = true;
+        }
+    }
+    int s = 0;
+    for (int k = 0; k < 81; ++k) {
+        p.slot_of_key[k] = used[k] ? s : -1;
+        if (used[k]) p.key_of_slot[s++] = k;
+    }
+    p.nslots = s;
+    p.diag_slot = p.slot_of_key[key_of(0, 0, 0, C)];
+    return p;
+}
+
+template <int C>
+struct Plan {
+    static constexpr RowPlan v = make_plan(C);
+    static_assert(v.npairs == 4 && v.nslots == 13, "curl-curl row pattern");
+};
+
+struct MsAsm {
+    int64_t nx, ny, nz;
+    double penalty;
+    const double* ls[3];
+    const double* nu;
+};
+
+__device__ __forceinline__ bool interior(const MsAsm& a, int64_t i, int64_t j, int64_t k)
+{
+    return i > 0 && j > 0 && k > 0 && i + 1 < a.nx && j + 1 < a.ny && k + 1 < a.nz;
+}
+
+// one interior row (component C at node (i,j,k)): COUNT writes the number of
+// stored entries, otherwise the row is written at rp[r]
+template <int C, bool COUNT>
+__device__ __forceinline__ void assemble_row(const MsAsm& a, int64_t i, int64_t j, int64_t k, int64_t* __restrict__ cnt,
+                                             const int32_t* __restrict__ rp, int32_t* __restrict__ col,
+                                             double* __restrict__ val)
+{
+#pragma clang fp contract(off)
+    constexpr RowPlan P = Plan<C>::v;
+    const int64_t node = i + a.nx * (j + a.ny * k);
+    const int64_t r = 3 * node + C;
+    double acc[P.nslots];
+#pragma unroll
+    for (int s = 0; s < P.nslots; ++s) acc[s] = 0.0;
+    uint32_t present = 0;
+    {
+        const double lx = a.ls[0][i], ly = a.ls[1][j], lz = a.ls[2][k];
+        const double wa = C == 0 ? ly * lz / lx : (C == 1 ? lx * lz / ly : lx * ly / lz);
+        const double g = a.penalty * wa;
+        if (g != 0.0) {
+            acc[P.diag_slot] += g;
+            present |= 1u << P.diag_slot;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < P.npairs; ++q) {
+        const int m = P.pair_m[q];
+        const Tap t = kMoment[m][P.pair_t[q]];
+        const int64_t si = i - t.di, sj = j - t.dj, sk = k - t.dk;
+        if (!interior(a, si, sj, sk)) continue;
+        const double sx = a.ls[0][si], sy = a.ls[1][sj], sz = a.ls[2][sk];
+        const double hf = m == 0 ? sx / (sy * sz) : (m == 1 ? sy / (sx * sz) : sz / (sx * sy));
+        const double jc = hf * a.nu[si + a.nx * (sj + a.ny * sk)];
+        if (jc == 0.0) continue;
+        const double rjc = t.coeff * jc;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const Tap v = kMoment[m][u];
+            if (!interior(a, si + v.di, sj + v.dj, sk + v.dk)) continue;
+            const int slot = P.slot_of_key[key_of(v.di - t.di, v.dj - t.dj, v.dk - t.dk, v.comp)];
+            acc[slot] += rjc * v.coeff;
+            present |= 1u << slot;
+        }
+    }
+    if constexpr (COUNT) {
+        cnt[r] = __popc(present);
+    } else {
+        int32_t out = rp[r];
+#pragma unroll
+        for (int s = 0; s < P.nslots; ++s) {
+            if (!((present >> s) & 1u)) continue;
+            const int key = P.key_of_slot[s];
+            const int comp = key % 3, di = (key / 3) % 3 - 1, dj = (key / 9) % 3 - 1, dk = key / 27 - 1;
+            col[out] = static_cast<int32_t>(3 * (node + di + a.nx * (dj + a.ny * dk)) + comp);
+            val[out] = acc[s];
+            ++out;
+        }
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kAsmBlock) void k_ms3d_assemble(MsAsm a, int64_t* __restrict__ cnt,
+                                                             const int32_t* __restrict__ rp, int32_t* __restrict__ col,
+                                                             double* __restrict__ val)
+{
+    const int64_t nn = a.nx * a.ny * a.nz;
+    for (int64_t node = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; node < nn;
+         node += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = node % a.nx, j = (node / a.nx) % a.ny, k = node / (a.nx * a.ny);
+        if (!interior(a, i, j, k)) {
+            for (int c = 0; c < 3; ++c) {
+                const int64_t r = 3 * node + c;
+                if constexpr (COUNT) {
+                    cnt[r] = 1;
+                } else {
+                    col[rp[r]] = static_cast<int32_t>(r);
+                    val[rp[r]] = 1.0;
+                }
+            }
+            continue;
+        }
+        assemble_row<0, COUNT>(a, i, j, k, cnt, rp, col, val);
+        assemble_row<1, COUNT>(a, i, j, k, cnt, rp, col, val);
+        assemble_row<2, COUNT>(a, i, j, k, cnt, rp, col, val);
+    }
+}
+
+// --------------------------------------------------------------- helpers ---
+
+inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 16)
+{
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, cap)));
+}
+
+// row pointers from per-row counts (int64): rp32[0] = 0, rp32[r+1] = Σ_{q≤r}
+// counts; fails when the total exceeds the int32 index range
+__global__ void k_rowptr_narrow(const int64_t* __restrict__ incl, int32_t* __restrict__ rp, int64_t n)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r <= n; r += (int64_t)gridDim.x * blockDim.x)
+        rp[r] = r == 0 ? 0 : static_cast<int32_t>(incl[r - 1]);
+}
+
+int rowptr_from_counts(const int64_t* counts, int64_t n, int32_t* rp, int64_t* nnz_out, hipStream_t s)
+{
+    int64_t* incl = nullptr;
+    void* tmp = nullptr;
+    size_t bytes = 0;
+    int rc = SIL_OK;
+    if (hipcub::DeviceScan::InclusiveSum(nullptr, bytes, counts, incl, static_cast<int>(n), s) != hipSuccess) {
+        set_error("row pointer scan: temporary size query failed");
+        return SIL_ERR_HIP;
+    }
+    SIL_CHECK_HIP(hipMalloc(&incl, sizeof(int64_t) * std::max<int64_t>(n, 1)));
+    if (hipMalloc(&tmp, bytes) != hipSuccess) {
+        (void)hipFree(incl);
+        set_error("row pointer scan: allocation failed");
+        return SIL_ERR_HIP;
+    }
+    int64_t total = 0;
+    if (n > 0 && hipcub::DeviceScan::InclusiveSum(tmp, bytes, counts, incl, static_cast<int>(n), s) != hipSuccess) {
+        set_error("row pointer scan failed");
+        rc = SIL_ERR_HIP;
+    }
+    if (rc == SIL_OK && n > 0 &&
+        (hipMemcpyAsync(&total, incl + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+         hipStreamSynchronize(s) != hipSuccess)) {
+        set_error("row pointer scan: reading the total failed");
+        rc = SIL_ERR_HIP;
+    }
+    if (rc == SIL_OK && total > INT32_MAX) {
+        set_error("CSR: %lld stored entries exceed the int32 index range", (long long)total);
+        rc = SIL_ERR_INVALID;
+    }
+    if (rc == SIL_OK) {
+        hipLaunchKernelGGL(k_rowptr_narrow, dim3(grid_for(n + 1, 256)), dim3(256), 0, s, incl, rp, n);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("row pointer kernel launch failed");
+            rc = SIL_ERR_HIP;
+        }
+    }
+    if (rc == SIL_OK && hipStreamSynchronize(s) != hipSuccess) {
+        set_error("row pointer kernel failed");
+        rc = SIL_ERR_HIP;
+    }
+    (void)hipFree(tmp);
+    (void)hipFree(incl);
+    *nnz_out = total;
+    return rc;
+}
+
+// ------------------------------------------------------------ sparse product ---
+
+struct CsrView {
+    int64_t n;
+    const int32_t* rp;
+    const int32_t* col;
+    const double* val;
+};
+
+inline CsrView view(const CsrData& a) { return CsrView{a.n, a.row_ptr, a.col, a.val}; }
+
+// y = A x; with `w` set, per-block partial sums of ⟨w, y⟩ go to partials[blockIdx]
+template <int L>
+__global__ __launch_bounds__(kSpmvBlock) void k_csr_spmv(CsrView A, const double* __restrict__ x,
+                                                         double* __restrict__ y, const double* __restrict__ w,
+                                                         double* __restrict__ partials, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    constexpr int kRows = kSpmvBlock / L;
+    const int lane = threadIdx.x % L;
+    double dot = 0.0;
+    for (int64_t base = (int64_t)blockIdx.x * kRows; base < A.n; base += (int64_t)gridDim.x * kRows) {
+        const int64_t row = base + threadIdx.x / L;
+        double v = 0.0;
+        if (row < A.n) {
+            const int32_t e = A.rp[row + 1];
+            for (int32_t q = A.rp[row] + lane; q < e; q += L) v += A.val[q] * x[A.col[q]];
+        }
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, L);
+        if (lane == 0 && row < A.n) {
+            y[row] = v;
+            if (w) dot += w[row] * v;
+        }
+    }
+    if (partials) {
+        using BR = hipcub::BlockReduce<double, kSpmvBlock>;
+        __shared__ typename BR::TempStorage tmp;
+        const double sum = BR(tmp).Sum(dot);
+        if (threadIdx.x == 0) partials[blockIdx.x] = sum;
+    }
+}
+
+__global__ void k_csr_diag(CsrView A, double* __restrict__ d)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < A.n; r += (int64_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int32_t q = A.rp[r]; q < A.rp[r + 1]; ++q)
+            if (A.col[q] == r) v = A.val[q];
+        d[r] = v;
+    }
+}
+
+// flag bit 0: row pointers not monotone / out of [0, nnz]; bit 1: column
+// out of range; bit 2: columns not strictly ascending
+__global__ void k_csr_check(int64_t n, int64_t nnz, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            int* __restrict__ flag)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = rp[r], e = rp[r + 1];
+        if (b < 0 || e < b || e > nnz || (r == 0 && b != 0) || (r == n - 1 && e != nnz)) {
+            atomicOr(flag, 1);
+            continue;
+        }
+        for (int32_t q = b; q < e; ++q) {
+            if (col[q] < 0 || col[q] >= n) atomicOr(flag, 2);
+            if (q > b && col[q] <= col[q - 1]) atomicOr(flag, 4);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ FSAI ---
+
+// lower counts m_i = |{stored j ≤ i}|; flag bit 0: diagonal missing; the
+// block maxima of m_i go to maxm
+__global__ __launch_bounds__(256) void k_fsai_count(CsrView A, int64_t* __restrict__ cnt, int* __restrict__ flag,
+                                                    int* __restrict__ maxm)
+{
+    int local_max = 0;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < A.n; r += (int64_t)gridDim.x * blockDim.x) {
+        int32_t q = A.rp[r];
+        const int32_t e = A.rp[r + 1];
+        while (q < e && A.col[q] <= r) ++q;
+        const int m = q - A.rp[r];
+        if (m == 0 || A.col[q - 1] != r) atomicOr(flag, 1);
+        cnt[r] = m;
+        local_max = std::max(local_max, m);
+    }
+    using BR = hipcub::BlockReduce<int, 256>;
+    __shared__ typename BR::TempStorage tmp;
+    const int bm = BR(tmp).Reduce(local_max, hipcub::Max());
+    if (threadIdx.x == 0) atomicMax(maxm, bm);
+}
+
+// G row i: Cholesky of S = A[J,J] (entry (a,b), b ≤ a, from row J[a]) and
+// Lᵀ g = e_m.  flag bit 1: a local system is not positive definite.
+template <int M>
+__global__ __launch_bounds__(64) void k_fsai_generate(CsrView A, const int32_t* __restrict__ grp,
+                                                      int32_t* __restrict__ gcol, double* __restrict__ gval,
+                                                      int* __restrict__ flag)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const int m = grp[i + 1] - grp[i];
+    const int32_t ab = A.rp[i];
+    int J[M];
+    double S[M * (M + 1) / 2]; // packed lower triangle, (a, b) at a(a+1)/2 + b
+#pragma unroll
+    for (int a = 0; a < M; ++a) J[a] = a < m ? A.col[ab + a] : INT32_MAX;
+#pragma unroll
+    for (int q = 0; q < M * (M + 1) / 2; ++q) S[q] = 0.0;
+#pragma unroll
+    for (int a = 0; a < M; ++a) {
+        if (a < m) {
+            const int32_t e = A.rp[J[a] + 1];
+            for (int32_t q = A.rp[J[a]]; q < e; ++q) {
+                const int32_t c = A.col[q];
+                if (c > J[a]) break;
+                const double v = A.val[q];
+#pragma unroll
+                for (int b = 0; b <= a; ++b)
+                    if (c == J[b]) S[a * (a + 1) / 2 + b] = v;
+            }
+        }
+    }
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < M; ++a) {
+        if (a < m) {
+#pragma unroll
+            for (int b = 0; b < a; ++b) {
+                double v = S[a * (a + 1) / 2 + b];
+#pragma unroll
+                for (int c = 0; c < b; ++c) v -= S[a * (a + 1) / 2 + c] * S[b * (b + 1) / 2 + c];
+                S[a * (a + 1) / 2 + b] = v / S[b * (b + 1) / 2 + b];
+            }
+            double d = S[a * (a + 1) / 2 + a];
+#pragma unroll
+            for (int c = 0; c < a; ++c) d -= S[a * (a + 1) / 2 + c] * S[a * (a + 1) / 2 + c];
+            ok = ok && d > 0.0;
+            S[a * (a + 1) / 2 + a] = ok ? sqrt(d) : 1.0;
+        }
+    }
+    if (!ok) atomicOr(flag, 2);
+    double g[M];
+#pragma unroll
+    for (int a = M - 1; a >= 0; --a) {
+        if (a >= m) {
+            g[a] = 0.0;
+            continue;
+        }
+        double v = a == m - 1 ? 1.0 : 0.0;
+#pragma unroll
+        for (int c = a + 1; c < M; ++c)
+            if (c < m) v -= S[c * (c + 1) / 2 + a] * g[c];
+        g[a] = v / S[a * (a + 1) / 2 + a];
+    }
+    const int32_t gb = grp[i];
+#pragma unroll
+    for (int a = 0; a < M; ++a) {
+        if (a < m) {
+            gcol[gb + a] = J[a];
+            gval[gb + a] = g[a];
+        }
+    }
+}
+
+// transpose: column counts, then scatter through per-row cursors, then sort
+// each row by column (rows of Gᵀ are short)
+__global__ void k_t_count(CsrView G, int64_t* __restrict__ cnt)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < G.n; r += (int64_t)gridDim.x * blockDim.x)
+        for (int32_t q = G.rp[r]; q < G.rp[r + 1]; ++q)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[G.col[q]]), 1ull);
+}
+
+__global__ void k_t_fill(CsrView G, int32_t* __restrict__ cursor, int32_t* __restrict__ tcol,
+                         double* __restrict__ tval)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < G.n; r += (int64_t)gridDim.x * blockDim.x)
+        for (int32_t q = G.rp[r]; q < G.rp[r + 1]; ++q) {
+            const int32_t pos = atomicAdd(&cursor[G.col[q]], 1);
+            tcol[pos] = static_cast<int32_t>(r);
+            tval[pos] = G.val[q];
+        }
+}
+
+__global__ void k_sort_rows(int64_t n, const int32_t* __restrict__ rp, int32_t* __restrict__ col,
+                            double* __restrict__ val)
+{
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = rp[r], e = rp[r + 1];
+        for (int32_t q = b + 1; q < e; ++q) {
+            const int32_t kc = col[q];
+            const double kv = val[q];
+            int32_t p = q - 1;
+            while (p >= b && col[p] > kc) {
+                col[p + 1] = col[p];
+                val[p + 1] = val[p];
+                --p;
+            }
+            col[p + 1] = kc;
+            val[p + 1] = kv;
+        }
+    }
+}
+
+int alloc_csr(CsrData& a, int64_t n, int64_t nnz)
+{
+    a.n = n;
+    a.nnz = nnz;
+    SIL_CHECK_HIP(hipMalloc(&a.row_ptr, sizeof(int32_t) * (n + 1)));
+    SIL_CHECK_HIP(hipMalloc(&a.col, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+    SIL_CHECK_HIP(hipMalloc(&a.val, sizeof(double) * std::max<int64_t>(nnz, 1)));
+    return SIL_OK;
+}
+
+void choose_lanes(CsrData& a)
+{
+    const double mean = a.n > 0 ? static_cast<double>(a.nnz) / static_cast<double>(a.n) : 1.0;
+    a.lanes = mean > 10.0 ? 16 : (mean > 5.0 ? 8 : (mean > 2.5 ? 4 : 2));
+}
+
+int read_flag(const int* dflag, int* out, hipStream_t s)
+{
+    SIL_CHECK_HIP(hipMemcpyAsync(out, dflag, sizeof(int), hipMemcpyDeviceToHost, s));
+    SIL_CHECK_HIP(hipStreamSynchronize(s));
+    return SIL_OK;
+}
+
+}  // namespace
+
+void csr_free(CsrData& a)
+{
+    for (void* p : {static_cast<void*>(a.row_ptr), static_cast<void*>(a.col), static_cast<void*>(a.val)})
+        if (p) (void)hipFree(p);
+    a = CsrData{};
+}
+
+int csr_assemble_ms3d(const Ms3dData& d, CsrData& out, hipStream_t s)
+{
+    SIL_REQUIRE(3 * d.nx * d.ny * d.nz < INT32_MAX, SIL_ERR_INVALID,
+                "assemble: %lld rows exceed the int32 index range", (long long)(3 * d.nx * d.ny * d.nz));
+    const int64_t nn = d.nx * d.ny * d.nz, n = 3 * nn;
+    MsAsm a{d.nx, d.ny, d.nz, d.penalty, {d.ls[0], d.ls[1], d.ls[2]}, d.nu};
+    int64_t* cnt = nullptr;
+    SIL_CHECK_HIP(hipMalloc(&cnt, sizeof(int64_t) * n));
+    CsrData m{};
+    int rc = SIL_OK;
+    const unsigned g = grid_for(nn, kAsmBlock);
+    hipLaunchKernelGGL((k_ms3d_assemble<true>), dim3(g), dim3(kAsmBlock), 0, s, a, cnt, nullptr, nullptr, nullptr);
+    if (hipGetLastError() != hipSuccess) {
+        set_error("assemble: count kernel launch failed");
+        rc = SIL_ERR_HIP;
+    }
+    int64_t nnz = 0;
+    if (rc == SIL_OK && hipMalloc(&m.row_ptr, sizeof(int32_t) * (n + 1)) != hipSuccess) {
+        set_error("assemble: allocation failed");
+        rc = SIL_ERR_HIP;
+    }
+    if (rc == SIL_OK) rc = rowptr_from_counts(cnt, n, m.row_ptr, &nnz, s);
+    (void)hipFree(cnt);
+    if (rc == SIL_OK) {
+        m.n = n;
+        m.nnz = nnz;
+        if (hipMalloc(&m.col, sizeof(int32_t) * std::max<int64_t>(nnz, 1)) != hipSuccess ||
+            hipMalloc(&m.val, sizeof(double) * std::max<int64_t>(nnz, 1)) != hipSuccess) {
+            set_error("assemble: allocation of %lld entries failed", (long long)nnz);
+            rc = SIL_ERR_HIP;
+        }
+    }
+    if (rc == SIL_OK) {
+        hipLaunchKernelGGL((k_ms3d_assemble<false>), dim3(g), dim3(kAsmBlock), 0, s, a, nullptr, m.row_ptr, m.col,
+                           m.val);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("assemble: fill kernel launch failed");
+            rc = SIL_ERR_HIP;
+        }
+    }
+    if (rc != SIL_OK) {
+        csr_free(m);
+        return rc;
+    }
+    choose_lanes(m);
+    out = m;
+    return SIL_OK;
+}
+
+int csr_import(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* col, const double* val, CsrData& out,
+               hipStream_t s)
+{
+    SIL_REQUIRE(n > 0 && n < INT32_MAX && nnz >= 0 && nnz <= INT32_MAX, SIL_ERR_INVALID,
+                "csr_create: size %lld / %lld entries outside the int32 index range", (long long)n, (long long)nnz);
+    CsrData m{};
+    int rc = alloc_csr(m, n, nnz);
+    int* flag = nullptr;
+    if (rc == SIL_OK && hipMalloc(&flag, sizeof(int)) != hipSuccess) {
+        set_error("csr_create: allocation failed");
+        rc = SIL_ERR_HIP;
+    }
+    if (rc == SIL_OK &&
+        (hipMemcpyAsync(m.row_ptr, rp, sizeof(int32_t) * (n + 1), hipMemcpyDefault, s) != hipSuccess ||
+         (nnz > 0 && hipMemcpyAsync(m.col, col, sizeof(int32_t) * nnz, hipMemcpyDefault, s) != hipSuccess) ||
+         (nnz > 0 && hipMemcpyAsync(m.val, val, sizeof(double) * nnz, hipMemcpyDefault, s) != hipSuccess) ||
+         hipMemsetAsync(flag, 0, sizeof(int), s) != hipSuccess)) {
+        set_error("csr_create: copy failed");
+        rc = SIL_ERR_HIP;
+    }
+    int h = 0;
+    if (rc == SIL_OK) {
+        hipLaunchKernelGGL(k_csr_check, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nnz, m.row_ptr, m.col, flag);
+        rc = read_flag(flag, &h, s);
+    }
+    if (flag) (void)hipFree(flag);
+    if (rc == SIL_OK && h) {
+        set_error("csr_create: %s", (h & 1) ? "row pointers are not monotone from 0 to nnz"
+                                            : ((h & 2) ? "a column index is out of range"
+                                                       : "columns are not strictly ascending within a row"));
+        rc = SIL_ERR_INVALID;
+    }
+    if (rc != SIL_OK) {
+        csr_free(m);
+        return rc;
+    }
+    choose_lanes(m);
+    out = m;
+    return SIL_OK;
+}
+
+int csr_spmv(const CsrData& a, const double* x, double* y, const double* w, double* partials, const int* done,
+             int* nb_out, hipStream_t s)
+{
+    const int rows = kSpmvBlock / a.lanes;
+    const unsigned g = grid_for(a.n, rows, kSpmvMaxBlocks);
+    const CsrView v = view(a);
+    switch (a.lanes) {
+    case 2: hipLaunchKernelGGL((k_csr_spmv<2>), dim3(g), dim3(kSpmvBlock), 0, s, v, x, y, w, partials, done); break;
+    case 4: hipLaunchKernelGGL((k_csr_spmv<4>), dim3(g), dim3(kSpmvBlock), 0, s, v, x, y, w, partials, done); break;
+    case 8: hipLaunchKernelGGL((k_csr_spmv<8>), dim3(g), dim3(kSpmvBlock), 0, s, v, x, y, w, partials, done); break;
+    default: hipLaunchKernelGGL((k_csr_spmv<16>), dim3(g), dim3(kSpmvBlock), 0, s, v, x, y, w, partials, done); break;
+    }
+    SIL_LAUNCH_CHECK();
+    if (nb_out) *nb_out = static_cast<int>(g);
+    return SIL_OK;
+}
+
+int csr_apply64(const CsrData& a, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,
+                int* nb_out, hipStream_t s)
+{
+    const double* x = in;
+    const int* done = nullptr;
+    if (pu) {
+        int rc = direction_pass64(*pu, a.n, s);
+        if (rc) return rc;
+        x = pu->p_new;
+        done = pu->done;
+    }
+    return csr_spmv(a, x, out, dot.partials ? x : nullptr, dot.partials, done, nb_out, s);
+}
+
+int csr_diagonal(const CsrData& a, double* d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_csr_diag, dim3(grid_for(a.n, 256)), dim3(256), 0, s, view(a), d);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int spd_isai_generate(const CsrData& a, CsrData& g_out, CsrData& gt_out, hipStream_t s)
+{
+    const int64_t n = a.n;
+    int64_t* cnt = nullptr;
+    int* flag = nullptr; // [0] flags, [1] max lower count
+    SIL_CHECK_HIP(hipMalloc(&cnt, sizeof(int64_t) * n));
+    CsrData g{}, gt{};
+    int rc = SIL_OK;
+    auto fail = [&](const char* what) {
+        set_error("spd_isai: %s", what);
+        rc = SIL_ERR_HIP;
+    };
+    if (hipMalloc(&flag, 2 * sizeof(int)) != hipSuccess || hipMemsetAsync(flag, 0, 2 * sizeof(int), s) != hipSuccess)
+        fail("allocation failed");
+    int h[2] = {0, 0};
+    if (rc == SIL_OK) {
+        hipLaunchKernelGGL(k_fsai_count, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, view(a), cnt, flag, flag + 1);
+        if (hipMemcpyAsync(h, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            fail("count kernel failed");
+    }
+    if (rc == SIL_OK && (h[0] & 1)) {
+        set_error("spd_isai: a row has no stored diagonal entry");
+        rc = SIL_ERR_INVALID;
+    }
+    if (rc == SIL_OK && h[1] > kFsaiMaxRow) {
+        set_error("spd_isai: a row has %d stored entries in its lower triangle (at most %d)", h[1], kFsaiMaxRow);
+        rc = SIL_ERR_INVALID;
+    }
+    int64_t gnnz = 0;
+    if (rc == SIL_OK && hipMalloc(&g.row_ptr, sizeof(int32_t) * (n + 1)) != hipSuccess) fail("allocation failed");
+    if (rc == SIL_OK) rc = rowptr_from_counts(cnt, n, g.row_ptr, &gnnz, s);
+    if (rc == SIL_OK) {
+        g.n = n;
+        g.nnz = gnnz;
+        if (hipMalloc(&g.col, sizeof(int32_t) * std::max<int64_t>(gnnz, 1)) != hipSuccess ||
+            hipMalloc(&g.val, sizeof(double) * std::max<int64_t>(gnnz, 1)) != hipSuccess)
+            fail("allocation failed");
+    }
+    if (rc == SIL_OK) {
+        const dim3 grid(static_cast<unsigned>((n + 63) / 64));
+        if (h[1] <= 8)
+            hipLaunchKernelGGL((k_fsai_generate<8>), grid, dim3(64), 0, s, view(a), g.row_ptr, g.col, g.val, flag);
+        else if (h[1] <= 16)
+            hipLaunchKernelGGL((k_fsai_generate<16>), grid, dim3(64), 0, s, view(a), g.row_ptr, g.col, g.val, flag);
+        else
+            hipLaunchKernelGGL((k_fsai_generate<32>), grid, dim3(64), 0, s, view(a), g.row_ptr, g.col, g.val, flag);
+        if (hipMemcpyAsync(h, flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            fail("generation kernel failed");
+    }
+    if (rc == SIL_OK && (h[0] & 2)) {
+        set_error("spd_isai: a local system A[J,J] is not positive definite");
+        rc = SIL_ERR_INVALID;
+    }
+    // Gᵀ
+    int32_t* cursor = nullptr;
+    if (rc == SIL_OK && hipMemsetAsync(cnt, 0, sizeof(int64_t) * n, s) != hipSuccess) fail("memset failed");
+    if (rc == SIL_OK) {
+        hipLaunchKernelGGL(k_t_count, dim3(grid_for(n, 256)), dim3(256), 0, s, view(g), cnt);
+        if (hipMalloc(&gt.row_ptr, sizeof(int32_t) * (n + 1)) != hipSuccess) fail("allocation failed");
+    }
+    int64_t tnnz = 0;
+    if (rc == SIL_OK) rc = rowptr_from_counts(cnt, n, gt.row_ptr, &tnnz, s);
+    if (rc == SIL_OK) {
+        gt.n = n;
+        gt.nnz = tnnz;
+        if (hipMalloc(&gt.col, sizeof(int32_t) * std::max<int64_t>(tnnz, 1)) != hipSuccess ||
+            hipMalloc(&gt.val, sizeof(double) * std::max<int64_t>(tnnz, 1)) != hipSuccess ||
+            hipMalloc(&cursor, sizeof(int32_t) * (n + 1)) != hipSuccess ||
+            hipMemcpyAsync(cursor, gt.row_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, s) != hipSuccess)
+            fail("allocation failed");
+    }
+    if (rc == SIL_OK) {
+        hipLaunchKernelGGL(k_t_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, view(g), cursor, gt.col, gt.val);
+        hipLaunchKernelGGL(k_sort_rows, dim3(grid_for(n, 256)), dim3(256), 0, s, n, gt.row_ptr, gt.col, gt.val);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) fail("transpose failed");
+    }
+    if (cursor) (void)hipFree(cursor);
+    (void)hipFree(cnt);
+    if (flag) (void)hipFree(flag);
+    if (rc != SIL_OK) {
+        csr_free(g);
+        csr_free(gt);
+        return rc;
+    }
+    choose_lanes(g);
+    choose_lanes(gt);
+    g_out = g;
+    gt_out = gt;
+    return SIL_OK;
+}
+
+}  // namespace sil

@@ -56,7 +56,25 @@ void free_operator(sil_operator op)
     f(op->p2.wy);
     f(op->p2.wx32);
     f(op->p2.wy32);
+    csr_free(op->csr);
     delete op;
+}
+
+// a new operator owning an assembled matrix
+int wrap_csr(const CsrData& m, sil_operator* out)
+{
+    auto* op = new (std::nothrow) sil_operator_s();
+    if (!op) {
+        CsrData tmp = m;
+        csr_free(tmp);
+        set_error("out of host memory");
+        return SIL_ERR_INVALID;
+    }
+    op->kind = OP_CSR;
+    op->n = m.n;
+    op->csr = m;
+    *out = op;
+    return SIL_OK;
 }
 
 }  // namespace
@@ -181,6 +199,71 @@ int sil_poisson2d_create(sil_operator* out, int64_t nx, int64_t ny, const double
     return SIL_OK;
 }
 
+int sil_magnetostatics3d_assemble(sil_operator op, sil_operator* csr_out, void* stream)
+{
+    SIL_REQUIRE(op && csr_out, SIL_ERR_INVALID, "magnetostatics3d_assemble: null argument");
+    SIL_REQUIRE(op->kind == OP_MAGNETOSTATICS3D, SIL_ERR_INVALID,
+                "magnetostatics3d_assemble: not a magnetostatics operator");
+    hipStream_t s = as_stream(stream);
+    CsrData m{};
+    int rc = csr_assemble_ms3d(op->ms, m, s);
+    if (rc) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        csr_free(m);
+        set_error("magnetostatics3d_assemble: assembly kernels failed");
+        return SIL_ERR_HIP;
+    }
+    return wrap_csr(m, csr_out);
+}
+
+int sil_csr_create(sil_operator* out, int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* cols,
+                   const double* vals, void* stream)
+{
+    SIL_REQUIRE(out && row_ptr && (nnz == 0 || (cols && vals)), SIL_ERR_INVALID, "csr_create: null argument");
+    CsrData m{};
+    int rc = csr_import(n, nnz, row_ptr, cols, vals, m, as_stream(stream));
+    if (rc) return rc;
+    return wrap_csr(m, out);
+}
+
+int64_t sil_csr_nnz(sil_operator op) { return op && op->kind == OP_CSR ? op->csr.nnz : -1; }
+
+int sil_csr_copy(sil_operator op, int32_t* row_ptr, int32_t* cols, double* vals, void* stream)
+{
+    SIL_REQUIRE(op && row_ptr && cols && vals, SIL_ERR_INVALID, "csr_copy: null argument");
+    SIL_REQUIRE(op->kind == OP_CSR, SIL_ERR_INVALID, "csr_copy: not an assembled operator");
+    hipStream_t s = as_stream(stream);
+    const CsrData& m = op->csr;
+    SIL_CHECK_HIP(hipMemcpyAsync(row_ptr, m.row_ptr, sizeof(int32_t) * (m.n + 1), hipMemcpyDefault, s));
+    if (m.nnz > 0) {
+        SIL_CHECK_HIP(hipMemcpyAsync(cols, m.col, sizeof(int32_t) * m.nnz, hipMemcpyDefault, s));
+        SIL_CHECK_HIP(hipMemcpyAsync(vals, m.val, sizeof(double) * m.nnz, hipMemcpyDefault, s));
+    }
+    SIL_CHECK_HIP(hipStreamSynchronize(s));
+    return SIL_OK;
+}
+
+int sil_spd_isai_create(sil_operator a, sil_operator* g_out, void* stream)
+{
+    SIL_REQUIRE(a && g_out, SIL_ERR_INVALID, "spd_isai_create: null argument");
+    SIL_REQUIRE(a->kind == OP_CSR || a->kind == OP_MAGNETOSTATICS3D, SIL_ERR_INVALID,
+                "spd_isai_create: needs an assembled operator or the magnetostatics operator");
+    hipStream_t s = as_stream(stream);
+    CsrData assembled{};
+    const CsrData* m = &a->csr;
+    int rc;
+    if (a->kind == OP_MAGNETOSTATICS3D) {
+        if ((rc = csr_assemble_ms3d(a->ms, assembled, s))) return rc;
+        m = &assembled;
+    }
+    CsrData g{}, gt{};
+    rc = spd_isai_generate(*m, g, gt, s);
+    csr_free(assembled);
+    if (rc) return rc;
+    csr_free(gt);
+    return wrap_csr(g, g_out);
+}
+
 int64_t sil_operator_size(sil_operator op) { return op ? op->n : -1; }
 
 int sil_operator_apply(sil_operator op, const double* in, double* out, void* stream)
@@ -228,7 +311,8 @@ int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs,
     SIL_REQUIRE(settings->relative_tolerance >= 0.0, SIL_ERR_INVALID, "negative relative tolerance");
     SIL_REQUIRE(settings->precision == SIL_PRECISION_FP64 || settings->precision == SIL_PRECISION_MIXED,
                 SIL_ERR_INVALID, "unknown precision mode %d", settings->precision);
-    SIL_REQUIRE(settings->preconditioner == SIL_PRECOND_IDENTITY || settings->preconditioner == SIL_PRECOND_JACOBI,
+    SIL_REQUIRE(settings->preconditioner == SIL_PRECOND_IDENTITY || settings->preconditioner == SIL_PRECOND_JACOBI ||
+                    settings->preconditioner == SIL_PRECOND_SPD_ISAI,
                 SIL_ERR_INVALID, "unknown preconditioner %d", settings->preconditioner);
     *diagnostics = sil_solver_diagnostics{};
     return cg_solve(op, rhs, solution, *settings, *diagnostics, as_stream(stream));

@@ -45,8 +45,12 @@ struct Workspace {
     CgScalars* host_sc = nullptr; // pinned
     int apply_blocks = 0;
     bool jacobi_ready = false;
-    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}; // [precision][precond]
-    int graph_chunk[2][2] = {{0, 0}, {0, 0}};
+    // SpdIsai: factor G, its transpose, t = G r and z = Gᵀ t
+    CsrData isai_g, isai_gt;
+    double *isai_t = nullptr, *isai_z = nullptr;
+    bool isai_ready = false;
+    hipGraphExec_t graph[2][3] = {}; // [precision][preconditioner: identity, Jacobi, SpdIsai]
+    int graph_chunk[2][3] = {};
 };
 
 namespace {
@@ -133,11 +137,11 @@ __global__ __launch_bounds__(kUpdBlock) void k_init(const double* __restrict__ b
     }
 }
 
-__global__ void k_scalars_init(CgScalars* sc, const double* part_bb, const double* part_rz, int count, double tol,
-                               int max_iter)
+__global__ void k_scalars_init(CgScalars* sc, const double* part_bb, int count_bb, const double* part_rz,
+                               int count_rz, double tol, int max_iter)
 {
-    const double bb = reduce_partials(part_bb, count);
-    const double rz = reduce_partials(part_rz, count);
+    const double bb = reduce_partials(part_bb, count_bb);
+    const double rz = reduce_partials(part_rz, count_rz);
     if (threadIdx.x == 0) {
         sc->bb = bb;
         sc->rr = bb;
@@ -203,11 +207,11 @@ __global__ __launch_bounds__(kUpdBlock) void k_update(const CgScalars* __restric
     }
 }
 
-__global__ void k_beta(CgScalars* sc, const double* part_rz, const double* part_rr, int count)
+__global__ void k_beta(CgScalars* sc, const double* part_rz, int count_rz, const double* part_rr, int count_rr)
 {
     if (sc->done) return;
-    const double rz = reduce_partials(part_rz, count);
-    const double rr = reduce_partials(part_rr, count);
+    const double rz = reduce_partials(part_rz, count_rz);
+    const double rr = reduce_partials(part_rr, count_rr);
     if (threadIdx.x == 0) {
         sc->beta = rz / sc->rz;
         sc->rz_prev = sc->rz;
@@ -320,6 +324,7 @@ int apply64(sil_operator op, const double* in, double* out, DotEpilogue dot, con
             hipStream_t s)
 {
     if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_apply64(op->ms, in, out, dot, pu, nb, s);
+    if (op->kind == OP_CSR) return csr_apply64(op->csr, in, out, dot, pu, nb, s);
     return p2d_apply64(op->p2, in, out, dot, pu, nb, s);
 }
 
@@ -327,13 +332,24 @@ int apply32(sil_operator op, const float* in, float* out, DotEpilogue dot, const
             hipStream_t s)
 {
     if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_apply32(op->ms, in, out, dot, pu, nb, s);
+    SIL_REQUIRE(op->kind != OP_CSR, SIL_ERR_UNSUPPORTED, "assembled operators have FP64 values only");
     return p2d_apply32(op->p2, in, out, dot, pu, nb, s);
 }
 
 int diagonal(sil_operator op, double* d, hipStream_t s)
 {
     if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_diagonal(op->ms, d, s);
+    if (op->kind == OP_CSR) return csr_diagonal(op->csr, d, s);
     return p2d_diagonal(op->p2, d, s);
+}
+
+// z = Gᵀ (G r) with per-block ⟨r, z⟩ partials; both products return at once
+// when *done is set
+int isai_apply(Workspace& w, const double* r, double* part, int* nb, const int* done, hipStream_t s)
+{
+    int rc = csr_spmv(w.isai_g, r, w.isai_t, nullptr, nullptr, done, nullptr, s);
+    if (rc) return rc;
+    return csr_spmv(w.isai_gt, w.isai_t, w.isai_z, r, part, done, nb, s);
 }
 
 template <typename P>
@@ -362,7 +378,7 @@ int ensure_workspace(sil_operator op, bool mixed, hipStream_t s)
         int nb = 0;
         if (op->kind == OP_MAGNETOSTATICS3D)
             nb = static_cast<int>((op->ms.nx * op->ms.ny * op->ms.nz + 255) / 256);
-        else
+        else if (op->kind == OP_POISSON2D)
             nb = static_cast<int>((op->p2.nx * op->p2.ny + 255) / 256);
         w.apply_blocks = nb;
         (void)s;
@@ -379,15 +395,23 @@ int ensure_workspace(sil_operator op, bool mixed, hipStream_t s)
     return SIL_OK;
 }
 
-// enqueue one CG iteration; `parity` selects the p ping-pong buffers
+enum Precond { PC_IDENTITY = 0, PC_JACOBI = 1, PC_ISAI = 2 };
+
+// enqueue one CG iteration; pold / pnew are the p ping-pong buffers.  With
+// SpdIsai the direction is formed from z (not from r) and the update's
+// ⟨r, z⟩ partials are replaced by those of the preconditioner application.
 template <typename T>
-int enqueue_iteration(sil_operator op, T* r, T* q, T* pold, T* pnew, bool jacobi, hipStream_t s)
+int enqueue_iteration(sil_operator op, T* r, T* q, T* pold, T* pnew, int pc, hipStream_t s)
 {
     Workspace& w = *op->ws;
     double* part0 = w.partials;
     double* part1 = w.partials + max_dot_blocks();
     double* part2 = w.partials + 2 * max_dot_blocks();
-    PUpdate<T> pu{r, pold, pnew, &w.sc->beta, jacobi ? w.inv_diag : nullptr, &w.sc->done};
+    const double* invd = pc == PC_JACOBI ? w.inv_diag : nullptr;
+    const T* zsrc = r;
+    if constexpr (sizeof(T) == 8)
+        if (pc == PC_ISAI) zsrc = w.isai_z;
+    PUpdate<T> pu{zsrc, pold, pnew, &w.sc->beta, invd, &w.sc->done};
     int nb = 0;
     int rc;
     if constexpr (sizeof(T) == 8)
@@ -396,20 +420,23 @@ int enqueue_iteration(sil_operator op, T* r, T* q, T* pold, T* pnew, bool jacobi
         rc = apply32(op, nullptr, q, DotEpilogue{part0}, &pu, &nb, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, nb);
-    hipLaunchKernelGGL((k_update<T>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.sc, w.x, pnew, r, q,
-                       jacobi ? w.inv_diag : nullptr, op->n, part1, part2);
-    hipLaunchKernelGGL(k_beta, dim3(1), dim3(kRedThreads), 0, s, w.sc, part1, part2, kUpdBlocks);
+    hipLaunchKernelGGL((k_update<T>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.sc, w.x, pnew, r, q, invd, op->n,
+                       part1, part2);
+    int nb_rz = kUpdBlocks;
+    if constexpr (sizeof(T) == 8)
+        if (pc == PC_ISAI && (rc = isai_apply(w, r, part1, &nb_rz, &w.sc->done, s))) return rc;
+    hipLaunchKernelGGL(k_beta, dim3(1), dim3(kRedThreads), 0, s, w.sc, part1, nb_rz, part2, kUpdBlocks);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
 
 template <typename T>
-int get_graph(sil_operator op, int prec, bool jacobi, int chunk, T* r, T* q, T* const* p, hipStream_t s,
+int get_graph(sil_operator op, int prec, int pc, int chunk, T* r, T* q, T* const* p, hipStream_t s,
               hipGraphExec_t* out)
 {
     Workspace& w = *op->ws;
-    hipGraphExec_t& g = w.graph[prec][jacobi ? 1 : 0];
-    if (g && w.graph_chunk[prec][jacobi ? 1 : 0] == chunk) {
+    hipGraphExec_t& g = w.graph[prec][pc];
+    if (g && w.graph_chunk[prec][pc] == chunk) {
         *out = g;
         return SIL_OK;
     }
@@ -423,7 +450,7 @@ int get_graph(sil_operator op, int prec, bool jacobi, int chunk, T* r, T* q, T* 
     SIL_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     int rc = SIL_OK;
     for (int it = 0; it < chunk && rc == SIL_OK; ++it)
-        rc = enqueue_iteration<T>(op, r, q, p[it & 1], p[(it + 1) & 1], jacobi, cs);
+        rc = enqueue_iteration<T>(op, r, q, p[it & 1], p[(it + 1) & 1], pc, cs);
     hipError_t e = hipStreamEndCapture(cs, &graph);
     if (rc != SIL_OK) {
         (void)hipStreamDestroy(cs);
@@ -441,7 +468,7 @@ int get_graph(sil_operator op, int prec, bool jacobi, int chunk, T* r, T* q, T* 
         set_error("graph instantiation failed: %s", hipGetErrorString(e));
         return SIL_ERR_HIP;
     }
-    w.graph_chunk[prec][jacobi ? 1 : 0] = chunk;
+    w.graph_chunk[prec][pc] = chunk;
     (void)s;
     *out = g;
     return SIL_OK;
@@ -470,8 +497,11 @@ int true_residual(sil_operator op, hipStream_t s)
 void destroy_workspace(Workspace* w)
 {
     if (!w) return;
-    for (double* p : {w->x, w->b, w->r, w->q, w->p[0], w->p[1], w->inv_diag, w->tmp, w->partials})
+    for (double* p : {w->x, w->b, w->r, w->q, w->p[0], w->p[1], w->inv_diag, w->tmp, w->partials, w->isai_t,
+                      w->isai_z})
         if (p) (void)hipFree(p);
+    csr_free(w->isai_g);
+    csr_free(w->isai_gt);
     for (float* p : {w->r32, w->q32, w->p32[0], w->p32[1]})
         if (p) (void)hipFree(p);
     if (w->sc) (void)hipFree(w->sc);
@@ -487,11 +517,34 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
 {
     const bool mixed = st.precision == SIL_PRECISION_MIXED;
     const bool jacobi = st.preconditioner == SIL_PRECOND_JACOBI;
+    const int pc = jacobi ? PC_JACOBI : (st.preconditioner == SIL_PRECOND_SPD_ISAI ? PC_ISAI : PC_IDENTITY);
+    SIL_REQUIRE(!(mixed && op->kind == OP_CSR), SIL_ERR_UNSUPPORTED,
+                "mixed precision needs a matrix-free operator (assembled values are FP64)");
+    SIL_REQUIRE(!(mixed && pc == PC_ISAI), SIL_ERR_UNSUPPORTED, "the SpdIsai preconditioner runs in FP64 only");
     int rc = ensure_workspace(op, mixed, s);
     if (rc) return rc;
     Workspace& w = *op->ws;
     const int64_t n = op->n;
     const auto t0 = std::chrono::steady_clock::now();
+
+    if (pc == PC_ISAI && !w.isai_ready) {
+        // the reference assembles the matrix to generate any preconditioner
+        // other than the identity, also when it then applies the operator
+        // matrix-free (minimize_strong_formulation_residual.hpp:1332-1353)
+        CsrData assembled{};
+        const CsrData* a = &op->csr;
+        if (op->kind == OP_MAGNETOSTATICS3D) {
+            if ((rc = csr_assemble_ms3d(op->ms, assembled, s))) return rc;
+            a = &assembled;
+        }
+        SIL_REQUIRE(op->kind != OP_POISSON2D, SIL_ERR_UNSUPPORTED,
+                    "SpdIsai needs an assembled operator or the magnetostatics operator");
+        rc = spd_isai_generate(*a, w.isai_g, w.isai_gt, s);
+        csr_free(assembled);
+        if (rc) return rc;
+        if (!w.isai_t && ((rc = alloc(&w.isai_t, n)) || (rc = alloc(&w.isai_z, n)))) return rc;
+        w.isai_ready = true;
+    }
 
     if (jacobi && !w.inv_diag) {
         if ((rc = alloc(&w.inv_diag, n))) return rc;
@@ -513,7 +566,9 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
     else
         hipLaunchKernelGGL((k_init<double>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r, w.p[0], invd, n,
                            part0, part1);
-    hipLaunchKernelGGL(k_scalars_init, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, part1, kUpdBlocks,
+    int nb_rz = kUpdBlocks;
+    if (pc == PC_ISAI && (rc = isai_apply(w, w.r, part1, &nb_rz, nullptr, s))) return rc;
+    hipLaunchKernelGGL(k_scalars_init, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, kUpdBlocks, part1, nb_rz,
                        st.relative_tolerance, st.max_iterations);
     SIL_LAUNCH_CHECK();
 
@@ -522,9 +577,9 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
     const int replace = st.replace_interval > 0 ? st.replace_interval : 200;
     hipGraphExec_t g = nullptr;
     if (mixed)
-        rc = get_graph<float>(op, 1, jacobi, chunk, w.r32, w.q32, w.p32, s, &g);
+        rc = get_graph<float>(op, 1, pc, chunk, w.r32, w.q32, w.p32, s, &g);
     else
-        rc = get_graph<double>(op, 0, jacobi, chunk, w.r, w.q, w.p, s, &g);
+        rc = get_graph<double>(op, 0, pc, chunk, w.r, w.q, w.p, s, &g);
     if (rc) return rc;
 
     if ((rc = read_scalars(w, s))) return rc;

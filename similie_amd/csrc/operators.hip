@@ -524,6 +524,14 @@ int p2_apply(const Poisson2dData& d, const T* wx, const T* wy, const T* in, T* o
 
 int64_t max_dot_blocks() { return kMaxDotBlocks; }
 
+int direction_pass64(const PUpdate<double>& pu, int64_t n, hipStream_t s)
+{
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 8192));
+    hipLaunchKernelGGL((k_direction<double>), dim3(g), dim3(kBlock), 0, s, pu, n);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
 int ms3d_apply64(const Ms3dData& d, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,
                  int* nb, hipStream_t s)
 {

@@ -5,7 +5,7 @@
 
 namespace sil {
 
-enum OperatorKind { OP_MAGNETOSTATICS3D = 1, OP_POISSON2D = 2 };
+enum OperatorKind { OP_MAGNETOSTATICS3D = 1, OP_POISSON2D = 2, OP_CSR = 3 };
 
 struct Workspace; // CG vectors, scalars and captured graphs (krylov.hip)
 
@@ -57,6 +57,35 @@ int ms3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const double* cell_mu, dou
 int ms3d_rhs(const Ms3dData& d, const double* cell_current, double* rhs, hipStream_t s);
 int ms3d_nu_from_mu(const double* mu, double* nu, float* nu32, int64_t n, hipStream_t s);
 
+// p_new = z + β p_old over n rows (the CG direction) as its own pass
+int direction_pass64(const PUpdate<double>& pu, int64_t n, hipStream_t s);
+
+// ------------------------------------------------- assembled (CSR) form ---
+
+// square CSR matrix, int32 row pointers and columns (ascending within a
+// row), FP64 values — gko::matrix::Csr<double, gko::int32>
+struct CsrData {
+    int64_t n = 0, nnz = 0;
+    int32_t* row_ptr = nullptr; // n + 1
+    int32_t* col = nullptr;
+    double* val = nullptr;
+    int lanes = 16;             // lanes per row in the sparse product
+};
+
+int csr_assemble_ms3d(const Ms3dData& d, CsrData& out, hipStream_t s);
+int csr_import(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* col, const double* val, CsrData& out,
+               hipStream_t s);
+void csr_free(CsrData& a);
+// y = A x; with w set, per-block ⟨w, y⟩ partials (nb_out = their count);
+// returns at once when *done is set
+int csr_spmv(const CsrData& a, const double* x, double* y, const double* w, double* partials, const int* done,
+             int* nb_out, hipStream_t s);
+int csr_apply64(const CsrData& a, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,
+                int* nb_out, hipStream_t s);
+int csr_diagonal(const CsrData& a, double* d, hipStream_t s);
+// SPD sparse approximate inverse M = Gᵀ G: G (pattern of tril(A)) and Gᵀ
+int spd_isai_generate(const CsrData& a, CsrData& g, CsrData& gt, hipStream_t s);
+
 int p2d_apply64(const Poisson2dData& d, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,
                 int* nblocks_out, hipStream_t s);
 int p2d_apply32(const Poisson2dData& d, const float* in, float* out, DotEpilogue dot, const PUpdate<float>* pu,
@@ -74,5 +103,6 @@ struct sil_operator_s {
     int64_t n;
     sil::Ms3dData ms;
     sil::Poisson2dData p2;
+    sil::CsrData csr;
     sil::Workspace* ws;
 };

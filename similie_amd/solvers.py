@@ -97,6 +97,66 @@ class MagnetostaticsOperator3D(LinearOperator):
         return b
 
 
+    def assemble(self, stream=None):
+        """assemble_matrix_data + csr_from_matrix_data
+        (magnetostatics_onelab.hpp:3115-3536,
+        minimize_strong_formulation_residual.hpp:565-646): the same operator
+        as a CSR matrix assembled on the device."""
+        h = ctypes.c_void_p()
+        L.check(L.lib().sil_magnetostatics3d_assemble(self.handle, ctypes.byref(h), L.stream_handle(stream)))
+        return AssembledOperator._adopt(h)
+
+
+class AssembledOperator(LinearOperator):
+    """gko::matrix::Csr<double, int32>: a square CSR matrix held on the
+    device (int32 row pointers and columns, ascending within a row, FP64
+    values); apply is the sparse product."""
+
+    @classmethod
+    def _adopt(cls, handle):
+        op = cls.__new__(cls)
+        op.handle = handle
+        op._destroy = L.lib().sil_operator_destroy
+        return op
+
+    @classmethod
+    def from_csr(cls, row_ptr, cols, vals, stream=None):
+        """Csr::create(exec, size, values, columns, row_ptrs)
+        (minimize_strong_formulation_residual.hpp:613-620) from int32 / int32
+        / float64 tensors (host or CUDA; copied)."""
+        rp = row_ptr.to(torch.int32).contiguous()
+        cc = cols.to(torch.int32).contiguous()
+        vv = vals.to(torch.float64).contiguous()
+        h = ctypes.c_void_p()
+        L.check(L.lib().sil_csr_create(ctypes.byref(h), rp.numel() - 1, cc.numel(), ctypes.c_void_p(rp.data_ptr()),
+                                       ctypes.c_void_p(cc.data_ptr()), ctypes.c_void_p(vv.data_ptr()),
+                                       L.stream_handle(stream)))
+        return cls._adopt(h)
+
+    def nnz(self):
+        return int(L.lib().sil_csr_nnz(self.handle))
+
+    def csr(self, stream=None):
+        """(row_ptr int32, cols int32, vals float64) CUDA tensors"""
+        n, nnz = self.size(), self.nnz()
+        rp = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+        cc = torch.empty(max(nnz, 1), dtype=torch.int32, device="cuda")
+        vv = torch.empty(max(nnz, 1), dtype=torch.float64, device="cuda")
+        L.check(L.lib().sil_csr_copy(self.handle, ctypes.c_void_p(rp.data_ptr()), ctypes.c_void_p(cc.data_ptr()),
+                                     ctypes.c_void_p(vv.data_ptr()), L.stream_handle(stream)))
+        return rp, cc[:nnz], vv[:nnz]
+
+
+def spd_isai(operator, stream=None):
+    """gko::preconditioner::SpdIsai::generate (minimize_strong_formulation_
+    residual.hpp:963-967): the factor G of M = Gᵀ G ≈ A⁻¹ (pattern of tril(A))
+    as an AssembledOperator; `operator` is assembled or the magnetostatics
+    operator."""
+    h = ctypes.c_void_p()
+    L.check(L.lib().sil_spd_isai_create(operator.handle, ctypes.byref(h), L.stream_handle(stream)))
+    return AssembledOperator._adopt(h)
+
+
 class Poisson2D(LinearOperator):
     """Dirichlet-zero d0ᵀ ⋆1^κ d0 on the interior of a rectilinear nx×ny grid
     (x slowest); vectors have nx·ny entries with zero boundary entries."""
@@ -122,7 +182,7 @@ class StrongFormulationSolverSettings:
 
     max_iterations: int = 2000
     relative_tolerance: float = 1.0e-12
-    preconditioner: str = "jacobi"  # "jacobi" | "identity"
+    preconditioner: str = "jacobi"  # "jacobi" | "identity" | "spd_isai"
     precision: str = "fp64"  # "fp64" | "mixed"
     check_interval: int = 0
     replace_interval: int = 0
@@ -132,7 +192,8 @@ class StrongFormulationSolverSettings:
         s.max_iterations = int(self.max_iterations)
         s.relative_tolerance = float(self.relative_tolerance)
         s.precision = {"fp64": L.PRECISION_FP64, "mixed": L.PRECISION_MIXED}[self.precision]
-        s.preconditioner = {"identity": L.PRECOND_IDENTITY, "jacobi": L.PRECOND_JACOBI}[self.preconditioner]
+        s.preconditioner = {"identity": L.PRECOND_IDENTITY, "jacobi": L.PRECOND_JACOBI,
+                            "spd_isai": L.PRECOND_SPD_ISAI}[self.preconditioner]
         s.check_interval = int(self.check_interval)
         s.replace_interval = int(self.replace_interval)
         return s

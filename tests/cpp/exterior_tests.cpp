@@ -268,6 +268,49 @@ TEST(Laplacian, 3D1Form)
     }
 }
 
+// minimize_strong_formulation_residual on the matrix-free operator and on its
+// assembled matrix (use_matrix_free = true / false) with the Jacobi and SpdIsai
+// preconditioners: every solve converges and all reach the same solution.
+TEST(Solvers, AssembledMatchesMatrixFree)
+{
+    const int nx = 9, ny = 8, nz = 10;
+    auto x = uniform(0., 1., nx), y = uniform(0., .8, ny), z = uniform(0., 1.2, nz);
+    const int nn = nx * ny * nz;
+    std::vector<double> mu(nn), rhs(3 * nn, 0.);
+    for (int q = 0; q < nn; ++q) mu[q] = 1. + 0.5 * std::sin(0.37 * q);
+    for (int k = 1; k < nz - 1; ++k)
+        for (int j = 1; j < ny - 1; ++j)
+            for (int i = 1; i < nx - 1; ++i)
+                for (int c = 0; c < 3; ++c) rhs[3 * (i + nx * (j + ny * k)) + c] = std::cos(0.1 * (i + 2 * j + 3 * k + c));
+    Dev dmu(mu), db(rhs), x0(3 * nn), x1(3 * nn), x2(3 * nn);
+    sil_operator op = nullptr;
+    if (sil_magnetostatics3d_create(&op, nx, ny, nz, x.data(), y.data(), z.data(), dmu.p, 20.0) != SIL_OK)
+        throw std::runtime_error(sil_last_error());
+    sil_operator m = solvers::build_matrix(nullptr, op);
+    solvers::StrongFormulationSolverSettings st;
+    st.relative_tolerance = 1e-11;
+    auto d0 = solvers::minimize_strong_formulation_residual(nullptr, op, db.p, x0.p, st);
+    auto d1 = solvers::minimize_strong_formulation_residual(nullptr, m, db.p, x1.p, st);
+    st.preconditioner = SIL_PRECOND_SPD_ISAI;
+    auto d2 = solvers::minimize_strong_formulation_residual(nullptr, m, db.p, x2.p, st);
+    EXPECT_EQ(d0.converged, 1);
+    EXPECT_EQ(d1.converged, 1);
+    EXPECT_EQ(d2.converged, 1);
+    EXPECT_NEAR(d1.iterations, d0.iterations, 2);
+    EXPECT_EQ(d2.iterations < d1.iterations, 1);
+    auto h0 = x0.host(), h1 = x1.host(), h2 = x2.host();
+    double scale = 0., e1 = 0., e2 = 0.;
+    for (size_t q = 0; q < h0.size(); ++q) {
+        scale = std::fmax(scale, std::fabs(h0[q]));
+        e1 = std::fmax(e1, std::fabs(h1[q] - h0[q]));
+        e2 = std::fmax(e2, std::fabs(h2[q] - h0[q]));
+    }
+    EXPECT_NEAR(e1 / scale, 0., 1e-8);
+    EXPECT_NEAR(e2 / scale, 0., 1e-8);
+    sil_operator_destroy(m);
+    sil_operator_destroy(op);
+}
+
 }  // namespace
 
 int main()

@@ -34,6 +34,8 @@ def lib():
         _lib.orc_cg.restype = ctypes.c_int
         _lib.orc_poisson2d_cg.restype = ctypes.c_int
         _lib.orc_ms_cg.restype = ctypes.c_int
+        _lib.orc_ms_assemble_csr.restype = ctypes.c_long
+        _lib.orc_spd_isai.restype = ctypes.c_long
     return _lib
 
 
@@ -250,6 +252,43 @@ def ms_cg(coords, mu, penalty, b, tol, max_it, diag=None):
                          ctypes.c_double(penalty), _d(bb), dg, ctypes.c_double(tol), ctypes.c_int(max_it), _d(x),
                          _d(res))
     return x, it, res[0], res[1]
+
+
+def ms_assemble_csr(coords, mu, penalty, y_sign=1.0):
+    """(row_ptr int64, cols int32, vals float64) of the assembled operator,
+    ascending columns per row."""
+    nx, ny, nz = (len(c) for c in coords)
+    xc, yc, zc = (_f64(c) for c in coords)
+    mu = _f64(mu)
+    n = 3 * nx * ny * nz
+    rp = np.zeros(n + 1, dtype=np.int64)
+    cols = np.zeros(17 * n, dtype=np.int32)
+    vals = np.zeros(17 * n)
+    nnz = lib().orc_ms_assemble_csr(ctypes.c_long(nx), ctypes.c_long(ny), ctypes.c_long(nz), _d(xc), _d(yc),
+                                    _d(zc), _d(mu), ctypes.c_double(penalty), ctypes.c_double(y_sign),
+                                    rp.ctypes.data_as(_lp), cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                    _d(vals))
+    if nnz < 0:
+        raise RuntimeError("row capacity exceeded")
+    return rp, cols[:nnz].copy(), vals[:nnz].copy()
+
+
+def spd_isai(row_ptr, cols, vals):
+    """factor G of the SPD sparse approximate inverse (M = Gᵀ G), CSR with
+    the pattern of tril(A)"""
+    n = len(row_ptr) - 1
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    cc = np.ascontiguousarray(cols, dtype=np.int32)
+    vv = _f64(vals)
+    grp = np.zeros(n + 1, dtype=np.int64)
+    gc = np.zeros(max(len(cc), 1), dtype=np.int32)
+    gv = np.zeros(max(len(cc), 1))
+    rc = lib().orc_spd_isai(ctypes.c_long(n), rp.ctypes.data_as(_lp), cc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                            _d(vv), grp.ctypes.data_as(_lp), gc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(gv))
+    if rc != 0:
+        raise ValueError(f"spd_isai failed with code {rc}")
+    nnz = int(grp[-1])
+    return grp, gc[:nnz].copy(), gv[:nnz].copy()
 
 
 def poisson2d_apply(hx, hy, kappa, u):
