@@ -3,11 +3,12 @@
  *
  * Restates the 3D magnetostatics operator of the reference's ONELAB
  * interface, its right-hand side and nodal permeability rules, its assembled
- * CSR form, the SPD sparse approximate inverse preconditioner of Ginkgo 1.8
- * (a dependency the reference does not vendor: the published FSAI algorithm
- * is restated), the 2D Dirichlet Poisson operators the solvers run on, and
- * textbook conjugate gradients.  Used only by tests/, __graft_entry__.smoke()
- * and bench.py's CPU-baseline leg.
+ * CSR form, the SPD sparse approximate inverse preconditioner of Ginkgo
+ * v1.11.0 (docker/similie_env/Dockerfile:70; a dependency the reference does
+ * not vendor, so the published FSAI algorithm is restated), the 2D Dirichlet
+ * Poisson operators the solvers run on, and textbook conjugate gradients.
+ * Used only by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg.
  */
 #include <math.h>
 #include <stdint.h>
@@ -315,7 +316,7 @@ long orc_ms_assemble_csr(long nx, long ny, long nz, const double* xc, const doub
     return nnz;
 }
 
-/* SpdIsai of Ginkgo 1.8 (gko::preconditioner::SpdIsai, sparsity power 1;
+/* SpdIsai of Ginkgo v1.11.0 (gko::preconditioner::SpdIsai, sparsity power 1;
  * selected at src/similie/solvers/minimize_strong_formulation_residual.hpp
  * :963-967), i.e. the factorized sparse approximate inverse of Kolotilina &
  * Yeremin (1993): for row i let J = the stored columns j ≤ i of row i

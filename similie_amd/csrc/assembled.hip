@@ -19,7 +19,7 @@
 //   optional per-block partial sum of ⟨w, y⟩.
 // * The Jacobi diagonal (gko::preconditioner::Jacobi, max block size 1, the
 //   reference's default preconditioner, :921-930).
-// * SpdIsai (:963-967): Ginkgo 1.8's factorized sparse approximate inverse on
+// * SpdIsai (:963-967): Ginkgo v1.11.0's factorized sparse approximate inverse on
 //   the pattern of tril(A) — for row i, J = stored columns ≤ i, solve
 //   A[J,J] y = e_|J| and scale by 1/√y_|J|; with A[J,J] = L Lᵀ this is
 //   G[i,J] = L⁻ᵀ e_|J|, one Cholesky factorisation and one back substitution
@@ -29,6 +29,8 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdio>
+#include <cstdlib>
 
 #include "operators.hpp"
 
@@ -38,8 +40,9 @@ namespace {
 
 constexpr int kAsmBlock = 128;
 constexpr int kSpmvBlock = 256;
-constexpr int kSpmvMaxBlocks = 4096;
+constexpr int kSpmvMaxBlocks = 8192;
 constexpr int kFsaiMaxRow = 32;
+constexpr int kXcds = 8; // MI355X: 8 XCDs, each with its own L2
 
 // ----------------------------------------------------------- stencil table ---
 
@@ -106,7 +109,7 @@ constexpr RowPlan make_plan(int C)
 template <int C>
 struct Plan {
     static constexpr RowPlan v = make_plan(C);
-    static_assert(v.npairs == 4 && v.nslots == 13, "curl-curl row pattern");
+    static_assert(v.npairs == 4 && v.nslots == 13, "curl-curl row pattern (13 entries at most)");
 };
 
 struct MsAsm {
@@ -122,11 +125,11 @@ __device__ __forceinline__ bool interior(const MsAsm& a, int64_t i, int64_t j, i
 }
 
 // one interior row (component C at node (i,j,k)): COUNT writes the number of
-// stored entries, otherwise the row is written at rp[r]
+// stored entries, otherwise the row is written at col/val[rp[r] − base]
 template <int C, bool COUNT>
 __device__ __forceinline__ void assemble_row(const MsAsm& a, int64_t i, int64_t j, int64_t k, int64_t* __restrict__ cnt,
                                              const int32_t* __restrict__ rp, int32_t* __restrict__ col,
-                                             double* __restrict__ val)
+                                             double* __restrict__ val, int32_t base)
 {
 #pragma clang fp contract(off)
     constexpr RowPlan P = Plan<C>::v;
@@ -168,7 +171,7 @@ __device__ __forceinline__ void assemble_row(const MsAsm& a, int64_t i, int64_t 
     if constexpr (COUNT) {
         cnt[r] = __popc(present);
     } else {
-        int32_t out = rp[r];
+        int32_t out = rp[r] - base;
 #pragma unroll
         for (int s = 0; s < P.nslots; ++s) {
             if (!((present >> s) & 1u)) continue;
@@ -181,30 +184,58 @@ __device__ __forceinline__ void assemble_row(const MsAsm& a, int64_t i, int64_t 
     }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(kAsmBlock) void k_ms3d_assemble(MsAsm a, int64_t* __restrict__ cnt,
-                                                             const int32_t* __restrict__ rp, int32_t* __restrict__ col,
-                                                             double* __restrict__ val)
+// count pass: entries per row
+__global__ __launch_bounds__(kAsmBlock) void k_ms3d_count(MsAsm a, int64_t* __restrict__ cnt)
 {
     const int64_t nn = a.nx * a.ny * a.nz;
     for (int64_t node = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; node < nn;
          node += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = node % a.nx, j = (node / a.nx) % a.ny, k = node / (a.nx * a.ny);
         if (!interior(a, i, j, k)) {
-            for (int c = 0; c < 3; ++c) {
-                const int64_t r = 3 * node + c;
-                if constexpr (COUNT) {
-                    cnt[r] = 1;
-                } else {
-                    col[rp[r]] = static_cast<int32_t>(r);
-                    val[rp[r]] = 1.0;
-                }
-            }
+            for (int c = 0; c < 3; ++c) cnt[3 * node + c] = 1;
             continue;
         }
-        assemble_row<0, COUNT>(a, i, j, k, cnt, rp, col, val);
-        assemble_row<1, COUNT>(a, i, j, k, cnt, rp, col, val);
-        assemble_row<2, COUNT>(a, i, j, k, cnt, rp, col, val);
+        assemble_row<0, true>(a, i, j, k, cnt, nullptr, nullptr, nullptr, 0);
+        assemble_row<1, true>(a, i, j, k, cnt, nullptr, nullptr, nullptr, 0);
+        assemble_row<2, true>(a, i, j, k, cnt, nullptr, nullptr, nullptr, 0);
+    }
+}
+
+// fill pass: a workgroup assembles the rows of kAsmBlock consecutive nodes —
+// one contiguous stretch of the CSR arrays — into LDS, then writes the
+// stretch with consecutive lanes on consecutive entries (the per-thread rows
+// written directly are scattered 4- and 8-byte stores: partial-line writes)
+constexpr int kAsmRowMax = 13;
+__global__ __launch_bounds__(kAsmBlock) void k_ms3d_fill(MsAsm a, const int32_t* __restrict__ rp,
+                                                         int32_t* __restrict__ col, double* __restrict__ val)
+{
+    __shared__ int32_t s_col[kAsmBlock * 3 * kAsmRowMax];
+    __shared__ double s_val[kAsmBlock * 3 * kAsmRowMax];
+    const int64_t nn = a.nx * a.ny * a.nz;
+    for (int64_t node0 = (int64_t)blockIdx.x * kAsmBlock; node0 < nn; node0 += (int64_t)gridDim.x * kAsmBlock) {
+        const int64_t node_end = node0 + kAsmBlock < nn ? node0 + kAsmBlock : nn;
+        const int32_t beg = rp[3 * node0], len = rp[3 * node_end] - beg;
+        const int64_t node = node0 + threadIdx.x;
+        if (node < node_end) {
+            const int64_t i = node % a.nx, j = (node / a.nx) % a.ny, k = node / (a.nx * a.ny);
+            if (!interior(a, i, j, k)) {
+                for (int c = 0; c < 3; ++c) {
+                    const int64_t r = 3 * node + c;
+                    s_col[rp[r] - beg] = static_cast<int32_t>(r);
+                    s_val[rp[r] - beg] = 1.0;
+                }
+            } else {
+                assemble_row<0, false>(a, i, j, k, nullptr, rp, s_col, s_val, beg);
+                assemble_row<1, false>(a, i, j, k, nullptr, rp, s_col, s_val, beg);
+                assemble_row<2, false>(a, i, j, k, nullptr, rp, s_col, s_val, beg);
+            }
+        }
+        __syncthreads();
+        for (int32_t q = threadIdx.x; q < len; q += kAsmBlock) {
+            col[beg + q] = s_col[q];
+            val[beg + q] = s_val[q];
+        }
+        __syncthreads();
     }
 }
 
@@ -282,28 +313,65 @@ struct CsrView {
 
 inline CsrView view(const CsrData& a) { return CsrView{a.n, a.row_ptr, a.col, a.val}; }
 
-// y = A x; with `w` set, per-block partial sums of ⟨w, y⟩ go to partials[blockIdx]
-template <int L>
+// y = A x with L lanes per row; with `w` set, per-block partial sums of
+// ⟨w, y⟩ go to partials[blockIdx].  A workgroup takes R groups of 256/L rows
+// per step; each lane first issues the K column and value loads of its share
+// of each of its R rows (independent of one another, streamed past the
+// caches: the matrix is read once per product), then the R·K gathers of x,
+// so a wave keeps 2·R·K loads per lane in flight instead of one row-pointer →
+// column → x chain; entries beyond K·L take a plain loop.
+template <int L, int K, int R>
 __global__ __launch_bounds__(kSpmvBlock) void k_csr_spmv(CsrView A, const double* __restrict__ x,
                                                          double* __restrict__ y, const double* __restrict__ w,
                                                          double* __restrict__ partials, const int* __restrict__ done)
 {
     if (done && *done) return;
     constexpr int kRows = kSpmvBlock / L;
+    constexpr int kStep = kRows * R;
     const int lane = threadIdx.x % L;
+    // workgroups are dealt to the XCDs round-robin: XCD x takes the x-th
+    // eighth of the rows, so the x planes one L2 serves stay few
+    const int xcd = blockIdx.x % kXcds;
+    const int64_t local = blockIdx.x / kXcds, per = gridDim.x / kXcds;
+    const int64_t span = ((A.n + kXcds - 1) / kXcds + kStep - 1) / kStep * kStep;
+    const int64_t lo = xcd * span, hi = lo + span < A.n ? lo + span : A.n;
     double dot = 0.0;
-    for (int64_t base = (int64_t)blockIdx.x * kRows; base < A.n; base += (int64_t)gridDim.x * kRows) {
-        const int64_t row = base + threadIdx.x / L;
-        double v = 0.0;
-        if (row < A.n) {
-            const int32_t e = A.rp[row + 1];
-            for (int32_t q = A.rp[row] + lane; q < e; q += L) v += A.val[q] * x[A.col[q]];
+    for (int64_t base = lo + local * kStep; base < hi; base += per * kStep) {
+        int64_t row[R];
+        int32_t b[R], e[R];
+#pragma unroll
+        for (int g = 0; g < R; ++g) {
+            row[g] = base + g * kRows + threadIdx.x / L;
+            b[g] = row[g] < hi ? A.rp[row[g]] : 0;
+            e[g] = row[g] < hi ? A.rp[row[g] + 1] : 0;
+        }
+        int32_t c[R][K];
+        double a[R][K];
+#pragma unroll
+        for (int g = 0; g < R; ++g)
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int32_t q = b[g] + lane + k * L;
+                c[g][k] = q < e[g] ? __builtin_nontemporal_load(&A.col[q]) : -1;
+                a[g][k] = q < e[g] ? __builtin_nontemporal_load(&A.val[q]) : 0.0;
+            }
+        double v[R];
+#pragma unroll
+        for (int g = 0; g < R; ++g) {
+            v[g] = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (c[g][k] >= 0) v[g] += a[g][k] * x[c[g][k]];
+            for (int32_t q = b[g] + lane + K * L; q < e[g]; q += L) v[g] += A.val[q] * x[A.col[q]];
         }
 #pragma unroll
-        for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, L);
-        if (lane == 0 && row < A.n) {
-            y[row] = v;
-            if (w) dot += w[row] * v;
+        for (int g = 0; g < R; ++g) {
+#pragma unroll
+            for (int off = L / 2; off > 0; off >>= 1) v[g] += __shfl_xor(v[g], off, L);
+            if (lane == 0 && row[g] < hi) {
+                y[row[g]] = v[g];
+                if (w) dot += w[row[g]] * v[g];
+            }
         }
     }
     if (partials) {
@@ -488,10 +556,28 @@ int alloc_csr(CsrData& a, int64_t n, int64_t nnz)
     return SIL_OK;
 }
 
+// lanes per row and loads per lane of the sparse product from the mean row
+// length; SIL_CSR_SPMV="lanes,loads,row groups[,max workgroups]" overrides
+// (tuning)
 void choose_lanes(CsrData& a)
 {
     const double mean = a.n > 0 ? static_cast<double>(a.nnz) / static_cast<double>(a.n) : 1.0;
-    a.lanes = mean > 10.0 ? 16 : (mean > 5.0 ? 8 : (mean > 2.5 ? 4 : 2));
+    // measured on the 192³ magnetostatics matrix (12.6 entries per row: 8
+    // lanes × 2 loads) and its SpdIsai factor (6.8 per row: 4 lanes × 2)
+    a.lanes = mean > 10.0 ? 8 : (mean > 5.0 ? 4 : (mean > 2.5 ? 4 : 2));
+    a.unroll = mean > 5.0 ? 2 : 1;
+    a.groups = 1;
+    a.max_blocks = kSpmvMaxBlocks;
+    if (const char* e = std::getenv("SIL_CSR_SPMV")) {
+        int l = 0, k = 0, r = 0, b = 0;
+        const int got = std::sscanf(e, "%d,%d,%d,%d", &l, &k, &r, &b);
+        if (got >= 3) {
+            a.lanes = l;
+            a.unroll = k;
+            a.groups = r;
+        }
+        if (got == 4 && b > 0) a.max_blocks = b;
+    }
 }
 
 int read_flag(const int* dflag, int* out, hipStream_t s)
@@ -521,7 +607,7 @@ int csr_assemble_ms3d(const Ms3dData& d, CsrData& out, hipStream_t s)
     CsrData m{};
     int rc = SIL_OK;
     const unsigned g = grid_for(nn, kAsmBlock);
-    hipLaunchKernelGGL((k_ms3d_assemble<true>), dim3(g), dim3(kAsmBlock), 0, s, a, cnt, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_ms3d_count, dim3(g), dim3(kAsmBlock), 0, s, a, cnt);
     if (hipGetLastError() != hipSuccess) {
         set_error("assemble: count kernel launch failed");
         rc = SIL_ERR_HIP;
@@ -543,8 +629,7 @@ int csr_assemble_ms3d(const Ms3dData& d, CsrData& out, hipStream_t s)
         }
     }
     if (rc == SIL_OK) {
-        hipLaunchKernelGGL((k_ms3d_assemble<false>), dim3(g), dim3(kAsmBlock), 0, s, a, nullptr, m.row_ptr, m.col,
-                           m.val);
+        hipLaunchKernelGGL(k_ms3d_fill, dim3(g), dim3(kAsmBlock), 0, s, a, m.row_ptr, m.col, m.val);
         if (hipGetLastError() != hipSuccess) {
             set_error("assemble: fill kernel launch failed");
             rc = SIL_ERR_HIP;
@@ -603,18 +688,26 @@ int csr_import(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* col, co
 int csr_spmv(const CsrData& a, const double* x, double* y, const double* w, double* partials, const int* done,
              int* nb_out, hipStream_t s)
 {
-    const int rows = kSpmvBlock / a.lanes;
-    const unsigned g = grid_for(a.n, rows, kSpmvMaxBlocks);
+    const int rows = kSpmvBlock / a.lanes * a.groups;
+    // a multiple of the XCD count (the kernel splits the rows per XCD)
+    const int64_t cap = std::max<int64_t>(kXcds, std::min<int64_t>(a.max_blocks, max_dot_blocks()) / kXcds * kXcds);
+    const unsigned g = static_cast<unsigned>(
+        std::min<int64_t>((static_cast<int64_t>(grid_for(a.n, rows, cap)) + kXcds - 1) / kXcds * kXcds, cap));
     const CsrView v = view(a);
-    switch (a.lanes) {
-    case 2: hipLaunchKernelGGL((k_csr_spmv<2>), dim3(g), dim3(kSpmvBlock), 0, s, v, x, y, w, partials, done); break;
-    case 4: hipLaunchKernelGGL((k_csr_spmv<4>), dim3(g), dim3(kSpmvBlock), 0, s, v, x, y, w, partials, done); break;
-    case 8: hipLaunchKernelGGL((k_csr_spmv<8>), dim3(g), dim3(kSpmvBlock), 0, s, v, x, y, w, partials, done); break;
-    default: hipLaunchKernelGGL((k_csr_spmv<16>), dim3(g), dim3(kSpmvBlock), 0, s, v, x, y, w, partials, done); break;
+#define SIL_SPMV(L, K, R)                                                                                        \
+    if (a.lanes == L && a.unroll == K && a.groups == R) {                                                        \
+        hipLaunchKernelGGL((k_csr_spmv<L, K, R>), dim3(g), dim3(kSpmvBlock), 0, s, v, x, y, w, partials, done); \
+        SIL_LAUNCH_CHECK();                                                                                      \
+        if (nb_out) *nb_out = static_cast<int>(g);                                                               \
+        return SIL_OK;                                                                                           \
     }
-    SIL_LAUNCH_CHECK();
-    if (nb_out) *nb_out = static_cast<int>(g);
-    return SIL_OK;
+    SIL_SPMV(2, 1, 1) SIL_SPMV(2, 2, 1) SIL_SPMV(2, 4, 1)
+    SIL_SPMV(4, 1, 1) SIL_SPMV(4, 2, 1) SIL_SPMV(4, 2, 2) SIL_SPMV(4, 2, 4) SIL_SPMV(4, 4, 1) SIL_SPMV(4, 4, 2)
+    SIL_SPMV(8, 1, 1) SIL_SPMV(8, 1, 2) SIL_SPMV(8, 2, 1) SIL_SPMV(8, 2, 2) SIL_SPMV(8, 2, 4)
+    SIL_SPMV(16, 1, 1) SIL_SPMV(16, 1, 2) SIL_SPMV(16, 1, 4)
+#undef SIL_SPMV
+    set_error("sparse product: no kernel for %d lanes x %d loads x %d row groups", a.lanes, a.unroll, a.groups);
+    return SIL_ERR_UNSUPPORTED;
 }
 
 int csr_apply64(const CsrData& a, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,

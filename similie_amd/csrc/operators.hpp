@@ -70,6 +70,9 @@ struct CsrData {
     int32_t* col = nullptr;
     double* val = nullptr;
     int lanes = 16;             // lanes per row in the sparse product
+    int unroll = 1;             // loads in flight per lane and row
+    int groups = 1;             // rows per lane and step
+    int max_blocks = 8192;      // workgroups of the sparse product
 };
 
 int csr_assemble_ms3d(const Ms3dData& d, CsrData& out, hipStream_t s);
