@@ -154,6 +154,32 @@ int sil_magnetostatics3d_create(sil_operator* out, int64_t nx, int64_t ny, int64
                                 const double* y_coords, const double* z_coords, const double* mu_nodal,
                                 double gauge_penalty);
 
+/* MagnetostaticsOperator3D with MaterialMagnetostaticsHamiltonian
+ * (magnetostatics_onelab.hpp:957-1011, 5108-5121): nodes whose
+ * ferromagnetic fraction (ferromagnetic_nodal, device, the mean over the
+ * adjacent cells of the nonlinear-material flag, :5026-5063) exceeds 0.5
+ * follow NonlinearMagnetostaticsHamiltonian with the InterpolatedNonlinear-
+ * BHCurve of (bh_b, bh_h) (host arrays, 2..64 samples, bh_b increasing;
+ * src/similie/physics/magnetostatics/nonlinear_magnetostatics.py:14-227), the
+ * others are linear with mu_nodal.  sil_operator_apply gives F(x) and keeps
+ * x's material state; sil_minimize_strong_formulation_residual runs the
+ * reference's Newton branch (minimize_strong_formulation_residual.hpp
+ * :1383-1569; FP64, identity or Jacobi).  Needs >= 3 nodes per axis. */
+int sil_magnetostatics3d_nonlinear_create(sil_operator* out, int64_t nx, int64_t ny, int64_t nz,
+                                          const double* x_coords, const double* y_coords, const double* z_coords,
+                                          const double* mu_nodal, const double* ferromagnetic_nodal,
+                                          const double* bh_b, const double* bh_h, int32_t nsamples,
+                                          double gauge_penalty);
+
+/* apply_jacobian (magnetostatics_onelab.hpp:2918-3113): out = J(state)·in,
+ * the derivative of the nonlinear operator at the potential `state`. */
+int sil_magnetostatics3d_apply_jacobian(sil_operator op, const double* state, const double* in, double* out,
+                                        void* stream);
+
+/* the diagonal of J(state) (the Jacobi preconditioner of each Newton step,
+ * which the reference takes from the assembled Jacobian, :1434-1438) */
+int sil_magnetostatics3d_jacobian_diagonal(sil_operator op, const double* state, double* diag, void* stream);
+
 /* Nodal permeability = mean over adjacent cells (magnetostatics_onelab.hpp
  * :5029-5066); cell_mu is [(nx-1)(ny-1)(nz-1)] with cells ordered
  * ci + (nx-1)(cj + (ny-1)ck). */
@@ -230,6 +256,7 @@ typedef struct sil_solver_diagnostics {
     double final_relative_residual;
     double true_relative_residual;  /* FP64 ‖b − A x‖2 / ‖b‖2 recomputed at exit */
     double duration;                /* seconds, host wall clock around the solve */
+    int32_t outer_iterations;       /* Newton steps of a nonlinear operator (0 for a linear one) */
 } sil_solver_diagnostics;
 
 void sil_default_solver_settings(sil_solver_settings* settings);

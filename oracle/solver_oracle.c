@@ -526,3 +526,344 @@ int orc_ms_cg(long nx, long ny, long nz, const double* xc, const double* yc, con
     ms_ctx c = {nx, ny, nz, xc, yc, zc, mu, penalty};
     return orc_cg(3 * nx * ny * nz, ms_mv, &c, b, diag, tol, max_it, x, res_out);
 }
+
+/* ------------------------------------------------------------------ */
+/* nonlinear magnetostatics                                            */
+/* ------------------------------------------------------------------ */
+
+#define ORC_BH_MAX 64 /* InterpolatedNonlinearBHCurve<64> (magnetostatics_onelab.hpp:4550) */
+
+/* InterpolatedNonlinearBHCurve (src/similie/physics/magnetostatics/
+ * nonlinear_magnetostatics.py:14-181): h(b) piecewise linear in b between
+ * the samples, bracketed through q = b²; ν = h/b per sample with ν₀ = ν₁. */
+typedef struct {
+    int n;
+    double b[ORC_BH_MAX], h[ORC_BH_MAX], q[ORC_BH_MAX], nu[ORC_BH_MAX];
+    double dnu_dq[ORC_BH_MAX - 1], dh_db[ORC_BH_MAX - 1];
+} orc_bh;
+
+static void bh_init(orc_bh* c, const double* b, const double* h, int n)
+{
+    c->n = n;
+    for (int i = 0; i < n; ++i) {
+        c->b[i] = b[i];
+        c->h[i] = h[i];
+        c->q[i] = b[i] * b[i];
+    }
+    c->nu[0] = c->h[1] / c->b[1];
+    for (int i = 1; i < n; ++i) c->nu[i] = c->h[i] / c->b[i];
+    for (int i = 0; i + 1 < n; ++i) {
+        c->dnu_dq[i] = (c->nu[i + 1] - c->nu[i]) / (c->q[i + 1] - c->q[i]);
+        c->dh_db[i] = (c->h[i + 1] - c->h[i]) / (c->b[i + 1] - c->b[i]);
+    }
+}
+
+static int bh_bracket_q(const orc_bh* c, double q)
+{
+    if (q <= c->q[0]) return 0;
+    for (int i = 0; i + 1 < c->n; ++i)
+        if (q <= c->q[i + 1]) return i;
+    return c->n - 2;
+}
+
+static double bh_h_from_b(const orc_bh* c, double b)
+{
+    if (b <= c->b[0]) return c->h[0];
+    if (b >= c->b[c->n - 1]) {
+        int i = c->n - 2;
+        return c->h[i] + c->dh_db[i] * (b - c->b[i]);
+    }
+    int i = bh_bracket_q(c, b * b);
+    return c->h[i] + c->dh_db[i] * (b - c->b[i]);
+}
+
+static double bh_dh_db_from_b(const orc_bh* c, double b)
+{
+    if (b <= c->b[0]) return c->dh_db[0];
+    if (b >= c->b[c->n - 1]) return c->dh_db[c->n - 2];
+    return c->dh_db[bh_bracket_q(c, b * b)];
+}
+
+static double bh_h_over_b_from_b(const orc_bh* c, double b)
+{
+    if (b == 0.0) return bh_dh_db_from_b(c, 0.0);
+    return bh_h_from_b(c, b) / b;
+}
+
+static double bh_d_h_over_b_db_from_b(const orc_bh* c, double b)
+{
+    if (b == 0.0) return 0.0;
+    return (bh_dh_db_from_b(c, b) * b - bh_h_from_b(c, b)) / (b * b);
+}
+
+static double bh_d2_h_over_b_db2_from_b(const orc_bh* c, double b)
+{
+    if (b == 0.0) return 0.0;
+    double intercept = bh_h_from_b(c, b) - bh_dh_db_from_b(c, b) * b;
+    return 2.0 * intercept / (b * b * b);
+}
+
+static double bh_f(const orc_bh* c, double q) { return bh_h_over_b_from_b(c, sqrt(q)); }
+
+static double bh_f1(const orc_bh* c, double q)
+{
+    if (q == 0.0) return 0.0;
+    double b = sqrt(q);
+    return bh_d_h_over_b_db_from_b(c, b) / (2.0 * b);
+}
+
+static double bh_f2(const orc_bh* c, double q)
+{
+    if (q == 0.0) return 0.0;
+    double b = sqrt(q);
+    return bh_d2_h_over_b_db2_from_b(c, b) / (4.0 * q) - bh_d_h_over_b_db_from_b(c, b) / (4.0 * q * b);
+}
+
+/* one B-H curve function at `arg` (test hook): 0 h_from_b, 1 dh_db_from_b,
+ * 2 h_over_b_from_q, 3 d_h_over_b_dq_from_q, 4 d2_h_over_b_dq2_from_q,
+ * 5 nu at sample `arg` */
+double orc_bh_eval(const double* b, const double* h, int n, int which, double arg)
+{
+    orc_bh c;
+    bh_init(&c, b, h, n);
+    switch (which) {
+    case 0: return bh_h_from_b(&c, arg);
+    case 1: return bh_dh_db_from_b(&c, arg);
+    case 2: return bh_f(&c, arg);
+    case 3: return bh_f1(&c, arg);
+    case 4: return bh_f2(&c, arg);
+    default: return c.nu[(int)arg];
+    }
+}
+
+/* MaterialMagnetostaticsHamiltonian (magnetostatics_onelab.hpp:957-1011):
+ * a node whose ferromagnetic fraction exceeds 0.5 follows the generated
+ * NonlinearMagnetostaticsHamiltonian — H = ½ q f(q), q = |B|², f = h/b of the
+ * curve (nonlinear_magnetostatics.py:184-227), so ∂H/∂B_m = B_m (f + q f′)
+ * and ∂²H/∂B_m∂B_c = δ_mc (f + q f′) + 2 B_m B_c (2 f′ + q f″) (the
+ * derivatives generate_cpp_hamiltonian.py:306-397 emits); other nodes are
+ * linear: B_m / μ and δ_mc / μ. */
+static void nl_response(const orc_bh* c, int ferro, double mu, const double B[3], double H[3], double J[9])
+{
+    if (ferro) {
+        double q = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+        double f = bh_f(c, q), f1 = bh_f1(c, q), f2 = bh_f2(c, q);
+        double a = f + q * f1, cc = 2.0 * (2.0 * f1 + q * f2);
+        for (int m = 0; m < 3; ++m) {
+            if (H) H[m] = B[m] * a;
+            if (J)
+                for (int n = 0; n < 3; ++n) J[3 * m + n] = (m == n ? a : 0.0) + cc * B[m] * B[n];
+        }
+    } else {
+        for (int m = 0; m < 3; ++m) {
+            if (H) H[m] = B[m] / mu;
+            if (J)
+                for (int n = 0; n < 3; ++n) J[3 * m + n] = m == n ? 1.0 / mu : 0.0;
+        }
+    }
+}
+
+static void moments_at(const ms_grid* g, long i, long j, long k, const double* x, double B[3])
+{
+    for (int m = 0; m < 3; ++m) {
+        double v = 0.0;
+        for (int t = 0; t < 4; ++t) {
+            const ms_tap* tp = &MOMENT[m][t];
+            long pi = i + tp->di, pj = j + tp->dj, pk = k + tp->dk;
+            if (is_boundary(g, pi, pj, pk)) continue;
+            v += tp->coeff * x[dof(g, pi, pj, pk, tp->comp)];
+        }
+        B[m] = v;
+    }
+}
+
+/* the operator of apply_with_precomputed_stencils (magnetostatics_onelab.hpp
+ * :2241-2396) with the material Hamiltonian (state = NULL: out = F(in)) or
+ * apply_jacobian (:2918-3113; out = J(state)·in); boundary rows identity,
+ * interior rows add the gauge term P·w^A·in */
+static void nl_operator(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
+                        const double* mu, const double* ferro, const orc_bh* c, double penalty, const double* state,
+                        const double* in, double* out)
+{
+    ms_grid g = {nx, ny, nz, xc, yc, zc};
+    long ndof = 3 * nx * ny * nz;
+    for (long r = 0; r < ndof; ++r) out[r] = 0.0;
+    for (long k = 1; k + 1 < nz; ++k)
+        for (long j = 1; j + 1 < ny; ++j)
+            for (long i = 1; i + 1 < nx; ++i) {
+                long s = i + nx * (j + ny * k);
+                double dx = local_spacing(xc, nx, i), dy = local_spacing(yc, ny, j), dz = local_spacing(zc, nz, k);
+                double wb[3] = {dx / (dy * dz), dy / (dx * dz), dz / (dx * dy)};
+                int fe = ferro[s] > 0.5;
+                double Bin[3], H[3];
+                moments_at(&g, i, j, k, in, Bin);
+                if (state) {
+                    double B0[3], J[9];
+                    moments_at(&g, i, j, k, state, B0);
+                    nl_response(c, fe, mu[s], B0, NULL, J);
+                    for (int m = 0; m < 3; ++m) H[m] = J[3 * m] * Bin[0] + J[3 * m + 1] * Bin[1] + J[3 * m + 2] * Bin[2];
+                } else {
+                    nl_response(c, fe, mu[s], Bin, H, NULL);
+                }
+                for (int m = 0; m < 3; ++m) {
+                    double hm = wb[m] * H[m];
+                    for (int t = 0; t < 4; ++t) {
+                        const ms_tap* tp = &MOMENT[m][t];
+                        long pi = i + tp->di, pj = j + tp->dj, pk = k + tp->dk;
+                        if (is_boundary(&g, pi, pj, pk)) continue;
+                        out[dof(&g, pi, pj, pk, tp->comp)] += tp->coeff * hm;
+                    }
+                }
+            }
+    for (long k = 0; k < nz; ++k)
+        for (long j = 0; j < ny; ++j)
+            for (long i = 0; i < nx; ++i) {
+                if (is_boundary(&g, i, j, k)) {
+                    for (int cc = 0; cc < 3; ++cc) out[dof(&g, i, j, k, cc)] = in[dof(&g, i, j, k, cc)];
+                    continue;
+                }
+                double dx = local_spacing(xc, nx, i), dy = local_spacing(yc, ny, j), dz = local_spacing(zc, nz, k);
+                double wa[3] = {dy * dz / dx, dx * dz / dy, dx * dy / dz};
+                for (int cc = 0; cc < 3; ++cc) out[dof(&g, i, j, k, cc)] += penalty * wa[cc] * in[dof(&g, i, j, k, cc)];
+            }
+}
+
+void orc_nl_apply(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc, const double* mu,
+                  const double* ferro, const double* bh_b, const double* bh_h, int nbh, double penalty,
+                  const double* in, double* out)
+{
+    orc_bh c;
+    bh_init(&c, bh_b, bh_h, nbh);
+    nl_operator(nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, NULL, in, out);
+}
+
+void orc_nl_jacobian_apply(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
+                           const double* mu, const double* ferro, const double* bh_b, const double* bh_h, int nbh,
+                           double penalty, const double* state, const double* in, double* out)
+{
+    orc_bh c;
+    bh_init(&c, bh_b, bh_h, nbh);
+    nl_operator(nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, state, in, out);
+}
+
+/* Jacobi diagonal of J(state), probed column by column in 27 colour
+ * classes (the same probing as orc_ms_diagonal) */
+void orc_nl_jacobian_diagonal(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
+                              const double* mu, const double* ferro, const double* bh_b, const double* bh_h, int nbh,
+                              double penalty, const double* state, double* diag)
+{
+    orc_bh c;
+    bh_init(&c, bh_b, bh_h, nbh);
+    long ndof = 3 * nx * ny * nz;
+    double* e = (double*)calloc(ndof, sizeof(double));
+    double* o = (double*)malloc(sizeof(double) * ndof);
+    for (int ci = 0; ci < 3; ++ci)
+        for (int cj = 0; cj < 3; ++cj)
+            for (int ck = 0; ck < 3; ++ck)
+                for (int cc = 0; cc < 3; ++cc) {
+                    for (long r = 0; r < ndof; ++r) e[r] = 0.0;
+                    for (long k = ck; k < nz; k += 3)
+                        for (long j = cj; j < ny; j += 3)
+                            for (long i = ci; i < nx; i += 3) e[3 * (i + nx * (j + ny * k)) + cc] = 1.0;
+                    nl_operator(nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, state, e, o);
+                    for (long k = ck; k < nz; k += 3)
+                        for (long j = cj; j < ny; j += 3)
+                            for (long i = ci; i < nx; i += 3) {
+                                long r = 3 * (i + nx * (j + ny * k)) + cc;
+                                diag[r] = o[r];
+                            }
+                }
+    free(e);
+    free(o);
+}
+
+typedef struct {
+    long nx, ny, nz;
+    const double *x, *y, *z, *mu, *ferro;
+    const orc_bh* c;
+    double penalty;
+    const double* state;
+} nl_ctx;
+
+static void nl_jmv(void* p, const double* in, double* out)
+{
+    nl_ctx* k = (nl_ctx*)p;
+    nl_operator(k->nx, k->ny, k->nz, k->x, k->y, k->z, k->mu, k->ferro, k->c, k->penalty, k->state, in, out);
+}
+
+/* the nonlinear branch of minimize_strong_formulation_residual
+ * (src/similie/solvers/minimize_strong_formulation_residual.hpp:1383-1569):
+ * x₀ = 0, r = b − F(x); at most 30 Newton steps while ‖r‖/‖r₀‖ > tol; each
+ * solves J(x) δ = r by CG to the same relative tolerance (Jacobi from the
+ * Jacobian's diagonal when use_jacobi), then a backtracking line search
+ * (α = 1, halved up to 8 times until ‖r(x + αδ)‖ < ‖r(x)‖, keeping the best
+ * α tried).  out[0] = Newton steps, out[1] = Σ inner CG iterations,
+ * out[2] = ‖r₀‖, out[3] = final ‖r‖; returns 1 when converged. */
+int orc_nl_newton(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc, const double* mu,
+                  const double* ferro, const double* bh_b, const double* bh_h, int nbh, double penalty,
+                  const double* b, double tol, int max_inner, int use_jacobi, double* x, double* out)
+{
+    orc_bh c;
+    bh_init(&c, bh_b, bh_h, nbh);
+    long n = 3 * nx * ny * nz;
+    double* r = (double*)malloc(sizeof(double) * n);
+    double* fx = (double*)malloc(sizeof(double) * n);
+    double* delta = (double*)malloc(sizeof(double) * n);
+    double* xc_ = (double*)malloc(sizeof(double) * n);
+    double* rc = (double*)malloc(sizeof(double) * n);
+    double* diag = use_jacobi ? (double*)malloc(sizeof(double) * n) : NULL;
+    for (long i = 0; i < n; ++i) x[i] = 0.0;
+    nl_operator(nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, NULL, x, fx);
+    for (long i = 0; i < n; ++i) r[i] = b[i] - fx[i];
+    double r0 = sqrt(dot(n, r, r)), rn = r0;
+    int steps = 0, inner = 0, converged = 0;
+    if (r0 == 0.0) converged = 1;
+    for (int it = 0; it < 30 && !converged; ++it) {
+        rn = sqrt(dot(n, r, r));
+        if (rn / r0 <= tol) {
+            converged = 1;
+            break;
+        }
+        nl_ctx ctx = {nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, x};
+        if (diag)
+            orc_nl_jacobian_diagonal(nx, ny, nz, xc, yc, zc, mu, ferro, bh_b, bh_h, nbh, penalty, x, diag);
+        double res[2];
+        inner += orc_cg(n, nl_jmv, &ctx, r, diag, tol, max_inner, delta, res);
+        double alpha = 1.0, best_alpha = 0.0, best = INFINITY;
+        for (int ls = 0; ls < 8; ++ls) {
+            for (long i = 0; i < n; ++i) xc_[i] = x[i] + alpha * delta[i];
+            nl_operator(nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, NULL, xc_, fx);
+            for (long i = 0; i < n; ++i) rc[i] = b[i] - fx[i];
+            double cn = sqrt(dot(n, rc, rc));
+            if (cn < best) {
+                best_alpha = alpha;
+                best = cn;
+            }
+            if (cn < rn) break;
+            alpha *= 0.5;
+        }
+        if (best_alpha != alpha) {
+            alpha = best_alpha;
+            for (long i = 0; i < n; ++i) xc_[i] = x[i] + alpha * delta[i];
+            nl_operator(nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, NULL, xc_, fx);
+            for (long i = 0; i < n; ++i) rc[i] = b[i] - fx[i];
+        }
+        for (long i = 0; i < n; ++i) {
+            x[i] = xc_[i];
+            r[i] = rc[i];
+        }
+        ++steps;
+    }
+    rn = sqrt(dot(n, r, r));
+    out[0] = steps;
+    out[1] = inner;
+    out[2] = r0;
+    out[3] = rn;
+    free(r);
+    free(fx);
+    free(delta);
+    free(xc_);
+    free(rc);
+    free(diag);
+    return converged;
+}

@@ -55,6 +55,7 @@ class SolverDiagnostics(ctypes.Structure):
         ("final_relative_residual", ctypes.c_double),
         ("true_relative_residual", ctypes.c_double),
         ("duration", ctypes.c_double),
+        ("outer_iterations", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -104,6 +105,10 @@ SIGNATURES = {
     "sil_magnetostatics3d_nodal_mu": (ctypes.c_int, [_I64, _I64, _I64, _P, _D, _P, _P]),
     "sil_magnetostatics3d_rhs": (ctypes.c_int, [_P, _P, _P, _P]),
     "sil_poisson2d_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _P, _P, _P]),
+    "sil_magnetostatics3d_nonlinear_create": (
+        ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _I32, _D]),
+    "sil_magnetostatics3d_apply_jacobian": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "sil_magnetostatics3d_jacobian_diagonal": (ctypes.c_int, [_P, _P, _P, _P]),
     "sil_magnetostatics3d_assemble": (ctypes.c_int, [_P, ctypes.POINTER(_P), _P]),
     "sil_csr_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _P, _P, _P, _P]),
     "sil_csr_nnz": (_I64, [_P]),

@@ -13,6 +13,10 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
 int operator_apply64(sil_operator op, const double* in, double* out, hipStream_t s);
 int operator_apply32(sil_operator op, const float* in, float* out, hipStream_t s);
 int operator_diagonal(sil_operator op, double* d, hipStream_t s);
+int newton_solve(sil_operator op, const double* rhs, double* x, const sil_solver_settings& st,
+                 sil_solver_diagnostics& dg, hipStream_t s);
+void destroy_newton(NewtonBuffers* nb);
+int ms3d_ferro_flags(const double* frac, uint8_t* flags, int64_t n, hipStream_t s);
 }  // namespace sil
 
 using namespace sil;
@@ -57,6 +61,17 @@ void free_operator(sil_operator op)
     f(op->p2.wx32);
     f(op->p2.wy32);
     csr_free(op->csr);
+    f(const_cast<BhCurve*>(op->nl.curve));
+    f(op->nl.ferro);
+    f(op->nl.st_a);
+    f(op->nl.st_c);
+    for (int q = 0; q < 3; ++q) f(op->nl.st_b[q]);
+    if (op->jac) {
+        // shares ms / nl with op: only its own solver workspace is freed
+        destroy_workspace(op->jac->ws);
+        delete op->jac;
+    }
+    destroy_newton(op->newton);
     delete op;
 }
 
@@ -137,6 +152,91 @@ int sil_magnetostatics3d_create(sil_operator* out, int64_t nx, int64_t ny, int64
     return SIL_OK;
 }
 
+int sil_magnetostatics3d_nonlinear_create(sil_operator* out, int64_t nx, int64_t ny, int64_t nz,
+                                          const double* x_coords, const double* y_coords, const double* z_coords,
+                                          const double* mu_nodal, const double* ferromagnetic_nodal,
+                                          const double* bh_b, const double* bh_h, int32_t nsamples,
+                                          double gauge_penalty)
+{
+    SIL_REQUIRE(out && ferromagnetic_nodal && bh_b && bh_h, SIL_ERR_INVALID,
+                "magnetostatics3d_nonlinear_create: null argument");
+    SIL_REQUIRE(nsamples >= 2 && nsamples <= kBhMax, SIL_ERR_INVALID,
+                "invalid nonlinear B-H curve sample count %d (2..%d)", nsamples, kBhMax);
+    for (int i = 0; i + 1 < nsamples; ++i)
+        SIL_REQUIRE(bh_b[i + 1] > bh_b[i], SIL_ERR_INVALID, "B-H curve: induction samples must increase");
+    SIL_REQUIRE(nx >= 3 && ny >= 3 && nz >= 3, SIL_ERR_INVALID,
+                "magnetostatics3d_nonlinear_create: need >= 3 nodes per axis");
+    sil_operator op = nullptr;
+    int rc = sil_magnetostatics3d_create(&op, nx, ny, nz, x_coords, y_coords, z_coords, mu_nodal, gauge_penalty);
+    if (rc) return rc;
+    op->kind = OP_MS3D_NONLINEAR;
+    const int64_t nn = nx * ny * nz;
+    // InterpolatedNonlinearBHCurve (nonlinear_magnetostatics.py:26-55)
+    BhCurve c{};
+    c.n = nsamples;
+    for (int i = 0; i < nsamples; ++i) {
+        c.b[i] = bh_b[i];
+        c.h[i] = bh_h[i];
+        c.q[i] = bh_b[i] * bh_b[i];
+    }
+    for (int i = 0; i + 1 < nsamples; ++i) c.dh_db[i] = (c.h[i + 1] - c.h[i]) / (c.b[i + 1] - c.b[i]);
+    BhCurve* dc = nullptr;
+    NlData& nl = op->nl;
+    bool ok = hipMalloc(&dc, sizeof(BhCurve)) == hipSuccess;
+    nl.curve = dc;
+    ok = ok && hipMemcpy(dc, &c, sizeof(BhCurve), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMalloc(&nl.ferro, nn) == hipSuccess;
+    ok = ok && hipMalloc(&nl.st_a, sizeof(double) * nn) == hipSuccess;
+    ok = ok && hipMalloc(&nl.st_c, sizeof(double) * nn) == hipSuccess;
+    for (int q = 0; q < 3; ++q) ok = ok && hipMalloc(&nl.st_b[q], sizeof(double) * nn) == hipSuccess;
+    if (!ok) {
+        free_operator(op);
+        set_error("magnetostatics3d_nonlinear_create: device allocation failed");
+        return SIL_ERR_HIP;
+    }
+    rc = ms3d_ferro_flags(ferromagnetic_nodal, nl.ferro, nn, 0);
+    if (rc == SIL_OK && hipDeviceSynchronize() != hipSuccess) {
+        set_error("magnetostatics3d_nonlinear_create: flag kernel failed");
+        rc = SIL_ERR_HIP;
+    }
+    op->jac = rc == SIL_OK ? new (std::nothrow) sil_operator_s() : nullptr;
+    if (rc == SIL_OK && !op->jac) {
+        set_error("magnetostatics3d_nonlinear_create: out of host memory");
+        rc = SIL_ERR_INVALID;
+    }
+    if (rc != SIL_OK) {
+        free_operator(op);
+        return rc;
+    }
+    op->jac->kind = OP_MS3D_JACOBIAN;
+    op->jac->n = op->n;
+    op->jac->ms = op->ms;
+    op->jac->nl = op->nl;
+    *out = op;
+    return SIL_OK;
+}
+
+int sil_magnetostatics3d_apply_jacobian(sil_operator op, const double* state, const double* in, double* out,
+                                        void* stream)
+{
+    SIL_REQUIRE(op && state && in && out, SIL_ERR_INVALID, "magnetostatics3d_apply_jacobian: null argument");
+    SIL_REQUIRE(op->kind == OP_MS3D_NONLINEAR, SIL_ERR_INVALID, "apply_jacobian: not a nonlinear operator");
+    hipStream_t s = as_stream(stream);
+    int rc = ms3d_state(op->ms, op->nl, state, s);
+    if (rc) return rc;
+    return ms3d_jacobian_apply64(op->ms, op->nl, in, out, DotEpilogue{nullptr}, nullptr, nullptr, s);
+}
+
+int sil_magnetostatics3d_jacobian_diagonal(sil_operator op, const double* state, double* diag, void* stream)
+{
+    SIL_REQUIRE(op && state && diag, SIL_ERR_INVALID, "magnetostatics3d_jacobian_diagonal: null argument");
+    SIL_REQUIRE(op->kind == OP_MS3D_NONLINEAR, SIL_ERR_INVALID, "jacobian_diagonal: not a nonlinear operator");
+    hipStream_t s = as_stream(stream);
+    int rc = ms3d_state(op->ms, op->nl, state, s);
+    if (rc) return rc;
+    return ms3d_jacobian_diagonal(op->ms, op->nl, diag, s);
+}
+
 int sil_magnetostatics3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const double* cell_mu, double mu0,
                                   double* mu_nodal, void* stream)
 {
@@ -148,7 +248,8 @@ int sil_magnetostatics3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const doub
 int sil_magnetostatics3d_rhs(sil_operator op, const double* cell_current, double* rhs, void* stream)
 {
     SIL_REQUIRE(op && cell_current && rhs, SIL_ERR_INVALID, "magnetostatics3d_rhs: null argument");
-    SIL_REQUIRE(op->kind == OP_MAGNETOSTATICS3D, SIL_ERR_INVALID, "magnetostatics3d_rhs: not a magnetostatics operator");
+    SIL_REQUIRE(op->kind == OP_MAGNETOSTATICS3D || op->kind == OP_MS3D_NONLINEAR, SIL_ERR_INVALID,
+                "magnetostatics3d_rhs: not a magnetostatics operator");
     return ms3d_rhs(op->ms, cell_current, rhs, as_stream(stream));
 }
 
@@ -269,6 +370,13 @@ int64_t sil_operator_size(sil_operator op) { return op ? op->n : -1; }
 int sil_operator_apply(sil_operator op, const double* in, double* out, void* stream)
 {
     SIL_REQUIRE(op && in && out, SIL_ERR_INVALID, "operator_apply: null argument");
+    if (op->kind == OP_MS3D_NONLINEAR) {
+        // F(in); the operator keeps in's state (a later Jacobian is taken there)
+        hipStream_t s = as_stream(stream);
+        int rc = ms3d_state(op->ms, op->nl, in, s);
+        if (rc) return rc;
+        return ms3d_state_apply64(op->ms, op->nl, in, out, s);
+    }
     return operator_apply64(op, in, out, as_stream(stream));
 }
 
@@ -315,6 +423,13 @@ int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs,
                     settings->preconditioner == SIL_PRECOND_SPD_ISAI,
                 SIL_ERR_INVALID, "unknown preconditioner %d", settings->preconditioner);
     *diagnostics = sil_solver_diagnostics{};
+    if (op->kind == OP_MS3D_NONLINEAR) {
+        SIL_REQUIRE(settings->precision == SIL_PRECISION_FP64, SIL_ERR_UNSUPPORTED,
+                    "nonlinear solve: FP64 only");
+        SIL_REQUIRE(settings->preconditioner != SIL_PRECOND_SPD_ISAI, SIL_ERR_UNSUPPORTED,
+                    "nonlinear solve: identity or Jacobi preconditioning");
+        return newton_solve(op, rhs, solution, *settings, *diagnostics, as_stream(stream));
+    }
     return cg_solve(op, rhs, solution, *settings, *diagnostics, as_stream(stream));
 }
 

@@ -325,6 +325,8 @@ int apply64(sil_operator op, const double* in, double* out, DotEpilogue dot, con
 {
     if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_apply64(op->ms, in, out, dot, pu, nb, s);
     if (op->kind == OP_CSR) return csr_apply64(op->csr, in, out, dot, pu, nb, s);
+    if (op->kind == OP_MS3D_JACOBIAN) return ms3d_jacobian_apply64(op->ms, op->nl, in, out, dot, pu, nb, s);
+    SIL_REQUIRE(op->kind == OP_POISSON2D, SIL_ERR_UNSUPPORTED, "operator kind %d has no linear application", op->kind);
     return p2d_apply64(op->p2, in, out, dot, pu, nb, s);
 }
 
@@ -333,6 +335,7 @@ int apply32(sil_operator op, const float* in, float* out, DotEpilogue dot, const
 {
     if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_apply32(op->ms, in, out, dot, pu, nb, s);
     SIL_REQUIRE(op->kind != OP_CSR, SIL_ERR_UNSUPPORTED, "assembled operators have FP64 values only");
+    SIL_REQUIRE(op->kind == OP_POISSON2D, SIL_ERR_UNSUPPORTED, "operator kind %d has no FP32 application", op->kind);
     return p2d_apply32(op->p2, in, out, dot, pu, nb, s);
 }
 
@@ -340,6 +343,8 @@ int diagonal(sil_operator op, double* d, hipStream_t s)
 {
     if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_diagonal(op->ms, d, s);
     if (op->kind == OP_CSR) return csr_diagonal(op->csr, d, s);
+    if (op->kind == OP_MS3D_JACOBIAN) return ms3d_jacobian_diagonal(op->ms, op->nl, d, s);
+    SIL_REQUIRE(op->kind == OP_POISSON2D, SIL_ERR_UNSUPPORTED, "operator kind %d has no diagonal", op->kind);
     return p2d_diagonal(op->p2, d, s);
 }
 
@@ -376,7 +381,7 @@ int ensure_workspace(sil_operator op, bool mixed, hipStream_t s)
         SIL_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&w.host_sc), sizeof(CgScalars)));
         // number of blocks of one operator application
         int nb = 0;
-        if (op->kind == OP_MAGNETOSTATICS3D)
+        if (op->kind == OP_MAGNETOSTATICS3D || op->kind == OP_MS3D_JACOBIAN)
             nb = static_cast<int>((op->ms.nx * op->ms.ny * op->ms.nz + 255) / 256);
         else if (op->kind == OP_POISSON2D)
             nb = static_cast<int>((op->p2.nx * op->p2.ny + 255) / 256);
@@ -493,6 +498,13 @@ int true_residual(sil_operator op, hipStream_t s)
 }
 
 }  // namespace
+
+// the operator's diagonal changed (a Jacobian at a new state): Jacobi is
+// recomputed by the next solve
+void invalidate_jacobi(sil_operator op)
+{
+    if (op && op->ws) op->ws->jacobi_ready = false;
+}
 
 void destroy_workspace(Workspace* w)
 {

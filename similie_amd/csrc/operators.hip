@@ -55,7 +55,16 @@ struct MsArgs {
     const T* ls[3];
     const T* inv_ls[3];
     const T* nu;
+    // nonlinear material state (FP64): a, c and B0 per node
+    const double* st_a;
+    const double* st_c;
+    const double* st_b[3];
 };
+
+// material response of the curl-curl march: ν·B (linear), a·B (the
+// nonlinear operator at the state's own x) or a·δB + c·B0·(B0·δB) (its
+// Jacobian at the state)
+enum MsResponse { RESP_LINEAR = 0, RESP_SCALED = 1, RESP_JACOBIAN = 2 };
 
 // p_new = z + β p_old with z = D⁻¹ r (Jacobi) or r: the CG direction, in
 // the arithmetic of p_value
@@ -144,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_ms3d_apply(MsArgs<T> a, const T* __r
 constexpr int kMsTY = 8;
 constexpr int kMsChunk = 32;
 
-template <typename T, bool PU>
+template <typename T, bool PU, int RESP = RESP_LINEAR>
 __global__ __launch_bounds__(64 * kMsTY) void k_ms3d_march(MsArgs<T> a, const T* __restrict__ x, T* __restrict__ out,
                                                            double* __restrict__ partials, PUpdate<T> pu, int ntx,
                                                            int nty)
@@ -199,13 +208,28 @@ __global__ __launch_bounds__(64 * kMsTY) void k_ms3d_march(MsArgs<T> a, const T*
         if (ik) {
             const int64_t kk = k;
             const T lz = a.ls[2][kk], ilz = a.inv_ls[2][kk];
-            const T nu = a.nu[lin + plane * kk];
             const T bx = ay2 - A0[2] - A1[1] + A0[1];
             const T by = -ax2 + A0[2] + A1[0] - A0[0];
             const T bz = ax1 - A0[1] - ay0 + A0[0];
-            H[0] = (lx * ily * ilz) * nu * bx;
-            H[1] = (ly * ilx * ilz) * nu * by;
-            H[2] = (lz * ilx * ily) * nu * bz;
+            const int64_t node = lin + plane * kk;
+            if constexpr (RESP == RESP_LINEAR) {
+                const T nu = a.nu[node];
+                H[0] = (lx * ily * ilz) * nu * bx;
+                H[1] = (ly * ilx * ilz) * nu * by;
+                H[2] = (lz * ilx * ily) * nu * bz;
+            } else if constexpr (RESP == RESP_SCALED) {
+                const T sa = a.st_a[node];
+                H[0] = (lx * ily * ilz) * (bx * sa);
+                H[1] = (ly * ilx * ilz) * (by * sa);
+                H[2] = (lz * ilx * ily) * (bz * sa);
+            } else {
+                const T sa = a.st_a[node], sc = a.st_c[node];
+                const T b0x = a.st_b[0][node], b0y = a.st_b[1][node], b0z = a.st_b[2][node];
+                const T cd = sc * (b0x * bx + b0y * by + b0z * bz);
+                H[0] = (lx * ily * ilz) * (sa * bx + cd * b0x);
+                H[1] = (ly * ilx * ilz) * (sa * by + cd * b0y);
+                H[2] = (lz * ilx * ily) * (sa * bz + cd * b0z);
+            }
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) sH[c][row][lane] = H[c];
@@ -445,9 +469,9 @@ MsArgs<float> ms_args<float>(const Ms3dData& d)
     return a;
 }
 
-template <typename T>
+template <typename T, int RESP = RESP_LINEAR>
 int ms_apply(const Ms3dData& d, const T* in, T* out, DotEpilogue dot, const PUpdate<T>* pu, int* nb_out,
-             hipStream_t s)
+             hipStream_t s, const NlData* nl = nullptr)
 {
     const int64_t nn = d.nx * d.ny * d.nz;
     const unsigned nb = nblk(nn);
@@ -458,6 +482,13 @@ int ms_apply(const Ms3dData& d, const T* in, T* out, DotEpilogue dot, const PUpd
     }
     if (nb_out) *nb_out = (int)nb;
     MsArgs<T> a = ms_args<T>(d);
+    if (nl) {
+        a.st_a = nl->st_a;
+        a.st_c = nl->st_c;
+        for (int q = 0; q < 3; ++q) a.st_b[q] = nl->st_b[q];
+    }
+    SIL_REQUIRE(RESP == RESP_LINEAR || (d.nx >= 3 && d.ny >= 3 && d.nz >= 3), SIL_ERR_UNSUPPORTED,
+                "nonlinear magnetostatics needs at least 3 nodes per axis");
     if (d.nx >= 3 && d.ny >= 3 && d.nz >= 3) {
         // march: tiles of 62 × (kMsTY − 2) output nodes, chunks of kMsChunk planes
         const int ntx = (int)((d.nx + 61) / 62), nty = (int)((d.ny + kMsTY - 3) / (kMsTY - 2));
@@ -478,8 +509,8 @@ int ms_apply(const Ms3dData& d, const T* in, T* out, DotEpilogue dot, const PUpd
             src = pu->p_new;
             skip.done = pu->done;
         }
-        hipLaunchKernelGGL((k_ms3d_march<T, false>), dim3((unsigned)nbm), dim3(64 * kMsTY), 0, s, a, src, out,
-                           dot.partials, skip, ntx, nty);
+        hipLaunchKernelGGL((k_ms3d_march<T, false, RESP>), dim3((unsigned)nbm), dim3(64 * kMsTY), 0, s, a, src,
+                           out, dot.partials, skip, ntx, nty);
     } else if (pu) {
         // the direction is formed in its own pass: formed inside the
         // direct kernel, every one of the ~40 neighbour reads of a node would
@@ -523,6 +554,17 @@ int p2_apply(const Poisson2dData& d, const T* wx, const T* wy, const T* in, T* o
 }  // namespace
 
 int64_t max_dot_blocks() { return kMaxDotBlocks; }
+
+int ms3d_state_apply64(const Ms3dData& d, const NlData& nl, const double* x, double* out, hipStream_t s)
+{
+    return ms_apply<double, RESP_SCALED>(d, x, out, DotEpilogue{nullptr}, nullptr, nullptr, s, &nl);
+}
+
+int ms3d_jacobian_apply64(const Ms3dData& d, const NlData& nl, const double* in, double* out, DotEpilogue dot,
+                          const PUpdate<double>* pu, int* nb, hipStream_t s)
+{
+    return ms_apply<double, RESP_JACOBIAN>(d, in, out, dot, pu, nb, s, &nl);
+}
 
 int direction_pass64(const PUpdate<double>& pu, int64_t n, hipStream_t s)
 {

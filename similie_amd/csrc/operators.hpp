@@ -5,9 +5,10 @@
 
 namespace sil {
 
-enum OperatorKind { OP_MAGNETOSTATICS3D = 1, OP_POISSON2D = 2, OP_CSR = 3 };
+enum OperatorKind { OP_MAGNETOSTATICS3D = 1, OP_POISSON2D = 2, OP_CSR = 3, OP_MS3D_NONLINEAR = 4, OP_MS3D_JACOBIAN = 5 };
 
-struct Workspace; // CG vectors, scalars and captured graphs (krylov.hip)
+struct Workspace;     // CG vectors, scalars and captured graphs (krylov.hip)
+struct NewtonBuffers; // Newton vectors (nonlinear.hip)
 
 struct Ms3dData {
     int64_t nx, ny, nz;
@@ -19,6 +20,23 @@ struct Ms3dData {
     float* nu32;        // FP32 copy
     float* ls32[3];
     float* inv_ls32[3];
+};
+
+// nonlinear material: interpolated B-H curve (≤ 64 samples, device copy)
+// and the per-node state of the last operator evaluation
+constexpr int kBhMax = 64;
+struct BhCurve {
+    int n;
+    double b[kBhMax], h[kBhMax], q[kBhMax], dh_db[kBhMax];
+};
+
+struct NlData {
+    const BhCurve* curve = nullptr; // device
+    uint8_t* ferro = nullptr;       // per node: ferromagnetic fraction > 0.5
+    // state of the last evaluation of F at x, per node: H = w·(a·δB + c·B0·(B0·δB))
+    double* st_a = nullptr;
+    double* st_c = nullptr;
+    double* st_b[3] = {nullptr, nullptr, nullptr};
 };
 
 struct Poisson2dData {
@@ -56,6 +74,13 @@ int ms3d_diagonal(const Ms3dData& d, double* diag, hipStream_t s);
 int ms3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const double* cell_mu, double mu0, double* mu, hipStream_t s);
 int ms3d_rhs(const Ms3dData& d, const double* cell_current, double* rhs, hipStream_t s);
 int ms3d_nu_from_mu(const double* mu, double* nu, float* nu32, int64_t n, hipStream_t s);
+
+// nonlinear magnetostatics (nonlinear.hip / operators.hip)
+int ms3d_state(const Ms3dData& d, const NlData& nl, const double* x, hipStream_t s);       // state of x
+int ms3d_state_apply64(const Ms3dData& d, const NlData& nl, const double* x, double* out, hipStream_t s); // F(x), state given
+int ms3d_jacobian_apply64(const Ms3dData& d, const NlData& nl, const double* in, double* out, DotEpilogue dot,
+                          const PUpdate<double>* pu, int* nblocks_out, hipStream_t s);
+int ms3d_jacobian_diagonal(const Ms3dData& d, const NlData& nl, double* diag, hipStream_t s);
 
 // p_new = z + β p_old over n rows (the CG direction) as its own pass
 int direction_pass64(const PUpdate<double>& pu, int64_t n, hipStream_t s);
@@ -107,5 +132,8 @@ struct sil_operator_s {
     sil::Ms3dData ms;
     sil::Poisson2dData p2;
     sil::CsrData csr;
+    sil::NlData nl;
     sil::Workspace* ws;
+    sil_operator_s* jac; // nonlinear operator: its Jacobian at the current state (shares ms / nl)
+    sil::NewtonBuffers* newton;
 };

@@ -107,6 +107,55 @@ class MagnetostaticsOperator3D(LinearOperator):
         return AssembledOperator._adopt(h)
 
 
+class NonlinearMagnetostaticsOperator3D(MagnetostaticsOperator3D):
+    """MagnetostaticsOperator3D with MaterialMagnetostaticsHamiltonian
+    (magnetostatics_onelab.hpp:957-1011): nodes whose ferromagnetic fraction
+    exceeds 0.5 follow the interpolated B-H curve (bh_b, bh_h; ≤ 64 samples,
+    nonlinear_magnetostatics.py:14-227), the others are linear with mu_nodal.
+    apply(x) is F(x); minimize_strong_formulation_residual runs Newton
+    (minimize_strong_formulation_residual.hpp:1383-1569)."""
+
+    def __init__(self, x_coords, y_coords, z_coords, mu_nodal, ferromagnetic_nodal, bh_b, bh_h,
+                 gauge_penalty=20.0):
+        self.shape = (len(x_coords), len(y_coords), len(z_coords))
+        _f64(mu_nodal, "mu_nodal")
+        _f64(ferromagnetic_nodal, "ferromagnetic_nodal")
+        if len(bh_b) != len(bh_h):
+            raise ValueError("bh_b and bh_h must have the same length")
+        h = ctypes.c_void_p()
+        xs, ys, zs = _host_array(x_coords), _host_array(y_coords), _host_array(z_coords)
+        bb, hh = _host_array(bh_b), _host_array(bh_h)
+        nx, ny, nz = self.shape
+        L.check(L.lib().sil_magnetostatics3d_nonlinear_create(
+            ctypes.byref(h), nx, ny, nz, ctypes.cast(xs, ctypes.c_void_p), ctypes.cast(ys, ctypes.c_void_p),
+            ctypes.cast(zs, ctypes.c_void_p), L.ptr(mu_nodal), L.ptr(ferromagnetic_nodal),
+            ctypes.cast(bb, ctypes.c_void_p), ctypes.cast(hh, ctypes.c_void_p), len(bh_b), float(gauge_penalty)))
+        self.handle = h
+        self._destroy = L.lib().sil_operator_destroy
+
+    @staticmethod
+    def nodal_fraction(shape, cell_flags, stream=None):
+        """mean over the adjacent cells of a 0/1 nonlinear-material flag
+        (magnetostatics_onelab.hpp:5026-5063; 0 where no cell is adjacent)"""
+        return MagnetostaticsOperator3D.nodal_mu(shape, cell_flags, 0.0, stream)
+
+    def apply_jacobian(self, state, v, out=None, stream=None):
+        """apply_jacobian (magnetostatics_onelab.hpp:2918-3113): J(state)·v"""
+        _f64(state, "state")
+        _f64(v, "v")
+        out = torch.empty_like(v) if out is None else _f64(out, "out")
+        L.check(L.lib().sil_magnetostatics3d_apply_jacobian(self.handle, L.ptr(state), L.ptr(v), L.ptr(out),
+                                                             L.stream_handle(stream)))
+        return out
+
+    def jacobian_diagonal(self, state, stream=None):
+        _f64(state, "state")
+        d = torch.empty(self.size(), dtype=torch.float64, device="cuda")
+        L.check(L.lib().sil_magnetostatics3d_jacobian_diagonal(self.handle, L.ptr(state), L.ptr(d),
+                                                               L.stream_handle(stream)))
+        return d
+
+
 class AssembledOperator(LinearOperator):
     """gko::matrix::Csr<double, int32>: a square CSR matrix held on the
     device (int32 row pointers and columns, ascending within a row, FP64

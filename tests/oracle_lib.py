@@ -36,6 +36,8 @@ def lib():
         _lib.orc_ms_cg.restype = ctypes.c_int
         _lib.orc_ms_assemble_csr.restype = ctypes.c_long
         _lib.orc_spd_isai.restype = ctypes.c_long
+        _lib.orc_bh_eval.restype = ctypes.c_double
+        _lib.orc_nl_newton.restype = ctypes.c_int
     return _lib
 
 
@@ -289,6 +291,65 @@ def spd_isai(row_ptr, cols, vals):
         raise ValueError(f"spd_isai failed with code {rc}")
     nnz = int(grp[-1])
     return grp, gc[:nnz].copy(), gv[:nnz].copy()
+
+
+# --------------------------------------------------- nonlinear magnetostatics ---
+
+BH_FUNCTIONS = {"h_from_b": 0, "dh_db_from_b": 1, "h_over_b_from_q": 2, "d_h_over_b_dq_from_q": 3,
+                "d2_h_over_b_dq2_from_q": 4, "nu": 5}
+
+
+def bh_eval(bh_b, bh_h, name, arg):
+    b, h = _f64(bh_b), _f64(bh_h)
+    return float(lib().orc_bh_eval(_d(b), _d(h), ctypes.c_int(len(b)), ctypes.c_int(BH_FUNCTIONS[name]),
+                                   ctypes.c_double(arg)))
+
+
+class Nonlinear:
+    """grid, nodal μ, ferromagnetic fraction, B-H samples and gauge penalty
+    of the nonlinear magnetostatics operator"""
+
+    def __init__(self, coords, mu, ferro, bh_b, bh_h, penalty):
+        self.shape = tuple(len(c) for c in coords)
+        self.coords = [_f64(c) for c in coords]
+        self.mu, self.ferro = _f64(mu), _f64(ferro)
+        self.bh_b, self.bh_h = _f64(bh_b), _f64(bh_h)
+        self.penalty = float(penalty)
+        self.n = 3 * int(np.prod(self.shape))
+
+    def _head(self):
+        nx, ny, nz = self.shape
+        xc, yc, zc = self.coords
+        return [ctypes.c_long(nx), ctypes.c_long(ny), ctypes.c_long(nz), _d(xc), _d(yc), _d(zc), _d(self.mu),
+                _d(self.ferro), _d(self.bh_b), _d(self.bh_h), ctypes.c_int(len(self.bh_b)),
+                ctypes.c_double(self.penalty)]
+
+    def apply(self, x):
+        out = np.zeros(self.n)
+        xx = _f64(x)
+        lib().orc_nl_apply(*self._head(), _d(xx), _d(out))
+        return out
+
+    def jacobian_apply(self, state, v):
+        out = np.zeros(self.n)
+        st, vv = _f64(state), _f64(v)
+        lib().orc_nl_jacobian_apply(*self._head(), _d(st), _d(vv), _d(out))
+        return out
+
+    def jacobian_diagonal(self, state):
+        out = np.zeros(self.n)
+        st = _f64(state)
+        lib().orc_nl_jacobian_diagonal(*self._head(), _d(st), _d(out))
+        return out
+
+    def newton(self, b, tol, max_inner, jacobi=True):
+        x = np.zeros(self.n)
+        out = np.zeros(4)
+        bb = _f64(b)
+        ok = lib().orc_nl_newton(*self._head(), _d(bb), ctypes.c_double(tol), ctypes.c_int(max_inner),
+                                 ctypes.c_int(1 if jacobi else 0), _d(x), _d(out))
+        return x, {"converged": bool(ok), "newton_steps": int(out[0]), "iterations": int(out[1]),
+                   "initial_residual_l2": out[2], "final_residual_l2": out[3]}
 
 
 def poisson2d_apply(hx, hy, kappa, u):
