@@ -241,6 +241,35 @@ def magnetostatics_assembled(torch, max_iterations=20000):
     return out
 
 
+def magnetostatics_nonlinear(torch, max_iterations=20000):
+    """192³-cell inductor with a nonlinear (EIcore B-H) core: Newton with
+    line search to 1e-10, inner Jacobi CG in FP64 and in mixed precision."""
+    from similie_amd import solvers as S
+    from similie_amd.workloads import MU0, magnetostatics_nonlinear_inductor
+
+    curve = json.loads((ROOT / "tests" / "golden" / "eicore_bh_curve.json").read_text())
+    coords, cell_mu, flags, cell_j = magnetostatics_nonlinear_inductor()
+    shape = tuple(len(c) for c in coords)
+    mu = S.MagnetostaticsOperator3D.nodal_mu(shape, torch.from_numpy(cell_mu).cuda(), MU0)
+    frac = S.NonlinearMagnetostaticsOperator3D.nodal_fraction(shape, torch.from_numpy(flags).cuda())
+    op = S.NonlinearMagnetostaticsOperator3D(*coords, mu, frac, curve["b"], curve["h"], gauge_penalty=20.0)
+    rhs = op.rhs(torch.from_numpy(np.ascontiguousarray(cell_j)).cuda())
+    out = {"workload": "magnetostatics inductor 192^3 cells, unit spacing, EIcore B-H curve in the core "
+                       "(examples/onelab_inductor/BH.pro), coil current 2000 per unit area, gauge penalty 20",
+           "dofs": int(rhs.numel())}
+    for prec in ("fp64", "mixed"):
+        print(f"bench:   nonlinear {prec} ...", file=sys.stderr, flush=True)
+        st = S.StrongFormulationSolverSettings(max_iterations=max_iterations, relative_tolerance=1e-10,
+                                               precision=prec)
+        _, d = S.minimize_strong_formulation_residual(op, rhs, settings=st)
+        out[prec] = {"seconds": round(d["duration"], 4), "newton_steps": d["outer_iterations"],
+                     "cg_iterations": d["iterations"], "converged": bool(d["converged"]),
+                     "final_relative_residual": d["final_relative_residual"]}
+    if out["mixed"]["seconds"] > 0:
+        out["mixed_speedup"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    return out
+
+
 def solve_uneven_batch(torch):
     """512 problems, sides log-uniform in [32, 512], one persistent launch."""
     from similie_amd.batched import BatchedPoisson2D
@@ -391,6 +420,7 @@ def main():
             "learned_preconditioner": lambda: solve_learned(torch),
             "magnetostatics_inductor": lambda: solve_magnetostatics(torch, args.ms_iterations),
             "magnetostatics_assembled": lambda: magnetostatics_assembled(torch, args.ms_iterations),
+            "magnetostatics_nonlinear": lambda: magnetostatics_nonlinear(torch, args.ms_iterations),
         }
         pick = [k for k in args.solves.split(",") if k]
         unknown = [k for k in pick if k not in solves]

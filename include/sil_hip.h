@@ -164,7 +164,9 @@ int sil_magnetostatics3d_create(sil_operator* out, int64_t nx, int64_t ny, int64
  * others are linear with mu_nodal.  sil_operator_apply gives F(x) and keeps
  * x's material state; sil_minimize_strong_formulation_residual runs the
  * reference's Newton branch (minimize_strong_formulation_residual.hpp
- * :1383-1569; FP64, identity or Jacobi).  Needs >= 3 nodes per axis. */
+ * :1383-1569; identity or Jacobi; with SIL_PRECISION_MIXED the inner CG
+ * applies the Jacobian in FP32 and converges on its FP64 residual, F(x)
+ * stays FP64).  Needs >= 3 nodes per axis. */
 int sil_magnetostatics3d_nonlinear_create(sil_operator* out, int64_t nx, int64_t ny, int64_t nz,
                                           const double* x_coords, const double* y_coords, const double* z_coords,
                                           const double* mu_nodal, const double* ferromagnetic_nodal,

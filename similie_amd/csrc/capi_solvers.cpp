@@ -66,6 +66,9 @@ void free_operator(sil_operator op)
     f(op->nl.st_a);
     f(op->nl.st_c);
     for (int q = 0; q < 3; ++q) f(op->nl.st_b[q]);
+    f(op->nl.st_a32);
+    f(op->nl.st_c32);
+    for (int q = 0; q < 3; ++q) f(op->nl.st_b32[q]);
     if (op->jac) {
         // shares ms / nl with op: only its own solver workspace is freed
         destroy_workspace(op->jac->ws);
@@ -189,6 +192,9 @@ int sil_magnetostatics3d_nonlinear_create(sil_operator* out, int64_t nx, int64_t
     ok = ok && hipMalloc(&nl.st_a, sizeof(double) * nn) == hipSuccess;
     ok = ok && hipMalloc(&nl.st_c, sizeof(double) * nn) == hipSuccess;
     for (int q = 0; q < 3; ++q) ok = ok && hipMalloc(&nl.st_b[q], sizeof(double) * nn) == hipSuccess;
+    ok = ok && hipMalloc(&nl.st_a32, sizeof(float) * nn) == hipSuccess;
+    ok = ok && hipMalloc(&nl.st_c32, sizeof(float) * nn) == hipSuccess;
+    for (int q = 0; q < 3; ++q) ok = ok && hipMalloc(&nl.st_b32[q], sizeof(float) * nn) == hipSuccess;
     if (!ok) {
         free_operator(op);
         set_error("magnetostatics3d_nonlinear_create: device allocation failed");
@@ -424,8 +430,6 @@ int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs,
                 SIL_ERR_INVALID, "unknown preconditioner %d", settings->preconditioner);
     *diagnostics = sil_solver_diagnostics{};
     if (op->kind == OP_MS3D_NONLINEAR) {
-        SIL_REQUIRE(settings->precision == SIL_PRECISION_FP64, SIL_ERR_UNSUPPORTED,
-                    "nonlinear solve: FP64 only");
         SIL_REQUIRE(settings->preconditioner != SIL_PRECOND_SPD_ISAI, SIL_ERR_UNSUPPORTED,
                     "nonlinear solve: identity or Jacobi preconditioning");
         return newton_solve(op, rhs, solution, *settings, *diagnostics, as_stream(stream));

@@ -334,6 +334,7 @@ int apply32(sil_operator op, const float* in, float* out, DotEpilogue dot, const
             hipStream_t s)
 {
     if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_apply32(op->ms, in, out, dot, pu, nb, s);
+    if (op->kind == OP_MS3D_JACOBIAN) return ms3d_jacobian_apply32(op->ms, op->nl, in, out, dot, pu, nb, s);
     SIL_REQUIRE(op->kind != OP_CSR, SIL_ERR_UNSUPPORTED, "assembled operators have FP64 values only");
     SIL_REQUIRE(op->kind == OP_POISSON2D, SIL_ERR_UNSUPPORTED, "operator kind %d has no FP32 application", op->kind);
     return p2d_apply32(op->p2, in, out, dot, pu, nb, s);

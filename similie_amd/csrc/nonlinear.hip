@@ -102,6 +102,9 @@ struct StateArgs {
     double* a;
     double* c;
     double* b[3];
+    float* a32;
+    float* c32;
+    float* b32[3];
 };
 
 // B at every interior node (boundary potentials dropped, the term order of
@@ -136,6 +139,11 @@ __global__ __launch_bounds__(kBlock) void k_nl_state(StateArgs s, const double* 
         s.b[0][t] = bx;
         s.b[1][t] = by;
         s.b[2][t] = bz;
+        s.a32[t] = (float)sa;
+        s.c32[t] = (float)sc;
+        s.b32[0][t] = (float)bx;
+        s.b32[1][t] = (float)by;
+        s.b32[2][t] = (float)bz;
     }
 }
 
@@ -277,7 +285,10 @@ int ms3d_ferro_flags(const double* frac, uint8_t* flags, int64_t n, hipStream_t 
 
 int ms3d_state(const Ms3dData& d, const NlData& nl, const double* x, hipStream_t s)
 {
-    StateArgs a{d.nx, d.ny, d.nz, d.nu, nl.ferro, nl.curve, nl.st_a, nl.st_c, {nl.st_b[0], nl.st_b[1], nl.st_b[2]}};
+    StateArgs a{d.nx,     d.ny,      d.nz,     d.nu,
+                nl.ferro, nl.curve,  nl.st_a,  nl.st_c,
+                {nl.st_b[0], nl.st_b[1], nl.st_b[2]},
+                nl.st_a32, nl.st_c32, {nl.st_b32[0], nl.st_b32[1], nl.st_b32[2]}};
     const int64_t nn = d.nx * d.ny * d.nz;
     const unsigned g = static_cast<unsigned>(std::min<int64_t>((nn + kBlock - 1) / kBlock, 16384));
     hipLaunchKernelGGL(k_nl_state, dim3(g), dim3(kBlock), 0, s, a, x);
@@ -352,8 +363,9 @@ int newton_solve(sil_operator op, const double* rhs, double* x, const sil_solver
     dg.outer_iterations = 0;
     dg.converged = r0 == 0.0 ? 1 : 0;
     double rn = r0;
+    // inner CG to the same relative tolerance; with mixed precision its
+    // Jacobian applications run in FP32 and it converges on the FP64 residual
     sil_solver_settings inner = st;
-    inner.precision = SIL_PRECISION_FP64;
     for (int it = 0; it < 30 && !dg.converged; ++it) {
         if (rn / r0 <= st.relative_tolerance) {
             dg.converged = 1;

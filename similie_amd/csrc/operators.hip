@@ -55,10 +55,10 @@ struct MsArgs {
     const T* ls[3];
     const T* inv_ls[3];
     const T* nu;
-    // nonlinear material state (FP64): a, c and B0 per node
-    const double* st_a;
-    const double* st_c;
-    const double* st_b[3];
+    // nonlinear material state: a, c and B0 per node
+    const T* st_a;
+    const T* st_c;
+    const T* st_b[3];
 };
 
 // material response of the curl-curl march: ν·B (linear), a·B (the
@@ -483,9 +483,15 @@ int ms_apply(const Ms3dData& d, const T* in, T* out, DotEpilogue dot, const PUpd
     if (nb_out) *nb_out = (int)nb;
     MsArgs<T> a = ms_args<T>(d);
     if (nl) {
-        a.st_a = nl->st_a;
-        a.st_c = nl->st_c;
-        for (int q = 0; q < 3; ++q) a.st_b[q] = nl->st_b[q];
+        if constexpr (sizeof(T) == 8) {
+            a.st_a = nl->st_a;
+            a.st_c = nl->st_c;
+            for (int q = 0; q < 3; ++q) a.st_b[q] = nl->st_b[q];
+        } else {
+            a.st_a = nl->st_a32;
+            a.st_c = nl->st_c32;
+            for (int q = 0; q < 3; ++q) a.st_b[q] = nl->st_b32[q];
+        }
     }
     SIL_REQUIRE(RESP == RESP_LINEAR || (d.nx >= 3 && d.ny >= 3 && d.nz >= 3), SIL_ERR_UNSUPPORTED,
                 "nonlinear magnetostatics needs at least 3 nodes per axis");
@@ -564,6 +570,12 @@ int ms3d_jacobian_apply64(const Ms3dData& d, const NlData& nl, const double* in,
                           const PUpdate<double>* pu, int* nb, hipStream_t s)
 {
     return ms_apply<double, RESP_JACOBIAN>(d, in, out, dot, pu, nb, s, &nl);
+}
+
+int ms3d_jacobian_apply32(const Ms3dData& d, const NlData& nl, const float* in, float* out, DotEpilogue dot,
+                          const PUpdate<float>* pu, int* nb, hipStream_t s)
+{
+    return ms_apply<float, RESP_JACOBIAN>(d, in, out, dot, pu, nb, s, &nl);
 }
 
 int direction_pass64(const PUpdate<double>& pu, int64_t n, hipStream_t s)

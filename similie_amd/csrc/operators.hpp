@@ -37,6 +37,10 @@ struct NlData {
     double* st_a = nullptr;
     double* st_c = nullptr;
     double* st_b[3] = {nullptr, nullptr, nullptr};
+    // FP32 copies for mixed-precision Jacobian applications
+    float* st_a32 = nullptr;
+    float* st_c32 = nullptr;
+    float* st_b32[3] = {nullptr, nullptr, nullptr};
 };
 
 struct Poisson2dData {
@@ -80,6 +84,8 @@ int ms3d_state(const Ms3dData& d, const NlData& nl, const double* x, hipStream_t
 int ms3d_state_apply64(const Ms3dData& d, const NlData& nl, const double* x, double* out, hipStream_t s); // F(x), state given
 int ms3d_jacobian_apply64(const Ms3dData& d, const NlData& nl, const double* in, double* out, DotEpilogue dot,
                           const PUpdate<double>* pu, int* nblocks_out, hipStream_t s);
+int ms3d_jacobian_apply32(const Ms3dData& d, const NlData& nl, const float* in, float* out, DotEpilogue dot,
+                          const PUpdate<float>* pu, int* nblocks_out, hipStream_t s);
 int ms3d_jacobian_diagonal(const Ms3dData& d, const NlData& nl, double* diag, hipStream_t s);
 
 // p_new = z + β p_old over n rows (the CG direction) as its own pass

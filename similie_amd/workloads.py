@@ -59,6 +59,18 @@ def magnetostatics_inductor(cells=192, core=(48, 48, 96), coil=(32, 40, 64)):
     return [coords] * 3, mu.reshape(-1), current
 
 
+def magnetostatics_nonlinear_inductor(cells=192, current=2.0e3):
+    """The inductor of magnetostatics_inductor on unit-spaced nodes, its core
+    a nonlinear material: (coordinates, cell μ, cell nonlinear-material flag,
+    cell current scaled by `current`).  The B-H curve acts on the discrete
+    induction (face fluxes), so unit spacing makes them Tesla-sized; with
+    current 2000 the core reaches |B| ≈ 0.3, several segments into a
+    typical curve."""
+    coords, mu, j = magnetostatics_inductor(cells)
+    flags = (mu > 1.5 * MU0).astype(np.float64)
+    return [np.arange(cells + 1, dtype=np.float64)] * 3, mu, flags, j * current
+
+
 def gaussian_random_field(shape, sigma, corr, generator, device="cuda"):
     """One field of the given 2D shape with covariance σ² exp(−r²/(2ℓ²)),
     spectral synthesis on a periodic grid twice as large, cropped and

@@ -114,12 +114,44 @@ def test_newton_single_step_on_a_linear_curve(S):
     assert d["converged"] and d["outer_iterations"] <= 2
 
 
+def test_newton_with_mixed_precision_inner_solves(S):
+    """FP32 Jacobian applications inside CG, FP64 residual replacement and
+    FP64 F(x): the same Newton steps and a verified solution"""
+    shape = (17, 9, 12)
+    op, ref, rng = make(S, shape, 21)
+    b = ref.apply(0.1 * rng.standard_normal(op.size()))
+    tol = 1e-10
+    res = {}
+    for prec in ("fp64", "mixed"):
+        st = S.StrongFormulationSolverSettings(relative_tolerance=tol, max_iterations=20000, precision=prec)
+        x, d = S.minimize_strong_formulation_residual(op, cuda(b), settings=st)
+        assert d["converged"], (prec, d)
+        r = np.linalg.norm(b - ref.apply(x.cpu().numpy())) / np.linalg.norm(b)
+        assert r <= 1.0001 * tol, (prec, r)
+        res[prec] = (x.cpu().numpy(), d)
+    assert abs(res["fp64"][1]["outer_iterations"] - res["mixed"][1]["outer_iterations"]) <= 1
+    assert rel(res["mixed"][0], res["fp64"][0]) <= 1e-5
+
+
+def test_newton_stagnation_matches_the_oracle(S):
+    """a field strong enough that Newton with backtracking stalls on the
+    kinked curve: after the 30 steps the GPU and the oracle stop at the same
+    residual"""
+    shape = (17, 9, 12)
+    op, ref, rng = make(S, shape, 21)
+    b = ref.apply(rng.standard_normal(op.size()))
+    st = S.StrongFormulationSolverSettings(relative_tolerance=1e-10, max_iterations=20000)
+    x, d = S.minimize_strong_formulation_residual(op, cuda(b), settings=st)
+    xr, dr = ref.newton(b, 1e-10, 20000)
+    assert not d["converged"] and not dr["converged"]
+    assert d["outer_iterations"] == dr["newton_steps"] == 30
+    assert d["final_residual_l2"] == pytest.approx(dr["final_residual_l2"], rel=1e-6)
+
+
 def test_nonlinear_solve_rejects_unsupported_settings(S):
     L = __import__("similie_amd._lib", fromlist=["SilError"])
     op, ref, rng = make(S, (6, 5, 7), 2)
     b = cuda(ref.apply(rng.standard_normal(op.size())))
-    with pytest.raises(L.SilError, match="FP64"):
-        S.minimize_strong_formulation_residual(op, b, settings=S.StrongFormulationSolverSettings(precision="mixed"))
     with pytest.raises(L.SilError, match="Jacobi"):
         S.minimize_strong_formulation_residual(op, b, settings=S.StrongFormulationSolverSettings(
             preconditioner="spd_isai"))
