@@ -274,6 +274,106 @@ int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs,
                                              sil_solver_diagnostics* diagnostics, void* stream);
 
 /* ------------------------------------------------------------------ */
+/* input formats (host only: no device memory, no stream)              */
+/* ------------------------------------------------------------------ */
+
+/* gmsh::parse_supported_msh2_mesh + build_structured_grid
+ * (onelab_interface/gmsh_structured_grid.hpp:239-553): a Gmsh MSH2 ASCII
+ * mesh made of quadrilaterals (2D) or hexahedra (3D) on a full rectilinear
+ * grid; points, lines and triangles are skipped.  The grid keeps the sorted
+ * axis coordinates (merged within 1e-12) and the physical tag of every cell,
+ * x fastest. */
+typedef struct sil_structured_grid_s* sil_structured_grid;
+int sil_msh2_read(const char* path, sil_structured_grid* out);
+int sil_msh2_parse(const char* text, int64_t length, sil_structured_grid* out);
+/* ndim = 2 or 3; nodes[3] = nodes per axis (1 for the missing axis) */
+int sil_structured_grid_shape(sil_structured_grid grid, int32_t* ndim, int64_t* nodes);
+int sil_structured_grid_coords(sil_structured_grid grid, int32_t axis, double* out);
+/* z of the plane of a quadrilateral mesh */
+double sil_structured_grid_plane(sil_structured_grid grid);
+int sil_structured_grid_cell_tags(sil_structured_grid grid, int32_t* out);
+int sil_structured_grid_destroy(sil_structured_grid grid);
+
+/* .silpro problem files (onelab_interface/onelab_interface.hpp:160-560):
+ * sections of `Key value;` entries, # and // comments, quoted strings. */
+typedef struct sil_silpro_s* sil_silpro;
+int sil_silpro_read(const char* path, sil_silpro* out);
+int sil_silpro_parse(const char* text, int64_t length, sil_silpro* out);
+/* value at a dotted path, e.g. "Solver.RelativeTolerance" (NUL-terminated
+ * into buf of `size` bytes) */
+int sil_silpro_value(sil_silpro h, const char* path, char* buf, int64_t size);
+/* index-th value of a section, in key order (e.g. the tags of
+ * "Preprocess.MagneticMaterialTags") */
+int sil_silpro_section_value(sil_silpro h, const char* section, int32_t index, char* buf, int64_t size);
+int sil_silpro_destroy(sil_silpro h);
+
+enum sil_physics {
+    SIL_PHYSICS_LINEAR_MAGNETOSTATICS = 0,
+    SIL_PHYSICS_NONLINEAR_MAGNETOSTATICS = 1,
+    SIL_PHYSICS_SCALAR_FIELD_WITH_POWER_COUPLING = 2
+};
+
+/* PreconditionerType (src/similie/solvers/minimize_strong_formulation_residual.hpp:54-63) */
+enum sil_silpro_preconditioner {
+    SIL_SILPRO_PRECOND_IDENTITY = 0,
+    SIL_SILPRO_PRECOND_JACOBI = 1,
+    SIL_SILPRO_PRECOND_SPD_ISAI = 2,
+    SIL_SILPRO_PRECOND_SYMMETRIC_GAUSS_SEIDEL = 3,
+    SIL_SILPRO_PRECOND_SSOR = 4,
+    SIL_SILPRO_PRECOND_CHEBYSHEV_JACOBI = 5,
+    SIL_SILPRO_PRECOND_IR_JACOBI = 6,
+    SIL_SILPRO_PRECOND_GENERAL_ISAI = 7
+};
+
+/* the Problem / Solver sections of a .silpro file: MinimizeStrongFormulation-
+ * ResidualProblem (onelab_interface.hpp:96-111) with its defaults */
+typedef struct sil_silpro_settings {
+    int32_t physics;                 /* enum sil_physics */
+    int64_t max_iterations;          /* 10000 */
+    double relative_tolerance;       /* 1e-10 */
+    int32_t jacobi_max_block_size;   /* 1 */
+    int32_t use_matrix_free;         /* 1 */
+    int32_t criterion;               /* 0 potential, 1 moments (default), 2 both temporal derivatives */
+    int32_t preconditioner;          /* enum sil_silpro_preconditioner, Jacobi */
+    double vector_potential_gauge_penalty; /* 20 */
+    double sor_relaxation_factor;    /* 1.2 */
+    int32_t chebyshev_iterations;    /* 8 */
+    double chebyshev_lower_bound;    /* 0.05 */
+    double chebyshev_upper_bound;    /* 1.5 */
+    int32_t ir_iterations;           /* 8 */
+    double ir_relaxation_factor;     /* 1 */
+} sil_silpro_settings;
+
+int sil_silpro_problem_settings(sil_silpro h, sil_silpro_settings* out);
+/* the solver settings of this library for a parsed problem; the
+ * preconditioners not implemented here (Gauss-Seidel, SSOR, Chebyshev, IR,
+ * GeneralIsai, block Jacobi) are an error */
+int sil_silpro_solver_settings(const sil_silpro_settings* in, sil_solver_settings* out);
+
+/* the cell inputs of the magnetostatics preprocess
+ * (magnetostatics_onelab.hpp:4870-4897): μ0, no current and linear by
+ * default; magnetic-material tags take core_mu (and the nonlinear flag when
+ * enabled); otherwise ±x and then ±z conductor tags take ±current density. */
+typedef struct sil_cell_materials {
+    double mu0, core_mu, current_density_x, current_density_z;
+    int32_t use_nonlinear_magnetic_material;
+    const int32_t* magnetic_material_tags;
+    int32_t n_magnetic_material_tags;
+    const int32_t* positive_x_tags;
+    int32_t n_positive_x_tags;
+    const int32_t* negative_x_tags;
+    int32_t n_negative_x_tags;
+    const int32_t* positive_z_tags;
+    int32_t n_positive_z_tags;
+    const int32_t* negative_z_tags;
+    int32_t n_negative_z_tags;
+} sil_cell_materials;
+
+/* host arrays: cell_mu[ncells], cell_current[ncells][3], nonlinear_flags[ncells] (0/1) */
+int sil_magnetostatics_cell_inputs(const int32_t* cell_tags, int64_t ncells, const sil_cell_materials* materials,
+                                   double* cell_mu, double* cell_current, double* nonlinear_flags);
+
+/* ------------------------------------------------------------------ */
 /* batched persistent solves                                           */
 /* ------------------------------------------------------------------ */
 

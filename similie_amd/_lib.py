@@ -62,6 +62,44 @@ class SolverDiagnostics(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class SilproSettings(ctypes.Structure):
+    _fields_ = [
+        ("physics", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int64),
+        ("relative_tolerance", ctypes.c_double),
+        ("jacobi_max_block_size", ctypes.c_int32),
+        ("use_matrix_free", ctypes.c_int32),
+        ("criterion", ctypes.c_int32),
+        ("preconditioner", ctypes.c_int32),
+        ("vector_potential_gauge_penalty", ctypes.c_double),
+        ("sor_relaxation_factor", ctypes.c_double),
+        ("chebyshev_iterations", ctypes.c_int32),
+        ("chebyshev_lower_bound", ctypes.c_double),
+        ("chebyshev_upper_bound", ctypes.c_double),
+        ("ir_iterations", ctypes.c_int32),
+        ("ir_relaxation_factor", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_I32P = ctypes.POINTER(ctypes.c_int32)
+
+
+class CellMaterials(ctypes.Structure):
+    _fields_ = [
+        ("mu0", ctypes.c_double), ("core_mu", ctypes.c_double),
+        ("current_density_x", ctypes.c_double), ("current_density_z", ctypes.c_double),
+        ("use_nonlinear_magnetic_material", ctypes.c_int32),
+        ("magnetic_material_tags", _I32P), ("n_magnetic_material_tags", ctypes.c_int32),
+        ("positive_x_tags", _I32P), ("n_positive_x_tags", ctypes.c_int32),
+        ("negative_x_tags", _I32P), ("n_negative_x_tags", ctypes.c_int32),
+        ("positive_z_tags", _I32P), ("n_positive_z_tags", ctypes.c_int32),
+        ("negative_z_tags", _I32P), ("n_negative_z_tags", ctypes.c_int32),
+    ]
+
+
 class BatchResult(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int32),
@@ -114,6 +152,21 @@ SIGNATURES = {
     "sil_csr_nnz": (_I64, [_P]),
     "sil_csr_copy": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "sil_spd_isai_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), _P]),
+    "sil_msh2_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "sil_msh2_parse": (ctypes.c_int, [ctypes.c_char_p, _I64, ctypes.POINTER(_P)]),
+    "sil_structured_grid_shape": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
+    "sil_structured_grid_coords": (ctypes.c_int, [_P, _I32, _P]),
+    "sil_structured_grid_plane": (ctypes.c_double, [_P]),
+    "sil_structured_grid_cell_tags": (ctypes.c_int, [_P, _P]),
+    "sil_structured_grid_destroy": (ctypes.c_int, [_P]),
+    "sil_silpro_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "sil_silpro_parse": (ctypes.c_int, [ctypes.c_char_p, _I64, ctypes.POINTER(_P)]),
+    "sil_silpro_value": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_char_p, _I64]),
+    "sil_silpro_section_value": (ctypes.c_int, [_P, ctypes.c_char_p, _I32, ctypes.c_char_p, _I64]),
+    "sil_silpro_problem_settings": (ctypes.c_int, [_P, ctypes.POINTER(SilproSettings)]),
+    "sil_silpro_solver_settings": (ctypes.c_int, [ctypes.POINTER(SilproSettings), ctypes.POINTER(SolverSettings)]),
+    "sil_silpro_destroy": (ctypes.c_int, [_P]),
+    "sil_magnetostatics_cell_inputs": (ctypes.c_int, [_P, _I64, ctypes.POINTER(CellMaterials), _P, _P, _P]),
     "sil_operator_size": (_I64, [_P]),
     "sil_operator_apply": (ctypes.c_int, [_P, _P, _P, _P]),
     "sil_operator_apply_f32": (ctypes.c_int, [_P, _P, _P, _P]),

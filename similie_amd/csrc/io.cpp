@@ -1,0 +1,761 @@
+// Input formats on the host: Gmsh MSH2 structured grids, .silpro problem
+// files, and the cell inputs of the magnetostatics preprocess.
+//
+// * MSH2 (onelab_interface/gmsh_structured_grid.hpp:239-553): ASCII version
+//   2 only; a mesh made entirely of quadrilaterals (type 3) or entirely of
+//   hexahedra (type 5), with points, lines and triangles (types 15, 1, 2)
+//   skipped as boundary elements;
+//   the nodes must form a full rectilinear grid (coordinates merged within
+//   1e-12) and every cell must be exactly one grid cell.  The reference
+//   reads through iostreams, keeps node tags in a std::map and locates every
+//   coordinate by a linear scan of the axis; here the file is read in one
+//   block and tokenised in place with std::from_chars, axes are located by
+//   binary search (keeping the reference's rule: the first axis value within
+//   the tolerance), and node tags index a dense table when they are dense.
+// * .silpro (onelab_interface/onelab_interface.hpp:160-560): the same lexer
+//   (# and // comments, quoted strings with backslash escapes, words, braces,
+//   semicolons) and tree (sections of `Key value;` entries; the first of
+//   repeated keys or sections wins), the solver settings with the defaults
+//   of MinimizeStrongFormulationResidualProblem (:96-111).
+// * Cell inputs (magnetostatics_onelab.hpp:4870-4897): μ, current density
+//   and the nonlinear-material flag of every cell from its physical tag.
+#include <algorithm>
+#include <charconv>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <map>
+#include <new>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+
+struct sil_structured_grid_s {
+    int ndim = 0;
+    std::vector<double> coords[3];
+    double z_value = 0.0;
+    std::vector<int32_t> cell_tags; // x fastest
+};
+
+namespace sil {
+struct SilproSection {
+    std::map<std::string, std::string> values;
+    std::map<std::string, SilproSection> sections;
+};
+}  // namespace sil
+
+struct sil_silpro_s {
+    sil::SilproSection root;
+};
+
+namespace sil {
+namespace {
+
+constexpr double kCoordTol = 1.0e-12;
+
+struct ParseError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// ------------------------------------------------------------------ MSH2 ---
+
+class Tokens {
+  public:
+    Tokens(const char* b, const char* e) : p_(b), e_(e) {}
+    bool next(std::string_view& t)
+    {
+        while (p_ < e_ && is_space(*p_)) ++p_;
+        if (p_ >= e_) return false;
+        const char* s = p_;
+        while (p_ < e_ && !is_space(*p_)) ++p_;
+        t = std::string_view(s, static_cast<size_t>(p_ - s));
+        return true;
+    }
+    template <typename T>
+    T number()
+    {
+        std::string_view t;
+        if (!next(t)) throw ParseError("unexpected end of mesh file");
+        if (t.size() > 1 && t[0] == '+') t.remove_prefix(1); // accepted by iostream extraction
+        T v{};
+        auto r = std::from_chars(t.data(), t.data() + t.size(), v);
+        if (r.ec != std::errc() || r.ptr != t.data() + t.size())
+            throw ParseError("invalid number '" + std::string(t) + "' in mesh file");
+        return v;
+    }
+
+  private:
+    static bool is_space(char c) { return c == ' ' || c == '\n' || c == '\r' || c == '\t' || c == '\v' || c == '\f'; }
+    const char* p_;
+    const char* e_;
+};
+
+int element_dimension(int type)
+{
+    switch (type) {
+    case 15: return 0;
+    case 1: return 1;
+    case 2:
+    case 3: return 2;
+    case 4:
+    case 5:
+    case 6:
+    case 7: return 3;
+    default: return -1;
+    }
+}
+
+int nodes_per_element(int type)
+{
+    switch (type) {
+    case 1: return 2;
+    case 2: return 3;
+    case 3: return 4;
+    case 4: return 4;
+    case 5: return 8;
+    case 6: return 6;
+    case 7: return 5;
+    case 15: return 1;
+    default: return 0;
+    }
+}
+
+bool near(double a, double b) { return std::abs(a - b) < kCoordTol; }
+
+std::vector<double> unique_sorted(std::vector<double> v)
+{
+    std::sort(v.begin(), v.end());
+    std::vector<double> out;
+    out.reserve(v.size());
+    for (double x : v)
+        if (out.empty() || !near(out.back(), x)) out.push_back(x);
+    return out;
+}
+
+// first axis index whose value is within the tolerance of x
+size_t axis_index(const std::vector<double>& axis, double x)
+{
+    auto it = std::lower_bound(axis.begin(), axis.end(), x - kCoordTol);
+    size_t i = static_cast<size_t>(it - axis.begin());
+    // the first within tolerance is at i or i + 1 (a value below x − tol is not)
+    for (size_t k = (i > 0 ? i - 1 : 0); k < std::min(axis.size(), i + 2); ++k)
+        if (near(axis[k], x)) return k;
+    throw ParseError("coordinate does not belong to the detected structured grid");
+}
+
+struct Mesh {
+    std::vector<int64_t> node_tag;
+    std::vector<double> x, y, z;
+    int cell_dim = 0;              // 2 (quadrilaterals) or 3 (hexahedra)
+    std::vector<int32_t> cell_tag; // physical tag per cell
+    std::vector<int64_t> cell_nodes;
+};
+
+Mesh parse_msh2(const char* b, const char* e)
+{
+    Tokens tk(b, e);
+    Mesh m;
+    std::string_view t;
+    bool saw_elements = false;
+    while (tk.next(t)) {
+        if (t == "$MeshFormat") {
+            std::string_view v;
+            if (!tk.next(v)) throw ParseError("unexpected end of mesh file");
+            double version = 0.0;
+            auto r = std::from_chars(v.data(), v.data() + v.size(), version);
+            if (r.ec != std::errc()) throw ParseError("invalid mesh format version");
+            const int file_type = tk.number<int>();
+            (void)tk.number<int>(); // data size
+            if (version < 2.0 || version >= 3.0 || file_type != 0)
+                throw ParseError("only Gmsh msh2 ASCII meshes are supported");
+        } else if (t == "$Nodes") {
+            const int64_t count = tk.number<int64_t>();
+            m.node_tag.resize(count);
+            m.x.resize(count);
+            m.y.resize(count);
+            m.z.resize(count);
+            for (int64_t i = 0; i < count; ++i) {
+                m.node_tag[i] = tk.number<int64_t>();
+                m.x[i] = tk.number<double>();
+                m.y[i] = tk.number<double>();
+                m.z[i] = tk.number<double>();
+            }
+        } else if (t == "$Elements") {
+            saw_elements = true;
+            const int64_t count = tk.number<int64_t>();
+            for (int64_t i = 0; i < count; ++i) {
+                (void)tk.number<int64_t>(); // element tag
+                const int type = tk.number<int>();
+                const int ntags = tk.number<int>();
+                int32_t physical = 0;
+                for (int q = 0; q < ntags; ++q) {
+                    const int32_t v = tk.number<int32_t>();
+                    if (q == 0) physical = v;
+                }
+                if (element_dimension(type) < 0)
+                    throw ParseError("unsupported Gmsh element type " + std::to_string(type) + " found in mesh");
+                const int nn = nodes_per_element(type);
+                if (type == 3 || type == 5) {
+                    // every quadrilateral is a cell: a mesh mixing them with
+                    // hexahedra is rejected, as the reference does
+                    const int dim = type == 3 ? 2 : 3;
+                    if (m.cell_dim == 0) m.cell_dim = dim;
+                    if (m.cell_dim != dim)
+                        throw ParseError("unsupported mesh topology: SimiLie currently requires the whole mesh to be "
+                                         "made of quadrilaterals or hexahedra");
+                    m.cell_tag.push_back(physical);
+                    for (int q = 0; q < nn; ++q) m.cell_nodes.push_back(tk.number<int64_t>());
+                    continue;
+                }
+                if (type == 1 || type == 2 || type == 15) {
+                    for (int q = 0; q < nn; ++q) (void)tk.number<int64_t>();
+                    continue;
+                }
+                throw ParseError("unsupported mesh topology: SimiLie currently requires the whole mesh to be made "
+                                 "of quadrilaterals or hexahedra");
+            }
+        }
+    }
+    if (m.node_tag.empty()) throw ParseError("the provided mesh does not contain any node");
+    if (!saw_elements || m.cell_tag.empty())
+        throw ParseError("unsupported mesh topology: SimiLie currently requires the whole mesh to be made of "
+                         "quadrilaterals or hexahedra");
+    return m;
+}
+
+sil_structured_grid_s* build_grid(const Mesh& m)
+{
+    const int nd = m.cell_dim;
+    const size_t nnodes = m.node_tag.size();
+    auto* g = new sil_structured_grid_s();
+    g->ndim = nd;
+    const std::vector<double>* src[3] = {&m.x, &m.y, &m.z};
+    for (int a = 0; a < 3; ++a) g->coords[a] = unique_sorted(*src[a]);
+    if (nd == 2) {
+        if (g->coords[2].size() != 1) {
+            delete g;
+            throw ParseError("the quadrilateral mesh nodes do not lie on a single plane");
+        }
+        g->z_value = g->coords[2][0];
+    }
+    const size_t n0 = g->coords[0].size(), n1 = g->coords[1].size(), n2 = nd == 3 ? g->coords[2].size() : 1;
+    const size_t total = n0 * n1 * n2;
+    // node tag → linear grid index (dense table when the tags are dense)
+    int64_t max_tag = 0, min_tag = std::numeric_limits<int64_t>::max();
+    for (int64_t t : m.node_tag) {
+        max_tag = std::max(max_tag, t);
+        min_tag = std::min(min_tag, t);
+    }
+    const bool dense = min_tag >= 0 && max_tag <= static_cast<int64_t>(4 * nnodes + 16);
+    std::vector<int64_t> by_tag_dense(dense ? static_cast<size_t>(max_tag + 1) : 0, -1);
+    std::unordered_map<int64_t, int64_t> by_tag;
+    if (!dense) by_tag.reserve(nnodes * 2);
+    std::vector<int64_t> owner(total, -1); // mesh node occupying each grid node
+    for (size_t q = 0; q < nnodes; ++q) {
+        const size_t i = axis_index(g->coords[0], m.x[q]);
+        const size_t j = axis_index(g->coords[1], m.y[q]);
+        const size_t k = nd == 3 ? axis_index(g->coords[2], m.z[q]) : 0;
+        const int64_t lin = static_cast<int64_t>(i + n0 * (j + n1 * k));
+        if (owner[lin] >= 0) {
+            const int64_t r = owner[lin];
+            if (!near(m.x[r], m.x[q]) || !near(m.y[r], m.y[q]) || !near(m.z[r], m.z[q])) {
+                delete g;
+                throw ParseError("duplicated nodes on the detected rectilinear grid");
+            }
+        } else {
+            owner[lin] = static_cast<int64_t>(q);
+        }
+        if (dense) {
+            if (by_tag_dense[m.node_tag[q]] < 0) by_tag_dense[m.node_tag[q]] = lin;
+        } else {
+            by_tag.emplace(m.node_tag[q], lin);
+        }
+    }
+    for (int64_t o : owner)
+        if (o < 0) {
+            delete g;
+            throw ParseError("the mesh nodes do not form a full rectilinear grid");
+        }
+    const size_t c0 = n0 - 1, c1 = n1 - 1, c2 = nd == 3 ? n2 - 1 : 1;
+    const size_t ncells = c0 * c1 * c2;
+    const char* what = nd == 3 ? "hexahedral" : "quadrilateral";
+    const char* one = nd == 3 ? "a hexahedron" : "a quadrilateral";
+    if (m.cell_tag.size() != ncells) {
+        delete g;
+        throw ParseError(std::string("the ") + what + " cells do not cover a full rectilinear grid");
+    }
+    g->cell_tags.assign(ncells, 0);
+    std::vector<char> used(ncells, 0);
+    const int per = nd == 3 ? 8 : 4;
+    for (size_t c = 0; c < ncells; ++c) {
+        size_t lo[3] = {SIZE_MAX, SIZE_MAX, SIZE_MAX}, hi[3] = {0, 0, 0};
+        for (int q = 0; q < per; ++q) {
+            const int64_t tag = m.cell_nodes[c * per + q];
+            int64_t lin = -1;
+            if (dense) {
+                if (tag >= 0 && tag <= max_tag) lin = by_tag_dense[tag];
+            } else {
+                auto it = by_tag.find(tag);
+                if (it != by_tag.end()) lin = it->second;
+            }
+            if (lin < 0) {
+                delete g;
+                throw ParseError(std::string(one) + " references an unknown node");
+            }
+            const size_t idx[3] = {static_cast<size_t>(lin) % n0, (static_cast<size_t>(lin) / n0) % n1,
+                                   static_cast<size_t>(lin) / (n0 * n1)};
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], idx[a]);
+                hi[a] = std::max(hi[a], idx[a]);
+            }
+        }
+        bool unit = hi[0] == lo[0] + 1 && hi[1] == lo[1] + 1 && (nd == 2 || hi[2] == lo[2] + 1);
+        if (!unit) {
+            delete g;
+            throw ParseError(std::string(one) + " does not match a single rectilinear grid cell");
+        }
+        const size_t lin = lo[0] + c0 * (lo[1] + c1 * (nd == 3 ? lo[2] : 0));
+        if (used[lin]) {
+            delete g;
+            throw ParseError(std::string("duplicated ") + (nd == 3 ? "hexahedron" : "quadrilateral") +
+                             " on the detected rectilinear grid");
+        }
+        used[lin] = 1;
+        g->cell_tags[lin] = m.cell_tag[c];
+    }
+    if (nd == 2) g->coords[2].clear();
+    return g;
+}
+
+int read_file(const char* path, std::string& out)
+{
+    std::ifstream f(path, std::ios::binary);
+    SIL_REQUIRE(f.is_open(), SIL_ERR_INVALID, "failed to open file: %s", path);
+    f.seekg(0, std::ios::end);
+    const std::streamoff size = f.tellg();
+    f.seekg(0, std::ios::beg);
+    out.resize(static_cast<size_t>(size));
+    if (size > 0) f.read(out.data(), size);
+    SIL_REQUIRE(f.good() || size == 0, SIL_ERR_INVALID, "failed to read file: %s", path);
+    return SIL_OK;
+}
+
+int grid_from_text(const char* b, const char* e, sil_structured_grid* out)
+{
+    try {
+        *out = build_grid(parse_msh2(b, e));
+        return SIL_OK;
+    } catch (const std::bad_alloc&) {
+        set_error("msh2: out of host memory");
+        return SIL_ERR_INVALID;
+    } catch (const std::exception& ex) {
+        set_error("%s", ex.what());
+        return SIL_ERR_INVALID;
+    }
+}
+
+// ---------------------------------------------------------------- silpro ---
+
+enum TokKind { T_WORD, T_STRING, T_LBRACE, T_RBRACE, T_SEMI };
+struct Tok {
+    TokKind kind;
+    std::string text;
+};
+
+std::vector<Tok> lex_silpro(const std::string& s)
+{
+    std::vector<Tok> toks;
+    size_t i = 0;
+    auto space = [](char c) { return std::isspace(static_cast<unsigned char>(c)) != 0; };
+    while (i < s.size()) {
+        const char c = s[i];
+        if (space(c)) {
+            ++i;
+        } else if (c == '#' || (c == '/' && i + 1 < s.size() && s[i + 1] == '/')) {
+            while (i < s.size() && s[i] != '\n') ++i;
+        } else if (c == '{') {
+            toks.push_back({T_LBRACE, "{"});
+            ++i;
+        } else if (c == '}') {
+            toks.push_back({T_RBRACE, "}"});
+            ++i;
+        } else if (c == ';') {
+            toks.push_back({T_SEMI, ";"});
+            ++i;
+        } else if (c == '"') {
+            ++i;
+            std::string text;
+            while (i < s.size() && s[i] != '"') {
+                if (s[i] == '\\' && i + 1 < s.size()) ++i;
+                text.push_back(s[i]);
+                ++i;
+            }
+            if (i == s.size()) throw ParseError("unterminated string literal in .silpro file");
+            ++i;
+            toks.push_back({T_STRING, text});
+        } else {
+            std::string text;
+            while (i < s.size() && !space(s[i]) && s[i] != '{' && s[i] != '}' && s[i] != ';' &&
+                   !(s[i] == '/' && i + 1 < s.size() && s[i + 1] == '/'))
+                text.push_back(s[i++]);
+            if (!text.empty()) toks.push_back({T_WORD, text});
+        }
+    }
+    return toks;
+}
+
+class SilproParser {
+  public:
+    explicit SilproParser(const std::vector<Tok>& t) : t_(t) {}
+    SilproSection document()
+    {
+        SilproSection root;
+        while (i_ < t_.size()) {
+            std::string name = name_token();
+            root.sections.emplace(name, body());
+        }
+        return root;
+    }
+
+  private:
+    const Tok& peek() const
+    {
+        if (i_ >= t_.size()) throw ParseError("unexpected end of .silpro file");
+        return t_[i_];
+    }
+    std::string name_token()
+    {
+        const Tok& t = peek();
+        if (t.kind != T_WORD && t.kind != T_STRING) throw ParseError("expected identifier in .silpro file");
+        ++i_;
+        return t.text;
+    }
+    void expect(TokKind k, const char* msg)
+    {
+        if (peek().kind != k) throw ParseError(msg);
+        ++i_;
+    }
+    SilproSection body()
+    {
+        expect(T_LBRACE, "expected '{' after section name in .silpro file");
+        SilproSection s;
+        while (peek().kind != T_RBRACE) {
+            std::string name = name_token();
+            if (peek().kind == T_LBRACE) {
+                s.sections.emplace(name, body());
+                continue;
+            }
+            std::string value = name_token();
+            expect(T_SEMI, "expected ';' after assignment in .silpro file");
+            s.values.emplace(name, value);
+        }
+        expect(T_RBRACE, "expected '}' in .silpro file");
+        return s;
+    }
+    const std::vector<Tok>& t_;
+    size_t i_ = 0;
+};
+
+// section at a dotted path ("Preprocess.MagneticMaterialTags"); null if absent
+const SilproSection* find_section(const SilproSection& root, const std::string& path)
+{
+    const SilproSection* s = &root;
+    size_t b = 0;
+    while (b <= path.size() && !path.empty()) {
+        const size_t e = path.find('.', b);
+        const std::string part = path.substr(b, e == std::string::npos ? std::string::npos : e - b);
+        auto it = s->sections.find(part);
+        if (it == s->sections.end()) return nullptr;
+        s = &it->second;
+        if (e == std::string::npos) break;
+        b = e + 1;
+    }
+    return s;
+}
+
+const std::string* find_value(const SilproSection& root, const std::string& path)
+{
+    const size_t dot = path.rfind('.');
+    const SilproSection* s = dot == std::string::npos ? &root : find_section(root, path.substr(0, dot));
+    if (!s) return nullptr;
+    auto it = s->values.find(dot == std::string::npos ? path : path.substr(dot + 1));
+    return it == s->values.end() ? nullptr : &it->second;
+}
+
+std::string trim_lower(std::string v)
+{
+    auto sp = [](unsigned char c) { return std::isspace(c) != 0; };
+    while (!v.empty() && sp(v.back())) v.pop_back();
+    size_t b = 0;
+    while (b < v.size() && sp(v[b])) ++b;
+    v = v.substr(b);
+    for (char& c : v) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+    return v;
+}
+
+bool parse_bool(const std::string& v)
+{
+    const std::string n = trim_lower(v);
+    if (n == "1" || n == "true" || n == "yes" || n == "on") return true;
+    if (n == "0" || n == "false" || n == "no" || n == "off") return false;
+    throw ParseError("invalid boolean value '" + v + "'");
+}
+
+template <typename T>
+T parse_number(const std::string& v)
+{
+    std::istringstream s(v);
+    T x{};
+    s >> x;
+    if (!s || !s.eof()) throw ParseError("invalid numeric value '" + v + "'");
+    return x;
+}
+
+int parse_preconditioner(const std::string& n)
+{
+    if (n == "Identity" || n == "identity") return SIL_SILPRO_PRECOND_IDENTITY;
+    if (n == "Jacobi" || n == "jacobi") return SIL_SILPRO_PRECOND_JACOBI;
+    if (n == "SpdIsai" || n == "spd-isai" || n == "spd_isai") return SIL_SILPRO_PRECOND_SPD_ISAI;
+    if (n == "SymmetricGaussSeidel" || n == "symmetric-gauss-seidel" || n == "symmetric_gauss_seidel")
+        return SIL_SILPRO_PRECOND_SYMMETRIC_GAUSS_SEIDEL;
+    if (n == "Ssor" || n == "SSOR" || n == "ssor") return SIL_SILPRO_PRECOND_SSOR;
+    if (n == "ChebyshevJacobi" || n == "chebyshev-jacobi" || n == "chebyshev_jacobi")
+        return SIL_SILPRO_PRECOND_CHEBYSHEV_JACOBI;
+    if (n == "IrJacobi" || n == "ir-jacobi" || n == "ir_jacobi") return SIL_SILPRO_PRECOND_IR_JACOBI;
+    if (n == "GeneralIsai" || n == "general-isai" || n == "general_isai" || n == "isai")
+        return SIL_SILPRO_PRECOND_GENERAL_ISAI;
+    throw ParseError("unknown strong-formulation preconditioner: " + n);
+}
+
+int parse_physics(const std::string& v)
+{
+    if (v == "LinearMagnetostatics") return SIL_PHYSICS_LINEAR_MAGNETOSTATICS;
+    if (v == "NonLinearMagnetostatics") return SIL_PHYSICS_NONLINEAR_MAGNETOSTATICS;
+    if (v == "ScalarFieldWithPowerCoupling") return SIL_PHYSICS_SCALAR_FIELD_WITH_POWER_COUPLING;
+    throw ParseError("unsupported physics '" + v + "' in .silpro file");
+}
+
+int parse_criterion(const std::string& v)
+{
+    if (v == "PotentialTemporalDerivative") return 0;
+    if (v == "MomentsTemporalDerivative") return 1;
+    if (v == "PotentialAndMomentsTemporalDerivative") return 2;
+    throw ParseError("unsupported solver criterion '" + v + "' in .silpro file");
+}
+
+int copy_out(const std::string& v, char* buf, int64_t size)
+{
+    SIL_REQUIRE(buf && size > 0, SIL_ERR_INVALID, "silpro: no output buffer");
+    SIL_REQUIRE(static_cast<int64_t>(v.size()) < size, SIL_ERR_INVALID, "silpro: value of %zu bytes does not fit",
+                v.size());
+    std::memcpy(buf, v.c_str(), v.size() + 1);
+    return SIL_OK;
+}
+
+}  // namespace
+}  // namespace sil
+
+using namespace sil;
+
+extern "C" {
+
+int sil_msh2_read(const char* path, sil_structured_grid* out)
+{
+    SIL_REQUIRE(path && out, SIL_ERR_INVALID, "msh2_read: null argument");
+    std::string text;
+    int rc = read_file(path, text);
+    if (rc) return rc;
+    return grid_from_text(text.data(), text.data() + text.size(), out);
+}
+
+int sil_msh2_parse(const char* text, int64_t length, sil_structured_grid* out)
+{
+    SIL_REQUIRE(text && out && length >= 0, SIL_ERR_INVALID, "msh2_parse: null argument");
+    return grid_from_text(text, text + length, out);
+}
+
+int sil_structured_grid_shape(sil_structured_grid g, int32_t* ndim, int64_t* nodes)
+{
+    SIL_REQUIRE(g && ndim && nodes, SIL_ERR_INVALID, "structured_grid_shape: null argument");
+    *ndim = g->ndim;
+    for (int a = 0; a < 3; ++a) nodes[a] = a < g->ndim ? static_cast<int64_t>(g->coords[a].size()) : 1;
+    return SIL_OK;
+}
+
+int sil_structured_grid_coords(sil_structured_grid g, int32_t axis, double* out)
+{
+    SIL_REQUIRE(g && out && axis >= 0 && axis < g->ndim, SIL_ERR_INVALID, "structured_grid_coords: bad argument");
+    std::copy(g->coords[axis].begin(), g->coords[axis].end(), out);
+    return SIL_OK;
+}
+
+double sil_structured_grid_plane(sil_structured_grid g) { return g ? g->z_value : 0.0; }
+
+int sil_structured_grid_cell_tags(sil_structured_grid g, int32_t* out)
+{
+    SIL_REQUIRE(g && out, SIL_ERR_INVALID, "structured_grid_cell_tags: null argument");
+    std::copy(g->cell_tags.begin(), g->cell_tags.end(), out);
+    return SIL_OK;
+}
+
+int sil_structured_grid_destroy(sil_structured_grid g)
+{
+    delete g;
+    return SIL_OK;
+}
+
+int sil_silpro_parse(const char* text, int64_t length, sil_silpro* out)
+{
+    SIL_REQUIRE(text && out && length >= 0, SIL_ERR_INVALID, "silpro_parse: null argument");
+    try {
+        auto* h = new sil_silpro_s();
+        try {
+            h->root = SilproParser(lex_silpro(std::string(text, static_cast<size_t>(length)))).document();
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+        return SIL_OK;
+    } catch (const std::exception& ex) {
+        set_error("%s", ex.what());
+        return SIL_ERR_INVALID;
+    }
+}
+
+int sil_silpro_read(const char* path, sil_silpro* out)
+{
+    SIL_REQUIRE(path && out, SIL_ERR_INVALID, "silpro_read: null argument");
+    std::string text;
+    int rc = read_file(path, text);
+    if (rc) return rc;
+    return sil_silpro_parse(text.data(), static_cast<int64_t>(text.size()), out);
+}
+
+int sil_silpro_value(sil_silpro h, const char* path, char* buf, int64_t size)
+{
+    SIL_REQUIRE(h && path, SIL_ERR_INVALID, "silpro_value: null argument");
+    const std::string* v = find_value(h->root, path);
+    SIL_REQUIRE(v != nullptr, SIL_ERR_INVALID, "silpro: no value '%s'", path);
+    return copy_out(*v, buf, size);
+}
+
+int sil_silpro_section_value(sil_silpro h, const char* section, int32_t index, char* buf, int64_t size)
+{
+    SIL_REQUIRE(h && section && index >= 0, SIL_ERR_INVALID, "silpro_section_value: bad argument");
+    const SilproSection* s = find_section(h->root, section);
+    SIL_REQUIRE(s != nullptr, SIL_ERR_INVALID, "silpro: no section '%s'", section);
+    SIL_REQUIRE(index < static_cast<int32_t>(s->values.size()), SIL_ERR_INVALID, "silpro: '%s' has %zu values",
+                section, s->values.size());
+    auto it = s->values.begin();
+    std::advance(it, index);
+    return copy_out(it->second, buf, size);
+}
+
+int sil_silpro_problem_settings(sil_silpro h, sil_silpro_settings* out)
+{
+    SIL_REQUIRE(h && out, SIL_ERR_INVALID, "silpro_problem_settings: null argument");
+    try {
+        const SilproSection* problem = find_section(h->root, "Problem");
+        const SilproSection* solver = find_section(h->root, "Solver");
+        if (!problem) throw ParseError("missing section 'Problem' in .silpro file");
+        if (!solver) throw ParseError("missing section 'Solver' in .silpro file");
+        auto get = [](const SilproSection* s, const char* key, const std::string& dflt) {
+            auto it = s->values.find(key);
+            return it == s->values.end() ? dflt : it->second;
+        };
+        sil_silpro_settings st{};
+        st.physics = parse_physics(get(problem, "Physics", "LinearMagnetostatics"));
+        if (get(problem, "Solver", "MinimizeStrongFormulationResidual") != "MinimizeStrongFormulationResidual")
+            throw ParseError("unsupported solver '" + get(problem, "Solver", "") + "' in .silpro file");
+        // MinimizeStrongFormulationResidualProblem defaults (onelab_interface.hpp:96-111)
+        st.max_iterations = static_cast<int64_t>(parse_number<unsigned int>(get(solver, "MaxIterations", "10000")));
+        st.relative_tolerance = parse_number<double>(get(solver, "RelativeTolerance", "1e-10"));
+        st.jacobi_max_block_size = static_cast<int32_t>(parse_number<unsigned int>(get(solver, "JacobiMaxBlockSize", "1")));
+        st.use_matrix_free = parse_bool(get(solver, "UseMatrixFree", "1")) ? 1 : 0;
+        st.criterion = parse_criterion(get(solver, "Criterion", "MomentsTemporalDerivative"));
+        st.preconditioner = parse_preconditioner(get(solver, "Preconditioner", "Jacobi"));
+        st.vector_potential_gauge_penalty = parse_number<double>(get(solver, "VectorPotentialGaugePenalty", "20"));
+        st.sor_relaxation_factor = parse_number<double>(get(solver, "SorRelaxationFactor", "1.2"));
+        st.chebyshev_iterations = static_cast<int32_t>(parse_number<unsigned int>(get(solver, "ChebyshevIterations", "8")));
+        st.chebyshev_lower_bound = parse_number<double>(get(solver, "ChebyshevLowerBound", "0.05"));
+        st.chebyshev_upper_bound = parse_number<double>(get(solver, "ChebyshevUpperBound", "1.5"));
+        st.ir_iterations = static_cast<int32_t>(parse_number<unsigned int>(get(solver, "IrIterations", "8")));
+        st.ir_relaxation_factor = parse_number<double>(get(solver, "IrRelaxationFactor", "1"));
+        *out = st;
+        return SIL_OK;
+    } catch (const std::exception& ex) {
+        set_error("%s", ex.what());
+        return SIL_ERR_INVALID;
+    }
+}
+
+int sil_silpro_solver_settings(const sil_silpro_settings* in, sil_solver_settings* out)
+{
+    SIL_REQUIRE(in && out, SIL_ERR_INVALID, "silpro_solver_settings: null argument");
+    SIL_REQUIRE(in->max_iterations <= INT32_MAX, SIL_ERR_INVALID, "MaxIterations %lld exceeds the int32 range",
+                (long long)in->max_iterations);
+    SIL_REQUIRE(in->jacobi_max_block_size == 1, SIL_ERR_UNSUPPORTED,
+                "JacobiMaxBlockSize %d: only scalar Jacobi (block size 1) is implemented", in->jacobi_max_block_size);
+    int pc;
+    switch (in->preconditioner) {
+    case SIL_SILPRO_PRECOND_IDENTITY: pc = SIL_PRECOND_IDENTITY; break;
+    case SIL_SILPRO_PRECOND_JACOBI: pc = SIL_PRECOND_JACOBI; break;
+    case SIL_SILPRO_PRECOND_SPD_ISAI: pc = SIL_PRECOND_SPD_ISAI; break;
+    default:
+        set_error("preconditioner %d (GaussSeidel / SSOR / Chebyshev / IR / GeneralIsai) is not implemented",
+                  in->preconditioner);
+        return SIL_ERR_UNSUPPORTED;
+    }
+    sil_default_solver_settings(out);
+    out->max_iterations = static_cast<int32_t>(in->max_iterations);
+    out->relative_tolerance = in->relative_tolerance;
+    out->preconditioner = pc;
+    return SIL_OK;
+}
+
+int sil_silpro_destroy(sil_silpro h)
+{
+    delete h;
+    return SIL_OK;
+}
+
+int sil_magnetostatics_cell_inputs(const int32_t* cell_tags, int64_t ncells, const sil_cell_materials* mat,
+                                   double* cell_mu, double* cell_current, double* nonlinear_flags)
+{
+    SIL_REQUIRE(cell_tags && mat && cell_mu && cell_current && nonlinear_flags, SIL_ERR_INVALID,
+                "magnetostatics_cell_inputs: null argument");
+    auto has = [](const int32_t* tags, int32_t n, int32_t t) {
+        for (int32_t i = 0; i < n; ++i)
+            if (tags[i] == t) return true;
+        return false;
+    };
+    for (int64_t c = 0; c < ncells; ++c) {
+        const int32_t t = cell_tags[c];
+        double mu = mat->mu0, j[3] = {0.0, 0.0, 0.0}, nl = 0.0;
+        if (has(mat->magnetic_material_tags, mat->n_magnetic_material_tags, t)) {
+            mu = mat->core_mu;
+            nl = mat->use_nonlinear_magnetic_material ? 1.0 : 0.0;
+        } else if (has(mat->positive_x_tags, mat->n_positive_x_tags, t)) {
+            j[0] = mat->current_density_x;
+        } else if (has(mat->negative_x_tags, mat->n_negative_x_tags, t)) {
+            j[0] = -mat->current_density_x;
+        } else if (has(mat->positive_z_tags, mat->n_positive_z_tags, t)) {
+            j[2] = mat->current_density_z;
+        } else if (has(mat->negative_z_tags, mat->n_negative_z_tags, t)) {
+            j[2] = -mat->current_density_z;
+        }
+        cell_mu[c] = mu;
+        for (int a = 0; a < 3; ++a) cell_current[3 * c + a] = j[a];
+        nonlinear_flags[c] = nl;
+    }
+    return SIL_OK;
+}
+
+}  // extern "C"
