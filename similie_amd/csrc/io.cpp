@@ -31,7 +31,9 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "common.hpp"
@@ -77,6 +79,8 @@ class Tokens {
         t = std::string_view(s, static_cast<size_t>(p_ - s));
         return true;
     }
+    const char* pos() const { return p_; }
+    void seek(const char* p) { p_ = p; }
     template <typename T>
     T number()
     {
@@ -128,12 +132,18 @@ int nodes_per_element(int type)
 
 bool near(double a, double b) { return std::abs(a - b) < kCoordTol; }
 
-std::vector<double> unique_sorted(std::vector<double> v)
+// sorted values merged within the tolerance (consecutive values closer than
+// it to the last kept one are dropped); exact repeats are removed first
+// through a hash set — a grid axis repeats each value nx·ny times — which
+// leaves the merge unchanged
+std::vector<double> unique_sorted(const std::vector<double>& v)
 {
-    std::sort(v.begin(), v.end());
+    std::unordered_set<double> distinct(v.begin(), v.end());
+    std::vector<double> s(distinct.begin(), distinct.end());
+    std::sort(s.begin(), s.end());
     std::vector<double> out;
-    out.reserve(v.size());
-    for (double x : v)
+    out.reserve(s.size());
+    for (double x : s)
         if (out.empty() || !near(out.back(), x)) out.push_back(x);
     return out;
 }
@@ -157,6 +167,198 @@ struct Mesh {
     std::vector<int64_t> cell_nodes;
 };
 
+// ----- parallel reading of the node and element sections -----
+//
+// Gmsh writes one node or element per line; the sections are split into
+// line-aligned chunks read by one thread each and merged in order.  Anything
+// irregular (a record over several lines, a count that does not match, an
+// element the fast path does not handle) makes the caller read the section
+// serially instead, so results and errors are those of the serial reader.
+
+int reader_threads()
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    return static_cast<int>(std::max(1u, std::min(hw == 0 ? 1u : hw, 16u)));
+}
+
+// chunk starts: line starts of [b, e), at most T + 1 entries ending with e
+std::vector<const char*> line_chunks(const char* b, const char* e, int T)
+{
+    std::vector<const char*> cuts{b};
+    const size_t len = static_cast<size_t>(e - b);
+    for (int t = 1; t < T; ++t) {
+        const char* p = b + len * t / T;
+        if (p <= cuts.back()) continue;
+        const void* nl = std::memchr(p, '\n', static_cast<size_t>(e - p));
+        if (!nl) break;
+        p = static_cast<const char*>(nl) + 1;
+        if (p > cuts.back() && p < e) cuts.push_back(p);
+    }
+    cuts.push_back(e);
+    return cuts;
+}
+
+template <class F>
+void run_threads(int n, F f)
+{
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int t = 0; t < n; ++t) th.emplace_back(f, t);
+    for (auto& x : th) x.join();
+}
+
+// number of non-blank lines of [b, e)
+int64_t count_records(const char* b, const char* e)
+{
+    int64_t n = 0;
+    bool content = false;
+    for (const char* p = b; p < e; ++p) {
+        if (*p == '\n') {
+            n += content;
+            content = false;
+        } else if (!(*p == ' ' || *p == '\r' || *p == '\t')) {
+            content = true;
+        }
+    }
+    return n + content;
+}
+
+const char* find_marker(const char* b, const char* e, const char* marker)
+{
+    const size_t ml = std::strlen(marker);
+    for (const char* p = b; p + ml <= e; ++p) {
+        p = static_cast<const char*>(std::memchr(p, marker[0], static_cast<size_t>(e - p)));
+        if (!p || p + ml > e) return nullptr;
+        if (std::memcmp(p, marker, ml) == 0) return p;
+    }
+    return nullptr;
+}
+
+// the $Nodes body [b, end) holding `count` records; false: read serially
+bool fast_nodes(const char* b, const char* end, int64_t count, Mesh& m)
+{
+    const int T = reader_threads();
+    const auto cuts = line_chunks(b, end, T);
+    const int nc = static_cast<int>(cuts.size()) - 1;
+    std::vector<int64_t> n(nc + 1, 0);
+    run_threads(nc, [&](int t) { n[t + 1] = count_records(cuts[t], cuts[t + 1]); });
+    for (int t = 0; t < nc; ++t) n[t + 1] += n[t];
+    if (n[nc] != count) return false;
+    m.node_tag.resize(count);
+    m.x.resize(count);
+    m.y.resize(count);
+    m.z.resize(count);
+    std::vector<char> ok(nc, 1);
+    run_threads(nc, [&](int t) {
+        try {
+            const char* p = cuts[t];
+            int64_t r = n[t];
+            while (p < cuts[t + 1]) {
+                const void* nl = std::memchr(p, '\n', static_cast<size_t>(cuts[t + 1] - p));
+                const char* le = nl ? static_cast<const char*>(nl) : cuts[t + 1];
+                Tokens line(p, le);
+                std::string_view tok;
+                Tokens probe(p, le);
+                if (probe.next(tok)) {
+                    m.node_tag[r] = line.number<int64_t>();
+                    m.x[r] = line.number<double>();
+                    m.y[r] = line.number<double>();
+                    m.z[r] = line.number<double>();
+                    if (line.next(tok)) throw ParseError("extra token");
+                    ++r;
+                }
+                p = le + 1;
+            }
+        } catch (const std::exception&) {
+            ok[t] = 0;
+        }
+    });
+    return std::all_of(ok.begin(), ok.end(), [](char c) { return c != 0; });
+}
+
+struct ElementChunk {
+    std::vector<int32_t> tags;
+    std::vector<int64_t> nodes;
+    int dim = 0;
+    bool ok = true;
+};
+
+// the $Elements body: quadrilaterals or hexahedra plus points, lines and
+// triangles, one per line; false: read serially
+bool fast_elements(const char* b, const char* end, int64_t count, Mesh& m)
+{
+    const int T = reader_threads();
+    const auto cuts = line_chunks(b, end, T);
+    const int nc = static_cast<int>(cuts.size()) - 1;
+    std::vector<int64_t> n(nc, 0);
+    run_threads(nc, [&](int t) { n[t] = count_records(cuts[t], cuts[t + 1]); });
+    int64_t total = 0;
+    for (int64_t v : n) total += v;
+    if (total != count) return false;
+    std::vector<ElementChunk> ch(nc);
+    run_threads(nc, [&](int t) {
+        ElementChunk& c = ch[t];
+        try {
+            const char* p = cuts[t];
+            while (p < cuts[t + 1]) {
+                const void* nl = std::memchr(p, '\n', static_cast<size_t>(cuts[t + 1] - p));
+                const char* le = nl ? static_cast<const char*>(nl) : cuts[t + 1];
+                Tokens line(p, le);
+                std::string_view tok;
+                Tokens probe(p, le);
+                if (probe.next(tok)) {
+                    (void)line.number<int64_t>();
+                    const int type = line.number<int>();
+                    const int ntags = line.number<int>();
+                    int32_t physical = 0;
+                    for (int q = 0; q < ntags; ++q) {
+                        const int32_t v = line.number<int32_t>();
+                        if (q == 0) physical = v;
+                    }
+                    const int nn = nodes_per_element(type);
+                    if (type == 3 || type == 5) {
+                        const int dim = type == 3 ? 2 : 3;
+                        if (c.dim == 0) c.dim = dim;
+                        if (c.dim != dim) throw ParseError("mixed");
+                        c.tags.push_back(physical);
+                        for (int q = 0; q < nn; ++q) c.nodes.push_back(line.number<int64_t>());
+                    } else if (type == 1 || type == 2 || type == 15) {
+                        for (int q = 0; q < nn; ++q) (void)line.number<int64_t>();
+                    } else {
+                        throw ParseError("other");
+                    }
+                    if (line.next(tok)) throw ParseError("extra token");
+                }
+                p = le + 1;
+            }
+        } catch (const std::exception&) {
+            c.ok = false;
+        }
+    });
+    int dim = 0;
+    size_t ncell = 0;
+    for (const auto& c : ch) {
+        if (!c.ok) return false;
+        if (c.dim != 0) {
+            if (dim == 0) dim = c.dim;
+            if (dim != c.dim) return false;
+        }
+        ncell += c.tags.size();
+    }
+    if (m.cell_dim != 0 && dim != 0 && dim != m.cell_dim) return false;
+    if (dim != 0) m.cell_dim = dim;
+    const int per = dim == 3 ? 8 : 4;
+    size_t at = m.cell_tag.size();
+    m.cell_tag.resize(at + ncell);
+    m.cell_nodes.resize((at + ncell) * per);
+    for (const auto& c : ch) {
+        std::copy(c.tags.begin(), c.tags.end(), m.cell_tag.begin() + at);
+        std::copy(c.nodes.begin(), c.nodes.end(), m.cell_nodes.begin() + at * per);
+        at += c.tags.size();
+    }
+    return true;
+}
+
 Mesh parse_msh2(const char* b, const char* e)
 {
     Tokens tk(b, e);
@@ -176,6 +378,12 @@ Mesh parse_msh2(const char* b, const char* e)
                 throw ParseError("only Gmsh msh2 ASCII meshes are supported");
         } else if (t == "$Nodes") {
             const int64_t count = tk.number<int64_t>();
+            const char* body = tk.pos();
+            const char* stop = find_marker(body, e, "$EndNodes");
+            if (stop && count > 4096 && fast_nodes(body, stop, count, m)) {
+                tk.seek(stop);
+                continue;
+            }
             m.node_tag.resize(count);
             m.x.resize(count);
             m.y.resize(count);
@@ -189,6 +397,25 @@ Mesh parse_msh2(const char* b, const char* e)
         } else if (t == "$Elements") {
             saw_elements = true;
             const int64_t count = tk.number<int64_t>();
+            const char* body = tk.pos();
+            const char* stop = find_marker(body, e, "$EndElements");
+            if (stop && count > 4096) {
+                Mesh trial;
+                trial.cell_dim = m.cell_dim;
+                if (fast_elements(body, stop, count, trial)) {
+                    if (m.cell_tag.empty()) {
+                        m.cell_dim = trial.cell_dim;
+                        m.cell_tag = std::move(trial.cell_tag);
+                        m.cell_nodes = std::move(trial.cell_nodes);
+                    } else {
+                        m.cell_dim = trial.cell_dim;
+                        m.cell_tag.insert(m.cell_tag.end(), trial.cell_tag.begin(), trial.cell_tag.end());
+                        m.cell_nodes.insert(m.cell_nodes.end(), trial.cell_nodes.begin(), trial.cell_nodes.end());
+                    }
+                    tk.seek(stop);
+                    continue;
+                }
+            }
             for (int64_t i = 0; i < count; ++i) {
                 (void)tk.number<int64_t>(); // element tag
                 const int type = tk.number<int>();
@@ -236,7 +463,7 @@ sil_structured_grid_s* build_grid(const Mesh& m)
     auto* g = new sil_structured_grid_s();
     g->ndim = nd;
     const std::vector<double>* src[3] = {&m.x, &m.y, &m.z};
-    for (int a = 0; a < 3; ++a) g->coords[a] = unique_sorted(*src[a]);
+    run_threads(3, [&](int a) { g->coords[a] = unique_sorted(*src[a]); });
     if (nd == 2) {
         if (g->coords[2].size() != 1) {
             delete g;
@@ -256,12 +483,31 @@ sil_structured_grid_s* build_grid(const Mesh& m)
     std::vector<int64_t> by_tag_dense(dense ? static_cast<size_t>(max_tag + 1) : 0, -1);
     std::unordered_map<int64_t, int64_t> by_tag;
     if (!dense) by_tag.reserve(nnodes * 2);
+    // grid index of every mesh node in parallel (−1: off the axes), then the
+    // checks in node order
+    std::vector<int64_t> node_lin(nnodes);
+    {
+        const int T = reader_threads();
+        run_threads(T, [&](int t) {
+            for (size_t q = nnodes * t / T; q < nnodes * (t + 1) / T; ++q) {
+                try {
+                    const size_t i = axis_index(g->coords[0], m.x[q]);
+                    const size_t j = axis_index(g->coords[1], m.y[q]);
+                    const size_t k = nd == 3 ? axis_index(g->coords[2], m.z[q]) : 0;
+                    node_lin[q] = static_cast<int64_t>(i + n0 * (j + n1 * k));
+                } catch (const std::exception&) {
+                    node_lin[q] = -1;
+                }
+            }
+        });
+    }
     std::vector<int64_t> owner(total, -1); // mesh node occupying each grid node
     for (size_t q = 0; q < nnodes; ++q) {
-        const size_t i = axis_index(g->coords[0], m.x[q]);
-        const size_t j = axis_index(g->coords[1], m.y[q]);
-        const size_t k = nd == 3 ? axis_index(g->coords[2], m.z[q]) : 0;
-        const int64_t lin = static_cast<int64_t>(i + n0 * (j + n1 * k));
+        const int64_t lin = node_lin[q];
+        if (lin < 0) {
+            delete g;
+            throw ParseError("coordinate does not belong to the detected structured grid");
+        }
         if (owner[lin] >= 0) {
             const int64_t r = owner[lin];
             if (!near(m.x[r], m.x[q]) || !near(m.y[r], m.y[q]) || !near(m.z[r], m.z[q])) {
@@ -293,34 +539,53 @@ sil_structured_grid_s* build_grid(const Mesh& m)
     g->cell_tags.assign(ncells, 0);
     std::vector<char> used(ncells, 0);
     const int per = nd == 3 ? 8 : 4;
+    // grid cell of every mesh cell in parallel (−1: an unknown node, −2: not
+    // one grid cell), then the checks in cell order
+    std::vector<int64_t> cell_lin(ncells);
+    {
+        const int T = reader_threads();
+        run_threads(T, [&](int t) {
+            for (size_t c = ncells * t / T; c < ncells * (t + 1) / T; ++c) {
+                size_t lo[3] = {SIZE_MAX, SIZE_MAX, SIZE_MAX}, hi[3] = {0, 0, 0};
+                int64_t status = 0;
+                for (int q = 0; q < per && status == 0; ++q) {
+                    const int64_t tag = m.cell_nodes[c * per + q];
+                    int64_t lin = -1;
+                    if (dense) {
+                        if (tag >= 0 && tag <= max_tag) lin = by_tag_dense[tag];
+                    } else {
+                        auto it = by_tag.find(tag);
+                        if (it != by_tag.end()) lin = it->second;
+                    }
+                    if (lin < 0) {
+                        status = -1;
+                        break;
+                    }
+                    const size_t idx[3] = {static_cast<size_t>(lin) % n0, (static_cast<size_t>(lin) / n0) % n1,
+                                           static_cast<size_t>(lin) / (n0 * n1)};
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = std::min(lo[a], idx[a]);
+                        hi[a] = std::max(hi[a], idx[a]);
+                    }
+                }
+                if (status == 0) {
+                    const bool unit = hi[0] == lo[0] + 1 && hi[1] == lo[1] + 1 && (nd == 2 || hi[2] == lo[2] + 1);
+                    status = unit ? static_cast<int64_t>(lo[0] + c0 * (lo[1] + c1 * (nd == 3 ? lo[2] : 0))) : -2;
+                }
+                cell_lin[c] = status;
+            }
+        });
+    }
     for (size_t c = 0; c < ncells; ++c) {
-        size_t lo[3] = {SIZE_MAX, SIZE_MAX, SIZE_MAX}, hi[3] = {0, 0, 0};
-        for (int q = 0; q < per; ++q) {
-            const int64_t tag = m.cell_nodes[c * per + q];
-            int64_t lin = -1;
-            if (dense) {
-                if (tag >= 0 && tag <= max_tag) lin = by_tag_dense[tag];
-            } else {
-                auto it = by_tag.find(tag);
-                if (it != by_tag.end()) lin = it->second;
-            }
-            if (lin < 0) {
-                delete g;
-                throw ParseError(std::string(one) + " references an unknown node");
-            }
-            const size_t idx[3] = {static_cast<size_t>(lin) % n0, (static_cast<size_t>(lin) / n0) % n1,
-                                   static_cast<size_t>(lin) / (n0 * n1)};
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = std::min(lo[a], idx[a]);
-                hi[a] = std::max(hi[a], idx[a]);
-            }
+        if (cell_lin[c] == -1) {
+            delete g;
+            throw ParseError(std::string(one) + " references an unknown node");
         }
-        bool unit = hi[0] == lo[0] + 1 && hi[1] == lo[1] + 1 && (nd == 2 || hi[2] == lo[2] + 1);
-        if (!unit) {
+        if (cell_lin[c] == -2) {
             delete g;
             throw ParseError(std::string(one) + " does not match a single rectilinear grid cell");
         }
-        const size_t lin = lo[0] + c0 * (lo[1] + c1 * (nd == 3 ? lo[2] : 0));
+        const size_t lin = static_cast<size_t>(cell_lin[c]);
         if (used[lin]) {
             delete g;
             throw ParseError(std::string("duplicated ") + (nd == 3 ? "hexahedron" : "quadrilateral") +
