@@ -270,6 +270,58 @@ def magnetostatics_nonlinear(torch, max_iterations=20000):
     return out
 
 
+def magnetostatics_post_process(torch, repeats=50):
+    """Cell B, H, Maxwell stress and xy-slice force density of a potential
+    on the 192³-cell inductor grid (its cell μ and nonlinear-core flags, the
+    EIcore curve), timed with HIP events on the launching stream; the oracle
+    on a 48³-cell sample of the same inductor beside it."""
+    from similie_amd import solvers as S
+    from similie_amd.workloads import MU0, magnetostatics_nonlinear_inductor
+
+    curve = json.loads((ROOT / "tests" / "golden" / "eicore_bh_curve.json").read_text())
+    coords, cell_mu, flags, _ = magnetostatics_nonlinear_inductor()
+    shape = tuple(len(c) for c in coords)
+    nodes, cells = int(np.prod(shape)), int(cell_mu.size)
+    mu = S.MagnetostaticsOperator3D.nodal_mu(shape, torch.from_numpy(cell_mu).cuda(), MU0)
+    frac = S.NonlinearMagnetostaticsOperator3D.nodal_fraction(shape, torch.from_numpy(flags).cuda())
+    op = S.NonlinearMagnetostaticsOperator3D(*coords, mu, frac, curve["b"], curve["h"], gauge_penalty=20.0)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    pot = torch.randn(3 * nodes, generator=gen, dtype=torch.float64, device="cuda") * 1e-2
+    cmu, cfl = torch.from_numpy(cell_mu).cuda(), torch.from_numpy(flags).cuda()
+    for _ in range(3):
+        op.post_process(pot, cmu, cfl)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(repeats):
+        op.post_process(pot, cmu, cfl, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / repeats * 1e3
+    nbytes = 24 * nodes + (8 + 8 + 120) * cells
+    out = {"workload": "cell post-processing of the 192^3-cell nonlinear inductor grid (B, H, Maxwell stress, "
+                       "xy-slice force density), random potential",
+           "cells": cells, "us_per_application": round(us, 2),
+           "algorithmic_bytes": nbytes, "GBps": round(nbytes / (us * 1e-6) / 1e9, 1),
+           "hbm_frac": round(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBPS, 4),
+           "Mcells_per_s": round(cells / us, 1)}
+    try:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as orc
+
+        sc, smu, sfl, _ = magnetostatics_nonlinear_inductor(cells=48)
+        scells = int(smu.size)
+        spot = np.random.default_rng(5).standard_normal(3 * int(np.prod([len(c) for c in sc]))) * 1e-2
+        t0 = time.perf_counter()
+        orc.ms_post_process(sc, spot, smu, sfl, curve["b"], curve["h"])
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"Mcells_per_s": round(scells / dt / 1e6, 4), "cores": 1, "kind": "port",
+                               "sample": "oracle post-processing of the 48^3-cell inductor, one pass"}
+    except Exception as exc:  # noqa: BLE001 - the oracle is optional beside the measurement
+        out["cpu_baseline"] = {"error": f"{type(exc).__name__}: {exc}"}
+    return out
+
+
 def solve_uneven_batch(torch):
     """512 problems, sides log-uniform in [32, 512], one persistent launch."""
     from similie_amd.batched import BatchedPoisson2D
@@ -421,6 +473,7 @@ def main():
             "magnetostatics_inductor": lambda: solve_magnetostatics(torch, args.ms_iterations),
             "magnetostatics_assembled": lambda: magnetostatics_assembled(torch, args.ms_iterations),
             "magnetostatics_nonlinear": lambda: magnetostatics_nonlinear(torch, args.ms_iterations),
+            "magnetostatics_post_process": lambda: magnetostatics_post_process(torch),
         }
         pick = [k for k in args.solves.split(",") if k]
         unknown = [k for k in pick if k not in solves]

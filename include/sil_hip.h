@@ -192,6 +192,28 @@ int sil_magnetostatics3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const doub
  * node (magnetostatics_onelab.hpp:4919-4975); cell_current is [cell][3]. */
 int sil_magnetostatics3d_rhs(sil_operator op, const double* cell_current, double* rhs, void* stream);
 
+/* Cell post-processing of a potential on the operator's grid
+ * (fill_post_process_fields_on_cell_domain_3d, make_cell_post_process_fields
+ * and fill_force_density_on_hexahedral_grid_xy_slices,
+ * magnetostatics_onelab.hpp:3984-4046, 4048-4168, 4215-4254).  Per cell
+ * (ci + (nx-1)(cj + (ny-1)ck)):
+ *   induction [cell][3]  B_m = moment_m / face area (the potential's
+ *                        forward_value stencil, all nodes read);
+ *   field [cell][3]      H = h(|B|)/|B|·B where cell_nonlinear != 0 (the
+ *                        operator's B-H curve, H = 0 at B = 0), B/μ_cell
+ *                        elsewhere;
+ *   maxwell_stress [cell][6]  B_xH_x − ½B·H, B_yH_y − ½B·H, B_zH_z − ½B·H,
+ *                        B_xH_y, B_xH_z, B_yH_z;
+ *   force_density [cell][3]   per xy slice, −δ of (T_xx, T_xy), (T_xy, T_yy),
+ *                        (T_xz, T_yz) on the cell-centre grid, identity
+ *                        metric.
+ * op: a linear or nonlinear magnetostatics operator; cell_nonlinear (device
+ * [cell], or NULL for all-linear) needs a nonlinear operator.  Any output
+ * may be NULL. */
+int sil_magnetostatics3d_post_process(sil_operator op, const double* potential, const double* cell_mu,
+                                      const double* cell_nonlinear, double* induction, double* field,
+                                      double* maxwell_stress, double* force_density, void* stream);
+
 /* Dirichlet-zero 2D Poisson operator d0ᵀ ⋆1^κ d0 on the interior nodes of a
  * rectilinear nx×ny grid (x slowest): the ⋆0-weighted DEC 0-form Laplacian of
  * the reference, with optional nodal coefficient κ (edge value = mean of its

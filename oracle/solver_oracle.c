@@ -867,3 +867,116 @@ int orc_nl_newton(long nx, long ny, long nz, const double* xc, const double* yc,
     free(diag);
     return converged;
 }
+
+/* ------------------------------------------------------------------ */
+/* cell post-processing of a 3D solution                               */
+/* ------------------------------------------------------------------ */
+
+void orc_codifferential(int n, const long* ext, int pos_kind, const double* const* axis, const double* nodal,
+                        int metric_kind, const double* metric, int k, const double* in, const long* out_ext,
+                        double* out);
+
+/* Per cell (i, j, k), i fastest, of the (nx-1)(ny-1)(nz-1) cells:
+ *   induction  B_m = moment_m / (product of the two other cell widths), the
+ *              moment read from the full potential with the forward_value
+ *              stencil (magnetostatics_onelab.hpp:3826-3864, 3984-4006);
+ *   field      H = h(|B|)/|B|·B on nonlinear cells (0 at B = 0), B/μ_cell
+ *              otherwise, unit Hodge factors (:4007-4024,
+ *              nonlinear_magnetostatics.py:266-281, linear_magnetostatics.py
+ *              :38-49);
+ *   stress     B_0H_0 − ½B·H, B_1H_1 − ½B·H, B_2H_2 − ½B·H, B_0H_1, B_0H_2,
+ *              B_1H_2 (make_cell_post_process_fields, :4029-4046);
+ *   force      per xy slice of cells, −δ of the one-forms (T_0, T_3),
+ *              (T_3, T_1), (T_4, T_5) on the grid of cell centres with a
+ *              stored identity metric (fill_force_density_on_cell_domain
+ *              and its xy-slice driver, :4048-4168, 4215-4254), δ being
+ *              this oracle's codifferential (codifferential.hpp:321-498).
+ * Any output may be NULL except that force needs stress. */
+void orc_ms_post_process(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
+                         const double* potential, const double* cell_mu, const double* cell_nonlinear,
+                         const double* bh_b, const double* bh_h, int nbh, double* induction, double* field,
+                         double* stress, double* force)
+{
+    ms_grid g = {nx, ny, nz, xc, yc, zc};
+    long mx = nx - 1, my = ny - 1, mz = nz - 1;
+    long ncell = mx * my * mz;
+    orc_bh c;
+    if (nbh >= 2) bh_init(&c, bh_b, bh_h, nbh);
+    double* T = stress ? stress : (double*)malloc(sizeof(double) * 6 * (ncell > 0 ? ncell : 1));
+    for (long kk = 0; kk < mz; ++kk)
+        for (long j = 0; j < my; ++j)
+            for (long i = 0; i < mx; ++i) {
+                long cell = i + mx * (j + my * kk);
+                double w[3] = {xc[i + 1] - xc[i], yc[j + 1] - yc[j], zc[kk + 1] - zc[kk]};
+                double mom[3];
+                for (int m = 0; m < 3; ++m) {
+                    double v = 0.0;
+                    for (int t = 0; t < 4; ++t) {
+                        const ms_tap* tp = &MOMENT[m][t];
+                        v += tp->coeff * potential[dof(&g, i + tp->di, j + tp->dj, kk + tp->dk, tp->comp)];
+                    }
+                    mom[m] = v;
+                }
+                double B[3] = {mom[0] / (w[1] * w[2]), mom[1] / (w[0] * w[2]), mom[2] / (w[0] * w[1])};
+                double H[3] = {0.0, 0.0, 0.0};
+                if (cell_nonlinear && cell_nonlinear[cell] != 0.0) {
+                    double q = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+                    double bn = sqrt(q);
+                    if (bn != 0.0) {
+                        double scale = bh_h_from_b(&c, bn) / bn;
+                        for (int m = 0; m < 3; ++m) H[m] = 1.0 * scale * B[m];
+                    }
+                } else {
+                    for (int m = 0; m < 3; ++m) H[m] = 1.0 * B[m] / cell_mu[cell];
+                }
+                double half = 0.5 * (B[0] * H[0] + B[1] * H[1] + B[2] * H[2]);
+                double* t = T + 6 * cell;
+                t[0] = B[0] * H[0] - half;
+                t[1] = B[1] * H[1] - half;
+                t[2] = B[2] * H[2] - half;
+                t[3] = B[0] * H[1];
+                t[4] = B[0] * H[2];
+                t[5] = B[1] * H[2];
+                for (int m = 0; m < 3; ++m) {
+                    if (induction) induction[3 * cell + m] = B[m];
+                    if (field) field[3 * cell + m] = H[m];
+                }
+            }
+    if (force && mx > 0 && my > 0) {
+        double* cx = (double*)malloc(sizeof(double) * mx);
+        double* cy = (double*)malloc(sizeof(double) * my);
+        for (long i = 0; i < mx; ++i) cx[i] = 0.5 * (xc[i] + xc[i + 1]);
+        for (long j = 0; j < my; ++j) cy[j] = 0.5 * (yc[j] + yc[j + 1]);
+        const double* axis[2] = {cx, cy};
+        long ext[2] = {mx, my};
+        long nn = mx * my;
+        double* metric = (double*)malloc(sizeof(double) * 3 * nn);
+        double* one = (double*)malloc(sizeof(double) * 2 * nn);
+        double* out = (double*)malloc(sizeof(double) * nn);
+        for (long p = 0; p < nn; ++p) {
+            metric[3 * p] = 1.0;
+            metric[3 * p + 1] = 0.0;
+            metric[3 * p + 2] = 1.0;
+        }
+        static const int pick[3][2] = {{0, 3}, {3, 1}, {4, 5}};
+        for (long kk = 0; kk < mz; ++kk)
+            for (int comp = 0; comp < 3; ++comp) {
+                /* the oracle's node order has the first axis slowest */
+                for (long i = 0; i < mx; ++i)
+                    for (long j = 0; j < my; ++j) {
+                        const double* t = T + 6 * (i + mx * (j + my * kk));
+                        one[2 * (i * my + j)] = t[pick[comp][0]];
+                        one[2 * (i * my + j) + 1] = t[pick[comp][1]];
+                    }
+                orc_codifferential(2, ext, 0, axis, NULL, 2, metric, 1, one, ext, out);
+                for (long i = 0; i < mx; ++i)
+                    for (long j = 0; j < my; ++j) force[3 * (i + mx * (j + my * kk)) + comp] = -out[i * my + j];
+            }
+        free(cx);
+        free(cy);
+        free(metric);
+        free(one);
+        free(out);
+    }
+    if (!stress) free(T);
+}

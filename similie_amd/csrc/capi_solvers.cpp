@@ -243,6 +243,22 @@ int sil_magnetostatics3d_jacobian_diagonal(sil_operator op, const double* state,
     return ms3d_jacobian_diagonal(op->ms, op->nl, diag, s);
 }
 
+int sil_magnetostatics3d_post_process(sil_operator op, const double* potential, const double* cell_mu,
+                                      const double* cell_nonlinear, double* induction, double* field,
+                                      double* maxwell_stress, double* force_density, void* stream)
+{
+    SIL_REQUIRE(op && potential && cell_mu, SIL_ERR_INVALID, "magnetostatics3d_post_process: null argument");
+    SIL_REQUIRE(op->kind == OP_MAGNETOSTATICS3D || op->kind == OP_MS3D_NONLINEAR, SIL_ERR_INVALID,
+                "magnetostatics3d_post_process: not a magnetostatics operator");
+    SIL_REQUIRE(!cell_nonlinear || op->kind == OP_MS3D_NONLINEAR, SIL_ERR_INVALID,
+                "magnetostatics3d_post_process: nonlinear cells need a nonlinear operator (its B-H curve)");
+    SIL_REQUIRE(op->ms.nx >= 2 && op->ms.ny >= 2 && op->ms.nz >= 2, SIL_ERR_INVALID,
+                "magnetostatics3d_post_process: need ≥ 2 nodes per axis");
+    const BhCurve* curve = op->kind == OP_MS3D_NONLINEAR ? op->nl.curve : nullptr;
+    return ms3d_post_process(op->ms, curve, potential, cell_mu, cell_nonlinear, induction, field, maxwell_stress,
+                             force_density, as_stream(stream));
+}
+
 int sil_magnetostatics3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const double* cell_mu, double mu0,
                                   double* mu_nodal, void* stream)
 {

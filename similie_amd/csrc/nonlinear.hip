@@ -27,6 +27,7 @@
 #include <cmath>
 #include <new>
 
+#include "bh_curve.hpp"
 #include "operators.hpp"
 
 namespace sil {
@@ -47,32 +48,6 @@ constexpr int kBlock = 256;
 constexpr int kNormBlocks = 1024;
 
 // ------------------------------------------------------------ B-H curve ---
-
-__device__ int bracket_q(const BhCurve& c, double q)
-{
-    if (q <= c.q[0]) return 0;
-    for (int i = 0; i + 1 < c.n; ++i)
-        if (q <= c.q[i + 1]) return i;
-    return c.n - 2;
-}
-
-__device__ double h_from_b(const BhCurve& c, double b)
-{
-    if (b <= c.b[0]) return c.h[0];
-    if (b >= c.b[c.n - 1]) {
-        const int i = c.n - 2;
-        return c.h[i] + c.dh_db[i] * (b - c.b[i]);
-    }
-    const int i = bracket_q(c, b * b);
-    return c.h[i] + c.dh_db[i] * (b - c.b[i]);
-}
-
-__device__ double dh_db_from_b(const BhCurve& c, double b)
-{
-    if (b <= c.b[0]) return c.dh_db[0];
-    if (b >= c.b[c.n - 1]) return c.dh_db[c.n - 2];
-    return c.dh_db[bracket_q(c, b * b)];
-}
 
 // f, f′, f″ of f(q) = h(√q)/√q (h_over_b_from_q and its q-derivatives)
 __device__ void h_over_b_q(const BhCurve& c, double q, double& f, double& f1, double& f2)

@@ -88,6 +88,12 @@ int ms3d_jacobian_apply32(const Ms3dData& d, const NlData& nl, const float* in, 
                           const PUpdate<float>* pu, int* nblocks_out, hipStream_t s);
 int ms3d_jacobian_diagonal(const Ms3dData& d, const NlData& nl, double* diag, hipStream_t s);
 
+// cell B, H, Maxwell stress and force density of a potential (postprocess.hip);
+// curve may be null when cell_nonlinear is
+int ms3d_post_process(const Ms3dData& d, const BhCurve* curve, const double* potential, const double* cell_mu,
+                      const double* cell_nonlinear, double* induction, double* field, double* stress,
+                      double* force, hipStream_t s);
+
 // p_new = z + β p_old over n rows (the CG direction) as its own pass
 int direction_pass64(const PUpdate<double>& pu, int64_t n, hipStream_t s);
 

@@ -8,6 +8,7 @@ float64 CUDA tensors; everything runs through libsil_hip.so.
 """
 import ctypes
 from dataclasses import dataclass
+from typing import NamedTuple
 
 import torch
 
@@ -61,6 +62,16 @@ class LinearOperator:
             self.handle = None
 
 
+class CellPostProcessFields(NamedTuple):
+    """per-cell post-processing outputs (CellPostProcessFields,
+    magnetostatics_onelab.hpp:402-408)"""
+
+    magnetic_induction: torch.Tensor
+    magnetic_field: torch.Tensor
+    maxwell_stress: torch.Tensor
+    force_density: torch.Tensor
+
+
 class MagnetostaticsOperator3D(LinearOperator):
     """MagnetostaticsOperator3D with precomputed-stencil semantics: DOF order
     3·(i + nx·(j + ny·k)) + c, identity rows on boundary nodes, gauge
@@ -95,6 +106,24 @@ class MagnetostaticsOperator3D(LinearOperator):
         b = torch.empty(self.size(), dtype=torch.float64, device="cuda")
         L.check(L.lib().sil_magnetostatics3d_rhs(self.handle, L.ptr(cell_current), L.ptr(b), L.stream_handle(stream)))
         return b
+
+    def post_process(self, potential, cell_mu, cell_nonlinear=None, stream=None):
+        """CellPostProcessFields of every cell (magnetostatics_onelab.hpp
+        :3984-4046, 4048-4168, 4215-4254): magnetic_induction (cells, 3),
+        magnetic_field (cells, 3), maxwell_stress (cells, 6: xx yy zz xy xz
+        yz) and force_density (cells, 3), cells x fastest.  cell_nonlinear
+        (0/1 per cell) selects the B-H curve and needs a nonlinear operator."""
+        _f64(potential, "potential")
+        _f64(cell_mu, "cell_mu")
+        if cell_nonlinear is not None:
+            _f64(cell_nonlinear, "cell_nonlinear")
+        nx, ny, nz = self.shape
+        cells = max(nx - 1, 0) * max(ny - 1, 0) * max(nz - 1, 0)
+        out = [torch.empty((cells, w), dtype=torch.float64, device="cuda") for w in (3, 3, 6, 3)]
+        L.check(L.lib().sil_magnetostatics3d_post_process(
+            self.handle, L.ptr(potential), L.ptr(cell_mu), L.ptr(cell_nonlinear) if cell_nonlinear is not None else None,
+            *[L.ptr(o) for o in out], L.stream_handle(stream)))
+        return CellPostProcessFields(*out)
 
 
     def assemble(self, stream=None):

@@ -352,6 +352,24 @@ class Nonlinear:
                    "initial_residual_l2": out[2], "final_residual_l2": out[3]}
 
 
+def ms_post_process(coords, potential, cell_mu, cell_nonlinear=None, bh_b=None, bh_h=None):
+    """cell induction [cells][3], field [cells][3], Maxwell stress [cells][6]
+    and force density [cells][3] of a 3D potential [nodes][3] (cells x
+    fastest)"""
+    nx, ny, nz = (len(c) for c in coords)
+    xc, yc, zc = (_f64(c) for c in coords)
+    ncell = (nx - 1) * (ny - 1) * (nz - 1)
+    pot, mu = _f64(potential), _f64(cell_mu)
+    nl = _f64(cell_nonlinear) if cell_nonlinear is not None else None
+    bb = _f64(bh_b if bh_b is not None else [0.0])
+    hh = _f64(bh_h if bh_h is not None else [0.0])
+    out = [np.zeros((ncell, w)) for w in (3, 3, 6, 3)]
+    lib().orc_ms_post_process(ctypes.c_long(nx), ctypes.c_long(ny), ctypes.c_long(nz), _d(xc), _d(yc), _d(zc),
+                              _d(pot), _d(mu), _d(nl) if nl is not None else None, _d(bb), _d(hh),
+                              ctypes.c_int(len(bb) if bh_b is not None else 0), *[_d(o) for o in out])
+    return tuple(out)
+
+
 def poisson2d_apply(hx, hy, kappa, u):
     nx, ny = u.shape
     hx, hy, uu = _f64(hx), _f64(hy), _f64(u)
