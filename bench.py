@@ -270,6 +270,60 @@ def magnetostatics_nonlinear(torch, max_iterations=20000):
     return out
 
 
+def magnetostatics_2d(torch, max_iterations=20000):
+    """2048²-cell inductor cross-section on the A_z operator: linear (μ_r 1000
+    core) and nonlinear (EIcore core) solves to 1e-10, FP64 and mixed, the
+    operator application timed on its own, the 2D cell post-processing."""
+    from similie_amd import solvers as S
+    from similie_amd.workloads import MU0, magnetostatics_2d_inductor
+
+    curve = json.loads((ROOT / "tests" / "golden" / "eicore_bh_curve.json").read_text())
+    coords, cell_mu, flags, jz = magnetostatics_2d_inductor()
+    shape = tuple(len(c) for c in coords)
+    nodes = shape[0] * shape[1]
+    cmu, cfl, cj = (torch.from_numpy(a).cuda() for a in (cell_mu, flags, jz))
+    mu = S.MagnetostaticsOperator2D.nodal_mu(shape, cmu, MU0)
+    out = {"workload": "2D inductor cross-section, 2048^2 unit cells, core 512x1024 (mu_r 1000 / EIcore B-H), "
+                       "coil slabs 256 wide at J_z = +-3000", "dofs": nodes}
+    op = S.MagnetostaticsOperator2D(*coords, mu)
+    rhs = op.rhs(cj)
+    x = torch.randn(nodes, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        op.apply(x, y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(100):
+        op.apply(x, y)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / 100 * 1e3
+    out["apply_us"] = round(us, 2)
+    out["apply_GBps"] = round(24 * nodes / (us * 1e-6) / 1e9, 1)
+    for prec in ("fp64", "mixed"):
+        st = S.StrongFormulationSolverSettings(max_iterations=max_iterations, relative_tolerance=1e-10,
+                                               precision=prec)
+        _, d = S.minimize_strong_formulation_residual(op, rhs, settings=st)
+        out["linear_" + prec] = {"seconds": round(d["duration"], 4), "iterations": d["iterations"],
+                                 "converged": bool(d["converged"]),
+                                 "true_relative_residual": d["true_relative_residual"]}
+    frac = S.NonlinearMagnetostaticsOperator2D.nodal_fraction(shape, cfl)
+    mu0 = S.MagnetostaticsOperator2D.nodal_mu(shape, torch.full_like(cmu, MU0), MU0)
+    nop = S.NonlinearMagnetostaticsOperator2D(*coords, mu0, frac, curve["b"], curve["h"])
+    for prec in ("fp64", "mixed"):
+        print(f"bench:   2D nonlinear {prec} ...", file=sys.stderr, flush=True)
+        st = S.StrongFormulationSolverSettings(max_iterations=max_iterations, relative_tolerance=1e-10,
+                                               precision=prec)
+        xs, d = S.minimize_strong_formulation_residual(nop, rhs, settings=st)
+        out["nonlinear_" + prec] = {"seconds": round(d["duration"], 4), "newton_steps": d["outer_iterations"],
+                                    "cg_iterations": d["iterations"], "converged": bool(d["converged"]),
+                                    "final_relative_residual": d["final_relative_residual"]}
+    pp = nop.post_process(xs, torch.full_like(cmu, MU0), cfl)
+    torch.cuda.synchronize()
+    out["core_max_B"] = round(float(torch.max(torch.linalg.norm(pp.magnetic_induction, dim=1)).item()), 4)
+    return out
+
+
 def magnetostatics_post_process(torch, repeats=50):
     """Cell B, H, Maxwell stress and xy-slice force density of a potential
     on the 192³-cell inductor grid (its cell μ and nonlinear-core flags, the
@@ -474,6 +528,7 @@ def main():
             "magnetostatics_assembled": lambda: magnetostatics_assembled(torch, args.ms_iterations),
             "magnetostatics_nonlinear": lambda: magnetostatics_nonlinear(torch, args.ms_iterations),
             "magnetostatics_post_process": lambda: magnetostatics_post_process(torch),
+            "magnetostatics_2d": lambda: magnetostatics_2d(torch, args.ms_iterations),
         }
         pick = [k for k in args.solves.split(",") if k]
         unknown = [k for k in pick if k not in solves]

@@ -214,6 +214,48 @@ int sil_magnetostatics3d_post_process(sil_operator op, const double* potential, 
                                       const double* cell_nonlinear, double* induction, double* field,
                                       double* maxwell_stress, double* force_density, void* stream);
 
+/* MagnetostaticsOperator2D (onelab_interface/magnetostatics_onelab.hpp
+ * :1013-1522): A_z on an nx×ny rectilinear node grid, one unknown per node
+ * (row i + nx·j).  Interior rows gather ∂H/∂B at the interior sampled nodes
+ * s of the moments B_x = A(s+e_y) − A(s), B_y = A(s) − A(s+e_x) (no Hodge
+ * weight; boundary values of A are read); boundary rows are the identity.
+ * criterion: 0 potential, 1 moments (the reference's default), 2 both
+ * temporal derivatives (the row sum counted twice).  mu_nodal: device [node]. */
+int sil_magnetostatics2d_create(sil_operator* out, int64_t nx, int64_t ny, const double* x_coords,
+                                const double* y_coords, const double* mu_nodal, int32_t criterion);
+
+/* The same operator with MaterialMagnetostaticsHamiltonian (:957-1011): nodes
+ * whose ferromagnetic fraction exceeds 0.5 follow the interpolated B-H curve
+ * (2..64 increasing samples); Newton in
+ * sil_minimize_strong_formulation_residual, as for the 3D operator. */
+int sil_magnetostatics2d_nonlinear_create(sil_operator* out, int64_t nx, int64_t ny, const double* x_coords,
+                                          const double* y_coords, const double* mu_nodal,
+                                          const double* ferromagnetic_nodal, const double* bh_b, const double* bh_h,
+                                          int32_t nsamples, int32_t criterion);
+
+/* J(state)·in and the diagonal of J(state) of a 2D nonlinear operator
+ * (apply_jacobian for the 2D operator, :2790-2920) */
+int sil_magnetostatics2d_apply_jacobian(sil_operator op, const double* state, const double* in, double* out,
+                                        void* stream);
+int sil_magnetostatics2d_jacobian_diagonal(sil_operator op, const double* state, double* diag, void* stream);
+
+/* mean μ of the cells around each node (:4467-4500); cell c = ci + (nx-1)·cj */
+int sil_magnetostatics2d_nodal_mu(int64_t nx, int64_t ny, const double* cell_mu, double mu0, double* mu_nodal,
+                                  void* stream);
+
+/* ¼·area·J_z summed over the cells around each interior node (:4413-4445);
+ * zero everywhere for the potential criterion */
+int sil_magnetostatics2d_rhs(sil_operator op, const double* cell_current_z, double* rhs, void* stream);
+
+/* Cell post-processing of a 2D solution (fill_post_process_fields_on_cell_domain
+ * and fill_force_density_on_quadrilateral_grid, :3866-3982, 4048-4213): B =
+ * (ΔyA/h_y, −ΔxA/h_x, 0) reconstructed per cell, H, the Maxwell stress and
+ * −δ of the stress one-forms on the cell-centre grid; layouts as for
+ * sil_magnetostatics3d_post_process with cell = ci + (nx-1)·cj. */
+int sil_magnetostatics2d_post_process(sil_operator op, const double* potential, const double* cell_mu,
+                                      const double* cell_nonlinear, double* induction, double* field,
+                                      double* maxwell_stress, double* force_density, void* stream);
+
 /* Dirichlet-zero 2D Poisson operator d0ᵀ ⋆1^κ d0 on the interior nodes of a
  * rectilinear nx×ny grid (x slowest): the ⋆0-weighted DEC 0-form Laplacian of
  * the reference, with optional nodal coefficient κ (edge value = mean of its

@@ -777,20 +777,6 @@ void orc_nl_jacobian_diagonal(long nx, long ny, long nz, const double* xc, const
     free(o);
 }
 
-typedef struct {
-    long nx, ny, nz;
-    const double *x, *y, *z, *mu, *ferro;
-    const orc_bh* c;
-    double penalty;
-    const double* state;
-} nl_ctx;
-
-static void nl_jmv(void* p, const double* in, double* out)
-{
-    nl_ctx* k = (nl_ctx*)p;
-    nl_operator(k->nx, k->ny, k->nz, k->x, k->y, k->z, k->mu, k->ferro, k->c, k->penalty, k->state, in, out);
-}
-
 /* the nonlinear branch of minimize_strong_formulation_residual
  * (src/similie/solvers/minimize_strong_formulation_residual.hpp:1383-1569):
  * x₀ = 0, r = b − F(x); at most 30 Newton steps while ‖r‖/‖r₀‖ > tol; each
@@ -799,13 +785,31 @@ static void nl_jmv(void* p, const double* in, double* out)
  * (α = 1, halved up to 8 times until ‖r(x + αδ)‖ < ‖r(x)‖, keeping the best
  * α tried).  out[0] = Newton steps, out[1] = Σ inner CG iterations,
  * out[2] = ‖r₀‖, out[3] = final ‖r‖; returns 1 when converged. */
-int orc_nl_newton(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc, const double* mu,
-                  const double* ferro, const double* bh_b, const double* bh_h, int nbh, double penalty,
-                  const double* b, double tol, int max_inner, int use_jacobi, double* x, double* out)
+/* one nonlinear problem for the Newton loop: F(x), J(state)·v and the
+ * diagonal of J(state) */
+typedef struct nl_problem {
+    long n;
+    void (*F)(const struct nl_problem* P, const double* x, double* fx);
+    void (*J)(const struct nl_problem* P, const double* state, const double* in, double* out);
+    void (*D)(const struct nl_problem* P, const double* state, double* diag);
+    const void* data;
+} nl_problem;
+
+typedef struct {
+    const nl_problem* P;
+    const double* state;
+} nl_jctx;
+
+static void nl_problem_jmv(void* p, const double* in, double* out)
 {
-    orc_bh c;
-    bh_init(&c, bh_b, bh_h, nbh);
-    long n = 3 * nx * ny * nz;
+    nl_jctx* k = (nl_jctx*)p;
+    k->P->J(k->P, k->state, in, out);
+}
+
+static int newton_core(const nl_problem* P, const double* b, double tol, int max_inner, int use_jacobi, double* x,
+                       double* out)
+{
+    long n = P->n;
     double* r = (double*)malloc(sizeof(double) * n);
     double* fx = (double*)malloc(sizeof(double) * n);
     double* delta = (double*)malloc(sizeof(double) * n);
@@ -813,7 +817,7 @@ int orc_nl_newton(long nx, long ny, long nz, const double* xc, const double* yc,
     double* rc = (double*)malloc(sizeof(double) * n);
     double* diag = use_jacobi ? (double*)malloc(sizeof(double) * n) : NULL;
     for (long i = 0; i < n; ++i) x[i] = 0.0;
-    nl_operator(nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, NULL, x, fx);
+    P->F(P, x, fx);
     for (long i = 0; i < n; ++i) r[i] = b[i] - fx[i];
     double r0 = sqrt(dot(n, r, r)), rn = r0;
     int steps = 0, inner = 0, converged = 0;
@@ -824,15 +828,14 @@ int orc_nl_newton(long nx, long ny, long nz, const double* xc, const double* yc,
             converged = 1;
             break;
         }
-        nl_ctx ctx = {nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, x};
-        if (diag)
-            orc_nl_jacobian_diagonal(nx, ny, nz, xc, yc, zc, mu, ferro, bh_b, bh_h, nbh, penalty, x, diag);
+        nl_jctx ctx = {P, x};
+        if (diag) P->D(P, x, diag);
         double res[2];
-        inner += orc_cg(n, nl_jmv, &ctx, r, diag, tol, max_inner, delta, res);
+        inner += orc_cg(n, nl_problem_jmv, &ctx, r, diag, tol, max_inner, delta, res);
         double alpha = 1.0, best_alpha = 0.0, best = INFINITY;
         for (int ls = 0; ls < 8; ++ls) {
             for (long i = 0; i < n; ++i) xc_[i] = x[i] + alpha * delta[i];
-            nl_operator(nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, NULL, xc_, fx);
+            P->F(P, xc_, fx);
             for (long i = 0; i < n; ++i) rc[i] = b[i] - fx[i];
             double cn = sqrt(dot(n, rc, rc));
             if (cn < best) {
@@ -845,7 +848,7 @@ int orc_nl_newton(long nx, long ny, long nz, const double* xc, const double* yc,
         if (best_alpha != alpha) {
             alpha = best_alpha;
             for (long i = 0; i < n; ++i) xc_[i] = x[i] + alpha * delta[i];
-            nl_operator(nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, NULL, xc_, fx);
+            P->F(P, xc_, fx);
             for (long i = 0; i < n; ++i) rc[i] = b[i] - fx[i];
         }
         for (long i = 0; i < n; ++i) {
@@ -866,6 +869,45 @@ int orc_nl_newton(long nx, long ny, long nz, const double* xc, const double* yc,
     free(rc);
     free(diag);
     return converged;
+}
+
+typedef struct {
+    long nx, ny, nz;
+    const double *x, *y, *z, *mu, *ferro;
+    const double *bh_b, *bh_h;
+    int nbh;
+    const orc_bh* c;
+    double penalty;
+} nl3_data;
+
+static void nl3_F(const nl_problem* P, const double* x, double* fx)
+{
+    const nl3_data* d = (const nl3_data*)P->data;
+    nl_operator(d->nx, d->ny, d->nz, d->x, d->y, d->z, d->mu, d->ferro, d->c, d->penalty, NULL, x, fx);
+}
+
+static void nl3_J(const nl_problem* P, const double* state, const double* in, double* out)
+{
+    const nl3_data* d = (const nl3_data*)P->data;
+    nl_operator(d->nx, d->ny, d->nz, d->x, d->y, d->z, d->mu, d->ferro, d->c, d->penalty, state, in, out);
+}
+
+static void nl3_D(const nl_problem* P, const double* state, double* diag)
+{
+    const nl3_data* d = (const nl3_data*)P->data;
+    orc_nl_jacobian_diagonal(d->nx, d->ny, d->nz, d->x, d->y, d->z, d->mu, d->ferro, d->bh_b, d->bh_h, d->nbh,
+                             d->penalty, state, diag);
+}
+
+int orc_nl_newton(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc, const double* mu,
+                  const double* ferro, const double* bh_b, const double* bh_h, int nbh, double penalty,
+                  const double* b, double tol, int max_inner, int use_jacobi, double* x, double* out)
+{
+    orc_bh c;
+    bh_init(&c, bh_b, bh_h, nbh);
+    nl3_data d = {nx, ny, nz, xc, yc, zc, mu, ferro, bh_b, bh_h, nbh, &c, penalty};
+    nl_problem P = {3 * nx * ny * nz, nl3_F, nl3_J, nl3_D, &d};
+    return newton_core(&P, b, tol, max_inner, use_jacobi, x, out);
 }
 
 /* ------------------------------------------------------------------ */
@@ -892,6 +934,73 @@ void orc_codifferential(int n, const long* ext, int pos_kind, const double* cons
  *              and its xy-slice driver, :4048-4168, 4215-4254), δ being
  *              this oracle's codifferential (codifferential.hpp:321-498).
  * Any output may be NULL except that force needs stress. */
+static void ms2_moments(long nx, long s, const double* v, double B[3]);
+void orc_reconstruction(int n, const long* ext, int pos_kind, const double* const* axis, const double* nodal,
+                        int k, const double* in, double* out);
+
+/* H and the Maxwell stress of one cell from its B */
+static void cell_fields(const double B[3], int nonlinear, double mu, const orc_bh* c, double H[3], double t[6])
+{
+    for (int m = 0; m < 3; ++m) H[m] = 0.0;
+    if (nonlinear) {
+        double q = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+        double bn = sqrt(q);
+        if (bn != 0.0) {
+            double scale = bh_h_from_b(c, bn) / bn;
+            for (int m = 0; m < 3; ++m) H[m] = 1.0 * scale * B[m];
+        }
+    } else {
+        for (int m = 0; m < 3; ++m) H[m] = 1.0 * B[m] / mu;
+    }
+    double half = 0.5 * (B[0] * H[0] + B[1] * H[1] + B[2] * H[2]);
+    t[0] = B[0] * H[0] - half;
+    t[1] = B[1] * H[1] - half;
+    t[2] = B[2] * H[2] - half;
+    t[3] = B[0] * H[1];
+    t[4] = B[0] * H[2];
+    t[5] = B[1] * H[2];
+}
+
+/* fill_force_density_on_cell_domain (:4048-4168) on one slice of mx × my
+ * cells (x fastest): −δ of (T_0, T_3), (T_3, T_1), (T_4, T_5) on the grid of
+ * cell centres with a stored identity metric */
+static void slice_force(const double* xc, const double* yc, long mx, long my, const double* T, double* force)
+{
+    double* cx = (double*)malloc(sizeof(double) * mx);
+    double* cy = (double*)malloc(sizeof(double) * my);
+    for (long i = 0; i < mx; ++i) cx[i] = 0.5 * (xc[i] + xc[i + 1]);
+    for (long j = 0; j < my; ++j) cy[j] = 0.5 * (yc[j] + yc[j + 1]);
+    const double* axis[2] = {cx, cy};
+    long ext[2] = {mx, my};
+    long nn = mx * my;
+    double* metric = (double*)malloc(sizeof(double) * 3 * nn);
+    double* one = (double*)malloc(sizeof(double) * 2 * nn);
+    double* out = (double*)malloc(sizeof(double) * nn);
+    for (long q = 0; q < nn; ++q) {
+        metric[3 * q] = 1.0;
+        metric[3 * q + 1] = 0.0;
+        metric[3 * q + 2] = 1.0;
+    }
+    static const int pick[3][2] = {{0, 3}, {3, 1}, {4, 5}};
+    for (int comp = 0; comp < 3; ++comp) {
+        /* the oracle's node order has the first axis slowest */
+        for (long i = 0; i < mx; ++i)
+            for (long j = 0; j < my; ++j) {
+                const double* t = T + 6 * (i + mx * j);
+                one[2 * (i * my + j)] = t[pick[comp][0]];
+                one[2 * (i * my + j) + 1] = t[pick[comp][1]];
+            }
+        orc_codifferential(2, ext, 0, axis, NULL, 2, metric, 1, one, ext, out);
+        for (long i = 0; i < mx; ++i)
+            for (long j = 0; j < my; ++j) force[3 * (i + mx * j) + comp] = -out[i * my + j];
+    }
+    free(cx);
+    free(cy);
+    free(metric);
+    free(one);
+    free(out);
+}
+
 void orc_ms_post_process(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
                          const double* potential, const double* cell_mu, const double* cell_nonlinear,
                          const double* bh_b, const double* bh_h, int nbh, double* induction, double* field,
@@ -918,65 +1027,314 @@ void orc_ms_post_process(long nx, long ny, long nz, const double* xc, const doub
                     mom[m] = v;
                 }
                 double B[3] = {mom[0] / (w[1] * w[2]), mom[1] / (w[0] * w[2]), mom[2] / (w[0] * w[1])};
-                double H[3] = {0.0, 0.0, 0.0};
-                if (cell_nonlinear && cell_nonlinear[cell] != 0.0) {
-                    double q = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
-                    double bn = sqrt(q);
-                    if (bn != 0.0) {
-                        double scale = bh_h_from_b(&c, bn) / bn;
-                        for (int m = 0; m < 3; ++m) H[m] = 1.0 * scale * B[m];
-                    }
-                } else {
-                    for (int m = 0; m < 3; ++m) H[m] = 1.0 * B[m] / cell_mu[cell];
-                }
-                double half = 0.5 * (B[0] * H[0] + B[1] * H[1] + B[2] * H[2]);
-                double* t = T + 6 * cell;
-                t[0] = B[0] * H[0] - half;
-                t[1] = B[1] * H[1] - half;
-                t[2] = B[2] * H[2] - half;
-                t[3] = B[0] * H[1];
-                t[4] = B[0] * H[2];
-                t[5] = B[1] * H[2];
+                double H[3];
+                cell_fields(B, cell_nonlinear && cell_nonlinear[cell] != 0.0, cell_mu[cell], &c, H, T + 6 * cell);
                 for (int m = 0; m < 3; ++m) {
                     if (induction) induction[3 * cell + m] = B[m];
                     if (field) field[3 * cell + m] = H[m];
                 }
             }
-    if (force && mx > 0 && my > 0) {
-        double* cx = (double*)malloc(sizeof(double) * mx);
-        double* cy = (double*)malloc(sizeof(double) * my);
-        for (long i = 0; i < mx; ++i) cx[i] = 0.5 * (xc[i] + xc[i + 1]);
-        for (long j = 0; j < my; ++j) cy[j] = 0.5 * (yc[j] + yc[j + 1]);
-        const double* axis[2] = {cx, cy};
-        long ext[2] = {mx, my};
-        long nn = mx * my;
-        double* metric = (double*)malloc(sizeof(double) * 3 * nn);
-        double* one = (double*)malloc(sizeof(double) * 2 * nn);
-        double* out = (double*)malloc(sizeof(double) * nn);
-        for (long p = 0; p < nn; ++p) {
-            metric[3 * p] = 1.0;
-            metric[3 * p + 1] = 0.0;
-            metric[3 * p + 2] = 1.0;
-        }
-        static const int pick[3][2] = {{0, 3}, {3, 1}, {4, 5}};
-        for (long kk = 0; kk < mz; ++kk)
-            for (int comp = 0; comp < 3; ++comp) {
-                /* the oracle's node order has the first axis slowest */
-                for (long i = 0; i < mx; ++i)
-                    for (long j = 0; j < my; ++j) {
-                        const double* t = T + 6 * (i + mx * (j + my * kk));
-                        one[2 * (i * my + j)] = t[pick[comp][0]];
-                        one[2 * (i * my + j) + 1] = t[pick[comp][1]];
-                    }
-                orc_codifferential(2, ext, 0, axis, NULL, 2, metric, 1, one, ext, out);
-                for (long i = 0; i < mx; ++i)
-                    for (long j = 0; j < my; ++j) force[3 * (i + mx * (j + my * kk)) + comp] = -out[i * my + j];
-            }
-        free(cx);
-        free(cy);
-        free(metric);
-        free(one);
-        free(out);
-    }
+    if (force && mx > 0 && my > 0)
+        for (long kk = 0; kk < mz; ++kk) slice_force(xc, yc, mx, my, T + 6 * mx * my * kk, force + 3 * mx * my * kk);
     if (!stress) free(T);
+}
+
+/* The 2D counterpart (fill_post_process_fields_on_cell_domain, :3866-3982,
+ * and fill_force_density_on_quadrilateral_grid, :4170-4213): per cell s the
+ * in-plane moment pair, reduced (X: −ΔxA, Y: ΔyA) and reconstructed at the
+ * cell's node with the node positions (the diagonal inverse of the edge
+ * lengths, this oracle's orc_reconstruction), B = (rec_Y, rec_X, 0); H,
+ * stress and force as in 3D on the single slice.  potential: A_z per node. */
+void orc_ms2d_post_process(long nx, long ny, const double* xc, const double* yc, const double* potential,
+                           const double* cell_mu, const double* cell_nonlinear, const double* bh_b,
+                           const double* bh_h, int nbh, double* induction, double* field, double* stress,
+                           double* force)
+{
+    long mx = nx - 1, my = ny - 1, ncell = mx * my;
+    orc_bh c;
+    if (nbh >= 2) bh_init(&c, bh_b, bh_h, nbh);
+    double* T = stress ? stress : (double*)malloc(sizeof(double) * 6 * (ncell > 0 ? ncell : 1));
+    const double* axis[2] = {xc, yc};
+    long ext[2] = {nx, ny};
+    double* reduced = (double*)malloc(sizeof(double) * 2 * nx * ny);
+    double* rec = (double*)malloc(sizeof(double) * 2 * nx * ny);
+    for (long p = 0; p < 2 * nx * ny; ++p) reduced[p] = 0.0;
+    for (long j = 0; j < my; ++j)
+        for (long i = 0; i < mx; ++i) {
+            long s = i + nx * j;
+            double Bm[3];
+            ms2_moments(nx, s, potential, Bm);
+            /* the oracle's node order has the first axis slowest */
+            reduced[2 * (i * ny + j)] = Bm[1];
+            reduced[2 * (i * ny + j) + 1] = Bm[0];
+        }
+    orc_reconstruction(2, ext, 0, axis, NULL, 1, reduced, rec);
+    for (long j = 0; j < my; ++j)
+        for (long i = 0; i < mx; ++i) {
+            long cell = i + mx * j;
+            double B[3] = {rec[2 * (i * ny + j) + 1], rec[2 * (i * ny + j)], 0.0};
+            double H[3];
+            cell_fields(B, cell_nonlinear && cell_nonlinear[cell] != 0.0, cell_mu[cell], &c, H, T + 6 * cell);
+            for (int m = 0; m < 3; ++m) {
+                if (induction) induction[3 * cell + m] = B[m];
+                if (field) field[3 * cell + m] = H[m];
+            }
+        }
+    if (force && mx > 0 && my > 0) slice_force(xc, yc, mx, my, T, force);
+    free(reduced);
+    free(rec);
+    if (!stress) free(T);
+}
+
+/* ------------------------------------------------------------------ */
+/* 2D magnetostatics on A_z                                            */
+/* ------------------------------------------------------------------ */
+
+/* The tables of MagnetostaticsOperator2D (magnetostatics_onelab.hpp
+ * :1051-1262), restated literally.  Per node, the two moment stencils of
+ * MagneticVectorPotentialToMagneticInduction<X, Y>::forward_value
+ * (magnetostatics_quantities.hpp:97-122) in the stencil domain's order:
+ * B_x = −A(s) + A(s+e_y), B_y = +A(s) − A(s+e_x) (flat index i + nx·j, also
+ * past the grid's edge, where only boundary nodes reach).  Per row, the
+ * transposed slots: sampled rows ascending, interior sampled nodes only;
+ * the outer slots hold the negated coefficients. */
+#define MS2_SLOTS 4
+typedef struct {
+    long nx, ny;
+    long mcol[2][2];
+    double mcoef[2][2];
+    int tcount[2], ocount[2];
+    long tcol[2][MS2_SLOTS], ocol[2][MS2_SLOTS];
+    double tcoef[2][MS2_SLOTS], ocoef[2][MS2_SLOTS];
+} ms2_row_tables;
+
+typedef struct {
+    long nx, ny;
+    const double* mu;
+    const double* ferro; /* NULL: linear Hamiltonian */
+    const orc_bh* c;
+    int criterion;
+} ms2_model;
+
+static void ms2_stencil(long nx, long s, int m, long col[2], double coef[2])
+{
+    if (m == 0) {
+        col[0] = s;
+        coef[0] = -1.0;
+        col[1] = s + nx;
+        coef[1] = 1.0;
+    } else {
+        col[0] = s;
+        coef[0] = 1.0;
+        col[1] = s + 1;
+        coef[1] = -1.0;
+    }
+}
+
+static int ms2_interior(long nx, long ny, long s)
+{
+    long i = s % nx, j = s / nx;
+    return !(i == 0 || j == 0 || i + 1 == nx || j + 1 == ny);
+}
+
+/* the transposed and outer slots of one row: the scan of the constructor
+ * (:1180-1241) restricted to the sampled rows whose stencils can reach it */
+static void ms2_row(long nx, long ny, long row, ms2_row_tables* t)
+{
+    memset(t, 0, sizeof *t);
+    long cand[3] = {row - nx, row - 1, row}; /* ascending */
+    for (int q = 0; q < 3; ++q) {
+        long s = cand[q];
+        if (s < 0 || !ms2_interior(nx, ny, s)) continue;
+        for (int m = 0; m < 2; ++m) {
+            long col[2];
+            double coef[2];
+            ms2_stencil(nx, s, m, col, coef);
+            for (int k = 0; k < 2; ++k) {
+                if (col[k] != row || col[k] >= nx * ny) continue;
+                t->tcol[m][t->tcount[m]] = s;
+                t->tcoef[m][t->tcount[m]++] = coef[k];
+                t->ocol[m][t->ocount[m]] = s;
+                t->ocoef[m][t->ocount[m]++] = -coef[k];
+            }
+        }
+    }
+}
+
+static void ms2_moments(long nx, long s, const double* v, double B[3])
+{
+    for (int m = 0; m < 2; ++m) {
+        long col[2];
+        double coef[2];
+        ms2_stencil(nx, s, m, col, coef);
+        double acc = 0.0;
+        for (int k = 0; k < 2; ++k) acc += coef[k] * v[col[k]];
+        B[m] = acc;
+    }
+    B[2] = 0.0;
+}
+
+/* dpotential_dt<m>(moments, s) at the state of v, or with state given the
+ * Jacobian row m applied to δB (jacobian_component, :666-682) */
+static double ms2_response(const ms2_model* M, long s, int m, const double* state, const double* v)
+{
+    double B[3], H[3], J[9];
+    int fe = M->ferro && M->ferro[s] > 0.5;
+    if (!state) {
+        ms2_moments(M->nx, s, v, B);
+        if (!M->ferro) return B[m] / M->mu[s];
+        nl_response(M->c, fe, M->mu[s], B, H, NULL);
+        return H[m];
+    }
+    double dB[3];
+    ms2_moments(M->nx, s, state, B);
+    ms2_moments(M->nx, s, v, dB);
+    if (!M->ferro) return (1.0 / M->mu[s]) * dB[m];
+    nl_response(M->c, fe, M->mu[s], B, NULL, J);
+    return J[3 * m] * dB[0] + J[3 * m + 1] * dB[1];
+}
+
+/* apply (:1277-1439) / apply_jacobian (:2790-2920) of the 2D operator */
+static void ms2_operator(const ms2_model* M, const double* state, const double* in, double* out)
+{
+    long nx = M->nx, ny = M->ny;
+    for (long row = 0; row < nx * ny; ++row) {
+        if (!ms2_interior(nx, ny, row)) {
+            out[row] = in[row];
+            continue;
+        }
+        ms2_row_tables t;
+        ms2_row(nx, ny, row, &t);
+        double residual = 0.0;
+        if (M->criterion == 0 || M->criterion == 2)
+            for (int m = 0; m < 2; ++m)
+                for (int k = 0; k < t.tcount[m]; ++k)
+                    residual += t.tcoef[m][k] * ms2_response(M, t.tcol[m][k], m, state, in);
+        if (M->criterion == 1 || M->criterion == 2)
+            for (int m = 0; m < 2; ++m)
+                for (int k = 0; k < t.ocount[m]; ++k)
+                    residual -= t.ocoef[m][k] * ms2_response(M, t.ocol[m][k], m, state, in);
+        out[row] = residual;
+    }
+}
+
+/* F(in) (state = NULL; linear when ferro = NULL) or J(state)·in */
+void orc_ms2d_apply(long nx, long ny, const double* mu, const double* ferro, const double* bh_b, const double* bh_h,
+                    int nbh, int criterion, const double* state, const double* in, double* out)
+{
+    orc_bh c;
+    if (ferro) bh_init(&c, bh_b, bh_h, nbh);
+    ms2_model M = {nx, ny, mu, ferro, ferro ? &c : NULL, criterion};
+    ms2_operator(&M, state, in, out);
+}
+
+/* the diagonal of the linear operator (state = NULL) or of J(state), probed
+ * with unit vectors in 3×3 colour classes */
+void orc_ms2d_diagonal(long nx, long ny, const double* mu, const double* ferro, const double* bh_b,
+                       const double* bh_h, int nbh, int criterion, const double* state, double* diag)
+{
+    orc_bh c;
+    if (ferro) bh_init(&c, bh_b, bh_h, nbh);
+    ms2_model M = {nx, ny, mu, ferro, ferro ? &c : NULL, criterion};
+    long n = nx * ny;
+    double* e = (double*)calloc(n, sizeof(double));
+    double* o = (double*)malloc(sizeof(double) * n);
+    const double* st = state;
+    double* zero = NULL;
+    if (ferro && !st) {
+        zero = (double*)calloc(n, sizeof(double));
+        st = zero;
+    }
+    for (int ci = 0; ci < 3; ++ci)
+        for (int cj = 0; cj < 3; ++cj) {
+            for (long r = 0; r < n; ++r) e[r] = 0.0;
+            for (long j = cj; j < ny; j += 3)
+                for (long i = ci; i < nx; i += 3) e[i + nx * j] = 1.0;
+            ms2_operator(&M, ferro ? st : NULL, e, o);
+            for (long j = cj; j < ny; j += 3)
+                for (long i = ci; i < nx; i += 3) diag[i + nx * j] = o[i + nx * j];
+        }
+    free(e);
+    free(o);
+    free(zero);
+}
+
+/* ¼·area·J_z of the adjacent cells at interior nodes (:4413-4445) */
+void orc_ms2d_rhs(long nx, long ny, const double* xc, const double* yc, const double* jz, int criterion, double* rhs)
+{
+    long mx = nx - 1, my = ny - 1;
+    for (long j = 0; j < ny; ++j)
+        for (long i = 0; i < nx; ++i) {
+            long node = i + nx * j;
+            if (i == 0 || j == 0 || i + 1 == nx || j + 1 == ny || criterion == 0) {
+                rhs[node] = 0.0;
+                continue;
+            }
+            double acc = 0.0;
+            for (int dj = -1; dj <= 0; ++dj)
+                for (int di = -1; di <= 0; ++di) {
+                    long ci = i + di, cj = j + dj;
+                    if (ci < 0 || cj < 0 || ci >= mx || cj >= my) continue;
+                    double area = (xc[ci + 1] - xc[ci]) * (yc[cj + 1] - yc[cj]);
+                    acc += 0.25 * area * jz[ci + mx * cj];
+                }
+            rhs[node] = acc;
+        }
+}
+
+/* mean of the adjacent cells (:4467-4500) */
+void orc_ms2d_nodal_mu(long nx, long ny, const double* cell_mu, double mu0, double* mu)
+{
+    long mx = nx - 1, my = ny - 1;
+    for (long j = 0; j < ny; ++j)
+        for (long i = 0; i < nx; ++i) {
+            double acc = 0.0;
+            long count = 0;
+            for (int dj = -1; dj <= 0; ++dj)
+                for (int di = -1; di <= 0; ++di) {
+                    long ci = i + di, cj = j + dj;
+                    if (ci < 0 || cj < 0 || ci >= mx || cj >= my) continue;
+                    acc += cell_mu[ci + mx * cj];
+                    ++count;
+                }
+            mu[i + nx * j] = count == 0 ? mu0 : acc / (double)count;
+        }
+}
+
+static void ms2_F(const nl_problem* P, const double* x, double* fx) { ms2_operator((const ms2_model*)P->data, NULL, x, fx); }
+
+static void ms2_J(const nl_problem* P, const double* state, const double* in, double* out)
+{
+    ms2_operator((const ms2_model*)P->data, state, in, out);
+}
+
+static void ms2_D(const nl_problem* P, const double* state, double* diag)
+{
+    const ms2_model* M = (const ms2_model*)P->data;
+    long n = M->nx * M->ny;
+    double* e = (double*)calloc(n, sizeof(double));
+    double* o = (double*)malloc(sizeof(double) * n);
+    for (int ci = 0; ci < 3; ++ci)
+        for (int cj = 0; cj < 3; ++cj) {
+            for (long r = 0; r < n; ++r) e[r] = 0.0;
+            for (long j = cj; j < M->ny; j += 3)
+                for (long i = ci; i < M->nx; i += 3) e[i + M->nx * j] = 1.0;
+            ms2_operator(M, state, e, o);
+            for (long j = cj; j < M->ny; j += 3)
+                for (long i = ci; i < M->nx; i += 3) diag[i + M->nx * j] = o[i + M->nx * j];
+        }
+    free(e);
+    free(o);
+}
+
+/* Newton on the 2D nonlinear operator, as orc_nl_newton */
+int orc_ms2d_newton(long nx, long ny, const double* mu, const double* ferro, const double* bh_b, const double* bh_h,
+                    int nbh, int criterion, const double* b, double tol, int max_inner, int use_jacobi, double* x,
+                    double* out)
+{
+    orc_bh c;
+    bh_init(&c, bh_b, bh_h, nbh);
+    ms2_model M = {nx, ny, mu, ferro, &c, criterion};
+    nl_problem P = {nx * ny, ms2_F, ms2_J, ms2_D, &M};
+    return newton_core(&P, b, tol, max_inner, use_jacobi, x, out);
 }

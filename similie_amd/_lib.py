@@ -143,6 +143,14 @@ SIGNATURES = {
     "sil_magnetostatics3d_nodal_mu": (ctypes.c_int, [_I64, _I64, _I64, _P, _D, _P, _P]),
     "sil_magnetostatics3d_rhs": (ctypes.c_int, [_P, _P, _P, _P]),
     "sil_magnetostatics3d_post_process": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "sil_magnetostatics2d_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _P, _P, _P, _I32]),
+    "sil_magnetostatics2d_nonlinear_create": (
+        ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _P, _P, _P, _P, _P, _P, _I32, _I32]),
+    "sil_magnetostatics2d_apply_jacobian": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "sil_magnetostatics2d_jacobian_diagonal": (ctypes.c_int, [_P, _P, _P, _P]),
+    "sil_magnetostatics2d_nodal_mu": (ctypes.c_int, [_I64, _I64, _P, _D, _P, _P]),
+    "sil_magnetostatics2d_rhs": (ctypes.c_int, [_P, _P, _P, _P]),
+    "sil_magnetostatics2d_post_process": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "sil_poisson2d_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _P, _P, _P]),
     "sil_magnetostatics3d_nonlinear_create": (
         ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _I32, _D]),

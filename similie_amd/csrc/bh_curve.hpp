@@ -34,4 +34,22 @@ __device__ inline double dh_db_from_b(const BhCurve& c, double b)
     return c.dh_db[bracket_q(c, b * b)];
 }
 
+// f, f′, f″ of f(q) = h(√q)/√q (h_over_b_from_q and its q-derivatives)
+__device__ inline void h_over_b_q(const BhCurve& c, double q, double& f, double& f1, double& f2)
+{
+    if (q == 0.0) {
+        f = dh_db_from_b(c, 0.0);
+        f1 = 0.0;
+        f2 = 0.0;
+        return;
+    }
+    const double b = sqrt(q);
+    const double h = h_from_b(c, b), dh = dh_db_from_b(c, b);
+    f = h / b;
+    const double d1 = (dh * b - h) / (b * b);             // d(h/b)/db
+    const double d2 = 2.0 * (h - dh * b) / (b * b * b);   // d²(h/b)/db²
+    f1 = d1 / (2.0 * b);
+    f2 = d2 / (4.0 * q) - d1 / (4.0 * q * b);
+}
+
 }  // namespace sil

@@ -56,6 +56,9 @@ void free_operator(sil_operator op)
     }
     f(op->ms.nu);
     f(op->ms.nu32);
+    for (int q = 0; q < 2; ++q) f(op->m2.coord[q]);
+    f(op->m2.nu);
+    f(op->m2.nu32);
     f(op->p2.wx);
     f(op->p2.wy);
     f(op->p2.wx32);
@@ -93,6 +96,59 @@ int wrap_csr(const CsrData& m, sil_operator* out)
     op->csr = m;
     *out = op;
     return SIL_OK;
+}
+
+// InterpolatedNonlinearBHCurve's preconditions (validate_nonlinear_bh_curve)
+int check_curve(const double* bh_b, int32_t nsamples)
+{
+    SIL_REQUIRE(nsamples >= 2 && nsamples <= kBhMax, SIL_ERR_INVALID,
+                "invalid nonlinear B-H curve sample count %d (2..%d)", nsamples, kBhMax);
+    for (int i = 0; i + 1 < nsamples; ++i)
+        SIL_REQUIRE(bh_b[i + 1] > bh_b[i], SIL_ERR_INVALID, "B-H curve: induction samples must increase");
+    return SIL_OK;
+}
+
+// the B-H curve, the ferromagnetic flags, the state arrays and the Jacobian
+// operator of a nonlinear operator over nn nodes (InterpolatedNonlinearBHCurve,
+// nonlinear_magnetostatics.py:26-55); on failure the caller frees op
+int attach_material(sil_operator op, int64_t nn, const double* ferromagnetic_nodal, const double* bh_b,
+                    const double* bh_h, int32_t nsamples)
+{
+    BhCurve c{};
+    c.n = nsamples;
+    for (int i = 0; i < nsamples; ++i) {
+        c.b[i] = bh_b[i];
+        c.h[i] = bh_h[i];
+        c.q[i] = bh_b[i] * bh_b[i];
+    }
+    for (int i = 0; i + 1 < nsamples; ++i) c.dh_db[i] = (c.h[i + 1] - c.h[i]) / (c.b[i + 1] - c.b[i]);
+    BhCurve* dc = nullptr;
+    NlData& nl = op->nl;
+    bool ok = hipMalloc(&dc, sizeof(BhCurve)) == hipSuccess;
+    nl.curve = dc;
+    ok = ok && hipMemcpy(dc, &c, sizeof(BhCurve), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMalloc(&nl.ferro, nn) == hipSuccess;
+    ok = ok && hipMalloc(&nl.st_a, sizeof(double) * nn) == hipSuccess;
+    ok = ok && hipMalloc(&nl.st_c, sizeof(double) * nn) == hipSuccess;
+    for (int q = 0; q < 3; ++q) ok = ok && hipMalloc(&nl.st_b[q], sizeof(double) * nn) == hipSuccess;
+    ok = ok && hipMalloc(&nl.st_a32, sizeof(float) * nn) == hipSuccess;
+    ok = ok && hipMalloc(&nl.st_c32, sizeof(float) * nn) == hipSuccess;
+    for (int q = 0; q < 3; ++q) ok = ok && hipMalloc(&nl.st_b32[q], sizeof(float) * nn) == hipSuccess;
+    if (!ok) {
+        set_error("nonlinear_create: device allocation failed");
+        return SIL_ERR_HIP;
+    }
+    int rc = ms3d_ferro_flags(ferromagnetic_nodal, nl.ferro, nn, 0);
+    if (rc == SIL_OK && hipDeviceSynchronize() != hipSuccess) {
+        set_error("nonlinear_create: flag kernel failed");
+        rc = SIL_ERR_HIP;
+    }
+    op->jac = rc == SIL_OK ? new (std::nothrow) sil_operator_s() : nullptr;
+    if (rc == SIL_OK && !op->jac) {
+        set_error("nonlinear_create: out of host memory");
+        rc = SIL_ERR_INVALID;
+    }
+    return rc;
 }
 
 }  // namespace
@@ -163,53 +219,15 @@ int sil_magnetostatics3d_nonlinear_create(sil_operator* out, int64_t nx, int64_t
 {
     SIL_REQUIRE(out && ferromagnetic_nodal && bh_b && bh_h, SIL_ERR_INVALID,
                 "magnetostatics3d_nonlinear_create: null argument");
-    SIL_REQUIRE(nsamples >= 2 && nsamples <= kBhMax, SIL_ERR_INVALID,
-                "invalid nonlinear B-H curve sample count %d (2..%d)", nsamples, kBhMax);
-    for (int i = 0; i + 1 < nsamples; ++i)
-        SIL_REQUIRE(bh_b[i + 1] > bh_b[i], SIL_ERR_INVALID, "B-H curve: induction samples must increase");
+    int rc = check_curve(bh_b, nsamples);
+    if (rc) return rc;
     SIL_REQUIRE(nx >= 3 && ny >= 3 && nz >= 3, SIL_ERR_INVALID,
                 "magnetostatics3d_nonlinear_create: need >= 3 nodes per axis");
     sil_operator op = nullptr;
-    int rc = sil_magnetostatics3d_create(&op, nx, ny, nz, x_coords, y_coords, z_coords, mu_nodal, gauge_penalty);
+    rc = sil_magnetostatics3d_create(&op, nx, ny, nz, x_coords, y_coords, z_coords, mu_nodal, gauge_penalty);
     if (rc) return rc;
     op->kind = OP_MS3D_NONLINEAR;
-    const int64_t nn = nx * ny * nz;
-    // InterpolatedNonlinearBHCurve (nonlinear_magnetostatics.py:26-55)
-    BhCurve c{};
-    c.n = nsamples;
-    for (int i = 0; i < nsamples; ++i) {
-        c.b[i] = bh_b[i];
-        c.h[i] = bh_h[i];
-        c.q[i] = bh_b[i] * bh_b[i];
-    }
-    for (int i = 0; i + 1 < nsamples; ++i) c.dh_db[i] = (c.h[i + 1] - c.h[i]) / (c.b[i + 1] - c.b[i]);
-    BhCurve* dc = nullptr;
-    NlData& nl = op->nl;
-    bool ok = hipMalloc(&dc, sizeof(BhCurve)) == hipSuccess;
-    nl.curve = dc;
-    ok = ok && hipMemcpy(dc, &c, sizeof(BhCurve), hipMemcpyHostToDevice) == hipSuccess;
-    ok = ok && hipMalloc(&nl.ferro, nn) == hipSuccess;
-    ok = ok && hipMalloc(&nl.st_a, sizeof(double) * nn) == hipSuccess;
-    ok = ok && hipMalloc(&nl.st_c, sizeof(double) * nn) == hipSuccess;
-    for (int q = 0; q < 3; ++q) ok = ok && hipMalloc(&nl.st_b[q], sizeof(double) * nn) == hipSuccess;
-    ok = ok && hipMalloc(&nl.st_a32, sizeof(float) * nn) == hipSuccess;
-    ok = ok && hipMalloc(&nl.st_c32, sizeof(float) * nn) == hipSuccess;
-    for (int q = 0; q < 3; ++q) ok = ok && hipMalloc(&nl.st_b32[q], sizeof(float) * nn) == hipSuccess;
-    if (!ok) {
-        free_operator(op);
-        set_error("magnetostatics3d_nonlinear_create: device allocation failed");
-        return SIL_ERR_HIP;
-    }
-    rc = ms3d_ferro_flags(ferromagnetic_nodal, nl.ferro, nn, 0);
-    if (rc == SIL_OK && hipDeviceSynchronize() != hipSuccess) {
-        set_error("magnetostatics3d_nonlinear_create: flag kernel failed");
-        rc = SIL_ERR_HIP;
-    }
-    op->jac = rc == SIL_OK ? new (std::nothrow) sil_operator_s() : nullptr;
-    if (rc == SIL_OK && !op->jac) {
-        set_error("magnetostatics3d_nonlinear_create: out of host memory");
-        rc = SIL_ERR_INVALID;
-    }
+    rc = attach_material(op, nx * ny * nz, ferromagnetic_nodal, bh_b, bh_h, nsamples);
     if (rc != SIL_OK) {
         free_operator(op);
         return rc;
@@ -256,6 +274,124 @@ int sil_magnetostatics3d_post_process(sil_operator op, const double* potential, 
                 "magnetostatics3d_post_process: need ≥ 2 nodes per axis");
     const BhCurve* curve = op->kind == OP_MS3D_NONLINEAR ? op->nl.curve : nullptr;
     return ms3d_post_process(op->ms, curve, potential, cell_mu, cell_nonlinear, induction, field, maxwell_stress,
+                             force_density, as_stream(stream));
+}
+
+int sil_magnetostatics2d_create(sil_operator* out, int64_t nx, int64_t ny, const double* x_coords,
+                                const double* y_coords, const double* mu_nodal, int32_t criterion)
+{
+    SIL_REQUIRE(out && x_coords && y_coords && mu_nodal, SIL_ERR_INVALID, "magnetostatics2d_create: null argument");
+    SIL_REQUIRE(nx >= 2 && ny >= 2, SIL_ERR_INVALID, "magnetostatics2d_create: need ≥ 2 nodes per axis");
+    SIL_REQUIRE(criterion >= 0 && criterion <= 2, SIL_ERR_INVALID, "magnetostatics2d_create: unknown criterion %d",
+                criterion);
+    auto* op = new (std::nothrow) sil_operator_s();
+    SIL_REQUIRE(op != nullptr, SIL_ERR_INVALID, "magnetostatics2d_create: out of host memory");
+    op->kind = OP_MAGNETOSTATICS2D;
+    op->n = nx * ny;
+    Ms2dData& d = op->m2;
+    d.nx = nx;
+    d.ny = ny;
+    d.criterion = criterion;
+    d.terms = criterion == 2 ? 2.0 : 1.0;
+    int rc = upload(&d.coord[0], std::vector<double>(x_coords, x_coords + nx));
+    if (rc == SIL_OK) rc = upload(&d.coord[1], std::vector<double>(y_coords, y_coords + ny));
+    if (rc == SIL_OK) {
+        hipError_t e1 = hipMalloc(&d.nu, sizeof(double) * nx * ny);
+        hipError_t e2 = hipMalloc(&d.nu32, sizeof(float) * nx * ny);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            set_error("magnetostatics2d_create: device allocation failed");
+            rc = SIL_ERR_HIP;
+        } else {
+            rc = ms3d_nu_from_mu(mu_nodal, d.nu, d.nu32, nx * ny, 0);
+            hipError_t e3 = hipDeviceSynchronize();
+            if (rc == SIL_OK && e3 != hipSuccess) {
+                set_error("magnetostatics2d_create: %s", hipGetErrorString(e3));
+                rc = SIL_ERR_HIP;
+            }
+        }
+    }
+    if (rc != SIL_OK) {
+        free_operator(op);
+        return rc;
+    }
+    *out = op;
+    return SIL_OK;
+}
+
+int sil_magnetostatics2d_nonlinear_create(sil_operator* out, int64_t nx, int64_t ny, const double* x_coords,
+                                          const double* y_coords, const double* mu_nodal,
+                                          const double* ferromagnetic_nodal, const double* bh_b, const double* bh_h,
+                                          int32_t nsamples, int32_t criterion)
+{
+    SIL_REQUIRE(out && ferromagnetic_nodal && bh_b && bh_h, SIL_ERR_INVALID,
+                "magnetostatics2d_nonlinear_create: null argument");
+    int rc = check_curve(bh_b, nsamples);
+    if (rc) return rc;
+    sil_operator op = nullptr;
+    rc = sil_magnetostatics2d_create(&op, nx, ny, x_coords, y_coords, mu_nodal, criterion);
+    if (rc) return rc;
+    op->kind = OP_MS2D_NONLINEAR;
+    rc = attach_material(op, nx * ny, ferromagnetic_nodal, bh_b, bh_h, nsamples);
+    if (rc != SIL_OK) {
+        free_operator(op);
+        return rc;
+    }
+    op->jac->kind = OP_MS2D_JACOBIAN;
+    op->jac->n = op->n;
+    op->jac->m2 = op->m2;
+    op->jac->nl = op->nl;
+    *out = op;
+    return SIL_OK;
+}
+
+int sil_magnetostatics2d_apply_jacobian(sil_operator op, const double* state, const double* in, double* out,
+                                        void* stream)
+{
+    SIL_REQUIRE(op && state && in && out, SIL_ERR_INVALID, "magnetostatics2d_apply_jacobian: null argument");
+    SIL_REQUIRE(op->kind == OP_MS2D_NONLINEAR, SIL_ERR_INVALID, "apply_jacobian: not a 2D nonlinear operator");
+    hipStream_t s = as_stream(stream);
+    int rc = ms2d_state(op->m2, op->nl, state, s);
+    if (rc) return rc;
+    return ms2d_jacobian_apply64(op->m2, op->nl, in, out, DotEpilogue{nullptr}, nullptr, nullptr, s);
+}
+
+int sil_magnetostatics2d_jacobian_diagonal(sil_operator op, const double* state, double* diag, void* stream)
+{
+    SIL_REQUIRE(op && state && diag, SIL_ERR_INVALID, "magnetostatics2d_jacobian_diagonal: null argument");
+    SIL_REQUIRE(op->kind == OP_MS2D_NONLINEAR, SIL_ERR_INVALID, "jacobian_diagonal: not a 2D nonlinear operator");
+    hipStream_t s = as_stream(stream);
+    int rc = ms2d_state(op->m2, op->nl, state, s);
+    if (rc) return rc;
+    return ms2d_jacobian_diagonal(op->m2, op->nl, diag, s);
+}
+
+int sil_magnetostatics2d_nodal_mu(int64_t nx, int64_t ny, const double* cell_mu, double mu0, double* mu_nodal,
+                                  void* stream)
+{
+    SIL_REQUIRE(cell_mu && mu_nodal, SIL_ERR_INVALID, "magnetostatics2d_nodal_mu: null argument");
+    SIL_REQUIRE(nx >= 2 && ny >= 2, SIL_ERR_INVALID, "magnetostatics2d_nodal_mu: need ≥ 2 nodes per axis");
+    return ms2d_nodal_mu(nx, ny, cell_mu, mu0, mu_nodal, as_stream(stream));
+}
+
+int sil_magnetostatics2d_rhs(sil_operator op, const double* cell_current_z, double* rhs, void* stream)
+{
+    SIL_REQUIRE(op && cell_current_z && rhs, SIL_ERR_INVALID, "magnetostatics2d_rhs: null argument");
+    SIL_REQUIRE(op->kind == OP_MAGNETOSTATICS2D || op->kind == OP_MS2D_NONLINEAR, SIL_ERR_INVALID,
+                "magnetostatics2d_rhs: not a 2D magnetostatics operator");
+    return ms2d_rhs(op->m2, cell_current_z, op->m2.criterion == 0, rhs, as_stream(stream));
+}
+
+int sil_magnetostatics2d_post_process(sil_operator op, const double* potential, const double* cell_mu,
+                                      const double* cell_nonlinear, double* induction, double* field,
+                                      double* maxwell_stress, double* force_density, void* stream)
+{
+    SIL_REQUIRE(op && potential && cell_mu, SIL_ERR_INVALID, "magnetostatics2d_post_process: null argument");
+    SIL_REQUIRE(op->kind == OP_MAGNETOSTATICS2D || op->kind == OP_MS2D_NONLINEAR, SIL_ERR_INVALID,
+                "magnetostatics2d_post_process: not a 2D magnetostatics operator");
+    SIL_REQUIRE(!cell_nonlinear || op->kind == OP_MS2D_NONLINEAR, SIL_ERR_INVALID,
+                "magnetostatics2d_post_process: nonlinear cells need a nonlinear operator (its B-H curve)");
+    const BhCurve* curve = op->kind == OP_MS2D_NONLINEAR ? op->nl.curve : nullptr;
+    return ms2d_post_process(op->m2, curve, potential, cell_mu, cell_nonlinear, induction, field, maxwell_stress,
                              force_density, as_stream(stream));
 }
 
@@ -399,6 +535,12 @@ int sil_operator_apply(sil_operator op, const double* in, double* out, void* str
         if (rc) return rc;
         return ms3d_state_apply64(op->ms, op->nl, in, out, s);
     }
+    if (op->kind == OP_MS2D_NONLINEAR) {
+        hipStream_t s = as_stream(stream);
+        int rc = ms2d_state(op->m2, op->nl, in, s);
+        if (rc) return rc;
+        return ms2d_state_apply64(op->m2, op->nl, in, out, s);
+    }
     return operator_apply64(op, in, out, as_stream(stream));
 }
 
@@ -445,7 +587,7 @@ int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs,
                     settings->preconditioner == SIL_PRECOND_SPD_ISAI,
                 SIL_ERR_INVALID, "unknown preconditioner %d", settings->preconditioner);
     *diagnostics = sil_solver_diagnostics{};
-    if (op->kind == OP_MS3D_NONLINEAR) {
+    if (op->kind == OP_MS3D_NONLINEAR || op->kind == OP_MS2D_NONLINEAR) {
         SIL_REQUIRE(settings->preconditioner != SIL_PRECOND_SPD_ISAI, SIL_ERR_UNSUPPORTED,
                     "nonlinear solve: identity or Jacobi preconditioning");
         return newton_solve(op, rhs, solution, *settings, *diagnostics, as_stream(stream));

@@ -326,6 +326,8 @@ int apply64(sil_operator op, const double* in, double* out, DotEpilogue dot, con
     if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_apply64(op->ms, in, out, dot, pu, nb, s);
     if (op->kind == OP_CSR) return csr_apply64(op->csr, in, out, dot, pu, nb, s);
     if (op->kind == OP_MS3D_JACOBIAN) return ms3d_jacobian_apply64(op->ms, op->nl, in, out, dot, pu, nb, s);
+    if (op->kind == OP_MAGNETOSTATICS2D) return ms2d_apply64(op->m2, in, out, dot, pu, nb, s);
+    if (op->kind == OP_MS2D_JACOBIAN) return ms2d_jacobian_apply64(op->m2, op->nl, in, out, dot, pu, nb, s);
     SIL_REQUIRE(op->kind == OP_POISSON2D, SIL_ERR_UNSUPPORTED, "operator kind %d has no linear application", op->kind);
     return p2d_apply64(op->p2, in, out, dot, pu, nb, s);
 }
@@ -335,6 +337,8 @@ int apply32(sil_operator op, const float* in, float* out, DotEpilogue dot, const
 {
     if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_apply32(op->ms, in, out, dot, pu, nb, s);
     if (op->kind == OP_MS3D_JACOBIAN) return ms3d_jacobian_apply32(op->ms, op->nl, in, out, dot, pu, nb, s);
+    if (op->kind == OP_MAGNETOSTATICS2D) return ms2d_apply32(op->m2, in, out, dot, pu, nb, s);
+    if (op->kind == OP_MS2D_JACOBIAN) return ms2d_jacobian_apply32(op->m2, op->nl, in, out, dot, pu, nb, s);
     SIL_REQUIRE(op->kind != OP_CSR, SIL_ERR_UNSUPPORTED, "assembled operators have FP64 values only");
     SIL_REQUIRE(op->kind == OP_POISSON2D, SIL_ERR_UNSUPPORTED, "operator kind %d has no FP32 application", op->kind);
     return p2d_apply32(op->p2, in, out, dot, pu, nb, s);
@@ -345,6 +349,8 @@ int diagonal(sil_operator op, double* d, hipStream_t s)
     if (op->kind == OP_MAGNETOSTATICS3D) return ms3d_diagonal(op->ms, d, s);
     if (op->kind == OP_CSR) return csr_diagonal(op->csr, d, s);
     if (op->kind == OP_MS3D_JACOBIAN) return ms3d_jacobian_diagonal(op->ms, op->nl, d, s);
+    if (op->kind == OP_MAGNETOSTATICS2D) return ms2d_diagonal(op->m2, d, s);
+    if (op->kind == OP_MS2D_JACOBIAN) return ms2d_jacobian_diagonal(op->m2, op->nl, d, s);
     SIL_REQUIRE(op->kind == OP_POISSON2D, SIL_ERR_UNSUPPORTED, "operator kind %d has no diagonal", op->kind);
     return p2d_diagonal(op->p2, d, s);
 }
@@ -386,6 +392,8 @@ int ensure_workspace(sil_operator op, bool mixed, hipStream_t s)
             nb = static_cast<int>((op->ms.nx * op->ms.ny * op->ms.nz + 255) / 256);
         else if (op->kind == OP_POISSON2D)
             nb = static_cast<int>((op->p2.nx * op->p2.ny + 255) / 256);
+        else if (op->kind == OP_MAGNETOSTATICS2D || op->kind == OP_MS2D_JACOBIAN)
+            nb = static_cast<int>((op->m2.nx * op->m2.ny + 255) / 256);
         w.apply_blocks = nb;
         (void)s;
     }
@@ -550,8 +558,8 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
             if ((rc = csr_assemble_ms3d(op->ms, assembled, s))) return rc;
             a = &assembled;
         }
-        SIL_REQUIRE(op->kind != OP_POISSON2D, SIL_ERR_UNSUPPORTED,
-                    "SpdIsai needs an assembled operator or the magnetostatics operator");
+        SIL_REQUIRE(op->kind == OP_CSR || op->kind == OP_MAGNETOSTATICS3D, SIL_ERR_UNSUPPORTED,
+                    "SpdIsai needs an assembled operator or the 3D magnetostatics operator");
         rc = spd_isai_generate(*a, w.isai_g, w.isai_gt, s);
         csr_free(assembled);
         if (rc) return rc;

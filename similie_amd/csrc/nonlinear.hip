@@ -49,24 +49,6 @@ constexpr int kNormBlocks = 1024;
 
 // ------------------------------------------------------------ B-H curve ---
 
-// f, f′, f″ of f(q) = h(√q)/√q (h_over_b_from_q and its q-derivatives)
-__device__ void h_over_b_q(const BhCurve& c, double q, double& f, double& f1, double& f2)
-{
-    if (q == 0.0) {
-        f = dh_db_from_b(c, 0.0);
-        f1 = 0.0;
-        f2 = 0.0;
-        return;
-    }
-    const double b = sqrt(q);
-    const double h = h_from_b(c, b), dh = dh_db_from_b(c, b);
-    f = h / b;
-    const double d1 = (dh * b - h) / (b * b);             // d(h/b)/db
-    const double d2 = 2.0 * (h - dh * b) / (b * b * b);   // d²(h/b)/db²
-    f1 = d1 / (2.0 * b);
-    f2 = d2 / (4.0 * q) - d1 / (4.0 * q * b);
-}
-
 // ---------------------------------------------------------------- state ---
 
 struct StateArgs {
@@ -298,7 +280,11 @@ int evaluate(sil_operator op, const double* b, const double* x, double* fx, doub
 {
     NewtonBuffers& nb = *op->newton;
     int rc;
-    if ((rc = ms3d_state(op->ms, op->nl, x, s)) || (rc = ms3d_state_apply64(op->ms, op->nl, x, fx, s))) return rc;
+    if (op->kind == OP_MS2D_NONLINEAR) {
+        if ((rc = ms2d_state(op->m2, op->nl, x, s)) || (rc = ms2d_state_apply64(op->m2, op->nl, x, fx, s))) return rc;
+    } else if ((rc = ms3d_state(op->ms, op->nl, x, s)) || (rc = ms3d_state_apply64(op->ms, op->nl, x, fx, s))) {
+        return rc;
+    }
     hipLaunchKernelGGL(k_residual, dim3(kNormBlocks), dim3(kBlock), 0, s, b, fx, r, op->n, nb.partials);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, nb.partials, kNormBlocks, nb.partials + kNormBlocks);
     SIL_LAUNCH_CHECK();

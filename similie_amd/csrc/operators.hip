@@ -18,33 +18,15 @@
 
 #include <algorithm>
 
+#include "apply_epilogue.hpp"
 #include "operators.hpp"
 
 namespace sil {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = kApplyBlock;
 constexpr int64_t kMaxDotBlocks = 1 << 18;
-
-template <typename T>
-__device__ __forceinline__ T p_value(const PUpdate<T>* pu, const T* x, int64_t idx)
-{
-    // value of the direction at row idx: either x[idx] or z + β p_old
-    if (pu == nullptr) return x[idx];
-    const double beta = *pu->beta;
-    const double z = pu->inv_diag ? (double)pu->r[idx] * pu->inv_diag[idx] : (double)pu->r[idx];
-    return (T)(z + beta * (double)pu->p_old[idx]);
-}
-
-template <typename T>
-__device__ __forceinline__ void block_dot_store(double v, double* partials)
-{
-    using BR = hipcub::BlockReduce<double, kBlock>;
-    __shared__ typename BR::TempStorage tmp;
-    double s = BR(tmp).Sum(v);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
-}
 
 // --------------------------------------------------------- magnetostatics ---
 

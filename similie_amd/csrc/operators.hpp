@@ -5,7 +5,16 @@
 
 namespace sil {
 
-enum OperatorKind { OP_MAGNETOSTATICS3D = 1, OP_POISSON2D = 2, OP_CSR = 3, OP_MS3D_NONLINEAR = 4, OP_MS3D_JACOBIAN = 5 };
+enum OperatorKind {
+    OP_MAGNETOSTATICS3D = 1,
+    OP_POISSON2D = 2,
+    OP_CSR = 3,
+    OP_MS3D_NONLINEAR = 4,
+    OP_MS3D_JACOBIAN = 5,
+    OP_MAGNETOSTATICS2D = 6,
+    OP_MS2D_NONLINEAR = 7,
+    OP_MS2D_JACOBIAN = 8
+};
 
 struct Workspace;     // CG vectors, scalars and captured graphs (krylov.hip)
 struct NewtonBuffers; // Newton vectors (nonlinear.hip)
@@ -41,6 +50,19 @@ struct NlData {
     float* st_a32 = nullptr;
     float* st_c32 = nullptr;
     float* st_b32[3] = {nullptr, nullptr, nullptr};
+};
+
+// 2D magnetostatics on A_z (one unknown per node, i + nx·j): the moment
+// stencils B_x = A_z(s+e_y) − A_z(s), B_y = A_z(s) − A_z(s+e_x) sampled at
+// interior nodes, no Hodge weights; `terms` is 2 when the criterion adds the
+// potential and the moments parts (each equal), else 1
+struct Ms2dData {
+    int64_t nx, ny;
+    int criterion;      // 0 potential, 1 moments, 2 both temporal derivatives
+    double terms;
+    double* coord[2];   // coordinates (device)
+    double* nu;         // 1/μ per node (device)
+    float* nu32;
 };
 
 struct Poisson2dData {
@@ -94,6 +116,25 @@ int ms3d_post_process(const Ms3dData& d, const BhCurve* curve, const double* pot
                       const double* cell_nonlinear, double* induction, double* field, double* stress,
                       double* force, hipStream_t s);
 
+// 2D magnetostatics (magnetostatics2d.hip); nl uses st_b[0..1] only
+int ms2d_apply64(const Ms2dData& d, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,
+                 int* nblocks_out, hipStream_t s);
+int ms2d_apply32(const Ms2dData& d, const float* in, float* out, DotEpilogue dot, const PUpdate<float>* pu,
+                 int* nblocks_out, hipStream_t s);
+int ms2d_diagonal(const Ms2dData& d, double* diag, hipStream_t s);
+int ms2d_nodal_mu(int64_t nx, int64_t ny, const double* cell_mu, double mu0, double* mu, hipStream_t s);
+int ms2d_rhs(const Ms2dData& d, const double* cell_current_z, bool zero, double* rhs, hipStream_t s);
+int ms2d_state(const Ms2dData& d, const NlData& nl, const double* x, hipStream_t s);
+int ms2d_state_apply64(const Ms2dData& d, const NlData& nl, const double* x, double* out, hipStream_t s);
+int ms2d_jacobian_apply64(const Ms2dData& d, const NlData& nl, const double* in, double* out, DotEpilogue dot,
+                          const PUpdate<double>* pu, int* nblocks_out, hipStream_t s);
+int ms2d_jacobian_apply32(const Ms2dData& d, const NlData& nl, const float* in, float* out, DotEpilogue dot,
+                          const PUpdate<float>* pu, int* nblocks_out, hipStream_t s);
+int ms2d_jacobian_diagonal(const Ms2dData& d, const NlData& nl, double* diag, hipStream_t s);
+int ms2d_post_process(const Ms2dData& d, const BhCurve* curve, const double* potential, const double* cell_mu,
+                      const double* cell_nonlinear, double* induction, double* field, double* stress,
+                      double* force, hipStream_t s);
+
 // p_new = z + β p_old over n rows (the CG direction) as its own pass
 int direction_pass64(const PUpdate<double>& pu, int64_t n, hipStream_t s);
 
@@ -143,6 +184,7 @@ struct sil_operator_s {
     int64_t n;
     sil::Ms3dData ms;
     sil::Poisson2dData p2;
+    sil::Ms2dData m2;
     sil::CsrData csr;
     sil::NlData nl;
     sil::Workspace* ws;

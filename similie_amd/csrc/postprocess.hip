@@ -1,4 +1,5 @@
-// Cell post-processing of a 3D magnetostatics solution.
+// Cell post-processing of a magnetostatics solution (3D grids, and 2D grids
+// as one slice of cells).
 //
 // The reference post-processes on the host, cell by cell
 // (onelab_interface/magnetostatics_onelab.hpp:3984-4046): the induction
@@ -63,6 +64,11 @@ __device__ inline double centre_spacing(const double* c, int64_t m, int64_t i)
     return 0.0;
 }
 
+// TWO_D: one slice of a 2D grid whose potential is A_z per node; the cell
+// induction is the reconstructed in-plane moment pair
+// (fill_post_process_fields_on_cell_domain, :3866-3982): B_x = (A_z(s+e_y)
+// − A_z(s))/h_y, B_y = (A_z(s) − A_z(s+e_x))/h_x, B_z = 0
+template <bool TWO_D>
 __global__ void __launch_bounds__(kPostBlock) k_ms3d_post(PostArgs p)
 {
     __shared__ double s_t[6][kHy * kHx];
@@ -79,9 +85,8 @@ __global__ void __launch_bounds__(kPostBlock) k_ms3d_post(PostArgs p)
     const int64_t i0 = tx * kTx, j0 = ty * kTy;
     const double* xc = p.coord[0];
     const double* yc = p.coord[1];
-    const double* zc = p.coord[2];
-    const int64_t sx = 3, sy = 3 * p.nx, sz = 3 * p.nx * p.ny;
-    const double wz = zc[k + 1] - zc[k];
+    const int64_t sx = TWO_D ? 1 : 3, sy = sx * p.nx, sz = sy * p.ny;
+    const double wz = TWO_D ? 1.0 : p.coord[2][k + 1] - p.coord[2][k];
 
     for (int t = threadIdx.x; t < kHx * kHy; t += kPostBlock) {
         const int lx = t % kHx, ly = t / kHx;
@@ -90,21 +95,34 @@ __global__ void __launch_bounds__(kPostBlock) k_ms3d_post(PostArgs p)
         if (i >= 0 && i < mx && j >= 0 && j < my) {
             const int64_t cell = i + mx * (j + my * k);
             const double* a0 = p.a + (i * sx + j * sy + k * sz);
-            // forward_value moments, the taps in the stencil's order
-            double m0 = a0[sy + 2];
-            m0 = m0 - a0[2];
-            m0 = m0 - a0[sz + 1];
-            m0 = m0 + a0[1];
-            double m1 = -a0[sx + 2];
-            m1 = m1 + a0[2];
-            m1 = m1 + a0[sz + 0];
-            m1 = m1 - a0[0];
-            double m2 = a0[sx + 1];
-            m2 = m2 - a0[1];
-            m2 = m2 - a0[sy + 0];
-            m2 = m2 + a0[0];
             const double wx = xc[i + 1] - xc[i], wy = yc[j + 1] - yc[j];
-            const double B[3] = {m0 / (wy * wz), m1 / (wx * wz), m2 / (wx * wy)};
+            double B[3];
+            if constexpr (TWO_D) {
+                const double mx_ = -a0[0] + a0[sy];  // moment of B_x, the stencil's order
+                const double my_ = a0[0] - a0[sx];   // moment of B_y
+                const double rx = fabs(wx) < 1e-14 ? 0.0 : 1.0 / wx;
+                const double ry = fabs(wy) < 1e-14 ? 0.0 : 1.0 / wy;
+                B[0] = ry * mx_;
+                B[1] = rx * my_;
+                B[2] = 0.0;
+            } else {
+                // forward_value moments, the taps in the stencil's order
+                double m0 = a0[sy + 2];
+                m0 = m0 - a0[2];
+                m0 = m0 - a0[sz + 1];
+                m0 = m0 + a0[1];
+                double m1 = -a0[sx + 2];
+                m1 = m1 + a0[2];
+                m1 = m1 + a0[sz + 0];
+                m1 = m1 - a0[0];
+                double m2 = a0[sx + 1];
+                m2 = m2 - a0[1];
+                m2 = m2 - a0[sy + 0];
+                m2 = m2 + a0[0];
+                B[0] = m0 / (wy * wz);
+                B[1] = m1 / (wx * wz);
+                B[2] = m2 / (wx * wy);
+            }
             double H[3] = {0.0, 0.0, 0.0};
             if (p.nonlinear && p.nonlinear[cell] != 0.0) {
                 const double q = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
@@ -172,7 +190,45 @@ __global__ void __launch_bounds__(kPostBlock) k_ms3d_post(PostArgs p)
     }
 }
 
+int launch_post(PostArgs p, bool two_d, hipStream_t s)
+{
+    const int64_t mx = p.nx - 1, my = p.ny - 1, mz = p.nz - 1;
+    p.tiles_x = (mx + kTx - 1) / kTx;
+    p.tiles_y = (my + kTy - 1) / kTy;
+    p.tiles = p.tiles_x * p.tiles_y * mz;
+    p.per_xcd = (p.tiles + kXcdCount - 1) / kXcdCount;
+    const int64_t blocks = p.per_xcd * kXcdCount;
+    SIL_REQUIRE(blocks <= INT32_MAX, SIL_ERR_UNSUPPORTED, "magnetostatics post_process: grid too large");
+    if (two_d)
+        hipLaunchKernelGGL(k_ms3d_post<true>, dim3((unsigned)blocks), dim3(kPostBlock), 0, s, p);
+    else
+        hipLaunchKernelGGL(k_ms3d_post<false>, dim3((unsigned)blocks), dim3(kPostBlock), 0, s, p);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
 }  // namespace
+
+int ms2d_post_process(const Ms2dData& d, const BhCurve* curve, const double* potential, const double* cell_mu,
+                      const double* cell_nonlinear, double* induction, double* field, double* stress,
+                      double* force, hipStream_t s)
+{
+    PostArgs p{};
+    p.nx = d.nx;
+    p.ny = d.ny;
+    p.nz = 2;  // one slice of cells
+    p.coord[0] = d.coord[0];
+    p.coord[1] = d.coord[1];
+    p.a = potential;
+    p.mu = cell_mu;
+    p.nonlinear = cell_nonlinear;
+    p.curve = curve;
+    p.induction = induction;
+    p.field = field;
+    p.stress = stress;
+    p.force = force;
+    return launch_post(p, true, s);
+}
 
 int ms3d_post_process(const Ms3dData& d, const BhCurve* curve, const double* potential, const double* cell_mu,
                       const double* cell_nonlinear, double* induction, double* field, double* stress,
@@ -191,16 +247,7 @@ int ms3d_post_process(const Ms3dData& d, const BhCurve* curve, const double* pot
     p.field = field;
     p.stress = stress;
     p.force = force;
-    const int64_t mx = d.nx - 1, my = d.ny - 1, mz = d.nz - 1;
-    p.tiles_x = (mx + kTx - 1) / kTx;
-    p.tiles_y = (my + kTy - 1) / kTy;
-    p.tiles = p.tiles_x * p.tiles_y * mz;
-    p.per_xcd = (p.tiles + kXcdCount - 1) / kXcdCount;
-    const int64_t blocks = p.per_xcd * kXcdCount;
-    SIL_REQUIRE(blocks <= INT32_MAX, SIL_ERR_UNSUPPORTED, "magnetostatics3d_post_process: grid too large");
-    hipLaunchKernelGGL(k_ms3d_post, dim3((unsigned)blocks), dim3(kPostBlock), 0, s, p);
-    SIL_LAUNCH_CHECK();
-    return SIL_OK;
+    return launch_post(p, false, s);
 }
 
 }  // namespace sil

@@ -235,6 +235,107 @@ def spd_isai(operator, stream=None):
     return AssembledOperator._adopt(h)
 
 
+CRITERIA = {"potential": 0, "moments": 1, "potential_and_moments": 2}
+
+
+class MagnetostaticsOperator2D(LinearOperator):
+    """MagnetostaticsOperator2D (magnetostatics_onelab.hpp:1013-1522): A_z on
+    a rectilinear node grid, row i + nx·j; interior rows gather ∂H/∂B at the
+    interior sampled nodes of the moments B_x = A(s+e_y) − A(s),
+    B_y = A(s) − A(s+e_x); boundary rows are the identity.  criterion:
+    "moments" (the reference's default), "potential" or
+    "potential_and_moments" (row sums counted twice)."""
+
+    def __init__(self, x_coords, y_coords, mu_nodal, criterion="moments"):
+        self.shape = (len(x_coords), len(y_coords))
+        self.criterion = criterion
+        _f64(mu_nodal, "mu_nodal")
+        h = ctypes.c_void_p()
+        xs, ys = _host_array(x_coords), _host_array(y_coords)
+        L.check(L.lib().sil_magnetostatics2d_create(ctypes.byref(h), *self.shape, ctypes.cast(xs, ctypes.c_void_p),
+                                                    ctypes.cast(ys, ctypes.c_void_p), L.ptr(mu_nodal),
+                                                    CRITERIA[criterion]))
+        self.handle = h
+        self._destroy = L.lib().sil_operator_destroy
+
+    @staticmethod
+    def nodal_mu(shape, cell_mu, mu0, stream=None):
+        """mean of the adjacent cells' μ (magnetostatics_onelab.hpp:4467-4500)"""
+        nx, ny = shape
+        _f64(cell_mu, "cell_mu")
+        mu = torch.empty(nx * ny, dtype=torch.float64, device="cuda")
+        L.check(L.lib().sil_magnetostatics2d_nodal_mu(nx, ny, L.ptr(cell_mu), float(mu0), L.ptr(mu),
+                                                      L.stream_handle(stream)))
+        return mu
+
+    def rhs(self, cell_current_z, stream=None):
+        """¼·area·J_z over the cells around each interior node
+        (magnetostatics_onelab.hpp:4413-4445); cell_current_z: (cells,)"""
+        _f64(cell_current_z, "cell_current_z")
+        b = torch.empty(self.size(), dtype=torch.float64, device="cuda")
+        L.check(L.lib().sil_magnetostatics2d_rhs(self.handle, L.ptr(cell_current_z), L.ptr(b),
+                                                 L.stream_handle(stream)))
+        return b
+
+    def post_process(self, potential, cell_mu, cell_nonlinear=None, stream=None):
+        """CellPostProcessFields of every cell of the 2D grid
+        (magnetostatics_onelab.hpp:3866-3982, 4048-4213), cells x fastest"""
+        _f64(potential, "potential")
+        _f64(cell_mu, "cell_mu")
+        if cell_nonlinear is not None:
+            _f64(cell_nonlinear, "cell_nonlinear")
+        nx, ny = self.shape
+        cells = (nx - 1) * (ny - 1)
+        out = [torch.empty((cells, w), dtype=torch.float64, device="cuda") for w in (3, 3, 6, 3)]
+        L.check(L.lib().sil_magnetostatics2d_post_process(
+            self.handle, L.ptr(potential), L.ptr(cell_mu), L.ptr(cell_nonlinear) if cell_nonlinear is not None else None,
+            *[L.ptr(o) for o in out], L.stream_handle(stream)))
+        return CellPostProcessFields(*out)
+
+
+class NonlinearMagnetostaticsOperator2D(MagnetostaticsOperator2D):
+    """MagnetostaticsOperator2D with MaterialMagnetostaticsHamiltonian
+    (magnetostatics_onelab.hpp:957-1011): apply(x) is F(x) and
+    minimize_strong_formulation_residual runs Newton."""
+
+    def __init__(self, x_coords, y_coords, mu_nodal, ferromagnetic_nodal, bh_b, bh_h, criterion="moments"):
+        self.shape = (len(x_coords), len(y_coords))
+        self.criterion = criterion
+        _f64(mu_nodal, "mu_nodal")
+        _f64(ferromagnetic_nodal, "ferromagnetic_nodal")
+        if len(bh_b) != len(bh_h):
+            raise ValueError("bh_b and bh_h must have the same length")
+        h = ctypes.c_void_p()
+        xs, ys = _host_array(x_coords), _host_array(y_coords)
+        bb, hh = _host_array(bh_b), _host_array(bh_h)
+        L.check(L.lib().sil_magnetostatics2d_nonlinear_create(
+            ctypes.byref(h), *self.shape, ctypes.cast(xs, ctypes.c_void_p), ctypes.cast(ys, ctypes.c_void_p),
+            L.ptr(mu_nodal), L.ptr(ferromagnetic_nodal), ctypes.cast(bb, ctypes.c_void_p),
+            ctypes.cast(hh, ctypes.c_void_p), len(bh_b), CRITERIA[criterion]))
+        self.handle = h
+        self._destroy = L.lib().sil_operator_destroy
+
+    @staticmethod
+    def nodal_fraction(shape, cell_flags, stream=None):
+        """mean over the adjacent cells of a 0/1 nonlinear-material flag"""
+        return MagnetostaticsOperator2D.nodal_mu(shape, cell_flags, 0.0, stream)
+
+    def apply_jacobian(self, state, v, out=None, stream=None):
+        _f64(state, "state")
+        _f64(v, "v")
+        out = torch.empty_like(v) if out is None else _f64(out, "out")
+        L.check(L.lib().sil_magnetostatics2d_apply_jacobian(self.handle, L.ptr(state), L.ptr(v), L.ptr(out),
+                                                             L.stream_handle(stream)))
+        return out
+
+    def jacobian_diagonal(self, state, stream=None):
+        _f64(state, "state")
+        d = torch.empty(self.size(), dtype=torch.float64, device="cuda")
+        L.check(L.lib().sil_magnetostatics2d_jacobian_diagonal(self.handle, L.ptr(state), L.ptr(d),
+                                                               L.stream_handle(stream)))
+        return d
+
+
 class Poisson2D(LinearOperator):
     """Dirichlet-zero d0ᵀ ⋆1^κ d0 on the interior of a rectilinear nx×ny grid
     (x slowest); vectors have nx·ny entries with zero boundary entries."""

@@ -71,6 +71,23 @@ def magnetostatics_nonlinear_inductor(cells=192, current=2.0e3):
     return [np.arange(cells + 1, dtype=np.float64)] * 3, mu, flags, j * current
 
 
+def magnetostatics_2d_inductor(cells=2048, current=3.0e3):
+    """A 2D cross-section of an inductor on unit-spaced nodes, for the A_z
+    operator: a core of cells/4 × cells/2 in the middle (μ_r 1000, or the
+    nonlinear material), two coil slabs cells/8 wide beside it carrying ±J_z
+    = ±current.  (coordinates, cell μ, cell nonlinear flag, cell J_z), cells
+    x fastest.  With current 3000 the core reaches |B| of order 1."""
+    mid = np.arange(cells) + 0.5 - cells / 2.0
+    dx = mid[None, :]
+    dy = mid[:, None]
+    core = (np.abs(dx) < cells / 8) & (np.abs(dy) < cells / 4)
+    mu = np.where(core, 1000.0, 1.0) * MU0
+    slab = (np.abs(dx) >= cells / 6) & (np.abs(dx) < cells / 6 + cells / 8) & (np.abs(dy) < cells / 4)
+    jz = np.where(slab, np.sign(dx) * current, 0.0)
+    coords = [np.arange(cells + 1, dtype=np.float64)] * 2
+    return coords, mu.reshape(-1), core.astype(np.float64).reshape(-1), jz.reshape(-1)
+
+
 def gaussian_random_field(shape, sigma, corr, generator, device="cuda"):
     """One field of the given 2D shape with covariance σ² exp(−r²/(2ℓ²)),
     spectral synthesis on a periodic grid twice as large, cropped and

@@ -38,6 +38,7 @@ def lib():
         _lib.orc_spd_isai.restype = ctypes.c_long
         _lib.orc_bh_eval.restype = ctypes.c_double
         _lib.orc_nl_newton.restype = ctypes.c_int
+        _lib.orc_ms2d_newton.restype = ctypes.c_int
     return _lib
 
 
@@ -421,3 +422,79 @@ class StagedLaplacian3D1:
 
 def threads():
     return int(lib().orc_threads())
+
+
+CRITERIA = {"potential": 0, "moments": 1, "potential_and_moments": 2}
+
+
+class Magnetostatics2D:
+    """the 2D A_z operator (linear when ferro is None) on an nx × ny node
+    grid, rows i + nx·j"""
+
+    def __init__(self, shape, mu, ferro=None, bh_b=None, bh_h=None, criterion="moments"):
+        self.nx, self.ny = shape
+        self.mu = _f64(mu)
+        self.ferro = _f64(ferro) if ferro is not None else None
+        self.bh_b = _f64(bh_b if bh_b is not None else [0.0, 1.0])
+        self.bh_h = _f64(bh_h if bh_h is not None else [0.0, 1.0])
+        self.criterion = CRITERIA[criterion]
+        self.n = self.nx * self.ny
+
+    def _head(self):
+        return [ctypes.c_long(self.nx), ctypes.c_long(self.ny), _d(self.mu),
+                _d(self.ferro) if self.ferro is not None else None, _d(self.bh_b), _d(self.bh_h),
+                ctypes.c_int(len(self.bh_b)), ctypes.c_int(self.criterion)]
+
+    def apply(self, x, state=None):
+        out = np.zeros(self.n)
+        xx = _f64(x)
+        st = _f64(state) if state is not None else None
+        lib().orc_ms2d_apply(*self._head(), _d(st) if st is not None else None, _d(xx), _d(out))
+        return out
+
+    def diagonal(self, state=None):
+        out = np.zeros(self.n)
+        st = _f64(state) if state is not None else None
+        lib().orc_ms2d_diagonal(*self._head(), _d(st) if st is not None else None, _d(out))
+        return out
+
+    def newton(self, b, tol, max_inner, jacobi=True):
+        x = np.zeros(self.n)
+        out = np.zeros(4)
+        bb = _f64(b)
+        ok = lib().orc_ms2d_newton(*self._head(), _d(bb), ctypes.c_double(tol), ctypes.c_int(max_inner),
+                                   ctypes.c_int(1 if jacobi else 0), _d(x), _d(out))
+        return x, {"converged": bool(ok), "newton_steps": int(out[0]), "iterations": int(out[1]),
+                   "initial_residual_l2": out[2], "final_residual_l2": out[3]}
+
+
+def ms2d_rhs(coords, cell_jz, criterion="moments"):
+    xc, yc = (_f64(c) for c in coords)
+    out = np.zeros(len(xc) * len(yc))
+    jz = _f64(cell_jz)
+    lib().orc_ms2d_rhs(ctypes.c_long(len(xc)), ctypes.c_long(len(yc)), _d(xc), _d(yc), _d(jz),
+                       ctypes.c_int(CRITERIA[criterion]), _d(out))
+    return out
+
+
+def ms2d_nodal_mu(shape, cell_mu, mu0):
+    out = np.zeros(shape[0] * shape[1])
+    cm = _f64(cell_mu)
+    lib().orc_ms2d_nodal_mu(ctypes.c_long(shape[0]), ctypes.c_long(shape[1]), _d(cm), ctypes.c_double(mu0), _d(out))
+    return out
+
+
+def ms2d_post_process(coords, potential, cell_mu, cell_nonlinear=None, bh_b=None, bh_h=None):
+    """cell induction, field, Maxwell stress and force density of a 2D A_z
+    potential (cells x fastest)"""
+    xc, yc = (_f64(c) for c in coords)
+    ncell = (len(xc) - 1) * (len(yc) - 1)
+    pot, mu = _f64(potential), _f64(cell_mu)
+    nl = _f64(cell_nonlinear) if cell_nonlinear is not None else None
+    bb = _f64(bh_b if bh_b is not None else [0.0])
+    hh = _f64(bh_h if bh_h is not None else [0.0])
+    out = [np.zeros((ncell, w)) for w in (3, 3, 6, 3)]
+    lib().orc_ms2d_post_process(ctypes.c_long(len(xc)), ctypes.c_long(len(yc)), _d(xc), _d(yc), _d(pot), _d(mu),
+                                _d(nl) if nl is not None else None, _d(bb), _d(hh),
+                                ctypes.c_int(len(bb) if bh_b is not None else 0), *[_d(o) for o in out])
+    return tuple(out)
