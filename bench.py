@@ -284,7 +284,7 @@ def magnetostatics_2d(torch, max_iterations=20000):
     cmu, cfl, cj = (torch.from_numpy(a).cuda() for a in (cell_mu, flags, jz))
     mu = S.MagnetostaticsOperator2D.nodal_mu(shape, cmu, MU0)
     out = {"workload": "2D inductor cross-section, 2048^2 unit cells, core 512x1024 (mu_r 1000 / EIcore B-H), "
-                       "coil slabs 256 wide at J_z = +-375", "dofs": nodes}
+                       "coil slabs 256 wide at J_z = +-187.5", "dofs": nodes}
     op = S.MagnetostaticsOperator2D(*coords, mu)
     rhs = op.rhs(cj)
     x = torch.randn(nodes, dtype=torch.float64, device="cuda")
@@ -320,7 +320,7 @@ def magnetostatics_2d(torch, max_iterations=20000):
                                     "final_relative_residual": d["final_relative_residual"]}
     pp = nop.post_process(xs, torch.full_like(cmu, MU0), cfl)
     torch.cuda.synchronize()
-    out["core_max_B"] = round(float(torch.max(torch.linalg.norm(pp.magnetic_induction, dim=1)).item()), 4)
+    out["core_max_B"] = round(float(torch.max(torch.linalg.norm(pp.magnetic_induction, dim=1)[cfl > 0]).item()), 4)
     return out
 
 
