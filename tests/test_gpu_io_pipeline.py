@@ -138,7 +138,7 @@ def test_quadrilateral_mesh_drives_the_2d_solve_and_post_processing(physics):
     sp = IO.Silpro.parse(SILPRO % physics)
     settings = sp.solver_settings()
     nonlinear = sp.problem_settings()["physics"] == "NonLinearMagnetostatics"
-    current = 384000.0 / n  # core |B| ≈ 0.76, a few Newton steps
+    current = 768000.0 / n  # deep enough into the curve for several Newton steps (3 here)
     mu, cur, flags = IO.magnetostatics_cell_inputs(
         grid.cell_tags, MU0, core_mu=1000 * MU0, current_density_z=current,
         magnetic_material_tags=[int(v) for v in sp.values("Preprocess.MagneticMaterialTags")],
