@@ -27,6 +27,8 @@
 // tile's B, H and T, then the force from LDS.  Per cell it reads the
 // potential at the cell's four sampled nodes (mostly cache hits: 24 B of
 // HBM per node), μ and the material flag, and writes 15 doubles.
+#include <cstdlib>
+
 #include "bh_curve.hpp"
 #include "operators.hpp"
 
@@ -68,10 +70,15 @@ __device__ inline double centre_spacing(const double* c, int64_t m, int64_t i)
 // induction is the reconstructed in-plane moment pair
 // (fill_post_process_fields_on_cell_domain, :3866-3982): B_x = (A_z(s+e_y)
 // − A_z(s))/h_y, B_y = (A_z(s) − A_z(s+e_x))/h_x, B_z = 0
-template <bool TWO_D>
+// STAGED: the tile's outputs go through LDS and leave as whole rows (each
+// wavefront store covers 512 contiguous bytes) instead of one 24- or 48-byte
+// record per lane
+template <bool TWO_D, bool STAGED>
 __global__ void __launch_bounds__(kPostBlock) k_ms3d_post(PostArgs p)
 {
+    static_assert(kTx * kTy == 2 * kPostBlock, "two tile cells per thread");
     __shared__ double s_t[6][kHy * kHx];
+    __shared__ double s_o[6][STAGED ? kTx * kTy : 1];
 
     // XCD-contiguous tiles: workgroup b runs on XCD b % 8, which takes a
     // contiguous range of tiles so vertical neighbours share its L2
@@ -143,20 +150,53 @@ __global__ void __launch_bounds__(kPostBlock) k_ms3d_post(PostArgs p)
             T[4] = B[0] * H[2];
             T[5] = B[1] * H[2];
             if (lx > 0 && ly > 0) {
-                if (p.induction)
-                    for (int m = 0; m < 3; ++m) p.induction[3 * cell + m] = B[m];
-                if (p.field)
-                    for (int m = 0; m < 3; ++m) p.field[3 * cell + m] = H[m];
-                if (p.stress)
-                    for (int m = 0; m < 6; ++m) p.stress[6 * cell + m] = T[m];
+                if constexpr (STAGED) {
+                    const int o = (ly - 1) * kTx + (lx - 1);
+                    for (int m = 0; m < 3; ++m) {
+                        s_o[m][o] = B[m];
+                        s_o[3 + m][o] = H[m];
+                    }
+                } else {
+                    if (p.induction)
+                        for (int m = 0; m < 3; ++m) p.induction[3 * cell + m] = B[m];
+                    if (p.field)
+                        for (int m = 0; m < 3; ++m) p.field[3 * cell + m] = H[m];
+                    if (p.stress)
+                        for (int m = 0; m < 6; ++m) p.stress[6 * cell + m] = T[m];
+                }
             }
         }
         for (int m = 0; m < 6; ++m) s_t[m][t] = T[m];
     }
-    if (!p.force) return;
-    __syncthreads();
+    const int64_t nv = mx - i0 < kTx ? mx - i0 : kTx;  // valid cells of each tile row
+    if constexpr (STAGED) {
+        __syncthreads();
+        if (p.induction || p.field)
+            for (int e = threadIdx.x; e < kTy * 3 * kTx; e += kPostBlock) {
+                const int r = e / (3 * kTx), q = e % (3 * kTx), cl = q / 3, m = q % 3;
+                const int64_t j = j0 + r;
+                if (j >= my || cl >= nv) continue;
+                const int64_t base = 3 * (i0 + mx * (j + my * k));
+                if (p.induction) p.induction[base + q] = s_o[m][r * kTx + cl];
+                if (p.field) p.field[base + q] = s_o[3 + m][r * kTx + cl];
+            }
+        if (p.stress)
+            for (int e = threadIdx.x; e < kTy * 6 * kTx; e += kPostBlock) {
+                const int r = e / (6 * kTx), q = e % (6 * kTx), cl = q / 6, m = q % 6;
+                const int64_t j = j0 + r;
+                if (j >= my || cl >= nv) continue;
+                p.stress[6 * (i0 + mx * (j + my * k)) + q] = s_t[m][(r + 1) * kHx + cl + 1];
+            }
+        if (!p.force) return;
+    } else {
+        if (!p.force) return;
+        __syncthreads();
+    }
 
-    for (int t = threadIdx.x; t < kTx * kTy; t += kPostBlock) {
+    double fo[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    int slot = 0;
+
+    for (int t = threadIdx.x; t < kTx * kTy; t += kPostBlock, ++slot) {
         const int lx = 1 + t % kTx, ly = 1 + t / kTx;
         const int64_t i = i0 - 1 + lx, j = j0 - 1 + ly;
         if (i >= mx || j >= my) continue;
@@ -185,7 +225,23 @@ __global__ void __launch_bounds__(kPostBlock) k_ms3d_post(PostArgs p)
             acc = acc + eta_x;
             acc = acc + eta_y_w;
             acc = acc - eta_x_s;
-            p.force[3 * cell + f] = c2 * -acc;
+            if constexpr (STAGED)
+                fo[slot][f] = c2 * -acc;
+            else
+                p.force[3 * cell + f] = c2 * -acc;
+        }
+    }
+    if constexpr (STAGED) {
+        __syncthreads();  // every read of s_o and s_t is done
+        slot = 0;
+        for (int t = threadIdx.x; t < kTx * kTy; t += kPostBlock, ++slot)
+            for (int f = 0; f < 3; ++f) s_o[f][t] = fo[slot][f];
+        __syncthreads();
+        for (int e = threadIdx.x; e < kTy * 3 * kTx; e += kPostBlock) {
+            const int r = e / (3 * kTx), q = e % (3 * kTx), cl = q / 3, m = q % 3;
+            const int64_t j = j0 + r;
+            if (j >= my || cl >= nv) continue;
+            p.force[3 * (i0 + mx * (j + my * k)) + q] = s_o[m][r * kTx + cl];
         }
     }
 }
@@ -199,10 +255,19 @@ int launch_post(PostArgs p, bool two_d, hipStream_t s)
     p.per_xcd = (p.tiles + kXcdCount - 1) / kXcdCount;
     const int64_t blocks = p.per_xcd * kXcdCount;
     SIL_REQUIRE(blocks <= INT32_MAX, SIL_ERR_UNSUPPORTED, "magnetostatics post_process: grid too large");
-    if (two_d)
-        hipLaunchKernelGGL(k_ms3d_post<true>, dim3((unsigned)blocks), dim3(kPostBlock), 0, s, p);
+    static const bool staged = [] {
+        const char* e = std::getenv("SIL_POST_STAGED");
+        return !(e && e[0] == '0');
+    }();
+    const dim3 g((unsigned)blocks), b(kPostBlock);
+    if (two_d && staged)
+        hipLaunchKernelGGL((k_ms3d_post<true, true>), g, b, 0, s, p);
+    else if (two_d)
+        hipLaunchKernelGGL((k_ms3d_post<true, false>), g, b, 0, s, p);
+    else if (staged)
+        hipLaunchKernelGGL((k_ms3d_post<false, true>), g, b, 0, s, p);
     else
-        hipLaunchKernelGGL(k_ms3d_post<false>, dim3((unsigned)blocks), dim3(kPostBlock), 0, s, p);
+        hipLaunchKernelGGL((k_ms3d_post<false, false>), g, b, 0, s, p);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
