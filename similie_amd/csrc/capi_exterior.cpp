@@ -269,7 +269,7 @@ namespace sil {
 // the environment (SIL_LAP3D_TILE="variant,ty,xchunk[,zw]").
 FusedTile default_lap3d_tile()
 {
-    FusedTile t{FUSED_PIPE, 64, 12, 32};
+    FusedTile t{FUSED_PIPE, 64, 12, 32, 1};
     if (const char* e = std::getenv("SIL_LAP3D_TILE")) {
         int v = 1, ty = 8, xc = 32, zw = 64;
         const int n = std::sscanf(e, "%d,%d,%d,%d", &v, &ty, &xc, &zw);
@@ -280,6 +280,7 @@ FusedTile default_lap3d_tile()
             t.tz = n == 4 ? zw : 64;
         }
     }
+    if (const char* e = std::getenv("SIL_LAP3D_NT")) t.nt = std::atoi(e) ? 1 : 0;
     return t;
 }
 }  // namespace sil

@@ -33,6 +33,7 @@ struct FusedTile {
     int variant;  // FusedVariant
     int tz, ty;   // tile in (z, y)
     int xchunk;   // x planes per workgroup (march)
+    int nt = 0;   // pipelined 64 × 12 tile: non-temporal output stores
 };
 
 int launch_laplacian_2d_0form(const FusedGeom& g, const double* in, double* out, int64_t ox, int64_t oy,
