@@ -177,15 +177,15 @@ __global__ void __launch_bounds__(kPostBlock) k_ms3d_post(PostArgs p)
                 const int64_t j = j0 + r;
                 if (j >= my || cl >= nv) continue;
                 const int64_t base = 3 * (i0 + mx * (j + my * k));
-                if (p.induction) p.induction[base + q] = s_o[m][r * kTx + cl];
-                if (p.field) p.field[base + q] = s_o[3 + m][r * kTx + cl];
+                if (p.induction) __builtin_nontemporal_store(s_o[m][r * kTx + cl], p.induction + base + q);
+                if (p.field) __builtin_nontemporal_store(s_o[3 + m][r * kTx + cl], p.field + base + q);
             }
         if (p.stress)
             for (int e = threadIdx.x; e < kTy * 6 * kTx; e += kPostBlock) {
                 const int r = e / (6 * kTx), q = e % (6 * kTx), cl = q / 6, m = q % 6;
                 const int64_t j = j0 + r;
                 if (j >= my || cl >= nv) continue;
-                p.stress[6 * (i0 + mx * (j + my * k)) + q] = s_t[m][(r + 1) * kHx + cl + 1];
+                __builtin_nontemporal_store(s_t[m][(r + 1) * kHx + cl + 1], p.stress + 6 * (i0 + mx * (j + my * k)) + q);
             }
         if (!p.force) return;
     } else {
@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(kPostBlock) k_ms3d_post(PostArgs p)
             const int r = e / (3 * kTx), q = e % (3 * kTx), cl = q / 3, m = q % 3;
             const int64_t j = j0 + r;
             if (j >= my || cl >= nv) continue;
-            p.force[3 * (i0 + mx * (j + my * k)) + q] = s_o[m][r * kTx + cl];
+            __builtin_nontemporal_store(s_o[m][r * kTx + cl], p.force + 3 * (i0 + mx * (j + my * k)) + q);
         }
     }
 }
