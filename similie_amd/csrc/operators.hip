@@ -17,6 +17,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "apply_epilogue.hpp"
 #include "operators.hpp"
@@ -135,7 +136,8 @@ __global__ __launch_bounds__(kBlock) void k_ms3d_apply(MsArgs<T> a, const T* __r
 constexpr int kMsTY = 8;
 constexpr int kMsChunk = 32;
 
-template <typename T, bool PU, int RESP = RESP_LINEAR>
+// NT: the product leaves through non-temporal stores
+template <typename T, bool PU, int RESP = RESP_LINEAR, bool NT = false>
 __global__ __launch_bounds__(64 * kMsTY) void k_ms3d_march(MsArgs<T> a, const T* __restrict__ x, T* __restrict__ out,
                                                            double* __restrict__ partials, PUpdate<T> pu, int ntx,
                                                            int nty)
@@ -246,7 +248,10 @@ __global__ __launch_bounds__(64 * kMsTY) void k_ms3d_march(MsArgs<T> a, const T*
             const int64_t idx = 3 * (lin + plane * k);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                out[idx + c] = o[c];
+                if constexpr (NT)
+                    __builtin_nontemporal_store(o[c], out + idx + c);
+                else
+                    out[idx + c] = o[c];
                 if (PU) pu.p_new[idx + c] = P0[c];
                 dot += (double)P0[c] * (double)o[c];
             }
@@ -497,8 +502,16 @@ int ms_apply(const Ms3dData& d, const T* in, T* out, DotEpilogue dot, const PUpd
             src = pu->p_new;
             skip.done = pu->done;
         }
-        hipLaunchKernelGGL((k_ms3d_march<T, false, RESP>), dim3((unsigned)nbm), dim3(64 * kMsTY), 0, s, a, src,
-                           out, dot.partials, skip, ntx, nty);
+        static const bool nt = [] {
+            const char* e = std::getenv("SIL_MS3D_NT");
+            return !(e && e[0] == '0');
+        }();
+        if (nt)
+            hipLaunchKernelGGL((k_ms3d_march<T, false, RESP, true>), dim3((unsigned)nbm), dim3(64 * kMsTY), 0, s, a,
+                               src, out, dot.partials, skip, ntx, nty);
+        else
+            hipLaunchKernelGGL((k_ms3d_march<T, false, RESP>), dim3((unsigned)nbm), dim3(64 * kMsTY), 0, s, a, src,
+                               out, dot.partials, skip, ntx, nty);
     } else if (pu) {
         // the direction is formed in its own pass: formed inside the
         // direct kernel, every one of the ~40 neighbour reads of a node would
