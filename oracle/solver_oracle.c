@@ -1228,34 +1228,37 @@ void orc_ms2d_apply(long nx, long ny, const double* mu, const double* ferro, con
     ms2_operator(&M, state, in, out);
 }
 
-/* the diagonal of the linear operator (state = NULL) or of J(state), probed
- * with unit vectors in 3×3 colour classes */
+/* the diagonal of ms2_operator(M, state, ·), probed with unit vectors in
+ * 3×3 colour classes (a row reaches columns at most one node away) */
+static void ms2_probe_diagonal(const ms2_model* M, const double* state, double* diag)
+{
+    long n = M->nx * M->ny;
+    double* e = (double*)calloc(n, sizeof(double));
+    double* o = (double*)malloc(sizeof(double) * n);
+    for (int ci = 0; ci < 3; ++ci)
+        for (int cj = 0; cj < 3; ++cj) {
+            for (long r = 0; r < n; ++r) e[r] = 0.0;
+            for (long j = cj; j < M->ny; j += 3)
+                for (long i = ci; i < M->nx; i += 3) e[i + M->nx * j] = 1.0;
+            ms2_operator(M, state, e, o);
+            for (long j = cj; j < M->ny; j += 3)
+                for (long i = ci; i < M->nx; i += 3) diag[i + M->nx * j] = o[i + M->nx * j];
+        }
+    free(e);
+    free(o);
+}
+
+/* the diagonal of the linear operator (ferro = NULL) or of J(state) (a zero
+ * state when state = NULL) */
 void orc_ms2d_diagonal(long nx, long ny, const double* mu, const double* ferro, const double* bh_b,
                        const double* bh_h, int nbh, int criterion, const double* state, double* diag)
 {
     orc_bh c;
     if (ferro) bh_init(&c, bh_b, bh_h, nbh);
     ms2_model M = {nx, ny, mu, ferro, ferro ? &c : NULL, criterion};
-    long n = nx * ny;
-    double* e = (double*)calloc(n, sizeof(double));
-    double* o = (double*)malloc(sizeof(double) * n);
-    const double* st = state;
     double* zero = NULL;
-    if (ferro && !st) {
-        zero = (double*)calloc(n, sizeof(double));
-        st = zero;
-    }
-    for (int ci = 0; ci < 3; ++ci)
-        for (int cj = 0; cj < 3; ++cj) {
-            for (long r = 0; r < n; ++r) e[r] = 0.0;
-            for (long j = cj; j < ny; j += 3)
-                for (long i = ci; i < nx; i += 3) e[i + nx * j] = 1.0;
-            ms2_operator(&M, ferro ? st : NULL, e, o);
-            for (long j = cj; j < ny; j += 3)
-                for (long i = ci; i < nx; i += 3) diag[i + nx * j] = o[i + nx * j];
-        }
-    free(e);
-    free(o);
+    if (ferro && !state) state = zero = (double*)calloc(nx * ny, sizeof(double));
+    ms2_probe_diagonal(&M, ferro ? state : NULL, diag);
     free(zero);
 }
 
@@ -1310,21 +1313,7 @@ static void ms2_J(const nl_problem* P, const double* state, const double* in, do
 
 static void ms2_D(const nl_problem* P, const double* state, double* diag)
 {
-    const ms2_model* M = (const ms2_model*)P->data;
-    long n = M->nx * M->ny;
-    double* e = (double*)calloc(n, sizeof(double));
-    double* o = (double*)malloc(sizeof(double) * n);
-    for (int ci = 0; ci < 3; ++ci)
-        for (int cj = 0; cj < 3; ++cj) {
-            for (long r = 0; r < n; ++r) e[r] = 0.0;
-            for (long j = cj; j < M->ny; j += 3)
-                for (long i = ci; i < M->nx; i += 3) e[i + M->nx * j] = 1.0;
-            ms2_operator(M, state, e, o);
-            for (long j = cj; j < M->ny; j += 3)
-                for (long i = ci; i < M->nx; i += 3) diag[i + M->nx * j] = o[i + M->nx * j];
-        }
-    free(e);
-    free(o);
+    ms2_probe_diagonal((const ms2_model*)P->data, state, diag);
 }
 
 /* Newton on the 2D nonlinear operator, as orc_nl_newton */
