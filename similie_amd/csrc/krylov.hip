@@ -18,6 +18,7 @@
 // FP64; the residual is periodically replaced by the FP64 residual b − A x
 // (van der Vorst & Ye reliable updating), and convergence is declared only
 // on that FP64 residual.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <new>
@@ -49,6 +50,8 @@ struct Workspace {
     CsrData isai_g, isai_gt;
     double *isai_t = nullptr, *isai_z = nullptr;
     bool isai_ready = false;
+    // residual and correction of the FP64 continuation of a stalled mixed solve
+    double *fb_r = nullptr, *fb_d = nullptr;
     hipGraphExec_t graph[2][3] = {}; // [precision][preconditioner: identity, Jacobi, SpdIsai]
     int graph_chunk[2][3] = {};
 };
@@ -287,6 +290,20 @@ __global__ void k_after_replace(CgScalars* sc, const double* part_rr, const doub
     }
 }
 
+// out = b − ax
+__global__ void k_sub(const double* __restrict__ b, const double* __restrict__ ax, double* __restrict__ out, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = b[i] - ax[i];
+}
+
+// x += d
+__global__ void k_add(double* __restrict__ x, const double* __restrict__ d, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] += d[i];
+}
+
 __global__ void k_inv(const double* __restrict__ d, double* __restrict__ inv, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -519,7 +536,7 @@ void destroy_workspace(Workspace* w)
 {
     if (!w) return;
     for (double* p : {w->x, w->b, w->r, w->q, w->p[0], w->p[1], w->inv_diag, w->tmp, w->partials, w->isai_t,
-                      w->isai_z})
+                      w->isai_z, w->fb_r, w->fb_d})
         if (p) (void)hipFree(p);
     csr_free(w->isai_g);
     csr_free(w->isai_gt);
@@ -607,6 +624,12 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
     dg.initial_residual_l2 = std::sqrt(w.host_sc->bb);
     int since_replace = 0;
     double rr_ref = w.host_sc->bb;
+    // a mixed solve whose FP64-verified residual falls by less than 5 % over
+    // five replacement intervals in a row has reached what the FP32 products
+    // can resolve; it continues in FP64 from its iterate
+    double rr_prev = w.host_sc->bb;
+    int stalls = 0;
+    bool stalled = false;
     while (!w.host_sc->done) {
         const int iter_before = w.host_sc->iter;
         SIL_CHECK_HIP(hipGraphLaunch(g, s));
@@ -631,16 +654,54 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
             if ((rc = read_scalars(w, s))) return rc;
             since_replace = 0;
             rr_ref = w.host_sc->rr_true;
+            if (!w.host_sc->converged) {
+                stalls = rr_ref > 0.9025 * rr_prev ? stalls + 1 : 0;
+                rr_prev = rr_ref;
+                if (stalls >= 5) {
+                    stalled = true;
+                    break;
+                }
+            }
         } else if (h.converged && hit_max) {
             break;
         }
+    }
+    int extra_iterations = 0;
+    if (stalled) {
+        // FP64 CG on the correction A d = b − A x, then x + d
+        const int mixed_iterations = w.host_sc->iter;
+        const double bb = w.host_sc->bb, rr = w.host_sc->rr_true;
+        if (!w.fb_r && ((rc = alloc(&w.fb_r, n)) || (rc = alloc(&w.fb_d, n)))) return rc;
+        SIL_CHECK_HIP(hipMemcpyAsync(solution, w.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        if ((rc = apply64(op, solution, w.tmp, DotEpilogue{nullptr}, nullptr, nullptr, s))) return rc;
+        hipLaunchKernelGGL(k_sub, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, rhs, w.tmp, w.fb_r, n);
+        SIL_LAUNCH_CHECK();
+        sil_solver_settings fp64 = st;
+        fp64.precision = SIL_PRECISION_FP64;
+        fp64.max_iterations = std::max(1, st.max_iterations - mixed_iterations);
+        // half the target: the FP64 recurrence residual drifts a little from the
+        // true one, and the solve is judged on the true residual
+        fp64.relative_tolerance = rr > 0.0 ? std::min(1.0, 0.5 * st.relative_tolerance * std::sqrt(bb / rr)) : 1.0;
+        sil_solver_diagnostics dd{};
+        if ((rc = cg_solve(op, w.fb_r, w.fb_d, fp64, dd, s))) return rc;
+        hipLaunchKernelGGL(k_add, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, solution, w.fb_d, n);
+        SIL_LAUNCH_CHECK();
+        // the solve's own vectors hold the correction problem now: restore
+        // b and x for the final residual
+        SIL_CHECK_HIP(hipMemcpyAsync(w.b, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        SIL_CHECK_HIP(hipMemcpyAsync(w.x, solution, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        if ((rc = read_scalars(w, s))) return rc;
+        w.host_sc->bb = bb;
+        w.host_sc->iter = mixed_iterations;
+        SIL_CHECK_HIP(hipMemcpyAsync(w.sc, w.host_sc, sizeof(CgScalars), hipMemcpyHostToDevice, s));
+        extra_iterations = dd.iterations;
     }
     if ((rc = true_residual(op, s))) return rc;
     if ((rc = read_scalars(w, s))) return rc;
     SIL_CHECK_HIP(hipMemcpyAsync(solution, w.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     SIL_CHECK_HIP(hipStreamSynchronize(s));
     const CgScalars& h = *w.host_sc;
-    dg.iterations = h.iter;
+    dg.iterations = h.iter + extra_iterations;
     const double bnorm = std::sqrt(h.bb);
     dg.final_residual_l2 = std::sqrt(mixed ? h.rr_true : h.rr);
     dg.final_relative_residual = bnorm == 0.0 ? 0.0 : dg.final_residual_l2 / bnorm;

@@ -185,3 +185,24 @@ def test_post_process_matches_oracle(shape, nonlinear):
                                 CURVE["h"] if nonlinear else None)
     for got, r in zip(out, ref):
         assert rel(got, r) <= 1e-12
+
+
+def test_mixed_precision_continues_in_fp64_when_it_stalls():
+    """On the 2048²-cell μ_r-1000 cross-section the FP32 products cannot
+    resolve the residual below ~1e-8; the mixed solve notices the stall and
+    finishes in FP64 to the requested FP64-verified tolerance"""
+    import similie_amd.solvers as S
+    from similie_amd.workloads import magnetostatics_2d_inductor
+
+    print("ms2d mixed-precision stall -> FP64 continuation (2048^2 cells)", flush=True)
+    coords, cmu, _, jz = magnetostatics_2d_inductor()
+    shape = tuple(len(c) for c in coords)
+    mu = S.MagnetostaticsOperator2D.nodal_mu(shape, dev(cmu), MU0)
+    op = S.MagnetostaticsOperator2D(*coords, mu)
+    b = op.rhs(dev(jz))
+    st = S.StrongFormulationSolverSettings(max_iterations=30000, relative_tolerance=1e-10, precision="mixed")
+    x, d = S.minimize_strong_formulation_residual(op, b, settings=st)
+    assert d["converged"], d
+    assert d["true_relative_residual"] <= 1e-10 * (1 + 1e-6)
+    r = b - op.apply(x)
+    assert (torch.linalg.norm(r) / torch.linalg.norm(b)).item() <= 1.0001e-10
