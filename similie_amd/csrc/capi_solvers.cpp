@@ -72,6 +72,8 @@ void free_operator(sil_operator op)
     f(op->nl.st_a32);
     f(op->nl.st_c32);
     for (int q = 0; q < 3; ++q) f(op->nl.st_b32[q]);
+    f(op->nl.st_pack);
+    f(op->nl.st_pack32);
     if (op->jac) {
         // shares ms / nl with op: only its own solver workspace is freed
         destroy_workspace(op->jac->ws);
@@ -332,6 +334,11 @@ int sil_magnetostatics2d_nonlinear_create(sil_operator* out, int64_t nx, int64_t
     if (rc) return rc;
     op->kind = OP_MS2D_NONLINEAR;
     rc = attach_material(op, nx * ny, ferromagnetic_nodal, bh_b, bh_h, nsamples);
+    if (rc == SIL_OK && (hipMalloc(&op->nl.st_pack, sizeof(double) * 4 * nx * ny) != hipSuccess ||
+                         hipMalloc(&op->nl.st_pack32, sizeof(float) * 4 * nx * ny) != hipSuccess)) {
+        set_error("magnetostatics2d_nonlinear_create: device allocation failed");
+        rc = SIL_ERR_HIP;
+    }
     if (rc != SIL_OK) {
         free_operator(op);
         return rc;

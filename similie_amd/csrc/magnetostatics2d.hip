@@ -36,14 +36,18 @@ constexpr int kBlock = kApplyBlock;
 
 enum Resp2 { R2_LINEAR = 0, R2_SCALED = 1, R2_JACOBIAN = 2 };
 
+// the material state of a node, one aligned record: a, c, B0_x, B0_y
+template <typename T>
+struct alignas(4 * sizeof(T)) State2 {
+    T a, c, bx, by;
+};
+
 template <typename T>
 struct Ms2Args {
     int64_t nx, ny;
     T terms;
     const T* nu;
-    const T* st_a;
-    const T* st_c;
-    const T* st_b[2];
+    const State2<T>* st;
 };
 
 inline unsigned nblk(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
@@ -56,15 +60,14 @@ __device__ __forceinline__ void response(const Ms2Args<T>& a, int64_t s, T bx, T
         hx = nu * bx;
         hy = nu * by;
     } else if (RESP == R2_SCALED) {
-        const T k = a.st_a[s];
+        const T k = a.st[s].a;
         hx = k * bx;
         hy = k * by;
     } else {
-        const T k = a.st_a[s], c = a.st_c[s];
-        const T b0x = a.st_b[0][s], b0y = a.st_b[1][s];
-        const T proj = c * (b0x * bx + b0y * by);
-        hx = k * bx + proj * b0x;
-        hy = k * by + proj * b0y;
+        const State2<T> q = a.st[s];
+        const T proj = q.c * (q.bx * bx + q.by * by);
+        hx = q.a * bx + proj * q.bx;
+        hy = q.a * by + proj * q.by;
     }
 }
 
@@ -130,23 +133,36 @@ __global__ void k_ms2d_diag(Ms2Args<double> a, double* __restrict__ diag)
     double acc = 0.0;
     if (j > 1) {
         const int64_t s = t - nx;
-        acc += JAC ? a.st_a[s] + a.st_c[s] * a.st_b[0][s] * a.st_b[0][s] : a.nu[s];
+        if (JAC) {
+            const State2<double> q = a.st[s];
+            acc += q.a + q.c * q.bx * q.bx;
+        } else {
+            acc += a.nu[s];
+        }
     }
-    {
-        const double g = JAC ? a.st_b[1][t] - a.st_b[0][t] : 0.0;
-        acc += JAC ? 2.0 * a.st_a[t] + a.st_c[t] * g * g : 2.0 * a.nu[t];
+    if (JAC) {
+        const State2<double> q = a.st[t];
+        const double g = q.by - q.bx;
+        acc += 2.0 * q.a + q.c * g * g;
+    } else {
+        acc += 2.0 * a.nu[t];
     }
     if (i > 1) {
         const int64_t s = t - 1;
-        acc += JAC ? a.st_a[s] + a.st_c[s] * a.st_b[1][s] * a.st_b[1][s] : a.nu[s];
+        if (JAC) {
+            const State2<double> q = a.st[s];
+            acc += q.a + q.c * q.by * q.by;
+        } else {
+            acc += a.nu[s];
+        }
     }
     diag[t] = a.terms * acc;
 }
 
 // the state of x at interior nodes: B, a, c (FP64 and FP32 copies)
 __global__ void k_ms2d_state(int64_t nx, int64_t ny, const double* __restrict__ nu, const uint8_t* __restrict__ ferro,
-                             const BhCurve* __restrict__ curve, const double* __restrict__ x, double* sa, double* sc,
-                             double* bx_out, double* by_out, float* sa32, float* sc32, float* bx32, float* by32)
+                             const BhCurve* __restrict__ curve, const double* __restrict__ x,
+                             State2<double>* __restrict__ st, State2<float>* __restrict__ st32)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= nx * ny) return;
@@ -165,14 +181,8 @@ __global__ void k_ms2d_state(int64_t nx, int64_t ny, const double* __restrict__ 
             a = nu[t];
         }
     }
-    sa[t] = a;
-    sc[t] = c;
-    bx_out[t] = bx;
-    by_out[t] = by;
-    sa32[t] = (float)a;
-    sc32[t] = (float)c;
-    bx32[t] = (float)bx;
-    by32[t] = (float)by;
+    st[t] = State2<double>{a, c, bx, by};
+    st32[t] = State2<float>{(float)a, (float)c, (float)bx, (float)by};
 }
 
 // mean of the adjacent cells' μ (magnetostatics_onelab.hpp:4467-4500)
@@ -223,18 +233,10 @@ Ms2Args<T> args(const Ms2dData& d, const NlData* nl, bool fp32)
     a.terms = (T)d.terms;
     if (fp32) {
         a.nu = reinterpret_cast<const T*>(d.nu32);
-        if (nl) {
-            a.st_a = reinterpret_cast<const T*>(nl->st_a32);
-            a.st_c = reinterpret_cast<const T*>(nl->st_c32);
-            for (int q = 0; q < 2; ++q) a.st_b[q] = reinterpret_cast<const T*>(nl->st_b32[q]);
-        }
+        if (nl) a.st = reinterpret_cast<const State2<T>*>(nl->st_pack32);
     } else {
         a.nu = reinterpret_cast<const T*>(d.nu);
-        if (nl) {
-            a.st_a = reinterpret_cast<const T*>(nl->st_a);
-            a.st_c = reinterpret_cast<const T*>(nl->st_c);
-            for (int q = 0; q < 2; ++q) a.st_b[q] = reinterpret_cast<const T*>(nl->st_b[q]);
-        }
+        if (nl) a.st = reinterpret_cast<const State2<T>*>(nl->st_pack);
     }
     return a;
 }
@@ -308,9 +310,10 @@ int ms2d_jacobian_diagonal(const Ms2dData& d, const NlData& nl, double* diag, hi
 
 int ms2d_state(const Ms2dData& d, const NlData& nl, const double* x, hipStream_t s)
 {
+    SIL_REQUIRE(nl.st_pack && nl.st_pack32, SIL_ERR_INVALID, "ms2d_state: no packed state arrays");
     hipLaunchKernelGGL(k_ms2d_state, dim3(nblk(d.nx * d.ny)), dim3(kBlock), 0, s, d.nx, d.ny, d.nu, nl.ferro,
-                       nl.curve, x, nl.st_a, nl.st_c, nl.st_b[0], nl.st_b[1], nl.st_a32, nl.st_c32, nl.st_b32[0],
-                       nl.st_b32[1]);
+                       nl.curve, x, reinterpret_cast<State2<double>*>(nl.st_pack),
+                       reinterpret_cast<State2<float>*>(nl.st_pack32));
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }

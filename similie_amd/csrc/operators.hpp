@@ -50,6 +50,9 @@ struct NlData {
     float* st_a32 = nullptr;
     float* st_c32 = nullptr;
     float* st_b32[3] = {nullptr, nullptr, nullptr};
+    // 2D operators: the state packed per node as (a, c, B_x, B_y), FP64 and FP32
+    double* st_pack = nullptr;
+    float* st_pack32 = nullptr;
 };
 
 // 2D magnetostatics on A_z (one unknown per node, i + nx·j): the moment
@@ -116,7 +119,7 @@ int ms3d_post_process(const Ms3dData& d, const BhCurve* curve, const double* pot
                       const double* cell_nonlinear, double* induction, double* field, double* stress,
                       double* force, hipStream_t s);
 
-// 2D magnetostatics (magnetostatics2d.hip); nl uses st_b[0..1] only
+// 2D magnetostatics (magnetostatics2d.hip); nl uses st_pack / st_pack32
 int ms2d_apply64(const Ms2dData& d, const double* in, double* out, DotEpilogue dot, const PUpdate<double>* pu,
                  int* nblocks_out, hipStream_t s);
 int ms2d_apply32(const Ms2dData& d, const float* in, float* out, DotEpilogue dot, const PUpdate<float>* pu,
