@@ -230,6 +230,11 @@ int sil_magnetostatics3d_nonlinear_create(sil_operator* out, int64_t nx, int64_t
     if (rc) return rc;
     op->kind = OP_MS3D_NONLINEAR;
     rc = attach_material(op, nx * ny * nz, ferromagnetic_nodal, bh_b, bh_h, nsamples);
+    if (rc == SIL_OK && (hipMalloc(&op->nl.st_pack, sizeof(double) * 4 * nx * ny * nz) != hipSuccess ||
+                         hipMalloc(&op->nl.st_pack32, sizeof(float) * 4 * nx * ny * nz) != hipSuccess)) {
+        set_error("magnetostatics3d_nonlinear_create: device allocation failed");
+        rc = SIL_ERR_HIP;
+    }
     if (rc != SIL_OK) {
         free_operator(op);
         return rc;

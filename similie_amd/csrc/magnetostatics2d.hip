@@ -36,11 +36,8 @@ constexpr int kBlock = kApplyBlock;
 
 enum Resp2 { R2_LINEAR = 0, R2_SCALED = 1, R2_JACOBIAN = 2 };
 
-// the material state of a node, one aligned record: a, c, B0_x, B0_y
 template <typename T>
-struct alignas(4 * sizeof(T)) State2 {
-    T a, c, bx, by;
-};
+using State2 = NodeState<T>;
 
 template <typename T>
 struct Ms2Args {

@@ -42,6 +42,7 @@ struct MsArgs {
     const T* st_a;
     const T* st_c;
     const T* st_b[3];
+    const NodeState<T>* st4; // (a, c, B0_x, B0_y) per node
 };
 
 // material response of the curl-curl march: ν·B (linear), a·B (the
@@ -207,8 +208,8 @@ __global__ __launch_bounds__(64 * kMsTY) void k_ms3d_march(MsArgs<T> a, const T*
                 H[1] = (ly * ilx * ilz) * (by * sa);
                 H[2] = (lz * ilx * ily) * (bz * sa);
             } else {
-                const T sa = a.st_a[node], sc = a.st_c[node];
-                const T b0x = a.st_b[0][node], b0y = a.st_b[1][node], b0z = a.st_b[2][node];
+                const NodeState<T> q = a.st4[node];
+                const T sa = q.a, sc = q.c, b0x = q.bx, b0y = q.by, b0z = a.st_b[2][node];
                 const T cd = sc * (b0x * bx + b0y * by + b0z * bz);
                 H[0] = (lx * ily * ilz) * (sa * bx + cd * b0x);
                 H[1] = (ly * ilx * ilz) * (sa * by + cd * b0y);
@@ -474,10 +475,12 @@ int ms_apply(const Ms3dData& d, const T* in, T* out, DotEpilogue dot, const PUpd
             a.st_a = nl->st_a;
             a.st_c = nl->st_c;
             for (int q = 0; q < 3; ++q) a.st_b[q] = nl->st_b[q];
+            a.st4 = reinterpret_cast<const NodeState<T>*>(nl->st_pack);
         } else {
             a.st_a = nl->st_a32;
             a.st_c = nl->st_c32;
             for (int q = 0; q < 3; ++q) a.st_b[q] = nl->st_b32[q];
+            a.st4 = reinterpret_cast<const NodeState<T>*>(nl->st_pack32);
         }
     }
     SIL_REQUIRE(RESP == RESP_LINEAR || (d.nx >= 3 && d.ny >= 3 && d.nz >= 3), SIL_ERR_UNSUPPORTED,

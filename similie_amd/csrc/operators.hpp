@@ -39,6 +39,14 @@ struct BhCurve {
     double b[kBhMax], h[kBhMax], q[kBhMax], dh_db[kBhMax];
 };
 
+// a node's material state as one aligned record: a, c and the first two
+// components of B0 (the Jacobian's response reads it in one or two vector
+// loads)
+template <typename T>
+struct alignas(4 * sizeof(T)) NodeState {
+    T a, c, bx, by;
+};
+
 struct NlData {
     const BhCurve* curve = nullptr; // device
     uint8_t* ferro = nullptr;       // per node: ferromagnetic fraction > 0.5
@@ -50,7 +58,8 @@ struct NlData {
     float* st_a32 = nullptr;
     float* st_c32 = nullptr;
     float* st_b32[3] = {nullptr, nullptr, nullptr};
-    // 2D operators: the state packed per node as (a, c, B_x, B_y), FP64 and FP32
+    // the state packed per node as NodeState (a, c, B_x, B_y), FP64 and FP32;
+    // the 3D Jacobian takes B_z from st_b[2]
     double* st_pack = nullptr;
     float* st_pack32 = nullptr;
 };
