@@ -147,7 +147,7 @@ def solve_poisson_bumps(torch):
     _, d = S.minimize_strong_formulation_residual(op, rhs, settings=st)
     return {"workload": "Dirichlet Poisson 256^2, h = 1/256, three Gaussian bumps", "method": "FP64 Jacobi CG",
             "seconds": round(d["duration"], 5), "iterations": d["iterations"],
-            "true_relative_residual": d["true_relative_residual"]}
+            "verified": bool(d["verified"]), "true_relative_residual": d["true_relative_residual"]}
 
 
 def solve_magnetostatics(torch, max_iterations=20000):
@@ -168,7 +168,7 @@ def solve_magnetostatics(torch, max_iterations=20000):
         print(f"bench:   magnetostatics {prec} ...", file=sys.stderr, flush=True)
         _, d = S.minimize_strong_formulation_residual(op, rhs, settings=st)
         out[prec] = {"seconds": round(d["duration"], 4), "iterations": d["iterations"],
-                     "converged": bool(d["converged"]), "true_relative_residual": d["true_relative_residual"]}
+                     "converged": bool(d["converged"]), "verified": bool(d["verified"]), "true_relative_residual": d["true_relative_residual"]}
     if out["mixed"]["seconds"] > 0:
         out["mixed_speedup"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
     return out
@@ -237,7 +237,7 @@ def magnetostatics_assembled(torch, max_iterations=20000):
                                                preconditioner=pre)
         _, d = S.minimize_strong_formulation_residual(oper, rhs, settings=st)
         out[name] = {"seconds": round(d["duration"], 4), "iterations": d["iterations"],
-                     "converged": bool(d["converged"]), "true_relative_residual": d["true_relative_residual"]}
+                     "converged": bool(d["converged"]), "verified": bool(d["verified"]), "true_relative_residual": d["true_relative_residual"]}
     return out
 
 
@@ -263,7 +263,7 @@ def magnetostatics_nonlinear(torch, max_iterations=20000):
                                                precision=prec)
         _, d = S.minimize_strong_formulation_residual(op, rhs, settings=st)
         out[prec] = {"seconds": round(d["duration"], 4), "newton_steps": d["outer_iterations"],
-                     "cg_iterations": d["iterations"], "converged": bool(d["converged"]),
+                     "cg_iterations": d["iterations"], "converged": bool(d["converged"]), "verified": bool(d["verified"]),
                      "final_relative_residual": d["final_relative_residual"]}
     if out["mixed"]["seconds"] > 0:
         out["mixed_speedup"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
@@ -305,7 +305,7 @@ def magnetostatics_2d(torch, max_iterations=20000):
                                                precision=prec)
         _, d = S.minimize_strong_formulation_residual(op, rhs, settings=st)
         out["linear_" + prec] = {"seconds": round(d["duration"], 4), "iterations": d["iterations"],
-                                 "converged": bool(d["converged"]),
+                                 "converged": bool(d["converged"]), "verified": bool(d["verified"]),
                                  "true_relative_residual": d["true_relative_residual"]}
     frac = S.NonlinearMagnetostaticsOperator2D.nodal_fraction(shape, cfl)
     mu0 = S.MagnetostaticsOperator2D.nodal_mu(shape, torch.full_like(cmu, MU0), MU0)
@@ -316,7 +316,7 @@ def magnetostatics_2d(torch, max_iterations=20000):
                                                precision=prec)
         xs, d = S.minimize_strong_formulation_residual(nop, rhs, settings=st)
         out["nonlinear_" + prec] = {"seconds": round(d["duration"], 4), "newton_steps": d["outer_iterations"],
-                                    "cg_iterations": d["iterations"], "converged": bool(d["converged"]),
+                                    "cg_iterations": d["iterations"], "converged": bool(d["converged"]), "verified": bool(d["verified"]),
                                     "final_relative_residual": d["final_relative_residual"]}
     pp = nop.post_process(xs, torch.full_like(cmu, MU0), cfl)
     torch.cuda.synchronize()

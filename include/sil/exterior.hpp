@@ -41,6 +41,7 @@ struct FormView {
     std::array<int64_t, N> extent{};
     std::array<int64_t, N> stride{};
     int64_t comp_stride = 1;
+    int64_t comp_extent = 1; // components along the component axis: C(N, k) for a k-form
     std::vector<std::pair<int64_t, int64_t>> spectators; // (extent, stride)
 
     sil_view view() const
@@ -52,6 +53,7 @@ struct FormView {
             v.stride[a] = stride[a];
         }
         v.comp_stride = comp_stride;
+        v.comp_extent = comp_extent;
         v.nbatch = static_cast<int32_t>(spectators.size());
         for (size_t b = 0; b < spectators.size(); ++b) {
             v.batch_extent[b] = spectators[b].first;
@@ -74,12 +76,13 @@ FormView<N> index_last(double* data, std::array<int64_t, N> extent, int64_t ncom
         s *= extent[a];
     }
     f.comp_stride = 1;
+    f.comp_extent = ncomp;
     return f;
 }
 
 // component index first: [c][x][y]... (tests/exterior/derivative.cpp:19-58)
 template <int N>
-FormView<N> index_first(double* data, std::array<int64_t, N> extent)
+FormView<N> index_first(double* data, std::array<int64_t, N> extent, int64_t ncomp = 1)
 {
     FormView<N> f;
     f.data = data;
@@ -90,6 +93,7 @@ FormView<N> index_first(double* data, std::array<int64_t, N> extent)
         s *= extent[a];
     }
     f.comp_stride = s;
+    f.comp_extent = ncomp;
     return f;
 }
 
@@ -192,6 +196,7 @@ template <int N>
 class StagedLaplacian {
 public:
     StagedLaplacian(Grid<N> const& grid, FormView<N> out, FormView<N> in, int degree)
+        : in_(in), out_(out)
     {
         sil_view vi = in.view(), vo = out.view();
         check(sil_make_staged_laplacian(&handle_, grid.handle(), degree, &vi, &vo));
@@ -199,13 +204,24 @@ public:
     StagedLaplacian(const StagedLaplacian&) = delete;
     StagedLaplacian& operator=(const StagedLaplacian&) = delete;
     ~StagedLaplacian() { sil_staged_laplacian_destroy(handle_); }
+    // the handle keeps the layouts it was built for: views of another shape
+    // or other strides are refused
     FormView<N> run(void* stream, FormView<N> out, FormView<N> in)
     {
+        if (!same_layout(in, in_) || !same_layout(out, out_))
+            throw Error(SIL_ERR_INVALID, "staged laplacian called with layouts it was not built for");
         check(sil_staged_laplacian_run(handle_, out.data, in.data, stream));
         return out;
     }
 
 private:
+    static bool same_layout(FormView<N> const& a, FormView<N> const& b)
+    {
+        return a.extent == b.extent && a.stride == b.stride && a.comp_stride == b.comp_stride &&
+               a.comp_extent == b.comp_extent && a.spectators == b.spectators;
+    }
+
+    FormView<N> in_, out_;
     sil_staged_laplacian handle_ = nullptr;
 };
 

@@ -66,6 +66,8 @@ typedef struct sil_view {
     int64_t comp_stride;         /* element stride between components */
     int64_t batch_extent[SIL_MAX_DIM];
     int64_t batch_stride[SIL_MAX_DIM];
+    int64_t comp_extent; /* components the array holds along its component axis (1 when it has none);
+                            every entry point requires C(N, k) for a k-form */
 } sil_view;
 
 const char* sil_last_error(void);
@@ -323,6 +325,8 @@ typedef struct sil_solver_diagnostics {
     double true_relative_residual;  /* FP64 ‖b − A x‖2 / ‖b‖2 recomputed at exit */
     double duration;                /* seconds, host wall clock around the solve */
     int32_t outer_iterations;       /* Newton steps of a nonlinear operator (0 for a linear one) */
+    int32_t verified;               /* 1 when the FP64 residual recomputed at exit meets the tolerance
+                                       (‖b − A x‖2 ≤ tol·‖b‖2; Newton: the FP64 nonlinear residual) */
 } sil_solver_diagnostics;
 
 void sil_default_solver_settings(sil_solver_settings* settings);

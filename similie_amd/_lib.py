@@ -32,6 +32,7 @@ class View(ctypes.Structure):
         ("comp_stride", ctypes.c_int64),
         ("batch_extent", ctypes.c_int64 * SIL_MAX_DIM),
         ("batch_stride", ctypes.c_int64 * SIL_MAX_DIM),
+        ("comp_extent", ctypes.c_int64),
     ]
 
 
@@ -56,6 +57,7 @@ class SolverDiagnostics(ctypes.Structure):
         ("true_relative_residual", ctypes.c_double),
         ("duration", ctypes.c_double),
         ("outer_iterations", ctypes.c_int32),
+        ("verified", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -63,8 +63,13 @@ using namespace sil;
 
 namespace {
 
+// kout: degree of the output form as a function of the input degree k and
+// the dimension (the kernels index C(N, k) input and C(N, kout) output
+// components along the component axis)
+enum class OutDegree { Up, Same, Down, Dual };
+
 int check_form_args(int k, int n, const void* in, const sil_view* vin, const void* out, const sil_view* vout,
-                    const char* what)
+                    const char* what, OutDegree kout)
 {
     int rc = validate_view(vin, what);
     if (rc) return rc;
@@ -74,6 +79,15 @@ int check_form_args(int k, int n, const void* in, const sil_view* vin, const voi
     SIL_REQUIRE(n <= 0 || vin->ndim == n, SIL_ERR_INVALID, "%s: view dimension %d does not match the grid (%d)", what,
                 vin->ndim, n);
     SIL_REQUIRE(k >= 0 && k <= vin->ndim, SIL_ERR_INVALID, "%s: degree %d outside 0..%d", what, k, vin->ndim);
+    const int dim = vin->ndim;
+    const int ko = kout == OutDegree::Up ? k + 1 : kout == OutDegree::Down ? k - 1 : kout == OutDegree::Dual ? dim - k : k;
+    const int64_t cin = binom4(dim, k), cout = (ko >= 0 && ko <= dim) ? binom4(dim, ko) : 0;
+    SIL_REQUIRE(vin->comp_extent == cin, SIL_ERR_INVALID,
+                "%s: input holds %lld components, a %d-form in %d dimensions has %lld", what,
+                (long long)vin->comp_extent, k, dim, (long long)cin);
+    SIL_REQUIRE(cout == 0 || vout->comp_extent == cout, SIL_ERR_INVALID,
+                "%s: output holds %lld components, a %d-form in %d dimensions has %lld", what,
+                (long long)vout->comp_extent, ko, dim, (long long)cout);
     bool empty = view_nodes(*vout) * view_batches(*vout) == 0;
     SIL_REQUIRE(empty || (in != nullptr && out != nullptr), SIL_ERR_INVALID, "%s: null array", what);
     return SIL_OK;
@@ -180,7 +194,7 @@ int sil_grid_destroy(sil_grid grid)
 int sil_coboundary(int32_t k, const double* in, const sil_view* in_view, double* out, const sil_view* out_view,
                    void* stream)
 {
-    int rc = check_form_args(k, -1, in, in_view, out, out_view, "coboundary");
+    int rc = check_form_args(k, -1, in, in_view, out, out_view, "coboundary", OutDegree::Up);
     if (rc) return rc;
     SIL_REQUIRE(k < in_view->ndim, SIL_ERR_INVALID, "coboundary: degree %d has no coboundary in %d dimensions", k,
                 in_view->ndim);
@@ -199,7 +213,7 @@ int sil_coboundary(int32_t k, const double* in, const sil_view* in_view, double*
 int sil_transposed_coboundary(int32_t k, const double* in, const sil_view* in_view, double* out,
                               const sil_view* out_view, void* stream)
 {
-    int rc = check_form_args(k, -1, in, in_view, out, out_view, "transposed_coboundary");
+    int rc = check_form_args(k, -1, in, in_view, out, out_view, "transposed_coboundary", OutDegree::Up);
     if (rc) return rc;
     SIL_REQUIRE(k < in_view->ndim, SIL_ERR_INVALID, "transposed_coboundary: degree %d too large", k);
     SIL_REQUIRE(in_view->nbatch == out_view->nbatch, SIL_ERR_INVALID,
@@ -221,7 +235,7 @@ int sil_hodge_star(sil_grid grid, int32_t k, const double* in, const sil_view* i
                    const sil_view* out_view, void* stream)
 {
     SIL_REQUIRE(grid != nullptr, SIL_ERR_INVALID, "hodge_star: null grid");
-    int rc = check_form_args(k, grid->geom.ndim, in, in_view, out, out_view, "hodge_star");
+    int rc = check_form_args(k, grid->geom.ndim, in, in_view, out, out_view, "hodge_star", OutDegree::Dual);
     if (rc) return rc;
     if ((rc = check_grid_view(grid, in_view, "hodge_star"))) return rc;
     if ((rc = check_grid_view(grid, out_view, "hodge_star"))) return rc;
@@ -232,7 +246,7 @@ int sil_codifferential(sil_grid grid, int32_t k, const double* in, const sil_vie
                        const sil_view* out_view, void* stream)
 {
     SIL_REQUIRE(grid != nullptr, SIL_ERR_INVALID, "codifferential: null grid");
-    int rc = check_form_args(k, grid->geom.ndim, in, in_view, out, out_view, "codifferential");
+    int rc = check_form_args(k, grid->geom.ndim, in, in_view, out, out_view, "codifferential", OutDegree::Down);
     if (rc) return rc;
     SIL_REQUIRE(k >= 1, SIL_ERR_INVALID, "codifferential: degree must be at least 1");
     if ((rc = check_grid_view(grid, in_view, "codifferential"))) return rc;
@@ -291,7 +305,7 @@ int sil_make_staged_laplacian(sil_staged_laplacian* out, sil_grid grid, int32_t 
                               const sil_view* out_view)
 {
     SIL_REQUIRE(out != nullptr && grid != nullptr, SIL_ERR_INVALID, "laplacian: null argument");
-    int rc = check_form_args(k, grid->geom.ndim, (void*)1, in_view, (void*)1, out_view, "laplacian");
+    int rc = check_form_args(k, grid->geom.ndim, (void*)1, in_view, (void*)1, out_view, "laplacian", OutDegree::Same);
     if (rc) return rc;
     if ((rc = check_grid_view(grid, in_view, "laplacian"))) return rc;
     SIL_REQUIRE(out_view->nbatch == 0, SIL_ERR_UNSUPPORTED, "laplacian: spectator dimensions unsupported");

@@ -12,11 +12,11 @@
 //   block and tokenised in place with std::from_chars, axes are located by
 //   binary search (keeping the reference's rule: the first axis value within
 //   the tolerance), and node tags index a dense table when they are dense.
-// * .silpro (onelab_interface/onelab_interface.hpp:160-560): the same lexer
-//   (# and // comments, quoted strings with backslash escapes, words, braces,
-//   semicolons) and tree (sections of `Key value;` entries; the first of
-//   repeated keys or sections wins), the solver settings with the defaults
-//   of MinimizeStrongFormulationResidualProblem (:96-111).
+// * .silpro (the format of onelab_interface/onelab_interface.hpp:160-560):
+//   read by a scannerless recursive descent straight from the text (grammar
+//   beside SilproReader) into a store keyed by block path; the first of a
+//   repeated key or block wins; the solver settings with the defaults of
+//   MinimizeStrongFormulationResidualProblem (:96-111).
 // * Cell inputs (magnetostatics_onelab.hpp:4870-4897): μ, current density
 //   and the nonlinear-material flag of every cell from its physical tag.
 #include <algorithm>
@@ -28,6 +28,8 @@
 #include <limits>
 #include <map>
 #include <new>
+#include <set>
+#include <string_view>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -45,15 +47,14 @@ struct sil_structured_grid_s {
     std::vector<int32_t> cell_tags; // x fastest
 };
 
-namespace sil {
-struct SilproSection {
-    std::map<std::string, std::string> values;
-    std::map<std::string, SilproSection> sections;
-};
-}  // namespace sil
-
+// A parsed .silpro file: every assignment under the path of its key (the
+// names of the enclosing blocks, then the key), and the path of every block.
+// Paths compare element by element, so the assignments of one block are
+// contiguous and in key order.
 struct sil_silpro_s {
-    sil::SilproSection root;
+    using Path = std::vector<std::string>;
+    std::map<Path, std::string> assignments;
+    std::set<Path> blocks;
 };
 
 namespace sil {
@@ -627,191 +628,250 @@ int grid_from_text(const char* b, const char* e, sil_structured_grid* out)
 
 // ---------------------------------------------------------------- silpro ---
 
-enum TokKind { T_WORD, T_STRING, T_LBRACE, T_RBRACE, T_SEMI };
-struct Tok {
-    TokKind kind;
-    std::string text;
-};
-
-std::vector<Tok> lex_silpro(const std::string& s)
-{
-    std::vector<Tok> toks;
-    size_t i = 0;
-    auto space = [](char c) { return std::isspace(static_cast<unsigned char>(c)) != 0; };
-    while (i < s.size()) {
-        const char c = s[i];
-        if (space(c)) {
-            ++i;
-        } else if (c == '#' || (c == '/' && i + 1 < s.size() && s[i + 1] == '/')) {
-            while (i < s.size() && s[i] != '\n') ++i;
-        } else if (c == '{') {
-            toks.push_back({T_LBRACE, "{"});
-            ++i;
-        } else if (c == '}') {
-            toks.push_back({T_RBRACE, "}"});
-            ++i;
-        } else if (c == ';') {
-            toks.push_back({T_SEMI, ";"});
-            ++i;
-        } else if (c == '"') {
-            ++i;
-            std::string text;
-            while (i < s.size() && s[i] != '"') {
-                if (s[i] == '\\' && i + 1 < s.size()) ++i;
-                text.push_back(s[i]);
-                ++i;
-            }
-            if (i == s.size()) throw ParseError("unterminated string literal in .silpro file");
-            ++i;
-            toks.push_back({T_STRING, text});
-        } else {
-            std::string text;
-            while (i < s.size() && !space(s[i]) && s[i] != '{' && s[i] != '}' && s[i] != ';' &&
-                   !(s[i] == '/' && i + 1 < s.size() && s[i + 1] == '/'))
-                text.push_back(s[i++]);
-            if (!text.empty()) toks.push_back({T_WORD, text});
-        }
-    }
-    return toks;
-}
-
-class SilproParser {
+// Grammar of a .silpro file.  Trivia — whitespace, and comments running
+// from '#' or "//" to the end of the line — may stand between any two
+// symbols:
+//
+//   file   := block* END
+//   block  := atom '{' member* '}'
+//   member := block | atom atom ';'
+//   atom   := word | string
+//   word   := a first character that is not whitespace, '{', '}', ';', '"'
+//             or '#' and does not open "//", then every character up to
+//             whitespace, '{', '}', ';' or "//"
+//   string := '"' ( '\' any | any but '"' )* '"'   — the backslash keeps the
+//             character after it, whatever it is
+//
+// Only blocks stand at the top level.  Within a block, an atom followed by
+// '{' opens a nested block; otherwise the atom is a key and the next atom
+// its value.  The first assignment of a key in a block wins, and a block
+// whose path was already read is parsed and then dropped whole.
+class SilproReader {
   public:
-    explicit SilproParser(const std::vector<Tok>& t) : t_(t) {}
-    SilproSection document()
+    SilproReader(std::string_view text, sil_silpro_s& out) : s_(text), out_(out) {}
+
+    void file()
     {
-        SilproSection root;
-        while (i_ < t_.size()) {
-            std::string name = name_token();
-            root.sections.emplace(name, body());
+        sil_silpro_s::Path path;
+        while (true) {
+            skip_trivia();
+            if (i_ >= s_.size()) return;
+            block(path, true);
         }
-        return root;
     }
 
   private:
-    const Tok& peek() const
+    std::string_view s_;
+    size_t i_ = 0;
+    sil_silpro_s& out_;
+
+    [[noreturn]] void fail(const std::string& what) const
     {
-        if (i_ >= t_.size()) throw ParseError("unexpected end of .silpro file");
-        return t_[i_];
+        // 1-based line and column of the reading position
+        size_t line = 1, col = 1;
+        for (size_t k = 0; k < i_ && k < s_.size(); ++k) {
+            if (s_[k] == '\n') {
+                ++line;
+                col = 1;
+            } else {
+                ++col;
+            }
+        }
+        throw ParseError(".silpro line " + std::to_string(line) + ", column " + std::to_string(col) + ": " + what);
     }
-    std::string name_token()
+
+    bool comment_at(size_t k) const
     {
-        const Tok& t = peek();
-        if (t.kind != T_WORD && t.kind != T_STRING) throw ParseError("expected identifier in .silpro file");
+        return s_[k] == '#' || (s_[k] == '/' && k + 1 < s_.size() && s_[k + 1] == '/');
+    }
+
+    static bool blank(char c) { return std::isspace(static_cast<unsigned char>(c)) != 0; }
+
+    void skip_trivia()
+    {
+        while (i_ < s_.size()) {
+            if (blank(s_[i_])) {
+                ++i_;
+            } else if (comment_at(i_)) {
+                const size_t nl = s_.find('\n', i_);
+                i_ = nl == std::string_view::npos ? s_.size() : nl;
+            } else {
+                return;
+            }
+        }
+    }
+
+    // the next symbol after trivia, or 0 at the end of the text
+    char upcoming()
+    {
+        skip_trivia();
+        return i_ < s_.size() ? s_[i_] : '\0';
+    }
+
+    std::string atom(const std::string& role)
+    {
+        const char c = upcoming();
+        if (i_ >= s_.size()) fail("the file ends where " + role + " should be");
+        if (c == '"') {
+            std::string text;
+            size_t k = i_ + 1;
+            while (k < s_.size() && s_[k] != '"') {
+                if (s_[k] == '\\' && k + 1 < s_.size()) ++k;
+                text.push_back(s_[k++]);
+            }
+            if (k >= s_.size()) fail("a quoted string is never closed");
+            i_ = k + 1;
+            return text;
+        }
+        if (c == '{' || c == '}' || c == ';') fail(std::string("found '") + c + "' where " + role + " should be");
+        const size_t begin = i_;
+        while (i_ < s_.size() && !blank(s_[i_]) && s_[i_] != '{' && s_[i_] != '}' && s_[i_] != ';' &&
+               !(s_[i_] == '/' && i_ + 1 < s_.size() && s_[i_ + 1] == '/'))
+            ++i_;
+        return std::string(s_.substr(begin, i_ - begin));
+    }
+
+    // a block named by the next atom, inside the block at `parent`; `keep`
+    // is false inside a block that is dropped
+    void block(sil_silpro_s::Path& parent, bool keep)
+    {
+        parent.push_back(atom("a block name"));
+        if (upcoming() != '{') {
+            const std::string name = parent.back();
+            if (i_ >= s_.size()) fail("the file ends after '" + name + "', where '{' should open its block");
+            fail("'" + name + "' must be followed by '{' to open a block");
+        }
         ++i_;
-        return t.text;
-    }
-    void expect(TokKind k, const char* msg)
-    {
-        if (peek().kind != k) throw ParseError(msg);
-        ++i_;
-    }
-    SilproSection body()
-    {
-        expect(T_LBRACE, "expected '{' after section name in .silpro file");
-        SilproSection s;
-        while (peek().kind != T_RBRACE) {
-            std::string name = name_token();
-            if (peek().kind == T_LBRACE) {
-                s.sections.emplace(name, body());
+        keep = keep && out_.blocks.insert(parent).second;
+        while (true) {
+            const char c = upcoming();
+            if (i_ >= s_.size()) fail("the file ends inside block '" + parent.back() + "'");
+            if (c == '}') {
+                ++i_;
+                break;
+            }
+            const size_t mark = i_;
+            const std::string key = atom("a key or a block name");
+            if (upcoming() == '{') {
+                i_ = mark; // read again as a block name
+                block(parent, keep);
                 continue;
             }
-            std::string value = name_token();
-            expect(T_SEMI, "expected ';' after assignment in .silpro file");
-            s.values.emplace(name, value);
+            const std::string value = atom("the value of '" + key + "'");
+            if (upcoming() != ';') {
+                if (i_ >= s_.size()) fail("the file ends before the ';' that ends '" + key + "'");
+                fail("the value of '" + key + "' must be followed by ';'");
+            }
+            ++i_;
+            if (keep) {
+                parent.push_back(key);
+                out_.assignments.try_emplace(parent, value);
+                parent.pop_back();
+            }
         }
-        expect(T_RBRACE, "expected '}' in .silpro file");
-        return s;
+        parent.pop_back();
     }
-    const std::vector<Tok>& t_;
-    size_t i_ = 0;
 };
 
-// section at a dotted path ("Preprocess.MagneticMaterialTags"); null if absent
-const SilproSection* find_section(const SilproSection& root, const std::string& path)
+sil_silpro_s::Path dotted(const std::string& path)
 {
-    const SilproSection* s = &root;
+    sil_silpro_s::Path p;
     size_t b = 0;
-    while (b <= path.size() && !path.empty()) {
+    while (true) {
         const size_t e = path.find('.', b);
-        const std::string part = path.substr(b, e == std::string::npos ? std::string::npos : e - b);
-        auto it = s->sections.find(part);
-        if (it == s->sections.end()) return nullptr;
-        s = &it->second;
-        if (e == std::string::npos) break;
+        p.push_back(path.substr(b, e == std::string::npos ? std::string::npos : e - b));
+        if (e == std::string::npos) return p;
         b = e + 1;
     }
-    return s;
 }
 
-const std::string* find_value(const SilproSection& root, const std::string& path)
+const std::string* value_at(const sil_silpro_s& f, const sil_silpro_s::Path& key)
 {
-    const size_t dot = path.rfind('.');
-    const SilproSection* s = dot == std::string::npos ? &root : find_section(root, path.substr(0, dot));
-    if (!s) return nullptr;
-    auto it = s->values.find(dot == std::string::npos ? path : path.substr(dot + 1));
-    return it == s->values.end() ? nullptr : &it->second;
+    auto it = f.assignments.find(key);
+    return it == f.assignments.end() ? nullptr : &it->second;
 }
 
-std::string trim_lower(std::string v)
+// the values assigned directly in the block at `block`, in key order
+std::vector<const std::string*> block_values(const sil_silpro_s& f, const sil_silpro_s::Path& block)
 {
-    auto sp = [](unsigned char c) { return std::isspace(c) != 0; };
-    while (!v.empty() && sp(v.back())) v.pop_back();
-    size_t b = 0;
-    while (b < v.size() && sp(v[b])) ++b;
-    v = v.substr(b);
-    for (char& c : v) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
-    return v;
+    std::vector<const std::string*> vals;
+    for (auto it = f.assignments.lower_bound(block); it != f.assignments.end(); ++it) {
+        const auto& k = it->first;
+        if (k.size() < block.size() || !std::equal(block.begin(), block.end(), k.begin())) break;
+        if (k.size() == block.size() + 1) vals.push_back(&it->second);
+    }
+    return vals;
 }
 
-bool parse_bool(const std::string& v)
+// Scalar setting values.  Booleans take the words of the reference's
+// parse_bool in any case; numbers must fill the whole value.
+bool boolean_setting(const std::string& key, const std::string& v)
 {
-    const std::string n = trim_lower(v);
-    if (n == "1" || n == "true" || n == "yes" || n == "on") return true;
-    if (n == "0" || n == "false" || n == "no" || n == "off") return false;
-    throw ParseError("invalid boolean value '" + v + "'");
+    static const std::pair<const char*, bool> words[] = {{"1", true},  {"true", true},   {"yes", true},
+                                                         {"on", true}, {"0", false},     {"false", false},
+                                                         {"no", false}, {"off", false}};
+    std::string low;
+    for (char c : v)
+        if (!std::isspace(static_cast<unsigned char>(c)))
+            low.push_back(static_cast<char>(std::tolower(static_cast<unsigned char>(c))));
+    for (const auto& [w, b] : words)
+        if (low == w) return b;
+    throw ParseError(key + ": '" + v + "' is not a boolean (1/0, true/false, yes/no, on/off)");
 }
 
-template <typename T>
-T parse_number(const std::string& v)
+double real_setting(const std::string& key, const std::string& v)
 {
-    std::istringstream s(v);
-    T x{};
-    s >> x;
-    if (!s || !s.eof()) throw ParseError("invalid numeric value '" + v + "'");
+    const char* b = v.data();
+    const char* e = b + v.size();
+    if (b != e && *b == '+') ++b;
+    double x = 0.0;
+    const auto r = std::from_chars(b, e, x, std::chars_format::general);
+    if (b == e || r.ec != std::errc() || r.ptr != e || !std::isfinite(x))
+        throw ParseError(key + ": '" + v + "' is not a number");
     return x;
 }
 
-int parse_preconditioner(const std::string& n)
+uint32_t count_setting(const std::string& key, const std::string& v)
 {
-    if (n == "Identity" || n == "identity") return SIL_SILPRO_PRECOND_IDENTITY;
-    if (n == "Jacobi" || n == "jacobi") return SIL_SILPRO_PRECOND_JACOBI;
-    if (n == "SpdIsai" || n == "spd-isai" || n == "spd_isai") return SIL_SILPRO_PRECOND_SPD_ISAI;
-    if (n == "SymmetricGaussSeidel" || n == "symmetric-gauss-seidel" || n == "symmetric_gauss_seidel")
-        return SIL_SILPRO_PRECOND_SYMMETRIC_GAUSS_SEIDEL;
-    if (n == "Ssor" || n == "SSOR" || n == "ssor") return SIL_SILPRO_PRECOND_SSOR;
-    if (n == "ChebyshevJacobi" || n == "chebyshev-jacobi" || n == "chebyshev_jacobi")
-        return SIL_SILPRO_PRECOND_CHEBYSHEV_JACOBI;
-    if (n == "IrJacobi" || n == "ir-jacobi" || n == "ir_jacobi") return SIL_SILPRO_PRECOND_IR_JACOBI;
-    if (n == "GeneralIsai" || n == "general-isai" || n == "general_isai" || n == "isai")
-        return SIL_SILPRO_PRECOND_GENERAL_ISAI;
-    throw ParseError("unknown strong-formulation preconditioner: " + n);
+    const char* b = v.data();
+    const char* e = b + v.size();
+    if (b != e && *b == '+') ++b;
+    uint32_t x = 0;
+    const auto r = std::from_chars(b, e, x, 10);
+    if (b == e || r.ec != std::errc() || r.ptr != e)
+        throw ParseError(key + ": '" + v + "' is not a non-negative integer that fits 32 bits");
+    return x;
 }
 
-int parse_physics(const std::string& v)
+// accepted spellings of each strong-formulation preconditioner
+// (minimize_strong_formulation_residual.hpp:88-117)
+int preconditioner_setting(const std::string& v)
 {
-    if (v == "LinearMagnetostatics") return SIL_PHYSICS_LINEAR_MAGNETOSTATICS;
-    if (v == "NonLinearMagnetostatics") return SIL_PHYSICS_NONLINEAR_MAGNETOSTATICS;
-    if (v == "ScalarFieldWithPowerCoupling") return SIL_PHYSICS_SCALAR_FIELD_WITH_POWER_COUPLING;
-    throw ParseError("unsupported physics '" + v + "' in .silpro file");
+    static const std::pair<int, std::vector<const char*>> names[] = {
+            {SIL_SILPRO_PRECOND_IDENTITY, {"Identity", "identity"}},
+            {SIL_SILPRO_PRECOND_JACOBI, {"Jacobi", "jacobi"}},
+            {SIL_SILPRO_PRECOND_SPD_ISAI, {"SpdIsai", "spd-isai", "spd_isai"}},
+            {SIL_SILPRO_PRECOND_SYMMETRIC_GAUSS_SEIDEL,
+             {"SymmetricGaussSeidel", "symmetric-gauss-seidel", "symmetric_gauss_seidel"}},
+            {SIL_SILPRO_PRECOND_SSOR, {"Ssor", "SSOR", "ssor"}},
+            {SIL_SILPRO_PRECOND_CHEBYSHEV_JACOBI, {"ChebyshevJacobi", "chebyshev-jacobi", "chebyshev_jacobi"}},
+            {SIL_SILPRO_PRECOND_IR_JACOBI, {"IrJacobi", "ir-jacobi", "ir_jacobi"}},
+            {SIL_SILPRO_PRECOND_GENERAL_ISAI, {"GeneralIsai", "general-isai", "general_isai", "isai"}},
+    };
+    for (const auto& [id, spellings] : names)
+        for (const char* n : spellings)
+            if (v == n) return id;
+    throw ParseError("Preconditioner: '" + v + "' names no strong-formulation preconditioner");
 }
 
-int parse_criterion(const std::string& v)
+int enum_setting(const std::string& key, const std::string& v, std::initializer_list<const char*> names)
 {
-    if (v == "PotentialTemporalDerivative") return 0;
-    if (v == "MomentsTemporalDerivative") return 1;
-    if (v == "PotentialAndMomentsTemporalDerivative") return 2;
-    throw ParseError("unsupported solver criterion '" + v + "' in .silpro file");
+    int id = 0;
+    for (const char* n : names) {
+        if (v == n) return id;
+        ++id;
+    }
+    throw ParseError(key + ": '" + v + "' is not supported");
 }
 
 int copy_out(const std::string& v, char* buf, int64_t size)
@@ -881,7 +941,7 @@ int sil_silpro_parse(const char* text, int64_t length, sil_silpro* out)
     try {
         auto* h = new sil_silpro_s();
         try {
-            h->root = SilproParser(lex_silpro(std::string(text, static_cast<size_t>(length)))).document();
+            SilproReader(std::string_view(text, static_cast<size_t>(length)), *h).file();
         } catch (...) {
             delete h;
             throw;
@@ -906,7 +966,7 @@ int sil_silpro_read(const char* path, sil_silpro* out)
 int sil_silpro_value(sil_silpro h, const char* path, char* buf, int64_t size)
 {
     SIL_REQUIRE(h && path, SIL_ERR_INVALID, "silpro_value: null argument");
-    const std::string* v = find_value(h->root, path);
+    const std::string* v = value_at(*h, dotted(path));
     SIL_REQUIRE(v != nullptr, SIL_ERR_INVALID, "silpro: no value '%s'", path);
     return copy_out(*v, buf, size);
 }
@@ -914,45 +974,51 @@ int sil_silpro_value(sil_silpro h, const char* path, char* buf, int64_t size)
 int sil_silpro_section_value(sil_silpro h, const char* section, int32_t index, char* buf, int64_t size)
 {
     SIL_REQUIRE(h && section && index >= 0, SIL_ERR_INVALID, "silpro_section_value: bad argument");
-    const SilproSection* s = find_section(h->root, section);
-    SIL_REQUIRE(s != nullptr, SIL_ERR_INVALID, "silpro: no section '%s'", section);
-    SIL_REQUIRE(index < static_cast<int32_t>(s->values.size()), SIL_ERR_INVALID, "silpro: '%s' has %zu values",
-                section, s->values.size());
-    auto it = s->values.begin();
-    std::advance(it, index);
-    return copy_out(it->second, buf, size);
+    const sil_silpro_s::Path block = dotted(section);
+    SIL_REQUIRE(h->blocks.count(block) != 0, SIL_ERR_INVALID, "silpro: no section '%s'", section);
+    const auto vals = block_values(*h, block);
+    SIL_REQUIRE(index < static_cast<int32_t>(vals.size()), SIL_ERR_INVALID, "silpro: '%s' has %zu values", section,
+                vals.size());
+    return copy_out(*vals[static_cast<size_t>(index)], buf, size);
 }
 
 int sil_silpro_problem_settings(sil_silpro h, sil_silpro_settings* out)
 {
     SIL_REQUIRE(h && out, SIL_ERR_INVALID, "silpro_problem_settings: null argument");
     try {
-        const SilproSection* problem = find_section(h->root, "Problem");
-        const SilproSection* solver = find_section(h->root, "Solver");
-        if (!problem) throw ParseError("missing section 'Problem' in .silpro file");
-        if (!solver) throw ParseError("missing section 'Solver' in .silpro file");
-        auto get = [](const SilproSection* s, const char* key, const std::string& dflt) {
-            auto it = s->values.find(key);
-            return it == s->values.end() ? dflt : it->second;
+        for (const char* b : {"Problem", "Solver"})
+            if (!h->blocks.count({b})) throw ParseError(std::string("the file has no '") + b + "' block");
+        // the value of Block.key, or the default when the block does not set it
+        auto get = [&](const char* block, const char* key, const char* dflt) -> std::string {
+            const std::string* v = value_at(*h, {block, key});
+            return v ? *v : std::string(dflt);
         };
         sil_silpro_settings st{};
-        st.physics = parse_physics(get(problem, "Physics", "LinearMagnetostatics"));
-        if (get(problem, "Solver", "MinimizeStrongFormulationResidual") != "MinimizeStrongFormulationResidual")
-            throw ParseError("unsupported solver '" + get(problem, "Solver", "") + "' in .silpro file");
+        st.physics = enum_setting("Physics", get("Problem", "Physics", "LinearMagnetostatics"),
+                                  {"LinearMagnetostatics", "NonLinearMagnetostatics", "ScalarFieldWithPowerCoupling"});
+        static_assert(SIL_PHYSICS_LINEAR_MAGNETOSTATICS == 0 && SIL_PHYSICS_NONLINEAR_MAGNETOSTATICS == 1 &&
+                      SIL_PHYSICS_SCALAR_FIELD_WITH_POWER_COUPLING == 2);
+        enum_setting("Problem.Solver", get("Problem", "Solver", "MinimizeStrongFormulationResidual"),
+                     {"MinimizeStrongFormulationResidual"});
         // MinimizeStrongFormulationResidualProblem defaults (onelab_interface.hpp:96-111)
-        st.max_iterations = static_cast<int64_t>(parse_number<unsigned int>(get(solver, "MaxIterations", "10000")));
-        st.relative_tolerance = parse_number<double>(get(solver, "RelativeTolerance", "1e-10"));
-        st.jacobi_max_block_size = static_cast<int32_t>(parse_number<unsigned int>(get(solver, "JacobiMaxBlockSize", "1")));
-        st.use_matrix_free = parse_bool(get(solver, "UseMatrixFree", "1")) ? 1 : 0;
-        st.criterion = parse_criterion(get(solver, "Criterion", "MomentsTemporalDerivative"));
-        st.preconditioner = parse_preconditioner(get(solver, "Preconditioner", "Jacobi"));
-        st.vector_potential_gauge_penalty = parse_number<double>(get(solver, "VectorPotentialGaugePenalty", "20"));
-        st.sor_relaxation_factor = parse_number<double>(get(solver, "SorRelaxationFactor", "1.2"));
-        st.chebyshev_iterations = static_cast<int32_t>(parse_number<unsigned int>(get(solver, "ChebyshevIterations", "8")));
-        st.chebyshev_lower_bound = parse_number<double>(get(solver, "ChebyshevLowerBound", "0.05"));
-        st.chebyshev_upper_bound = parse_number<double>(get(solver, "ChebyshevUpperBound", "1.5"));
-        st.ir_iterations = static_cast<int32_t>(parse_number<unsigned int>(get(solver, "IrIterations", "8")));
-        st.ir_relaxation_factor = parse_number<double>(get(solver, "IrRelaxationFactor", "1"));
+        st.max_iterations = count_setting("MaxIterations", get("Solver", "MaxIterations", "10000"));
+        st.relative_tolerance = real_setting("RelativeTolerance", get("Solver", "RelativeTolerance", "1e-10"));
+        st.jacobi_max_block_size =
+                static_cast<int32_t>(count_setting("JacobiMaxBlockSize", get("Solver", "JacobiMaxBlockSize", "1")));
+        st.use_matrix_free = boolean_setting("UseMatrixFree", get("Solver", "UseMatrixFree", "1")) ? 1 : 0;
+        st.criterion = enum_setting("Criterion", get("Solver", "Criterion", "MomentsTemporalDerivative"),
+                                    {"PotentialTemporalDerivative", "MomentsTemporalDerivative",
+                                     "PotentialAndMomentsTemporalDerivative"});
+        st.preconditioner = preconditioner_setting(get("Solver", "Preconditioner", "Jacobi"));
+        st.vector_potential_gauge_penalty =
+                real_setting("VectorPotentialGaugePenalty", get("Solver", "VectorPotentialGaugePenalty", "20"));
+        st.sor_relaxation_factor = real_setting("SorRelaxationFactor", get("Solver", "SorRelaxationFactor", "1.2"));
+        st.chebyshev_iterations =
+                static_cast<int32_t>(count_setting("ChebyshevIterations", get("Solver", "ChebyshevIterations", "8")));
+        st.chebyshev_lower_bound = real_setting("ChebyshevLowerBound", get("Solver", "ChebyshevLowerBound", "0.05"));
+        st.chebyshev_upper_bound = real_setting("ChebyshevUpperBound", get("Solver", "ChebyshevUpperBound", "1.5"));
+        st.ir_iterations = static_cast<int32_t>(count_setting("IrIterations", get("Solver", "IrIterations", "8")));
+        st.ir_relaxation_factor = real_setting("IrRelaxationFactor", get("Solver", "IrRelaxationFactor", "1"));
         *out = st;
         return SIL_OK;
     } catch (const std::exception& ex) {

