@@ -341,6 +341,22 @@ int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs,
                                              const sil_solver_settings* settings,
                                              sil_solver_diagnostics* diagnostics, void* stream);
 
+/* No counterpart in the reference.  The same FP64 CG (identity or Jacobi
+ * preconditioning; Poisson 2D and linear magnetostatics 2D / 3D operators)
+ * for callers that cannot allocate device memory: code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  It calls
+ * no allocating routine and instantiates no graph: its vectors and partial
+ * sums live in a file-scope device array of the library (about 29 MB on
+ * every device it runs on, held until the process ends), so problems of at
+ * most sil_noalloc_max_unknowns() unknowns (2^19) are accepted and larger
+ * ones are refused with SIL_ERR_UNSUPPORTED.  `solution` is the iterate,
+ * `rhs` is read in place.  Every call on one device shares that array:
+ * calls that overlap on one device race.  Synchronises `stream` on return. */
+int sil_minimize_strong_formulation_residual_noalloc(sil_operator op, const double* rhs, double* solution,
+                                                     const sil_solver_settings* settings,
+                                                     sil_solver_diagnostics* diagnostics, void* stream);
+int64_t sil_noalloc_max_unknowns(void);
+
 /* ------------------------------------------------------------------ */
 /* input formats (host only: no device memory, no stream)              */
 /* ------------------------------------------------------------------ */

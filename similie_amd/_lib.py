@@ -11,6 +11,7 @@ LIB_PATH = _PKG / "libsil_hip.so"
 
 SIL_MAX_DIM = 4
 SIL_OK = 0
+SIL_ERR_INVALID, SIL_ERR_UNSUPPORTED = -1, -2
 
 METRIC_IDENTITY, METRIC_DIAGONAL, METRIC_SYMMETRIC = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
@@ -188,6 +189,11 @@ SIGNATURES = {
         ctypes.c_int,
         [_P, _P, _P, ctypes.POINTER(SolverSettings), ctypes.POINTER(SolverDiagnostics), _P],
     ),
+    "sil_minimize_strong_formulation_residual_noalloc": (
+        ctypes.c_int,
+        [_P, _P, _P, ctypes.POINTER(SolverSettings), ctypes.POINTER(SolverDiagnostics), _P],
+    ),
+    "sil_noalloc_max_unknowns": (_I64, []),
     "sil_batched_poisson2d_create": (ctypes.c_int, [ctypes.POINTER(_P), _I32, ctypes.POINTER(_I32), _P]),
     "sil_batched_poisson2d_offset": (_I64, [_P, _I32]),
     "sil_batched_poisson2d_set_block": (ctypes.c_int, [_P, _I32]),

@@ -10,6 +10,9 @@ struct Workspace;
 void destroy_workspace(Workspace* w);
 int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_solver_settings& st,
              sil_solver_diagnostics& dg, hipStream_t s);
+int cg_solve_noalloc(sil_operator op, const double* rhs, double* solution, const sil_solver_settings& st,
+                     sil_solver_diagnostics& dg, hipStream_t s);
+int64_t noalloc_max_dofs();
 int operator_apply64(sil_operator op, const double* in, double* out, hipStream_t s);
 int operator_apply32(sil_operator op, const float* in, float* out, hipStream_t s);
 int operator_diagonal(sil_operator op, double* d, hipStream_t s);
@@ -606,5 +609,20 @@ int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs,
     }
     return cg_solve(op, rhs, solution, *settings, *diagnostics, as_stream(stream));
 }
+
+int sil_minimize_strong_formulation_residual_noalloc(sil_operator op, const double* rhs, double* solution,
+                                                     const sil_solver_settings* settings,
+                                                     sil_solver_diagnostics* diagnostics, void* stream)
+{
+    SIL_REQUIRE(op && rhs && solution && settings && diagnostics, SIL_ERR_INVALID,
+                "minimize_strong_formulation_residual_noalloc: null argument");
+    SIL_REQUIRE(static_cast<const void*>(rhs) != static_cast<const void*>(solution), SIL_ERR_INVALID,
+                "minimize_strong_formulation_residual_noalloc: the solution (the iterate) must not alias rhs");
+    SIL_REQUIRE(settings->relative_tolerance >= 0.0, SIL_ERR_INVALID, "negative relative tolerance");
+    *diagnostics = sil_solver_diagnostics{};
+    return cg_solve_noalloc(op, rhs, solution, *settings, *diagnostics, as_stream(stream));
+}
+
+int64_t sil_noalloc_max_unknowns(void) { return noalloc_max_dofs(); }
 
 }  // extern "C"

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <new>
 
 #include "operators.hpp"
@@ -41,7 +42,8 @@ struct Workspace {
     double* tmp = nullptr;
     // FP32 recurrence
     float *r32 = nullptr, *q32 = nullptr, *p32[2] = {nullptr, nullptr};
-    double* partials = nullptr; // [3][max blocks]
+    double* partials = nullptr; // [3][part_stride]
+    int64_t part_stride = 0;    // slots per partial-sum array
     CgScalars* sc = nullptr;
     CgScalars* host_sc = nullptr; // pinned
     int apply_blocks = 0;
@@ -402,6 +404,7 @@ int ensure_workspace(sil_operator op, bool mixed, hipStream_t s)
         if ((rc = alloc(&w.x, n)) || (rc = alloc(&w.b, n)) || (rc = alloc(&w.tmp, n)) ||
             (rc = alloc(&w.partials, 3 * max_dot_blocks())) || (rc = alloc(&w.sc, 1)))
             return rc;
+        w.part_stride = max_dot_blocks();
         SIL_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&w.host_sc), sizeof(CgScalars)));
         // number of blocks of one operator application
         int nb = 0;
@@ -432,12 +435,11 @@ enum Precond { PC_IDENTITY = 0, PC_JACOBI = 1, PC_ISAI = 2 };
 // SpdIsai the direction is formed from z (not from r) and the update's
 // ⟨r, z⟩ partials are replaced by those of the preconditioner application.
 template <typename T>
-int enqueue_iteration(sil_operator op, T* r, T* q, T* pold, T* pnew, int pc, hipStream_t s)
+int enqueue_iteration(sil_operator op, Workspace& w, T* r, T* q, T* pold, T* pnew, int pc, hipStream_t s)
 {
-    Workspace& w = *op->ws;
     double* part0 = w.partials;
-    double* part1 = w.partials + max_dot_blocks();
-    double* part2 = w.partials + 2 * max_dot_blocks();
+    double* part1 = w.partials + w.part_stride;
+    double* part2 = w.partials + 2 * w.part_stride;
     const double* invd = pc == PC_JACOBI ? w.inv_diag : nullptr;
     const T* zsrc = r;
     if constexpr (sizeof(T) == 8)
@@ -481,7 +483,7 @@ int get_graph(sil_operator op, int prec, int pc, int chunk, T* r, T* q, T* const
     SIL_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     int rc = SIL_OK;
     for (int it = 0; it < chunk && rc == SIL_OK; ++it)
-        rc = enqueue_iteration<T>(op, r, q, p[it & 1], p[(it + 1) & 1], pc, cs);
+        rc = enqueue_iteration<T>(op, w, r, q, p[it & 1], p[(it + 1) & 1], pc, cs);
     hipError_t e = hipStreamEndCapture(cs, &graph);
     if (rc != SIL_OK) {
         (void)hipStreamDestroy(cs);
@@ -512,9 +514,8 @@ int read_scalars(Workspace& w, hipStream_t s)
     return SIL_OK;
 }
 
-int true_residual(sil_operator op, hipStream_t s)
+int true_residual(sil_operator op, Workspace& w, hipStream_t s)
 {
-    Workspace& w = *op->ws;
     int rc = apply64(op, w.x, w.tmp, DotEpilogue{nullptr}, nullptr, nullptr, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_residual_f64, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.tmp, op->n, w.partials);
@@ -596,7 +597,7 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
     }
     SIL_CHECK_HIP(hipMemcpyAsync(w.b, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     double* part0 = w.partials;
-    double* part1 = w.partials + max_dot_blocks();
+    double* part1 = w.partials + w.part_stride;
     const double* invd = jacobi ? w.inv_diag : nullptr;
     if (mixed)
         hipLaunchKernelGGL((k_init<float>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r32, w.p32[0], invd,
@@ -696,7 +697,7 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
         SIL_CHECK_HIP(hipMemcpyAsync(w.sc, w.host_sc, sizeof(CgScalars), hipMemcpyHostToDevice, s));
         extra_iterations = dd.iterations;
     }
-    if ((rc = true_residual(op, s))) return rc;
+    if ((rc = true_residual(op, w, s))) return rc;
     if ((rc = read_scalars(w, s))) return rc;
     SIL_CHECK_HIP(hipMemcpyAsync(solution, w.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     SIL_CHECK_HIP(hipStreamSynchronize(s));
@@ -727,5 +728,133 @@ int operator_apply32(sil_operator op, const float* in, float* out, hipStream_t s
 }
 
 int operator_diagonal(sil_operator op, double* d, hipStream_t s) { return diagonal(op, d, s); }
+
+// ------------------------------------------------ solve without allocation ---
+//
+// FP64 CG (identity or Jacobi) for callers that cannot allocate device
+// memory — code under stream capture, serving loops, harnesses that forbid
+// hipMalloc.  Every vector and partial sum the solve needs lives in one
+// file-scope device array, `scratch`; its address on a device is looked up
+// once per device (hipGetSymbolAddress, that device current) and only that
+// address is handed to kernels.  The solution array is the iterate and the
+// right-hand side is read in place; no graph is instantiated (iterations are
+// launched in chunks on the caller's stream).  Every region a solve reads is
+// written by that solve first — k_init writes r, p_old and x, the scalar
+// kernels the scalars, each partial array is written before it is reduced —
+// so nothing is cleared between calls.  All calls on one device share the
+// array: calls that overlap on one device race.
+constexpr int64_t kNoAllocMaxDofs = int64_t(1) << 19;
+// partial-sum slots per array: one per workgroup of an application (a march
+// workgroup covers at least one node, the other kernels 256 rows) or of the
+// update kernel
+constexpr int64_t kNoAllocSlots = std::max<int64_t>(kUpdBlocks, kNoAllocMaxDofs / 3 + 64);
+constexpr int64_t kNoAllocVecBytes = kNoAllocMaxDofs * 8;
+constexpr int kNoAllocVectors = 6; // r, q, p[0], p[1], D⁻¹, tmp
+constexpr int64_t kNoAllocPartsOffset = kNoAllocVectors * kNoAllocVecBytes;
+constexpr int64_t kNoAllocScalarsOffset = kNoAllocPartsOffset + 3 * kNoAllocSlots * 8;
+constexpr int64_t kNoAllocBytes = kNoAllocScalarsOffset + ((int64_t(sizeof(CgScalars)) + 255) / 256) * 256;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+// address of `scratch` on the current device (looked up on its first use)
+int scratch_on_current_device(unsigned char** out)
+{
+    int dev = 0;
+    SIL_CHECK_HIP(hipGetDevice(&dev));
+    SIL_REQUIRE(dev >= 0 && dev < kMaxDevices, SIL_ERR_UNSUPPORTED, "device %d outside 0..%d", dev, kMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error("scratch array address on device %d: %s", dev, hipGetErrorString(e));
+            return SIL_ERR_HIP;
+        }
+        g_scratch[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch[dev];
+    return SIL_OK;
+}
+}  // namespace
+
+int64_t noalloc_max_dofs() { return kNoAllocMaxDofs; }
+
+int cg_solve_noalloc(sil_operator op, const double* rhs, double* solution, const sil_solver_settings& st,
+                     sil_solver_diagnostics& dg, hipStream_t s)
+{
+    SIL_REQUIRE(st.precision == SIL_PRECISION_FP64, SIL_ERR_UNSUPPORTED,
+                "solve without allocation: FP64 only (mixed precision needs FP32 vectors)");
+    SIL_REQUIRE(st.preconditioner == SIL_PRECOND_IDENTITY || st.preconditioner == SIL_PRECOND_JACOBI,
+                SIL_ERR_UNSUPPORTED, "solve without allocation: identity or Jacobi preconditioning");
+    SIL_REQUIRE(op->kind == OP_POISSON2D || op->kind == OP_MAGNETOSTATICS3D || op->kind == OP_MAGNETOSTATICS2D,
+                SIL_ERR_UNSUPPORTED, "solve without allocation: linear matrix-free operators (Poisson 2D, "
+                                     "magnetostatics 2D / 3D) only");
+    const int64_t n = op->n;
+    SIL_REQUIRE(n <= kNoAllocMaxDofs, SIL_ERR_UNSUPPORTED,
+                "solve without allocation: %lld unknowns exceed the %lld the scratch array holds", (long long)n,
+                (long long)kNoAllocMaxDofs);
+    unsigned char* base = nullptr;
+    int rc = scratch_on_current_device(&base);
+    if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int pc = st.preconditioner == SIL_PRECOND_JACOBI ? PC_JACOBI : PC_IDENTITY;
+    Workspace w{};
+    w.n = n;
+    w.x = solution;
+    w.b = const_cast<double*>(rhs); // read only
+    auto vec = [&](int i) { return reinterpret_cast<double*>(base + i * kNoAllocVecBytes); };
+    w.r = vec(0);
+    w.q = vec(1);
+    w.p[0] = vec(2);
+    w.p[1] = vec(3);
+    w.inv_diag = vec(4);
+    w.tmp = vec(5);
+    w.partials = reinterpret_cast<double*>(base + kNoAllocPartsOffset);
+    w.part_stride = kNoAllocSlots;
+    w.sc = reinterpret_cast<CgScalars*>(base + kNoAllocScalarsOffset);
+    CgScalars host{}; // pageable copy of the device scalars
+    w.host_sc = &host;
+
+    const double* invd = nullptr;
+    if (pc == PC_JACOBI) {
+        if ((rc = diagonal(op, w.tmp, s))) return rc;
+        hipLaunchKernelGGL(k_inv, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.tmp, w.inv_diag, n);
+        SIL_LAUNCH_CHECK();
+        invd = w.inv_diag;
+    }
+    double* part0 = w.partials;
+    double* part1 = w.partials + w.part_stride;
+    hipLaunchKernelGGL((k_init<double>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r, w.p[0], invd, n,
+                       part0, part1);
+    hipLaunchKernelGGL(k_scalars_init, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, kUpdBlocks, part1, kUpdBlocks,
+                       st.relative_tolerance, st.max_iterations);
+    SIL_LAUNCH_CHECK();
+    int chunk = st.check_interval > 0 ? st.check_interval : (n > (1 << 20) ? 16 : 32);
+    chunk += chunk & 1; // even: the p ping-pong returns to buffer 0
+    if ((rc = read_scalars(w, s))) return rc;
+    dg.initial_residual_l2 = std::sqrt(host.bb);
+    while (!host.done) {
+        for (int it = 0; it < chunk; ++it)
+            if ((rc = enqueue_iteration<double>(op, w, w.r, w.q, w.p[it & 1], w.p[(it + 1) & 1], pc, s))) return rc;
+        if ((rc = read_scalars(w, s))) return rc;
+    }
+    if ((rc = true_residual(op, w, s))) return rc;
+    if ((rc = read_scalars(w, s))) return rc;
+    dg.iterations = host.iter;
+    const double bnorm = std::sqrt(host.bb);
+    dg.final_residual_l2 = std::sqrt(host.rr);
+    dg.final_relative_residual = bnorm == 0.0 ? 0.0 : dg.final_residual_l2 / bnorm;
+    dg.true_relative_residual = bnorm == 0.0 ? 0.0 : std::sqrt(host.rr_true) / bnorm;
+    dg.converged = bnorm == 0.0 ? 1 : host.converged;
+    dg.verified = (bnorm == 0.0 || dg.true_relative_residual <= st.relative_tolerance * (1.0 + 1e-6)) ? 1 : 0;
+    dg.outer_iterations = 0;
+    dg.duration = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return SIL_OK;
+}
 
 }  // namespace sil
