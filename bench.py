@@ -58,40 +58,72 @@ def profiled_traffic():
     return float(m.group(1)) if m else None
 
 
+_CPU = {}
+
+
+def cpu_lib():
+    """The CPU baselines (oracle/cpu_baseline.c, oracle/cpu_fast.c) built on
+    this host with -O3 -march=native -fopenmp, once per run; the literal
+    oracle build when that fails.  (CpuBaseline, build flags)"""
+    if "base" not in _CPU:
+        import tempfile
+
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as orc
+
+        path = orc.build_native(tempfile.mkdtemp(prefix="sil_cpu_"))
+        _CPU["base"] = orc.CpuBaseline(path)
+        _CPU["build"] = "-O3 -march=native -fopenmp" if path else "-O2 -fopenmp (the native build failed)"
+    return _CPU["base"], _CPU["build"]
+
+
+def _median_time(fn, seconds, least=5, most=20):
+    fn()  # warm-up
+    times = []
+    t_end = time.perf_counter() + seconds
+    while len(times) < least or (len(times) < most and time.perf_counter() < t_end):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return sorted(times)[len(times) // 2], len(times)
+
+
 def cpu_baseline(seconds):
-    """Reference-structured staged CPU Laplacian (oracle/cpu_baseline.c) on a
-    bounded sample of the same workload: the 96³ corner sub-box."""
-    import numpy as np
-
-    sys.path.insert(0, str(ROOT / "tests"))
-    import oracle_lib as orc
-
-    n = 96
-    h = 1.01 ** np.arange(N - 1)
-    c_full = np.concatenate([[0.0], np.cumsum(h)]) / h.sum()
-    c = c_full[:n]
+    """The full 256³ workload on the host cores: the reference-structured
+    staged Laplacian (90 ⋆ doubles per node filled once, six OpenMP passes
+    per application, oracle/cpu_baseline.c) is the value; the fused
+    one-sweep CPU Laplacian (oracle/cpu_fast.c) is reported beside it."""
+    base, build = cpu_lib()
+    n = N
+    h = 1.01 ** np.arange(n - 1)
+    c = np.concatenate([[0.0], np.cumsum(h)]) / h.sum()
     gx = 1 + c * c
     g = np.empty((n, n, n, 3))
     g[..., 0] = gx[:, None, None]
     g[..., 1] = gx[None, :, None]
     g[..., 2] = gx[None, None, :]
+    g = g.reshape(-1, 3)
     w = np.random.default_rng(1234).standard_normal((n, n, n, 3))
-    st = orc.StagedLaplacian3D1([n] * 3, [c] * 3, g.reshape(-1, 3))
-    st.run(w, [n - 1] * 3)  # warm-up
-    times = []
-    t_end = time.perf_counter() + seconds
-    while time.perf_counter() < t_end or len(times) < 3:
-        t0 = time.perf_counter()
-        st.run(w, [n - 1] * 3)
-        times.append(time.perf_counter() - t0)
-    t = sorted(times)[len(times) // 2]
+    out = np.empty((n - 1, n - 1, n - 1, 3))
+    oext = [n - 1] * 3
+    t0 = time.perf_counter()
+    st = base.staged([n] * 3, [c] * 3, g)
+    setup = time.perf_counter() - t0
+    t_staged, runs = _median_time(lambda: st.run(w, oext, out), seconds)
+    del st
+    t_fused, runs_f = _median_time(lambda: base.fused_lap3d1([n] * 3, [c] * 3, g, w, oext, out), seconds)
+    B = algorithmic_bytes(n)
     return {
-        "value": round(algorithmic_bytes(n) / t / 1e9, 3),
+        "value": round(B / t_staged / 1e9, 3),
         "unit": "GB/s",
-        "cores": orc.threads(),
+        "cores": base.threads(),
         "kind": "port",
-        "sample": f"reference-structured staged Laplacian (90 star doubles/node, six OpenMP passes) on the "
-                  f"{n}^3 corner sub-box of the same workload, median of {len(times)} applications",
+        "sample": f"the full 256^3 workload: reference-structured staged Laplacian (90 star doubles per node "
+                  f"filled once in {setup:.1f} s, six OpenMP passes), median of {runs} applications; built "
+                  f"{build}",
+        "fused_value": round(B / t_fused / 1e9, 3),
+        "fused_sample": f"one-sweep CPU Laplacian (per-thread x-plane ranges, rolling plane buffers), median of "
+                        f"{runs_f} applications",
     }
 
 
@@ -134,8 +166,9 @@ def laplacian_2d_0form(torch):
             "note": "1 MB in + out: the application is launch-latency bound, not HBM bound"}
 
 
-def solve_poisson_bumps(torch):
-    """256² Dirichlet Poisson, three Gaussian bumps, plain FP64 Jacobi CG to 1e-10."""
+def solve_poisson_bumps(torch, cpu=True):
+    """256² Dirichlet Poisson, three Gaussian bumps, plain FP64 Jacobi CG to
+    1e-10; the same solve on the host cores beside it."""
     from similie_amd import solvers as S
     from similie_amd.workloads import poisson_bumps
 
@@ -145,13 +178,25 @@ def solve_poisson_bumps(torch):
     st = S.StrongFormulationSolverSettings(max_iterations=20000, relative_tolerance=1e-10)
     S.minimize_strong_formulation_residual(op, rhs, settings=st)  # warm-up (graph capture)
     _, d = S.minimize_strong_formulation_residual(op, rhs, settings=st)
-    return {"workload": "Dirichlet Poisson 256^2, h = 1/256, three Gaussian bumps", "method": "FP64 Jacobi CG",
-            "seconds": round(d["duration"], 5), "iterations": d["iterations"],
-            "verified": bool(d["verified"]), "true_relative_residual": d["true_relative_residual"]}
+    out = {"workload": "Dirichlet Poisson 256^2, h = 1/256, three Gaussian bumps", "method": "FP64 Jacobi CG",
+           "seconds": round(d["duration"], 5), "iterations": d["iterations"],
+           "verified": bool(d["verified"]), "true_relative_residual": d["true_relative_residual"]}
+    if cpu:
+        base, build = cpu_lib()
+        h = np.full(len(x), 1.0 / 256.0)
+        base.poisson2d_cg(h, h, None, b, 1e-10, 20000, jacobi=True)  # warm-up
+        t0 = time.perf_counter()
+        _, it, rn, bn = base.poisson2d_cg(h, h, None, b, 1e-10, 20000, jacobi=True)
+        out["cpu_baseline"] = {"seconds": round(time.perf_counter() - t0, 5), "iterations": it,
+                               "relative_residual": rn / bn, "cores": base.threads(), "kind": "port",
+                               "sample": f"the full solve: FP64 Jacobi CG, OpenMP, built {build}"}
+    return out
 
 
-def solve_magnetostatics(torch, max_iterations=20000):
-    """192³-cell inductor (μ_r 1000 core, coil annulus): FP64 vs mixed CG to 1e-10."""
+def solve_magnetostatics(torch, max_iterations=20000, cpu=True):
+    """192³-cell inductor (μ_r 1000 core, coil annulus): FP64 vs mixed CG to
+    1e-10; FP64 Jacobi CG on the host cores beside it (a sample of
+    iterations, projected to the FP64 solve's iteration count)."""
     from similie_amd import solvers as S
     from similie_amd.workloads import MU0, magnetostatics_inductor
 
@@ -171,6 +216,22 @@ def solve_magnetostatics(torch, max_iterations=20000):
                      "converged": bool(d["converged"]), "verified": bool(d["verified"]), "true_relative_residual": d["true_relative_residual"]}
     if out["mixed"]["seconds"] > 0:
         out["mixed_speedup"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    if cpu:
+        base, build = cpu_lib()
+        b = rhs.cpu().numpy()
+        muh = mu.cpu().numpy()
+        times = {}
+        for k in (2, 22):
+            t0 = time.perf_counter()
+            base.ms3d_cg(coords, muh, 1e7, b, 1e-10, -k, jacobi=True)
+            times[k] = time.perf_counter() - t0
+        per = (times[22] - times[2]) / 20
+        its = out["fp64"]["iterations"]
+        out["cpu_baseline"] = {"seconds": round(per * its, 3), "seconds_per_iteration": round(per, 5),
+                               "cores": base.threads(), "kind": "port",
+                               "sample": f"FP64 Jacobi CG (OpenMP, built {build}) on the same 192^3 system: 20 "
+                                         f"iterations timed (a 22-iteration run minus a 2-iteration run), projected "
+                                         f"to the {its} iterations of the FP64 solve"}
     return out
 
 
@@ -394,7 +455,7 @@ def solve_uneven_batch(torch):
             "unknowns": int(rhs.numel())}
 
 
-def solve_learned(torch):
+def solve_learned(torch, cpu=True):
     """64 held-out 256² problems (κ = exp(GRF σ 1.5, corr 16)): plain
     (unpreconditioned) FP64 CG, Jacobi FP64 CG, mixed CG and flexible CG with
     the trained network, to an FP64-verified 1e-10."""
@@ -407,10 +468,25 @@ def solve_learned(torch):
     net.set_params(torch.from_numpy(np.load(DEFAULT_WEIGHTS)))
     res = evaluate(net, 64, 256, 1.5, 16.0, 900000, log=lambda m: None)
     best = min(res["fcg_learned"]["seconds"], res["fcg_learned_mixed"]["seconds"])
-    return {"workload": "64 held-out Poisson problems 256^2, kappa = exp(GRF sigma 1.5, corr 16), N(0,1) rhs",
-            "methods": res,
-            "learned_speedup_vs_plain_fp64_cg": round(res["cg_plain"]["seconds"] / best, 3),
-            "learned_speedup_vs_jacobi_fp64_cg": round(res["cg"]["seconds"] / best, 3)}
+    out = {"workload": "64 held-out Poisson problems 256^2, kappa = exp(GRF sigma 1.5, corr 16), N(0,1) rhs",
+           "methods": res,
+           "learned_speedup_vs_plain_fp64_cg": round(res["cg_plain"]["seconds"] / best, 3),
+           "learned_speedup_vs_jacobi_fp64_cg": round(res["cg"]["seconds"] / best, 3)}
+    if cpu:
+        from similie_amd.preconditioner_training import make_problems
+
+        base, build = cpu_lib()
+        kappa, b = make_problems(64, 256, 1.5, 16.0, 900000)
+        kh, bh = kappa[:4].cpu().numpy(), b[:4].cpu().numpy()
+        ones = np.ones(256)
+        t0 = time.perf_counter()
+        its = [base.poisson2d_cg(ones, ones, kh[p], bh[p], 1e-10, 20000, jacobi=True)[1] for p in range(4)]
+        secs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"seconds": round(secs * 16, 3), "iterations_mean": float(np.mean(its)),
+                               "cores": base.threads(), "kind": "port",
+                               "sample": f"FP64 Jacobi CG (OpenMP, built {build}) on the first 4 of the 64 problems "
+                                         f"to 1e-10, their time x 16"}
+    return out
 
 
 def main():
@@ -418,7 +494,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0,
+                    help="time budget of each CPU Laplacian baseline (at least 5 applications)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solves", action="store_true", help="skip the solver workloads")
     ap.add_argument("--solves", default="", help="comma-separated subset of the solver workloads to run")
@@ -519,12 +596,13 @@ def main():
     if not args.no_solves:
         del grid, g, w, out, staged
         torch.cuda.empty_cache()
+        cpu = not args.no_cpu_baseline
         solves = {
             "laplacian_2d_0form": lambda: laplacian_2d_0form(torch),
-            "poisson2d_bumps": lambda: solve_poisson_bumps(torch),
+            "poisson2d_bumps": lambda: solve_poisson_bumps(torch, cpu),
             "uneven_batch_512": lambda: solve_uneven_batch(torch),
-            "learned_preconditioner": lambda: solve_learned(torch),
-            "magnetostatics_inductor": lambda: solve_magnetostatics(torch, args.ms_iterations),
+            "learned_preconditioner": lambda: solve_learned(torch, cpu),
+            "magnetostatics_inductor": lambda: solve_magnetostatics(torch, args.ms_iterations, cpu),
             "magnetostatics_assembled": lambda: magnetostatics_assembled(torch, args.ms_iterations),
             "magnetostatics_nonlinear": lambda: magnetostatics_nonlinear(torch, args.ms_iterations),
             "magnetostatics_post_process": lambda: magnetostatics_post_process(torch),

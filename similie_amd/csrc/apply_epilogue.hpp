@@ -16,7 +16,7 @@ __device__ __forceinline__ T p_value(const PUpdate<T>* pu, const T* x, int64_t i
     // value of the direction at row idx: either x[idx] or z + β p_old
     if (pu == nullptr) return x[idx];
     const double beta = *pu->beta;
-    const double z = pu->inv_diag ? (double)pu->r[idx] * pu->inv_diag[idx] : (double)pu->r[idx];
+    const double z = pu->inv_diag ? (double)pu->r[idx] * (double)pu->inv_diag[idx] : (double)pu->r[idx];
     return (T)(z + beta * (double)pu->p_old[idx]);
 }
 

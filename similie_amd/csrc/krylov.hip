@@ -40,8 +40,10 @@ struct Workspace {
     double *x = nullptr, *b = nullptr, *r = nullptr, *q = nullptr, *p[2] = {nullptr, nullptr};
     double* inv_diag = nullptr;
     double* tmp = nullptr;
-    // FP32 recurrence
+    // FP32 recurrence: r, q, p, the Jacobi inverse and the correction of x
+    // accumulated since the last FP64 residual replacement
     float *r32 = nullptr, *q32 = nullptr, *p32[2] = {nullptr, nullptr};
+    float *inv_diag32 = nullptr, *xc32 = nullptr;
     double* partials = nullptr; // [3][part_stride]
     int64_t part_stride = 0;    // slots per partial-sum array
     CgScalars* sc = nullptr;
@@ -108,18 +110,20 @@ __global__ __launch_bounds__(kUpdBlock) void k_dot2(const T* __restrict__ a, con
 template <typename T>
 __global__ __launch_bounds__(kUpdBlock) void k_init(const double* __restrict__ b, double* __restrict__ x,
                                                     T* __restrict__ r, T* __restrict__ pold,
-                                                    const double* __restrict__ inv_diag, int64_t n,
-                                                    double* __restrict__ part_bb, double* __restrict__ part_rz)
+                                                    const T* __restrict__ inv_diag, int64_t n,
+                                                    double* __restrict__ part_bb, double* __restrict__ part_rz,
+                                                    float* __restrict__ xc)
 {
     double vb = 0.0, vz = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double bi = b[i];
         x[i] = 0.0;
+        if (xc) xc[i] = 0.0f;
         r[i] = (T)bi;
         pold[i] = T(0);
         const double ri = (double)(T)bi;
         vb += bi * bi;
-        vz += ri * (inv_diag ? ri * inv_diag[i] : ri);
+        vz += ri * (inv_diag ? ri * (double)inv_diag[i] : ri);
     }
     __shared__ double sh[2][kUpdBlock / 64];
     for (int off = 32; off > 0; off >>= 1) {
@@ -173,23 +177,29 @@ __global__ void k_alpha(CgScalars* sc, const double* part, int count)
     }
 }
 
-template <typename T>
+// XC: x accumulates in the FP32 correction xc (folded into the FP64 x at
+// every residual replacement) instead of in x itself
+template <typename T, bool XC = false>
 __global__ __launch_bounds__(kUpdBlock) void k_update(const CgScalars* __restrict__ sc, double* __restrict__ x,
                                                       const T* __restrict__ p, T* __restrict__ r,
-                                                      const T* __restrict__ q, const double* __restrict__ inv_diag,
+                                                      const T* __restrict__ q, const T* __restrict__ inv_diag,
                                                       int64_t n, double* __restrict__ part_rz,
-                                                      double* __restrict__ part_rr)
+                                                      double* __restrict__ part_rr, float* __restrict__ xc)
 {
     if (sc->done) return;
     const double alpha = sc->alpha;
+    const float alpha32 = (float)alpha;
     double vz = 0.0, vr = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        x[i] += alpha * (double)p[i];
+        if constexpr (XC)
+            xc[i] = fmaf(alpha32, (float)p[i], xc[i]);
+        else
+            x[i] += alpha * (double)p[i];
         const T ri = (T)((double)r[i] - alpha * (double)q[i]);
         r[i] = ri;
         const double rd = (double)ri;
         vr += rd * rd;
-        vz += rd * (inv_diag ? rd * inv_diag[i] : rd);
+        vz += rd * (inv_diag ? rd * (double)inv_diag[i] : rd);
     }
     __shared__ double sh[2][kUpdBlock / 64];
     for (int off = 32; off > 0; off >>= 1) {
@@ -236,7 +246,7 @@ __global__ void k_beta(CgScalars* sc, const double* part_rz, int count_rz, const
 // <r,r> (FP64, exact residual) and <r,z> of the rounded residual
 template <typename T>
 __global__ __launch_bounds__(kUpdBlock) void k_replace(const double* __restrict__ b, const double* __restrict__ ax,
-                                                       T* __restrict__ r, const double* __restrict__ inv_diag,
+                                                       T* __restrict__ r, const T* __restrict__ inv_diag,
                                                        int64_t n, double* __restrict__ part_rr,
                                                        double* __restrict__ part_rz)
 {
@@ -247,7 +257,7 @@ __global__ __launch_bounds__(kUpdBlock) void k_replace(const double* __restrict_
         r[i] = rt;
         vr += res * res;
         const double rd = (double)rt;
-        vz += rd * (inv_diag ? rd * inv_diag[i] : rd);
+        vz += rd * (inv_diag ? rd * (double)inv_diag[i] : rd);
     }
     __shared__ double sh[2][kUpdBlock / 64];
     for (int off = 32; off > 0; off >>= 1) {
@@ -304,6 +314,21 @@ __global__ void k_add(double* __restrict__ x, const double* __restrict__ d, int6
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         x[i] += d[i];
+}
+
+// x += xc; xc = 0: the FP32 correction folded into the FP64 iterate
+__global__ void k_flush(double* __restrict__ x, float* __restrict__ xc, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += (double)xc[i];
+        xc[i] = 0.0f;
+    }
+}
+
+__global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
 }
 
 __global__ void k_inv(const double* __restrict__ d, double* __restrict__ inv, int64_t n)
@@ -423,7 +448,7 @@ int ensure_workspace(sil_operator op, bool mixed, hipStream_t s)
     }
     if (mixed && !w.r32) {
         if ((rc = alloc(&w.r32, n)) || (rc = alloc(&w.q32, n)) || (rc = alloc(&w.p32[0], n)) ||
-            (rc = alloc(&w.p32[1], n)))
+            (rc = alloc(&w.p32[1], n)) || (rc = alloc(&w.xc32, n)))
             return rc;
     }
     return SIL_OK;
@@ -440,7 +465,13 @@ int enqueue_iteration(sil_operator op, Workspace& w, T* r, T* q, T* pold, T* pne
     double* part0 = w.partials;
     double* part1 = w.partials + w.part_stride;
     double* part2 = w.partials + 2 * w.part_stride;
-    const double* invd = pc == PC_JACOBI ? w.inv_diag : nullptr;
+    const T* invd = nullptr;
+    if (pc == PC_JACOBI) {
+        if constexpr (sizeof(T) == 8)
+            invd = w.inv_diag;
+        else
+            invd = w.inv_diag32;
+    }
     const T* zsrc = r;
     if constexpr (sizeof(T) == 8)
         if (pc == PC_ISAI) zsrc = w.isai_z;
@@ -453,8 +484,12 @@ int enqueue_iteration(sil_operator op, Workspace& w, T* r, T* q, T* pold, T* pne
         rc = apply32(op, nullptr, q, DotEpilogue{part0}, &pu, &nb, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, nb);
-    hipLaunchKernelGGL((k_update<T>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.sc, w.x, pnew, r, q, invd, op->n,
-                       part1, part2);
+    if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL((k_update<T, true>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.sc, w.x, pnew, r, q, invd,
+                           op->n, part1, part2, w.xc32);
+    else
+        hipLaunchKernelGGL((k_update<T, false>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.sc, w.x, pnew, r, q, invd,
+                           op->n, part1, part2, nullptr);
     int nb_rz = kUpdBlocks;
     if constexpr (sizeof(T) == 8)
         if (pc == PC_ISAI && (rc = isai_apply(w, r, part1, &nb_rz, &w.sc->done, s))) return rc;
@@ -541,7 +576,7 @@ void destroy_workspace(Workspace* w)
         if (p) (void)hipFree(p);
     csr_free(w->isai_g);
     csr_free(w->isai_gt);
-    for (float* p : {w->r32, w->q32, w->p32[0], w->p32[1]})
+    for (float* p : {w->r32, w->q32, w->p32[0], w->p32[1], w->inv_diag32, w->xc32})
         if (p) (void)hipFree(p);
     if (w->sc) (void)hipFree(w->sc);
     if (w->host_sc) (void)hipHostFree(w->host_sc);
@@ -589,9 +624,15 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
         if ((rc = alloc(&w.inv_diag, n))) return rc;
         w.jacobi_ready = false;
     }
+    if (jacobi && mixed && !w.inv_diag32) {
+        if ((rc = alloc(&w.inv_diag32, n))) return rc;
+        w.jacobi_ready = false;
+    }
     if (jacobi && !w.jacobi_ready) {
         if ((rc = diagonal(op, w.tmp, s))) return rc;
         hipLaunchKernelGGL(k_inv, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.tmp, w.inv_diag, n);
+        if (w.inv_diag32)
+            hipLaunchKernelGGL(k_to_f32, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.inv_diag, w.inv_diag32, n);
         SIL_LAUNCH_CHECK();
         w.jacobi_ready = true;
     }
@@ -599,12 +640,13 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
     double* part0 = w.partials;
     double* part1 = w.partials + w.part_stride;
     const double* invd = jacobi ? w.inv_diag : nullptr;
+    const float* invd32 = jacobi ? w.inv_diag32 : nullptr;
     if (mixed)
-        hipLaunchKernelGGL((k_init<float>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r32, w.p32[0], invd,
-                           n, part0, part1);
+        hipLaunchKernelGGL((k_init<float>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r32, w.p32[0],
+                           invd32, n, part0, part1, w.xc32);
     else
         hipLaunchKernelGGL((k_init<double>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r, w.p[0], invd, n,
-                           part0, part1);
+                           part0, part1, nullptr);
     int nb_rz = kUpdBlocks;
     if (pc == PC_ISAI && (rc = isai_apply(w, w.r, part1, &nb_rz, nullptr, s))) return rc;
     hipLaunchKernelGGL(k_scalars_init, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, kUpdBlocks, part1, nb_rz,
@@ -647,8 +689,9 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
         const bool want = h.converged || since_replace >= replace || h.rr < 1e-4 * rr_ref;
         if (want && !hit_max) {
             // FP64 residual replacement; keeps p (the direction buffer in use is p32[0])
+            hipLaunchKernelGGL(k_flush, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.x, w.xc32, n);
             if ((rc = apply64(op, w.x, w.tmp, DotEpilogue{nullptr}, nullptr, nullptr, s))) return rc;
-            hipLaunchKernelGGL((k_replace<float>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.tmp, w.r32, invd,
+            hipLaunchKernelGGL((k_replace<float>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.tmp, w.r32, invd32,
                                n, part0, part1);
             hipLaunchKernelGGL(k_after_replace, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, part1, kUpdBlocks);
             SIL_LAUNCH_CHECK();
@@ -667,6 +710,8 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
             break;
         }
     }
+    // the FP32 correction since the last replacement joins the iterate
+    if (mixed) hipLaunchKernelGGL(k_flush, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.x, w.xc32, n);
     int extra_iterations = 0;
     if (stalled) {
         // FP64 CG on the correction A d = b − A x, then x + d
@@ -830,7 +875,7 @@ int cg_solve_noalloc(sil_operator op, const double* rhs, double* solution, const
     double* part0 = w.partials;
     double* part1 = w.partials + w.part_stride;
     hipLaunchKernelGGL((k_init<double>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r, w.p[0], invd, n,
-                       part0, part1);
+                       part0, part1, nullptr);
     hipLaunchKernelGGL(k_scalars_init, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, kUpdBlocks, part1, kUpdBlocks,
                        st.relative_tolerance, st.max_iterations);
     SIL_LAUNCH_CHECK();

@@ -100,7 +100,7 @@ struct PUpdate {
     const T* p_old;
     T* p_new;
     const double* beta;  // device scalar
-    const double* inv_diag; // Jacobi (FP64) or null
+    const T* inv_diag;   // Jacobi, in the recurrence's arithmetic, or null
     const int* done;     // device flag: skip when set
 };
 

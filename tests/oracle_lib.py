@@ -395,10 +395,12 @@ def poisson2d_cg(hx, hy, kappa, b, tol, max_it):
 
 class StagedLaplacian3D1:
     """Reference-structured CPU staged Laplacian (90 ⋆ doubles per node, six
-    OpenMP passes) used as bench.py's CPU baseline."""
+    OpenMP passes) used as bench.py's CPU baseline; `L` is the library it
+    runs from (the literal oracle build by default)."""
 
-    def __init__(self, ext, axes, metric_diag):
-        L = lib()
+    def __init__(self, ext, axes, metric_diag, L=None):
+        L = L or lib()
+        self.L = L
         L.orc_staged_lap3d1_create.restype = ctypes.c_void_p
         L.orc_staged_lap3d1_run.argtypes = [ctypes.c_void_p, _dp, _dp, _lp]
         L.orc_staged_lap3d1_destroy.argtypes = [ctypes.c_void_p]
@@ -408,16 +410,81 @@ class StagedLaplacian3D1:
         ax = (_dp * 3)(*[_d(a) for a in self.axes])
         self.h = L.orc_staged_lap3d1_create(_l(self.ext), ax, _d(self.metric) if self.metric is not None else None)
 
-    def run(self, w, out_ext):
-        out = np.zeros(tuple(out_ext) + (3,))
+    def run(self, w, out_ext, out=None):
+        if out is None:
+            out = np.zeros(tuple(out_ext) + (3,))
         ww = _f64(w)
-        lib().orc_staged_lap3d1_run(self.h, _d(ww), _d(out), _l(out_ext))
+        self.L.orc_staged_lap3d1_run(self.h, _d(ww), _d(out), _l(out_ext))
         return out
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_staged_lap3d1_destroy(self.h)
+            self.L.orc_staged_lap3d1_destroy(self.h)
             self.h = None
+
+
+class CpuBaseline:
+    """The CPU baselines of oracle/cpu_baseline.c and oracle/cpu_fast.c from
+    one build: the literal oracle library, or the -O3 -march=native build
+    that bench.py makes on the host it times (build_native)."""
+
+    def __init__(self, path=None):
+        self.L = ctypes.CDLL(str(path)) if path else lib()
+        self.L.orc_threads.restype = ctypes.c_int
+        self.L.orc_fast_poisson2d_cg.restype = ctypes.c_int
+        self.L.orc_fast_ms3d_cg.restype = ctypes.c_int
+
+    def threads(self):
+        return int(self.L.orc_threads())
+
+    def staged(self, ext, axes, metric_diag):
+        return StagedLaplacian3D1(ext, axes, metric_diag, L=self.L)
+
+    def fused_lap3d1(self, ext, axes, metric_diag, w, out_ext, out=None):
+        """Δ of a 1-form, one sweep per thread; metric_diag [nodes][3]"""
+        if out is None:
+            out = np.zeros(tuple(out_ext) + (3,))
+        axes = [_f64(a) for a in axes]
+        ax = (_dp * 3)(*[_d(a) for a in axes])
+        g, ww = _f64(metric_diag), _f64(w)
+        self.L.orc_fused_lap3d1(_l(ext), ax, _d(g), _d(ww), _d(out), _l(out_ext))
+        return out
+
+    def poisson2d_cg(self, hx, hy, kappa, b, tol, max_it, jacobi=True):
+        """(x, iterations, final recurrence ‖r‖, ‖b‖); max_it < 0 runs exactly −max_it iterations"""
+        nx, ny = b.shape
+        hx, hy, bb = _f64(hx), _f64(hy), _f64(b)
+        kp = _d(_f64(kappa)) if kappa is not None else None
+        x = np.zeros((nx, ny))
+        res = np.zeros(2)
+        it = self.L.orc_fast_poisson2d_cg(ctypes.c_long(nx), ctypes.c_long(ny), _d(hx), _d(hy), kp, _d(bb),
+                                          ctypes.c_double(tol), ctypes.c_int(max_it), ctypes.c_int(int(jacobi)),
+                                          _d(x), _d(res))
+        return x, it, res[0], res[1]
+
+    def ms3d_cg(self, coords, mu, penalty, b, tol, max_it, jacobi=True):
+        nx, ny, nz = (len(c) for c in coords)
+        cs = [_f64(c) for c in coords]
+        m, bb = _f64(mu), _f64(b)
+        x = np.zeros(3 * nx * ny * nz)
+        res = np.zeros(2)
+        it = self.L.orc_fast_ms3d_cg(ctypes.c_long(nx), ctypes.c_long(ny), ctypes.c_long(nz), _d(cs[0]), _d(cs[1]),
+                                     _d(cs[2]), _d(m), ctypes.c_double(penalty), _d(bb), ctypes.c_double(tol),
+                                     ctypes.c_int(max_it), ctypes.c_int(int(jacobi)), _d(x), _d(res))
+        return x, it, res[0], res[1]
+
+
+def build_native(directory):
+    """Builds the CPU baselines with -O3 -march=native into `directory` on
+    this host; returns the library path, or None when the build fails."""
+    out = Path(directory)
+    try:
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "native", f"NATIVE_DIR={out}/"], check=True,
+                       capture_output=True, timeout=300)
+    except (subprocess.CalledProcessError, subprocess.TimeoutExpired, OSError):
+        return None
+    path = out / "libcpubaseline.so"
+    return path if path.exists() else None
 
 
 def threads():
