@@ -231,6 +231,46 @@ StagedLaplacian<N> make_staged_laplacian(Grid<N> const& grid, FormView<N> out, F
     return StagedLaplacian<N>(grid, out, in, degree);
 }
 
+// StagedCodifferential(exec, tensor, metric, position).run(out, tensor)
+// (codifferential.hpp:321-508)
+template <int N>
+class StagedCodifferential {
+public:
+    StagedCodifferential(Grid<N> const& grid, FormView<N> out, FormView<N> in, int degree)
+        : in_(in), out_(out)
+    {
+        sil_view vi = in.view(), vo = out.view();
+        check(sil_make_staged_codifferential(&handle_, grid.handle(), degree, &vi, &vo));
+    }
+    StagedCodifferential(const StagedCodifferential&) = delete;
+    StagedCodifferential& operator=(const StagedCodifferential&) = delete;
+    ~StagedCodifferential() { sil_staged_codifferential_destroy(handle_); }
+    FormView<N> run(void* stream, FormView<N> out, FormView<N> in)
+    {
+        if (!same_layout(in, in_) || !same_layout(out, out_))
+            throw Error(SIL_ERR_INVALID, "staged codifferential called with layouts it was not built for");
+        check(sil_staged_codifferential_run(handle_, out.data, in.data, stream));
+        return out;
+    }
+
+private:
+    static bool same_layout(FormView<N> const& a, FormView<N> const& b)
+    {
+        return a.extent == b.extent && a.stride == b.stride && a.comp_stride == b.comp_stride &&
+               a.comp_extent == b.comp_extent && a.spectators == b.spectators;
+    }
+
+    FormView<N> in_, out_;
+    sil_staged_codifferential handle_ = nullptr;
+};
+
+template <int N>
+StagedCodifferential<N> make_staged_codifferential(Grid<N> const& grid, FormView<N> out, FormView<N> in,
+                                                   int degree)
+{
+    return StagedCodifferential<N>(grid, out, in, degree);
+}
+
 }  // namespace exterior
 
 namespace solvers {

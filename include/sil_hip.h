@@ -124,6 +124,16 @@ int sil_hodge_star(sil_grid grid, int32_t k, const double* in, const sil_view* i
 int sil_codifferential(sil_grid grid, int32_t k, const double* in, const sil_view* in_view, double* out,
                        const sil_view* out_view, void* stream);
 
+/* StagedCodifferential (codifferential.hpp:321-508): the request validated
+ * once, then run(out, in) any number of times on arrays of the same views.
+ * The Hodge coefficients are evaluated inside the kernel from the grid's
+ * geometry (the reference stores the two star tensors instead). */
+typedef struct sil_staged_codifferential_s* sil_staged_codifferential;
+int sil_make_staged_codifferential(sil_staged_codifferential* out, sil_grid grid, int32_t k, const sil_view* in_view,
+                                   const sil_view* out_view);
+int sil_staged_codifferential_run(sil_staged_codifferential op, double* out, const double* in, void* stream);
+int sil_staged_codifferential_destroy(sil_staged_codifferential op);
+
 /* laplacian (laplacian.hpp:814-951): Hodge Laplacian δd + dδ of a k-form on
  * the grid's node domain, written on out_view's extents (the reference's
  * callers use the node domain minus the last node of every axis).  Fused
@@ -174,6 +184,70 @@ int sil_magnetostatics3d_nonlinear_create(sil_operator* out, int64_t nx, int64_t
                                           const double* mu_nodal, const double* ferromagnetic_nodal,
                                           const double* bh_b, const double* bh_h, int32_t nsamples,
                                           double gauge_penalty);
+
+/* The 3D operator's switches (magnetostatics_onelab.hpp:796-828, read there
+ * from the environment at every application, and the 3D right-hand side's
+ * criterion, :4930-4935):
+ *   y_response_sign  −1 flips the sign of the y moment's response (<0 → −1,
+ *                    anything else → +1);
+ *   divergence_gauge nonzero: the penalty term P·w^A·A is replaced by the
+ *                    divergence gauge factor·δᵀ W δ A (:1893-1941,
+ *                    2370-2392: δ the codifferential of the interior
+ *                    potential at an identity metric, W = ℓx ℓy ℓz per node,
+ *                    factor = P · gauge_sign);
+ *   gauge_sign       sign of that factor (<0 → −1, else +1);
+ *   criterion        0 PotentialTemporalDerivative (the right-hand side is
+ *                    zero), 1 MomentsTemporalDerivative (the default),
+ *                    2 both. */
+typedef struct sil_ms3d_switches {
+    double y_response_sign;
+    int32_t divergence_gauge;
+    double gauge_sign;
+    int32_t criterion;
+} sil_ms3d_switches;
+
+/* the defaults, then SIMILIE_MAGNETIC_Y_RESPONSE_SIGN_3D,
+ * SIMILIE_DIVERGENCE_GAUGE_3D and SIMILIE_VECTOR_POTENTIAL_GAUGE_SIGN_3D
+ * parsed as the reference parses them (magnetostatics_onelab.hpp:796-828:
+ * a sign is +1 unless the variable holds a number < 0; the gauge is on when
+ * the variable is set, non-empty and does not start with '0') */
+int sil_ms3d_switches_from_environment(sil_ms3d_switches* out);
+
+/* sets the switches of a linear or nonlinear 3D magnetostatics operator
+ * (every later application, diagonal, assembly, right-hand side and solve);
+ * drops the operator's cached solver state */
+int sil_magnetostatics3d_set_switches(sil_operator op, const sil_ms3d_switches* switches);
+
+/* Air-gap diagnostics of a solution (magnetostatics_onelab.hpp:4666-4720 2D,
+ * 5246-5307 3D, 460-519): from the post-processing's cell fields (cell_b,
+ * cell_h: device [cells][3], cells x fastest) and a device mask per cell
+ * (bit 0: in the diagnostic region, bit 1: a positively oriented
+ * conductor).  Upper air-gap faces are the +y faces of region cells whose +y
+ * neighbour is outside the region (measure dx·dz, 2D dx·length_z); flux =
+ * Σ measure·B_y, traction T·n = (B_x H_y, B_y H_y − ½B·H, B_z H_y) summed
+ * with its magnitude, replaced by the x-mirror projection (y, z parts of
+ * pairs at ±x) when the faces are mirror-symmetric; current = Σ dx·dy·J_z
+ * over conductor cells (3D: the layer k = ncz/2).  Sums in the reference's
+ * cell order.  Synchronises `stream`. */
+typedef struct sil_air_gap_result {
+    int64_t faces;
+    int64_t current_cells;
+    double surface_measure;
+    double flux;
+    double traction[3];
+    double traction_magnitude;
+    double current;
+} sil_air_gap_result;
+
+int sil_magnetostatics3d_air_gap_diagnostics(sil_operator op, const double* cell_b, const double* cell_h,
+                                             const uint8_t* cell_mask, const double* cell_current,
+                                             sil_air_gap_result* out, void* stream);
+/* 2D: cell_current_z is [cells]; z_value is the z of every face (the
+ * reference's grid z) */
+int sil_magnetostatics2d_air_gap_diagnostics(sil_operator op, const double* cell_b, const double* cell_h,
+                                             const uint8_t* cell_mask, const double* cell_current_z,
+                                             double length_z, double z_value, sil_air_gap_result* out,
+                                             void* stream);
 
 /* apply_jacobian (magnetostatics_onelab.hpp:2918-3113): out = J(state)·in,
  * the derivative of the nonlinear operator at the potential `state`. */
@@ -275,6 +349,24 @@ int sil_poisson2d_create(sil_operator* out, int64_t nx, int64_t ny, const double
  * use_matrix_free = false path.  Synchronises `stream`. */
 int sil_magnetostatics3d_assemble(sil_operator op, sil_operator* csr_out, void* stream);
 
+/* assemble_matrix_data(model, state) (onelab_interface/magnetostatics_onelab
+ * .hpp:3115-3536): the Jacobian of a nonlinear magnetostatics operator at
+ * `state` (a potential), assembled on the device as above — the matrix the
+ * reference builds at every Newton step to generate its preconditioner
+ * (minimize_strong_formulation_residual.hpp:1432-1433).  Rows reach at most
+ * 23 columns.  Synchronises `stream`. */
+int sil_magnetostatics3d_assemble_jacobian(sil_operator op, const double* state, sil_operator* csr_out, void* stream);
+
+/* assemble_matrix_data(model) / assemble_matrix_data(model, state) of the 2D
+ * operator (onelab_interface/magnetostatics_onelab.hpp:3554-3748) + the
+ * CSR conversion (minimize_strong_formulation_residual.hpp:565-646): the
+ * linear 2D magnetostatics operator, or the Jacobian of a nonlinear one at
+ * `state`, assembled on the device into a new operator holding a CSR matrix
+ * as above (boundary rows: 1 on the diagonal; interior rows at most 7
+ * columns).  Synchronises `stream`. */
+int sil_magnetostatics2d_assemble(sil_operator op, sil_operator* csr_out, void* stream);
+int sil_magnetostatics2d_assemble_jacobian(sil_operator op, const double* state, sil_operator* csr_out, void* stream);
+
 /* gko::matrix::Csr<double, int32>::create(exec, size, values, columns,
  * row_ptrs) (minimize_strong_formulation_residual.hpp:613-620): an operator
  * over a caller's square CSR matrix (host or device arrays, copied).  Rows
@@ -294,8 +386,8 @@ int sil_csr_copy(sil_operator op, int32_t* row_ptr, int32_t* cols, double* vals,
  * .hpp:963-967): the factor G of M = Gᵀ G ≈ A⁻¹ as a new assembled operator
  * (pattern of tril(A); row i solves A[J,J] y = e with J its stored columns ≤ i
  * and is scaled by 1/√y_i).  `a` is an assembled operator or the
- * magnetostatics operator (assembled for the purpose, as the reference
- * does).  Rows may hold at most 32 entries in their lower triangle.
+ * linear 3D or 2D magnetostatics operator (assembled for the purpose, as
+ * the reference does).  Rows may hold at most 32 entries in their lower triangle.
  * Synchronises `stream`. */
 int sil_spd_isai_create(sil_operator a, sil_operator* g_out, void* stream);
 
@@ -514,6 +606,12 @@ int sil_unet_destroy(sil_unet net);
 int sil_unet_get_params(sil_unet net, float* params, void* stream);
 int sil_unet_set_params(sil_unet net, const float* params, void* stream);
 int sil_unet_forward(sil_unet net, int32_t batch, const float* input, float* output, void* stream);
+/* the stored BF16 activation `which` of the preceding sil_unet_forward, as
+ * FP32 [batch][H_l][W_l][32] (verification of single layers): 0 a0, 1 e0,
+ * 2 a1, 3 e1, 4 a2, 5 e2, 6 a3, 7 b3, 8 c2, 9 f2, 10 c1, 11 f1, 12 c0 — the
+ * outputs of conv_in and layers 1..12 in order; levels 0 0 1 1 2 2 3 3 2 2 1
+ * 1 0 (H_l = H >> l) */
+int sil_unet_activation(sil_unet net, int32_t which, int32_t batch, float* output, void* stream);
 /* parameter gradient of <dout, forward(input)>; uses the activations of the
  * preceding sil_unet_forward on the same input */
 int sil_unet_backward(sil_unet net, int32_t batch, const float* input, const float* dout, float* grad,

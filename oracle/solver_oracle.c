@@ -44,6 +44,149 @@ static int is_boundary(const ms_grid* g, long i, long j, long k)
     return i == 0 || j == 0 || k == 0 || i + 1 == g->nx || j + 1 == g->ny || k + 1 == g->nz;
 }
 
+/* The 3D operator's switches (magnetostatics_onelab.hpp:796-828, read there
+ * from SIMILIE_MAGNETIC_Y_RESPONSE_SIGN_3D, SIMILIE_DIVERGENCE_GAUGE_3D and
+ * SIMILIE_VECTOR_POTENTIAL_GAUGE_SIGN_3D): the sign of the y moment's
+ * response, and the divergence gauge factor·δᵀ W δ (factor = penalty ·
+ * gauge sign) in place of the penalty term P·w^A·x.  Oracle state shared by
+ * every 3D function below; set with orc_ms_set_switches. */
+static struct {
+    double y_sign;
+    int div_gauge;
+    double gauge_sign;
+} g_sw = {1.0, 0, 1.0};
+
+void orc_ms_set_switches(double y_sign, int div_gauge, double gauge_sign)
+{
+    g_sw.y_sign = y_sign < 0.0 ? -1.0 : 1.0;
+    g_sw.div_gauge = div_gauge != 0;
+    g_sw.gauge_sign = gauge_sign < 0.0 ? -1.0 : 1.0;
+}
+
+/* edge of the Hodge star's cell along an axis at node i: the forward edge,
+ * the backward one at the last node (dec_oracle.c's ⋆ restatement) */
+static double cell_edge(const double* c, long n, long i)
+{
+    if (n < 2) return 1.0;
+    return i + 1 < n ? c[i + 1] - c[i] : c[i] - c[i - 1];
+}
+
+/* row n = (i,j,k) of the divergence gauge's codifferential
+ * (magnetostatics_onelab.hpp:1893-1941: Codifferential::value of
+ * codifferential.hpp:191-310 at an identity metric, interior potential DOFs
+ * only).  On a rectilinear grid it reduces to the backward difference
+ *   δA(n) = Σ_c [A_c(n−e_c) / (e_c(n) e_c(n−e_c)) − A_c(n) / e_c(n)²]
+ * (e = cell_edge; pinned against dec_oracle.c's codifferential).  Entries in
+ * the stencil's order: (n−e_x, x), (n−e_y, y), (n−e_z, z), (n, x), (n, y),
+ * (n, z); returns their number. */
+static int ms_codiff_row(const ms_grid* g, long i, long j, long k, long* cols, double* coefs)
+{
+    const long idx[3] = {i, j, k}, ext[3] = {g->nx, g->ny, g->nz};
+    const double* ax[3] = {g->x, g->y, g->z};
+    double e[3];
+    for (int d = 0; d < 3; ++d) e[d] = cell_edge(ax[d], ext[d], idx[d]);
+    int cnt = 0;
+    for (int c = 0; c < 3; ++c) {
+        long q[3] = {i, j, k};
+        q[c] -= 1;
+        if (q[c] < 0 || is_boundary(g, q[0], q[1], q[2])) continue;
+        cols[cnt] = 3 * (q[0] + g->nx * (q[1] + g->ny * q[2])) + c;
+        coefs[cnt] = 1.0 / (e[c] * cell_edge(ax[c], ext[c], q[c]));
+        ++cnt;
+    }
+    if (!is_boundary(g, i, j, k))
+        for (int c = 0; c < 3; ++c) {
+            cols[cnt] = 3 * (i + g->nx * (j + g->ny * k)) + c;
+            coefs[cnt] = -1.0 / (e[c] * e[c]);
+            ++cnt;
+        }
+    return cnt;
+}
+
+/* the rows of the codifferential holding DOF (m, c) of an interior node m:
+ * n = m (coefficient −1/e_c(m)²) and n = m + e_c (1/(e_c(m+e_c) e_c(m))),
+ * ascending */
+static int ms_codiff_slots(const ms_grid* g, long i, long j, long k, int c, long* rows, double* coefs)
+{
+    const long idx[3] = {i, j, k}, ext[3] = {g->nx, g->ny, g->nz};
+    const double* ax[3] = {g->x, g->y, g->z};
+    const double ec = cell_edge(ax[c], ext[c], idx[c]);
+    long q[3] = {i, j, k};
+    q[c] += 1;
+    rows[0] = i + g->nx * (j + g->ny * k);
+    coefs[0] = -1.0 / (ec * ec);
+    rows[1] = q[0] + g->nx * (q[1] + g->ny * q[2]);
+    coefs[1] = 1.0 / (cell_edge(ax[c], ext[c], q[c]) * ec);
+    return 2;
+}
+
+/* δA at every node through ms_codiff_row (for the pinning tests) */
+void orc_ms_codifferential(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
+                           const double* in, double* out)
+{
+    ms_grid g = {nx, ny, nz, xc, yc, zc};
+    for (long n = 0; n < nx * ny * nz; ++n) {
+        long cols[6];
+        double cf[6];
+        int cnt = ms_codiff_row(&g, n % nx, (n / nx) % ny, n / (nx * ny), cols, cf);
+        double v = 0.0;
+        for (int t = 0; t < cnt; ++t) v += cf[t] * in[cols[t]];
+        out[n] = v;
+    }
+}
+
+/* codifferential weight of node n: ℓx ℓy ℓz (:1896-1898) */
+static double ms_codiff_weight(const ms_grid* g, long n)
+{
+    long i = n % g->nx, j = (n / g->nx) % g->ny, k = n / (g->nx * g->ny);
+    return local_spacing(g->x, g->nx, i) * local_spacing(g->y, g->ny, j) * local_spacing(g->z, g->nz, k);
+}
+
+/* the gauge term of every interior row, added to out (:2359-2392): P·w^A·in,
+ * or with the divergence gauge Σ_slots factor·w(n)·coef·(δin)(n) */
+static void ms_add_gauge(const ms_grid* g, double penalty, const double* in, double* out)
+{
+    long nx = g->nx, ny = g->ny, nz = g->nz, nn = nx * ny * nz;
+    double* dv = NULL;
+    if (g_sw.div_gauge) {
+        dv = (double*)malloc(sizeof(double) * (nn > 0 ? nn : 1));
+        for (long n = 0; n < nn; ++n) {
+            long cols[6];
+            double cf[6];
+            int cnt = ms_codiff_row(g, n % nx, (n / nx) % ny, n / (nx * ny), cols, cf);
+            double v = 0.0;
+            for (int t = 0; t < cnt; ++t) v += cf[t] * in[cols[t]];
+            dv[n] = v;
+        }
+    }
+    const double factor = penalty * g_sw.gauge_sign;
+    for (long k = 1; k + 1 < nz; ++k)
+        for (long j = 1; j + 1 < ny; ++j)
+            for (long i = 1; i + 1 < nx; ++i) {
+                double dx = local_spacing(g->x, nx, i), dy = local_spacing(g->y, ny, j), dz = local_spacing(g->z, nz, k);
+                double wa[3] = {dy * dz / dx, dx * dz / dy, dx * dy / dz};
+                for (int c = 0; c < 3; ++c) {
+                    long r = dof(g, i, j, k, c);
+                    if (!g_sw.div_gauge) {
+                        out[r] += penalty * wa[c] * in[r];
+                        continue;
+                    }
+                    long rows[2];
+                    double cf[2];
+                    int ns = ms_codiff_slots(g, i, j, k, c, rows, cf);
+                    double gr = 0.0;
+                    for (int t = 0; t < ns; ++t) gr += factor * ms_codiff_weight(g, rows[t]) * cf[t] * dv[rows[t]];
+                    out[r] += gr;
+                }
+            }
+    free(dv);
+}
+
+/* the gauge entries of interior row (i,j,k,c) for the assembly (:3204-3240),
+ * in slot order then the codifferential row's stencil order */
+static void ms_gauge_entries(const ms_grid* g, long i, long j, long k, int c, double penalty, int* count, int* lc,
+                             double* lv);
+
 /* The moment (magnetic induction) stencils of forward_value
  * (src/similie/physics/magnetostatics/magnetostatics_quantities.hpp:173-196):
  * B_x = (dA)_yz, B_y = -(dA)_xz, B_z = (dA)_xy, each a forward difference of
@@ -94,7 +237,7 @@ void orc_ms_apply(long nx, long ny, long nz, const double* xc, const double* yc,
                     B[m] = b;
                 }
                 for (int m = 0; m < 3; ++m) {
-                    double h = wb[m] * nu * B[m] * (m == 1 ? y_sign : 1.0);
+                    double h = wb[m] * nu * B[m] * (m == 1 ? y_sign * g_sw.y_sign : 1.0);
                     for (int t = 0; t < 4; ++t) {
                         const ms_tap* tp = &MOMENT[m][t];
                         long pi = i + tp->di, pj = j + tp->dj, pk = k + tp->dk;
@@ -105,15 +248,10 @@ void orc_ms_apply(long nx, long ny, long nz, const double* xc, const double* yc,
             }
     for (long k = 0; k < nz; ++k)
         for (long j = 0; j < ny; ++j)
-            for (long i = 0; i < nx; ++i) {
-                if (is_boundary(&g, i, j, k)) {
+            for (long i = 0; i < nx; ++i)
+                if (is_boundary(&g, i, j, k))
                     for (int c = 0; c < 3; ++c) out[dof(&g, i, j, k, c)] = in[dof(&g, i, j, k, c)];
-                    continue;
-                }
-                double dx = local_spacing(xc, nx, i), dy = local_spacing(yc, ny, j), dz = local_spacing(zc, nz, k);
-                double wa[3] = {dy * dz / dx, dx * dz / dy, dx * dy / dz};
-                for (int c = 0; c < 3; ++c) out[dof(&g, i, j, k, c)] += penalty * wa[c] * in[dof(&g, i, j, k, c)];
-            }
+    ms_add_gauge(&g, penalty, in, out);
 }
 
 /* right-hand side: ⅛·|dual face|·J over the ≤8 cells around an interior
@@ -219,6 +357,29 @@ static void add_entry(int* count, int* lc, double* lv, long col, double value)
     ++*count;
 }
 
+static void ms_gauge_entries(const ms_grid* g, long i, long j, long k, int c, double penalty, int* count, int* lc,
+                             double* lv)
+{
+    if (!g_sw.div_gauge) {
+        double dx = local_spacing(g->x, g->nx, i), dy = local_spacing(g->y, g->ny, j), dz = local_spacing(g->z, g->nz, k);
+        double wa = c == 0 ? dy * dz / dx : (c == 1 ? dx * dz / dy : dx * dy / dz);
+        add_entry(count, lc, lv, dof(g, i, j, k, c), penalty * wa);
+        return;
+    }
+    const double factor = penalty * g_sw.gauge_sign;
+    long rows[2];
+    double rcf[2];
+    int ns = ms_codiff_slots(g, i, j, k, c, rows, rcf);
+    for (int t = 0; t < ns; ++t) {
+        long n = rows[t];
+        long cols[6];
+        double cf[6];
+        int cnt = ms_codiff_row(g, n % g->nx, (n / g->nx) % g->ny, n / (g->nx * g->ny), cols, cf);
+        double w = ms_codiff_weight(g, n);
+        for (int q = 0; q < cnt; ++q) add_entry(count, lc, lv, cols[q], factor * w * rcf[t] * cf[q]);
+    }
+}
+
 /* assemble_matrix_data (onelab_interface/magnetostatics_onelab.hpp:3115-3536)
  * for the linear equations (∂H/∂B = δ_ab/μ): the transposed moment slots of
  * every potential DOF are collected in sampled-node order (the precompute of
@@ -266,9 +427,7 @@ long orc_ms_assemble_csr(long nx, long ny, long nz, const double* xc, const doub
             continue;
         }
         int count = 0;
-        double dx = local_spacing(xc, nx, i), dy = local_spacing(yc, ny, j), dz = local_spacing(zc, nz, k);
-        double wa = c == 0 ? dy * dz / dx : (c == 1 ? dx * dz / dy : dx * dy / dz);
-        add_entry(&count, lc, lv, r, penalty * wa);
+        ms_gauge_entries(&g, i, j, k, c, penalty, &count, lc, lv);
         for (int sidx = 0; sidx < tcount[r]; ++sidx) {
             long mrow = tslot[4 * r + sidx].moment_row;
             double rc = tslot[4 * r + sidx].coeff;
@@ -276,7 +435,8 @@ long orc_ms_assemble_csr(long nx, long ny, long nz, const double* xc, const doub
             int m = (int)(mrow % 3);
             long si = s % nx, sj = (s / nx) % ny, sk = s / (nx * ny);
             double sx = local_spacing(xc, nx, si), sy = local_spacing(yc, ny, sj), sz = local_spacing(zc, nz, sk);
-            double hf = m == 0 ? sx / (sy * sz) : (m == 1 ? y_sign * (sy / (sx * sz)) : sz / (sx * sy));
+            double ys = y_sign * g_sw.y_sign;
+            double hf = m == 0 ? sx / (sy * sz) : (m == 1 ? ys * (sy / (sx * sz)) : sz / (sx * sy));
             double jc = hf * (1.0 / mu[s]);
             if (jc == 0.0) continue;
             for (int t = 0; t < 4; ++t) {
@@ -424,11 +584,25 @@ void orc_poisson2d_apply(long nx, long ny, const double* hx, const double* hy, c
 
 typedef void (*orc_matvec_fn)(void* ctx, const double* in, double* out);
 
+/* fixed blocks summed in order: the result does not depend on the thread
+ * count or on the order in which threads finish (an OpenMP reduction
+ * combines its partial sums in an unspecified order) */
+#define ORC_DOT_BLOCK 4096
 static double dot(long n, const double* a, const double* b)
 {
+    long nb = (n + ORC_DOT_BLOCK - 1) / ORC_DOT_BLOCK;
+    double stack_part[64];
+    double* part = nb <= 64 ? stack_part : (double*)malloc(sizeof(double) * nb);
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < nb; ++q) {
+        long e = (q + 1) * ORC_DOT_BLOCK < n ? (q + 1) * ORC_DOT_BLOCK : n;
+        double s = 0.0;
+        for (long i = q * ORC_DOT_BLOCK; i < e; ++i) s += a[i] * b[i];
+        part[q] = s;
+    }
     double s = 0.0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-    for (long i = 0; i < n; ++i) s += a[i] * b[i];
+    for (long q = 0; q < nb; ++q) s += part[q];
+    if (part != stack_part) free(part);
     return s;
 }
 
@@ -706,7 +880,7 @@ static void nl_operator(long nx, long ny, long nz, const double* xc, const doubl
                     nl_response(c, fe, mu[s], Bin, H, NULL);
                 }
                 for (int m = 0; m < 3; ++m) {
-                    double hm = wb[m] * H[m];
+                    double hm = wb[m] * H[m] * (m == 1 ? g_sw.y_sign : 1.0);
                     for (int t = 0; t < 4; ++t) {
                         const ms_tap* tp = &MOMENT[m][t];
                         long pi = i + tp->di, pj = j + tp->dj, pk = k + tp->dk;
@@ -717,15 +891,10 @@ static void nl_operator(long nx, long ny, long nz, const double* xc, const doubl
             }
     for (long k = 0; k < nz; ++k)
         for (long j = 0; j < ny; ++j)
-            for (long i = 0; i < nx; ++i) {
-                if (is_boundary(&g, i, j, k)) {
+            for (long i = 0; i < nx; ++i)
+                if (is_boundary(&g, i, j, k))
                     for (int cc = 0; cc < 3; ++cc) out[dof(&g, i, j, k, cc)] = in[dof(&g, i, j, k, cc)];
-                    continue;
-                }
-                double dx = local_spacing(xc, nx, i), dy = local_spacing(yc, ny, j), dz = local_spacing(zc, nz, k);
-                double wa[3] = {dy * dz / dx, dx * dz / dy, dx * dy / dz};
-                for (int cc = 0; cc < 3; ++cc) out[dof(&g, i, j, k, cc)] += penalty * wa[cc] * in[dof(&g, i, j, k, cc)];
-            }
+    ms_add_gauge(&g, penalty, in, out);
 }
 
 void orc_nl_apply(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc, const double* mu,
@@ -744,6 +913,111 @@ void orc_nl_jacobian_apply(long nx, long ny, long nz, const double* xc, const do
     orc_bh c;
     bh_init(&c, bh_b, bh_h, nbh);
     nl_operator(nx, ny, nz, xc, yc, zc, mu, ferro, &c, penalty, state, in, out);
+}
+
+/* assemble_matrix_data(model, state) (onelab_interface/magnetostatics_onelab
+ * .hpp:3115-3536) of the Jacobian J(state): as orc_ms_assemble_csr, each
+ * transposed moment slot (moment row 3s+m, coefficient c_r) now contributes,
+ * for every moment component m2 whose coefficient w^B_m(s)·∂²H/∂B_m∂B_m2 at
+ * the state's induction is nonzero, c_r·(that coefficient)·c_t for every
+ * interior tap t of the moment stencil of m2 at s (:3243-3320).  Rows come
+ * out with ascending columns; cols/vals must hold 27·3·nx·ny·nz entries;
+ * returns the number of stored entries, or -1 when a row exceeds
+ * ORC_ROW_MAX. */
+long orc_nl_jacobian_assemble_csr(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc,
+                                  const double* mu, const double* ferro, const double* bh_b, const double* bh_h,
+                                  int nbh, double penalty, const double* state, long* row_ptr, int* cols,
+                                  double* vals)
+{
+    orc_bh cv;
+    bh_init(&cv, bh_b, bh_h, nbh);
+    ms_grid g = {nx, ny, nz, xc, yc, zc};
+    long ndof = 3 * nx * ny * nz, nn = nx * ny * nz;
+    int* tcount = (int*)calloc(ndof, sizeof(int));
+    orc_tslot* tslot = (orc_tslot*)malloc(sizeof(orc_tslot) * 4 * (ndof > 0 ? ndof : 1));
+    double* Jn = (double*)calloc(9 * (nn > 0 ? nn : 1), sizeof(double)); /* ∂²H/∂B∂B per interior node */
+    for (long k = 1; k + 1 < nz; ++k)
+        for (long j = 1; j + 1 < ny; ++j)
+            for (long i = 1; i + 1 < nx; ++i) {
+                long sn = i + nx * (j + ny * k);
+                double B0[3];
+                moments_at(&g, i, j, k, state, B0);
+                nl_response(&cv, ferro[sn] > 0.5, mu[sn], B0, NULL, Jn + 9 * sn);
+                for (int m = 0; m < 3; ++m)
+                    for (int t = 0; t < 4; ++t) {
+                        const ms_tap* tp = &MOMENT[m][t];
+                        long pi = i + tp->di, pj = j + tp->dj, pk = k + tp->dk;
+                        if (is_boundary(&g, pi, pj, pk)) continue;
+                        long r = dof(&g, pi, pj, pk, tp->comp);
+                        orc_tslot* sl = &tslot[4 * r + tcount[r]++];
+                        sl->moment_row = 3 * sn + m;
+                        sl->coeff = tp->coeff;
+                    }
+            }
+    long nnz = 0;
+    int lc[ORC_ROW_MAX];
+    double lv[ORC_ROW_MAX];
+    for (long r = 0; r < ndof; ++r) {
+        long node = r / 3;
+        int c = (int)(r % 3);
+        long i = node % nx, j = (node / nx) % ny, k = node / (nx * ny);
+        row_ptr[r] = nnz;
+        if (is_boundary(&g, i, j, k)) {
+            cols[nnz] = (int)r;
+            vals[nnz] = 1.0;
+            ++nnz;
+            continue;
+        }
+        int count = 0;
+        ms_gauge_entries(&g, i, j, k, c, penalty, &count, lc, lv);
+        for (int sidx = 0; sidx < tcount[r]; ++sidx) {
+            long mrow = tslot[4 * r + sidx].moment_row;
+            double rc = tslot[4 * r + sidx].coeff;
+            long sn = mrow / 3;
+            int m = (int)(mrow % 3);
+            long si = sn % nx, sj = (sn / nx) % ny, sk = sn / (nx * ny);
+            double sx = local_spacing(xc, nx, si), sy = local_spacing(yc, ny, sj), sz = local_spacing(zc, nz, sk);
+            double hf = m == 0 ? sx / (sy * sz) : (m == 1 ? g_sw.y_sign * (sy / (sx * sz)) : sz / (sx * sy));
+            for (int m2 = 0; m2 < 3; ++m2) {
+                double jc = hf * Jn[9 * sn + 3 * m + m2];
+                if (jc == 0.0) continue;
+                for (int t = 0; t < 4; ++t) {
+                    const ms_tap* tp = &MOMENT[m2][t];
+                    long pi = si + tp->di, pj = sj + tp->dj, pk = sk + tp->dk;
+                    if (is_boundary(&g, pi, pj, pk)) continue;
+                    if (count >= ORC_ROW_MAX) {
+                        free(tcount);
+                        free(tslot);
+                        free(Jn);
+                        return -1;
+                    }
+                    add_entry(&count, lc, lv, dof(&g, pi, pj, pk, tp->comp), rc * jc * tp->coeff);
+                }
+            }
+        }
+        for (int a = 1; a < count; ++a) {
+            int kc = lc[a];
+            double kv = lv[a];
+            int b = a - 1;
+            while (b >= 0 && lc[b] > kc) {
+                lc[b + 1] = lc[b];
+                lv[b + 1] = lv[b];
+                --b;
+            }
+            lc[b + 1] = kc;
+            lv[b + 1] = kv;
+        }
+        for (int a = 0; a < count; ++a) {
+            cols[nnz] = lc[a];
+            vals[nnz] = lv[a];
+            ++nnz;
+        }
+    }
+    row_ptr[ndof] = nnz;
+    free(tcount);
+    free(tslot);
+    free(Jn);
+    return nnz;
 }
 
 /* Jacobi diagonal of J(state), probed column by column in 27 colour
@@ -793,7 +1067,76 @@ typedef struct nl_problem {
     void (*J)(const struct nl_problem* P, const double* state, const double* in, double* out);
     void (*D)(const struct nl_problem* P, const double* state, double* diag);
     const void* data;
+    /* optional: J(state) assembled (ascending columns; returns nnz, or < 0) */
+    long (*A)(const struct nl_problem* P, const double* state, long* row_ptr, int* cols, double* vals);
 } nl_problem;
+
+/* textbook preconditioned CG as orc_cg with z = M r supplied by `M` */
+typedef void (*orc_prec_fn)(void* ctx, const double* r, double* z);
+
+static int orc_pcg(long n, orc_matvec_fn A, void* ctx, orc_prec_fn M, void* mctx, const double* b, double tol,
+                   int max_it, double* x, double* res_out)
+{
+    double* r = (double*)malloc(sizeof(double) * n);
+    double* z = (double*)malloc(sizeof(double) * n);
+    double* p = (double*)malloc(sizeof(double) * n);
+    double* q = (double*)malloc(sizeof(double) * n);
+    for (long i = 0; i < n; ++i) {
+        x[i] = 0.0;
+        r[i] = b[i];
+    }
+    M(mctx, r, z);
+    for (long i = 0; i < n; ++i) p[i] = z[i];
+    double bnorm = sqrt(dot(n, b, b));
+    double rz = dot(n, r, z);
+    double rnorm = bnorm;
+    int it = 0;
+    while (bnorm > 0.0 && it < max_it && rnorm > tol * bnorm) {
+        A(ctx, p, q);
+        double pq = dot(n, p, q);
+        double alpha = rz / pq;
+        for (long i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+        }
+        M(mctx, r, z);
+        double rz_new = dot(n, r, z);
+        rnorm = sqrt(dot(n, r, r));
+        double beta = rz_new / rz;
+        rz = rz_new;
+        for (long i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        ++it;
+    }
+    res_out[0] = rnorm;
+    res_out[1] = bnorm;
+    free(r);
+    free(z);
+    free(p);
+    free(q);
+    return it;
+}
+
+/* z = Gᵀ (G r) with the SpdIsai factor G (CSR) */
+typedef struct {
+    long n;
+    const long* rp;
+    const int* cols;
+    const double* vals;
+    double* t;
+} isai_ctx;
+
+static void isai_apply(void* c, const double* r, double* z)
+{
+    isai_ctx* k = (isai_ctx*)c;
+    for (long i = 0; i < k->n; ++i) {
+        double v = 0.0;
+        for (long q = k->rp[i]; q < k->rp[i + 1]; ++q) v += k->vals[q] * r[k->cols[q]];
+        k->t[i] = v;
+    }
+    for (long i = 0; i < k->n; ++i) z[i] = 0.0;
+    for (long i = 0; i < k->n; ++i)
+        for (long q = k->rp[i]; q < k->rp[i + 1]; ++q) z[k->cols[q]] += k->vals[q] * k->t[i];
+}
 
 typedef struct {
     const nl_problem* P;
@@ -815,7 +1158,7 @@ static int newton_core(const nl_problem* P, const double* b, double tol, int max
     double* delta = (double*)malloc(sizeof(double) * n);
     double* xc_ = (double*)malloc(sizeof(double) * n);
     double* rc = (double*)malloc(sizeof(double) * n);
-    double* diag = use_jacobi ? (double*)malloc(sizeof(double) * n) : NULL;
+    double* diag = use_jacobi == 1 ? (double*)malloc(sizeof(double) * n) : NULL;
     for (long i = 0; i < n; ++i) x[i] = 0.0;
     P->F(P, x, fx);
     for (long i = 0; i < n; ++i) r[i] = b[i] - fx[i];
@@ -829,9 +1172,35 @@ static int newton_core(const nl_problem* P, const double* b, double tol, int max
             break;
         }
         nl_jctx ctx = {P, x};
-        if (diag) P->D(P, x, diag);
         double res[2];
-        inner += orc_cg(n, nl_problem_jmv, &ctx, r, diag, tol, max_inner, delta, res);
+        if (use_jacobi == 2) {
+            /* SpdIsai of J(x), assembled at every Newton step
+             * (minimize_strong_formulation_residual.hpp:1432-1433) */
+            long* arp = (long*)malloc(sizeof(long) * (n + 1));
+            int* acol = (int*)malloc(sizeof(int) * 27 * n);
+            double* aval = (double*)malloc(sizeof(double) * 27 * n);
+            long* grp = (long*)malloc(sizeof(long) * (n + 1));
+            int* gcol = (int*)malloc(sizeof(int) * 27 * n);
+            double* gval = (double*)malloc(sizeof(double) * 27 * n);
+            long annz = P->A(P, x, arp, acol, aval);
+            long grc = annz >= 0 ? orc_spd_isai(n, arp, acol, aval, grp, gcol, gval) : -1;
+            isai_ctx ic = {n, grp, gcol, gval, (double*)malloc(sizeof(double) * n)};
+            if (grc == 0)
+                inner += orc_pcg(n, nl_problem_jmv, &ctx, isai_apply, &ic, r, tol, max_inner, delta, res);
+            else
+                for (long i = 0; i < n; ++i) delta[i] = 0.0;
+            free(ic.t);
+            free(arp);
+            free(acol);
+            free(aval);
+            free(grp);
+            free(gcol);
+            free(gval);
+            if (grc != 0) break;
+        } else {
+            if (diag) P->D(P, x, diag);
+            inner += orc_cg(n, nl_problem_jmv, &ctx, r, diag, tol, max_inner, delta, res);
+        }
         double alpha = 1.0, best_alpha = 0.0, best = INFINITY;
         for (int ls = 0; ls < 8; ++ls) {
             for (long i = 0; i < n; ++i) xc_[i] = x[i] + alpha * delta[i];
@@ -899,6 +1268,15 @@ static void nl3_D(const nl_problem* P, const double* state, double* diag)
                              d->penalty, state, diag);
 }
 
+static long nl3_A(const nl_problem* P, const double* state, long* row_ptr, int* cols, double* vals)
+{
+    const nl3_data* d = (const nl3_data*)P->data;
+    return orc_nl_jacobian_assemble_csr(d->nx, d->ny, d->nz, d->x, d->y, d->z, d->mu, d->ferro, d->bh_b, d->bh_h,
+                                        d->nbh, d->penalty, state, row_ptr, cols, vals);
+}
+
+/* use_jacobi: 0 none, 1 Jacobi, 2 SpdIsai of the assembled Jacobian at
+ * every step */
 int orc_nl_newton(long nx, long ny, long nz, const double* xc, const double* yc, const double* zc, const double* mu,
                   const double* ferro, const double* bh_b, const double* bh_h, int nbh, double penalty,
                   const double* b, double tol, int max_inner, int use_jacobi, double* x, double* out)
@@ -906,7 +1284,7 @@ int orc_nl_newton(long nx, long ny, long nz, const double* xc, const double* yc,
     orc_bh c;
     bh_init(&c, bh_b, bh_h, nbh);
     nl3_data d = {nx, ny, nz, xc, yc, zc, mu, ferro, bh_b, bh_h, nbh, &c, penalty};
-    nl_problem P = {3 * nx * ny * nz, nl3_F, nl3_J, nl3_D, &d};
+    nl_problem P = {3 * nx * ny * nz, nl3_F, nl3_J, nl3_D, &d, nl3_A};
     return newton_core(&P, b, tol, max_inner, use_jacobi, x, out);
 }
 
@@ -1228,6 +1606,128 @@ void orc_ms2d_apply(long nx, long ny, const double* mu, const double* ferro, con
     ms2_operator(&M, state, in, out);
 }
 
+/* assemble_matrix_data(model, state) of the 2D operator
+ * (onelab_interface/magnetostatics_onelab.hpp:3554-3748), restated: a
+ * boundary row holds 1 on its diagonal; an interior row gathers, for every
+ * transposed slot (potential criterion: moment x slots, then moment y
+ * slots) and then every outer slot (moments criterion, with the negated
+ * coefficient), the block of its sampled node s — at the state's induction
+ * B0(s), h = ∂²H/∂B∂B(s) (1/μ on the diagonal for the linear Hamiltonian) —
+ * rc·h_{x,m}·c_k over the moment-x stencil and rc·h_{y,m}·c_k over the
+ * moment-y stencil of s, where m is the slot's moment; a nonzero value is
+ * added to its column (summed with an earlier one in the row, or
+ * appended).  Columns whose sum is zero are dropped and the rest sorted
+ * (matrix_data → CSR).  ferro = NULL: the linear Hamiltonian.  cols/vals
+ * hold 8 entries per row. */
+static long ms2_assemble(const ms2_model* M, const double* state, long* row_ptr, int* cols, double* vals)
+{
+    const long nx = M->nx, ny = M->ny;
+    const double* mu = M->mu;
+    const double* ferro = M->ferro;
+    const int criterion = M->criterion;
+    long nnz = 0;
+    for (long row = 0; row < nx * ny; ++row) {
+        row_ptr[row] = nnz;
+        if (!ms2_interior(nx, ny, row)) {
+            cols[nnz] = (int)row;
+            vals[nnz] = 1.0;
+            ++nnz;
+            continue;
+        }
+        ms2_row_tables t;
+        ms2_row(nx, ny, row, &t);
+        int count = 0;
+        long lc[8];
+        double lv[8];
+        #define MS2_ADD(col_, val_)                                  \
+            do {                                                     \
+                int found_ = 0;                                      \
+                for (int q_ = 0; q_ < count; ++q_)                   \
+                    if (lc[q_] == (col_)) {                          \
+                        lv[q_] += (val_);                            \
+                        found_ = 1;                                  \
+                        break;                                       \
+                    }                                                \
+                if (!found_) {                                       \
+                    lc[count] = (col_);                              \
+                    lv[count] = (val_);                              \
+                    ++count;                                         \
+                }                                                    \
+            } while (0)
+        for (int part = 0; part < 2; ++part) {
+            if (part == 0 && !(criterion == 0 || criterion == 2)) continue;
+            if (part == 1 && !(criterion == 1 || criterion == 2)) continue;
+            for (int m = 0; m < 2; ++m) {
+                int cnt = part == 0 ? t.tcount[m] : t.ocount[m];
+                for (int k = 0; k < cnt; ++k) {
+                    long sn = part == 0 ? t.tcol[m][k] : t.ocol[m][k];
+                    double rc = part == 0 ? t.tcoef[m][k] : -t.ocoef[m][k];
+                    double B[3] = {0.0, 0.0, 0.0}, J[9];
+                    if (state) ms2_moments(nx, sn, state, B);
+                    if (ferro) {
+                        nl_response(M->c, ferro[sn] > 0.5, mu[sn], B, NULL, J);
+                    } else {
+                        for (int q = 0; q < 9; ++q) J[q] = 0.0;
+                        J[0] = J[4] = 1.0 / mu[sn];
+                    }
+                    /* h_{x,m} (moment-x stencil) and h_{y,m} (moment-y stencil) */
+                    double hx = m == 0 ? J[0] : J[3], hy = m == 0 ? J[1] : J[4];
+                    long col[2];
+                    double coef[2];
+                    ms2_stencil(nx, sn, 0, col, coef);
+                    for (int q = 0; q < 2; ++q) {
+                        double v = rc * (hx * coef[q]);
+                        if (v != 0.0) MS2_ADD(col[q], v);
+                    }
+                    ms2_stencil(nx, sn, 1, col, coef);
+                    for (int q = 0; q < 2; ++q) {
+                        double v = rc * (hy * coef[q]);
+                        if (v != 0.0) MS2_ADD(col[q], v);
+                    }
+                }
+            }
+        }
+        #undef MS2_ADD
+        /* drop zero sums, ascending columns */
+        int kept = 0;
+        for (int q = 0; q < count; ++q)
+            if (lv[q] != 0.0) {
+                lc[kept] = lc[q];
+                lv[kept] = lv[q];
+                ++kept;
+            }
+        for (int a = 1; a < kept; ++a) {
+            long kc = lc[a];
+            double kv = lv[a];
+            int b = a - 1;
+            while (b >= 0 && lc[b] > kc) {
+                lc[b + 1] = lc[b];
+                lv[b + 1] = lv[b];
+                --b;
+            }
+            lc[b + 1] = kc;
+            lv[b + 1] = kv;
+        }
+        for (int q = 0; q < kept; ++q) {
+            cols[nnz] = (int)lc[q];
+            vals[nnz] = lv[q];
+            ++nnz;
+        }
+    }
+    row_ptr[nx * ny] = nnz;
+    return nnz;
+}
+
+long orc_ms2d_assemble_csr(long nx, long ny, const double* mu, const double* ferro, const double* bh_b,
+                           const double* bh_h, int nbh, int criterion, const double* state, long* row_ptr, int* cols,
+                           double* vals)
+{
+    orc_bh cv;
+    if (ferro) bh_init(&cv, bh_b, bh_h, nbh);
+    ms2_model M = {nx, ny, mu, ferro, ferro ? &cv : NULL, criterion};
+    return ms2_assemble(&M, state, row_ptr, cols, vals);
+}
+
 /* the diagonal of ms2_operator(M, state, ·), probed with unit vectors in
  * 3×3 colour classes (a row reaches columns at most one node away) */
 static void ms2_probe_diagonal(const ms2_model* M, const double* state, double* diag)
@@ -1316,7 +1816,13 @@ static void ms2_D(const nl_problem* P, const double* state, double* diag)
     ms2_probe_diagonal((const ms2_model*)P->data, state, diag);
 }
 
-/* Newton on the 2D nonlinear operator, as orc_nl_newton */
+static long ms2_A(const nl_problem* P, const double* state, long* row_ptr, int* cols, double* vals)
+{
+    return ms2_assemble((const ms2_model*)P->data, state, row_ptr, cols, vals);
+}
+
+/* Newton on the 2D nonlinear operator, as orc_nl_newton (use_jacobi 2:
+ * SpdIsai of the Jacobian assembled at every step) */
 int orc_ms2d_newton(long nx, long ny, const double* mu, const double* ferro, const double* bh_b, const double* bh_h,
                     int nbh, int criterion, const double* b, double tol, int max_inner, int use_jacobi, double* x,
                     double* out)
@@ -1324,6 +1830,6 @@ int orc_ms2d_newton(long nx, long ny, const double* mu, const double* ferro, con
     orc_bh c;
     bh_init(&c, bh_b, bh_h, nbh);
     ms2_model M = {nx, ny, mu, ferro, &c, criterion};
-    nl_problem P = {nx * ny, ms2_F, ms2_J, ms2_D, &M};
+    nl_problem P = {nx * ny, ms2_F, ms2_J, ms2_D, &M, ms2_A};
     return newton_core(&P, b, tol, max_inner, use_jacobi, x, out);
 }

@@ -149,7 +149,50 @@ def conv_bwd32(x, w, dz):
     return tuple(a.astype(np.float32).astype(np.float64) for a in (dx, dw, db))
 
 
-def forward(params, x, emulate=False):
+# The same FP32 convolutions with every sum taken in another order: input
+# channels permuted (the matrix products sum them differently), taps in
+# reverse, pixels mirrored for the weight gradients.  Mathematically equal to
+# conv32 / conv_bwd32; the difference between the two is what the order of
+# FP32 accumulation alone does to a network with BF16 storage, the yardstick
+# for how far a GPU kernel that sums in its own order may land from the
+# emulating oracle.
+_PERM = np.random.default_rng(12345).permutation(C)
+
+
+def _perm(n):
+    return _PERM if n == C else np.arange(n)[::-1]
+
+
+def conv32_reordered(x, w, b):
+    x32, w32 = np.float32(x), np.float32(w)
+    pi = _perm(x.shape[-1])
+    x32, w32 = x32[..., pi], w32[:, :, pi]
+    B, H, W, _ = x.shape
+    xp = np.pad(x32, ((0, 0), (1, 1), (1, 1), (0, 0)))
+    y = np.zeros((B, H, W, w.shape[1]), dtype=np.float32)
+    for t in reversed(range(9)):
+        dy, dx = divmod(t, 3)
+        y += xp[:, dy:dy + H, dx:dx + W, :] @ w32[t].T
+    return (y + np.float32(b)).astype(np.float64)
+
+
+def conv_bwd32_reordered(x, w, dz):
+    x32, w32, dz32 = np.float32(x), np.float32(w), np.float32(dz)
+    po = _perm(dz.shape[-1])
+    B, H, W, _ = x.shape
+    xp = np.pad(x32, ((0, 0), (1, 1), (1, 1), (0, 0)))
+    dxp = np.zeros_like(xp)
+    dw = np.zeros_like(w32)
+    for t in reversed(range(9)):
+        dy, dx = divmod(t, 3)
+        dxp[:, dy:dy + H, dx:dx + W, :] += dz32[..., po] @ w32[t][po, :]
+        xs = xp[:, dy:dy + H, dx:dx + W, :][:, ::-1, ::-1, :]
+        dw[t] = np.einsum("bhwo,bhwc->oc", dz32[:, ::-1, ::-1, :], xs)
+    db = dz32[:, ::-1, ::-1, :].sum(axis=(0, 1, 2), dtype=np.float32)
+    return tuple(a.astype(np.float32).astype(np.float64) for a in (dxp[:, 1:-1, 1:-1, :], dw, db))
+
+
+def forward(params, x, emulate=False, reordered=False):
     """Network output [B,H,W] and the activations, x [B,H,W,2].
 
     emulate=True computes as the GPU kernels do: FP32 input and first/last
@@ -162,7 +205,7 @@ def forward(params, x, emulate=False):
     too can be compared tightly with the GPU."""
     q = bf16 if emulate else (lambda a: a)
     qw = f32 if emulate else (lambda a: a)
-    cv = conv32 if emulate else conv
+    cv = (conv32_reordered if reordered else conv32) if emulate else conv
     win, bin_, layers, wout, bout = unpack(params)
     act = {"x": qw(np.asarray(x, dtype=np.float64))}
     for op, src, dst in GRAPH:
@@ -180,12 +223,13 @@ def forward(params, x, emulate=False):
     return act["out"], act
 
 
-def backward(params, act, dout, emulate=False):
+def backward(params, act, dout, emulate=False, reordered=False):
     """Gradient of sum(dout * out) with respect to the parameter vector
-    (emulate=True: BF16 storage of the activation gradients, as on the GPU)."""
+    (emulate=True: BF16 storage of the activation gradients, as on the GPU;
+    reordered=True: every FP32 sum in another order)."""
     q = bf16 if emulate else (lambda a: a)
     qw = f32 if emulate else (lambda a: a)
-    cb = conv_bwd32 if emulate else conv_bwd
+    cb = (conv_bwd32_reordered if reordered else conv_bwd32) if emulate else conv_bwd
     win, bin_, layers, wout, bout = unpack(params)
     g = np.zeros(N_PARAMS)
     grad = {k: np.zeros_like(v) for k, v in act.items() if k != "out"}

@@ -65,6 +65,27 @@ class SolverDiagnostics(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Ms3dSwitches(ctypes.Structure):
+    _fields_ = [
+        ("y_response_sign", ctypes.c_double),
+        ("divergence_gauge", ctypes.c_int32),
+        ("gauge_sign", ctypes.c_double),
+        ("criterion", ctypes.c_int32),
+    ]
+
+
+class AirGapResult(ctypes.Structure):
+    _fields_ = [
+        ("faces", ctypes.c_int64),
+        ("current_cells", ctypes.c_int64),
+        ("surface_measure", ctypes.c_double),
+        ("flux", ctypes.c_double),
+        ("traction", ctypes.c_double * 3),
+        ("traction_magnitude", ctypes.c_double),
+        ("current", ctypes.c_double),
+    ]
+
+
 class SilproSettings(ctypes.Structure):
     _fields_ = [
         ("physics", ctypes.c_int32),
@@ -142,9 +163,18 @@ SIGNATURES = {
     "sil_make_staged_laplacian": (ctypes.c_int, [ctypes.POINTER(_P), _P, _I32, _VP, _VP]),
     "sil_staged_laplacian_run": (ctypes.c_int, [_P, _P, _P, _P]),
     "sil_staged_laplacian_destroy": (ctypes.c_int, [_P]),
+    "sil_make_staged_codifferential": (ctypes.c_int, [ctypes.POINTER(_P), _P, _I32, _VP, _VP]),
+    "sil_staged_codifferential_run": (ctypes.c_int, [_P, _P, _P, _P]),
+    "sil_staged_codifferential_destroy": (ctypes.c_int, [_P]),
     "sil_magnetostatics3d_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _I64, _P, _P, _P, _P, _D]),
     "sil_magnetostatics3d_nodal_mu": (ctypes.c_int, [_I64, _I64, _I64, _P, _D, _P, _P]),
     "sil_magnetostatics3d_rhs": (ctypes.c_int, [_P, _P, _P, _P]),
+    "sil_ms3d_switches_from_environment": (ctypes.c_int, [ctypes.POINTER(Ms3dSwitches)]),
+    "sil_magnetostatics3d_air_gap_diagnostics": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.POINTER(AirGapResult),
+                                                                 _P]),
+    "sil_magnetostatics2d_air_gap_diagnostics": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_double,
+                                                                 ctypes.c_double, ctypes.POINTER(AirGapResult), _P]),
+    "sil_magnetostatics3d_set_switches": (ctypes.c_int, [_P, ctypes.POINTER(Ms3dSwitches)]),
     "sil_magnetostatics3d_post_process": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "sil_magnetostatics2d_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _P, _P, _P, _I32]),
     "sil_magnetostatics2d_nonlinear_create": (
@@ -160,6 +190,9 @@ SIGNATURES = {
     "sil_magnetostatics3d_apply_jacobian": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "sil_magnetostatics3d_jacobian_diagonal": (ctypes.c_int, [_P, _P, _P, _P]),
     "sil_magnetostatics3d_assemble": (ctypes.c_int, [_P, ctypes.POINTER(_P), _P]),
+    "sil_magnetostatics3d_assemble_jacobian": (ctypes.c_int, [_P, _P, ctypes.POINTER(_P), _P]),
+    "sil_magnetostatics2d_assemble": (ctypes.c_int, [_P, ctypes.POINTER(_P), _P]),
+    "sil_magnetostatics2d_assemble_jacobian": (ctypes.c_int, [_P, _P, ctypes.POINTER(_P), _P]),
     "sil_csr_create": (ctypes.c_int, [ctypes.POINTER(_P), _I64, _I64, _P, _P, _P, _P]),
     "sil_csr_nnz": (_I64, [_P]),
     "sil_csr_copy": (ctypes.c_int, [_P, _P, _P, _P, _P]),
@@ -211,6 +244,7 @@ SIGNATURES = {
     "sil_unet_destroy": (ctypes.c_int, [_P]),
     "sil_unet_get_params": (ctypes.c_int, [_P, _P, _P]),
     "sil_unet_set_params": (ctypes.c_int, [_P, _P, _P]),
+    "sil_unet_activation": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
     "sil_unet_forward": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
     "sil_unet_backward": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P]),
     "sil_unet_loss": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),

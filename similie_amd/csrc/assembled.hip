@@ -69,12 +69,17 @@ struct RowPlan {
     int pair_m[4], pair_t[4]; // (moment, tap) pairs reaching the row, in slot order
     int npairs;
     int slot_of_key[81];      // -1: column not in the pattern
-    int key_of_slot[16];
+    int key_of_slot[32];
     int nslots;
     int diag_slot;
 };
 
-constexpr RowPlan make_plan(int C)
+// JAC: the row of the Jacobian at a state, whose response couples every
+// moment of a sampled node with the three moments there (∂²H/∂B_m∂B_m2):
+// each pair reaches the taps of all three moment stencils at its sampled node.
+// DIVG: the divergence gauge's columns — the codifferential rows n = m and
+// n = m + e_C each reach (n − e_d, d) and (n, d).
+constexpr RowPlan make_plan(int C, bool jac = false, bool divg = false)
 {
     RowPlan p{};
     int n = 0;
@@ -91,11 +96,21 @@ constexpr RowPlan make_plan(int C)
     used[key_of(0, 0, 0, C)] = true;
     for (int q = 0; q < n; ++q) {
         const Tap& t = kMoment[p.pair_m[q]][p.pair_t[q]];
-        for (int u = 0; u < 4; ++u) {
-            const Tap& v = kMoment[p.pair_m[q]][u];
-            used[key_of(v.di - t.di, v.dj - t.dj, v.dk - t.dk, v.comp)] = true;
+        for (int m2 = 0; m2 < 3; ++m2) {
+            if (!jac && m2 != p.pair_m[q]) continue;
+            for (int u = 0; u < 4; ++u) {
+                const Tap& v = kMoment[m2][u];
+                used[key_of(v.di - t.di, v.dj - t.dj, v.dk - t.dk, v.comp)] = true;
+            }
         }
     }
+    if (divg)
+        for (int o = 0; o < 2; ++o)
+            for (int d = 0; d < 3; ++d) {
+                const int ox = o && C == 0, oy = o && C == 1, oz = o && C == 2;
+                used[key_of(ox - (d == 0), oy - (d == 1), oz - (d == 2), d)] = true;
+                used[key_of(ox, oy, oz, d)] = true;
+            }
     int s = 0;
     for (int k = 0; k < 81; ++k) {
         p.slot_of_key[k] = used[k] ? s : -1;
@@ -106,17 +121,30 @@ constexpr RowPlan make_plan(int C)
     return p;
 }
 
-template <int C>
+template <int C, bool JAC = false, bool DIVG = false>
 struct Plan {
-    static constexpr RowPlan v = make_plan(C);
-    static_assert(v.npairs == 4 && v.nslots == 13, "curl-curl row pattern (13 entries at most)");
+    static constexpr RowPlan v = make_plan(C, JAC, DIVG);
+    static_assert(v.npairs == 4 && v.nslots == (JAC ? 23 : 13) + (DIVG ? 2 : 0),
+                  "curl-curl row pattern (13 entries at most, 23 for the Jacobian; the divergence gauge adds "
+                  "(m ± e_C, C))");
 };
+
+// largest row of a plan
+constexpr int row_max(bool jac, bool divg) { return (jac ? 23 : 13) + (divg ? 2 : 0); }
 
 struct MsAsm {
     int64_t nx, ny, nz;
     double penalty;
     const double* ls[3];
     const double* nu;
+    double ysign = 1.0;
+    // divergence gauge: factor and coordinates (cell edges)
+    double gauge_factor = 0.0;
+    const double* coord[3] = {nullptr, nullptr, nullptr};
+    // Jacobian at a state: ∂²H/∂B_m∂B_m2 = a δ_m,m2 + c B0_m B0_m2 per node
+    const double* st_a = nullptr;
+    const double* st_c = nullptr;
+    const double* st_b[3] = {nullptr, nullptr, nullptr};
 };
 
 __device__ __forceinline__ bool interior(const MsAsm& a, int64_t i, int64_t j, int64_t k)
@@ -126,26 +154,69 @@ __device__ __forceinline__ bool interior(const MsAsm& a, int64_t i, int64_t j, i
 
 // one interior row (component C at node (i,j,k)): COUNT writes the number of
 // stored entries, otherwise the row is written at col/val[rp[r] − base]
-template <int C, bool COUNT>
+__device__ __forceinline__ double asm_cell_edge(const double* c, int64_t n, int64_t i)
+{
+    return i + 1 < n ? c[i + 1] - c[i] : c[i] - c[i - 1];
+}
+
+template <int C, bool COUNT, bool JAC = false, bool DIVG = false>
 __device__ __forceinline__ void assemble_row(const MsAsm& a, int64_t i, int64_t j, int64_t k, int64_t* __restrict__ cnt,
                                              const int32_t* __restrict__ rp, int32_t* __restrict__ col,
                                              double* __restrict__ val, int32_t base)
 {
 #pragma clang fp contract(off)
-    constexpr RowPlan P = Plan<C>::v;
+    constexpr RowPlan P = Plan<C, JAC, DIVG>::v;
     const int64_t node = i + a.nx * (j + a.ny * k);
     const int64_t r = 3 * node + C;
     double acc[P.nslots];
 #pragma unroll
     for (int s = 0; s < P.nslots; ++s) acc[s] = 0.0;
     uint32_t present = 0;
-    {
+    if constexpr (!DIVG) {
         const double lx = a.ls[0][i], ly = a.ls[1][j], lz = a.ls[2][k];
         const double wa = C == 0 ? ly * lz / lx : (C == 1 ? lx * lz / ly : lx * ly / lz);
         const double g = a.penalty * wa;
         if (g != 0.0) {
             acc[P.diag_slot] += g;
             present |= 1u << P.diag_slot;
+        }
+    } else {
+        // the divergence gauge (:3204-3227): for the codifferential rows
+        // n = m (coefficient −1/e_C(m)²) and n = m + e_C (1/(e_C(n) e_C(m))),
+        // factor·w(n)·ρ·c over row n's entries (n − e_d, d) then (n, d)
+        const int64_t ext[3] = {a.nx, a.ny, a.nz}, idx[3] = {i, j, k};
+        const double ec = asm_cell_edge(a.coord[C], ext[C], idx[C]);
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            const int64_t n3[3] = {i + (o && C == 0), j + (o && C == 1), k + (o && C == 2)};
+            const double rho = o == 0 ? -1.0 / (ec * ec) : 1.0 / (asm_cell_edge(a.coord[C], ext[C], n3[C]) * ec);
+            const double w = a.ls[0][n3[0]] * a.ls[1][n3[1]] * a.ls[2][n3[2]];
+            double e[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) e[d] = asm_cell_edge(a.coord[d], ext[d], n3[d]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                int64_t q[3] = {n3[0], n3[1], n3[2]};
+                q[d] -= 1;
+                if (!interior(a, q[0], q[1], q[2])) continue;
+                const double cf = 1.0 / (e[d] * asm_cell_edge(a.coord[d], ext[d], q[d]));
+                const double v = a.gauge_factor * w * rho * cf;
+                if (v == 0.0) continue;
+                const int slot = P.slot_of_key[key_of((int)(q[0] - i), (int)(q[1] - j), (int)(q[2] - k), d)];
+                acc[slot] += v;
+                present |= 1u << slot;
+            }
+            if (interior(a, n3[0], n3[1], n3[2])) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double v = a.gauge_factor * w * rho * (-1.0 / (e[d] * e[d]));
+                    if (v == 0.0) continue;
+                    const int slot =
+                        P.slot_of_key[key_of((int)(n3[0] - i), (int)(n3[1] - j), (int)(n3[2] - k), d)];
+                    acc[slot] += v;
+                    present |= 1u << slot;
+                }
+            }
         }
     }
 #pragma unroll
@@ -155,17 +226,38 @@ __device__ __forceinline__ void assemble_row(const MsAsm& a, int64_t i, int64_t 
         const int64_t si = i - t.di, sj = j - t.dj, sk = k - t.dk;
         if (!interior(a, si, sj, sk)) continue;
         const double sx = a.ls[0][si], sy = a.ls[1][sj], sz = a.ls[2][sk];
-        const double hf = m == 0 ? sx / (sy * sz) : (m == 1 ? sy / (sx * sz) : sz / (sx * sy));
-        const double jc = hf * a.nu[si + a.nx * (sj + a.ny * sk)];
-        if (jc == 0.0) continue;
-        const double rjc = t.coeff * jc;
+        const double hf = m == 0 ? sx / (sy * sz) : (m == 1 ? a.ysign * (sy / (sx * sz)) : sz / (sx * sy));
+        const int64_t sn = si + a.nx * (sj + a.ny * sk);
+        if constexpr (!JAC) {
+            const double jc = hf * a.nu[sn];
+            if (jc == 0.0) continue;
+            const double rjc = t.coeff * jc;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const Tap v = kMoment[m][u];
-            if (!interior(a, si + v.di, sj + v.dj, sk + v.dk)) continue;
-            const int slot = P.slot_of_key[key_of(v.di - t.di, v.dj - t.dj, v.dk - t.dk, v.comp)];
-            acc[slot] += rjc * v.coeff;
-            present |= 1u << slot;
+            for (int u = 0; u < 4; ++u) {
+                const Tap v = kMoment[m][u];
+                if (!interior(a, si + v.di, sj + v.dj, sk + v.dk)) continue;
+                const int slot = P.slot_of_key[key_of(v.di - t.di, v.dj - t.dj, v.dk - t.dk, v.comp)];
+                acc[slot] += rjc * v.coeff;
+                present |= 1u << slot;
+            }
+        } else {
+            const double sa = a.st_a[sn], sc = a.st_c[sn];
+            const double b0[3] = {a.st_b[0][sn], a.st_b[1][sn], a.st_b[2][sn]};
+#pragma unroll
+            for (int m2 = 0; m2 < 3; ++m2) {
+                const double jmm = (m == m2 ? sa : 0.0) + sc * b0[m] * b0[m2];
+                const double jc = hf * jmm;
+                if (jc == 0.0) continue;
+                const double rjc = t.coeff * jc;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const Tap v = kMoment[m2][u];
+                    if (!interior(a, si + v.di, sj + v.dj, sk + v.dk)) continue;
+                    const int slot = P.slot_of_key[key_of(v.di - t.di, v.dj - t.dj, v.dk - t.dk, v.comp)];
+                    acc[slot] += rjc * v.coeff;
+                    present |= 1u << slot;
+                }
+            }
         }
     }
     if constexpr (COUNT) {
@@ -185,6 +277,7 @@ __device__ __forceinline__ void assemble_row(const MsAsm& a, int64_t i, int64_t 
 }
 
 // count pass: entries per row
+template <bool JAC, bool DIVG>
 __global__ __launch_bounds__(kAsmBlock) void k_ms3d_count(MsAsm a, int64_t* __restrict__ cnt)
 {
     const int64_t nn = a.nx * a.ny * a.nz;
@@ -195,9 +288,9 @@ __global__ __launch_bounds__(kAsmBlock) void k_ms3d_count(MsAsm a, int64_t* __re
             for (int c = 0; c < 3; ++c) cnt[3 * node + c] = 1;
             continue;
         }
-        assemble_row<0, true>(a, i, j, k, cnt, nullptr, nullptr, nullptr, 0);
-        assemble_row<1, true>(a, i, j, k, cnt, nullptr, nullptr, nullptr, 0);
-        assemble_row<2, true>(a, i, j, k, cnt, nullptr, nullptr, nullptr, 0);
+        assemble_row<0, true, JAC, DIVG>(a, i, j, k, cnt, nullptr, nullptr, nullptr, 0);
+        assemble_row<1, true, JAC, DIVG>(a, i, j, k, cnt, nullptr, nullptr, nullptr, 0);
+        assemble_row<2, true, JAC, DIVG>(a, i, j, k, cnt, nullptr, nullptr, nullptr, 0);
     }
 }
 
@@ -205,18 +298,20 @@ __global__ __launch_bounds__(kAsmBlock) void k_ms3d_count(MsAsm a, int64_t* __re
 // one contiguous stretch of the CSR arrays — into LDS, then writes the
 // stretch with consecutive lanes on consecutive entries (the per-thread rows
 // written directly are scattered 4- and 8-byte stores: partial-line writes)
-constexpr int kAsmRowMax = 13;
+template <bool JAC, bool DIVG>
 __global__ __launch_bounds__(kAsmBlock) void k_ms3d_fill(MsAsm a, const int32_t* __restrict__ rp,
                                                          int32_t* __restrict__ col, double* __restrict__ val)
 {
-    __shared__ int32_t s_col[kAsmBlock * 3 * kAsmRowMax];
-    __shared__ double s_val[kAsmBlock * 3 * kAsmRowMax];
+    constexpr int kAsmRowMax = row_max(JAC, DIVG);
+    constexpr int kNodes = JAC ? kAsmBlock / 2 : kAsmBlock; // nodes per pass (LDS)
+    __shared__ int32_t s_col[kNodes * 3 * kAsmRowMax];
+    __shared__ double s_val[kNodes * 3 * kAsmRowMax];
     const int64_t nn = a.nx * a.ny * a.nz;
-    for (int64_t node0 = (int64_t)blockIdx.x * kAsmBlock; node0 < nn; node0 += (int64_t)gridDim.x * kAsmBlock) {
-        const int64_t node_end = node0 + kAsmBlock < nn ? node0 + kAsmBlock : nn;
+    for (int64_t node0 = (int64_t)blockIdx.x * kNodes; node0 < nn; node0 += (int64_t)gridDim.x * kNodes) {
+        const int64_t node_end = node0 + kNodes < nn ? node0 + kNodes : nn;
         const int32_t beg = rp[3 * node0], len = rp[3 * node_end] - beg;
         const int64_t node = node0 + threadIdx.x;
-        if (node < node_end) {
+        if (threadIdx.x < kNodes && node < node_end) {
             const int64_t i = node % a.nx, j = (node / a.nx) % a.ny, k = node / (a.nx * a.ny);
             if (!interior(a, i, j, k)) {
                 for (int c = 0; c < 3; ++c) {
@@ -225,9 +320,9 @@ __global__ __launch_bounds__(kAsmBlock) void k_ms3d_fill(MsAsm a, const int32_t*
                     s_val[rp[r] - beg] = 1.0;
                 }
             } else {
-                assemble_row<0, false>(a, i, j, k, nullptr, rp, s_col, s_val, beg);
-                assemble_row<1, false>(a, i, j, k, nullptr, rp, s_col, s_val, beg);
-                assemble_row<2, false>(a, i, j, k, nullptr, rp, s_col, s_val, beg);
+                assemble_row<0, false, JAC, DIVG>(a, i, j, k, nullptr, rp, s_col, s_val, beg);
+                assemble_row<1, false, JAC, DIVG>(a, i, j, k, nullptr, rp, s_col, s_val, beg);
+                assemble_row<2, false, JAC, DIVG>(a, i, j, k, nullptr, rp, s_col, s_val, beg);
             }
         }
         __syncthreads();
@@ -596,18 +691,37 @@ void csr_free(CsrData& a)
     a = CsrData{};
 }
 
-int csr_assemble_ms3d(const Ms3dData& d, CsrData& out, hipStream_t s)
+namespace {
+// the linear operator (nl == null) or the Jacobian at the state of the last
+// evaluation of the nonlinear operator
+int assemble_ms3d(const Ms3dData& d, const NlData* nl, CsrData& out, hipStream_t s)
 {
     SIL_REQUIRE(3 * d.nx * d.ny * d.nz < INT32_MAX, SIL_ERR_INVALID,
                 "assemble: %lld rows exceed the int32 index range", (long long)(3 * d.nx * d.ny * d.nz));
     const int64_t nn = d.nx * d.ny * d.nz, n = 3 * nn;
     MsAsm a{d.nx, d.ny, d.nz, d.penalty, {d.ls[0], d.ls[1], d.ls[2]}, d.nu};
+    a.ysign = d.ysign;
+    a.gauge_factor = d.gauge_factor;
+    for (int q = 0; q < 3; ++q) a.coord[q] = d.coord[q];
+    const bool jac = nl != nullptr, divg = d.divg != 0;
+    if (jac) {
+        a.st_a = nl->st_a;
+        a.st_c = nl->st_c;
+        for (int q = 0; q < 3; ++q) a.st_b[q] = nl->st_b[q];
+    }
     int64_t* cnt = nullptr;
     SIL_CHECK_HIP(hipMalloc(&cnt, sizeof(int64_t) * n));
     CsrData m{};
     int rc = SIL_OK;
     const unsigned g = grid_for(nn, kAsmBlock);
-    hipLaunchKernelGGL(k_ms3d_count, dim3(g), dim3(kAsmBlock), 0, s, a, cnt);
+    if (jac && divg)
+        hipLaunchKernelGGL((k_ms3d_count<true, true>), dim3(g), dim3(kAsmBlock), 0, s, a, cnt);
+    else if (jac)
+        hipLaunchKernelGGL((k_ms3d_count<true, false>), dim3(g), dim3(kAsmBlock), 0, s, a, cnt);
+    else if (divg)
+        hipLaunchKernelGGL((k_ms3d_count<false, true>), dim3(g), dim3(kAsmBlock), 0, s, a, cnt);
+    else
+        hipLaunchKernelGGL((k_ms3d_count<false, false>), dim3(g), dim3(kAsmBlock), 0, s, a, cnt);
     if (hipGetLastError() != hipSuccess) {
         set_error("assemble: count kernel launch failed");
         rc = SIL_ERR_HIP;
@@ -629,7 +743,17 @@ int csr_assemble_ms3d(const Ms3dData& d, CsrData& out, hipStream_t s)
         }
     }
     if (rc == SIL_OK) {
-        hipLaunchKernelGGL(k_ms3d_fill, dim3(g), dim3(kAsmBlock), 0, s, a, m.row_ptr, m.col, m.val);
+        const dim3 gj(grid_for(nn, kAsmBlock / 2));
+        if (jac && divg)
+            hipLaunchKernelGGL((k_ms3d_fill<true, true>), gj, dim3(kAsmBlock), 0, s, a, m.row_ptr, m.col, m.val);
+        else if (jac)
+            hipLaunchKernelGGL((k_ms3d_fill<true, false>), gj, dim3(kAsmBlock), 0, s, a, m.row_ptr, m.col, m.val);
+        else if (divg)
+            hipLaunchKernelGGL((k_ms3d_fill<false, true>), dim3(g), dim3(kAsmBlock), 0, s, a, m.row_ptr, m.col,
+                               m.val);
+        else
+            hipLaunchKernelGGL((k_ms3d_fill<false, false>), dim3(g), dim3(kAsmBlock), 0, s, a, m.row_ptr, m.col,
+                               m.val);
         if (hipGetLastError() != hipSuccess) {
             set_error("assemble: fill kernel launch failed");
             rc = SIL_ERR_HIP;
@@ -642,6 +766,211 @@ int csr_assemble_ms3d(const Ms3dData& d, CsrData& out, hipStream_t s)
     choose_lanes(m);
     out = m;
     return SIL_OK;
+}
+}  // namespace
+
+namespace {
+// --------------------------------------------------------- 2D assembly ---
+//
+// assemble_matrix_data of MagnetostaticsOperator2D (magnetostatics_onelab
+// .hpp:3554-3748).  An interior row r gathers, for every transposed slot
+// (potential criterion: moment-x slots, then moment-y slots; sampled rows
+// ascending) and then every outer slot (moments criterion, coefficients
+// negated twice), the block of its sampled node s: with h = ∂²H/∂B∂B(s) at
+// the state's induction, rc·h_{x,m}·c over the moment-x stencil of s and
+// rc·h_{y,m}·c over its moment-y stencil (m the slot's moment), adding each
+// nonzero value to its column — summed with an earlier one in the row or
+// appended — in that order; zero sums are dropped and columns sorted.  The
+// slots of r are fixed by the stencils (B_x(s) = A(s+e_y) − A(s), B_y(s) =
+// A(s) − A(s+e_x)): moment x from s = r − e_y (+1) and s = r (−1), moment
+// y from s = r − e_x (−1) and s = r (+1), interior sampled nodes only.
+struct Ms2Asm {
+    int64_t nx, ny;
+    int criterion;
+    const double* nu;              // linear Hamiltonian (no state)
+    const NodeState<double>* st;   // Jacobian: (a, c, B0_x, B0_y) per node
+};
+
+__device__ __forceinline__ bool interior2(const Ms2Asm& a, int64_t s)
+{
+    const int64_t i = s % a.nx, j = s / a.nx;
+    return i > 0 && j > 0 && i + 1 < a.nx && j + 1 < a.ny;
+}
+
+// the stored entries of interior row r, ascending columns; returns the count
+template <bool JAC>
+__device__ __forceinline__ int ms2_row_entries(const Ms2Asm& a, int64_t r, int64_t col[8], double val[8])
+{
+#pragma clang fp contract(off)
+    int count = 0;
+    auto add = [&](int64_t c, double v) {
+        for (int q = 0; q < count; ++q)
+            if (col[q] == c) {
+                val[q] += v;
+                return;
+            }
+        col[count] = c;
+        val[count] = v;
+        ++count;
+    };
+    // sampled node and stencil coefficient of each slot, per moment
+    const int64_t sx[2] = {r - a.nx, r}, sy[2] = {r - 1, r};
+    const double cx[2] = {1.0, -1.0}, cy[2] = {-1.0, 1.0};
+    for (int part = 0; part < 2; ++part) {
+        if (part == 0 && !(a.criterion == 0 || a.criterion == 2)) continue;
+        if (part == 1 && !(a.criterion == 1 || a.criterion == 2)) continue;
+        for (int m = 0; m < 2; ++m) {
+            for (int k = 0; k < 2; ++k) {
+                const int64_t s = m == 0 ? sx[k] : sy[k];
+                if (s < 0 || !interior2(a, s)) continue;
+                // the outer slot holds −c and is applied with a minus sign
+                const double rc = part == 0 ? (m == 0 ? cx[k] : cy[k]) : -(-(m == 0 ? cx[k] : cy[k]));
+                double hx, hy; // h_{x,m}, h_{y,m}
+                if constexpr (JAC) {
+                    const NodeState<double> q = a.st[s];
+                    const double bb[2] = {q.bx, q.by};
+                    hx = (m == 0 ? q.a : 0.0) + q.c * bb[m] * bb[0];
+                    hy = (m == 1 ? q.a : 0.0) + q.c * bb[m] * bb[1];
+                } else {
+                    hx = m == 0 ? a.nu[s] : 0.0;
+                    hy = m == 1 ? a.nu[s] : 0.0;
+                }
+                // moment-x stencil of s: A(s) −1, A(s+e_y) +1
+                const double v0 = rc * (hx * -1.0), v1 = rc * (hx * 1.0);
+                if (v0 != 0.0) add(s, v0);
+                if (v1 != 0.0) add(s + a.nx, v1);
+                // moment-y stencil of s: A(s) +1, A(s+e_x) −1
+                const double w0 = rc * (hy * 1.0), w1 = rc * (hy * -1.0);
+                if (w0 != 0.0) add(s, w0);
+                if (w1 != 0.0) add(s + 1, w1);
+            }
+        }
+    }
+    int kept = 0;
+    for (int q = 0; q < count; ++q)
+        if (val[q] != 0.0) {
+            col[kept] = col[q];
+            val[kept] = val[q];
+            ++kept;
+        }
+    for (int p = 1; p < kept; ++p) {
+        const int64_t kc = col[p];
+        const double kv = val[p];
+        int b = p - 1;
+        while (b >= 0 && col[b] > kc) {
+            col[b + 1] = col[b];
+            val[b + 1] = val[b];
+            --b;
+        }
+        col[b + 1] = kc;
+        val[b + 1] = kv;
+    }
+    return kept;
+}
+
+template <bool JAC>
+__global__ __launch_bounds__(256) void k_ms2d_count(Ms2Asm a, int64_t* __restrict__ cnt)
+{
+    const int64_t n = a.nx * a.ny;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        if (!interior2(a, r)) {
+            cnt[r] = 1;
+            continue;
+        }
+        int64_t col[8];
+        double val[8];
+        cnt[r] = ms2_row_entries<JAC>(a, r, col, val);
+    }
+}
+
+template <bool JAC>
+__global__ __launch_bounds__(256) void k_ms2d_fill(Ms2Asm a, const int32_t* __restrict__ rp, int32_t* __restrict__ col,
+                                                   double* __restrict__ val)
+{
+    const int64_t n = a.nx * a.ny;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t o = rp[r];
+        if (!interior2(a, r)) {
+            col[o] = static_cast<int32_t>(r);
+            val[o] = 1.0;
+            continue;
+        }
+        int64_t c[8];
+        double v[8];
+        const int k = ms2_row_entries<JAC>(a, r, c, v);
+        for (int q = 0; q < k; ++q) {
+            col[o + q] = static_cast<int32_t>(c[q]);
+            val[o + q] = v[q];
+        }
+    }
+}
+
+int assemble_ms2d(const Ms2dData& d, const NlData* nl, CsrData& out, hipStream_t s)
+{
+    const int64_t n = d.nx * d.ny;
+    SIL_REQUIRE(n < INT32_MAX, SIL_ERR_INVALID, "assemble: %lld rows exceed the int32 index range", (long long)n);
+    Ms2Asm a{d.nx, d.ny, d.criterion, d.nu, nl ? reinterpret_cast<const NodeState<double>*>(nl->st_pack) : nullptr};
+    int64_t* cnt = nullptr;
+    SIL_CHECK_HIP(hipMalloc(&cnt, sizeof(int64_t) * std::max<int64_t>(n, 1)));
+    CsrData m{};
+    int rc = SIL_OK;
+    const unsigned g = grid_for(n, 256);
+    if (nl)
+        hipLaunchKernelGGL(k_ms2d_count<true>, dim3(g), dim3(256), 0, s, a, cnt);
+    else
+        hipLaunchKernelGGL(k_ms2d_count<false>, dim3(g), dim3(256), 0, s, a, cnt);
+    if (hipGetLastError() != hipSuccess) {
+        set_error("assemble 2D: count kernel launch failed");
+        rc = SIL_ERR_HIP;
+    }
+    int64_t nnz = 0;
+    if (rc == SIL_OK && hipMalloc(&m.row_ptr, sizeof(int32_t) * (n + 1)) != hipSuccess) {
+        set_error("assemble 2D: allocation failed");
+        rc = SIL_ERR_HIP;
+    }
+    if (rc == SIL_OK) rc = rowptr_from_counts(cnt, n, m.row_ptr, &nnz, s);
+    (void)hipFree(cnt);
+    if (rc == SIL_OK) {
+        m.n = n;
+        m.nnz = nnz;
+        if (hipMalloc(&m.col, sizeof(int32_t) * std::max<int64_t>(nnz, 1)) != hipSuccess ||
+            hipMalloc(&m.val, sizeof(double) * std::max<int64_t>(nnz, 1)) != hipSuccess) {
+            set_error("assemble 2D: allocation of %lld entries failed", (long long)nnz);
+            rc = SIL_ERR_HIP;
+        }
+    }
+    if (rc == SIL_OK) {
+        if (nl)
+            hipLaunchKernelGGL(k_ms2d_fill<true>, dim3(g), dim3(256), 0, s, a, m.row_ptr, m.col, m.val);
+        else
+            hipLaunchKernelGGL(k_ms2d_fill<false>, dim3(g), dim3(256), 0, s, a, m.row_ptr, m.col, m.val);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("assemble 2D: fill kernel launch failed");
+            rc = SIL_ERR_HIP;
+        }
+    }
+    if (rc != SIL_OK) {
+        csr_free(m);
+        return rc;
+    }
+    choose_lanes(m);
+    out = m;
+    return SIL_OK;
+}
+}  // namespace
+
+int csr_assemble_ms2d(const Ms2dData& d, CsrData& out, hipStream_t s) { return assemble_ms2d(d, nullptr, out, s); }
+
+int csr_assemble_ms2d_jacobian(const Ms2dData& d, const NlData& nl, CsrData& out, hipStream_t s)
+{
+    return assemble_ms2d(d, &nl, out, s);
+}
+
+int csr_assemble_ms3d(const Ms3dData& d, CsrData& out, hipStream_t s) { return assemble_ms3d(d, nullptr, out, s); }
+
+int csr_assemble_ms3d_jacobian(const Ms3dData& d, const NlData& nl, CsrData& out, hipStream_t s)
+{
+    return assemble_ms3d(d, &nl, out, s);
 }
 
 int csr_import(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* col, const double* val, CsrData& out,

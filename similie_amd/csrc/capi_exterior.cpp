@@ -257,6 +257,53 @@ int sil_codifferential(sil_grid grid, int32_t k, const double* in, const sil_vie
     return launch_codifferential(grid->geom, k, in, *in_view, out, *out_view, as_stream(stream));
 }
 
+}  // extern "C"
+
+struct sil_staged_codifferential_s {
+    sil_grid grid;
+    int k;
+    sil_view in_view, out_view;
+};
+
+extern "C" {
+
+int sil_make_staged_codifferential(sil_staged_codifferential* out, sil_grid grid, int32_t k, const sil_view* in_view,
+                                   const sil_view* out_view)
+{
+    SIL_REQUIRE(out != nullptr && grid != nullptr, SIL_ERR_INVALID, "codifferential: null argument");
+    int rc = check_form_args(k, grid->geom.ndim, (void*)1, in_view, (void*)1, out_view, "codifferential",
+                             OutDegree::Down);
+    if (rc) return rc;
+    SIL_REQUIRE(k >= 1, SIL_ERR_INVALID, "codifferential: degree must be at least 1");
+    if ((rc = check_grid_view(grid, in_view, "codifferential"))) return rc;
+    SIL_REQUIRE(out_view->nbatch == 0, SIL_ERR_UNSUPPORTED, "codifferential: spectator dimensions unsupported");
+    for (int a = 0; a < grid->geom.ndim; ++a)
+        SIL_REQUIRE(out_view->extent[a] <= grid->geom.ext[a], SIL_ERR_INVALID,
+                    "codifferential: output extent exceeds the grid on axis %d", a);
+    auto* op = new (std::nothrow) sil_staged_codifferential_s();
+    SIL_REQUIRE(op != nullptr, SIL_ERR_INVALID, "codifferential: out of host memory");
+    op->grid = grid;
+    op->k = k;
+    op->in_view = *in_view;
+    op->out_view = *out_view;
+    *out = op;
+    return SIL_OK;
+}
+
+int sil_staged_codifferential_run(sil_staged_codifferential op, double* out, const double* in, void* stream)
+{
+    SIL_REQUIRE(op != nullptr, SIL_ERR_INVALID, "codifferential: null staged operator");
+    if (view_nodes(op->out_view) == 0) return SIL_OK;
+    SIL_REQUIRE(in != nullptr && out != nullptr, SIL_ERR_INVALID, "codifferential: null array");
+    return launch_codifferential(op->grid->geom, op->k, in, op->in_view, out, op->out_view, as_stream(stream));
+}
+
+int sil_staged_codifferential_destroy(sil_staged_codifferential op)
+{
+    delete op;
+    return SIL_OK;
+}
+
 int sil_laplacian(sil_grid grid, int32_t k, const double* in, const sil_view* in_view, double* out,
                   const sil_view* out_view, void* stream)
 {
@@ -279,11 +326,13 @@ struct sil_staged_laplacian_s {
 };
 
 namespace sil {
-// Tile configuration of the fused 3D Laplacian; set by the autotuner or by
-// the environment (SIL_LAP3D_TILE="variant,ty,xchunk[,zw]").
+// Tile configuration of the fused 3D Laplacian; the environment overrides
+// it (SIL_LAP3D_TILE="variant,ty,xchunk[,zw or planes in flight]").
 FusedTile default_lap3d_tile()
 {
-    FusedTile t{FUSED_PIPE, 64, 12, 32, 1};
+    // staged march, 16-row tiles, two planes of loads in flight, 32 planes
+    // per workgroup, non-temporal output stores: the fastest measured shape
+    FusedTile t{FUSED_STAGE, 2, 16, 32, 1};
     if (const char* e = std::getenv("SIL_LAP3D_TILE")) {
         int v = 1, ty = 8, xc = 32, zw = 64;
         const int n = std::sscanf(e, "%d,%d,%d,%d", &v, &ty, &xc, &zw);

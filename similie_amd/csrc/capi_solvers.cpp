@@ -1,5 +1,6 @@
 // extern "C" entry points of the linear operators and the CG solve.
 #include <cmath>
+#include <cstdlib>
 #include <new>
 #include <vector>
 
@@ -250,6 +251,48 @@ int sil_magnetostatics3d_nonlinear_create(sil_operator* out, int64_t nx, int64_t
     return SIL_OK;
 }
 
+int sil_ms3d_switches_from_environment(sil_ms3d_switches* out)
+{
+    SIL_REQUIRE(out, SIL_ERR_INVALID, "ms3d_switches_from_environment: null argument");
+    auto sign_of = [](const char* name) {
+        const char* v = std::getenv(name);
+        if (v == nullptr || v[0] == '\0') return 1.0;
+        char* end = nullptr;
+        const double parsed = std::strtod(v, &end);
+        if (end == v || parsed == 0.0) return 1.0;
+        return parsed < 0.0 ? -1.0 : 1.0;
+    };
+    const char* g = std::getenv("SIMILIE_DIVERGENCE_GAUGE_3D");
+    out->y_response_sign = sign_of("SIMILIE_MAGNETIC_Y_RESPONSE_SIGN_3D");
+    out->divergence_gauge = (g != nullptr && g[0] != '\0' && g[0] != '0') ? 1 : 0;
+    out->gauge_sign = sign_of("SIMILIE_VECTOR_POTENTIAL_GAUGE_SIGN_3D");
+    out->criterion = 1;
+    return SIL_OK;
+}
+
+int sil_magnetostatics3d_set_switches(sil_operator op, const sil_ms3d_switches* sw)
+{
+    SIL_REQUIRE(op && sw, SIL_ERR_INVALID, "magnetostatics3d_set_switches: null argument");
+    SIL_REQUIRE(op->kind == OP_MAGNETOSTATICS3D || op->kind == OP_MS3D_NONLINEAR, SIL_ERR_INVALID,
+                "magnetostatics3d_set_switches: not a 3D magnetostatics operator");
+    SIL_REQUIRE(sw->criterion >= 0 && sw->criterion <= 2, SIL_ERR_INVALID,
+                "magnetostatics3d_set_switches: unknown criterion %d", sw->criterion);
+    Ms3dData& d = op->ms;
+    d.ysign = sw->y_response_sign < 0.0 ? -1.0 : 1.0;
+    d.divg = sw->divergence_gauge != 0 ? 1 : 0;
+    d.gauge_factor = d.penalty * (sw->gauge_sign < 0.0 ? -1.0 : 1.0);
+    d.criterion = sw->criterion;
+    // cached diagonals, factors and captured graphs hold the old operator
+    destroy_workspace(op->ws);
+    op->ws = nullptr;
+    if (op->jac) {
+        op->jac->ms = d;
+        destroy_workspace(op->jac->ws);
+        op->jac->ws = nullptr;
+    }
+    return SIL_OK;
+}
+
 int sil_magnetostatics3d_apply_jacobian(sil_operator op, const double* state, const double* in, double* out,
                                         void* stream)
 {
@@ -490,6 +533,59 @@ int sil_magnetostatics3d_assemble(sil_operator op, sil_operator* csr_out, void* 
     return wrap_csr(m, csr_out);
 }
 
+int sil_magnetostatics3d_assemble_jacobian(sil_operator op, const double* state, sil_operator* csr_out, void* stream)
+{
+    SIL_REQUIRE(op && state && csr_out, SIL_ERR_INVALID, "magnetostatics3d_assemble_jacobian: null argument");
+    SIL_REQUIRE(op->kind == OP_MS3D_NONLINEAR, SIL_ERR_INVALID,
+                "magnetostatics3d_assemble_jacobian: not a nonlinear magnetostatics operator");
+    hipStream_t s = as_stream(stream);
+    int rc = ms3d_state(op->ms, op->nl, state, s);
+    if (rc) return rc;
+    CsrData m{};
+    if ((rc = csr_assemble_ms3d_jacobian(op->ms, op->nl, m, s))) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        csr_free(m);
+        set_error("magnetostatics3d_assemble_jacobian: assembly kernels failed");
+        return SIL_ERR_HIP;
+    }
+    return wrap_csr(m, csr_out);
+}
+
+int sil_magnetostatics2d_assemble(sil_operator op, sil_operator* csr_out, void* stream)
+{
+    SIL_REQUIRE(op && csr_out, SIL_ERR_INVALID, "magnetostatics2d_assemble: null argument");
+    SIL_REQUIRE(op->kind == OP_MAGNETOSTATICS2D, SIL_ERR_INVALID,
+                "magnetostatics2d_assemble: not a linear 2D magnetostatics operator");
+    hipStream_t s = as_stream(stream);
+    CsrData m{};
+    int rc = csr_assemble_ms2d(op->m2, m, s);
+    if (rc) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        csr_free(m);
+        set_error("magnetostatics2d_assemble: assembly kernels failed");
+        return SIL_ERR_HIP;
+    }
+    return wrap_csr(m, csr_out);
+}
+
+int sil_magnetostatics2d_assemble_jacobian(sil_operator op, const double* state, sil_operator* csr_out, void* stream)
+{
+    SIL_REQUIRE(op && state && csr_out, SIL_ERR_INVALID, "magnetostatics2d_assemble_jacobian: null argument");
+    SIL_REQUIRE(op->kind == OP_MS2D_NONLINEAR, SIL_ERR_INVALID,
+                "magnetostatics2d_assemble_jacobian: not a nonlinear 2D magnetostatics operator");
+    hipStream_t s = as_stream(stream);
+    int rc = ms2d_state(op->m2, op->nl, state, s);
+    if (rc) return rc;
+    CsrData m{};
+    if ((rc = csr_assemble_ms2d_jacobian(op->m2, op->nl, m, s))) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        csr_free(m);
+        set_error("magnetostatics2d_assemble_jacobian: assembly kernels failed");
+        return SIL_ERR_HIP;
+    }
+    return wrap_csr(m, csr_out);
+}
+
 int sil_csr_create(sil_operator* out, int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* cols,
                    const double* vals, void* stream)
 {
@@ -520,14 +616,17 @@ int sil_csr_copy(sil_operator op, int32_t* row_ptr, int32_t* cols, double* vals,
 int sil_spd_isai_create(sil_operator a, sil_operator* g_out, void* stream)
 {
     SIL_REQUIRE(a && g_out, SIL_ERR_INVALID, "spd_isai_create: null argument");
-    SIL_REQUIRE(a->kind == OP_CSR || a->kind == OP_MAGNETOSTATICS3D, SIL_ERR_INVALID,
-                "spd_isai_create: needs an assembled operator or the magnetostatics operator");
+    SIL_REQUIRE(a->kind == OP_CSR || a->kind == OP_MAGNETOSTATICS3D || a->kind == OP_MAGNETOSTATICS2D,
+                SIL_ERR_INVALID, "spd_isai_create: needs an assembled operator or a linear magnetostatics operator");
     hipStream_t s = as_stream(stream);
     CsrData assembled{};
     const CsrData* m = &a->csr;
     int rc;
     if (a->kind == OP_MAGNETOSTATICS3D) {
         if ((rc = csr_assemble_ms3d(a->ms, assembled, s))) return rc;
+        m = &assembled;
+    } else if (a->kind == OP_MAGNETOSTATICS2D) {
+        if ((rc = csr_assemble_ms2d(a->m2, assembled, s))) return rc;
         m = &assembled;
     }
     CsrData g{}, gt{};
@@ -603,8 +702,7 @@ int sil_minimize_strong_formulation_residual(sil_operator op, const double* rhs,
                 SIL_ERR_INVALID, "unknown preconditioner %d", settings->preconditioner);
     *diagnostics = sil_solver_diagnostics{};
     if (op->kind == OP_MS3D_NONLINEAR || op->kind == OP_MS2D_NONLINEAR) {
-        SIL_REQUIRE(settings->preconditioner != SIL_PRECOND_SPD_ISAI, SIL_ERR_UNSUPPORTED,
-                    "nonlinear solve: identity or Jacobi preconditioning");
+        // SpdIsai: generated from the Jacobian assembled at every Newton step
         return newton_solve(op, rhs, solution, *settings, *diagnostics, as_stream(stream));
     }
     return cg_solve(op, rhs, solution, *settings, *diagnostics, as_stream(stream));

@@ -29,6 +29,7 @@
 // resources so that no branch separates a load from its consumer: the
 // compiler's wait counts then let a step's loads stay in flight across the
 // whole step instead of being drained at its top.
+#include "buffer_io.hpp"
 #include "dec_kernels.hpp"
 
 #include <utility>
@@ -346,25 +347,6 @@ __global__ __launch_bounds__(64 * TY) void k_lap3d_march(const double* __restric
 
 constexpr int kMaxXChunk = 256;
 
-typedef unsigned int U32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int U32x4 __attribute__((ext_vector_type(4)));
-
-// Raw buffer resources (stride 0, gfx9 data format word): a load at an offset
-// at or beyond the resource's byte range returns zero and a store there is
-// dropped, so lanes and planes outside the grid are masked without a branch.
-constexpr uint32_t kOutside = 0x80000000u;
-constexpr int64_t kMaxBufferBytes = 0x7FFFFFFFLL;
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* base, int64_t bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
-}
-
-__device__ __forceinline__ double as_double(unsigned lo, unsigned hi)
-{
-    return __builtin_bit_cast(double, static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32));
-}
-
 // three consecutive doubles at byte offset off (16 + 8 byte requests)
 template <int AUX = 0>
 __device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, uint32_t off, double v[3])
@@ -626,6 +608,243 @@ __global__ __launch_bounds__(ZW * TY) void k_lap3d_pipe(const double* __restrict
     });
 }
 
+// ----------------------------------- 3D 1-form, staged stores and DPP z ---
+
+// values of the neighbouring lanes of a 64-lane wavefront through DPP wave
+// shifts: from_up(v) is lane l+1's v (lane 63 keeps its own), from_down(v)
+// lane l−1's v (lane 0 reads zero)
+__device__ __forceinline__ double from_up(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = static_cast<int>(static_cast<uint32_t>(u)), hi = static_cast<int>(static_cast<uint32_t>(u >> 32));
+    const int l2 = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xF, 0xF, false); // wave_shl:1
+    const int h2 = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, static_cast<uint64_t>(static_cast<uint32_t>(l2)) |
+                                          (static_cast<uint64_t>(static_cast<uint32_t>(h2)) << 32));
+}
+
+__device__ __forceinline__ double from_down(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = static_cast<int>(static_cast<uint32_t>(u)), hi = static_cast<int>(static_cast<uint32_t>(u >> 32));
+    const int l2 = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, false); // wave_shr:1
+    const int h2 = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, static_cast<uint64_t>(static_cast<uint32_t>(l2)) |
+                                          (static_cast<uint64_t>(static_cast<uint32_t>(h2)) << 32));
+}
+
+// LDS ordering of a wavefront's own accesses around a cross-lane exchange
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int TY>
+struct StageLds {
+    // per row: ω_x(p), ω_z(p) (y+1 neighbours of dω) and ⋆ω_XZ(p+1) (y−1
+    // neighbour of δω); after the step's second barrier the row's 192
+    // doubles are free and stage the row's output for contiguous stores
+    double wep[TY][3][64];
+    double eta[2][TY][64]; // ⋆dω(p) components 0 and 2: y−1 neighbours
+    double dw[2][TY][64];  // δω ping-pong: planes p and p+1 (y+1 neighbour)
+    double hx[kMaxXChunk + 4];
+};
+
+// The march of k_lap3d_pipe (64-lane rows, one wavefront each) with the z
+// neighbours exchanged inside the wavefront by DPP wave shifts — LDS only
+// carries the y neighbours — and every row's output written as contiguous
+// 8-byte words: the row's 62 output nodes (186 doubles) are staged in LDS in
+// node order and stored by three instructions that each cover 512
+// consecutive bytes, instead of 24-byte records at a 24-byte lane stride.
+template <int TY, bool DIAG, int D, int SPOL = 0>
+__global__ __launch_bounds__(64 * TY) void k_lap3d_stage(const double* __restrict__ w, const double* __restrict__ g,
+                                                         const double* __restrict__ hxs, const double* __restrict__ hys,
+                                                         const double* __restrict__ hzs, double* __restrict__ out,
+                                                         int64_t Nx, int64_t Ny, int64_t Nz, int64_t Ox, int64_t Oy,
+                                                         int64_t Oz, int ntz, int nty, int nxc, int xchunk)
+{
+    constexpr int ZW = 64;
+    __shared__ StageLds<TY> s;
+    const int lane = threadIdx.x % ZW;
+    const int row = threadIdx.x / ZW;
+
+    const int nb = gridDim.x;
+    const int b = blockIdx.x;
+    const int per = (nb + 7) / 8;
+    const int rem = nb % 8;
+    int logical;
+    {
+        const int xcd = b % 8, slot = b / 8;
+        const int full = rem == 0 ? 8 : rem;
+        logical = (xcd < full) ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
+    }
+    const int tz = logical % ntz;
+    const int ty = (logical / ntz) % nty;
+    const int tx = logical / (ntz * nty);
+    if (tx >= nxc) return; // uniform per workgroup
+
+    const int64_t z0 = (int64_t)tz * (ZW - 2);
+    const int64_t z = z0 + lane;
+    const int64_t y = (int64_t)ty * (TY - 2) + row;
+    const int64_t xa = (int64_t)tx * xchunk;
+    const int64_t xb = min(xa + xchunk, Ox);
+    const bool pos_in = (z < Nz && y < Ny);
+    const bool wy = (row >= 1 || y == 0) && (row <= TY - 2 || y == Ny - 1) && y < Oy;
+    const bool has_y1 = (y + 1 < Ny), has_z1 = (z + 1 < Nz);
+    const double hy = pos_in ? hys[y] : 0.0;
+    const double hz = pos_in ? hzs[z] : 0.0;
+    const int64_t x0 = xa - 1;
+
+    const int64_t qa = max(x0, (int64_t)0), qe = min(xb + 1, Nx);
+    const int64_t plane_bytes = Ny * Nz * 24;
+    const __amdgpu_buffer_rsrc_t rw = buffer_of(w + qa * Ny * Nz * 3, (qe - qa) * plane_bytes);
+    const __amdgpu_buffer_rsrc_t rg = buffer_of(DIAG ? g + qa * Ny * Nz * 3 : w, DIAG ? (qe - qa) * plane_bytes : 0);
+    const __amdgpu_buffer_rsrc_t ro = buffer_of(out + xa * Oy * Oz * 3, (xb - xa) * Oy * Oz * 24);
+    const uint32_t yz_bytes = pos_in ? static_cast<uint32_t>((y * Nz + z) * 24) : 0u;
+    const uint32_t oplane_bytes = static_cast<uint32_t>(Oy * Oz * 24);
+    // the staged output row: doubles j = 64 k + lane of the row's segment
+    // (node z0 + j/3), stored when that node is an output of this tile
+    uint32_t seg_off[3];
+    bool seg_st[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int j = 64 * k + lane;
+        const int nl = j / 3;
+        const int64_t zn = z0 + nl;
+        seg_st[k] = wy && (nl >= 1 || zn == 0) && (nl <= ZW - 2 || zn == Nz - 1) && zn < Oz;
+        seg_off[k] = seg_st[k] ? static_cast<uint32_t>((y * Oz + z0) * 24 + 8 * j) : 0u;
+    }
+
+    for (int i = threadIdx.x; i < (int)(xb - xa) + 4; i += ZW * TY) {
+        const int64_t q = x0 + i;
+        s.hx[i] = (q >= 0 && q < qe) ? hxs[q] : 0.0;
+    }
+
+    auto offset = [&](int64_t q) {
+        const bool ok = pos_in && q >= qa && q < qe;
+        return ok ? static_cast<uint32_t>((q - qa) * plane_bytes) + yz_bytes : kOutside;
+    };
+    auto load = [&](int64_t q, double W[3], double G[3]) {
+        const uint32_t off = offset(q);
+        load3(rw, off, W);
+        if (DIAG) {
+            load3(rg, off, G);
+        } else {
+            G[0] = G[1] = G[2] = 1.0;
+        }
+    };
+    auto coefs_of = [&](const double G[3], int64_t q) { return node_coefs(G[0], G[1], G[2], s.hx[q - x0], hy, hz); };
+
+    double Wc[3], Wn[3], Gc[3], Gn[3], WF[D][3], GF[D][3];
+    int64_t p = x0;
+    load(p, Wc, Gc);
+    load(p + 1, Wn, Gn);
+    unrolled<D - 1>([&](auto J) { load(p + 2 + J(), WF[J() + 1], GF[J() + 1]); });
+    __syncthreads(); // s.hx
+    Coefs Kc = coefs_of(Gc, p);
+    Coefs Kn = coefs_of(Gn, p + 1);
+    double etaPrev[3] = {0.0, 0.0, 0.0};
+    double epc0 = Kc.b1 * Wc[0];
+    double dwc = 0.0;
+
+    const int ry = row + 1 < TY ? row + 1 : row;
+    const int rm = row > 0 ? row - 1 : row;
+    const bool ny = has_y1 && row + 1 < TY, nz = has_z1 && lane + 1 < ZW;
+    double* stage = &s.wep[row][0][0];
+
+    auto step = [&](auto K) {
+        constexpr int k = K(), u = (K() + 1) % D;
+        const int buf = static_cast<int>(p - x0) & 1;
+        load(p + 1 + D, WF[k], GF[k]);
+
+        const double epn0 = Kn.b1 * Wn[0], epn1 = -Kn.b2 * Wn[1], epn2 = Kn.b3 * Wn[2];
+        s.wep[row][0][lane] = Wc[0];
+        s.wep[row][1][lane] = Wc[2];
+        s.wep[row][2][lane] = epn1;
+        // z neighbours inside the wavefront
+        const double sz0 = from_up(Wc[0]), sz1 = from_up(Wc[1]);
+        const double ep_z2 = from_down(epn2);
+        __syncthreads();
+
+        const bool has_x1 = p + 1 < Nx;
+        const double wx1 = has_x1 ? Wn[1] : Wc[1], wx2 = has_x1 ? Wn[2] : Wc[2];
+        const double sy0 = s.wep[ry][0][lane], sy2 = s.wep[ry][1][lane];
+        const double wy0 = ny ? sy0 : Wc[0];
+        const double wy2 = ny ? sy2 : Wc[2];
+        const double wz0 = nz ? sz0 : Wc[0];
+        const double wz1 = nz ? sz1 : Wc[1];
+        const double Fxy = (wx1 - Wc[1]) - (wy0 - Wc[0]);
+        const double Fxz = (wx2 - Wc[2]) - (wz0 - Wc[0]);
+        const double Fyz = (wy2 - Wc[2]) - (wz1 - Wc[1]);
+        const double eta0 = Kc.a1 * Fyz, eta1 = -Kc.a2 * Fxz, eta2 = Kc.a3 * Fxy;
+
+        const double sep0 = s.wep[rm][2][lane];
+        const double ep_y1 = row > 0 ? sep0 : 0.0;
+        const double H = (epn0 - epc0) - (epn1 - ep_y1) + (epn2 - ep_z2);
+        const double dwn = -Kn.c * H;
+
+        s.eta[0][row][lane] = eta0;
+        s.eta[1][row][lane] = eta2;
+        s.dw[buf ^ 1][row][lane] = dwn;
+        const double ez0 = from_down(eta0), ez1 = from_down(eta1);
+        const double sdz = from_up(dwc);
+        __syncthreads();
+
+        const Coefs Knn = coefs_of(GF[u], p + 2);
+
+        const double sey0 = s.eta[0][rm][lane], sey2 = s.eta[1][rm][lane];
+        const double ey0 = row > 0 ? sey0 : 0.0, ey2 = row > 0 ? sey2 : 0.0;
+        const double Gxy = (eta1 - etaPrev[1]) - (eta0 - ey0);
+        const double Gxz = (eta2 - etaPrev[2]) - (eta0 - ez0);
+        const double Gyz = (eta2 - ey2) - (eta1 - ez1);
+        const double sdy = s.dw[buf][ry][lane];
+        const double ddx = has_x1 ? dwn - dwc : 0.0;
+        const double ddy = has_y1 ? sdy - dwc : 0.0;
+        const double ddz = has_z1 ? (lane + 1 < ZW ? sdz : dwc) - dwc : 0.0;
+        // this row's ω / ⋆ω staging words were last read before the barrier
+        // above: the row's output goes through them
+        stage[3 * lane + 0] = Kc.a1 * Gyz + ddx;
+        stage[3 * lane + 1] = -Kc.a2 * Gxz + ddy;
+        stage[3 * lane + 2] = Kc.a3 * Gxy + ddz;
+        wave_lds_sync();
+        const bool store = p >= xa;
+        const uint32_t pbase = store ? static_cast<uint32_t>(p - xa) * oplane_bytes : 0u;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const uint64_t v = __builtin_bit_cast(uint64_t, stage[64 * q + lane]);
+            const U32x2 vv = {static_cast<unsigned>(v), static_cast<unsigned>(v >> 32)};
+            __builtin_amdgcn_raw_buffer_store_b64(vv, ro, (store && seg_st[q]) ? pbase + seg_off[q] : kOutside, 0,
+                                                  SPOL ? 2 : 0);
+        }
+        wave_lds_sync();
+
+        etaPrev[0] = eta0;
+        etaPrev[1] = eta1;
+        etaPrev[2] = eta2;
+        epc0 = epn0;
+        dwc = dwn;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Wc[c] = Wn[c];
+            Wn[c] = moved(WF[u][c]);
+        }
+        Kc = Kn;
+        Kn = Knn;
+        ++p;
+    };
+
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    const int steps = static_cast<int>(xb - x0);
+    int done = 0;
+    for (; done + D <= steps; done += D) unrolled<D>(step);
+    const int rest = steps - done;
+    unrolled<D - 1>([&](auto K) {
+        if (K() < rest) step(K);
+    });
+}
+
 // ------------------------------------------------------ 2D 0-form, direct ---
 
 // Δf(n) = −(1/(sx sy)) [ (sy/sx)(n)(f(n+x)−f(n)) − (sy/sx)(n−x)(f(n)−f(n−x))
@@ -758,6 +977,31 @@ int launch_pipe(const FusedGeom& g, const double* in, double* out, const int64_t
     return SIL_OK;
 }
 
+template <int TY, int D, int SPOL = 0>
+int launch_stage(const FusedGeom& g, const double* in, double* out, const int64_t* o, int xchunk, hipStream_t stream)
+{
+    const int64_t ntz = halo_tiles(g.ext[2], o[2], 64), nty = halo_tiles(g.ext[1], o[1], TY);
+    const int64_t in_plane = g.ext[1] * g.ext[2] * 24, out_plane = o[1] * o[2] * 24;
+    const int64_t fit = std::min(kMaxBufferBytes / in_plane - 1, kMaxBufferBytes / std::max<int64_t>(out_plane, 1));
+    SIL_REQUIRE(fit >= 1, SIL_ERR_UNSUPPORTED, "laplacian: a %lld x %lld cross-section exceeds the staged kernel",
+                (long long)g.ext[1], (long long)g.ext[2]);
+    xchunk = static_cast<int>(std::min<int64_t>({(int64_t)xchunk, fit, (int64_t)kMaxXChunk}));
+    SIL_REQUIRE(xchunk >= 1, SIL_ERR_INVALID, "laplacian: x chunk %d outside [1, %d]", xchunk, kMaxXChunk);
+    const int64_t nxc = (o[0] + xchunk - 1) / xchunk;
+    const int64_t nb = ntz * nty * nxc;
+    SIL_REQUIRE(nb < (1LL << 31), SIL_ERR_UNSUPPORTED, "laplacian: grid too large for one launch");
+    if (g.metric)
+        hipLaunchKernelGGL((k_lap3d_stage<TY, true, D, SPOL>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in,
+                           g.metric, g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0],
+                           o[1], o[2], (int)ntz, (int)nty, (int)nxc, xchunk);
+    else
+        hipLaunchKernelGGL((k_lap3d_stage<TY, false, D, SPOL>), dim3((unsigned)nb), dim3(64 * TY), 0, stream, in,
+                           g.metric, g.spacing[0], g.spacing[1], g.spacing[2], out, g.ext[0], g.ext[1], g.ext[2], o[0],
+                           o[1], o[2], (int)ntz, (int)nty, (int)nxc, xchunk);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
 }  // namespace
 
 int launch_laplacian_3d_1form(const FusedGeom& g, const double* in, double* out, const int64_t* o,
@@ -792,6 +1036,23 @@ int launch_laplacian_3d_1form(const FusedGeom& g, const double* in, double* out,
         SIL_PIPE(256, 4);
 #undef SIL_PIPE
         set_error("laplacian: unsupported pipelined tile %d x %d", zw, tile.ty);
+        return SIL_ERR_INVALID;
+    }
+    if (tile.variant == FUSED_STAGE) {
+        // 64-lane rows; tile.ty rows; tile.tz: planes of loads in flight (1–3;
+        // other values: 2); tile.nt: non-temporal output stores
+        const int depth = (tile.tz >= 1 && tile.tz <= 3) ? tile.tz : 2;
+#define SIL_STAGE(TY, D)                                                                     \
+    if (tile.ty == TY && depth == D)                                                         \
+        return tile.nt ? launch_stage<TY, D, 1>(g, in, out, o, xchunk, stream)               \
+                       : launch_stage<TY, D, 0>(g, in, out, o, xchunk, stream)
+        SIL_STAGE(8, 2);
+        SIL_STAGE(12, 2);
+        SIL_STAGE(16, 1);
+        SIL_STAGE(16, 2);
+        SIL_STAGE(16, 3);
+#undef SIL_STAGE
+        set_error("laplacian: unsupported staged tile (%d rows, depth %d)", tile.ty, depth);
         return SIL_ERR_INVALID;
     }
     SIL_REQUIRE(tile.variant == FUSED_MARCH, SIL_ERR_INVALID, "laplacian: unknown kernel variant %d", tile.variant);

@@ -27,7 +27,8 @@ struct FusedGeom {
 // FUSED_DIRECT: one thread per output node, neighbours through the caches;
 // FUSED_MARCH: x-march with guarded global loads, one plane prefetched;
 // FUSED_PIPE: x-march with branch-free buffer accesses, two planes in flight
-enum FusedVariant { FUSED_DIRECT = 0, FUSED_MARCH = 1, FUSED_PIPE = 2 };
+// FUSED_STAGE: FUSED_PIPE with z neighbours by DPP and staged contiguous stores
+enum FusedVariant { FUSED_DIRECT = 0, FUSED_MARCH = 1, FUSED_PIPE = 2, FUSED_STAGE = 3 };
 
 struct FusedTile {
     int variant;  // FusedVariant

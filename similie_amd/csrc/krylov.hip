@@ -20,6 +20,7 @@
 // on that FP64 residual.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <mutex>
 #include <new>
@@ -58,6 +59,8 @@ struct Workspace {
     double *fb_r = nullptr, *fb_d = nullptr;
     hipGraphExec_t graph[2][3] = {}; // [precision][preconditioner: identity, Jacobi, SpdIsai]
     int graph_chunk[2][3] = {};
+    // mixed solves of the 3D curl-curl operator: the single-pass iteration
+    bool single_pass = false;
 };
 
 namespace {
@@ -107,12 +110,14 @@ __global__ __launch_bounds__(kUpdBlock) void k_dot2(const T* __restrict__ a, con
 }
 
 // r = b, x = 0, p_old = 0; partials for <b,b>, <r,z>
+// p_is_z: the first direction p = z itself (the single-pass iteration
+// starts from it); otherwise p_old = 0 and the first step forms it
 template <typename T>
 __global__ __launch_bounds__(kUpdBlock) void k_init(const double* __restrict__ b, double* __restrict__ x,
                                                     T* __restrict__ r, T* __restrict__ pold,
                                                     const T* __restrict__ inv_diag, int64_t n,
                                                     double* __restrict__ part_bb, double* __restrict__ part_rz,
-                                                    float* __restrict__ xc)
+                                                    float* __restrict__ xc, bool p_is_z = false)
 {
     double vb = 0.0, vz = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -120,8 +125,8 @@ __global__ __launch_bounds__(kUpdBlock) void k_init(const double* __restrict__ b
         x[i] = 0.0;
         if (xc) xc[i] = 0.0f;
         r[i] = (T)bi;
-        pold[i] = T(0);
         const double ri = (double)(T)bi;
+        pold[i] = p_is_z ? (T)(inv_diag ? ri * (double)inv_diag[i] : ri) : T(0);
         vb += bi * bi;
         vz += ri * (inv_diag ? ri * (double)inv_diag[i] : ri);
     }
@@ -229,6 +234,99 @@ __global__ void k_beta(CgScalars* sc, const double* part_rz, int count_rz, const
     const double rr = reduce_partials(part_rr, count_rr);
     if (threadIdx.x == 0) {
         sc->beta = rz / sc->rz;
+        sc->rz_prev = sc->rz;
+        sc->rz = rz;
+        sc->rr = rr;
+        sc->iter += 1;
+        if (rr <= sc->tol2bb) {
+            sc->done = 1;
+            sc->converged = 1;
+        } else if (sc->iter >= sc->max_iter) {
+            sc->done = 1;
+        }
+    }
+}
+
+// ---- single-pass FP32 iteration (3D curl-curl operator) ----
+//
+// The march that forms q = A p also sums ⟨q, z⟩ and ⟨q, D⁻¹ q⟩ (z = D⁻¹ r), so
+// the next ⟨r, z⟩ is known before r is updated:
+//   γ' = ⟨r − αq, D⁻¹(r − αq)⟩ = γ − 2α⟨q, z⟩ + α²⟨q, D⁻¹ q⟩,  β' = γ'/γ,
+// and one streaming pass updates r, the iterate and the next direction
+// p' = D⁻¹ r' + β' p together; it also sums the exact ⟨r', z'⟩ (the next α
+// uses it) and ⟨r', r'⟩.  Per iteration: the march and one vector pass, where
+// the standard iteration has the march, the update and the direction pass.
+
+// α = γ / ⟨p, q⟩ and β' from the expansion; three partial arrays at stride
+__global__ void k_alpha_expand(CgScalars* sc, const double* part, int64_t stride, int count)
+{
+    if (sc->done) return;
+    const double pq = reduce_partials(part, count);
+    const double qz = reduce_partials(part + stride, count);
+    const double qq = reduce_partials(part + 2 * stride, count);
+    if (threadIdx.x == 0) {
+        const double g = sc->rz;
+        const double alpha = g / pq;
+        const double gn = g - 2.0 * alpha * qz + alpha * alpha * qq;
+        sc->pq = pq;
+        sc->alpha = alpha;
+        sc->beta = gn / g;
+    }
+}
+
+// r' = r − αq; xc += αp; p' = D⁻¹ r' + β' p (arithmetic of p_value); partials
+// of ⟨r', z'⟩ and ⟨r', r'⟩
+__global__ __launch_bounds__(kUpdBlock) void k_update_dir(const CgScalars* __restrict__ sc,
+                                                          const float* __restrict__ p, float* __restrict__ pn,
+                                                          float* __restrict__ r, const float* __restrict__ q,
+                                                          const float* __restrict__ inv_diag, int64_t n,
+                                                          double* __restrict__ part_rz, double* __restrict__ part_rr,
+                                                          float* __restrict__ xc)
+{
+    if (sc->done) return;
+    const double alpha = sc->alpha, beta = sc->beta;
+    const float alpha32 = (float)alpha;
+    double vz = 0.0, vr = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float pi = p[i];
+        xc[i] = fmaf(alpha32, pi, xc[i]);
+        const float ri = (float)((double)r[i] - alpha * (double)q[i]);
+        r[i] = ri;
+        const double rd = (double)ri;
+        const double z = inv_diag ? rd * (double)inv_diag[i] : rd;
+        pn[i] = (float)(z + beta * (double)pi);
+        vr += rd * rd;
+        vz += rd * z;
+    }
+    __shared__ double sh[2][kUpdBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        vz += __shfl_down(vz, off, 64);
+        vr += __shfl_down(vr, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = vz;
+        sh[1][threadIdx.x >> 6] = vr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int i = 0; i < kUpdBlock / 64; ++i) {
+            s0 += sh[0][i];
+            s1 += sh[1][i];
+        }
+        part_rz[blockIdx.x] = s0;
+        part_rr[blockIdx.x] = s1;
+    }
+}
+
+// the exact ⟨r', z'⟩ and ⟨r', r'⟩: the next α, the stopping test (β' was
+// taken from the expansion)
+__global__ void k_gamma(CgScalars* sc, const double* part_rz, int count_rz, const double* part_rr, int count_rr)
+{
+    if (sc->done) return;
+    const double rz = reduce_partials(part_rz, count_rz);
+    const double rr = reduce_partials(part_rr, count_rr);
+    if (threadIdx.x == 0) {
         sc->rz_prev = sc->rz;
         sc->rz = rz;
         sc->rr = rr;
@@ -475,6 +573,21 @@ int enqueue_iteration(sil_operator op, Workspace& w, T* r, T* q, T* pold, T* pne
     const T* zsrc = r;
     if constexpr (sizeof(T) == 8)
         if (pc == PC_ISAI) zsrc = w.isai_z;
+    if constexpr (sizeof(T) == 4) {
+        if (w.single_pass) {
+            // pold holds the current direction p, pnew receives the next one
+            int nb = 0;
+            const NlData* nl = op->kind == OP_MS3D_JACOBIAN ? &op->nl : nullptr;
+            int rc = ms3d_qdots32(op->ms, nl, pold, q, r, invd, part0, w.part_stride, &w.sc->done, &nb, s);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_alpha_expand, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, w.part_stride, nb);
+            hipLaunchKernelGGL(k_update_dir, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.sc, pold, pnew, r, q, invd,
+                               op->n, part1, part2, w.xc32);
+            hipLaunchKernelGGL(k_gamma, dim3(1), dim3(kRedThreads), 0, s, w.sc, part1, kUpdBlocks, part2, kUpdBlocks);
+            SIL_LAUNCH_CHECK();
+            return SIL_OK;
+        }
+    }
     PUpdate<T> pu{zsrc, pold, pnew, &w.sc->beta, invd, &w.sc->done};
     int nb = 0;
     int rc;
@@ -568,6 +681,13 @@ void invalidate_jacobi(sil_operator op)
     if (op && op->ws) op->ws->jacobi_ready = false;
 }
 
+// the operator changed (a Jacobian at a new state): SpdIsai is regenerated
+// by the next solve that asks for it
+void invalidate_isai(sil_operator op)
+{
+    if (op && op->ws) op->ws->isai_ready = false;
+}
+
 void destroy_workspace(Workspace* w)
 {
     if (!w) return;
@@ -607,12 +727,33 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
         // matrix-free (minimize_strong_formulation_residual.hpp:1332-1353)
         CsrData assembled{};
         const CsrData* a = &op->csr;
+        SIL_REQUIRE(op->kind == OP_CSR || op->kind == OP_MAGNETOSTATICS3D || op->kind == OP_MS3D_JACOBIAN ||
+                        op->kind == OP_MAGNETOSTATICS2D || op->kind == OP_MS2D_JACOBIAN,
+                    SIL_ERR_UNSUPPORTED,
+                    "SpdIsai needs an assembled operator or a magnetostatics operator (linear or its Jacobian)");
         if (op->kind == OP_MAGNETOSTATICS3D) {
             if ((rc = csr_assemble_ms3d(op->ms, assembled, s))) return rc;
             a = &assembled;
+        } else if (op->kind == OP_MS3D_JACOBIAN) {
+            // the Jacobian at the state of the last evaluation, assembled at
+            // every Newton step (minimize_strong_formulation_residual.hpp:1432-1433)
+            if ((rc = csr_assemble_ms3d_jacobian(op->ms, op->nl, assembled, s))) return rc;
+            a = &assembled;
+        } else if (op->kind == OP_MAGNETOSTATICS2D) {
+            if ((rc = csr_assemble_ms2d(op->m2, assembled, s))) return rc;
+            a = &assembled;
+        } else if (op->kind == OP_MS2D_JACOBIAN) {
+            if ((rc = csr_assemble_ms2d_jacobian(op->m2, op->nl, assembled, s))) return rc;
+            a = &assembled;
         }
-        SIL_REQUIRE(op->kind == OP_CSR || op->kind == OP_MAGNETOSTATICS3D, SIL_ERR_UNSUPPORTED,
-                    "SpdIsai needs an assembled operator or the 3D magnetostatics operator");
+        // graphs captured with the previous factor hold its (freed) arrays
+        for (auto& row : w.graph)
+            if (row[PC_ISAI]) {
+                (void)hipGraphExecDestroy(row[PC_ISAI]);
+                row[PC_ISAI] = nullptr;
+            }
+        csr_free(w.isai_g);
+        csr_free(w.isai_gt);
         rc = spd_isai_generate(*a, w.isai_g, w.isai_gt, s);
         csr_free(assembled);
         if (rc) return rc;
@@ -641,9 +782,15 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
     double* part1 = w.partials + w.part_stride;
     const double* invd = jacobi ? w.inv_diag : nullptr;
     const float* invd32 = jacobi ? w.inv_diag32 : nullptr;
+    static const bool single_pass_enabled = [] {
+        const char* e = std::getenv("SIL_CG_SINGLE_PASS");
+        return !(e && e[0] == '0');
+    }();
+    w.single_pass = mixed && single_pass_enabled && pc != PC_ISAI &&
+                    (op->kind == OP_MAGNETOSTATICS3D || op->kind == OP_MS3D_JACOBIAN) && ms3d_single_pass_ok(op->ms);
     if (mixed)
         hipLaunchKernelGGL((k_init<float>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r32, w.p32[0],
-                           invd32, n, part0, part1, w.xc32);
+                           invd32, n, part0, part1, w.xc32, w.single_pass);
     else
         hipLaunchKernelGGL((k_init<double>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.x, w.r, w.p[0], invd, n,
                            part0, part1, nullptr);
@@ -683,10 +830,11 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
         // iteration `it` of a chunk writes its direction into p32[(it + 1) & 1]:
         // after an odd number of executed iterations the latest one sits in p32[1]
         const int executed = h.iter - iter_before;
-        if (h.done && (executed & 1))
-            SIL_CHECK_HIP(hipMemcpyAsync(w.p32[0], w.p32[1], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        const int cur = executed & 1;
         const bool hit_max = h.iter >= h.max_iter;
         const bool want = h.converged || since_replace >= replace || h.rr < 1e-4 * rr_ref;
+        if (h.done && cur && !(w.single_pass && want && !hit_max))
+            SIL_CHECK_HIP(hipMemcpyAsync(w.p32[0], w.p32[1], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
         if (want && !hit_max) {
             // FP64 residual replacement; keeps p (the direction buffer in use is p32[0])
             hipLaunchKernelGGL(k_flush, dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.x, w.xc32, n);
@@ -694,6 +842,15 @@ int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_sol
             hipLaunchKernelGGL((k_replace<float>), dim3(kUpdBlocks), dim3(kUpdBlock), 0, s, w.b, w.tmp, w.r32, invd32,
                                n, part0, part1);
             hipLaunchKernelGGL(k_after_replace, dim3(1), dim3(kRedThreads), 0, s, w.sc, part0, part1, kUpdBlocks);
+            if (w.single_pass) {
+                // the next direction was formed from the old residual: form it
+                // again from the replaced one, z + β p with β = γ_new / γ
+                PUpdate<float> pu{w.r32, w.p32[cur ^ 1], w.p32[cur], &w.sc->beta, invd32, &w.sc->done};
+                if ((rc = direction_pass32(pu, n, s))) return rc;
+                if (cur)
+                    SIL_CHECK_HIP(
+                        hipMemcpyAsync(w.p32[0], w.p32[1], sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+            }
             SIL_LAUNCH_CHECK();
             if ((rc = read_scalars(w, s))) return rc;
             since_replace = 0;

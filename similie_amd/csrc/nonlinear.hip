@@ -35,6 +35,7 @@ namespace sil {
 int cg_solve(sil_operator op, const double* rhs, double* solution, const sil_solver_settings& st,
              sil_solver_diagnostics& dg, hipStream_t s);
 void invalidate_jacobi(sil_operator op);
+void invalidate_isai(sil_operator op);
 
 struct NewtonBuffers {
     double *r = nullptr, *fx = nullptr, *delta = nullptr, *xc = nullptr, *rc = nullptr;
@@ -112,7 +113,8 @@ __global__ __launch_bounds__(kBlock) void k_nl_state(StateArgs s, const double* 
 
 struct DiagArgs {
     int64_t nx, ny, nz;
-    double penalty;
+    double penalty; // 0 under the divergence gauge (its diagonal is added apart)
+    double ysign;
     const double* ls[3];
     const double* inv_ls[3];
     const double* a;
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_nl_jdiag(DiagArgs d, double* __restr
             const double ilx = d.inv_ls[0][ii], ily = d.inv_ls[1][jj], ilz = d.inv_ls[2][kk];
             const double w = m == 0 ? lx * ily * ilz : (m == 1 ? ly * ilx * ilz : lz * ilx * ily);
             const double K = (m == mp ? d.a[s] : 0.0) + d.c[s] * d.b[m][s] * d.b[mp][s];
-            return w * K;
+            return m == 1 ? w * K * d.ysign : w * K;
         };
         const double lx = d.ls[0][i], ly = d.ls[1][j], lz = d.ls[2][k];
         const double ilx = d.inv_ls[0][i], ily = d.inv_ls[1][j], ilz = d.inv_ls[2][k];
@@ -261,13 +263,21 @@ int ms3d_state(const Ms3dData& d, const NlData& nl, const double* x, hipStream_t
 
 int ms3d_jacobian_diagonal(const Ms3dData& d, const NlData& nl, double* diag, hipStream_t s)
 {
-    DiagArgs a{d.nx, d.ny, d.nz, d.penalty, {d.ls[0], d.ls[1], d.ls[2]}, {d.inv_ls[0], d.inv_ls[1], d.inv_ls[2]},
-               nl.st_a, nl.st_c, {nl.st_b[0], nl.st_b[1], nl.st_b[2]}};
+    DiagArgs a{d.nx,
+               d.ny,
+               d.nz,
+               d.divg ? 0.0 : d.penalty,
+               d.ysign,
+               {d.ls[0], d.ls[1], d.ls[2]},
+               {d.inv_ls[0], d.inv_ls[1], d.inv_ls[2]},
+               nl.st_a,
+               nl.st_c,
+               {nl.st_b[0], nl.st_b[1], nl.st_b[2]}};
     const int64_t nn = d.nx * d.ny * d.nz;
     const unsigned g = static_cast<unsigned>(std::min<int64_t>((nn + kBlock - 1) / kBlock, 16384));
     hipLaunchKernelGGL(k_nl_jdiag, dim3(g), dim3(kBlock), 0, s, a, diag);
     SIL_LAUNCH_CHECK();
-    return SIL_OK;
+    return ms3d_gauge_diagonal_add(d, diag, s);
 }
 
 void destroy_newton(NewtonBuffers* nb)
@@ -341,6 +351,7 @@ int newton_solve(sil_operator op, const double* rhs, double* x, const sil_solver
         // J(x) δ = r: the Jacobian operator reads the state of x, which is
         // the state of the last evaluation
         invalidate_jacobi(op->jac);
+        invalidate_isai(op->jac);
         sil_solver_diagnostics di{};
         if ((rc = cg_solve(op->jac, nb.r, nb.delta, inner, di, s))) return rc;
         dg.iterations += di.iterations;

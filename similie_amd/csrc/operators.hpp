@@ -22,6 +22,14 @@ struct NewtonBuffers; // Newton vectors (nonlinear.hip)
 struct Ms3dData {
     int64_t nx, ny, nz;
     double penalty;
+    // switches (magnetostatics_onelab.hpp:796-828): sign of the y moment's
+    // response; divergence gauge factor·δᵀWδ (factor = penalty · gauge
+    // sign) in place of the penalty term; criterion 0 (potential temporal
+    // derivative) zeroes the right-hand side
+    double ysign = 1.0;
+    int divg = 0;
+    double gauge_factor = 0.0;
+    int criterion = 1;
     double* ls[3];      // local_spacing per axis (device)
     double* inv_ls[3];  // 1 / local_spacing
     double* coord[3];   // coordinates (device)
@@ -108,7 +116,16 @@ int ms3d_apply64(const Ms3dData& d, const double* in, double* out, DotEpilogue d
                  int* nblocks_out, hipStream_t s);
 int ms3d_apply32(const Ms3dData& d, const float* in, float* out, DotEpilogue dot, const PUpdate<float>* pu,
                  int* nblocks_out, hipStream_t s);
+// FP32 q = A p with the partials of ⟨p, q⟩, ⟨q, D⁻¹ r⟩ and ⟨q, D⁻¹ q⟩ at
+// partials, + pstride, + 2·pstride (inv_diag null: identity); nl: the
+// Jacobian at the stored state.  Supported when ms3d_single_pass_ok.
+bool ms3d_single_pass_ok(const Ms3dData& d);
+int ms3d_qdots32(const Ms3dData& d, const NlData* nl, const float* p, float* q, const float* r,
+                 const float* inv_diag, double* partials, int64_t pstride, const int* done, int* nb_out,
+                 hipStream_t s);
 int ms3d_diagonal(const Ms3dData& d, double* diag, hipStream_t s);
+// adds the divergence gauge's diagonal (nothing without the switch)
+int ms3d_gauge_diagonal_add(const Ms3dData& d, double* diag, hipStream_t s);
 int ms3d_nodal_mu(int64_t nx, int64_t ny, int64_t nz, const double* cell_mu, double mu0, double* mu, hipStream_t s);
 int ms3d_rhs(const Ms3dData& d, const double* cell_current, double* rhs, hipStream_t s);
 int ms3d_nu_from_mu(const double* mu, double* nu, float* nu32, int64_t n, hipStream_t s);
@@ -149,6 +166,7 @@ int ms2d_post_process(const Ms2dData& d, const BhCurve* curve, const double* pot
 
 // p_new = z + β p_old over n rows (the CG direction) as its own pass
 int direction_pass64(const PUpdate<double>& pu, int64_t n, hipStream_t s);
+int direction_pass32(const PUpdate<float>& pu, int64_t n, hipStream_t s);
 
 // ------------------------------------------------- assembled (CSR) form ---
 
@@ -166,6 +184,11 @@ struct CsrData {
 };
 
 int csr_assemble_ms3d(const Ms3dData& d, CsrData& out, hipStream_t s);
+// the Jacobian of the nonlinear operator at the state of its last evaluation
+int csr_assemble_ms3d_jacobian(const Ms3dData& d, const NlData& nl, CsrData& out, hipStream_t s);
+// the 2D operator (linear) and its Jacobian at the state of the last evaluation
+int csr_assemble_ms2d(const Ms2dData& d, CsrData& out, hipStream_t s);
+int csr_assemble_ms2d_jacobian(const Ms2dData& d, const NlData& nl, CsrData& out, hipStream_t s);
 int csr_import(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* col, const double* val, CsrData& out,
                hipStream_t s);
 void csr_free(CsrData& a);

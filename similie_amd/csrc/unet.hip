@@ -710,6 +710,12 @@ __global__ void k_pack33(const float* __restrict__ w, bf16* __restrict__ wf, bf1
     }
 }
 
+__global__ void k_bf16_to_f32(const bf16* __restrict__ a, float* __restrict__ b, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
 inline unsigned grid1(size_t n, int block = 256, size_t cap = 8192)
 {
     size_t g = (n + block - 1) / block;
@@ -852,6 +858,13 @@ int adam(float* w, const float* g, float* m, float* v, int n, float lr, int step
 int pack33(const float* w, bf16* wf, bf16* wb, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pack33, dim3(36), dim3(256), 0, s, w, wf, wb);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
+int to_f32(const bf16* a, float* b, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bf16_to_f32, dim3(grid1((size_t)n)), dim3(256), 0, s, a, b, n);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }

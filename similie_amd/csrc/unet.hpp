@@ -51,6 +51,7 @@ int up_bwd(const bf16* dy, bf16* dxc, int B, int h, int w, hipStream_t s);
 int cos_loss(const float* N, const float* r, int rs, const float* e, const float* wx, const float* wy, float* AN,
              float* part, int parts, float* dN, float* loss, int B, int n, hipStream_t s);
 int adam(float* w, const float* g, float* m, float* v, int n, float lr, int step, hipStream_t s);
+int to_f32(const bf16* a, float* b, int64_t n, hipStream_t s);
 // BF16 forward weights and the flipped, channel-transposed input-gradient weights of one 32→32 layer
 int pack33(const float* w, bf16* wf, bf16* wb, hipStream_t s);
 // output-head weights [tap][ci] → BF16 B matrix with hi/lo columns

@@ -294,6 +294,18 @@ int sil_unet_forward(sil_unet u, int32_t batch, const float* x, float* out, void
     return forward(u, batch, x, out, static_cast<hipStream_t>(stream));
 }
 
+int sil_unet_activation(sil_unet u, int32_t which, int32_t batch, float* out, void* stream)
+{
+    SIL_REQUIRE(u != nullptr && out != nullptr, SIL_ERR_INVALID, "unet: null argument");
+    UNET_TRY(check_batch(u, batch));
+    // graph order: a0 e0 a1 e1 a2 e2 a3 b3 c2 f2 c1 f1 c0, levels 0 0 1 1 2 2 3 3 2 2 1 1 0
+    const bf16* acts[13] = {u->a0, u->e0, u->a1, u->e1, u->a2, u->e2, u->a3, u->b3, u->c2, u->f2, u->c1, u->f1, u->c0};
+    static constexpr int level[13] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0};
+    SIL_REQUIRE(which >= 0 && which < 13, SIL_ERR_INVALID, "unet: activation %d outside 0..12", which);
+    const int64_t n = (int64_t)batch * u->Hl[level[which]] * u->Wl[level[which]] * C;
+    return sil::unet::to_f32(acts[which], out, n, sil::as_stream(stream));
+}
+
 int sil_unet_backward(sil_unet u, int32_t batch, const float* x, const float* dout, float* grad, void* stream)
 {
     SIL_REQUIRE(u != nullptr && x != nullptr && dout != nullptr && grad != nullptr, SIL_ERR_INVALID,

@@ -209,3 +209,41 @@ class StagedLaplacian:
 
 def make_staged_laplacian(grid, degree, out, tensor, comp_dim=-1):
     return StagedLaplacian(grid, degree, out, tensor, comp_dim)
+
+
+class StagedCodifferential:
+    """StagedCodifferential(exec, tensor, metric, position).run(out, tensor)
+    (codifferential.hpp:321-508): a degree-k form → its codifferential on
+    out's extents, validated once and run on arrays of the same layout."""
+
+    def __init__(self, grid, degree, out, tensor, comp_dim=-1):
+        vin, vout = _views(out, tensor, None, comp_dim, ())
+        self.grid = grid
+        self.layout_in = (tuple(tensor.shape), tuple(tensor.stride()))
+        self.layout_out = (tuple(out.shape), tuple(out.stride()))
+        h = ctypes.c_void_p()
+        L.check(L.lib().sil_make_staged_codifferential(ctypes.byref(h), grid.handle, degree, ctypes.byref(vin),
+                                                       ctypes.byref(vout)))
+        self.handle = h
+        self._destroy = L.lib().sil_staged_codifferential_destroy
+
+    def run(self, out, tensor, stream=None):
+        _check_tensor(out, "out")
+        _check_tensor(tensor, "tensor")
+        if (tuple(out.shape), tuple(out.stride())) != self.layout_out or \
+                (tuple(tensor.shape), tuple(tensor.stride())) != self.layout_in:
+            raise ValueError("staged codifferential called with shapes or strides it was not built for")
+        L.check(L.lib().sil_staged_codifferential_run(self.handle, L.ptr(out), L.ptr(tensor),
+                                                      L.stream_handle(stream)))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        fn = getattr(self, "_destroy", None)
+        if h is not None and h.value and fn is not None:
+            fn(h)
+            self.handle = None
+
+
+def make_staged_codifferential(grid, degree, out, tensor, comp_dim=-1):
+    return StagedCodifferential(grid, degree, out, tensor, comp_dim)

@@ -79,6 +79,18 @@ class UNet:
         L.check(L.lib().sil_unet_forward(self.handle, b, L.ptr(x), L.ptr(out), L.stream_handle(stream)))
         return out
 
+    ACTIVATIONS = ("a0", "e0", "a1", "e1", "a2", "e2", "a3", "b3", "c2", "f2", "c1", "f1", "c0")
+    LEVELS = (0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0)
+
+    def activation(self, name, batch, stream=None):
+        """The stored activation `name` of the last forward pass (the output
+        of conv_in or of one of the twelve layers), FP32 [batch][H_l][W_l][32]."""
+        which = self.ACTIVATIONS.index(name)
+        lv = self.LEVELS[which]
+        out = torch.empty((batch, self.height >> lv, self.width >> lv, 32), dtype=torch.float32, device="cuda")
+        L.check(L.lib().sil_unet_activation(self.handle, which, batch, L.ptr(out), L.stream_handle(stream)))
+        return out
+
     def backward(self, x, dout, stream=None):
         """Parameter gradient of <dout, forward(x)> (call forward(x) first)."""
         b = self._shape(x)

@@ -10,6 +10,7 @@ import ctypes
 from dataclasses import dataclass
 from typing import NamedTuple
 
+import numpy as np
 import torch
 
 from . import _lib as L
@@ -126,6 +127,17 @@ class MagnetostaticsOperator3D(LinearOperator):
         return CellPostProcessFields(*out)
 
 
+    def set_switches(self, y_response_sign=1.0, divergence_gauge=False, gauge_sign=1.0, criterion="moments"):
+        """the operator's switches (magnetostatics_onelab.hpp:796-828) and
+        the right-hand side's criterion (:4930-4935): y_response_sign −1 flips
+        the y moment's response; divergence_gauge replaces the penalty term
+        P·w^A·A by (P · gauge_sign)·δᵀWδA; criterion "potential" gives a zero
+        right-hand side."""
+        sw = L.Ms3dSwitches(float(y_response_sign), int(bool(divergence_gauge)), float(gauge_sign),
+                            CRITERIA[criterion])
+        L.check(L.lib().sil_magnetostatics3d_set_switches(self.handle, ctypes.byref(sw)))
+        return self
+
     def assemble(self, stream=None):
         """assemble_matrix_data + csr_from_matrix_data
         (magnetostatics_onelab.hpp:3115-3536,
@@ -176,6 +188,18 @@ class NonlinearMagnetostaticsOperator3D(MagnetostaticsOperator3D):
         L.check(L.lib().sil_magnetostatics3d_apply_jacobian(self.handle, L.ptr(state), L.ptr(v), L.ptr(out),
                                                              L.stream_handle(stream)))
         return out
+
+    def assemble_jacobian(self, state, stream=None):
+        """assemble_matrix_data(model, state) (magnetostatics_onelab.hpp
+        :3115-3536): the Jacobian at `state` as a CSR matrix assembled on the
+        device."""
+        _f64(state, "state")
+        if state.numel() != self.size():
+            raise ValueError(f"state must hold {self.size()} values")
+        h = ctypes.c_void_p()
+        L.check(L.lib().sil_magnetostatics3d_assemble_jacobian(self.handle, L.ptr(state), ctypes.byref(h),
+                                                               L.stream_handle(stream)))
+        return AssembledOperator._adopt(h)
 
     def jacobian_diagonal(self, state, stream=None):
         _f64(state, "state")
@@ -238,6 +262,60 @@ def spd_isai(operator, stream=None):
 CRITERIA = {"potential": 0, "moments": 1, "potential_and_moments": 2}
 
 
+def diagnostic_cell_mask(cell_tags, diagnostic_region_tags=(), positive_conductor_tags=()):
+    """the per-cell mask of the air-gap diagnostics from physical tags: bit 0
+    in a diagnostic region, bit 1 a positively oriented conductor (device
+    uint8)"""
+    tags = np.asarray(cell_tags if not torch.is_tensor(cell_tags) else cell_tags.cpu().numpy())
+    mask = np.isin(tags, list(diagnostic_region_tags)).astype(np.uint8)
+    mask |= (np.isin(tags, list(positive_conductor_tags)).astype(np.uint8) << 1)
+    return torch.from_numpy(mask).cuda()
+
+
+def air_gap_diagnostics(operator, fields, cell_mask, cell_current, num_turns=1.0, length_z=None, z_value=0.0,
+                        stream=None):
+    """the reference's air-gap diagnostics (magnetostatics_onelab.hpp
+    :4666-4720 2D, 5246-5307 3D, 328-388 for the published quantities):
+    upper air-gap faces of the diagnostic region, flux Φ, Maxwell traction
+    integral (x-mirror projected when the faces are symmetric) and mean
+    magnitude, the positive conductor's current and the inductance
+    Φ / (N I).  fields: CellPostProcessFields of the solution; cell_mask:
+    diagnostic_cell_mask; cell_current: (cells, 3) in 3D, (cells,) J_z in 2D;
+    length_z and z_value: the 2D extrusion length and face z."""
+    if cell_mask.dtype != torch.uint8 or not cell_mask.is_cuda or not cell_mask.is_contiguous():
+        raise ValueError("cell_mask must be a contiguous uint8 CUDA tensor")
+    b, h = _f64(fields.magnetic_induction, "magnetic_induction"), _f64(fields.magnetic_field, "magnetic_field")
+    jc = _f64(cell_current, "cell_current")
+    r = L.AirGapResult()
+    if isinstance(operator, MagnetostaticsOperator2D):
+        if length_z is None:
+            raise ValueError("a 2D diagnostic needs length_z")
+        L.check(L.lib().sil_magnetostatics2d_air_gap_diagnostics(
+            operator.handle, L.ptr(b), L.ptr(h), L.ptr(cell_mask), L.ptr(jc), float(length_z), float(z_value),
+            ctypes.byref(r), L.stream_handle(stream)))
+    else:
+        L.check(L.lib().sil_magnetostatics3d_air_gap_diagnostics(
+            operator.handle, L.ptr(b), L.ptr(h), L.ptr(cell_mask), L.ptr(jc), ctypes.byref(r),
+            L.stream_handle(stream)))
+    out = {"faces": r.faces, "surface_measure": r.surface_measure, "flux": r.flux, "traction": list(r.traction),
+           "traction_magnitude_integral": r.traction_magnitude, "current": r.current,
+           "current_cells": r.current_cells}
+    out["mean_traction_magnitude"] = 0.0 if r.surface_measure == 0.0 else r.traction_magnitude / r.surface_measure
+    out["terminal_current"] = r.current / num_turns
+    out["inductance"] = 0.0 if r.current == 0.0 else r.flux / (r.current * num_turns)
+    return out
+
+
+def ms3d_switches_from_environment():
+    """the 3D switches the reference reads from SIMILIE_MAGNETIC_Y_RESPONSE_
+    SIGN_3D, SIMILIE_DIVERGENCE_GAUGE_3D and SIMILIE_VECTOR_POTENTIAL_GAUGE_
+    SIGN_3D, as keyword arguments of set_switches"""
+    sw = L.Ms3dSwitches()
+    L.check(L.lib().sil_ms3d_switches_from_environment(ctypes.byref(sw)))
+    return {"y_response_sign": sw.y_response_sign, "divergence_gauge": bool(sw.divergence_gauge),
+            "gauge_sign": sw.gauge_sign}
+
+
 class MagnetostaticsOperator2D(LinearOperator):
     """MagnetostaticsOperator2D (magnetostatics_onelab.hpp:1013-1522): A_z on
     a rectilinear node grid, row i + nx·j; interior rows gather ∂H/∂B at the
@@ -292,6 +370,14 @@ class MagnetostaticsOperator2D(LinearOperator):
             *[L.ptr(o) for o in out], L.stream_handle(stream)))
         return CellPostProcessFields(*out)
 
+    def assemble(self, stream=None):
+        """assemble_matrix_data (magnetostatics_onelab.hpp:3554-3748) + the
+        CSR conversion (minimize_strong_formulation_residual.hpp:565-646):
+        the linear operator as a CSR matrix assembled on the device."""
+        h = ctypes.c_void_p()
+        L.check(L.lib().sil_magnetostatics2d_assemble(self.handle, ctypes.byref(h), L.stream_handle(stream)))
+        return AssembledOperator._adopt(h)
+
 
 class NonlinearMagnetostaticsOperator2D(MagnetostaticsOperator2D):
     """MagnetostaticsOperator2D with MaterialMagnetostaticsHamiltonian
@@ -334,6 +420,18 @@ class NonlinearMagnetostaticsOperator2D(MagnetostaticsOperator2D):
         L.check(L.lib().sil_magnetostatics2d_jacobian_diagonal(self.handle, L.ptr(state), L.ptr(d),
                                                                L.stream_handle(stream)))
         return d
+
+    def assemble_jacobian(self, state, stream=None):
+        """assemble_matrix_data(model, state) (magnetostatics_onelab.hpp
+        :3554-3748): the Jacobian at `state` as a CSR matrix assembled on the
+        device."""
+        _f64(state, "state")
+        if state.numel() != self.size():
+            raise ValueError(f"state must hold {self.size()} values")
+        h = ctypes.c_void_p()
+        L.check(L.lib().sil_magnetostatics2d_assemble_jacobian(self.handle, L.ptr(state), ctypes.byref(h),
+                                                               L.stream_handle(stream)))
+        return AssembledOperator._adopt(h)
 
 
 class Poisson2D(LinearOperator):

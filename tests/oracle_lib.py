@@ -216,6 +216,34 @@ def ms_apply(coords, mu, penalty, x, y_sign=1.0):
     return out
 
 
+class ms_switches:
+    """the 3D operator's switches for every oracle 3D function inside the
+    block: sign of the y moment's response, divergence gauge, gauge sign"""
+
+    def __init__(self, y_sign=1.0, divergence_gauge=False, gauge_sign=1.0):
+        self.args = (float(y_sign), int(bool(divergence_gauge)), float(gauge_sign))
+
+    def __enter__(self):
+        lib().orc_ms_set_switches(ctypes.c_double(self.args[0]), ctypes.c_int(self.args[1]),
+                                  ctypes.c_double(self.args[2]))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_ms_set_switches(ctypes.c_double(1.0), ctypes.c_int(0), ctypes.c_double(1.0))
+        return False
+
+
+def ms_codifferential(coords, x):
+    """δA per node of the divergence gauge's codifferential (interior DOFs)"""
+    nx, ny, nz = (len(c) for c in coords)
+    xc, yc, zc = (_f64(c) for c in coords)
+    out = np.zeros(nx * ny * nz)
+    xin = _f64(x)
+    lib().orc_ms_codifferential(ctypes.c_long(nx), ctypes.c_long(ny), ctypes.c_long(nz), _d(xc), _d(yc), _d(zc),
+                                _d(xin), _d(out))
+    return out
+
+
 def ms_rhs(coords, cell_current):
     nx, ny, nz = (len(c) for c in coords)
     xc, yc, zc = (_f64(c) for c in coords)
@@ -343,12 +371,29 @@ class Nonlinear:
         lib().orc_nl_jacobian_diagonal(*self._head(), _d(st), _d(out))
         return out
 
+    def jacobian_csr(self, state):
+        """(row_ptr int64, cols int32, vals float64) of J(state) assembled,
+        ascending columns per row"""
+        rp = np.zeros(self.n + 1, dtype=np.int64)
+        cols = np.zeros(27 * self.n, dtype=np.int32)
+        vals = np.zeros(27 * self.n)
+        st = _f64(state)
+        lib().orc_nl_jacobian_assemble_csr.restype = ctypes.c_long
+        nnz = lib().orc_nl_jacobian_assemble_csr(*self._head(), _d(st), rp.ctypes.data_as(_lp),
+                                                 cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(vals))
+        if nnz < 0:
+            raise RuntimeError("row capacity exceeded")
+        return rp, cols[:nnz].copy(), vals[:nnz].copy()
+
     def newton(self, b, tol, max_inner, jacobi=True):
         x = np.zeros(self.n)
         out = np.zeros(4)
         bb = _f64(b)
+        # jacobi: True (Jacobi), False (none) or "spd_isai" (SpdIsai of the
+        # Jacobian assembled at every step)
+        pc = 2 if jacobi == "spd_isai" else (1 if jacobi else 0)
         ok = lib().orc_nl_newton(*self._head(), _d(bb), ctypes.c_double(tol), ctypes.c_int(max_inner),
-                                 ctypes.c_int(1 if jacobi else 0), _d(x), _d(out))
+                                 ctypes.c_int(pc), _d(x), _d(out))
         return x, {"converged": bool(ok), "newton_steps": int(out[0]), "iterations": int(out[1]),
                    "initial_residual_l2": out[2], "final_residual_l2": out[3]}
 
@@ -525,12 +570,25 @@ class Magnetostatics2D:
         lib().orc_ms2d_diagonal(*self._head(), _d(st) if st is not None else None, _d(out))
         return out
 
+    def csr(self, state=None):
+        """(row_ptr int64, cols int32, vals float64) of the operator (linear)
+        or of its Jacobian at `state`, assembled as the reference does"""
+        rp = np.zeros(self.n + 1, dtype=np.int64)
+        cols = np.zeros(8 * self.n, dtype=np.int32)
+        vals = np.zeros(8 * self.n)
+        st = _f64(state) if state is not None else None
+        lib().orc_ms2d_assemble_csr.restype = ctypes.c_long
+        nnz = lib().orc_ms2d_assemble_csr(*self._head(), _d(st) if st is not None else None, rp.ctypes.data_as(_lp),
+                                          cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(vals))
+        return rp, cols[:nnz].copy(), vals[:nnz].copy()
+
     def newton(self, b, tol, max_inner, jacobi=True):
         x = np.zeros(self.n)
         out = np.zeros(4)
         bb = _f64(b)
+        pc = 2 if jacobi == "spd_isai" else (1 if jacobi else 0)
         ok = lib().orc_ms2d_newton(*self._head(), _d(bb), ctypes.c_double(tol), ctypes.c_int(max_inner),
-                                   ctypes.c_int(1 if jacobi else 0), _d(x), _d(out))
+                                   ctypes.c_int(pc), _d(x), _d(out))
         return x, {"converged": bool(ok), "newton_steps": int(out[0]), "iterations": int(out[1]),
                    "initial_residual_l2": out[2], "final_residual_l2": out[3]}
 

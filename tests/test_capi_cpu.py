@@ -120,3 +120,35 @@ def test_python_views_carry_the_component_extent():
     assert E.make_view(t).comp_extent == 3
     assert E.make_view(t.permute(3, 0, 1, 2), comp_dim=0).comp_extent == 3
     assert E.make_view(torch.zeros((4, 5), dtype=torch.float64), comp_dim=None).comp_extent == 1
+
+
+@pytest.mark.parametrize("env,expect", [
+    ({}, (1.0, False, 1.0)),
+    ({"SIMILIE_MAGNETIC_Y_RESPONSE_SIGN_3D": "-1", "SIMILIE_DIVERGENCE_GAUGE_3D": "1",
+      "SIMILIE_VECTOR_POTENTIAL_GAUGE_SIGN_3D": "-0.5"}, (-1.0, True, -1.0)),
+    ({"SIMILIE_MAGNETIC_Y_RESPONSE_SIGN_3D": "abc", "SIMILIE_DIVERGENCE_GAUGE_3D": "0",
+      "SIMILIE_VECTOR_POTENTIAL_GAUGE_SIGN_3D": "0"}, (1.0, False, 1.0)),
+    ({"SIMILIE_MAGNETIC_Y_RESPONSE_SIGN_3D": "", "SIMILIE_DIVERGENCE_GAUGE_3D": "yes",
+      "SIMILIE_VECTOR_POTENTIAL_GAUGE_SIGN_3D": "2.5"}, (1.0, True, 1.0)),
+    ({"SIMILIE_DIVERGENCE_GAUGE_3D": ""}, (1.0, False, 1.0)),
+])
+def test_switches_from_the_environment(lib, monkeypatch, env, expect):
+    """the reference's parsing (magnetostatics_onelab.hpp:796-828): a sign is
+    +1 unless the variable holds a number below zero; the gauge is on when set,
+    non-empty and not starting with '0'"""
+    from similie_amd import solvers as S
+
+    for name in ("SIMILIE_MAGNETIC_Y_RESPONSE_SIGN_3D", "SIMILIE_DIVERGENCE_GAUGE_3D",
+                 "SIMILIE_VECTOR_POTENTIAL_GAUGE_SIGN_3D"):
+        monkeypatch.delenv(name, raising=False)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    sw = S.ms3d_switches_from_environment()
+    assert (sw["y_response_sign"], sw["divergence_gauge"], sw["gauge_sign"]) == expect
+
+
+def test_switches_reject_unknown_criterion(lib):
+    from similie_amd import _lib as L
+
+    sw = L.Ms3dSwitches(1.0, 0, 1.0, 7)
+    assert lib.sil_magnetostatics3d_set_switches(None, ctypes.byref(sw)) == L.SIL_ERR_INVALID

@@ -207,7 +207,8 @@ def test_laplacian_matches_oracle(ext, n, k, kind):
 
 
 TILES = ["0,8,32", "1,4,3", "1,8,32", "1,8,5", "1,16,7", "1,8,1", "1,12,32", "1,12,5",
-         "2,8,5", "2,16,7", "2,12,32", "2,8,1", "2,16,64", "2,4,3,256", "2,4,7,128", "2,8,32,128", "2,4,2,256"]
+         "2,8,5", "2,16,7", "2,12,32", "2,8,1", "2,16,64", "2,4,3,256", "2,4,7,128", "2,8,32,128", "2,4,2,256",
+         "3,8,5", "3,12,32", "3,16,7", "3,8,1", "3,16,5,1", "3,16,32,3", "3,16,1,3"]
 
 
 @pytest.mark.parametrize("tile", TILES)
@@ -254,6 +255,31 @@ def test_staged_laplacian_reuse(ext):
         assert rel_err(out, orc.laplacian(og, 1, w, [11, 9, 13])) <= TOL
 
 
+@pytest.mark.parametrize("n,k", [(3, 1), (3, 2), (3, 3), (2, 1), (2, 2)])
+def test_staged_codifferential_reuse(ext, n, k):
+    """StagedCodifferential (codifferential.hpp:321-508): one staged handle
+    run on several forms gives the one-shot codifferential bit for bit and
+    the oracle's within the DEC tolerance"""
+    from math import comb
+
+    rng = np.random.default_rng(20 + 3 * n + k)
+    extents = [9, 7, 8][:n]
+    og, gg = make_pair(ext, n, extents, "diagonal", rng)
+    oext = [e - 1 for e in extents]
+    out = torch.empty(tuple(oext) + (comb(n, k - 1),), dtype=torch.float64, device="cuda")
+    one = torch.empty_like(out)
+    w0 = cuda(rng.standard_normal(tuple(extents) + (comb(n, k),)))
+    staged = ext.make_staged_codifferential(gg, k, out, w0)
+    for _ in range(3):
+        w = rng.standard_normal(tuple(extents) + (comb(n, k),))
+        staged.run(out, cuda(w))
+        ext.codifferential(gg, k, one, cuda(w))
+        assert torch.equal(out, one)
+        assert rel_err(out, orc.codifferential(og, k, w, oext)) <= TOL
+    with pytest.raises(ValueError):
+        staged.run(out[:-1], cuda(w))
+
+
 def config2_grid(ext, n=256, seed=2):
     """Geometric spacing ratio 1.01 per cell on [0, 1] and diagonal metric
     g_dd = 1 + x_d² (the 256³ throughput workload)."""
@@ -268,14 +294,18 @@ def config2_grid(ext, n=256, seed=2):
     return grid, c, g
 
 
-def test_fused_3d_1form_laplacian_full_size(ext, monkeypatch):
-    """At 256³: the x-marching kernel against the direct kernel everywhere,
-    and against the oracle on crops whose local result is exactly the full
-    grid's (interior crops, and crops touching the upper faces)."""
+@pytest.mark.parametrize("tile", [None, "2,12,32", "3,16,32"])
+def test_fused_3d_1form_laplacian_full_size(ext, monkeypatch, tile):
+    """At 256³: the x-marching kernels (the default and each variant) against
+    the direct kernel everywhere, and against the oracle on crops whose local
+    result is exactly the full grid's (interior crops, and crops touching the
+    upper faces)."""
     grid, c, g = config2_grid(ext)
     gen = torch.Generator(device="cuda").manual_seed(11)
     w = torch.randn((256, 256, 256, 3), generator=gen, dtype=torch.float64, device="cuda")
-    out = torch.empty((255, 255, 255, 3), dtype=torch.float64, device="cuda")
+    out = torch.full((255, 255, 255, 3), np.nan, dtype=torch.float64, device="cuda")
+    if tile is not None:
+        monkeypatch.setenv("SIL_LAP3D_TILE", tile)
     ext.laplacian(grid, 1, out, w)
     monkeypatch.setenv("SIL_LAP3D_TILE", "0,8,32")
     out_d = torch.empty_like(out)

@@ -206,3 +206,122 @@ def test_mixed_precision_continues_in_fp64_when_it_stalls():
     assert d["true_relative_residual"] <= 1e-10 * (1 + 1e-6)
     r = b - op.apply(x)
     assert (torch.linalg.norm(r) / torch.linalg.norm(b)).item() <= 1.0001e-10
+
+
+def host_csr(A):
+    rp, cc, vv = A.csr()
+    return rp.cpu().numpy().astype(np.int64), cc.cpu().numpy(), vv.cpu().numpy()
+
+
+@pytest.mark.parametrize("shape", [(9, 7), (300, 5), (130, 131)])
+@pytest.mark.parametrize("criterion", ["moments", "potential", "potential_and_moments"])
+def test_assembled_operator_matches_the_oracle(shape, criterion):
+    """assemble_matrix_data of the linear operator: the oracle's CSR arrays
+    bit for bit, and its product the matrix-free apply"""
+    import similie_amd.solvers as S
+
+    print(f"ms2d assemble shape={shape} criterion={criterion}", flush=True)
+    rng = np.random.default_rng(shape[0] * 3 + shape[1])
+    n = shape[0] * shape[1]
+    coords = grid(shape, rng)
+    mu = rng.uniform(1, 5, n) * MU0
+    op = S.MagnetostaticsOperator2D(*coords, dev(mu), criterion=criterion)
+    A = op.assemble()
+    rp, cc, vv = host_csr(A)
+    erp, ecc, evv = orc.Magnetostatics2D(shape, mu, criterion=criterion).csr()
+    np.testing.assert_array_equal(rp, erp)
+    np.testing.assert_array_equal(cc, ecc)
+    np.testing.assert_array_equal(vv, evv)
+    x = dev(rng.standard_normal(n))
+    y_mf, y_asm = op.apply(x), A.apply(x)
+    torch.cuda.synchronize()
+    assert rel(y_asm, y_mf.cpu().numpy()) <= 1e-13
+
+
+@pytest.mark.parametrize("level", [0.05, 0.8])
+@pytest.mark.parametrize("criterion", ["moments", "potential_and_moments"])
+def test_assembled_jacobian_matches_the_oracle(level, criterion):
+    """assemble_matrix_data(model, state): the oracle's Jacobian entry by
+    entry (relative to the largest, 1e-12), and its product apply_jacobian"""
+    import scipy.sparse as sps
+    import similie_amd.solvers as S
+
+    print(f"ms2d assemble jacobian level={level} criterion={criterion}", flush=True)
+    rng = np.random.default_rng(int(level * 1000) + 5)
+    shape = (70, 45)
+    coords, ferro, mu, x = nonlinear_case(shape, rng, level)
+    n = x.size
+    op = S.NonlinearMagnetostaticsOperator2D(*coords, dev(mu), dev(ferro), CURVE["b"], CURVE["h"],
+                                             criterion=criterion)
+    J = op.assemble_jacobian(dev(x))
+    rp, cc, vv = host_csr(J)
+    erp, ecc, evv = orc.Magnetostatics2D(shape, mu, ferro, CURVE["b"], CURVE["h"], criterion=criterion).csr(x)
+    got = sps.csr_matrix((vv, cc, rp), shape=(n, n))
+    ref = sps.csr_matrix((evv, ecc, erp), shape=(n, n))
+    assert abs(got - ref).max() <= 1e-12 * abs(ref).max()
+    assert np.all(np.diff(rp) <= 7)
+    v = dev(rng.standard_normal(n))
+    jv, av = op.apply_jacobian(dev(x), v), J.apply(v)
+    torch.cuda.synchronize()
+    assert rel(av, jv.cpu().numpy()) <= 1e-12
+
+
+def test_spd_isai_of_the_2d_operator_matches_the_oracle():
+    """SpdIsai generated from the linear 2D operator (assembled for the
+    purpose) against the oracle's factor of the oracle's matrix; CG with it
+    reaches the tolerance in fewer iterations than Jacobi"""
+    import similie_amd.solvers as S
+
+    print("ms2d spd_isai 90x70", flush=True)
+    rng = np.random.default_rng(21)
+    shape = (90, 70)
+    n = shape[0] * shape[1]
+    coords = grid(shape, rng)
+    mu = np.where(rng.random(n) < 0.3, 1000.0, 1.0) * MU0
+    op = S.MagnetostaticsOperator2D(*coords, dev(mu))
+    grp, gc, gv = host_csr(S.spd_isai(op))
+    erp, ecc, evv = orc.spd_isai(*orc.Magnetostatics2D(shape, mu).csr())
+    np.testing.assert_array_equal(grp, erp)
+    np.testing.assert_array_equal(gc, ecc)
+    for r in range(n):
+        seg = slice(erp[r], erp[r + 1])
+        assert np.max(np.abs(gv[seg] - evv[seg])) <= 1e-12 * np.max(np.abs(evv[seg]))
+    b = np.zeros(n)
+    b.reshape(shape[1], shape[0])[1:-1, 1:-1] = rng.standard_normal((shape[1] - 2, shape[0] - 2))
+    ref = orc.Magnetostatics2D(shape, mu)
+    iters = {}
+    for pc in ("spd_isai", "jacobi"):
+        st = S.StrongFormulationSolverSettings(max_iterations=20000, relative_tolerance=1e-10, preconditioner=pc)
+        x, d = S.minimize_strong_formulation_residual(op, dev(b), settings=st)
+        assert d["converged"] and d["verified"], (pc, d)
+        r = b - ref.apply(x.cpu().numpy())
+        assert np.linalg.norm(r) <= 1.0001e-10 * np.linalg.norm(b), pc
+        iters[pc] = d["iterations"]
+    assert iters["spd_isai"] < iters["jacobi"], iters
+
+
+def test_newton_with_spd_isai_matches_the_oracle():
+    """Newton with SpdIsai of the 2D Jacobian assembled at every step: the
+    oracle's step count (±1), a verified solution at the tolerance"""
+    import similie_amd.solvers as S
+
+    print("ms2d Newton spd_isai", flush=True)
+    shape = (20, 18)
+    n = 360
+    ferro = np.zeros(n)
+    ferro.reshape(18, 20)[6:12, 7:13] = 1.0
+    coords = [np.arange(20.0), np.arange(18.0)]
+    jz = np.zeros(19 * 17)
+    jz.reshape(17, 19)[7:10, 2:5] = 1e5
+    jz.reshape(17, 19)[7:10, 14:17] = -1e5
+    op = S.NonlinearMagnetostaticsOperator2D(*coords, dev(np.full(n, MU0)), dev(ferro), CURVE["b"], CURVE["h"])
+    b = op.rhs(dev(jz))
+    st = S.StrongFormulationSolverSettings(max_iterations=5000, relative_tolerance=1e-10, preconditioner="spd_isai")
+    x, d = S.minimize_strong_formulation_residual(op, b, settings=st)
+    ref = orc.Magnetostatics2D(shape, np.full(n, MU0), ferro, CURVE["b"], CURVE["h"])
+    xr, dr = ref.newton(b.cpu().numpy(), 1e-10, 5000, jacobi="spd_isai")
+    assert d["converged"] and d["verified"] and dr["converged"], (d, dr)
+    assert abs(d["outer_iterations"] - dr["newton_steps"]) <= 1, (d, dr)
+    assert rel(x, xr) <= 1e-7
+    r = b.cpu().numpy() - ref.apply(x.cpu().numpy())
+    assert np.linalg.norm(r) <= 1.0001e-10 * np.linalg.norm(b.cpu().numpy())

@@ -87,7 +87,10 @@ def test_cg_fp64_poisson_matches_oracle(S, precond):
     if precond == "identity":
         hx = np.full(65, 1 / 64)
         xr, it, rn, bn = orc.poisson2d_cg(hx, hx, kappa, b, 1e-10, 5000)
-        assert abs(it - dg["iterations"]) <= 2
+        # the count at a fixed tolerance moves with the rounding order of the
+        # dot products (the GPU's block partials and the oracle's fixed
+        # 4096-blocks are both deterministic, and different): ±1 %
+        assert abs(it - dg["iterations"]) <= max(2, it // 100), (it, dg["iterations"])
         err = np.max(np.abs(x.cpu().numpy().reshape(65, 65) - xr)) / np.max(np.abs(xr))
         assert err <= 1e-8
 

@@ -163,3 +163,70 @@ def test_newton_converges_on_the_eicore_curve(jacobi):
     r = np.linalg.norm(b - nl.apply(x))
     assert r <= 1e-10 * d["initial_residual_l2"] * 1.0001
     assert d["final_residual_l2"] == pytest.approx(r, rel=1e-6)
+
+
+def csr_dense(rp, cols, vals, n):
+    d = np.zeros((n, n))
+    for r in range(n):
+        for q in range(rp[r], rp[r + 1]):
+            d[r, cols[q]] += vals[q]
+    return d
+
+
+@pytest.mark.parametrize("scale", [0.05, 1.0])
+def test_jacobian_assembly_equals_the_matrix_free_jacobian(scale):
+    """The assembled J(state) is the matrix-free J(state) column by column,
+    its rows ascend, and the pattern holds at most 23 columns per row (the
+    three moments of the four sampled nodes that reach a DOF)."""
+    rng = np.random.default_rng(int(10 * scale) + 3)
+    shape = (5, 4, 6)
+    nl, _, _ = problem(shape, rng)
+    x = state(shape, rng, scale)
+    rp, cols, vals = nl.jacobian_csr(x)
+    n = nl.n
+    A = csr_dense(rp, cols, vals, n)
+    e = np.zeros(n)
+    for j in range(n):
+        e[j] = 1.0
+        col = nl.jacobian_apply(x, e)
+        e[j] = 0.0
+        assert np.max(np.abs(A[:, j] - col)) <= 1e-13 * max(np.max(np.abs(col)), 1e-300), j
+    assert all(np.all(np.diff(cols[rp[r]:rp[r + 1]]) > 0) for r in range(n))
+    assert np.max(np.diff(rp)) <= 23
+
+
+def test_jacobian_assembly_of_a_linear_curve_is_the_linear_assembly():
+    """Without ferromagnetic nodes the Jacobian is the linear operator: the
+    assembly reduces to orc_ms_assemble_csr entry for entry."""
+    rng = np.random.default_rng(12)
+    shape = (6, 5, 7)
+    nl, coords, mu = problem(shape, rng)
+    nl.ferro = np.zeros_like(nl.ferro)
+    x = state(shape, rng)
+    rp, cols, vals = nl.jacobian_csr(x)
+    rp0, cols0, vals0 = orc.ms_assemble_csr(coords, mu, nl.penalty)
+    assert np.array_equal(rp, rp0) and np.array_equal(cols, cols0)
+    assert np.array_equal(vals, vals0)
+
+
+def test_newton_with_spd_isai_of_the_assembled_jacobian():
+    """inductor.silpro's setting (SpdIsai with Newton): the Jacobian is
+    assembled at every step and its SpdIsai preconditions the inner CG.  On a
+    grid of equal spacings in all directions the Jacobian is symmetric
+    positive definite; Newton takes the Jacobi path's steps, with fewer inner
+    iterations."""
+    rng = np.random.default_rng(9)
+    shape = (7, 7, 7)
+    nl, _, _ = problem(shape, rng)
+    nl.coords = [np.linspace(0.0, 1.0, n) for n in shape]
+    b = rhs_for(nl, rng)
+    xj, dj = nl.newton(b, 1e-10, 20000, jacobi=True)
+    xi, di = nl.newton(b, 1e-10, 20000, jacobi="spd_isai")
+    assert di["converged"] and dj["converged"]
+    assert abs(di["newton_steps"] - dj["newton_steps"]) <= 1
+    assert di["iterations"] < dj["iterations"]
+    r = np.linalg.norm(b - nl.apply(xi))
+    assert r <= 1e-10 * di["initial_residual_l2"] * 1.0001
+    rp, cols, vals = nl.jacobian_csr(xi)
+    A = csr_dense(rp, cols, vals, nl.n)
+    assert np.max(np.abs(A - A.T)) <= 1e-14 * np.max(np.abs(A))
