@@ -151,6 +151,26 @@ int sil_make_staged_laplacian(sil_staged_laplacian* out, sil_grid grid, int32_t 
 int sil_staged_laplacian_run(sil_staged_laplacian op, double* out, const double* in, void* stream);
 int sil_staged_laplacian_destroy(sil_staged_laplacian op);
 
+/* The autotuner of the fused 3D 1-form Laplacian (no reference
+ * counterpart: the reference has one kernel shape): times each candidate
+ * kernel shape `repeats` times on the staged operator's own arrays (one
+ * warm-up run each, HIP events on `stream`), keeps the fastest for later
+ * runs, leaves its result in `out` and reports it.  variant 2 = pipelined
+ * x-march (lanes_or_depth: z-lanes per row), 3 = staged x-march
+ * (lanes_or_depth: planes of loads in flight).  Other paths, or a tile
+ * fixed by SIL_LAP3D_TILE, are reported unchanged (microseconds 0).
+ * Synchronises `stream`. */
+typedef struct sil_lap3d_tile {
+    int32_t variant;
+    int32_t rows;
+    int32_t xchunk;
+    int32_t lanes_or_depth;
+    int32_t nontemporal;
+    double microseconds;
+} sil_lap3d_tile;
+int sil_staged_laplacian_autotune(sil_staged_laplacian op, double* out, const double* in, int32_t repeats,
+                                  sil_lap3d_tile* chosen, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* linear operators and the CG solve                                   */
 /* ------------------------------------------------------------------ */
@@ -579,6 +599,21 @@ int sil_batched_poisson2d_set_block(sil_batched_poisson h, int32_t block);
  * problem and keeps the fastest */
 int sil_batched_poisson2d_autotune(sil_batched_poisson h, const double* rhs, int32_t probe_iterations,
                                    int32_t* chosen_block, double* seconds_per_candidate);
+/* the wider autotuner: the wide-path threshold (problems of at least
+ * wide_side nodes per side take the multi-workgroup path; candidates 96,
+ * 128, 192, 256, 384 and none, one per distinct split), the workgroups per
+ * wide problem (the default of four nodes per thread, its half and quarter)
+ * and the persistent kernel's workgroup size, each timed on
+ * `probe_iterations` iterations of every problem; keeps the fastest */
+typedef struct sil_batch_shape {
+    int32_t wide_side;
+    int32_t wide_workgroups;
+    int32_t block;
+    int32_t wide_problems;
+    double seconds;
+} sil_batch_shape;
+int sil_batched_poisson2d_autotune_shape(sil_batched_poisson h, const double* rhs, int32_t probe_iterations,
+                                         sil_batch_shape* chosen);
 /* results: host array of nproblems entries (or NULL); seconds: device time
  * of the graph launch (or NULL) */
 int sil_batched_poisson2d_solve(sil_batched_poisson h, const double* rhs, double* solution,
@@ -606,6 +641,10 @@ int sil_unet_destroy(sil_unet net);
 int sil_unet_get_params(sil_unet net, float* params, void* stream);
 int sil_unet_set_params(sil_unet net, const float* params, void* stream);
 int sil_unet_forward(sil_unet net, int32_t batch, const float* input, float* output, void* stream);
+/* the same output as sil_unet_forward, bit for bit, for inference: the
+ * input layer and layer 1 run as one kernel and the input layer's
+ * activation is not stored (no sil_unet_backward may follow) */
+int sil_unet_infer(sil_unet net, int32_t batch, const float* input, float* output, void* stream);
 /* the stored BF16 activation `which` of the preceding sil_unet_forward, as
  * FP32 [batch][H_l][W_l][32] (verification of single layers): 0 a0, 1 e0,
  * 2 a1, 3 e1, 4 a2, 5 e2, 6 a3, 7 b3, 8 c2, 9 f2, 10 c1, 11 f1, 12 c0 — the

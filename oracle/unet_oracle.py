@@ -127,11 +127,14 @@ def conv32(x, w, b):
 
 
 def conv_in_fma32(x, w, b):
-    """The first layer as the GPU computes it: per output channel an FP32
-    fused multiply-add chain a = b; a = fma(w[t][c][ci], x_t[ci], a) over
-    taps t = 0..8 and input channels ci = 0, 1.  Each step is emulated in
-    FP64 (the product of two FP32 values is exact there) with one rounding
-    to FP32."""
+    """The first layer in FP32 arithmetic: per output channel the chain
+    a = b; a = fma(w[t][c][ci], x_t[ci], a) over taps t = 0..8 and input
+    channels ci = 0, 1, each step emulated in FP64 (the product of two FP32
+    values is exact there) with one rounding to FP32.  The GPU forms the
+    same sum on its FP32 matrix cores (bias first, one tap — both input
+    channels — per step, taps in order), which differs from this chain only
+    in FP32 rounding: after the BF16 rounding of the activation the two
+    agree on every element of the layer-by-layer test."""
     B, H, W, _ = x.shape
     x = f32(x)
     w = f32(w)
@@ -197,8 +200,8 @@ def forward(params, x, emulate=False, reordered=False):
 
     emulate=True computes as the GPU kernels do: FP32 input and first/last
     layer weights, BF16 weights of the 32->32 layers, BF16 activations and
-    FP32 accumulation (the first layer as the GPU's chain of fused
-    multiply-adds, bit for bit).  BF16 storage makes a network of this depth sensitive
+    FP32 accumulation (the first layer as an FP32 multiply-add chain in the
+    GPU's tap order).  BF16 storage makes a network of this depth sensitive
     to the last bit of each accumulation (one in ten activations of the
     coarse levels rounds to the other neighbouring BF16 value when only the
     accumulation order changes), so only an oracle that accumulates in FP32

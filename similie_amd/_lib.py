@@ -86,6 +86,27 @@ class AirGapResult(ctypes.Structure):
     ]
 
 
+class Lap3dTile(ctypes.Structure):
+    _fields_ = [
+        ("variant", ctypes.c_int32),
+        ("rows", ctypes.c_int32),
+        ("xchunk", ctypes.c_int32),
+        ("lanes_or_depth", ctypes.c_int32),
+        ("nontemporal", ctypes.c_int32),
+        ("microseconds", ctypes.c_double),
+    ]
+
+
+class BatchShape(ctypes.Structure):
+    _fields_ = [
+        ("wide_side", ctypes.c_int32),
+        ("wide_workgroups", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("wide_problems", ctypes.c_int32),
+        ("seconds", ctypes.c_double),
+    ]
+
+
 class SilproSettings(ctypes.Structure):
     _fields_ = [
         ("physics", ctypes.c_int32),
@@ -163,6 +184,7 @@ SIGNATURES = {
     "sil_make_staged_laplacian": (ctypes.c_int, [ctypes.POINTER(_P), _P, _I32, _VP, _VP]),
     "sil_staged_laplacian_run": (ctypes.c_int, [_P, _P, _P, _P]),
     "sil_staged_laplacian_destroy": (ctypes.c_int, [_P]),
+    "sil_staged_laplacian_autotune": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.POINTER(Lap3dTile), _P]),
     "sil_make_staged_codifferential": (ctypes.c_int, [ctypes.POINTER(_P), _P, _I32, _VP, _VP]),
     "sil_staged_codifferential_run": (ctypes.c_int, [_P, _P, _P, _P]),
     "sil_staged_codifferential_destroy": (ctypes.c_int, [_P]),
@@ -234,6 +256,7 @@ SIGNATURES = {
         ctypes.c_int,
         [_P, _P, _I32, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)],
     ),
+    "sil_batched_poisson2d_autotune_shape": (ctypes.c_int, [_P, _P, _I32, ctypes.POINTER(BatchShape)]),
     "sil_batched_poisson2d_solve": (
         ctypes.c_int,
         [_P, _P, _P, ctypes.POINTER(SolverSettings), ctypes.POINTER(BatchResult), ctypes.POINTER(ctypes.c_double), _P],
@@ -246,6 +269,7 @@ SIGNATURES = {
     "sil_unet_set_params": (ctypes.c_int, [_P, _P, _P]),
     "sil_unet_activation": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
     "sil_unet_forward": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
+    "sil_unet_infer": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
     "sil_unet_backward": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P]),
     "sil_unet_loss": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "sil_unet_train_step": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P, ctypes.c_float, _P, _P]),

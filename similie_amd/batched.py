@@ -49,6 +49,17 @@ class BatchedPoisson2D:
         self.block = chosen.value
         return chosen.value, {256: secs[0], 512: secs[1], 1024: secs[2]}
 
+    def autotune_shape(self, rhs, probe_iterations=20):
+        """the wider autotuner (sil_batched_poisson2d_autotune_shape): the
+        wide-path threshold, the workgroups per wide problem and the
+        persistent workgroup size; returns the chosen shape"""
+        sh = L.BatchShape()
+        L.check(L.lib().sil_batched_poisson2d_autotune_shape(self.handle, L.ptr(rhs), int(probe_iterations),
+                                                             ctypes.byref(sh)))
+        self.block = sh.block
+        return {"wide_side": sh.wide_side, "wide_workgroups": sh.wide_workgroups, "block": sh.block,
+                "wide_problems": sh.wide_problems, "probe_seconds": sh.seconds}
+
     def solve(self, rhs, solution=None, settings=None, stream=None):
         settings = settings or StrongFormulationSolverSettings(relative_tolerance=1e-10, precision="mixed",
                                                                preconditioner="jacobi", max_iterations=20000)

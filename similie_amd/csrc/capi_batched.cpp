@@ -22,6 +22,7 @@ struct sil_batched_poisson_s {
     // wide path: problems of side >= wide_side, each over nbx workgroups
     int wide_side = 192;
     int nwide = 0, nbx = 1;
+    int nbx_max = 1; // workgroups per wide problem the partial-sum array holds
     int* wide_ids_dev = nullptr;
     WideState* wide_state = nullptr;
     double* wide_part = nullptr;
@@ -228,6 +229,86 @@ int solve_wide(sil_batched_poisson h, double tol, int max_it, int replace, hipSt
     return wide_finish(W, s);
 }
 
+// splits the problems between the persistent kernel (side < wide_side,
+// largest first) and the wide path, and sizes the wide path's arrays;
+// drops the captured graphs (they hold the previous split)
+int partition(sil_batched_poisson h, int wide_side)
+{
+    for (void* ptr : {(void*)h->order_dev, (void*)h->wide_ids_dev, (void*)h->wide_state, (void*)h->wide_part,
+                      (void*)h->wide_done})
+        if (ptr) (void)hipFree(ptr);
+    h->order_dev = nullptr;
+    h->wide_ids_dev = nullptr;
+    h->wide_state = nullptr;
+    h->wide_part = nullptr;
+    h->wide_done = nullptr;
+    if (h->wide_done_host) (void)hipHostFree(h->wide_done_host);
+    h->wide_done_host = nullptr;
+    if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    if (h->wide_graph) (void)hipGraphExecDestroy(h->wide_graph);
+    h->graph = nullptr;
+    h->wide_graph = nullptr;
+    h->graph_block = 0;
+    h->wide_side = wide_side;
+    std::vector<int> order, wide;
+    int64_t max_wide_nn = 0;
+    for (int i = 0; i < h->nprob; ++i) {
+        const int side = h->problems[i].side;
+        if (side >= wide_side) {
+            wide.push_back(i);
+            max_wide_nn = std::max<int64_t>(max_wide_nn, (int64_t)side * side);
+        } else {
+            order.push_back(i);
+        }
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return h->problems[a].side > h->problems[b].side; });
+    h->nsmall = (int)order.size();
+    h->nwide = (int)wide.size();
+    // four nodes per thread of the largest wide problem, at most 256 workgroups per problem
+    h->nbx = (int)std::min<int64_t>(256, std::max<int64_t>(1, (max_wide_nn + 1023) / 1024));
+    h->nbx_max = h->nbx;
+    int rc;
+    if ((rc = dalloc(&h->order_dev, std::max(h->nsmall, 1))) || (rc = dalloc(&h->wide_ids_dev, std::max(h->nwide, 1))) ||
+        (rc = dalloc(&h->wide_state, std::max(h->nwide, 1))) ||
+        (rc = dalloc(&h->wide_part, (int64_t)std::max(h->nwide, 1) * h->nbx_max * 2)) ||
+        (rc = dalloc(&h->wide_done, std::max(h->nwide, 1))))
+        return rc;
+    if (hipHostMalloc(reinterpret_cast<void**>(&h->wide_done_host), sizeof(int) * std::max(h->nwide, 1)) !=
+        hipSuccess) {
+        h->wide_done_host = nullptr;
+        set_error("batched: pinned allocation failed");
+        return SIL_ERR_HIP;
+    }
+    hipError_t e = hipSuccess;
+    if (h->nsmall) e = hipMemcpy(h->order_dev, order.data(), sizeof(int) * h->nsmall, hipMemcpyHostToDevice);
+    if (e == hipSuccess && h->nwide)
+        e = hipMemcpy(h->wide_ids_dev, wide.data(), sizeof(int) * h->nwide, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error("batched: %s", hipGetErrorString(e));
+        return SIL_ERR_HIP;
+    }
+    return SIL_OK;
+}
+
+// a fixed number of iterations of every problem (tolerance 0): equal work
+// for every candidate configuration; seconds of the probe
+int probe(sil_batched_poisson h, int iterations, double* seconds)
+{
+    const int replace = 400;
+    int rc = h->nsmall ? capture(h, h->block, 0.0, iterations, replace) : SIL_OK;
+    if (rc) return rc;
+    for (int rep = 0; rep < 2; ++rep) { // a warm-up, then the timed run
+        SIL_CHECK_HIP(hipDeviceSynchronize());
+        const auto t0 = std::chrono::steady_clock::now();
+        if (h->nwide && (rc = solve_wide(h, 0.0, iterations, replace, 0))) return rc;
+        if (h->nsmall) SIL_CHECK_HIP(hipGraphLaunch(h->graph, 0));
+        SIL_CHECK_HIP(hipDeviceSynchronize());
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return SIL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -252,50 +333,25 @@ int sil_batched_poisson2d_create(sil_batched_poisson* out, int32_t nproblems, co
         h->max_side = std::max(h->max_side, (int)sides[i]);
     }
     h->total = off;
-    if (const char* e = std::getenv("SIL_BATCH_WIDE_SIDE")) h->wide_side = std::max(3, std::atoi(e));
-    // small problems: one workgroup each in the persistent kernel, largest
-    // first; large ones: the wide path
-    std::vector<int> order, wide;
-    int64_t max_wide_nn = 0;
-    for (int i = 0; i < nproblems; ++i) {
-        if (sides[i] >= h->wide_side) {
-            wide.push_back(i);
-            max_wide_nn = std::max<int64_t>(max_wide_nn, (int64_t)sides[i] * sides[i]);
-        } else {
-            order.push_back(i);
-        }
-    }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sides[a] > sides[b]; });
-    h->nsmall = (int)order.size();
-    h->nwide = (int)wide.size();
-    // four nodes per thread of the largest wide problem, at most 256 workgroups per problem
-    h->nbx = (int)std::min<int64_t>(256, std::max<int64_t>(1, (max_wide_nn + 1023) / 1024));
+    int wide_side = 192;
+    if (const char* e = std::getenv("SIL_BATCH_WIDE_SIDE")) wide_side = std::max(3, std::atoi(e));
     int rc;
-    if ((rc = dalloc(&h->problems_dev, nproblems)) || (rc = dalloc(&h->order_dev, std::max(h->nsmall, 1))) ||
-        (rc = dalloc(&h->queue, 1)) || (rc = dalloc(&h->wx32, off)) || (rc = dalloc(&h->wy32, off)) ||
-        (rc = dalloc(&h->r, off)) || (rc = dalloc(&h->p, off)) || (rc = dalloc(&h->q, off)) ||
-        (rc = dalloc(&h->wx64, off)) || (rc = dalloc(&h->wy64, off)) || (rc = dalloc(&h->b, off)) ||
-        (rc = dalloc(&h->x, off)) || (rc = dalloc(&h->dinv32, off)) || (rc = dalloc(&h->results_dev, nproblems)) ||
-        (rc = dalloc(&h->wide_ids_dev, std::max(h->nwide, 1))) ||
-        (rc = dalloc(&h->wide_state, std::max(h->nwide, 1))) ||
-        (rc = dalloc(&h->wide_part, (int64_t)std::max(h->nwide, 1) * h->nbx * 2)) ||
-        (rc = dalloc(&h->wide_done, std::max(h->nwide, 1)))) {
+    if ((rc = dalloc(&h->problems_dev, nproblems)) || (rc = dalloc(&h->queue, 1)) || (rc = dalloc(&h->wx32, off)) ||
+        (rc = dalloc(&h->wy32, off)) || (rc = dalloc(&h->r, off)) || (rc = dalloc(&h->p, off)) ||
+        (rc = dalloc(&h->q, off)) || (rc = dalloc(&h->wx64, off)) || (rc = dalloc(&h->wy64, off)) ||
+        (rc = dalloc(&h->b, off)) || (rc = dalloc(&h->x, off)) || (rc = dalloc(&h->dinv32, off)) ||
+        (rc = dalloc(&h->results_dev, nproblems))) {
         release(h);
         return rc;
     }
-    if (hipHostMalloc(reinterpret_cast<void**>(&h->wide_done_host), sizeof(int) * std::max(h->nwide, 1)) !=
-        hipSuccess) {
-        h->wide_done_host = nullptr;
+    // small problems: one workgroup each in the persistent kernel, largest
+    // first; large ones: the wide path
+    if ((rc = partition(h, wide_side))) {
         release(h);
-        set_error("batched_poisson2d_create: pinned allocation failed");
-        return SIL_ERR_HIP;
+        return rc;
     }
     hipError_t e = hipMemcpy(h->problems_dev, h->problems.data(), sizeof(BatchProblem) * nproblems,
                              hipMemcpyHostToDevice);
-    if (e == hipSuccess && h->nsmall)
-        e = hipMemcpy(h->order_dev, order.data(), sizeof(int) * h->nsmall, hipMemcpyHostToDevice);
-    if (e == hipSuccess && h->nwide)
-        e = hipMemcpy(h->wide_ids_dev, wide.data(), sizeof(int) * h->nwide, hipMemcpyHostToDevice);
     int dev = 0;
     if (e == hipSuccess) e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -400,6 +456,54 @@ int sil_batched_poisson2d_autotune(sil_batched_poisson h, const double* rhs, int
     }
     h->block = best_block;
     *chosen_block = best_block;
+    return SIL_OK;
+}
+
+int sil_batched_poisson2d_autotune_shape(sil_batched_poisson h, const double* rhs, int32_t probe_iterations,
+                                         sil_batch_shape* chosen)
+{
+    SIL_REQUIRE(h && rhs && chosen, SIL_ERR_INVALID, "batched_poisson2d_autotune_shape: null argument");
+    SIL_REQUIRE(probe_iterations >= 1, SIL_ERR_INVALID, "batched_poisson2d_autotune_shape: probe iterations < 1");
+    SIL_CHECK_HIP(hipMemcpy(h->b, rhs, sizeof(double) * h->total, hipMemcpyDeviceToDevice));
+    // wide-path threshold, workgroups per wide problem, persistent workgroup size
+    const int sides[] = {96, 128, 192, 256, 384, h->max_side + 1};
+    sil_batch_shape best{};
+    best.seconds = 1e300;
+    std::vector<int> seen_splits;
+    for (int ws : sides) {
+        int nw = 0;
+        for (const BatchProblem& p : h->problems) nw += p.side >= ws ? 1 : 0;
+        if (std::find(seen_splits.begin(), seen_splits.end(), nw) != seen_splits.end()) continue;
+        seen_splits.push_back(nw);
+        int rc = partition(h, ws);
+        if (rc) return rc;
+        const int nbx0 = h->nbx_max;
+        const int nbx_c[3] = {nbx0, std::max(1, nbx0 / 2), std::max(1, nbx0 / 4)};
+        const int blocks[3] = {256, 512, 1024};
+        for (int a = 0; a < (h->nwide ? 3 : 1); ++a) {
+            if (a > 0 && nbx_c[a] == nbx_c[a - 1]) continue;
+            h->nbx = nbx_c[a];
+            if (h->wide_graph) (void)hipGraphExecDestroy(h->wide_graph);
+            h->wide_graph = nullptr;
+            for (int c = 0; c < (h->nsmall ? 3 : 1); ++c) {
+                h->block = blocks[c];
+                double dt = 0.0;
+                if ((rc = probe(h, probe_iterations, &dt))) return rc;
+                if (dt < best.seconds) {
+                    best.seconds = dt;
+                    best.wide_side = ws;
+                    best.wide_workgroups = h->nbx;
+                    best.block = h->block;
+                    best.wide_problems = h->nwide;
+                }
+            }
+        }
+    }
+    int rc = partition(h, best.wide_side);
+    if (rc) return rc;
+    h->nbx = std::min(best.wide_workgroups, h->nbx_max);
+    h->block = best.block;
+    *chosen = best;
     return SIL_OK;
 }
 

@@ -402,4 +402,76 @@ int sil_staged_laplacian_destroy(sil_staged_laplacian op)
     return SIL_OK;
 }
 
+int sil_staged_laplacian_autotune(sil_staged_laplacian op, double* out, const double* in, int32_t repeats,
+                                  sil_lap3d_tile* chosen, void* stream)
+{
+    SIL_REQUIRE(op != nullptr && out != nullptr && in != nullptr, SIL_ERR_INVALID, "laplacian autotune: null argument");
+    SIL_REQUIRE(repeats >= 1, SIL_ERR_INVALID, "laplacian autotune: repeats must be at least 1");
+    auto report = [&](const FusedTile& t, double us) {
+        if (!chosen) return;
+        chosen->variant = t.variant;
+        chosen->rows = t.ty;
+        chosen->xchunk = t.xchunk;
+        chosen->lanes_or_depth = t.tz;
+        chosen->nontemporal = t.nt;
+        chosen->microseconds = us;
+    };
+    // only the fused 3D 1-form path has shapes to choose; the environment's
+    // explicit tile is kept
+    if (op->path != 2 || std::getenv("SIL_LAP3D_TILE") != nullptr) {
+        report(op->tile, 0.0);
+        return SIL_OK;
+    }
+    hipStream_t s = as_stream(stream);
+    // staged x-march (z neighbours by DPP): rows, planes in flight, planes per
+    // workgroup; pipelined x-march: rows of 64 lanes
+    const FusedTile candidates[] = {
+        {FUSED_STAGE, 2, 16, 32, 1}, {FUSED_STAGE, 1, 16, 32, 1}, {FUSED_STAGE, 2, 12, 32, 1},
+        {FUSED_STAGE, 2, 8, 32, 1},  {FUSED_STAGE, 2, 16, 24, 1}, {FUSED_STAGE, 2, 16, 40, 1},
+        {FUSED_STAGE, 2, 16, 32, 0}, {FUSED_PIPE, 64, 12, 32, 1}, {FUSED_PIPE, 64, 16, 32, 0},
+        {FUSED_PIPE, 64, 8, 32, 0},
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SIL_CHECK_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_error("laplacian autotune: event creation failed");
+        return SIL_ERR_HIP;
+    }
+    const FusedGeom g = fused_geom(op->grid);
+    FusedTile best = op->tile;
+    double best_us = 0.0;
+    int rc = SIL_OK;
+    for (const FusedTile& t : candidates) {
+        if (launch_laplacian_3d_1form(g, in, out, op->out_view.extent, t, s) != SIL_OK) continue; // warm-up
+        if (hipEventRecord(e0, s) != hipSuccess) {
+            rc = SIL_ERR_HIP;
+            break;
+        }
+        int ok = SIL_OK;
+        for (int r = 0; r < repeats && ok == SIL_OK; ++r) ok = launch_laplacian_3d_1form(g, in, out, op->out_view.extent, t, s);
+        float ms = 0.f;
+        if (ok != SIL_OK || hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = SIL_ERR_HIP;
+            break;
+        }
+        const double us = 1000.0 * ms / repeats;
+        if (best_us == 0.0 || us < best_us) {
+            best_us = us;
+            best = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != SIL_OK) {
+        set_error("laplacian autotune: timing failed");
+        return rc;
+    }
+    op->tile = best;
+    report(best, best_us);
+    // the output holds the chosen kernel's result
+    return launch_laplacian_3d_1form(g, in, out, op->out_view.extent, best, s);
+}
+
 }  // extern "C"

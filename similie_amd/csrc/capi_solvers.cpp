@@ -118,7 +118,7 @@ int check_curve(const double* bh_b, int32_t nsamples)
 // operator of a nonlinear operator over nn nodes (InterpolatedNonlinearBHCurve,
 // nonlinear_magnetostatics.py:26-55); on failure the caller frees op
 int attach_material(sil_operator op, int64_t nn, const double* ferromagnetic_nodal, const double* bh_b,
-                    const double* bh_h, int32_t nsamples)
+                    const double* bh_h, int32_t nsamples, bool three_d)
 {
     BhCurve c{};
     c.n = nsamples;
@@ -134,12 +134,16 @@ int attach_material(sil_operator op, int64_t nn, const double* ferromagnetic_nod
     nl.curve = dc;
     ok = ok && hipMemcpy(dc, &c, sizeof(BhCurve), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMalloc(&nl.ferro, nn) == hipSuccess;
-    ok = ok && hipMalloc(&nl.st_a, sizeof(double) * nn) == hipSuccess;
-    ok = ok && hipMalloc(&nl.st_c, sizeof(double) * nn) == hipSuccess;
-    for (int q = 0; q < 3; ++q) ok = ok && hipMalloc(&nl.st_b[q], sizeof(double) * nn) == hipSuccess;
-    ok = ok && hipMalloc(&nl.st_a32, sizeof(float) * nn) == hipSuccess;
-    ok = ok && hipMalloc(&nl.st_c32, sizeof(float) * nn) == hipSuccess;
-    for (int q = 0; q < 3; ++q) ok = ok && hipMalloc(&nl.st_b32[q], sizeof(float) * nn) == hipSuccess;
+    if (three_d) {
+        // per-field state: FP64 for the Jacobian diagonal and assembly (and
+        // B_z for the FP64 Jacobian march), a and B_z in FP32 for the FP32
+        // marches; the 2D operator reads only its packed record
+        ok = ok && hipMalloc(&nl.st_a, sizeof(double) * nn) == hipSuccess;
+        ok = ok && hipMalloc(&nl.st_c, sizeof(double) * nn) == hipSuccess;
+        for (int q = 0; q < 3; ++q) ok = ok && hipMalloc(&nl.st_b[q], sizeof(double) * nn) == hipSuccess;
+        ok = ok && hipMalloc(&nl.st_a32, sizeof(float) * nn) == hipSuccess;
+        ok = ok && hipMalloc(&nl.st_b32[2], sizeof(float) * nn) == hipSuccess;
+    }
     if (!ok) {
         set_error("nonlinear_create: device allocation failed");
         return SIL_ERR_HIP;
@@ -233,7 +237,7 @@ int sil_magnetostatics3d_nonlinear_create(sil_operator* out, int64_t nx, int64_t
     rc = sil_magnetostatics3d_create(&op, nx, ny, nz, x_coords, y_coords, z_coords, mu_nodal, gauge_penalty);
     if (rc) return rc;
     op->kind = OP_MS3D_NONLINEAR;
-    rc = attach_material(op, nx * ny * nz, ferromagnetic_nodal, bh_b, bh_h, nsamples);
+    rc = attach_material(op, nx * ny * nz, ferromagnetic_nodal, bh_b, bh_h, nsamples, true);
     if (rc == SIL_OK && (hipMalloc(&op->nl.st_pack, sizeof(double) * 4 * nx * ny * nz) != hipSuccess ||
                          hipMalloc(&op->nl.st_pack32, sizeof(float) * 4 * nx * ny * nz) != hipSuccess)) {
         set_error("magnetostatics3d_nonlinear_create: device allocation failed");
@@ -384,7 +388,7 @@ int sil_magnetostatics2d_nonlinear_create(sil_operator* out, int64_t nx, int64_t
     rc = sil_magnetostatics2d_create(&op, nx, ny, x_coords, y_coords, mu_nodal, criterion);
     if (rc) return rc;
     op->kind = OP_MS2D_NONLINEAR;
-    rc = attach_material(op, nx * ny, ferromagnetic_nodal, bh_b, bh_h, nsamples);
+    rc = attach_material(op, nx * ny, ferromagnetic_nodal, bh_b, bh_h, nsamples, false);
     if (rc == SIL_OK && (hipMalloc(&op->nl.st_pack, sizeof(double) * 4 * nx * ny) != hipSuccess ||
                          hipMalloc(&op->nl.st_pack32, sizeof(float) * 4 * nx * ny) != hipSuccess)) {
         set_error("magnetostatics2d_nonlinear_create: device allocation failed");

@@ -60,9 +60,8 @@ struct StateArgs {
     double* a;
     double* c;
     double* b[3];
-    float* a32;
-    float* c32;
-    float* b32[3];
+    float* a32;  // FP32 scaled response (a), read by the FP32 march
+    float* bz32; // FP32 B_z, read by the FP32 Jacobian march (B_x, B_y are packed)
     NodeState<double>* pack;
     NodeState<float>* pack32;
 };
@@ -100,10 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_nl_state(StateArgs s, const double* 
         s.b[1][t] = by;
         s.b[2][t] = bz;
         s.a32[t] = (float)sa;
-        s.c32[t] = (float)sc;
-        s.b32[0][t] = (float)bx;
-        s.b32[1][t] = (float)by;
-        s.b32[2][t] = (float)bz;
+        s.bz32[t] = (float)bz;
         s.pack[t] = NodeState<double>{sa, sc, bx, by};
         s.pack32[t] = NodeState<float>{(float)sa, (float)sc, (float)bx, (float)by};
     }
@@ -251,9 +247,11 @@ int ms3d_state(const Ms3dData& d, const NlData& nl, const double* x, hipStream_t
     StateArgs a{d.nx,     d.ny,      d.nz,     d.nu,
                 nl.ferro, nl.curve,  nl.st_a,  nl.st_c,
                 {nl.st_b[0], nl.st_b[1], nl.st_b[2]},
-                nl.st_a32, nl.st_c32, {nl.st_b32[0], nl.st_b32[1], nl.st_b32[2]},
+                nl.st_a32, nl.st_b32[2],
                 reinterpret_cast<NodeState<double>*>(nl.st_pack), reinterpret_cast<NodeState<float>*>(nl.st_pack32)};
-    SIL_REQUIRE(nl.st_pack && nl.st_pack32, SIL_ERR_INVALID, "ms3d_state: no packed state arrays");
+    SIL_REQUIRE(nl.st_pack && nl.st_pack32 && nl.st_a && nl.st_c && nl.st_b[0] && nl.st_b[1] && nl.st_b[2] &&
+                    nl.st_a32 && nl.st_b32[2],
+                SIL_ERR_INVALID, "ms3d_state: state arrays missing");
     const int64_t nn = d.nx * d.ny * d.nz;
     const unsigned g = static_cast<unsigned>(std::min<int64_t>((nn + kBlock - 1) / kBlock, 16384));
     hipLaunchKernelGGL(k_nl_state, dim3(g), dim3(kBlock), 0, s, a, x);

@@ -59,10 +59,13 @@ struct NlData {
     const BhCurve* curve = nullptr; // device
     uint8_t* ferro = nullptr;       // per node: ferromagnetic fraction > 0.5
     // state of the last evaluation of F at x, per node: H = w·(a·δB + c·B0·(B0·δB))
+    // (3D only: the 2D operator keeps its state in the packed record alone)
     double* st_a = nullptr;
     double* st_c = nullptr;
     double* st_b[3] = {nullptr, nullptr, nullptr};
-    // FP32 copies for mixed-precision Jacobian applications
+    // FP32 copies for mixed-precision Jacobian applications: a, and B_z
+    // (st_b32[2]); no FP32 kernel reads c, B_x or B_y apart from the packed
+    // record, so st_c32 and st_b32[0..1] stay unallocated
     float* st_a32 = nullptr;
     float* st_c32 = nullptr;
     float* st_b32[3] = {nullptr, nullptr, nullptr};

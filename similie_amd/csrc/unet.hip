@@ -27,6 +27,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "buffer_io.hpp"
 #include "mfma_layout.hpp"
 #include "unet.hpp"
 
@@ -60,6 +61,34 @@ __device__ __forceinline__ Chunk8 zero_chunk()
 #pragma unroll
     for (int i = 0; i < 8; ++i) c.v[i] = to_bf16(0.f);
     return c;
+}
+
+// four consecutive channels as BF16, one 8-byte LDS write
+__device__ __forceinline__ void put4_bf16(bf16* __restrict__ dst, float a, float b, float c, float d)
+{
+    const unsigned lo = (unsigned)__builtin_bit_cast(unsigned short, to_bf16(a)) |
+                        ((unsigned)__builtin_bit_cast(unsigned short, to_bf16(b)) << 16);
+    const unsigned hi = (unsigned)__builtin_bit_cast(unsigned short, to_bf16(c)) |
+                        ((unsigned)__builtin_bit_cast(unsigned short, to_bf16(d)) << 16);
+    *reinterpret_cast<U32x2*>(dst) = U32x2{lo, hi};
+}
+
+// a D fragment of the transposed products (lane: one pixel, the channels
+// d_row(l, r)) after bias, as BF16 with optional ReLU, zeros when !keep,
+// into the pixel's LDS row px_row (32 channels)
+__device__ __forceinline__ void put_channels(bf16* __restrict__ px_row, const f32x16& acc, bool relu, bool keep)
+{
+    const int h = (threadIdx.x & 63) >> 5;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float a = relu ? fmaxf(acc[4 * g + j], 0.f) : acc[4 * g + j];
+            v[j] = keep ? a : 0.f;
+        }
+        put4_bf16(px_row + 8 * g + 4 * h, v[0], v[1], v[2], v[3]);
+    }
 }
 
 // ------------------------------------------------------ 3×3 32→32 conv ---
@@ -156,18 +185,84 @@ __device__ __forceinline__ void stage_tile(bf16* __restrict__ sX, const bf16* __
     }
 }
 
+// The raw chunks of one staged chunk through buffer resources over one image
+// of X and X2: a chunk outside the image (in == false) reads zeros and needs
+// no branch, so the compiler's wait before the chunks are combined counts
+// only the loads issued before them.
+__device__ __forceinline__ Chunk8 chunk_of(U32x4 v)
+{
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+    Chunk8 c;
+    __builtin_memcpy(&c, w, sizeof(c));
+    return c;
+}
+
+template <int MODE>
+struct RawBuf {
+    U32x4 c[RawChunk<MODE>::n];
+};
+
+template <int MODE>
+__device__ __forceinline__ void load_raw_buf(RawBuf<MODE>& r, __amdgpu_buffer_rsrc_t rx, __amdgpu_buffer_rsrc_t rx2,
+                                             int gy, int gx, int part, int W, bool in)
+{
+    if constexpr (MODE == ST_POOL) {
+        const int W2 = 2 * W;
+        const uint32_t base = (uint32_t)(((2 * gy) * W2 + 2 * gx) * C + part * 8) * 2;
+        const uint32_t row = (uint32_t)W2 * C * 2, px = C * 2;
+        r.c[0] = __builtin_amdgcn_raw_buffer_load_b128(rx, in ? base : kOutside, 0, 0);
+        r.c[1] = __builtin_amdgcn_raw_buffer_load_b128(rx, in ? base + px : kOutside, 0, 0);
+        r.c[2] = __builtin_amdgcn_raw_buffer_load_b128(rx, in ? base + row : kOutside, 0, 0);
+        r.c[3] = __builtin_amdgcn_raw_buffer_load_b128(rx, in ? base + row + px : kOutside, 0, 0);
+    } else {
+        const uint32_t off = (uint32_t)((gy * W + gx) * C + part * 8) * 2;
+        r.c[0] = __builtin_amdgcn_raw_buffer_load_b128(rx, in ? off : kOutside, 0, 0);
+        if constexpr (MODE == ST_UPADD) {
+            const uint32_t q = (uint32_t)(((gy / 2) * (W / 2) + gx / 2) * C + part * 8) * 2;
+            r.c[1] = __builtin_amdgcn_raw_buffer_load_b128(rx2, in ? q : kOutside, 0, 0);
+        } else if constexpr (MODE == ST_MASK) {
+            r.c[1] = __builtin_amdgcn_raw_buffer_load_b128(rx2, in ? off : kOutside, 0, 0);
+        }
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ Chunk8 combine_buf(const RawBuf<MODE>& r)
+{
+    RawChunk<MODE> c;
+#pragma unroll
+    for (int i = 0; i < RawChunk<MODE>::n; ++i) c.c[i] = chunk_of(r.c[i]);
+    return combine<MODE>(c);
+}
+
+// bytes of one image of X and of X2 for a staging mode on H×W outputs
+__host__ __device__ constexpr int64_t x_image_bytes(int mode, int H, int W)
+{
+    return (mode == ST_POOL ? 4 : 1) * (int64_t)H * W * C * 2;
+}
+__host__ __device__ constexpr int64_t x2_image_bytes(int mode, int H, int W)
+{
+    return mode == ST_UPADD ? (int64_t)(H / 2) * (W / 2) * C * 2 : (mode == ST_MASK ? (int64_t)H * W * C * 2 : 0);
+}
+
 // Y = act(conv(staged input) + bias) on H×W images: the forward pass, or
 // (ST_MASK, flipped weights, no bias, no ReLU) the input gradient.
 //
-// Persistent workgroups walk the 8×32-pixel output tiles with stride
+// Persistent workgroups walk the 8×32-pixel output tiles with stride G =
 // gridDim.x.  Each lane keeps its B fragments (the weights of its output
 // channel, 9 taps × 32 input channels) in registers for the whole walk; the
-// input tile is double-buffered in LDS, the next tile's global loads being
-// issued before the current tile's 36 MFMAs per wavefront; the results go
-// back through the consumed buffer so that every global store is a 16-byte
-// vector store of 8 channels.
+// input tile is double-buffered in LDS.  One step for tile t: the 36 MFMAs
+// per wavefront on the staged input of t; its results into the consumed
+// buffer; the raw chunks of tile t + G (loaded during the step before)
+// combined into the other buffer; the loads of tile t + 2G issued; then the
+// stores of t, every one a 16-byte vector store of 8 channels.  The loads are
+// issued before the step's stores and waited for one step later, so neither
+// the loads nor the stores of a step stall the next, and all global
+// accesses go through buffer resources (no branch between a load and its
+// use).
 constexpr int kStageChunks = (TH + 2) * (TW + 2) * 4; // 16-byte chunks of one input tile
 constexpr int kPerThread = (kStageChunks + NT - 1) / NT;
+static_assert(TH * TW * 4 % NT == 0 && TH * TW % NT == 0, "the output stores of a tile are straight-line");
 
 template <int MODE, bool HEAD = false>
 __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ X2,
@@ -181,6 +276,7 @@ __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_x = (W + TW - 1) / TW, tiles_y = (H + TH - 1) / TH;
     const int ntiles = tiles_x * tiles_y * B;
+    const int G = gridDim.x;
     int tile = blockIdx.x;
     if (tile >= ntiles) return; // uniform per workgroup
 
@@ -193,46 +289,56 @@ __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16
 #pragma unroll
         for (int kh = 0; kh < 2; ++kh)
             bw[tap][kh] = *reinterpret_cast<const bf16x8*>(&Wt[(tap * C + co_b) * C + kh * 16 + k0]);
-    const int co = Mfma32::d_col(lane);
-    const float bv = (bias && !HEAD) ? bias[co] : 0.f;
+    // the products are transposed (weights as A, pixels as B): the D
+    // fragment of a lane is pixel `col` of its row with the 16 output
+    // channels d_row(lane, r), so its results leave as 8-byte runs of four
+    // channels and the head's two columns sit in acc[0], acc[1] of lanes < 32
+    float bv[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) bv[r] = (bias && !HEAD) ? bias[Mfma32::d_row(lane, r)] : 0.f;
+    const float bh = (bias && HEAD) ? bias[0] : 0.f;
+    const int64_t xb = x_image_bytes(MODE, H, W), x2b = x2_image_bytes(MODE, H, W);
 
     auto coords = [&](int t, int& b, int& oy0, int& ox0) {
         b = t / (tiles_x * tiles_y);
         oy0 = ((t / tiles_x) % tiles_y) * TH;
         ox0 = (t % tiles_x) * TW;
     };
-    // fetch issues the loads of a tile; put combines them into LDS
-    auto fetch = [&](int t, RawChunk<MODE> (&pre)[kPerThread]) {
+    // fetch issues the loads of tile t (zeros for t beyond the last tile);
+    // put combines them into LDS
+    auto fetch = [&](int t, RawBuf<MODE> (&pre)[kPerThread]) {
         int b, oy0, ox0;
-        coords(t, b, oy0, ox0);
+        coords(min(t, ntiles - 1), b, oy0, ox0);
+        const __amdgpu_buffer_rsrc_t rx = buffer_of(X + (size_t)b * (xb / 2), xb);
+        const __amdgpu_buffer_rsrc_t rx2 = buffer_of(x2b ? X2 + (size_t)b * (x2b / 2) : X, x2b);
 #pragma unroll
         for (int k = 0; k < kPerThread; ++k) {
             const int c = tid + k * NT;
-            zero_raw<MODE>(pre[k]);
-            if (c < kStageChunks) {
-                const int pix = c >> 2, part = c & 3;
-                const int gy = oy0 - 1 + pix / (TW + 2), gx = ox0 - 1 + pix % (TW + 2);
-                if (gy >= 0 && gy < H && gx >= 0 && gx < W) load_raw<MODE>(pre[k], X, X2, b, gy, gx, part, H, W);
-            }
+            const int pix = c >> 2, part = c & 3;
+            const int gy = oy0 - 1 + pix / (TW + 2), gx = ox0 - 1 + pix % (TW + 2);
+            const bool in = t < ntiles && c < kStageChunks && gy >= 0 && gy < H && gx >= 0 && gx < W;
+            load_raw_buf<MODE>(pre[k], rx, rx2, gy, gx, part, W, in);
         }
     };
-    auto put = [&](bf16* dst, const RawChunk<MODE> (&pre)[kPerThread]) {
+    auto put = [&](bf16* dst, const RawBuf<MODE> (&pre)[kPerThread]) {
 #pragma unroll
         for (int k = 0; k < kPerThread; ++k) {
             const int c = tid + k * NT;
-            if (c < kStageChunks) *reinterpret_cast<Chunk8*>(&dst[(c >> 2) * CP + (c & 3) * 8]) = combine<MODE>(pre[k]);
+            if (c < kStageChunks)
+                *reinterpret_cast<Chunk8*>(&dst[(c >> 2) * CP + (c & 3) * 8]) = combine_buf<MODE>(pre[k]);
         }
     };
 
-    RawChunk<MODE> pre[kPerThread];
+    RawBuf<MODE> pre[kPerThread];
     fetch(tile, pre);
     put(sX[0], pre);
+    fetch(tile + G, pre);
     __syncthreads();
+    // the weights and bias are in registers before the walk: no wait inside
+    // it is for them
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     int buf = 0;
-    for (; tile < ntiles; tile += gridDim.x) {
-        const int next = tile + gridDim.x;
-        if (next < ntiles) fetch(next, pre);
-
+    for (; tile < ntiles; tile += G) {
         f32x16 acc[2];
 #pragma unroll
         for (int m = 0; m < 2; ++m)
@@ -248,52 +354,52 @@ __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16
                 for (int m = 0; m < 2; ++m) {
                     const int tr = 2 * wave + m + ky, tc = col + kx;
                     const bf16x8 ax = *reinterpret_cast<const bf16x8*>(&cur[(tr * (TW + 2) + tc) * CP + kh * 16 + k0]);
-                    acc[m] = Mfma32::mma(ax, bw[tap][kh], acc[m]);
+                    acc[m] = Mfma32::mma(bw[tap][kh], ax, acc[m]);
                 }
             }
         }
         __syncthreads(); // every wavefront is done with the input tile in sX[buf]
 
+        int b, oy0, ox0;
+        coords(tile, b, oy0, ox0);
         if constexpr (HEAD) {
-            // output head: columns 0 and 1 (hi / lo weights) → sX[buf] as
-            // [8 rows][32 pixels][2] FP32, then one FP32 output per pixel
-            float* ps = reinterpret_cast<float*>(sX[buf]);
-            if (co < 2) {
+            // output head: channels 0 and 1 (hi / lo weights) of the lane's
+            // pixel, one FP32 output per pixel straight from the registers
+            const __amdgpu_buffer_rsrc_t ry = buffer_of(Yh + (size_t)b * H * W, (int64_t)H * W * 4);
 #pragma unroll
-                for (int m = 0; m < 2; ++m)
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) ps[((2 * wave + m) * TW + Mfma32::d_row(lane, r)) * 2 + co] = acc[m][r];
+            for (int m = 0; m < 2; ++m) {
+                const int gy = oy0 + 2 * wave + m, gx = ox0 + col;
+                const float v = acc[m][0] + acc[m][1] + bh;
+                __builtin_amdgcn_raw_buffer_store_b32(
+                    __builtin_bit_cast(unsigned, v), ry,
+                    (lane < 32 && gy < H && gx < W) ? (uint32_t)(gy * W + gx) * 4 : kOutside, 0, 0);
             }
-            if (next < ntiles) put(sX[buf ^ 1], pre);
+            put(sX[buf ^ 1], pre);
+            fetch(tile + 2 * G, pre);
             __syncthreads();
-            int b, oy0, ox0;
-            coords(tile, b, oy0, ox0);
-            for (int c = tid; c < TH * TW; c += NT) {
-                const int gy = oy0 + c / TW, gx = ox0 + c % TW;
-                if (gy < H && gx < W) Yh[((size_t)b * H + gy) * W + gx] = ps[c * 2] + ps[c * 2 + 1] + bias[0];
-            }
         } else {
             // results → sX[buf] as [8 rows][32 pixels][32 channels] (stride CP)
             bf16* out_s = sX[buf];
 #pragma unroll
-            for (int m = 0; m < 2; ++m)
+            for (int m = 0; m < 2; ++m) {
+                f32x16 v;
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    float v = acc[m][r] + bv;
-                    if (relu) v = fmaxf(v, 0.f);
-                    out_s[((2 * wave + m) * TW + Mfma32::d_row(lane, r)) * CP + co] = to_bf16(v);
-                }
-            if (next < ntiles) put(sX[buf ^ 1], pre);
+                for (int r = 0; r < 16; ++r) v[r] = acc[m][r] + bv[r];
+                put_channels(&out_s[((2 * wave + m) * TW + col) * CP], v, relu != 0, true);
+            }
+            put(sX[buf ^ 1], pre);
+            fetch(tile + 2 * G, pre);
             __syncthreads();
 
-            int b, oy0, ox0;
-            coords(tile, b, oy0, ox0);
-            for (int c = tid; c < TH * TW * 4; c += NT) {
+            const __amdgpu_buffer_rsrc_t ry = buffer_of(Y + (size_t)b * H * W * C, (int64_t)H * W * C * 2);
+#pragma unroll
+            for (int k = 0; k < TH * TW * 4 / NT; ++k) {
+                const int c = tid + k * NT;
                 const int pix = c >> 2, part = c & 3;
                 const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
-                if (gy < H && gx < W)
-                    *reinterpret_cast<Chunk8*>(&Y[(((size_t)b * H + gy) * W + gx) * C + part * 8]) =
-                        *reinterpret_cast<const Chunk8*>(&out_s[pix * CP + part * 8]);
+                const U32x4 v = *reinterpret_cast<const U32x4*>(&out_s[pix * CP + part * 8]);
+                __builtin_amdgcn_raw_buffer_store_b128(
+                    v, ry, (gy < H && gx < W) ? (uint32_t)((gy * W + gx) * C + part * 8) * 2 : kOutside, 0, 0);
             }
         }
         buf ^= 1;
@@ -399,13 +505,46 @@ __global__ void k_reduce_parts(const float* __restrict__ part, int parts, int n,
 // -------------------------------------------------- input / output convs ---
 
 // a0 = relu(conv_in(x)): x FP32 [B][H][W][2], weights [tap][co][ci=2].
-// A workgroup stages a (TH+2)×(TW+2) tile of x and writes the 32 channels of
-// one pixel per thread (four 16-byte stores).
-// 3×3 conv 2→32 + ReLU on FP32 input, FP32 fused multiply-adds.  The
-// weights are read at workgroup-uniform addresses (scalar loads, scalar
-// operands of the FMAs), each thread computes the 32 channels of one pixel,
-// and the results go through LDS so that every global store is 8 channels
-// (16 bytes).
+//
+// The input layer on the FP32 matrix cores (v_mfma_f32_32x32x2_f32: K = 2 is
+// the two input channels of one tap) with the weights as A (32 output
+// channels × 2) and 32 pixels as B: lane l supplies the weight of output
+// channel l % 32, input channel l / 32, and input channel l / 32 of pixel
+// l % 32, read from a staged x tile (FP32 pairs, row stride XW) at
+// (py + ky, px + kx) for tap (ky, kx).  Accumulation starts at the bias and
+// adds one tap per instruction, taps in order 0..8.  The result holds, per
+// lane, pixel l % 32 and the 16 output channels Mfma32F32::d_row(l, r):
+// four runs of four consecutive channels, each one 8-byte write.  Every
+// kernel that forms a0 uses this sequence, so a0 is the same wherever it is
+// computed.
+template <int XW>
+__device__ __forceinline__ f32x16 conv_in_mma(const float* __restrict__ sx, int py, int px, const float (&wi)[9],
+                                              const float (&b16)[16])
+{
+    const int ci = Mfma32F32::b_k(threadIdx.x & 63);
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = b16[r];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc = Mfma32F32::mma(wi[t], sx[((py + t / 3) * XW + px + t % 3) * 2 + ci], acc);
+    return acc;
+}
+
+// the lane's A operands (one weight per tap) and the biases of its 16 D rows
+__device__ __forceinline__ void conv_in_weights(const float* __restrict__ w, const float* __restrict__ bias,
+                                                float (&wi)[9], float (&b16)[16])
+{
+    const int lane = threadIdx.x & 63;
+    const int co = Mfma32F32::a_row(lane), ci = Mfma32F32::a_k(lane);
+#pragma unroll
+    for (int t = 0; t < 9; ++t) wi[t] = w[(t * C + co) * 2 + ci];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b16[r] = bias[Mfma32F32::d_row(lane, r)];
+}
+
+// A workgroup stages the (TH+2)×(TW+2) tile of x around an 8×32-pixel output
+// tile; each wavefront computes two 32-pixel rows, and the results go through
+// LDS so that every global store is 8 channels (16 bytes).
 __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, const float* __restrict__ w,
                                                 const float* __restrict__ bias, bf16* __restrict__ y, int H, int W,
                                                 const int* __restrict__ dB)
@@ -414,7 +553,7 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
     __shared__ __attribute__((aligned(16))) bf16 sY[TH * TW * CP];
     const int b = blockIdx.z, oy0 = blockIdx.y * TH, ox0 = blockIdx.x * TW;
     if (dB && b >= *dB) return; // uniform per workgroup
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < (TH + 2) * (TW + 2); i += NT) {
         const int gy = oy0 - 1 + i / (TW + 2), gx = ox0 - 1 + i % (TW + 2);
         const bool ok = gy >= 0 && gy < H && gx >= 0 && gx < W;
@@ -422,30 +561,15 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
         sx[i * 2] = ok ? x[q * 2] : 0.f;
         sx[i * 2 + 1] = ok ? x[q * 2 + 1] : 0.f;
     }
+    float wi[9], b16[16];
+    conv_in_weights(w, bias, wi, b16);
     __syncthreads();
-    const int ty = tid / TW, tx = tid % TW;
-    float in[18];
+    const int px = Mfma32F32::b_col(lane);
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const int i = (ty + t / 3) * (TW + 2) + tx + t % 3;
-        in[2 * t] = sx[i * 2];
-        in[2 * t + 1] = sx[i * 2 + 1];
-    }
-#pragma unroll
-    for (int part = 0; part < 4; ++part) {
-        Chunk8 o;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int co = part * 8 + c;
-            float a = bias[co];
-#pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                a = fmaf(w[(t * C + co) * 2], in[2 * t], a);
-                a = fmaf(w[(t * C + co) * 2 + 1], in[2 * t + 1], a);
-            }
-            o.v[c] = to_bf16(fmaxf(a, 0.f));
-        }
-        *reinterpret_cast<Chunk8*>(&sY[tid * CP + part * 8]) = o;
+    for (int m = 0; m < 2; ++m) {
+        const int row = 2 * wave + m;
+        const f32x16 acc = conv_in_mma<TW + 2>(sx, row, px, wi, b16);
+        put_channels(&sY[(row * TW + px) * CP], acc, true, true);
     }
     __syncthreads();
     for (int c = tid; c < TH * TW * 4; c += NT) {
@@ -454,6 +578,172 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
         if (gy < H && gx < W)
             *reinterpret_cast<Chunk8*>(&y[(((size_t)b * H + gy) * W + gx) * C + part * 8]) =
                 *reinterpret_cast<const Chunk8*>(&sY[pix * CP + part * 8]);
+    }
+}
+
+// e0 = relu(conv(relu(conv_in(x)))): the input layer and the first 32→32
+// layer in one persistent kernel, for passes that keep no activations (a0 is
+// never written).  Per output tile a workgroup stages the (TH+4)×(TW+4) tile
+// of x (FP32 pairs), forms a0 on the (TH+2)×(TW+2) halo in LDS with
+// conv_in_mma — eleven 32-pixel blocks, zero outside the image as the
+// layer's zero padding requires — and runs the 32→32 layer on it exactly as
+// k_conv33 does (same instructions in the same order), so the output equals
+// conv_in followed by k_conv33 bit for bit.
+//
+// The x tiles are read two tiles ahead through a buffer resource (pixels
+// outside the image read zero, no branch between a load and its use): the
+// loads of tile t + 2G (G = grid size) are issued at the top of the step of
+// tile t and written to LDS before its stores, so the 36 matrix instructions
+// of tile t and a0 of tile t + G cover them; the stores of a step are its
+// last accesses and have the next step's work to drain.
+constexpr int XTH = TH + 4, XTW = TW + 4;           // x tile of the fused layers
+constexpr int kXPairs = XTH * XTW;                  // 432 FP32 pairs
+constexpr int kXPerThread = (kXPairs + NT - 1) / NT;
+constexpr int kHalo = (TH + 2) * (TW + 2);          // 340 a0 pixels
+constexpr int kHaloBlocks = (kHalo + 31) / 32;      // 11
+static_assert(TH * TW * 4 % NT == 0, "the output stores of a tile are straight-line");
+
+__global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x, const float* __restrict__ w_in,
+                                                     const float* __restrict__ b_in, const bf16* __restrict__ Wt,
+                                                     const float* __restrict__ bias, bf16* __restrict__ Y, int Bmax,
+                                                     const int* __restrict__ dB, int H, int W)
+{
+    const int B = dB ? min(*dB, Bmax) : Bmax;
+    __shared__ __attribute__((aligned(16))) bf16 sX[2][kHalo * CP];
+    __shared__ __attribute__((aligned(16))) float sx[kXPairs * 2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = (W + TW - 1) / TW, tiles_y = (H + TH - 1) / TH;
+    const int ntiles = tiles_x * tiles_y * B;
+    const int G = gridDim.x;
+    int tile = blockIdx.x;
+    if (tile >= ntiles) return; // uniform per workgroup
+
+    float wi[9], bi[16];
+    conv_in_weights(w_in, b_in, wi, bi);
+    const int col = Mfma32::a_row(lane);
+    const int k0 = Mfma32::a_k0(lane);
+    const int co_b = Mfma32::b_col(lane);
+    bf16x8 bw[9][2];
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int kh = 0; kh < 2; ++kh)
+            bw[tap][kh] = *reinterpret_cast<const bf16x8*>(&Wt[(tap * C + co_b) * C + kh * 16 + k0]);
+    float bv[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) bv[r] = bias[Mfma32::d_row(lane, r)];
+
+    auto coords = [&](int t, int& b, int& oy0, int& ox0) {
+        b = t / (tiles_x * tiles_y);
+        oy0 = ((t / tiles_x) % tiles_y) * TH;
+        ox0 = (t % tiles_x) * TW;
+    };
+    // the x pairs of tile t this thread stages (zero outside the image; a
+    // tile beyond the last reads zeros and is never used)
+    auto fetch = [&](int t, U32x2 (&xr)[kXPerThread]) {
+        int b, oy0, ox0;
+        coords(min(t, ntiles - 1), b, oy0, ox0); // t may lie beyond the last tile
+        const __amdgpu_buffer_rsrc_t rs = buffer_of(x + (size_t)b * H * W * 2, (int64_t)H * W * 8);
+#pragma unroll
+        for (int k = 0; k < kXPerThread; ++k) {
+            const int c = tid + k * NT;
+            const int gy = oy0 - 2 + c / XTW, gx = ox0 - 2 + c % XTW;
+            const bool in = c < kXPairs && t < ntiles && gy >= 0 && gy < H && gx >= 0 && gx < W;
+            xr[k] = __builtin_amdgcn_raw_buffer_load_b64(rs, in ? (uint32_t)((gy * W + gx) * 8) : kOutside, 0, 0);
+        }
+    };
+    auto put_x = [&](const U32x2 (&xr)[kXPerThread]) {
+#pragma unroll
+        for (int k = 0; k < kXPerThread; ++k) {
+            const int c = tid + k * NT;
+            if (c < kXPairs) *reinterpret_cast<U32x2*>(&sx[c * 2]) = xr[k];
+        }
+    };
+    // a0 of tile t on its halo, from the x tile in sx, into dst
+    auto make_a0 = [&](int t, bf16* __restrict__ dst) {
+        int b, oy0, ox0;
+        coords(t, b, oy0, ox0);
+        for (int j = wave; j < kHaloBlocks; j += NT / 64) {
+            // the lane's halo pixel (the last block runs past the halo: its
+            // extra lanes read the last pixel and write nothing)
+            const int hp = j * 32 + Mfma32F32::b_col(lane);
+            const int ha = min(hp, kHalo - 1);
+            const int hy = ha / (TW + 2), hx = ha % (TW + 2);
+            const f32x16 acc = conv_in_mma<XTW>(sx, hy, hx, wi, bi);
+            const int gy = oy0 - 1 + hy, gx = ox0 - 1 + hx;
+            const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
+            if (hp < kHalo) put_channels(&dst[hp * CP], acc, true, inside);
+        }
+    };
+
+    // prologue: a0 of the first tile in sX[0], x of the second tile in sx
+    U32x2 xr[kXPerThread];
+    fetch(tile, xr);
+    put_x(xr);
+    fetch(tile + G, xr);
+    __syncthreads();
+    make_a0(tile, sX[0]);
+    __syncthreads();
+    put_x(xr);
+    __syncthreads();
+    // the weights and biases are in registers before the walk: no wait
+    // inside it is for them
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    int buf = 0;
+    for (; tile < ntiles; tile += G) {
+        const int next = tile + G;
+        fetch(next + G, xr);
+
+        f32x16 acc[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[m][r] = 0.f;
+        const bf16* cur = sX[buf];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+#pragma unroll
+            for (int kh = 0; kh < 2; ++kh) {
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const int tr = 2 * wave + m + ky, tc = col + kx;
+                    const bf16x8 ax = *reinterpret_cast<const bf16x8*>(&cur[(tr * (TW + 2) + tc) * CP + kh * 16 + k0]);
+                    acc[m] = Mfma32::mma(bw[tap][kh], ax, acc[m]);
+                }
+            }
+        }
+        __syncthreads(); // sX[buf] (a0 of this tile) is consumed
+
+        bf16* out_s = sX[buf];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            f32x16 v;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = acc[m][r] + bv[r];
+            put_channels(&out_s[((2 * wave + m) * TW + col) * CP], v, true, true);
+        }
+        if (next < ntiles) make_a0(next, sX[buf ^ 1]); // from sx = x of tile next
+        __syncthreads(); // outputs of this tile and a0 of the next complete, sx consumed
+        put_x(xr);       // x of tile next + G (zeros beyond the last tile)
+
+        int b, oy0, ox0;
+        coords(tile, b, oy0, ox0);
+        {
+            // straight-line masked stores: pixels outside the image go to
+            // an offset beyond the image's resource and are dropped
+            const __amdgpu_buffer_rsrc_t ry = buffer_of(Y + (size_t)b * H * W * C, (int64_t)H * W * C * 2);
+#pragma unroll
+            for (int k = 0; k < TH * TW * 4 / NT; ++k) {
+                const int c = tid + k * NT;
+                const int pix = c >> 2, part = c & 3;
+                const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
+                const U32x4 v = *reinterpret_cast<const U32x4*>(&out_s[pix * CP + part * 8]);
+                __builtin_amdgcn_raw_buffer_store_b128(
+                    v, ry, (gy < H && gx < W) ? (uint32_t)(((gy * W + gx) * C + part * 8) * 2) : kOutside, 0, 0);
+            }
+        }
+        buf ^= 1;
     }
 }
 
@@ -747,6 +1037,9 @@ int by_mode(int mode, F&& f)
 int conv33(int mode, const bf16* X, const bf16* X2, const bf16* Wt, const float* bias, bf16* Y, int B, int H, int W,
            bool relu, hipStream_t s, const int* dB)
 {
+    // one buffer resource spans one image of each operand
+    SIL_REQUIRE(x_image_bytes(mode, H, W) <= kMaxBufferBytes, SIL_ERR_UNSUPPORTED,
+                "unet: %dx%d images exceed one buffer resource per image", H, W);
     // persistent grid (the LDS of a workgroup is 54 KB), capped by the
     // number of tiles
     const long ntiles = (long)((W + TW - 1) / TW) * ((H + TH - 1) / TH) * B;
@@ -788,6 +1081,19 @@ int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, i
     return SIL_OK;
 }
 
+int conv_in33(const float* x, const float* w_in, const float* b_in, const bf16* Wt, const float* bias, bf16* Y, int B,
+              int H, int W, hipStream_t s, const int* dB)
+{
+    SIL_REQUIRE((int64_t)H * W * C * 2 <= kMaxBufferBytes, SIL_ERR_UNSUPPORTED,
+                "unet: %dx%d images exceed one buffer resource per image", H, W);
+    const long ntiles = (long)((W + TW - 1) / TW) * ((H + TH - 1) / TH) * B;
+    const long cap = kComputeUnits * 2;
+    const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
+    hipLaunchKernelGGL(k_conv_in33, dim3(grid), dim3(NT), 0, s, x, w_in, b_in, Wt, bias, Y, B, dB, H, W);
+    SIL_LAUNCH_CHECK();
+    return SIL_OK;
+}
+
 int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, int parts, float* dWb, int B, int H,
                   int W, hipStream_t s)
 {
@@ -805,6 +1111,8 @@ int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, i
 int conv_out(const bf16* x, const bf16* Wt, const float* bias, float* y, int B, int H, int W, hipStream_t s,
              const int* dB)
 {
+    SIL_REQUIRE(x_image_bytes(ST_PLAIN, H, W) <= kMaxBufferBytes, SIL_ERR_UNSUPPORTED,
+                "unet: %dx%d images exceed one buffer resource per image", H, W);
     const long ntiles = (long)((W + TW - 1) / TW) * ((H + TH - 1) / TH) * B;
     const long cap = kComputeUnits * conv_wgs_per_cu(ST_PLAIN);
     const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);

@@ -35,6 +35,10 @@ int conv33_wgrad(int mode, const bf16* X, const bf16* X2, const bf16* dY, const 
 // 2 → 32 input convolution (FP32 input) and its weight gradient (dWb = [dW(9·32·2) | db(32)])
 int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s,
             const int* dB = nullptr);
+// relu(conv(relu(conv_in(x)))) in one kernel (a0 is formed in LDS and not
+// written): equal, bit for bit, to conv_in followed by conv33 in STAGE_PLAIN
+int conv_in33(const float* x, const float* w_in, const float* b_in, const bf16* Wt, const float* bias, bf16* Y, int B,
+              int H, int W, hipStream_t s, const int* dB = nullptr);
 int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, int parts, float* dWb, int B, int H,
                   int W, hipStream_t s);
 // 32 → 1 output convolution (FP32 output) and its backward (dWb = [dW(9·32) | db])
@@ -62,6 +66,8 @@ int pack_head(const float* w, bf16* Wt, hipStream_t s);
 // the images in x; the rest of `out` is left as it is.  Used inside captured
 // solver iterations, where the number of images still needing the network is
 // only known on the device.
+// The pass keeps no activations for a backward pass: the input layer and the
+// first 32→32 layer run fused.
 int forward_active(sil_unet_s* net, int batch, const int* d_count, const float* x, float* out, hipStream_t s);
 void net_shape(const struct ::sil_unet_s* net, int* max_batch, int* height, int* width);
 

@@ -101,13 +101,20 @@ const bf16* WF(const sil_unet_s* u, int l) { return u->wf + (size_t)(l - 1) * 9 
 const bf16* WB(const sil_unet_s* u, int l) { return u->wb + (size_t)(l - 1) * 9 * C * C; }
 const float* BIAS(const sil_unet_s* u, int l) { return u->params + layer_off(l) + 9 * C * C; }
 
-int forward(sil_unet_s* u, int B, const float* x, float* out, hipStream_t s, const int* dB = nullptr)
+// keep = false: inference only — a0 is not stored (the input layer and
+// layer 1 run as one kernel), the other activations are
+int forward(sil_unet_s* u, int B, const float* x, float* out, hipStream_t s, const int* dB = nullptr, bool keep = true)
 {
     namespace un = sil::unet;
     constexpr int P = un::STAGE_PLAIN, UP = un::STAGE_UPADD, PO = un::STAGE_POOL;
     const int *H = u->Hl, *W = u->Wl;
-    UNET_TRY(un::conv_in(x, u->params, u->params + 9 * C * 2, u->a0, B, H[0], W[0], s, dB));
-    UNET_TRY(un::conv33(P, u->a0, nullptr, WF(u, 1), BIAS(u, 1), u->e0, B, H[0], W[0], true, s, dB));
+    if (keep) {
+        UNET_TRY(un::conv_in(x, u->params, u->params + 9 * C * 2, u->a0, B, H[0], W[0], s, dB));
+        UNET_TRY(un::conv33(P, u->a0, nullptr, WF(u, 1), BIAS(u, 1), u->e0, B, H[0], W[0], true, s, dB));
+    } else {
+        UNET_TRY(un::conv_in33(x, u->params, u->params + 9 * C * 2, WF(u, 1), BIAS(u, 1), u->e0, B, H[0], W[0], s,
+                               dB));
+    }
     UNET_TRY(un::conv33(PO, u->e0, nullptr, WF(u, 2), BIAS(u, 2), u->a1, B, H[1], W[1], true, s, dB));
     UNET_TRY(un::conv33(P, u->a1, nullptr, WF(u, 3), BIAS(u, 3), u->e1, B, H[1], W[1], true, s, dB));
     UNET_TRY(un::conv33(PO, u->e1, nullptr, WF(u, 4), BIAS(u, 4), u->a2, B, H[2], W[2], true, s, dB));
@@ -182,7 +189,7 @@ int sil::unet::forward_active(sil_unet_s* net, int batch, const int* d_count, co
     SIL_REQUIRE(net != nullptr && d_count != nullptr && x != nullptr && out != nullptr, SIL_ERR_INVALID,
                 "unet: null argument");
     UNET_TRY(check_batch(net, batch));
-    return forward(net, batch, x, out, s, d_count);
+    return forward(net, batch, x, out, s, d_count, false);
 }
 
 void sil::unet::net_shape(const sil_unet_s* net, int* max_batch, int* height, int* width)
@@ -292,6 +299,13 @@ int sil_unet_forward(sil_unet u, int32_t batch, const float* x, float* out, void
     SIL_REQUIRE(u != nullptr && x != nullptr && out != nullptr, SIL_ERR_INVALID, "unet: null argument");
     UNET_TRY(check_batch(u, batch));
     return forward(u, batch, x, out, static_cast<hipStream_t>(stream));
+}
+
+int sil_unet_infer(sil_unet u, int32_t batch, const float* x, float* out, void* stream)
+{
+    SIL_REQUIRE(u != nullptr && x != nullptr && out != nullptr, SIL_ERR_INVALID, "unet: null argument");
+    UNET_TRY(check_batch(u, batch));
+    return forward(u, batch, x, out, static_cast<hipStream_t>(stream), nullptr, false);
 }
 
 int sil_unet_activation(sil_unet u, int32_t which, int32_t batch, float* out, void* stream)

@@ -197,6 +197,22 @@ class StagedLaplacian:
         L.check(L.lib().sil_staged_laplacian_run(self.handle, L.ptr(out), L.ptr(tensor), L.stream_handle(stream)))
         return out
 
+    def autotune(self, out, tensor, repeats=5, stream=None):
+        """times the fused 3D kernel shapes on these arrays and keeps the
+        fastest (sil_staged_laplacian_autotune); returns the chosen shape"""
+        _check_tensor(out, "out")
+        _check_tensor(tensor, "tensor")
+        if (tuple(out.shape), tuple(out.stride())) != self.layout_out or \
+                (tuple(tensor.shape), tuple(tensor.stride())) != self.layout_in:
+            raise ValueError("staged laplacian called with shapes or strides it was not built for")
+        t = L.Lap3dTile()
+        L.check(L.lib().sil_staged_laplacian_autotune(self.handle, L.ptr(out), L.ptr(tensor), int(repeats),
+                                                      ctypes.byref(t), L.stream_handle(stream)))
+        names = {0: "direct", 1: "march", 2: "pipe", 3: "stage"}
+        return {"variant": names.get(t.variant, str(t.variant)), "rows": t.rows, "xchunk": t.xchunk,
+                "lanes_or_depth": t.lanes_or_depth, "nontemporal": bool(t.nontemporal),
+                "microseconds": t.microseconds}
+
     def __del__(self):
         # the destroy function was bound at construction: module globals may
         # already be gone when this runs at interpreter exit

@@ -79,6 +79,16 @@ class UNet:
         L.check(L.lib().sil_unet_forward(self.handle, b, L.ptr(x), L.ptr(out), L.stream_handle(stream)))
         return out
 
+    def infer(self, x, out=None, stream=None):
+        """forward() without the activations a backward pass needs (the input
+        layer and layer 1 fused); the same output bit for bit"""
+        b = self._shape(x)
+        if out is None:
+            out = torch.empty((b, self.height, self.width), dtype=torch.float32, device="cuda")
+        _f32(out, "output", (b, self.height, self.width))
+        L.check(L.lib().sil_unet_infer(self.handle, b, L.ptr(x), L.ptr(out), L.stream_handle(stream)))
+        return out
+
     ACTIVATIONS = ("a0", "e0", "a1", "e1", "a2", "e2", "a3", "b3", "c2", "f2", "c1", "f1", "c0")
     LEVELS = (0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0)
 

@@ -81,3 +81,33 @@ def test_wide_path_reaches_fp64_verified_tolerance(wide_side, monkeypatch):
     # a second solve on the same handle reuses the captured graphs
     x2, results2, _ = solver.solve(b)
     assert [r["iterations"] for r in results2] == [r["iterations"] for r in results]
+
+
+def test_shape_autotune_keeps_every_problem_solved():
+    """The shape autotuner re-partitions the batch (wide-path threshold,
+    workgroups per wide problem, workgroup size) between timed probes; the
+    shape it leaves behind still solves every problem to the verified
+    tolerance, and a second solve repeats the first bit for bit."""
+    from similie_amd.batched import BatchedPoisson2D
+
+    sides, kap, rhs = problems(6, 30, 8, 260)
+    kappa = torch.from_numpy(np.concatenate([k.ravel() for k in kap])).cuda()
+    b = torch.from_numpy(np.concatenate([r.ravel() for r in rhs])).cuda()
+    solver = BatchedPoisson2D(sides, kappa)
+    shape = solver.autotune_shape(b, probe_iterations=8)
+    assert shape["block"] in (256, 512, 1024) and shape["probe_seconds"] > 0
+    assert shape["wide_side"] in (96, 128, 192, 256, 384, int(max(sides)) + 1)
+    assert shape["wide_problems"] == sum(int(n) >= shape["wide_side"] for n in sides)
+    assert shape["wide_workgroups"] >= 1
+    x, results, _ = solver.solve(b)
+    xh = x.cpu().numpy()
+    for p, n in enumerate(sides):
+        assert results[p]["converged"] == 1, (p, n, results[p])
+        assert results[p]["true_relative_residual"] <= 1e-10
+        o = solver.offset(p)
+        xp = xh[o:o + n * n].reshape(n, n)
+        ones = np.ones(n)
+        r = rhs[p] - orc.poisson2d_apply(ones, ones, kap[p], xp)
+        assert np.linalg.norm(r) <= 1.0001e-10 * np.linalg.norm(rhs[p])
+    x2, results2, _ = solver.solve(b)
+    assert torch.equal(x, x2) and [r["iterations"] for r in results2] == [r["iterations"] for r in results]

@@ -255,6 +255,27 @@ def test_staged_laplacian_reuse(ext):
         assert rel_err(out, orc.laplacian(og, 1, w, [11, 9, 13])) <= TOL
 
 
+def test_staged_laplacian_autotune(ext):
+    """the autotuner times the fused kernel shapes on the staged operator's
+    arrays, keeps one, leaves its result in out, and later runs use it"""
+    rng = np.random.default_rng(13)
+    extents = [40, 36, 70]
+    og, gg = make_pair(ext, 3, extents, "diagonal", rng)
+    oext = [e - 1 for e in extents]
+    out = torch.empty(tuple(oext) + (3,), dtype=torch.float64, device="cuda")
+    w = rng.standard_normal(tuple(extents) + (3,))
+    wd = cuda(w)
+    staged = ext.make_staged_laplacian(gg, 1, out, wd)
+    tile = staged.autotune(out, wd, repeats=2)
+    print(f"[laplacian autotune] {tile}", flush=True)
+    assert tile["variant"] in ("stage", "pipe") and tile["microseconds"] > 0.0
+    ref = orc.laplacian(og, 1, w, oext)
+    assert rel_err(out, ref) <= TOL
+    out.zero_()
+    staged.run(out, wd)
+    assert rel_err(out, ref) <= TOL
+
+
 @pytest.mark.parametrize("n,k", [(3, 1), (3, 2), (3, 3), (2, 1), (2, 2)])
 def test_staged_codifferential_reuse(ext, n, k):
     """StagedCodifferential (codifferential.hpp:321-508): one staged handle

@@ -55,16 +55,20 @@ def weights_of(kappa):
 # They are compared with the oracle computing the same way
 # (oracle.unet_oracle emulate=True), which they match up to accumulation
 # order, and with exact FP64, from which BF16 storage alone moves them by
-# about 1e-2 forward and 0.05-0.15 in the weight gradients of white-noise
-# inputs (the emulating oracle shows the same distance).  The order still
-# matters through BF16 storage: the first layer matches the oracle bit for
-# bit, yet the matrix cores' summation order flips 4 of the 90k encoder
-# activations of the backward test's input to the neighbouring BF16 value,
-# the decoder grows that to one c0 value in twelve, and the weight
-# gradients move by 1-3 %.  A wrong index or a dropped term moves them by
-# tens of percent.
+# about 1e-2 forward and 0.05-0.16 in the weight gradients of white-noise
+# inputs (the emulating oracle's own distance from FP64, block by block).
+# The order still matters through BF16 storage: the input layer matches the
+# oracle's rounding on every element, yet the matrix cores' summation order
+# flips a few of the 90k encoder activations of the backward test's input
+# to the neighbouring BF16 value, the decoder grows that, and the weight
+# gradients move by 1.1-2.7 % (measured per block: 0.0127, 0.013, 0.0215,
+# 0.020, 0.0271, 0.0253, 0.0223, 0.0204, 0.0227, 0.0203, 0.0203, 0.0188,
+# 0.011, 0.0021).  A wrong index or a dropped term moves them by tens of
+# percent.  Against FP64 each block is held to the emulating oracle's own
+# distance plus the distance to it (the triangle inequality), not to a flat
+# bound.
 FWD_TOL_EMULATED, FWD_TOL_FP64 = 1e-2, 3e-2
-GRAD_TOL_EMULATED, GRAD_TOL_FP64 = 5e-2, 0.25
+GRAD_TOL_EMULATED = 3.5e-2
 # the backward pass alone (oracle fed the GPU's forward activations): the
 # largest block error measured is 1.8e-3 (conv_in, at the end of the chain)
 GRAD_TOL_GPU_ACTIVATIONS = 5e-3
@@ -82,6 +86,25 @@ def test_unet_forward_matches_oracle(learned, shape):
     p64 = params.astype(np.float64)
     assert rel(out, U.forward(p64, x, emulate=True)[0]) <= FWD_TOL_EMULATED
     assert rel(out, U.forward(p64, x)[0]) <= FWD_TOL_FP64
+
+
+@pytest.mark.parametrize("shape", [(2, 32, 32), (1, 16, 40), (3, 24, 72), (4, 256, 256)])
+def test_unet_infer_equals_forward(learned, shape):
+    """The inference pass (input layer and layer 1 in one kernel, the input
+    layer's activation formed on each tile's halo and never stored) gives the
+    stored-activation forward pass's output bit for bit, on images that are
+    and are not multiples of the 8x32 tile, and on a full 256^2 batch."""
+    B, H, W = shape
+    rng = np.random.default_rng(50 + H + W)
+    params = make_params(rng)
+    x = torch.from_numpy(rng.standard_normal((B, H, W, 2)).astype(np.float32)).cuda()
+    net = learned.UNet(B, H, W, seed=1)
+    net.set_params(torch.from_numpy(params))
+    ref = net.forward(x)
+    e0 = net.activation("e0", B)
+    got = net.infer(x)
+    assert torch.equal(got, ref)
+    assert torch.equal(net.activation("e0", B), e0)  # layer 1's output, stored by both passes
 
 
 def test_unet_backward_matches_oracle(learned):
@@ -105,8 +128,14 @@ def test_unet_backward_matches_oracle(learned):
                                     for l in range(1, U.N_LAYERS + 1)] + [slice(U.OUT_BASE, U.N_PARAMS)]
     e_em = [rel(g[s], g_em[s]) for s in blocks]
     e64 = [rel(g[s], g64[s]) for s in blocks]
+    print("[unet backward] vs emulating oracle " + str([round(e, 4) for e in e_em]), flush=True)
     assert max(e_em) <= GRAD_TOL_EMULATED, e_em
-    assert max(e64) <= GRAD_TOL_FP64, e64
+    # against exact FP64, block by block: no further than the emulating
+    # oracle's own distance from FP64 (BF16 storage) plus the distance to the
+    # emulating oracle (triangle inequality, in FP64's norm)
+    for i, s in enumerate(blocks):
+        bound = rel(g_em[s], g64[s]) + e_em[i] * np.linalg.norm(g_em[s]) / np.linalg.norm(g64[s])
+        assert e64[i] <= bound * (1 + 1e-9), (i, e64[i], bound)
 
 
 def _layer_inputs(net, act, B):
