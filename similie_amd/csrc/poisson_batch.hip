@@ -24,6 +24,7 @@
 // recomputes the residual in FP64.
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <new>
 #include <vector>
 
@@ -311,9 +312,12 @@ __global__ __launch_bounds__(NT) void k_tl_pre(int n, const T* __restrict__ wx, 
 
 // The problems that still use the network (not converged, not fallen back
 // to Jacobi) get consecutive slots of the network's batch, in problem order:
-// slot[b] = rank among them or −1, *count = their number.  One wavefront.
+// slot[b] = rank among them or −1, *count = their number.  For the network's
+// planar input each occupied slot also gets its problem (src) and the scale
+// of its residual channel, 1 / rms(r₁).  One wavefront.
 __global__ __launch_bounds__(64) void k_tl_compact(int B, const Scal* __restrict__ sc, int* __restrict__ slot,
-                                                   int* __restrict__ count)
+                                                   int* __restrict__ count, int* __restrict__ src,
+                                                   float* __restrict__ scale)
 {
     const int lane = threadIdx.x;
     int next = 0;
@@ -323,12 +327,18 @@ __global__ __launch_bounds__(64) void k_tl_compact(int B, const Scal* __restrict
         const unsigned long long mask = __ballot(active);
         const int rank = __popcll(mask & ((1ull << lane) - 1ull));
         if (b < B) slot[b] = active ? next + rank : -1;
+        if (active && src) {
+            const double rho = sc[b].rho1;
+            src[next + rank] = b;
+            scale[next + rank] = rho > 0.0 ? (float)(1.0 / rho) : 0.f;
+        }
         next += __popcll(mask);
     }
     if (lane == 0) *count = next;
 }
 
-// network input in the problem's slot: channel 0 = r₁ / rms(r₁), channel 1 =
+// network input in the problem's slot, interleaved (the planar input reads
+// the same values in place): channel 0 = r₁ / rms(r₁), channel 1 =
 // normalised log κ
 __global__ __launch_bounds__(NT) void k_tl_input(int n, const float* __restrict__ r1, const float* __restrict__ logk,
                                                  const Scal* __restrict__ sc, const int* __restrict__ slot,
@@ -590,7 +600,10 @@ struct sil_poisson2d_batch_s {
     int* done_host = nullptr;   // pinned
     double *p64[2] = {nullptr, nullptr}, *q64[2] = {nullptr, nullptr}, *z64 = nullptr;
     float *p32[2] = {nullptr, nullptr}, *q32[2] = {nullptr, nullptr}, *z32 = nullptr;
-    float* xin = nullptr;  // network input [B][n][n][2]
+    float* xin = nullptr;  // interleaved network input [B][n][n][2] (SIL_FCG_PLANAR=0)
+    int* slot_src = nullptr;     // problem of each occupied network slot
+    float* slot_scale = nullptr; // scale of its residual channel
+    bool interleaved = false;    // this solve forms the interleaved input
     float* zn = nullptr;   // network output [B][n][n], problems in their slots
     int* slot = nullptr;   // network slot of each problem, −1 when it needs none
     int* nact = nullptr;   // number of occupied slots
@@ -668,9 +681,24 @@ int fcg_iteration(Batch* h, sil_unet net, T* const* p, T* const* q, const T* wx,
     scalars(h, sil::PH_TL_RHO, tol2, max_iter, 0, s);
     // the network runs only on the problems that still use it, packed into
     // the first slots of its batch
-    hipLaunchKernelGGL(sil::k_tl_compact, dim3(1), dim3(64), 0, s, h->B, h->sc, h->slot, h->nact);
-    hipLaunchKernelGGL(sil::k_tl_input, grid_of(h), dim3(NT), 0, s, h->n, h->r1, h->logk, h->sc, h->slot, h->xin);
-    const int rc = sil::unet::forward_active(net, h->B, h->nact, h->xin, h->zn, s);
+    int rc;
+    if (h->interleaved) {
+        hipLaunchKernelGGL(sil::k_tl_compact, dim3(1), dim3(64), 0, s, h->B, h->sc, h->slot, h->nact, (int*)nullptr,
+                           (float*)nullptr);
+        hipLaunchKernelGGL(sil::k_tl_input, grid_of(h), dim3(NT), 0, s, h->n, h->r1, h->logk, h->sc, h->slot,
+                           h->xin);
+        rc = sil::unet::forward_active(net, h->B, h->nact, h->xin, h->zn, s);
+    } else {
+        // the network reads r₁ and log κ in place
+        hipLaunchKernelGGL(sil::k_tl_compact, dim3(1), dim3(64), 0, s, h->B, h->sc, h->slot, h->nact, h->slot_src,
+                           h->slot_scale);
+        sil::unet::PlanarInput in;
+        in.c0 = h->r1;
+        in.c1 = h->logk;
+        in.src = h->slot_src;
+        in.scale = h->slot_scale;
+        rc = sil::unet::forward_active_planar(net, h->B, h->nact, in, h->zn, s);
+    }
     if (rc != SIL_OK) return rc;
     hipLaunchKernelGGL(sil::k_tl_an<T>, grid_of(h), dim3(NT), 0, s, h->n, wx, wy, h->zn, h->slot, h->r1, h->an, h->sc,
                        h->part);
@@ -820,8 +848,12 @@ int sil_poisson2d_batch_solve(sil_poisson2d_batch h, const double* b, double* x,
         if (!learned) A(h->z64, h->total);
     }
     if (learned) {
-        A(h->xin, h->total * 2), A(h->zn, h->total), A(h->r1, h->total), A(h->an, h->total), A(h->zf, h->total);
-        A(h->slot, h->B), A(h->nact, 1);
+        // the interleaved network input only when asked for (SIL_FCG_PLANAR=0)
+        const char* e = std::getenv("SIL_FCG_PLANAR");
+        h->interleaved = e != nullptr && std::atoi(e) == 0;
+        if (h->interleaved) A(h->xin, h->total * 2);
+        A(h->zn, h->total), A(h->r1, h->total), A(h->an, h->total), A(h->zf, h->total);
+        A(h->slot, h->B), A(h->nact, 1), A(h->slot_src, h->B), A(h->slot_scale, h->B);
     }
     if (rc != SIL_OK) return rc;
 

@@ -35,6 +35,8 @@ namespace sil {
 
 namespace {
 
+using unet::PlanarInput;
+
 constexpr int C = 32;      // channels
 constexpr int CP = 40;     // padded channel stride in LDS
 constexpr int TH = 8;      // output tile rows
@@ -261,15 +263,20 @@ __host__ __device__ constexpr int64_t x2_image_bytes(int mode, int H, int W)
 // accesses go through buffer resources (no branch between a load and its
 // use).
 constexpr int kStageChunks = (TH + 2) * (TW + 2) * 4; // 16-byte chunks of one input tile
-constexpr int kPerThread = (kStageChunks + NT - 1) / NT;
-static_assert(TH * TW * 4 % NT == 0 && TH * TW % NT == 0, "the output stores of a tile are straight-line");
+// threads per workgroup: the pooling mode stages four raw chunks per chunk
+// and runs one workgroup per compute unit, so it uses eight wavefronts (one
+// output row each) to keep as many in flight as two workgroups of four
+constexpr int conv_threads(int mode) { return mode == ST_POOL ? 2 * NT : NT; }
+static_assert(TH * TW * 4 % (2 * NT) == 0 && TH * TW % NT == 0, "the output stores of a tile are straight-line");
 
 template <int MODE, bool HEAD = false>
-__global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ X2,
+__global__ __launch_bounds__(conv_threads(MODE), conv_wgs_per_cu(MODE)) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ X2,
                                                const bf16* __restrict__ Wt, const float* __restrict__ bias,
                                                bf16* __restrict__ Y, float* __restrict__ Yh, int Bmax,
                                                const int* __restrict__ dB, int H, int W, int relu)
 {
+    constexpr int NTH = conv_threads(MODE), MR = TH / (NTH / 64); // threads; output rows per wavefront
+    constexpr int kPerThread = (kStageChunks + NTH - 1) / NTH;
     // images: the first *dB of the batch when a device-side count is given
     const int B = dB ? min(*dB, Bmax) : Bmax;
     __shared__ __attribute__((aligned(16))) bf16 sX[2][(TH + 2) * (TW + 2) * CP];
@@ -313,7 +320,7 @@ __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16
         const __amdgpu_buffer_rsrc_t rx2 = buffer_of(x2b ? X2 + (size_t)b * (x2b / 2) : X, x2b);
 #pragma unroll
         for (int k = 0; k < kPerThread; ++k) {
-            const int c = tid + k * NT;
+            const int c = tid + k * NTH;
             const int pix = c >> 2, part = c & 3;
             const int gy = oy0 - 1 + pix / (TW + 2), gx = ox0 - 1 + pix % (TW + 2);
             const bool in = t < ntiles && c < kStageChunks && gy >= 0 && gy < H && gx >= 0 && gx < W;
@@ -323,7 +330,7 @@ __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16
     auto put = [&](bf16* dst, const RawBuf<MODE> (&pre)[kPerThread]) {
 #pragma unroll
         for (int k = 0; k < kPerThread; ++k) {
-            const int c = tid + k * NT;
+            const int c = tid + k * NTH;
             if (c < kStageChunks)
                 *reinterpret_cast<Chunk8*>(&dst[(c >> 2) * CP + (c & 3) * 8]) = combine_buf<MODE>(pre[k]);
         }
@@ -339,9 +346,9 @@ __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16
     __builtin_amdgcn_s_waitcnt(0x0F70);
     int buf = 0;
     for (; tile < ntiles; tile += G) {
-        f32x16 acc[2];
+        f32x16 acc[MR];
 #pragma unroll
-        for (int m = 0; m < 2; ++m)
+        for (int m = 0; m < MR; ++m)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[m][r] = 0.f;
         const bf16* cur = sX[buf];
@@ -351,8 +358,8 @@ __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16
 #pragma unroll
             for (int kh = 0; kh < 2; ++kh) {
 #pragma unroll
-                for (int m = 0; m < 2; ++m) {
-                    const int tr = 2 * wave + m + ky, tc = col + kx;
+                for (int m = 0; m < MR; ++m) {
+                    const int tr = MR * wave + m + ky, tc = col + kx;
                     const bf16x8 ax = *reinterpret_cast<const bf16x8*>(&cur[(tr * (TW + 2) + tc) * CP + kh * 16 + k0]);
                     acc[m] = Mfma32::mma(bw[tap][kh], ax, acc[m]);
                 }
@@ -367,8 +374,8 @@ __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16
             // pixel, one FP32 output per pixel straight from the registers
             const __amdgpu_buffer_rsrc_t ry = buffer_of(Yh + (size_t)b * H * W, (int64_t)H * W * 4);
 #pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                const int gy = oy0 + 2 * wave + m, gx = ox0 + col;
+            for (int m = 0; m < MR; ++m) {
+                const int gy = oy0 + MR * wave + m, gx = ox0 + col;
                 const float v = acc[m][0] + acc[m][1] + bh;
                 __builtin_amdgcn_raw_buffer_store_b32(
                     __builtin_bit_cast(unsigned, v), ry,
@@ -381,11 +388,11 @@ __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16
             // results → sX[buf] as [8 rows][32 pixels][32 channels] (stride CP)
             bf16* out_s = sX[buf];
 #pragma unroll
-            for (int m = 0; m < 2; ++m) {
+            for (int m = 0; m < MR; ++m) {
                 f32x16 v;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) v[r] = acc[m][r] + bv[r];
-                put_channels(&out_s[((2 * wave + m) * TW + col) * CP], v, relu != 0, true);
+                put_channels(&out_s[((MR * wave + m) * TW + col) * CP], v, relu != 0, true);
             }
             put(sX[buf ^ 1], pre);
             fetch(tile + 2 * G, pre);
@@ -393,8 +400,8 @@ __global__ __launch_bounds__(NT, conv_wgs_per_cu(MODE)) void k_conv33(const bf16
 
             const __amdgpu_buffer_rsrc_t ry = buffer_of(Y + (size_t)b * H * W * C, (int64_t)H * W * C * 2);
 #pragma unroll
-            for (int k = 0; k < TH * TW * 4 / NT; ++k) {
-                const int c = tid + k * NT;
+            for (int k = 0; k < TH * TW * 4 / NTH; ++k) {
+                const int c = tid + k * NTH;
                 const int pix = c >> 2, part = c & 3;
                 const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
                 const U32x4 v = *reinterpret_cast<const U32x4*>(&out_s[pix * CP + part * 8]);
@@ -603,10 +610,18 @@ constexpr int kHalo = (TH + 2) * (TW + 2);          // 340 a0 pixels
 constexpr int kHaloBlocks = (kHalo + 31) / 32;      // 11
 static_assert(TH * TW * 4 % NT == 0, "the output stores of a tile are straight-line");
 
-__global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x, const float* __restrict__ w_in,
-                                                     const float* __restrict__ b_in, const bf16* __restrict__ Wt,
-                                                     const float* __restrict__ bias, bf16* __restrict__ Y, int Bmax,
-                                                     const int* __restrict__ dB, int H, int W)
+// With PLANAR the input is two FP32 planes read through a slot map (image
+// b of the batch is source image src[b]): channel 0 = c0[src[b]] ·
+// scale[b], channel 1 = c1[src[b]] — the same values an interleaved input
+// formed by one multiply per pixel would hold.  The (src, scale) pair of a
+// tile is loaded one step before the tile's pixels, so the address of those
+// loads never waits on a memory round trip.
+template <bool PLANAR>
+__global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x, PlanarInput pin,
+                                                     const float* __restrict__ w_in, const float* __restrict__ b_in,
+                                                     const bf16* __restrict__ Wt, const float* __restrict__ bias,
+                                                     bf16* __restrict__ Y, int Bmax, const int* __restrict__ dB, int H,
+                                                     int W)
 {
     const int B = dB ? min(*dB, Bmax) : Bmax;
     __shared__ __attribute__((aligned(16))) bf16 sX[2][kHalo * CP];
@@ -639,24 +654,58 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
         ox0 = (t % tiles_x) * TW;
     };
     // the x pairs of tile t this thread stages (zero outside the image; a
-    // tile beyond the last reads zeros and is never used)
-    auto fetch = [&](int t, U32x2 (&xr)[kXPerThread]) {
+    // tile beyond the last reads zeros and is never used); xs = the scale of
+    // channel 0 (planar input)
+    // the (src, scale) pair of tile t as two uniform vector loads
+    const __amdgpu_buffer_rsrc_t rsrc = buffer_of(pin.src, PLANAR ? (int64_t)Bmax * 4 : 0);
+    const __amdgpu_buffer_rsrc_t rscl = buffer_of(pin.scale, PLANAR ? (int64_t)Bmax * 4 : 0);
+    auto meta = [&](int t, U32x2& m) {
+        const int b = min(t, ntiles - 1) / (tiles_x * tiles_y);
+        m = U32x2{__builtin_amdgcn_raw_buffer_load_b32(rsrc, (uint32_t)b * 4, 0, 0),
+                  __builtin_amdgcn_raw_buffer_load_b32(rscl, (uint32_t)b * 4, 0, 0)};
+    };
+    auto fetch = [&](int t, U32x2 (&xr)[kXPerThread], float& xs, const U32x2& m) {
         int b, oy0, ox0;
         coords(min(t, ntiles - 1), b, oy0, ox0); // t may lie beyond the last tile
-        const __amdgpu_buffer_rsrc_t rs = buffer_of(x + (size_t)b * H * W * 2, (int64_t)H * W * 8);
+        if constexpr (PLANAR) {
+            const unsigned m0 = m.x, m1 = m.y;
+            const int src = __builtin_amdgcn_readfirstlane((int)m0);
+            xs = f32_of(m1);
+            const __amdgpu_buffer_rsrc_t r0 = buffer_of(pin.c0 + (size_t)src * H * W, (int64_t)H * W * 4);
+            const __amdgpu_buffer_rsrc_t r1 = buffer_of(pin.c1 + (size_t)src * H * W, (int64_t)H * W * 4);
 #pragma unroll
-        for (int k = 0; k < kXPerThread; ++k) {
-            const int c = tid + k * NT;
-            const int gy = oy0 - 2 + c / XTW, gx = ox0 - 2 + c % XTW;
-            const bool in = c < kXPairs && t < ntiles && gy >= 0 && gy < H && gx >= 0 && gx < W;
-            xr[k] = __builtin_amdgcn_raw_buffer_load_b64(rs, in ? (uint32_t)((gy * W + gx) * 8) : kOutside, 0, 0);
+            for (int k = 0; k < kXPerThread; ++k) {
+                const int c = tid + k * NT;
+                const int gy = oy0 - 2 + c / XTW, gx = ox0 - 2 + c % XTW;
+                const bool in = c < kXPairs && t < ntiles && gy >= 0 && gy < H && gx >= 0 && gx < W;
+                const uint32_t off = in ? (uint32_t)((gy * W + gx) * 4) : kOutside;
+                xr[k] = U32x2{__builtin_amdgcn_raw_buffer_load_b32(r0, off, 0, 0),
+                              __builtin_amdgcn_raw_buffer_load_b32(r1, off, 0, 0)};
+            }
+        } else {
+            xs = 1.f;
+            const __amdgpu_buffer_rsrc_t rs = buffer_of(x + (size_t)b * H * W * 2, (int64_t)H * W * 8);
+#pragma unroll
+            for (int k = 0; k < kXPerThread; ++k) {
+                const int c = tid + k * NT;
+                const int gy = oy0 - 2 + c / XTW, gx = ox0 - 2 + c % XTW;
+                const bool in = c < kXPairs && t < ntiles && gy >= 0 && gy < H && gx >= 0 && gx < W;
+                xr[k] = __builtin_amdgcn_raw_buffer_load_b64(rs, in ? (uint32_t)((gy * W + gx) * 8) : kOutside, 0, 0);
+            }
         }
     };
-    auto put_x = [&](const U32x2 (&xr)[kXPerThread]) {
+    auto put_x = [&](const U32x2 (&xr)[kXPerThread], float xs) {
 #pragma unroll
         for (int k = 0; k < kXPerThread; ++k) {
             const int c = tid + k * NT;
-            if (c < kXPairs) *reinterpret_cast<U32x2*>(&sx[c * 2]) = xr[k];
+            if (c < kXPairs) {
+                if constexpr (PLANAR) {
+                    const unsigned lo = xr[k].x, hi = xr[k].y;
+                    *reinterpret_cast<float2*>(&sx[c * 2]) = make_float2(f32_of(lo) * xs, f32_of(hi));
+                } else {
+                    *reinterpret_cast<U32x2*>(&sx[c * 2]) = xr[k];
+                }
+            }
         }
     };
     // a0 of tile t on its halo, from the x tile in sx, into dst
@@ -677,14 +726,18 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
     };
 
     // prologue: a0 of the first tile in sX[0], x of the second tile in sx
-    U32x2 xr[kXPerThread];
-    fetch(tile, xr);
-    put_x(xr);
-    fetch(tile + G, xr);
+    U32x2 xr[kXPerThread], mt{0u, 0u};
+    float xs;
+    if constexpr (PLANAR) meta(tile, mt);
+    fetch(tile, xr, xs, mt);
+    put_x(xr, xs);
+    if constexpr (PLANAR) meta(tile + G, mt);
+    fetch(tile + G, xr, xs, mt);
+    if constexpr (PLANAR) meta(tile + 2 * G, mt);
     __syncthreads();
     make_a0(tile, sX[0]);
     __syncthreads();
-    put_x(xr);
+    put_x(xr, xs);
     __syncthreads();
     // the weights and biases are in registers before the walk: no wait
     // inside it is for them
@@ -692,7 +745,8 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
     int buf = 0;
     for (; tile < ntiles; tile += G) {
         const int next = tile + G;
-        fetch(next + G, xr);
+        fetch(next + G, xr, xs, mt);
+        if constexpr (PLANAR) meta(next + 2 * G, mt);
 
         f32x16 acc[2];
 #pragma unroll
@@ -725,7 +779,7 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
         }
         if (next < ntiles) make_a0(next, sX[buf ^ 1]); // from sx = x of tile next
         __syncthreads(); // outputs of this tile and a0 of the next complete, sx consumed
-        put_x(xr);       // x of tile next + G (zeros beyond the last tile)
+        put_x(xr, xs);   // x of tile next + G (zeros beyond the last tile)
 
         int b, oy0, ox0;
         coords(tile, b, oy0, ox0);
@@ -1047,7 +1101,7 @@ int conv33(int mode, const bf16* X, const bf16* X2, const bf16* Wt, const float*
         constexpr int M = decltype(m)::value;
         const long cap = kComputeUnits * conv_wgs_per_cu(M);
         const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
-        hipLaunchKernelGGL((k_conv33<M, false>), dim3(grid), dim3(NT), 0, s, X, X2, Wt, bias, Y, nullptr, B, dB, H,
+        hipLaunchKernelGGL((k_conv33<M, false>), dim3(grid), dim3(conv_threads(M)), 0, s, X, X2, Wt, bias, Y, nullptr, B, dB, H,
                            W, relu ? 1 : 0);
         SIL_LAUNCH_CHECK();
         return (int)SIL_OK;
@@ -1081,15 +1135,23 @@ int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, i
     return SIL_OK;
 }
 
-int conv_in33(const float* x, const float* w_in, const float* b_in, const bf16* Wt, const float* bias, bf16* Y, int B,
-              int H, int W, hipStream_t s, const int* dB)
+int conv_in33(const float* x, const PlanarInput* planar, const float* w_in, const float* b_in, const bf16* Wt,
+              const float* bias, bf16* Y, int B, int H, int W, hipStream_t s, const int* dB)
 {
     SIL_REQUIRE((int64_t)H * W * C * 2 <= kMaxBufferBytes, SIL_ERR_UNSUPPORTED,
                 "unet: %dx%d images exceed one buffer resource per image", H, W);
     const long ntiles = (long)((W + TW - 1) / TW) * ((H + TH - 1) / TH) * B;
     const long cap = kComputeUnits * 2;
     const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
-    hipLaunchKernelGGL(k_conv_in33, dim3(grid), dim3(NT), 0, s, x, w_in, b_in, Wt, bias, Y, B, dB, H, W);
+    if (planar) {
+        SIL_REQUIRE(planar->c0 && planar->c1 && planar->src && planar->scale, SIL_ERR_INVALID,
+                    "unet: incomplete planar input");
+        hipLaunchKernelGGL(k_conv_in33<true>, dim3(grid), dim3(NT), 0, s, nullptr, *planar, w_in, b_in, Wt, bias, Y, B,
+                           dB, H, W);
+    } else {
+        hipLaunchKernelGGL(k_conv_in33<false>, dim3(grid), dim3(NT), 0, s, x, PlanarInput{}, w_in, b_in, Wt, bias, Y,
+                           B, dB, H, W);
+    }
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }

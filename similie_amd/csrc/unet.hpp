@@ -35,10 +35,21 @@ int conv33_wgrad(int mode, const bf16* X, const bf16* X2, const bf16* dY, const 
 // 2 → 32 input convolution (FP32 input) and its weight gradient (dWb = [dW(9·32·2) | db(32)])
 int conv_in(const float* x, const float* w, const float* bias, bf16* y, int B, int H, int W, hipStream_t s,
             const int* dB = nullptr);
+// The network input as two FP32 planes [images][H][W] read through a slot
+// map: image b of the batch is source image src[b], with channel 0 =
+// c0 · scale[b] and channel 1 = c1 (src and scale: device memory, one entry
+// per image of the batch).
+struct PlanarInput {
+    const float* c0 = nullptr;
+    const float* c1 = nullptr;
+    const int* src = nullptr;
+    const float* scale = nullptr;
+};
 // relu(conv(relu(conv_in(x)))) in one kernel (a0 is formed in LDS and not
-// written): equal, bit for bit, to conv_in followed by conv33 in STAGE_PLAIN
-int conv_in33(const float* x, const float* w_in, const float* b_in, const bf16* Wt, const float* bias, bf16* Y, int B,
-              int H, int W, hipStream_t s, const int* dB = nullptr);
+// written): equal, bit for bit, to conv_in followed by conv33 in STAGE_PLAIN;
+// the input is x ([B][H][W][2]) or, when planar is given, its planes
+int conv_in33(const float* x, const PlanarInput* planar, const float* w_in, const float* b_in, const bf16* Wt,
+              const float* bias, bf16* Y, int B, int H, int W, hipStream_t s, const int* dB = nullptr);
 int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, int parts, float* dWb, int B, int H,
                   int W, hipStream_t s);
 // 32 → 1 output convolution (FP32 output) and its backward (dWb = [dW(9·32) | db])
@@ -69,6 +80,9 @@ int pack_head(const float* w, bf16* Wt, hipStream_t s);
 // The pass keeps no activations for a backward pass: the input layer and the
 // first 32→32 layer run fused.
 int forward_active(sil_unet_s* net, int batch, const int* d_count, const float* x, float* out, hipStream_t s);
+// the same pass on a planar input (no interleaved copy of it is formed)
+int forward_active_planar(sil_unet_s* net, int batch, const int* d_count, const PlanarInput& in, float* out,
+                          hipStream_t s);
 void net_shape(const struct ::sil_unet_s* net, int* max_batch, int* height, int* width);
 
 }  // namespace sil::unet

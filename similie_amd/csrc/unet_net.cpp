@@ -103,7 +103,8 @@ const float* BIAS(const sil_unet_s* u, int l) { return u->params + layer_off(l) 
 
 // keep = false: inference only — a0 is not stored (the input layer and
 // layer 1 run as one kernel), the other activations are
-int forward(sil_unet_s* u, int B, const float* x, float* out, hipStream_t s, const int* dB = nullptr, bool keep = true)
+int forward(sil_unet_s* u, int B, const float* x, float* out, hipStream_t s, const int* dB = nullptr, bool keep = true,
+            const sil::unet::PlanarInput* planar = nullptr)
 {
     namespace un = sil::unet;
     constexpr int P = un::STAGE_PLAIN, UP = un::STAGE_UPADD, PO = un::STAGE_POOL;
@@ -112,8 +113,8 @@ int forward(sil_unet_s* u, int B, const float* x, float* out, hipStream_t s, con
         UNET_TRY(un::conv_in(x, u->params, u->params + 9 * C * 2, u->a0, B, H[0], W[0], s, dB));
         UNET_TRY(un::conv33(P, u->a0, nullptr, WF(u, 1), BIAS(u, 1), u->e0, B, H[0], W[0], true, s, dB));
     } else {
-        UNET_TRY(un::conv_in33(x, u->params, u->params + 9 * C * 2, WF(u, 1), BIAS(u, 1), u->e0, B, H[0], W[0], s,
-                               dB));
+        UNET_TRY(un::conv_in33(x, planar, u->params, u->params + 9 * C * 2, WF(u, 1), BIAS(u, 1), u->e0, B, H[0],
+                               W[0], s, dB));
     }
     UNET_TRY(un::conv33(PO, u->e0, nullptr, WF(u, 2), BIAS(u, 2), u->a1, B, H[1], W[1], true, s, dB));
     UNET_TRY(un::conv33(P, u->a1, nullptr, WF(u, 3), BIAS(u, 3), u->e1, B, H[1], W[1], true, s, dB));
@@ -190,6 +191,14 @@ int sil::unet::forward_active(sil_unet_s* net, int batch, const int* d_count, co
                 "unet: null argument");
     UNET_TRY(check_batch(net, batch));
     return forward(net, batch, x, out, s, d_count, false);
+}
+
+int sil::unet::forward_active_planar(sil_unet_s* net, int batch, const int* d_count, const PlanarInput& in,
+                                     float* out, hipStream_t s)
+{
+    SIL_REQUIRE(net != nullptr && d_count != nullptr && out != nullptr, SIL_ERR_INVALID, "unet: null argument");
+    UNET_TRY(check_batch(net, batch));
+    return forward(net, batch, nullptr, out, s, d_count, false, &in);
 }
 
 void sil::unet::net_shape(const sil_unet_s* net, int* max_batch, int* height, int* width)

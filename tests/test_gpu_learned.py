@@ -352,6 +352,34 @@ def test_fcg_with_an_identity_network_behaves_as_cg(learned, method):
     assert max(r["iterations"] for r in res) <= 600
 
 
+@pytest.mark.parametrize("method", ["fcg_learned", "fcg_learned_mixed"])
+def test_fcg_planar_network_input_equals_interleaved(learned, method, monkeypatch):
+    """The network reads r1 / rms(r1) and log kappa in place through the slot
+    map (default) or from an interleaved copy (SIL_FCG_PLANAR=0): the same
+    values, so the whole solve — iterates, iteration counts, residuals —
+    is the same bit for bit, with problems leaving the network's batch as
+    they converge (ragged slot maps)."""
+    from similie_amd.preconditioner_training import DEFAULT_WEIGHTS
+
+    B, n = 6, 96
+    _, b = problems(B, n, 21)
+    # contrasts from mild to strong, so the problems converge at different
+    # replacement points and the slot map changes during the solve
+    rng = np.random.default_rng(22)
+    kappa = np.exp(np.linspace(0.1, 2.5, B)[:, None, None] * rng.standard_normal((B, n, n)))
+    net = learned.UNet(B, n, n)
+    net.set_params(torch.from_numpy(np.load(DEFAULT_WEIGHTS)))
+    pb = learned.PoissonBatch(torch.from_numpy(kappa))
+    bt = torch.from_numpy(b).cuda()
+    x1, r1, _ = pb.solve(bt, method=method, tol=1e-10, max_iterations=4000, net=net)
+    monkeypatch.setenv("SIL_FCG_PLANAR", "0")
+    x0, r0, _ = pb.solve(bt, method=method, tol=1e-10, max_iterations=4000, net=net)
+    check_solution(kappa, b, x1, r1)
+    assert torch.equal(x1, x0)
+    assert [r["iterations"] for r in r1] == [r["iterations"] for r in r0]
+    assert len(set(r["iterations"] for r in r1)) > 1  # problems left the batch at different times
+
+
 def test_fcg_falls_back_to_cg_when_the_network_is_useless(learned):
     """An untrained network is no preconditioner; flexible CG must still
     reach the FP64-verified tolerance through its plain-CG fallback."""
