@@ -526,15 +526,30 @@ __global__ void k_reduce_parts(const float* __restrict__ part, int parts, int n,
 // computed.
 template <int XW>
 __device__ __forceinline__ f32x16 conv_in_mma(const float* __restrict__ sx, int py, int px, const float (&wi)[9],
-                                              const float (&b16)[16])
+                                              const f32x16& init)
 {
     const int ci = Mfma32F32::b_k(threadIdx.x & 63);
-    f32x16 acc;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = b16[r];
+    f32x16 acc = init;
 #pragma unroll
     for (int t = 0; t < 9; ++t) acc = Mfma32F32::mma(wi[t], sx[((py + t / 3) * XW + px + t % 3) * 2 + ci], acc);
     return acc;
+}
+
+// the biases of the lane's 16 D rows (four runs of four channels) from a
+// 32-channel array in LDS
+__device__ __forceinline__ f32x16 row_biases(const float* __restrict__ sb)
+{
+    const int h = (threadIdx.x & 63) >> 5;
+    f32x16 v;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const float4 q = *reinterpret_cast<const float4*>(&sb[8 * g + 4 * h]);
+        v[4 * g] = q.x;
+        v[4 * g + 1] = q.y;
+        v[4 * g + 2] = q.z;
+        v[4 * g + 3] = q.w;
+    }
+    return v;
 }
 
 // the lane's A operands (one weight per tap) and the biases of its 16 D rows
@@ -575,7 +590,10 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
         const int row = 2 * wave + m;
-        const f32x16 acc = conv_in_mma<TW + 2>(sx, row, px, wi, b16);
+        f32x16 init;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) init[r] = b16[r];
+        const f32x16 acc = conv_in_mma<TW + 2>(sx, row, px, wi, init);
         put_channels(&sY[(row * TW + px) * CP], acc, true, true);
     }
     __syncthreads();
@@ -597,12 +615,14 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
 // k_conv33 does (same instructions in the same order), so the output equals
 // conv_in followed by k_conv33 bit for bit.
 //
-// The x tiles are read two tiles ahead through a buffer resource (pixels
-// outside the image read zero, no branch between a load and its use): the
-// loads of tile t + 2G (G = grid size) are issued at the top of the step of
-// tile t and written to LDS before its stores, so the 36 matrix instructions
-// of tile t and a0 of tile t + G cover them; the stores of a step are its
-// last accesses and have the next step's work to drain.
+// One step for tile t: layer 1 of t (from a0 of t in one LDS buffer) with
+// a0 of tile t + G (G = grid size) formed into the other buffer in the same
+// phase, interleaved in segments so that both matrix chains and the VALU
+// epilogues overlap; a barrier; the x tile of t + 2G written to LDS, the
+// loads of t + 3G issued, layer 1's outputs of t stored straight from the
+// registers; a barrier.  The x tiles are read through a buffer resource
+// (pixels outside the image read zero, no branch between a load and its
+// use) a whole step before they are written to LDS.
 constexpr int XTH = TH + 4, XTW = TW + 4;           // x tile of the fused layers
 constexpr int kXPairs = XTH * XTW;                  // 432 FP32 pairs
 constexpr int kXPerThread = (kXPairs + NT - 1) / NT;
@@ -626,6 +646,7 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
     const int B = dB ? min(*dB, Bmax) : Bmax;
     __shared__ __attribute__((aligned(16))) bf16 sX[2][kHalo * CP];
     __shared__ __attribute__((aligned(16))) float sx[kXPairs * 2];
+    __shared__ __attribute__((aligned(16))) float sbias[2][C]; // input layer, layer 1
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_x = (W + TW - 1) / TW, tiles_y = (H + TH - 1) / TH;
     const int ntiles = tiles_x * tiles_y * B;
@@ -633,8 +654,15 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
     int tile = blockIdx.x;
     if (tile >= ntiles) return; // uniform per workgroup
 
-    float wi[9], bi[16];
-    conv_in_weights(w_in, b_in, wi, bi);
+    // the biases live in LDS (read per block and per epilogue) to leave the
+    // registers to the weights and accumulators
+    float wi[9];
+    {
+        const int co = Mfma32F32::a_row(lane), ci = Mfma32F32::a_k(lane);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) wi[t] = w_in[(t * C + co) * 2 + ci];
+    }
+    if (tid < 2 * C) sbias[tid / C][tid % C] = tid < C ? b_in[tid] : bias[tid - C];
     const int col = Mfma32::a_row(lane);
     const int k0 = Mfma32::a_k0(lane);
     const int co_b = Mfma32::b_col(lane);
@@ -644,9 +672,6 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
 #pragma unroll
         for (int kh = 0; kh < 2; ++kh)
             bw[tap][kh] = *reinterpret_cast<const bf16x8*>(&Wt[(tap * C + co_b) * C + kh * 16 + k0]);
-    float bv[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) bv[r] = bias[Mfma32::d_row(lane, r)];
 
     auto coords = [&](int t, int& b, int& oy0, int& ox0) {
         b = t / (tiles_x * tiles_y);
@@ -709,23 +734,21 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
         }
     };
     // a0 of tile t on its halo, from the x tile in sx, into dst
-    auto make_a0 = [&](int t, bf16* __restrict__ dst) {
-        int b, oy0, ox0;
-        coords(t, b, oy0, ox0);
-        for (int j = wave; j < kHaloBlocks; j += NT / 64) {
-            // the lane's halo pixel (the last block runs past the halo: its
-            // extra lanes read the last pixel and write nothing)
-            const int hp = j * 32 + Mfma32F32::b_col(lane);
-            const int ha = min(hp, kHalo - 1);
-            const int hy = ha / (TW + 2), hx = ha % (TW + 2);
-            const f32x16 acc = conv_in_mma<XTW>(sx, hy, hx, wi, bi);
-            const int gy = oy0 - 1 + hy, gx = ox0 - 1 + hx;
-            const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
-            if (hp < kHalo) put_channels(&dst[hp * CP], acc, true, inside);
-        }
+    // block j (32 halo pixels) of a0 of the tile at (oy0, ox0), from the x
+    // tile in sx, into dst; the last block runs past the halo: its extra
+    // lanes read the last pixel and write nothing
+    auto a0_block = [&](int j, int oy0, int ox0, bf16* __restrict__ dst) {
+        const int hp = j * 32 + Mfma32F32::b_col(lane);
+        const int ha = min(hp, kHalo - 1);
+        const int hy = ha / (TW + 2), hx = ha % (TW + 2);
+        const f32x16 acc = conv_in_mma<XTW>(sx, hy, hx, wi, row_biases(sbias[0]));
+        const int gy = oy0 - 1 + hy, gx = ox0 - 1 + hx;
+        const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        if (hp < kHalo) put_channels(&dst[hp * CP], acc, true, inside);
     };
 
-    // prologue: a0 of the first tile in sX[0], x of the second tile in sx
+    // prologue: a0 of the first tile in sX[0], x of the second tile in sx,
+    // x of the third in flight
     U32x2 xr[kXPerThread], mt{0u, 0u};
     float xs;
     if constexpr (PLANAR) meta(tile, mt);
@@ -735,9 +758,15 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
     fetch(tile + G, xr, xs, mt);
     if constexpr (PLANAR) meta(tile + 2 * G, mt);
     __syncthreads();
-    make_a0(tile, sX[0]);
+    {
+        int b, oy0, ox0;
+        coords(tile, b, oy0, ox0);
+        for (int j = wave; j < kHaloBlocks; j += NT / 64) a0_block(j, oy0, ox0, sX[0]);
+    }
     __syncthreads();
     put_x(xr, xs);
+    fetch(tile + 2 * G, xr, xs, mt);
+    if constexpr (PLANAR) meta(tile + 3 * G, mt);
     __syncthreads();
     // the weights and biases are in registers before the walk: no wait
     // inside it is for them
@@ -745,9 +774,14 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
     int buf = 0;
     for (; tile < ntiles; tile += G) {
         const int next = tile + G;
-        fetch(next + G, xr, xs, mt);
-        if constexpr (PLANAR) meta(next + 2 * G, mt);
+        int nb, noy0, nox0;
+        coords(min(next, ntiles - 1), nb, noy0, nox0);
+        const bool has_next = next < ntiles;
 
+        // one phase of matrix work: layer 1 of this tile from sX[buf], in
+        // three segments of three taps, each followed by one a0 block of
+        // the next tile (from sx into sX[buf ^ 1]) — independent chains, so
+        // the VALU epilogue of a block runs while layer 1's products do
         f32x16 acc[2];
 #pragma unroll
         for (int m = 0; m < 2; ++m)
@@ -755,48 +789,58 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
             for (int r = 0; r < 16; ++r) acc[m][r] = 0.f;
         const bf16* cur = sX[buf];
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const int ky = tap / 3, kx = tap % 3;
+        for (int q = 0; q < 3; ++q) {
 #pragma unroll
-            for (int kh = 0; kh < 2; ++kh) {
+            for (int tap = 3 * q; tap < 3 * q + 3; ++tap) {
+                const int ky = tap / 3, kx = tap % 3;
 #pragma unroll
-                for (int m = 0; m < 2; ++m) {
-                    const int tr = 2 * wave + m + ky, tc = col + kx;
-                    const bf16x8 ax = *reinterpret_cast<const bf16x8*>(&cur[(tr * (TW + 2) + tc) * CP + kh * 16 + k0]);
-                    acc[m] = Mfma32::mma(bw[tap][kh], ax, acc[m]);
+                for (int kh = 0; kh < 2; ++kh) {
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        const int tr = 2 * wave + m + ky, tc = col + kx;
+                        const bf16x8 ax =
+                            *reinterpret_cast<const bf16x8*>(&cur[(tr * (TW + 2) + tc) * CP + kh * 16 + k0]);
+                        acc[m] = Mfma32::mma(bw[tap][kh], ax, acc[m]);
+                    }
                 }
             }
+            const int j = wave + (NT / 64) * q;
+            if (has_next && j < kHaloBlocks) a0_block(j, noy0, nox0, sX[buf ^ 1]);
         }
-        __syncthreads(); // sX[buf] (a0 of this tile) is consumed
+        __syncthreads(); // sX[buf] consumed, a0 of the next tile complete, sx consumed
 
-        bf16* out_s = sX[buf];
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            f32x16 v;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) v[r] = acc[m][r] + bv[r];
-            put_channels(&out_s[((2 * wave + m) * TW + col) * CP], v, true, true);
-        }
-        if (next < ntiles) make_a0(next, sX[buf ^ 1]); // from sx = x of tile next
-        __syncthreads(); // outputs of this tile and a0 of the next complete, sx consumed
         put_x(xr, xs);   // x of tile next + G (zeros beyond the last tile)
+        fetch(next + 2 * G, xr, xs, mt);
+        if constexpr (PLANAR) meta(next + 3 * G, mt);
 
+        // layer 1's outputs straight from the registers: the lane's pixel,
+        // four runs of four channels as 8-byte stores (dropped outside the
+        // image)
         int b, oy0, ox0;
         coords(tile, b, oy0, ox0);
         {
-            // straight-line masked stores: pixels outside the image go to
-            // an offset beyond the image's resource and are dropped
             const __amdgpu_buffer_rsrc_t ry = buffer_of(Y + (size_t)b * H * W * C, (int64_t)H * W * C * 2);
+            const int h = lane >> 5;
+            const f32x16 bv = row_biases(sbias[1]);
 #pragma unroll
-            for (int k = 0; k < TH * TW * 4 / NT; ++k) {
-                const int c = tid + k * NT;
-                const int pix = c >> 2, part = c & 3;
-                const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
-                const U32x4 v = *reinterpret_cast<const U32x4*>(&out_s[pix * CP + part * 8]);
-                __builtin_amdgcn_raw_buffer_store_b128(
-                    v, ry, (gy < H && gx < W) ? (uint32_t)(((gy * W + gx) * C + part * 8) * 2) : kOutside, 0, 0);
+            for (int m = 0; m < 2; ++m) {
+                const int gy = oy0 + 2 * wave + m, gx = ox0 + col;
+                const bool in = gy < H && gx < W;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    float v[4];
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) v[jj] = fmaxf(acc[m][4 * g + jj] + bv[4 * g + jj], 0.f);
+                    const unsigned lo = (unsigned)__builtin_bit_cast(unsigned short, to_bf16(v[0])) |
+                                        ((unsigned)__builtin_bit_cast(unsigned short, to_bf16(v[1])) << 16);
+                    const unsigned hi = (unsigned)__builtin_bit_cast(unsigned short, to_bf16(v[2])) |
+                                        ((unsigned)__builtin_bit_cast(unsigned short, to_bf16(v[3])) << 16);
+                    __builtin_amdgcn_raw_buffer_store_b64(
+                        U32x2{lo, hi}, ry, in ? (uint32_t)(((gy * W + gx) * C + 8 * g + 4 * h) * 2) : kOutside, 0, 0);
+                }
             }
         }
+        __syncthreads(); // sx holds the x tile of next + G for the next phase
         buf ^= 1;
     }
 }
