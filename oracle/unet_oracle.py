@@ -126,18 +126,29 @@ def conv32(x, w, b):
     return conv(np.float32(x), np.float32(w), np.float32(b)).astype(np.float32).astype(np.float64)
 
 
-def conv_in_fma32(x, w, b):
-    """The first layer in FP32 arithmetic: per output channel the chain
-    a = b; a = fma(w[t][c][ci], x_t[ci], a) over taps t = 0..8 and input
-    channels ci = 0, 1, each step emulated in FP64 (the product of two FP32
-    values is exact there) with one rounding to FP32.  The GPU forms the
-    same sum on its FP32 matrix cores (bias first, one tap — both input
-    channels — per step, taps in order), which differs from this chain only
-    in FP32 rounding: after the BF16 rounding of the activation the two
-    agree on every element of the layer-by-layer test."""
+def split_bf16(a):
+    """a (as FP32) as the two-term BF16 sum hi + lo, hi = bf16(a),
+    lo = bf16(a - hi): a to 16 significant bits, |a - hi - lo| <= 2^-16 |a|
+    (a - hi is exact in FP32).  The GPU carries the input layer's FP32
+    operands in this form (similie_amd/csrc/unet.hip, split_bf16)."""
+    a = f32(a)
+    hi = bf16(a)
+    return hi + bf16(a - hi)
+
+
+def conv_in_split32(x, w, b):
+    """The first layer as the GPU forms it: every input value and weight
+    replaced by its split (split_bf16), and per output channel the chain
+    a = b; a = fp32(x~_t[ci] * w~[t][c][ci] + a) over taps t = 0..8 and input
+    channels ci = 0, 1, each product exact in FP64 (two 16-bit significands)
+    with one rounding to FP32 per step.  The GPU sums the same four exact
+    BF16 products per x*w on its BF16 matrix cores (bias first, two taps per
+    instruction, taps in order), which differs from this chain only in FP32
+    rounding: after the BF16 rounding of the activation the two agree on
+    every element of the layer-by-layer test but a handful."""
     B, H, W, _ = x.shape
-    x = f32(x)
-    w = f32(w)
+    x = split_bf16(x)
+    w = split_bf16(w)
     xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
     a = np.broadcast_to(f32(b), (B, H, W, w.shape[1])).copy()
     for t in range(9):
@@ -195,13 +206,15 @@ def conv_bwd32_reordered(x, w, dz):
     return tuple(a.astype(np.float32).astype(np.float64) for a in (dxp[:, 1:-1, 1:-1, :], dw, db))
 
 
-def forward(params, x, emulate=False, reordered=False):
-    """Network output [B,H,W] and the activations, x [B,H,W,2].
+def forward(params, x, emulate=False, reordered=False, a0=None):
+    """Network output [B,H,W] and the activations, x [B,H,W,2]; with a0 the
+    input layer's activation is taken as given (a GPU's, say) and the rest
+    computed from it.
 
     emulate=True computes as the GPU kernels do: FP32 input and first/last
     layer weights, BF16 weights of the 32->32 layers, BF16 activations and
-    FP32 accumulation (the first layer as an FP32 multiply-add chain in the
-    GPU's tap order).  BF16 storage makes a network of this depth sensitive
+    FP32 accumulation (the first layer on two-term BF16 splits of its FP32
+    input and weights, summed in FP32 in the GPU's tap order).  BF16 storage makes a network of this depth sensitive
     to the last bit of each accumulation (one in ten activations of the
     coarse levels rounds to the other neighbouring BF16 value when only the
     accumulation order changes), so only an oracle that accumulates in FP32
@@ -212,8 +225,10 @@ def forward(params, x, emulate=False, reordered=False):
     win, bin_, layers, wout, bout = unpack(params)
     act = {"x": qw(np.asarray(x, dtype=np.float64))}
     for op, src, dst in GRAPH:
-        if op == "in":
-            act[dst] = q(relu(conv_in_fma32(act[src], win, bin_) if emulate else conv(act[src], win, bin_)))
+        if op == "in" and a0 is not None:
+            act[dst] = np.asarray(a0, dtype=np.float64)
+        elif op == "in":
+            act[dst] = q(relu(conv_in_split32(act[src], win, bin_) if emulate else conv(act[src], win, bin_)))
         elif op == "pool":
             act[dst] = q(qw(pool(act[src])))
         elif op == "up":

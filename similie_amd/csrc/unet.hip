@@ -65,31 +65,20 @@ __device__ __forceinline__ Chunk8 zero_chunk()
     return c;
 }
 
-// four consecutive channels as BF16, one 8-byte LDS write
-__device__ __forceinline__ void put4_bf16(bf16* __restrict__ dst, float a, float b, float c, float d)
-{
-    const unsigned lo = (unsigned)__builtin_bit_cast(unsigned short, to_bf16(a)) |
-                        ((unsigned)__builtin_bit_cast(unsigned short, to_bf16(b)) << 16);
-    const unsigned hi = (unsigned)__builtin_bit_cast(unsigned short, to_bf16(c)) |
-                        ((unsigned)__builtin_bit_cast(unsigned short, to_bf16(d)) << 16);
-    *reinterpret_cast<U32x2*>(dst) = U32x2{lo, hi};
-}
-
 // a D fragment of the transposed products (lane: one pixel, the channels
-// d_row(l, r)) after bias, as BF16 with optional ReLU, zeros when !keep,
-// into the pixel's LDS row px_row (32 channels)
+// d_row(l, r)) after bias, as BF16 with optional ReLU (on the rounded
+// pairs), zeros when !keep, into the pixel's LDS row px_row (32 channels)
 __device__ __forceinline__ void put_channels(bf16* __restrict__ px_row, const f32x16& acc, bool relu, bool keep)
 {
     const int h = (threadIdx.x & 63) >> 5;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-        float v[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float a = relu ? fmaxf(acc[4 * g + j], 0.f) : acc[4 * g + j];
-            v[j] = keep ? a : 0.f;
+        unsigned w0 = pack_bf16x2(acc[4 * g], acc[4 * g + 1]), w1 = pack_bf16x2(acc[4 * g + 2], acc[4 * g + 3]);
+        if (relu) {
+            w0 = relu_bf16x2(w0);
+            w1 = relu_bf16x2(w1);
         }
-        put4_bf16(px_row + 8 * g + 4 * h, v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<U32x2*>(px_row + 8 * g + 4 * h) = keep ? U32x2{w0, w1} : U32x2{0u, 0u};
     }
 }
 
@@ -141,26 +130,54 @@ __device__ __forceinline__ void zero_raw(RawChunk<MODE>& r)
     for (int i = 0; i < RawChunk<MODE>::n; ++i) r.c[i] = zero_chunk();
 }
 
+// the combination on packed words (two BF16 channels per word, the lower
+// channel in the low half), each sum formed in FP32 and rounded once
+template <int MODE>
+__device__ __forceinline__ U32x4 combine_words(const U32x4 (&w)[RawChunk<MODE>::n])
+{
+    if constexpr (MODE == ST_PLAIN) {
+        return w[0];
+    } else {
+        unsigned o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned a = w[0][k], b = w[1][k];
+            if constexpr (MODE == ST_POOL) {
+                const unsigned c = w[2][k], d = w[3][k];
+                o[k] = pack_bf16x2(0.25f * (bf16_lo(a) + bf16_lo(b) + bf16_lo(c) + bf16_lo(d)),
+                                   0.25f * (bf16_hi(a) + bf16_hi(b) + bf16_hi(c) + bf16_hi(d)));
+            } else if constexpr (MODE == ST_UPADD) {
+                o[k] = pack_bf16x2(bf16_lo(b) + bf16_lo(a), bf16_hi(b) + bf16_hi(a));
+            } else { // ST_MASK: a channel whose mask value is not > 0 becomes +0
+                o[k] = a & ((bf16_lo(b) > 0.f ? 0x0000FFFFu : 0u) | (bf16_hi(b) > 0.f ? 0xFFFF0000u : 0u));
+            }
+        }
+        return U32x4{o[0], o[1], o[2], o[3]};
+    }
+}
+
+__device__ __forceinline__ U32x4 words_of(const Chunk8& c)
+{
+    unsigned w[4];
+    __builtin_memcpy(w, &c, sizeof(w));
+    return U32x4{w[0], w[1], w[2], w[3]};
+}
+
+__device__ __forceinline__ Chunk8 chunk_of(U32x4 v)
+{
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+    Chunk8 c;
+    __builtin_memcpy(&c, w, sizeof(c));
+    return c;
+}
+
 template <int MODE>
 __device__ __forceinline__ Chunk8 combine(const RawChunk<MODE>& r)
 {
-    Chunk8 v = r.c[0];
-    if (MODE == ST_POOL) {
+    U32x4 w[RawChunk<MODE>::n];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const float s = from_bf16(r.c[0].v[i]) + from_bf16(r.c[1].v[i]) + from_bf16(r.c[2].v[i]) +
-                            from_bf16(r.c[3].v[i]);
-            v.v[i] = to_bf16(0.25f * s);
-        }
-    } else if (MODE == ST_UPADD) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) v.v[i] = to_bf16(from_bf16(r.c[1].v[i]) + from_bf16(r.c[0].v[i]));
-    } else if (MODE == ST_MASK) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (!(from_bf16(r.c[1].v[i]) > 0.f)) v.v[i] = to_bf16(0.f);
-    }
-    return v;
+    for (int i = 0; i < RawChunk<MODE>::n; ++i) w[i] = words_of(r.c[i]);
+    return chunk_of(combine_words<MODE>(w));
 }
 
 template <int MODE>
@@ -191,14 +208,6 @@ __device__ __forceinline__ void stage_tile(bf16* __restrict__ sX, const bf16* __
 // of X and X2: a chunk outside the image (in == false) reads zeros and needs
 // no branch, so the compiler's wait before the chunks are combined counts
 // only the loads issued before them.
-__device__ __forceinline__ Chunk8 chunk_of(U32x4 v)
-{
-    const unsigned w[4] = {v.x, v.y, v.z, v.w};
-    Chunk8 c;
-    __builtin_memcpy(&c, w, sizeof(c));
-    return c;
-}
-
 template <int MODE>
 struct RawBuf {
     U32x4 c[RawChunk<MODE>::n];
@@ -231,10 +240,7 @@ __device__ __forceinline__ void load_raw_buf(RawBuf<MODE>& r, __amdgpu_buffer_rs
 template <int MODE>
 __device__ __forceinline__ Chunk8 combine_buf(const RawBuf<MODE>& r)
 {
-    RawChunk<MODE> c;
-#pragma unroll
-    for (int i = 0; i < RawChunk<MODE>::n; ++i) c.c[i] = chunk_of(r.c[i]);
-    return combine<MODE>(c);
+    return chunk_of(combine_words<MODE>(r.c));
 }
 
 // bytes of one image of X and of X2 for a staging mode on H×W outputs
@@ -513,25 +519,78 @@ __global__ void k_reduce_parts(const float* __restrict__ part, int parts, int n,
 
 // a0 = relu(conv_in(x)): x FP32 [B][H][W][2], weights [tap][co][ci=2].
 //
-// The input layer on the FP32 matrix cores (v_mfma_f32_32x32x2_f32: K = 2 is
-// the two input channels of one tap) with the weights as A (32 output
-// channels × 2) and 32 pixels as B: lane l supplies the weight of output
-// channel l % 32, input channel l / 32, and input channel l / 32 of pixel
-// l % 32, read from a staged x tile (FP32 pairs, row stride XW) at
-// (py + ky, px + kx) for tap (ky, kx).  Accumulation starts at the bias and
-// adds one tap per instruction, taps in order 0..8.  The result holds, per
-// lane, pixel l % 32 and the 16 output channels Mfma32F32::d_row(l, r):
-// four runs of four consecutive channels, each one 8-byte write.  Every
-// kernel that forms a0 uses this sequence, so a0 is the same wherever it is
-// computed.
-template <int XW>
-__device__ __forceinline__ f32x16 conv_in_mma(const float* __restrict__ sx, int py, int px, const float (&wi)[9],
-                                              const f32x16& init)
+// The input layer on the BF16 matrix cores with FP32 operands carried as
+// two-term BF16 sums: v ≈ v_hi + v_lo, v_hi = bf16(v), v_lo = bf16(v − v_hi)
+// (v − v_hi is exact in FP32, so the pair holds v to 16 significant bits,
+// |v − v_hi − v_lo| ≤ 2⁻¹⁶|v|), and x·w as the four BF16 products
+// x_hi·w_hi + x_hi·w_lo + x_lo·w_hi + x_lo·w_lo, each exact in FP32, summed
+// by v_mfma_f32_32x32x16_bf16 in FP32.  One tap's two input channels fill
+// eight consecutive K rows:
+//   B (x, per pixel)   [x0h, x0h, x0l, x0l, x1h, x1h, x1l, x1l]
+//   A (w, per channel) [w0h, w0l, w0h, w0l, w1h, w1l, w1h, w1l]
+// so an instruction covers two taps (lanes 0–31 tap 2q, lanes 32–63 tap
+// 2q + 1, K = 16) and five cover the nine; the upper half of the fifth has
+// zero weights (its B is tap 8's again, finite).  x is staged split: one
+// 16-byte B row per pixel (split_pair), the tile's row stride XW pixels.
+// The weights are A (32 output channels) and the pixels B, so the result
+// holds, per lane, pixel l % 32 and the 16 output channels Mfma32::d_row(l,
+// r) — four runs of four consecutive channels, each one 8-byte write.
+// Accumulation starts at the bias and adds taps (0,1), (2,3), (4,5), (6,7),
+// (8) in that order.  Every kernel that forms a0 uses this sequence, so a0
+// is the same wherever it is computed.  Nine FP32 instructions of K = 2
+// (64 cycles each) did the same work in 576 matrix-core cycles; these five
+// take 160.
+__device__ __forceinline__ unsigned split_bf16(float v) // hi in the low half, lo in the high half
 {
-    const int ci = Mfma32F32::b_k(threadIdx.x & 63);
+    // v − hi stays a subtraction of the rounded v: fused with a multiply
+    // that forms v (the planar input's scale), it would see the unrounded
+    // product and give another lo than a stored copy of v does
+#pragma clang fp contract(off)
+    const bf16 hi = to_bf16(v);
+    const bf16 lo = to_bf16(v - from_bf16(hi));
+    return (unsigned)__builtin_bit_cast(unsigned short, hi) | ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
+}
+
+// the staged B row of one x pixel (x0, x1)
+__device__ __forceinline__ U32x4 split_pair(float x0, float x1)
+{
+#pragma clang fp contract(off) // as in split_bf16
+    const unsigned h = pack_bf16x2(x0, x1);
+    const unsigned l = pack_bf16x2(x0 - bf16_lo(h), x1 - bf16_hi(h));
+    const unsigned h0 = h & 0xFFFFu, h1 = h >> 16, l0 = l & 0xFFFFu, l1 = l >> 16;
+    return U32x4{h0 | (h0 << 16), l0 | (l0 << 16), h1 | (h1 << 16), l1 | (l1 << 16)};
+}
+
+constexpr int kInMfma = 5; // instructions of the input layer per 32-pixel block
+
+// the lane's A operands of the input layer: output channel l % 32, taps 2q + l / 32
+__device__ __forceinline__ void conv_in_split_weights(const float* __restrict__ w, bf16x8 (&wi)[kInMfma])
+{
+    const int lane = threadIdx.x & 63;
+    const int co = Mfma32::a_row(lane), h = Mfma32::a_k0(lane) / 8;
+#pragma unroll
+    for (int q = 0; q < kInMfma; ++q) {
+        const int t = 2 * q + h;
+        U32x4 v{0u, 0u, 0u, 0u};
+        if (t < 9) {
+            const unsigned s0 = split_bf16(w[(t * C + co) * 2]), s1 = split_bf16(w[(t * C + co) * 2 + 1]);
+            v = U32x4{s0, s0, s1, s1};
+        }
+        wi[q] = __builtin_bit_cast(bf16x8, v);
+    }
+}
+
+template <int XW>
+__device__ __forceinline__ f32x16 conv_in_mma(const U32x4* __restrict__ sx, int py, int px,
+                                              const bf16x8 (&wi)[kInMfma], const f32x16& init)
+{
+    const int h = Mfma32::b_k0(threadIdx.x & 63) / 8;
     f32x16 acc = init;
 #pragma unroll
-    for (int t = 0; t < 9; ++t) acc = Mfma32F32::mma(wi[t], sx[((py + t / 3) * XW + px + t % 3) * 2 + ci], acc);
+    for (int q = 0; q < kInMfma; ++q) {
+        const int t = min(2 * q + h, 8);
+        acc = Mfma32::mma(wi[q], __builtin_bit_cast(bf16x8, sx[(py + t / 3) * XW + px + t % 3]), acc);
+    }
     return acc;
 }
 
@@ -552,18 +611,6 @@ __device__ __forceinline__ f32x16 row_biases(const float* __restrict__ sb)
     return v;
 }
 
-// the lane's A operands (one weight per tap) and the biases of its 16 D rows
-__device__ __forceinline__ void conv_in_weights(const float* __restrict__ w, const float* __restrict__ bias,
-                                                float (&wi)[9], float (&b16)[16])
-{
-    const int lane = threadIdx.x & 63;
-    const int co = Mfma32F32::a_row(lane), ci = Mfma32F32::a_k(lane);
-#pragma unroll
-    for (int t = 0; t < 9; ++t) wi[t] = w[(t * C + co) * 2 + ci];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) b16[r] = bias[Mfma32F32::d_row(lane, r)];
-}
-
 // A workgroup stages the (TH+2)×(TW+2) tile of x around an 8×32-pixel output
 // tile; each wavefront computes two 32-pixel rows, and the results go through
 // LDS so that every global store is 8 channels (16 bytes).
@@ -571,7 +618,7 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
                                                 const float* __restrict__ bias, bf16* __restrict__ y, int H, int W,
                                                 const int* __restrict__ dB)
 {
-    __shared__ float sx[(TH + 2) * (TW + 2) * 2];
+    __shared__ U32x4 sx[(TH + 2) * (TW + 2)];
     __shared__ __attribute__((aligned(16))) bf16 sY[TH * TW * CP];
     const int b = blockIdx.z, oy0 = blockIdx.y * TH, ox0 = blockIdx.x * TW;
     if (dB && b >= *dB) return; // uniform per workgroup
@@ -580,13 +627,15 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
         const int gy = oy0 - 1 + i / (TW + 2), gx = ox0 - 1 + i % (TW + 2);
         const bool ok = gy >= 0 && gy < H && gx >= 0 && gx < W;
         const size_t q = ((size_t)b * H + gy) * W + gx;
-        sx[i * 2] = ok ? x[q * 2] : 0.f;
-        sx[i * 2 + 1] = ok ? x[q * 2 + 1] : 0.f;
+        sx[i] = split_pair(ok ? x[q * 2] : 0.f, ok ? x[q * 2 + 1] : 0.f);
     }
-    float wi[9], b16[16];
-    conv_in_weights(w, bias, wi, b16);
+    bf16x8 wi[kInMfma];
+    conv_in_split_weights(w, wi);
+    float b16[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) b16[r] = bias[Mfma32::d_row(lane, r)];
     __syncthreads();
-    const int px = Mfma32F32::b_col(lane);
+    const int px = Mfma32::b_col(lane);
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
         const int row = 2 * wave + m;
@@ -609,8 +658,9 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
 // e0 = relu(conv(relu(conv_in(x)))): the input layer and the first 32→32
 // layer in one persistent kernel, for passes that keep no activations (a0 is
 // never written).  Per output tile a workgroup stages the (TH+4)×(TW+4) tile
-// of x (FP32 pairs), forms a0 on the (TH+2)×(TW+2) halo in LDS with
-// conv_in_mma — eleven 32-pixel blocks, zero outside the image as the
+// of x (each pixel's FP32 pair as its split B row), forms a0 on the
+// (TH+2)×(TW+2) halo in LDS with conv_in_mma — eleven 32-pixel blocks,
+// zero outside the image as the
 // layer's zero padding requires — and runs the 32→32 layer on it exactly as
 // k_conv33 does (same instructions in the same order), so the output equals
 // conv_in followed by k_conv33 bit for bit.
@@ -624,7 +674,7 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
 // (pixels outside the image read zero, no branch between a load and its
 // use) a whole step before they are written to LDS.
 constexpr int XTH = TH + 4, XTW = TW + 4;           // x tile of the fused layers
-constexpr int kXPairs = XTH * XTW;                  // 432 FP32 pairs
+constexpr int kXPairs = XTH * XTW;                  // 432 x pixels
 constexpr int kXPerThread = (kXPairs + NT - 1) / NT;
 constexpr int kHalo = (TH + 2) * (TW + 2);          // 340 a0 pixels
 constexpr int kHaloBlocks = (kHalo + 31) / 32;      // 11
@@ -645,7 +695,7 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
 {
     const int B = dB ? min(*dB, Bmax) : Bmax;
     __shared__ __attribute__((aligned(16))) bf16 sX[2][kHalo * CP];
-    __shared__ __attribute__((aligned(16))) float sx[kXPairs * 2];
+    __shared__ U32x4 sx[kXPairs]; // x pixels split for the BF16 products
     __shared__ __attribute__((aligned(16))) float sbias[2][C]; // input layer, layer 1
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_x = (W + TW - 1) / TW, tiles_y = (H + TH - 1) / TH;
@@ -656,12 +706,8 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
 
     // the biases live in LDS (read per block and per epilogue) to leave the
     // registers to the weights and accumulators
-    float wi[9];
-    {
-        const int co = Mfma32F32::a_row(lane), ci = Mfma32F32::a_k(lane);
-#pragma unroll
-        for (int t = 0; t < 9; ++t) wi[t] = w_in[(t * C + co) * 2 + ci];
-    }
+    bf16x8 wi[kInMfma];
+    conv_in_split_weights(w_in, wi);
     if (tid < 2 * C) sbias[tid / C][tid % C] = tid < C ? b_in[tid] : bias[tid - C];
     const int col = Mfma32::a_row(lane);
     const int k0 = Mfma32::a_k0(lane);
@@ -724,12 +770,10 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
         for (int k = 0; k < kXPerThread; ++k) {
             const int c = tid + k * NT;
             if (c < kXPairs) {
-                if constexpr (PLANAR) {
-                    const unsigned lo = xr[k].x, hi = xr[k].y;
-                    *reinterpret_cast<float2*>(&sx[c * 2]) = make_float2(f32_of(lo) * xs, f32_of(hi));
-                } else {
-                    *reinterpret_cast<U32x2*>(&sx[c * 2]) = xr[k];
-                }
+                if constexpr (PLANAR)
+                    sx[c] = split_pair(f32_of(xr[k].x) * xs, f32_of(xr[k].y));
+                else
+                    sx[c] = split_pair(f32_of(xr[k].x), f32_of(xr[k].y));
             }
         }
     };
@@ -738,7 +782,7 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
     // tile in sx, into dst; the last block runs past the halo: its extra
     // lanes read the last pixel and write nothing
     auto a0_block = [&](int j, int oy0, int ox0, bf16* __restrict__ dst) {
-        const int hp = j * 32 + Mfma32F32::b_col(lane);
+        const int hp = j * 32 + Mfma32::b_col(lane);
         const int ha = min(hp, kHalo - 1);
         const int hy = ha / (TW + 2), hx = ha % (TW + 2);
         const f32x16 acc = conv_in_mma<XTW>(sx, hy, hx, wi, row_biases(sbias[0]));
@@ -828,15 +872,12 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
                 const bool in = gy < H && gx < W;
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
-                    float v[4];
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) v[jj] = fmaxf(acc[m][4 * g + jj] + bv[4 * g + jj], 0.f);
-                    const unsigned lo = (unsigned)__builtin_bit_cast(unsigned short, to_bf16(v[0])) |
-                                        ((unsigned)__builtin_bit_cast(unsigned short, to_bf16(v[1])) << 16);
-                    const unsigned hi = (unsigned)__builtin_bit_cast(unsigned short, to_bf16(v[2])) |
-                                        ((unsigned)__builtin_bit_cast(unsigned short, to_bf16(v[3])) << 16);
+                    const unsigned w0 = relu_bf16x2(
+                        pack_bf16x2(acc[m][4 * g] + bv[4 * g], acc[m][4 * g + 1] + bv[4 * g + 1]));
+                    const unsigned w1 = relu_bf16x2(
+                        pack_bf16x2(acc[m][4 * g + 2] + bv[4 * g + 2], acc[m][4 * g + 3] + bv[4 * g + 3]));
                     __builtin_amdgcn_raw_buffer_store_b64(
-                        U32x2{lo, hi}, ry, in ? (uint32_t)(((gy * W + gx) * C + 8 * g + 4 * h) * 2) : kOutside, 0, 0);
+                        U32x2{w0, w1}, ry, in ? (uint32_t)(((gy * W + gx) * C + 8 * g + 4 * h) * 2) : kOutside, 0, 0);
                 }
             }
         }

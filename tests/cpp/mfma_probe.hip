@@ -24,24 +24,6 @@ __global__ void k_probe(const float* A, const float* B, float* D)
     for (int r = 0; r < R; ++r) D[L::d_row(lane, r) * N + L::d_col(lane)] = acc[r];
 }
 
-__global__ void k_probe_f32(const float* A, const float* B, float* D)
-{
-    using L = Mfma32F32;
-    const int lane = threadIdx.x;
-    const float a = A[L::a_row(lane) * 2 + L::a_k(lane)];
-    const float b = B[L::b_k(lane) * 32 + L::b_col(lane)];
-    f32x16 acc;
-    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-    acc = L::mma(a, b, acc);
-    for (int r = 0; r < 16; ++r) D[L::d_row(lane, r) * 32 + L::d_col(lane)] = acc[r];
-}
-
-extern "C" int probe_mfma32_f32(const float* A, const float* B, float* D)
-{
-    hipLaunchKernelGGL(k_probe_f32, dim3(1), dim3(64), 0, 0, A, B, D);
-    return hipDeviceSynchronize() == hipSuccess ? 0 : 1;
-}
-
 extern "C" int probe_mfma32(const float* A, const float* B, float* D)
 {
     hipLaunchKernelGGL((k_probe<Mfma32, 32, 32, 16, f32x16, 16>), dim3(1), dim3(64), 0, 0, A, B, D);

@@ -57,18 +57,29 @@ def weights_of(kappa):
 # order, and with exact FP64, from which BF16 storage alone moves them by
 # about 1e-2 forward and 0.05-0.16 in the weight gradients of white-noise
 # inputs (the emulating oracle's own distance from FP64, block by block).
-# The order still matters through BF16 storage: the input layer matches the
-# oracle's rounding on every element, yet the matrix cores' summation order
+# The order still matters through BF16 storage.  The input layer's matrix
+# instructions sum sixteen split products at a time in an order of their
+# own, so its activation lands on the other neighbouring BF16 value from the
+# oracle's FP32 chain in a handful of elements (4-19 in 65k-524k measured;
+# the layer-by-layer test bounds it), and a one-step change at the network's
+# input spreads through twelve BF16 layers to several percent of every
+# weight gradient: the oracle alone, with its input-layer sum rounded per
+# instruction instead of per product, moves the blocks by 2.6-9.3 %.  The
+# end-to-end gradient test therefore starts the oracle from the GPU's own
+# input-layer activation.  After it, the matrix cores' summation order
 # flips a few of the 90k encoder activations of the backward test's input
 # to the neighbouring BF16 value, the decoder grows that, and the weight
-# gradients move by 1.1-2.7 % (measured per block: 0.0127, 0.013, 0.0215,
-# 0.020, 0.0271, 0.0253, 0.0223, 0.0204, 0.0227, 0.0203, 0.0203, 0.0188,
-# 0.011, 0.0021).  A wrong index or a dropped term moves them by tens of
-# percent.  Against FP64 each block is held to the emulating oracle's own
-# distance plus the distance to it (the triangle inequality), not to a flat
-# bound.
+# gradients move by 2.6-6.7 % (measured per block: 0.032, 0.0326, 0.0472,
+# 0.0471, 0.0523, 0.0593, 0.0641, 0.0427, 0.0667, 0.0654, 0.0427, 0.0256,
+# 0.0265, 0.0031; a network whose input layer summed in FP32 measured
+# 1.1-2.7 % here — another draw of the same amplification).  A wrong index
+# or a dropped term moves some block by 27 % or more on this input
+# (test_unet_oracle.py plants such mistakes in the oracle and checks them
+# against GRAD_TOL_EMULATED).  Against FP64 each block is held to the
+# emulating oracle's own distance plus the distance to it (the triangle
+# inequality), not to a flat bound.
 FWD_TOL_EMULATED, FWD_TOL_FP64 = 1e-2, 3e-2
-GRAD_TOL_EMULATED = 3.5e-2
+GRAD_TOL_EMULATED = 0.1
 # the backward pass alone (oracle fed the GPU's forward activations): the
 # largest block error measured is 1.8e-3 (conv_in, at the end of the chain)
 GRAD_TOL_GPU_ACTIVATIONS = 5e-3
@@ -119,7 +130,8 @@ def test_unet_backward_matches_oracle(learned):
     net.forward(xt)
     g = net.backward(xt, torch.from_numpy(dout).cuda()).cpu().numpy()
     p64 = params.astype(np.float64)
-    _, act_em = U.forward(p64, x, emulate=True)
+    a0 = net.activation("a0", B).cpu().numpy().astype(np.float64)
+    _, act_em = U.forward(p64, x, emulate=True, a0=a0)
     g_em = U.backward(p64, act_em, dout, emulate=True)
     _, act = U.forward(p64, x)
     g64 = U.backward(p64, act, dout)
@@ -182,7 +194,7 @@ def test_unet_layers_match_oracle_on_gpu_inputs(learned):
         if op in ("pool", "up"):
             continue
         if op == "in":
-            ref = U.bf16(U.relu(U.conv_in_fma32(a[src], win, bin_)))
+            ref = U.bf16(U.relu(U.conv_in_split32(a[src], win, bin_)))
             got = a[dst]
         elif op == "out":
             ref = U.conv32(a[src], U.f32(wout)[:, None, :], U.f32(np.array([bout])))[..., 0]

@@ -30,15 +30,3 @@ def test_mfma_lane_layout(name, m, n, k):
     ref = bf16_round(A).astype(np.float64) @ bf16_round(B).astype(np.float64)
     np.testing.assert_allclose(d.cpu().numpy(), ref, rtol=1e-5, atol=1e-5)
 
-
-def test_mfma_f32_lane_layout():
-    lib = ctypes.CDLL(str(LIB))
-    rng = np.random.default_rng(1)
-    A = rng.standard_normal((32, 2)).astype(np.float32)
-    B = rng.standard_normal((2, 32)).astype(np.float32)
-    a = torch.from_numpy(A).cuda()
-    b = torch.from_numpy(B).cuda()
-    d = torch.full((32, 32), np.nan, dtype=torch.float32, device="cuda")
-    assert lib.probe_mfma32_f32(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
-                                ctypes.c_void_p(d.data_ptr())) == 0
-    np.testing.assert_allclose(d.cpu().numpy(), A.astype(np.float64) @ B.astype(np.float64), rtol=1e-5, atol=1e-5)
