@@ -679,6 +679,8 @@ constexpr int kXPerThread = (kXPairs + NT - 1) / NT;
 constexpr int kHalo = (TH + 2) * (TW + 2);          // 340 a0 pixels
 constexpr int kHaloBlocks = (kHalo + 31) / 32;      // 11
 static_assert(TH * TW * 4 % NT == 0, "the output stores of a tile are straight-line");
+static_assert(kHaloBlocks <= 3 * (NT / 64) && (3 * (NT / 64) - 1) * 32 >= kHalo,
+              "three blocks per wavefront cover the halo; the twelfth lies wholly past it");
 
 // With PLANAR the input is two FP32 planes read through a slot map (image
 // b of the batch is source image src[b]): channel 0 = c0[src[b]] ·
@@ -820,12 +822,16 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
         const int next = tile + G;
         int nb, noy0, nox0;
         coords(min(next, ntiles - 1), nb, noy0, nox0);
-        const bool has_next = next < ntiles;
 
         // one phase of matrix work: layer 1 of this tile from sX[buf], in
         // three segments of three taps, each followed by one a0 block of
         // the next tile (from sx into sX[buf ^ 1]) — independent chains, so
-        // the VALU epilogue of a block runs while layer 1's products do
+        // the VALU epilogue of a block runs while layer 1's products do.
+        // The blocks are unconditional, so the compiler can interleave
+        // their products with layer 1's: past the last tile a block forms
+        // a0 of the last tile again into the buffer no later phase reads,
+        // and block 11 (the third of wavefront 3) has no pixel in the halo
+        // and writes nothing.
         f32x16 acc[2];
 #pragma unroll
         for (int m = 0; m < 2; ++m)
@@ -849,7 +855,7 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
                 }
             }
             const int j = wave + (NT / 64) * q;
-            if (has_next && j < kHaloBlocks) a0_block(j, noy0, nox0, sX[buf ^ 1]);
+            a0_block(j, noy0, nox0, sX[buf ^ 1]);
         }
         __syncthreads(); // sX[buf] consumed, a0 of the next tile complete, sx consumed
 
