@@ -275,11 +275,11 @@ constexpr int kStageChunks = (TH + 2) * (TW + 2) * 4; // 16-byte chunks of one i
 constexpr int conv_threads(int mode) { return mode == ST_POOL ? 2 * NT : NT; }
 static_assert(TH * TW * 4 % (2 * NT) == 0 && TH * TW % NT == 0, "the output stores of a tile are straight-line");
 
-template <int MODE, bool HEAD = false>
+template <int MODE>
 __global__ __launch_bounds__(conv_threads(MODE), conv_wgs_per_cu(MODE)) void k_conv33(const bf16* __restrict__ X, const bf16* __restrict__ X2,
                                                const bf16* __restrict__ Wt, const float* __restrict__ bias,
-                                               bf16* __restrict__ Y, float* __restrict__ Yh, int Bmax,
-                                               const int* __restrict__ dB, int H, int W, int relu)
+                                               bf16* __restrict__ Y, int Bmax, const int* __restrict__ dB, int H,
+                                               int W, int relu)
 {
     constexpr int NTH = conv_threads(MODE), MR = TH / (NTH / 64); // threads; output rows per wavefront
     constexpr int kPerThread = (kStageChunks + NTH - 1) / NTH;
@@ -305,11 +305,10 @@ __global__ __launch_bounds__(conv_threads(MODE), conv_wgs_per_cu(MODE)) void k_c
     // the products are transposed (weights as A, pixels as B): the D
     // fragment of a lane is pixel `col` of its row with the 16 output
     // channels d_row(lane, r), so its results leave as 8-byte runs of four
-    // channels and the head's two columns sit in acc[0], acc[1] of lanes < 32
+    // channels
     float bv[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) bv[r] = (bias && !HEAD) ? bias[Mfma32::d_row(lane, r)] : 0.f;
-    const float bh = (bias && HEAD) ? bias[0] : 0.f;
+    for (int r = 0; r < 16; ++r) bv[r] = bias ? bias[Mfma32::d_row(lane, r)] : 0.f;
     const int64_t xb = x_image_bytes(MODE, H, W), x2b = x2_image_bytes(MODE, H, W);
 
     auto coords = [&](int t, int& b, int& oy0, int& ox0) {
@@ -375,45 +374,28 @@ __global__ __launch_bounds__(conv_threads(MODE), conv_wgs_per_cu(MODE)) void k_c
 
         int b, oy0, ox0;
         coords(tile, b, oy0, ox0);
-        if constexpr (HEAD) {
-            // output head: channels 0 and 1 (hi / lo weights) of the lane's
-            // pixel, one FP32 output per pixel straight from the registers
-            const __amdgpu_buffer_rsrc_t ry = buffer_of(Yh + (size_t)b * H * W, (int64_t)H * W * 4);
+        // results → sX[buf] as [8 rows][32 pixels][32 channels] (stride CP)
+        bf16* out_s = sX[buf];
 #pragma unroll
-            for (int m = 0; m < MR; ++m) {
-                const int gy = oy0 + MR * wave + m, gx = ox0 + col;
-                const float v = acc[m][0] + acc[m][1] + bh;
-                __builtin_amdgcn_raw_buffer_store_b32(
-                    __builtin_bit_cast(unsigned, v), ry,
-                    (lane < 32 && gy < H && gx < W) ? (uint32_t)(gy * W + gx) * 4 : kOutside, 0, 0);
-            }
-            put(sX[buf ^ 1], pre);
-            fetch(tile + 2 * G, pre);
-            __syncthreads();
-        } else {
-            // results → sX[buf] as [8 rows][32 pixels][32 channels] (stride CP)
-            bf16* out_s = sX[buf];
+        for (int m = 0; m < MR; ++m) {
+            f32x16 v;
 #pragma unroll
-            for (int m = 0; m < MR; ++m) {
-                f32x16 v;
-#pragma unroll
-                for (int r = 0; r < 16; ++r) v[r] = acc[m][r] + bv[r];
-                put_channels(&out_s[((MR * wave + m) * TW + col) * CP], v, relu != 0, true);
-            }
-            put(sX[buf ^ 1], pre);
-            fetch(tile + 2 * G, pre);
-            __syncthreads();
+            for (int r = 0; r < 16; ++r) v[r] = acc[m][r] + bv[r];
+            put_channels(&out_s[((MR * wave + m) * TW + col) * CP], v, relu != 0, true);
+        }
+        put(sX[buf ^ 1], pre);
+        fetch(tile + 2 * G, pre);
+        __syncthreads();
 
-            const __amdgpu_buffer_rsrc_t ry = buffer_of(Y + (size_t)b * H * W * C, (int64_t)H * W * C * 2);
+        const __amdgpu_buffer_rsrc_t ry = buffer_of(Y + (size_t)b * H * W * C, (int64_t)H * W * C * 2);
 #pragma unroll
-            for (int k = 0; k < TH * TW * 4 / NTH; ++k) {
-                const int c = tid + k * NTH;
-                const int pix = c >> 2, part = c & 3;
-                const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
-                const U32x4 v = *reinterpret_cast<const U32x4*>(&out_s[pix * CP + part * 8]);
-                __builtin_amdgcn_raw_buffer_store_b128(
-                    v, ry, (gy < H && gx < W) ? (uint32_t)((gy * W + gx) * C + part * 8) * 2 : kOutside, 0, 0);
-            }
+        for (int k = 0; k < TH * TW * 4 / NTH; ++k) {
+            const int c = tid + k * NTH;
+            const int pix = c >> 2, part = c & 3;
+            const int gy = oy0 + pix / TW, gx = ox0 + pix % TW;
+            const U32x4 v = *reinterpret_cast<const U32x4*>(&out_s[pix * CP + part * 8]);
+            __builtin_amdgcn_raw_buffer_store_b128(
+                v, ry, (gy < H && gx < W) ? (uint32_t)((gy * W + gx) * C + part * 8) * 2 : kOutside, 0, 0);
         }
         buf ^= 1;
     }
@@ -933,14 +915,95 @@ __global__ void k_conv_in_wgrad(const float* __restrict__ x, const bf16* __restr
 
 // head weights w[tap][ci] (FP32) → B matrix [tap][co][ci] (BF16) with
 // co = 0: hi = bf16(w), co = 1: lo = bf16(w − hi), other columns zero
-__global__ void k_pack_head(const float* __restrict__ w, bf16* __restrict__ Wt)
+// ------------------------------------------------------------ output head ---
+// out = conv_out(x): 32 → 1 channel, 3×3, BF16 x, FP32 weights, bias and
+// output.
+//
+// Channels first, taps second.  Per pixel p and tap t the channel sum
+// g_t(p) = Σ_c w[t][c]·x[p][c] is one row of a matrix product with the
+// weights as A — rows 2t and 2t+1 hold the BF16 hi and lo parts of tap t's
+// weights (w ≈ hi + lo to 16 bits), rows 18–31 are zero, K = the 32
+// channels — and 32 pixels as B: two v_mfma_f32_32x32x16_bf16 per 32
+// pixels.  Then out(q) = b + Σ_t g_t(q + off_t), taps in order, from the g
+// of the tile's halo in LDS.  x is read once per halo pixel, straight into
+// the B registers through a buffer resource (pixels outside the image read
+// zero, the layer's zero padding); no input tile is staged, and the matrix
+// work is 1/9 of a 32→32 layer's.
+constexpr int HTH = 16, HTW = 32;                       // output tile
+constexpr int kHHalo = (HTH + 2) * (HTW + 2);           // 612 halo pixels
+constexpr int kHBlocks = (kHHalo + 31) / 32;            // 20 blocks of 32 pixels
+constexpr int kHPerWave = kHBlocks / (NT / 64);         // 5
+constexpr int kHTaps = 9;                               // g values per halo pixel in LDS
+static_assert(kHBlocks % (NT / 64) == 0 && HTH * HTW % NT == 0, "the head's loops are straight-line");
+
+__global__ __launch_bounds__(NT) void k_head(const bf16* __restrict__ x, const bf16* __restrict__ Wa,
+                                             const float* __restrict__ bias, float* __restrict__ y, int H, int W,
+                                             const int* __restrict__ dB)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 9 * C * C; i += gridDim.x * blockDim.x) {
-        const int t = i / (C * C), co = (i / C) % C, ci = i % C;
-        const float wv = w[t * C + ci];
-        const bf16 hi = to_bf16(wv);
-        Wt[i] = co == 0 ? hi : (co == 1 ? to_bf16(wv - from_bf16(hi)) : to_bf16(0.f));
+    __shared__ float sg[kHHalo * kHTaps];
+    const int b = blockIdx.z, oy0 = blockIdx.y * HTH, ox0 = blockIdx.x * HTW;
+    if (dB && b >= *dB) return; // uniform per workgroup
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = Mfma32::b_col(lane), h = lane >> 5;
+    const __amdgpu_buffer_rsrc_t rx = buffer_of(x + (size_t)b * H * W * C, (int64_t)H * W * C * 2);
+    // every load of the wavefront's blocks first: lane l supplies channels
+    // 16·kh + 8·(l / 32) + 0..7 of pixel l % 32
+    U32x4 xr[kHPerWave][2];
+#pragma unroll
+    for (int i = 0; i < kHPerWave; ++i) {
+        const int hp = (wave + (NT / 64) * i) * 32 + n;
+        const int gy = oy0 - 1 + hp / (HTW + 2), gx = ox0 - 1 + hp % (HTW + 2);
+        const bool in = hp < kHHalo && gy >= 0 && gy < H && gx >= 0 && gx < W;
+        const uint32_t off = (uint32_t)((gy * W + gx) * C + Mfma32::b_k0(lane)) * 2;
+#pragma unroll
+        for (int kh = 0; kh < 2; ++kh)
+            xr[i][kh] = __builtin_amdgcn_raw_buffer_load_b128(rx, in ? off + kh * 32 : kOutside, 0, 0);
     }
+    bf16x8 wa[2];
+#pragma unroll
+    for (int kh = 0; kh < 2; ++kh)
+        wa[kh] = *reinterpret_cast<const bf16x8*>(&Wa[Mfma32::a_row(lane) * C + kh * 16 + Mfma32::a_k0(lane)]);
+    const float b0 = bias[0];
+#pragma unroll
+    for (int i = 0; i < kHPerWave; ++i) {
+        const int hp = (wave + (NT / 64) * i) * 32 + n;
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+        acc = Mfma32::mma(wa[0], __builtin_bit_cast(bf16x8, xr[i][0]), acc);
+        acc = Mfma32::mma(wa[1], __builtin_bit_cast(bf16x8, xr[i][1]), acc);
+        // D row 8g + 4h + j is part j & 1 of tap 4g + 2h + (j >> 1)
+        if (hp < kHHalo) {
+            float* g = &sg[hp * kHTaps];
+            g[2 * h] = acc[0] + acc[1];
+            g[2 * h + 1] = acc[2] + acc[3];
+            g[4 + 2 * h] = acc[4] + acc[5];
+            g[5 + 2 * h] = acc[6] + acc[7];
+            if (h == 0) g[8] = acc[8] + acc[9];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < HTH * HTW / NT; ++k) {
+        const int q = tid + k * NT;
+        const int py = q / HTW, px = q % HTW;
+        float v = b0;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) v += sg[((py + t / 3) * (HTW + 2) + px + t % 3) * kHTaps + t];
+        const int gy = oy0 + py, gx = ox0 + px;
+        if (gy < H && gx < W) y[((size_t)b * H + gy) * W + gx] = v;
+    }
+}
+
+// the head's A matrix: row 2t + s (t < 9) = part s (0 hi, 1 lo) of tap t's
+// 32 weights, rows 18–31 zero
+__global__ void k_pack_head(const float* __restrict__ w, bf16* __restrict__ Wa)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C * C) return;
+    const int m = i / C, c = i % C, t = m / 2;
+    const unsigned split = t < 9 ? split_bf16(w[t * C + c]) : 0u;
+    Wa[i] = __builtin_bit_cast(bf16, (unsigned short)(m % 2 == 0 ? split & 0xFFFFu : split >> 16));
 }
 
 // backward of conv_out: dX[q][ci] = Σ_t dout[q − off(t)] · w[t][ci] (BF16),
@@ -1192,7 +1255,7 @@ int conv33(int mode, const bf16* X, const bf16* X2, const bf16* Wt, const float*
         constexpr int M = decltype(m)::value;
         const long cap = kComputeUnits * conv_wgs_per_cu(M);
         const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
-        hipLaunchKernelGGL((k_conv33<M, false>), dim3(grid), dim3(conv_threads(M)), 0, s, X, X2, Wt, bias, Y, nullptr, B, dB, H,
+        hipLaunchKernelGGL((k_conv33<M>), dim3(grid), dim3(conv_threads(M)), 0, s, X, X2, Wt, bias, Y, B, dB, H,
                            W, relu ? 1 : 0);
         SIL_LAUNCH_CHECK();
         return (int)SIL_OK;
@@ -1256,21 +1319,19 @@ int conv_in_wgrad(const float* x, const bf16* dY, const bf16* Yf, float* part, i
     return SIL_OK;
 }
 
-// 3×3 conv 32→1 (the output head, no activation): the persistent
-// convolution with the B matrix of k_pack_head.  The FP32 weights are split
-// into w = hi + lo with hi, lo BF16 in columns 0 and 1, so the two output
-// columns sum to the product with the weights to about 16 bits, and the
-// activations (BF16 already) are multiplied exactly with FP32 accumulation.
+// 3×3 conv 32→1 (the output head, no activation) with the A matrix of
+// k_pack_head: the FP32 weights split into w = hi + lo (BF16 rows 2t and
+// 2t+1), so the two rows sum to the product with the weights to about 16
+// bits, and the activations (BF16 already) are multiplied exactly with FP32
+// accumulation.
 int conv_out(const bf16* x, const bf16* Wt, const float* bias, float* y, int B, int H, int W, hipStream_t s,
              const int* dB)
 {
     SIL_REQUIRE(x_image_bytes(ST_PLAIN, H, W) <= kMaxBufferBytes, SIL_ERR_UNSUPPORTED,
                 "unet: %dx%d images exceed one buffer resource per image", H, W);
-    const long ntiles = (long)((W + TW - 1) / TW) * ((H + TH - 1) / TH) * B;
-    const long cap = kComputeUnits * conv_wgs_per_cu(ST_PLAIN);
-    const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
-    hipLaunchKernelGGL((k_conv33<ST_PLAIN, true>), dim3(grid), dim3(NT), 0, s, x, nullptr, Wt, bias, nullptr, y, B, dB,
-                       H, W, 0);
+    SIL_REQUIRE(B <= 65535, SIL_ERR_UNSUPPORTED, "unet: at most 65535 images per batch");
+    hipLaunchKernelGGL(k_head, dim3((W + HTW - 1) / HTW, (H + HTH - 1) / HTH, B), dim3(NT), 0, s, x, Wt, bias, y,
+                       H, W, dB);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }
@@ -1332,7 +1393,7 @@ int to_f32(const bf16* a, float* b, int64_t n, hipStream_t s)
 
 int pack_head(const float* w, bf16* Wt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pack_head, dim3(36), dim3(256), 0, s, w, Wt);
+    hipLaunchKernelGGL(k_pack_head, dim3(C * C / 256), dim3(256), 0, s, w, Wt);
     SIL_LAUNCH_CHECK();
     return SIL_OK;
 }

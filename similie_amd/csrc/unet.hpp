@@ -69,7 +69,8 @@ int adam(float* w, const float* g, float* m, float* v, int n, float lr, int step
 int to_f32(const bf16* a, float* b, int64_t n, hipStream_t s);
 // BF16 forward weights and the flipped, channel-transposed input-gradient weights of one 32→32 layer
 int pack33(const float* w, bf16* wf, bf16* wb, hipStream_t s);
-// output-head weights [tap][ci] → BF16 B matrix with hi/lo columns
+// output-head weights [tap][ci] → the head's BF16 A matrix (32 × 32: rows 2t,
+// 2t+1 the hi and lo parts of tap t, rows 18–31 zero)
 int pack_head(const float* w, bf16* Wt, hipStream_t s);
 
 // maximum batch and image size of a network handle (defined with the handle)

@@ -42,7 +42,7 @@ struct sil_unet_s {
     float* adam_v = nullptr;
     bf16* wf = nullptr; // kLayers × 9·32·32 forward weights
     bf16* wb = nullptr; // kLayers × 9·32·32 input-gradient weights
-    bf16* wh = nullptr; // 9·32·32 output-head weights (hi / lo columns)
+    bf16* wh = nullptr; // 32·32 output-head A matrix (hi / lo rows per tap)
     // activations
     // (pooled and upsampled-plus-skip inputs are formed while staging)
     bf16 *a0 = nullptr, *e0 = nullptr, *c0 = nullptr;
@@ -239,7 +239,7 @@ int sil_unet_create(sil_unet* out, int32_t max_batch, int32_t height, int32_t wi
     A(u->adam_v, kParams);
     A(u->wf, (size_t)kLayers * 9 * C * C);
     A(u->wb, (size_t)kLayers * 9 * C * C);
-    A(u->wh, (size_t)9 * C * C);
+    A(u->wh, (size_t)C * C);
     for (bf16** p : {&u->a0, &u->e0, &u->c0, &u->g0a, &u->g0b}) A(*p, s0);
     for (bf16** p : {&u->a1, &u->e1, &u->c1, &u->f1, &u->g1a, &u->g1b}) A(*p, s1);
     for (bf16** p : {&u->a2, &u->e2, &u->c2, &u->f2, &u->g2a, &u->g2b}) A(*p, s2);
