@@ -12,12 +12,17 @@
 //   c1, f1 = relu(conv(up(f2) + e1)), relu(conv(c1))                (128²)
 //   c0     = relu(conv(up(f1) + e0))                                (256²)
 //   out    = conv_out(c0)           (1 ch, FP32, linear)
-// The 32→32 convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16:
-// a workgroup computes an 8×32-pixel output tile, stages the (8+2)×(32+2)
-// input tile and all 9·32·32 weights in LDS (pixel stride padded to 40
-// elements so 16-byte fragment reads are bank-conflict free), and each
-// wavefront accumulates two 32-pixel rows × 32 output channels over
-// K = 9 taps × 32 input channels.  The input gradient is the same kernel on
+// The 32→32 convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16 in
+// persistent workgroups that walk 8×32-pixel output tiles: the (8+2)×(32+2)
+// input tile is staged in LDS (pixel stride padded to 40 elements so
+// 16-byte fragment reads are bank-conflict free; double-buffered, the next
+// tile's loads in flight), the 9·32·32 weights stay in registers as A
+// fragments (weights as A, pixels as B, so a lane's results are 16
+// channels of one pixel), and each wavefront accumulates two 32-pixel
+// rows × 32 output channels over K = 9 taps × 32 input channels.  The 2→32
+// input layer runs on two-term BF16 splits of its FP32 operands (fused
+// with layer 1 for inference, k_conv_in33); the 32→1 output head reduces
+// channels first (k_head).  The input gradient is the same kernel on
 // the flipped, channel-transposed weights with the ReLU mask applied while
 // the tile is staged; the weight gradient is a second MFMA GEMM over pixels
 // with per-workgroup partial sums reduced in a fixed order.
