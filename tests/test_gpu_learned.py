@@ -171,14 +171,17 @@ def _bf16_ulps(got, ref):
     return np.abs(got - ref) / 2.0 ** (e - 7)
 
 
-def test_unet_layers_match_oracle_on_gpu_inputs(learned):
+@pytest.mark.parametrize("shape", [(2, 32, 32), (1, 24, 72)])
+def test_unet_layers_match_oracle_on_gpu_inputs(learned, shape):
     """Layer by layer, each layer's output against the emulating oracle fed
     the GPU's own input to that layer: the only difference left is the order
     of the FP32 sums before the BF16 rounding of the output, which moves an
     element by at most one BF16 step and only when its FP32 value lies within
     rounding distance of a tie — a handful per layer.  A wrong tap, channel or
-    stage moves most elements by many steps."""
-    B, H, W = 2, 32, 32
+    stage moves most elements by many steps.  24 x 72 leaves partial tiles
+    at the bottom and right of every level (8 x 32 tiles of the 32->32
+    layers, 16 x 32 tiles of the output head)."""
+    B, H, W = shape
     rng = np.random.default_rng(41)
     params = make_params(rng)
     x = rng.standard_normal((B, H, W, 2)).astype(np.float32)
