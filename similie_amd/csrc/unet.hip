@@ -87,6 +87,26 @@ __device__ __forceinline__ void put_channels(bf16* __restrict__ px_row, const f3
     }
 }
 
+// Persistent workgroups take tiles by XCD.  The hardware dispatches
+// workgroup b to XCD b mod 8, so the workgroups of XCD x share one
+// contiguous range of tiles (the tiles split in proportion to each XCD's
+// workgroup count) and walk it together, the j-th of them taking
+// start + j + k·n (n the XCD's workgroup count).  Neighbouring tiles, which
+// read each other's halo rows, then run on one XCD and share its L2.
+struct TileWalk {
+    int first, stride, end;
+};
+
+__device__ __forceinline__ TileWalk xcd_walk(int ntiles)
+{
+    const int G = gridDim.x, b = blockIdx.x, x = b % 8, j = b / 8;
+    auto before = [&](int xx) { return (G / 8) * xx + min(G % 8, xx); }; // workgroups on XCDs < xx
+    const int n = G / 8 + (x < G % 8 ? 1 : 0);
+    const int start = (int)((int64_t)ntiles * before(x) / G);
+    const int end = (int)((int64_t)ntiles * before(x + 1) / G);
+    return {start + j, n, end};
+}
+
 // ------------------------------------------------------ 3×3 32→32 conv ---
 
 // The input tile of a convolution is staged from one of four sources, so the
@@ -261,8 +281,9 @@ __host__ __device__ constexpr int64_t x2_image_bytes(int mode, int H, int W)
 // Y = act(conv(staged input) + bias) on H×W images: the forward pass, or
 // (ST_MASK, flipped weights, no bias, no ReLU) the input gradient.
 //
-// Persistent workgroups walk the 8×32-pixel output tiles with stride G =
-// gridDim.x.  Each lane keeps its B fragments (the weights of its output
+// Persistent workgroups walk the 8×32-pixel output tiles (xcd_walk: the
+// workgroups of one XCD share a contiguous range, each taking every G-th
+// tile of it, G their count).  Each lane keeps its B fragments (the weights of its output
 // channel, 9 taps × 32 input channels) in registers for the whole walk; the
 // input tile is double-buffered in LDS.  One step for tile t: the 36 MFMAs
 // per wavefront on the staged input of t; its results into the consumed
@@ -294,9 +315,10 @@ __global__ __launch_bounds__(conv_threads(MODE), conv_wgs_per_cu(MODE)) void k_c
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_x = (W + TW - 1) / TW, tiles_y = (H + TH - 1) / TH;
     const int ntiles = tiles_x * tiles_y * B;
-    const int G = gridDim.x;
-    int tile = blockIdx.x;
-    if (tile >= ntiles) return; // uniform per workgroup
+    const TileWalk tw = xcd_walk(ntiles);
+    const int G = tw.stride, tend = tw.end;
+    int tile = tw.first;
+    if (tile >= tend) return; // uniform per workgroup
 
     const int col = Mfma32::a_row(lane); // output column (pixel) of the A fragment
     const int k0 = Mfma32::a_k0(lane);
@@ -325,7 +347,7 @@ __global__ __launch_bounds__(conv_threads(MODE), conv_wgs_per_cu(MODE)) void k_c
     // put combines them into LDS
     auto fetch = [&](int t, RawBuf<MODE> (&pre)[kPerThread]) {
         int b, oy0, ox0;
-        coords(min(t, ntiles - 1), b, oy0, ox0);
+        coords(min(t, tend - 1), b, oy0, ox0);
         const __amdgpu_buffer_rsrc_t rx = buffer_of(X + (size_t)b * (xb / 2), xb);
         const __amdgpu_buffer_rsrc_t rx2 = buffer_of(x2b ? X2 + (size_t)b * (x2b / 2) : X, x2b);
 #pragma unroll
@@ -333,7 +355,7 @@ __global__ __launch_bounds__(conv_threads(MODE), conv_wgs_per_cu(MODE)) void k_c
             const int c = tid + k * NTH;
             const int pix = c >> 2, part = c & 3;
             const int gy = oy0 - 1 + pix / (TW + 2), gx = ox0 - 1 + pix % (TW + 2);
-            const bool in = t < ntiles && c < kStageChunks && gy >= 0 && gy < H && gx >= 0 && gx < W;
+            const bool in = t < tend && c < kStageChunks && gy >= 0 && gy < H && gx >= 0 && gx < W;
             load_raw_buf<MODE>(pre[k], rx, rx2, gy, gx, part, W, in);
         }
     };
@@ -355,7 +377,7 @@ __global__ __launch_bounds__(conv_threads(MODE), conv_wgs_per_cu(MODE)) void k_c
     // it is for them
     __builtin_amdgcn_s_waitcnt(0x0F70);
     int buf = 0;
-    for (; tile < ntiles; tile += G) {
+    for (; tile < tend; tile += G) {
         f32x16 acc[MR];
 #pragma unroll
         for (int m = 0; m < MR; ++m)
@@ -653,7 +675,7 @@ __global__ __launch_bounds__(NT) void k_conv_in(const float* __restrict__ x, con
 // conv_in followed by k_conv33 bit for bit.
 //
 // One step for tile t: layer 1 of t (from a0 of t in one LDS buffer) with
-// a0 of tile t + G (G = grid size) formed into the other buffer in the same
+// a0 of tile t + G (the workgroup's next tile, xcd_walk) formed into the other buffer in the same
 // phase, interleaved in segments so that both matrix chains and the VALU
 // epilogues overlap; a barrier; the x tile of t + 2G written to LDS, the
 // loads of t + 3G issued, layer 1's outputs of t stored straight from the
@@ -689,9 +711,10 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_x = (W + TW - 1) / TW, tiles_y = (H + TH - 1) / TH;
     const int ntiles = tiles_x * tiles_y * B;
-    const int G = gridDim.x;
-    int tile = blockIdx.x;
-    if (tile >= ntiles) return; // uniform per workgroup
+    const TileWalk tw = xcd_walk(ntiles);
+    const int G = tw.stride, tend = tw.end;
+    int tile = tw.first;
+    if (tile >= tend) return; // uniform per workgroup
 
     // the biases live in LDS (read per block and per epilogue) to leave the
     // registers to the weights and accumulators
@@ -720,13 +743,13 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
     const __amdgpu_buffer_rsrc_t rsrc = buffer_of(pin.src, PLANAR ? (int64_t)Bmax * 4 : 0);
     const __amdgpu_buffer_rsrc_t rscl = buffer_of(pin.scale, PLANAR ? (int64_t)Bmax * 4 : 0);
     auto meta = [&](int t, U32x2& m) {
-        const int b = min(t, ntiles - 1) / (tiles_x * tiles_y);
+        const int b = min(t, tend - 1) / (tiles_x * tiles_y);
         m = U32x2{__builtin_amdgcn_raw_buffer_load_b32(rsrc, (uint32_t)b * 4, 0, 0),
                   __builtin_amdgcn_raw_buffer_load_b32(rscl, (uint32_t)b * 4, 0, 0)};
     };
     auto fetch = [&](int t, U32x2 (&xr)[kXPerThread], float& xs, const U32x2& m) {
         int b, oy0, ox0;
-        coords(min(t, ntiles - 1), b, oy0, ox0); // t may lie beyond the last tile
+        coords(min(t, tend - 1), b, oy0, ox0); // t may lie beyond the last tile
         if constexpr (PLANAR) {
             const unsigned m0 = m.x, m1 = m.y;
             const int src = __builtin_amdgcn_readfirstlane((int)m0);
@@ -737,7 +760,7 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
             for (int k = 0; k < kXPerThread; ++k) {
                 const int c = tid + k * NT;
                 const int gy = oy0 - 2 + c / XTW, gx = ox0 - 2 + c % XTW;
-                const bool in = c < kXPairs && t < ntiles && gy >= 0 && gy < H && gx >= 0 && gx < W;
+                const bool in = c < kXPairs && t < tend && gy >= 0 && gy < H && gx >= 0 && gx < W;
                 const uint32_t off = in ? (uint32_t)((gy * W + gx) * 4) : kOutside;
                 xr[k] = U32x2{__builtin_amdgcn_raw_buffer_load_b32(r0, off, 0, 0),
                               __builtin_amdgcn_raw_buffer_load_b32(r1, off, 0, 0)};
@@ -749,7 +772,7 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
             for (int k = 0; k < kXPerThread; ++k) {
                 const int c = tid + k * NT;
                 const int gy = oy0 - 2 + c / XTW, gx = ox0 - 2 + c % XTW;
-                const bool in = c < kXPairs && t < ntiles && gy >= 0 && gy < H && gx >= 0 && gx < W;
+                const bool in = c < kXPairs && t < tend && gy >= 0 && gy < H && gx >= 0 && gx < W;
                 xr[k] = __builtin_amdgcn_raw_buffer_load_b64(rs, in ? (uint32_t)((gy * W + gx) * 8) : kOutside, 0, 0);
             }
         }
@@ -805,10 +828,10 @@ __global__ __launch_bounds__(NT, 2) void k_conv_in33(const float* __restrict__ x
     // inside it is for them
     __builtin_amdgcn_s_waitcnt(0x0F70);
     int buf = 0;
-    for (; tile < ntiles; tile += G) {
+    for (; tile < tend; tile += G) {
         const int next = tile + G;
         int nb, noy0, nox0;
-        coords(min(next, ntiles - 1), nb, noy0, nox0);
+        coords(min(next, tend - 1), nb, noy0, nox0);
 
         // one phase of matrix work: layer 1 of this tile from sX[buf], in
         // three segments of three taps, each followed by one a0 block of
